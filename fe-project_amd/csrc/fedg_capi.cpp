@@ -1,0 +1,931 @@
+// C-ABI layer: device context, uploads, drop-in entry points, time
+// integrator and the fused HEVE stepper.  Declarations and reference
+// citations are in include/fedg.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fedg_builder.h"
+#include "fedg_internal.h"
+
+namespace fedg {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+namespace {
+
+template <typename T>
+int dev_upload(T** dst, const T* src, size_t count) {
+  *dst = nullptr;
+  if (count == 0) return FEDG_OK;
+  hipError_t e = hipMalloc((void**)dst, sizeof(T) * count);
+  if (e != hipSuccess) {
+    set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+    *dst = nullptr;
+    return FEDG_ERR_NOMEM;
+  }
+  if (src) FEDG_HIP_CHECK(hipMemcpy(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice));
+  return FEDG_OK;
+}
+
+#define TRY(x)                       \
+  do {                               \
+    int rc_ = (x);                   \
+    if (rc_ != FEDG_OK) return rc_;  \
+  } while (0)
+
+void free_mesh(DevMesh& m) {
+  void* ptrs[] = {m.E11, m.E22, m.E33, m.Gsqrt, m.G13, m.G23, m.GsqrtH, m.G11, m.G12, m.G22, m.nx, m.ny, m.nz,
+                  m.Fscale, m.vmapM, m.vmapP, m.EMap3Dto2D, m.halo_src, m.velbc, m.nbr, m.nbr_bc, m.D1D,
+                  m.Lift, m.Itr};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  memset(&m, 0, sizeof(m));
+}
+
+bool all_equal(const double* a, size_t n, double v) {
+  for (size_t t = 0; t < n; ++t)
+    if (a[t] != v) return false;
+  return true;
+}
+
+}  // namespace
+}  // namespace fedg
+
+using namespace fedg;
+
+struct fedg_tint {
+  fedg_ctx* ctx;
+  int nstage, tend_buf_size, imex;
+  double dt;
+  double a_ex[4][4], b_ex[4], a_im[4][4], b_im[4];
+  int indmap[4];
+  double* var0;    // (Np,NeA,5)
+  double* varTmp;  // (Np,NeA,5)
+  double* tend_ex; // (Np,NeA,5,nbuf)
+  double* tend_im; // (Np,NeA,5,nbuf) or null
+};
+
+struct fedg_heve {
+  fedg_ctx* ctx;
+  fedg_tint tab;            // Butcher table (no buffers)
+  int flat;
+  double* block;            // one allocation for every device array below
+  double* buf[3][5];        // stage states (q^n and two intermediates)
+  double* VT[5];            // RK accumulator
+  int q0;                   // index of q^n in buf
+  double *DENS_hyd, *PRES_hyd, *THERM_hyd, *DPhydDx, *DPhydDy, *CORIOLIS;
+  double *Rtot, *CVtot, *CPtot;
+  double* pres_scratch;     // (Np,Ne) x2 for PRES/DPRES
+};
+
+namespace {
+
+int butcher(const char* name, double dt, fedg_tint* t) {
+  memset(t->a_ex, 0, sizeof(t->a_ex));
+  memset(t->a_im, 0, sizeof(t->a_im));
+  memset(t->b_ex, 0, sizeof(t->b_ex));
+  memset(t->b_im, 0, sizeof(t->b_im));
+  t->dt = dt;
+  // CMN/scale_timeint_rk_butcher_tab.F90:60-324
+  if (!strcmp(name, "ERK_4s4o") || !strcmp(name, "ERK_RK4")) {
+    t->nstage = 4;
+    t->tend_buf_size = 1;
+    t->imex = 0;
+    t->a_ex[1][0] = 0.5;
+    t->a_ex[2][1] = 0.5;
+    t->a_ex[3][2] = 1.0;
+    t->b_ex[0] = 1.0 / 6.0;
+    t->b_ex[1] = 2.0 / 6.0;
+    t->b_ex[2] = 2.0 / 6.0;
+    t->b_ex[3] = 1.0 / 6.0;
+    for (int s = 0; s < 4; ++s) t->indmap[s] = 0;
+  } else if (!strcmp(name, "ERK_1s1o") || !strcmp(name, "ERK_Euler")) {
+    t->nstage = 1;
+    t->tend_buf_size = 1;
+    t->imex = 0;
+    t->b_ex[0] = 1.0;
+    t->indmap[0] = 0;
+  } else if (!strcmp(name, "IMEX_ARK232")) {
+    const double alp = (3.0 + 2.0 * sqrt(2.0)) / 6.0;
+    const double gam = 1.0 - 1.0 / sqrt(2.0);
+    const double del = 1.0 / (2.0 * sqrt(2.0));
+    t->nstage = 3;
+    t->tend_buf_size = 3;
+    t->imex = 1;
+    t->a_ex[1][0] = 2.0 * gam;
+    t->a_ex[2][0] = 1.0 - alp;
+    t->a_ex[2][1] = alp;
+    t->b_ex[0] = del;
+    t->b_ex[1] = del;
+    t->b_ex[2] = gam;
+    t->a_im[1][0] = gam;
+    t->a_im[1][1] = gam;
+    t->a_im[2][0] = del;
+    t->a_im[2][1] = del;
+    t->a_im[2][2] = gam;
+    for (int s = 0; s < 3; ++s) t->b_im[s] = t->b_ex[s];
+    for (int s = 0; s < 3; ++s) t->indmap[s] = s;
+  } else if (!strcmp(name, "IMEX_ARK324")) {
+    t->nstage = 4;
+    t->tend_buf_size = 4;
+    t->imex = 1;
+    t->a_ex[1][0] = 1767732205903.0 / 2027836641118.0;
+    t->a_ex[2][0] = 5535828885825.0 / 10492691773637.0;
+    t->a_ex[2][1] = 788022342437.0 / 10882634858940.0;
+    t->a_ex[3][0] = 6485989280629.0 / 16251701735622.0;
+    t->a_ex[3][1] = -4246266847089.0 / 9704473918619.0;
+    t->a_ex[3][2] = 10755448449292.0 / 10357097424841.0;
+    t->b_ex[0] = 1471266399579.0 / 7840856788654.0;
+    t->b_ex[1] = -4482444167858.0 / 7529755066697.0;
+    t->b_ex[2] = 11266239266428.0 / 11593286722821.0;
+    t->b_ex[3] = 1767732205903.0 / 4055673282236.0;
+    t->a_im[1][0] = 1767732205903.0 / 4055673282236.0;
+    t->a_im[1][1] = 1767732205903.0 / 4055673282236.0;
+    t->a_im[2][0] = 2746238789719.0 / 10658868560708.0;
+    t->a_im[2][1] = -640167445237.0 / 6845629431997.0;
+    t->a_im[2][2] = 1767732205903.0 / 4055673282236.0;
+    for (int s = 0; s < 4; ++s) t->a_im[3][s] = t->b_ex[s];
+    for (int s = 0; s < 4; ++s) t->b_im[s] = t->b_ex[s];
+    for (int s = 0; s < 4; ++s) t->indmap[s] = s;
+  } else {
+    set_error(std::string("unsupported time integration scheme: ") + name);
+    return FEDG_ERR_ARG;
+  }
+  return FEDG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fedg_last_error(void) { return g_err.c_str(); }
+const char* fedg_version(void) { return "fedg 0.1 (gfx950)"; }
+
+int fedg_dev_alloc(void** ptr, size_t bytes) {
+  if (!ptr) return FEDG_ERR_ARG;
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) {
+    set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+    *ptr = nullptr;
+    return FEDG_ERR_NOMEM;
+  }
+  return FEDG_OK;
+}
+int fedg_dev_free(void* ptr) {
+  if (ptr) FEDG_HIP_CHECK(hipFree(ptr));
+  return FEDG_OK;
+}
+int fedg_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  FEDG_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return FEDG_OK;
+}
+int fedg_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  FEDG_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return FEDG_OK;
+}
+int fedg_memset_dev(void* dst, int value, size_t bytes) {
+  FEDG_HIP_CHECK(hipMemset(dst, value, bytes));
+  return FEDG_OK;
+}
+int fedg_device_sync(void) {
+  FEDG_HIP_CHECK(hipDeviceSynchronize());
+  return FEDG_OK;
+}
+int fedg_device_name(char* buf, int buflen) {
+  int dev = 0;
+  FEDG_HIP_CHECK(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  FEDG_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+  snprintf(buf, (size_t)buflen, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+  return FEDG_OK;
+}
+
+// ---------------------------------------------------------------------------
+int fedg_refelem_tables(int porder, double* x1d, double* w1d, double* D1D, double* V1D, double* invV1D,
+                        double* invM1D, double* Lift_mat, double* IntrpMat_VPOrdM1_tr, int32_t* IndexH2Dto3D,
+                        int32_t* IndexH2Dto3D_bnd) {
+  RefElem e;
+  try {
+    build_refelem(porder, e);
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+    return FEDG_ERR_ARG;
+  }
+  auto cp = [](auto* dst, const auto& v) {
+    if (dst) memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+  };
+  cp(x1d, e.x1d);
+  cp(w1d, e.w1d);
+  cp(D1D, e.D1D);
+  cp(V1D, e.V1D);
+  cp(invV1D, e.invV1D);
+  cp(invM1D, e.invM1D);
+  cp(Lift_mat, e.Lift_mat);
+  cp(IntrpMat_VPOrdM1_tr, e.IntrpMat_VPOrdM1_tr);
+  cp(IndexH2Dto3D, e.IndexH2Dto3D);
+  cp(IndexH2Dto3D_bnd, e.IndexH2Dto3D_bnd);
+  return FEDG_OK;
+}
+
+int fedg_box_get_sizes(const fedg_box_desc* d, fedg_box_sizes* s) {
+  if (!d || !s || d->porder < 1 || d->porder > 15 || d->NeX < 1 || d->NeY < 1 || d->NeZ < 1) {
+    set_error("invalid box description");
+    return FEDG_ERR_ARG;
+  }
+  const int n1 = d->porder + 1;
+  s->n1 = n1;
+  s->Np = n1 * n1 * n1;
+  s->Nfp_h = n1 * n1;
+  s->Nfp_v = n1 * n1;
+  s->NfpTot = 6 * n1 * n1;
+  s->Ne = d->NeX * d->NeY * d->NeZ;
+  s->NeA = s->Ne + 2 * (d->NeX + d->NeY) * d->NeZ + 2 * d->NeX * d->NeY;
+  s->Ne2D = d->NeX * d->NeY;
+  s->Ne2DA = d->NeX * d->NeY + 2 * (d->NeX + d->NeY);
+  s->nB = 2 * (d->NeX + d->NeY) * d->NeZ * s->Nfp_h + 2 * d->NeX * d->NeY * s->Nfp_v;
+  if ((double)s->Np * (double)s->NeA > 2.0e9) {
+    set_error("mesh too large for 32-bit node indices");
+    return FEDG_ERR_ARG;
+  }
+  return FEDG_OK;
+}
+
+int fedg_box_build(const fedg_box_desc* d, fedg_box_arrays* out) {
+  fedg_box_sizes s;
+  TRY(fedg_box_get_sizes(d, &s));
+  RefElem e;
+  BoxMesh m;
+  try {
+    build_refelem(d->porder, e);
+    BoxDesc bd{d->porder, d->NeX, d->NeY, d->NeZ, d->xmin, d->xmax, d->ymin, d->ymax, d->zmin, d->zmax,
+               d->periodic_x, d->periodic_y, d->periodic_z};
+    if (build_box(bd, e, m) != 0) {
+      set_error("box mesh construction failed");
+      return FEDG_ERR_ARG;
+    }
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+    return FEDG_ERR_ARG;
+  }
+  auto cp = [](auto* dst, const auto& v) {
+    if (dst) memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+  };
+  cp(out->pos_en, m.pos_en);
+  cp(out->J, m.J);
+  cp(out->Escale, m.Escale);
+  cp(out->sJ, m.sJ);
+  cp(out->Fscale, m.Fscale);
+  cp(out->normal_fn, m.normal_fn);
+  cp(out->vmapM, m.vmapM);
+  cp(out->vmapP, m.vmapP);
+  cp(out->vmapB, m.vmapB);
+  cp(out->halo_src, m.halo_src);
+  cp(out->halo_face, m.halo_face);
+  cp(out->EMap3Dto2D, m.EMap3Dto2D);
+  cp(out->Gsqrt, m.Gsqrt);
+  cp(out->GI3, m.GI3);
+  cp(out->GsqrtH, m.GsqrtH);
+  return FEDG_OK;
+}
+
+// ---------------------------------------------------------------------------
+int fedg_ctx_create(const fedg_mesh_desc* d, fedg_ctx** out) {
+  if (!d || !out) return FEDG_ERR_ARG;
+  *out = nullptr;
+  if (d->porder < 1 || d->porder > 7) {
+    set_error("polynomial order not built (supported: p = 1..7)");
+    return FEDG_ERR_UNSUPPORTED;
+  }
+  if (!d->Escale || !d->Gsqrt || !d->GI3 || !d->GsqrtH || !d->normal_fn || !d->Fscale || !d->vmapM || !d->vmapP ||
+      !d->EMap3Dto2D || !d->D1D || !d->Lift_mat || !d->IntrpMat_VPOrdM1_tr || d->Ne < 1 || d->NeA < d->Ne ||
+      (d->nB > 0 && (!d->halo_src || !d->velbc))) {
+    set_error("fedg_ctx_create: missing array or inconsistent sizes");
+    return FEDG_ERR_ARG;
+  }
+  fedg_ctx* ctx = new fedg_ctx();
+  memset(&ctx->m, 0, sizeof(ctx->m));
+  DevMesh& m = ctx->m;
+  const int n1 = d->porder + 1, Np = n1 * n1 * n1, Nfp = n1 * n1, NfpTot = 6 * Nfp;
+  m.n1 = n1;
+  m.Np = Np;
+  m.Nfp = Nfp;
+  m.NfpTot = NfpTot;
+  m.Ne = d->Ne;
+  m.NeA = d->NeA;
+  m.Ne2D = d->Ne2D;
+  m.Ne2DA = d->Ne2DA;
+  m.nB = d->nB;
+  m.NeX = d->NeX;
+  m.NeY = d->NeY;
+  m.NeZ = d->NeZ;
+  ctx->c.GRAV = d->GRAV != 0.0 ? d->GRAV : FEDG_DEFAULT_GRAV;
+  ctx->c.Rdry = d->Rdry != 0.0 ? d->Rdry : FEDG_DEFAULT_RDRY;
+  ctx->c.CPdry = d->CPdry != 0.0 ? d->CPdry : FEDG_DEFAULT_CPDRY;
+  ctx->c.CVdry = d->CVdry != 0.0 ? d->CVdry : FEDG_DEFAULT_CVDRY;
+  ctx->c.PRES00 = d->PRES00 != 0.0 ? d->PRES00 : FEDG_DEFAULT_PRES00;
+  if (hipGetDevice(&ctx->device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
+    set_error("no usable HIP device");
+    delete ctx;
+    return FEDG_ERR_HIP;
+  }
+  const size_t NpNe = (size_t)Np * m.Ne, NpNeA = (size_t)Np * m.NeA, nf = (size_t)NfpTot * m.Ne;
+  const size_t n2d = (size_t)Nfp * m.Ne2D;
+  int rc = FEDG_OK;
+  auto up = [&](auto** dst, const auto* src, size_t n) {
+    if (rc == FEDG_OK) rc = dev_upload(dst, src, n);
+  };
+  up(&m.E11, d->Escale, NpNe);
+  up(&m.E22, d->Escale + NpNe * 4, NpNe);
+  up(&m.E33, d->Escale + NpNe * 8, NpNe);
+  up(&m.Gsqrt, d->Gsqrt, NpNeA);
+  up(&m.G13, d->GI3, NpNeA);
+  up(&m.G23, d->GI3 + NpNeA, NpNeA);
+  up(&m.GsqrtH, d->GsqrtH, n2d);
+  std::vector<double> ones(n2d, 1.0), zeros(n2d, 0.0);
+  up(&m.G11, d->G11 ? d->G11 : ones.data(), n2d);
+  up(&m.G12, d->G12 ? d->G12 : zeros.data(), n2d);
+  up(&m.G22, d->G22 ? d->G22 : ones.data(), n2d);
+  up(&m.nx, d->normal_fn, nf);
+  up(&m.ny, d->normal_fn + nf, nf);
+  up(&m.nz, d->normal_fn + 2 * nf, nf);
+  up(&m.Fscale, d->Fscale, nf);
+  // integer maps to 0-based device copies
+  std::vector<int32_t> vm(nf), vp(nf), em(m.Ne), hs(m.nB > 0 ? m.nB : 0), nbr(nf);
+  std::vector<int8_t> nbc(nf, 0);
+  bool ok_maps = true;
+  for (size_t t = 0; t < nf; ++t) {
+    vm[t] = d->vmapM[t] - 1;
+    vp[t] = d->vmapP[t] - 1;
+    if (vm[t] < 0 || (size_t)vm[t] >= NpNe || vp[t] < 0 || (size_t)vp[t] >= NpNe + (size_t)m.nB) ok_maps = false;
+  }
+  for (int t = 0; t < m.Ne; ++t) {
+    em[t] = d->EMap3Dto2D[t] - 1;
+    if (em[t] < 0 || em[t] >= m.Ne2D) ok_maps = false;
+  }
+  for (int b = 0; b < m.nB; ++b) {
+    hs[b] = d->halo_src[b] - 1;
+    if (hs[b] < 0 || (size_t)hs[b] >= NpNe) ok_maps = false;
+  }
+  if (!ok_maps) {
+    set_error("fedg_ctx_create: index map out of range");
+    free_mesh(m);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return FEDG_ERR_ARG;
+  }
+  // exterior value of every face node resolved to an interior node, with the
+  // velocity BC that applyBC would apply in the halo slot
+  for (size_t t = 0; t < nf; ++t) {
+    if ((size_t)vp[t] < NpNe) {
+      nbr[t] = vp[t];
+    } else {
+      const int b = (int)((size_t)vp[t] - NpNe);
+      nbr[t] = hs[b];
+      nbc[t] = (int8_t)d->velbc[b];
+    }
+  }
+  up(&m.vmapM, vm.data(), nf);
+  up(&m.vmapP, vp.data(), nf);
+  up(&m.EMap3Dto2D, em.data(), (size_t)m.Ne);
+  up(&m.halo_src, hs.data(), (size_t)m.nB);
+  up(&m.velbc, d->velbc, (size_t)m.nB);
+  up(&m.nbr, nbr.data(), nf);
+  up(&m.nbr_bc, nbc.data(), nf);
+  up(&m.D1D, d->D1D, (size_t)n1 * n1);
+  up(&m.Lift, d->Lift_mat, (size_t)Np * 6);
+  up(&m.Itr, d->IntrpMat_VPOrdM1_tr, (size_t)n1 * n1);
+  if (rc != FEDG_OK) {
+    free_mesh(m);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return rc;
+  }
+
+  // Detect the flat-terrain uniform box: unit Gsqrt/GsqrtH, zero GI3, metric
+  // tensor (1,0,1), axis-aligned unit normals, Escale diagonal and Fscale
+  // constant over the mesh, and a face map that is the box's own.
+  m.flat = 0;
+  if (d->NeX > 0 && d->NeY > 0 && d->NeZ > 0 && d->NeX * d->NeY * d->NeZ == d->Ne && d->Ne2D == d->NeX * d->NeY) {
+    bool flat = all_equal(d->Gsqrt, NpNeA, 1.0) && all_equal(d->GI3, 2 * NpNeA, 0.0) &&
+                all_equal(d->GsqrtH, n2d, 1.0) && (!d->G11 || all_equal(d->G11, n2d, 1.0)) &&
+                (!d->G12 || all_equal(d->G12, n2d, 0.0)) && (!d->G22 || all_equal(d->G22, n2d, 1.0));
+    const double E0[3] = {d->Escale[0], d->Escale[NpNe * 4], d->Escale[NpNe * 8]};
+    for (int a = 0; a < 3 && flat; ++a)
+      for (int b = 0; b < 3 && flat; ++b) {
+        const double* E = d->Escale + NpNe * (a + 3 * b);
+        flat = all_equal(E, NpNe, a == b ? E0[a] : 0.0);
+      }
+    double F0[6];
+    for (int f = 0; f < 6; ++f) F0[f] = d->Fscale[f * Nfp];
+    const double nref[6][3] = {{0, -1, 0}, {1, 0, 0}, {0, 1, 0}, {-1, 0, 0}, {0, 0, -1}, {0, 0, 1}};
+    for (size_t t = 0; t < nf && flat; ++t) {
+      const int f = (int)((t % NfpTot) / Nfp);
+      if (d->Fscale[t] != F0[f]) flat = false;
+      for (int dd = 0; dd < 3; ++dd)
+        if (d->normal_fn[t + nf * dd] != nref[f][dd]) flat = false;
+    }
+    // the box's own face map, and one velocity BC per closed boundary
+    int bnd_bc[6] = {-1, -1, -1, -1, -1, -1};
+    int periodic[3] = {1, 1, 1};
+    for (int ke = 0; ke < m.Ne && flat; ++ke) {
+      const int ei = ke % m.NeX, ej = (ke / m.NeX) % m.NeY, ek = ke / (m.NeX * m.NeY);
+      for (int fp = 0; fp < NfpTot && flat; ++fp) {
+        const size_t t = fp + (size_t)NfpTot * ke;
+        const int f = fp / Nfp;
+        int ni = ei, nj = ej, nk = ek, axis;
+        switch (f) {
+          case 0: nj -= 1; axis = 1; break;
+          case 1: ni += 1; axis = 0; break;
+          case 2: nj += 1; axis = 1; break;
+          case 3: ni -= 1; axis = 0; break;
+          case 4: nk -= 1; axis = 2; break;
+          default: nk += 1; axis = 2; break;
+        }
+        const bool outside = ni < 0 || ni >= m.NeX || nj < 0 || nj >= m.NeY || nk < 0 || nk >= m.NeZ;
+        if (!outside) {
+          if ((size_t)vp[t] >= NpNe || vp[t] / Np != ni + m.NeX * (nj + m.NeY * nk)) flat = false;
+          continue;
+        }
+        if ((size_t)vp[t] < NpNe) {
+          flat = false;
+          continue;
+        }
+        const int own_elem = ke;
+        const int src_elem = nbr[t] / Np;
+        if (src_elem == own_elem) {
+          periodic[axis] = 0;
+          if (nbr[t] != vm[t]) flat = false;
+          if (bnd_bc[f] < 0) bnd_bc[f] = nbc[t];
+          else if (bnd_bc[f] != nbc[t]) flat = false;
+        } else {
+          if (nbc[t] != FEDG_BND_NONE) flat = false;
+          const int wi = (ni + m.NeX) % m.NeX, wj = (nj + m.NeY) % m.NeY, wk = (nk + m.NeZ) % m.NeZ;
+          if (src_elem != wi + m.NeX * (wj + m.NeY * wk)) flat = false;
+        }
+      }
+    }
+    if (flat) {
+      m.flat = 1;
+      for (int a = 0; a < 3; ++a) m.fE[a] = E0[a];
+      for (int f = 0; f < 6; ++f) {
+        m.fF[f] = F0[f];
+        m.bnd_bc[f] = bnd_bc[f] < 0 ? FEDG_BND_NONE : bnd_bc[f];
+      }
+      for (int a = 0; a < 3; ++a) m.periodic[a] = periodic[a];
+    }
+  }
+  *out = ctx;
+  return FEDG_OK;
+}
+
+int fedg_ctx_destroy(fedg_ctx* ctx) {
+  if (!ctx) return FEDG_OK;
+  free_mesh(ctx->m);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return FEDG_OK;
+}
+
+int fedg_ctx_sizes(const fedg_ctx* ctx, fedg_box_sizes* s) {
+  if (!ctx || !s) return FEDG_ERR_ARG;
+  const DevMesh& m = ctx->m;
+  s->n1 = m.n1;
+  s->Np = m.Np;
+  s->Nfp_h = m.Nfp;
+  s->Nfp_v = m.Nfp;
+  s->NfpTot = m.NfpTot;
+  s->Ne = m.Ne;
+  s->NeA = m.NeA;
+  s->Ne2D = m.Ne2D;
+  s->Ne2DA = m.Ne2DA;
+  s->nB = m.nB;
+  return FEDG_OK;
+}
+
+// ---------------------------------------------------------------------------
+int fedg_drhot2pres(fedg_ctx* ctx, double* PRES, double* DPRES, const double* DRHOT, const double* PRES_hyd,
+                    const double* THERM_hyd, const double* Rtot, const double* CVtot, const double* CPtot) {
+  if (!ctx || !PRES || !DPRES || !DRHOT || !PRES_hyd || !THERM_hyd || !Rtot || !CVtot || !CPtot)
+    return FEDG_ERR_ARG;
+  TRY(launch_drhot2pres(ctx, PRES, DPRES, DRHOT, PRES_hyd, THERM_hyd, Rtot, CVtot, CPtot,
+                        (long)ctx->m.Np * ctx->m.Ne));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return FEDG_OK;
+}
+
+int fedg_halo_exchange(fedg_ctx* ctx, int nfields, double* const* fields) {
+  if (!ctx || nfields < 0 || (nfields > 0 && !fields)) return FEDG_ERR_ARG;
+  for (int v = 0; v < nfields; ++v) TRY(launch_halo_exchange(ctx, fields[v]));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return FEDG_OK;
+}
+
+int fedg_apply_bc_prgvars(fedg_ctx* ctx, double* DDENS, double* MOMX, double* MOMY, double* MOMZ, double* THERM,
+                          const double* DENS_hyd, const double* PRES_hyd) {
+  (void)DDENS;
+  (void)THERM;
+  (void)DENS_hyd;
+  (void)PRES_hyd;
+  if (!ctx || !MOMX || !MOMY || !MOMZ) return FEDG_ERR_ARG;
+  TRY(launch_apply_bc(ctx, MOMX, MOMY, MOMZ));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return FEDG_OK;
+}
+
+int fedg_heve_cal_tend(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, double* MOMY_dt, double* MOMZ_dt,
+                       double* RHOT_dt, const double* DDENS_, const double* MOMX_, const double* MOMY_,
+                       const double* MOMZ_, const double* DRHOT_, const double* DPRES_, const double* DENS_hyd,
+                       const double* PRES_hyd, const double* PRES_hyd_ref, const double* THERM_hyd,
+                       const double* CORIOLIS, const double* Rtot, const double* CVtot, const double* CPtot,
+                       const double* DPhydDx, const double* DPhydDy) {
+  (void)PRES_hyd_ref;
+  if (!ctx || !DENS_dt || !MOMX_dt || !MOMY_dt || !MOMZ_dt || !RHOT_dt || !DDENS_ || !MOMX_ || !MOMY_ || !MOMZ_ ||
+      !DRHOT_ || !DPRES_ || !DENS_hyd || !PRES_hyd || !THERM_hyd || !CORIOLIS || !DPhydDx || !DPhydDy)
+    return FEDG_ERR_ARG;
+  TendIn in{};
+  in.q[0] = DDENS_;
+  in.q[1] = DRHOT_;
+  in.q[2] = MOMZ_;
+  in.q[3] = MOMX_;
+  in.q[4] = MOMY_;
+  in.DPRES = DPRES_;
+  in.DENS_hyd = DENS_hyd;
+  in.PRES_hyd = PRES_hyd;
+  in.THERM_hyd = THERM_hyd;
+  in.CORIOLIS = CORIOLIS;
+  in.DPhydDx = DPhydDx;
+  in.DPhydDy = DPhydDy;
+  in.Rtot = Rtot;
+  in.CVtot = CVtot;
+  in.CPtot = CPtot;
+  double* out[5] = {DENS_dt, RHOT_dt, MOMZ_dt, MOMX_dt, MOMY_dt};
+  TRY(launch_heve_dropin(ctx, in, out));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return FEDG_OK;
+}
+
+int fedg_calc_phyd_hgrad(fedg_ctx* ctx, double* DPhydDx, double* DPhydDy, const double* PRES_hyd,
+                         const double* PRES_hyd_ref) {
+  if (!ctx || !DPhydDx || !DPhydDy || !PRES_hyd || !PRES_hyd_ref) return FEDG_ERR_ARG;
+  TRY(launch_phyd_hgrad(ctx, DPhydDx, DPhydDy, PRES_hyd, PRES_hyd_ref));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return FEDG_OK;
+}
+
+// ---------------------------------------------------------------------------
+int fedg_tint_create(fedg_ctx* ctx, const char* scheme, double dt, fedg_tint** out) {
+  if (!ctx || !scheme || !out) return FEDG_ERR_ARG;
+  *out = nullptr;
+  fedg_tint* t = new fedg_tint();
+  memset(t, 0, sizeof(*t));
+  t->ctx = ctx;
+  int rc = butcher(scheme, dt, t);
+  if (rc != FEDG_OK) {
+    delete t;
+    return rc;
+  }
+  const size_t n = (size_t)ctx->m.Np * ctx->m.NeA * 5;
+  rc = dev_upload<double>(&t->var0, nullptr, n);
+  if (rc == FEDG_OK) rc = dev_upload<double>(&t->varTmp, nullptr, n);
+  if (rc == FEDG_OK) rc = dev_upload<double>(&t->tend_ex, nullptr, n * t->tend_buf_size);
+  if (rc == FEDG_OK && t->imex) rc = dev_upload<double>(&t->tend_im, nullptr, n * t->tend_buf_size);
+  if (rc != FEDG_OK) {
+    fedg_tint_destroy(t);
+    return rc;
+  }
+  *out = t;
+  return FEDG_OK;
+}
+
+int fedg_tint_destroy(fedg_tint* t) {
+  if (!t) return FEDG_OK;
+  double* ptrs[] = {t->var0, t->varTmp, t->tend_ex, t->tend_im};
+  for (double* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete t;
+  return FEDG_OK;
+}
+
+int fedg_tint_info(const fedg_tint* t, int* nstage, int* tend_buf_size, int* imex) {
+  if (!t) return FEDG_ERR_ARG;
+  if (nstage) *nstage = t->nstage;
+  if (tend_buf_size) *tend_buf_size = t->tend_buf_size;
+  if (imex) *imex = t->imex;
+  return FEDG_OK;
+}
+
+static double* tend_ptr(fedg_tint* t, double* base, int buf, int varID) {
+  const size_t NpNeA = (size_t)t->ctx->m.Np * t->ctx->m.NeA;
+  return base + ((size_t)buf * 5 + (size_t)(varID - 1)) * NpNeA;
+}
+
+double* fedg_tint_tend_ex(fedg_tint* t, int stage, int varID) {
+  if (!t || stage < 1 || stage > t->nstage || varID < 1 || varID > 5) return nullptr;
+  return tend_ptr(t, t->tend_ex, t->indmap[stage - 1], varID);
+}
+double* fedg_tint_tend_im(fedg_tint* t, int stage, int varID) {
+  if (!t || !t->tend_im || stage < 1 || stage > t->nstage || varID < 1 || varID > 5) return nullptr;
+  return tend_ptr(t, t->tend_im, t->indmap[stage - 1], varID);
+}
+
+int fedg_tint_store_var0(fedg_tint* t, const double* q, int varID) {
+  if (!t || !q || varID < 1 || varID > 5) return FEDG_ERR_ARG;
+  fedg_ctx* ctx = t->ctx;
+  const size_t NpNeA = (size_t)ctx->m.Np * ctx->m.NeA;
+  TRY(launch_copy(ctx, t->var0 + (size_t)(varID - 1) * NpNeA, q, (long)ctx->m.Np * ctx->m.Ne));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return FEDG_OK;
+}
+
+// rk_advance_general2D (CMN/scale_timeint_rk.F90:2201-2355) as streaming passes
+static int tint_advance_one(fedg_tint* t, int s, double* q, int varID) {
+  fedg_ctx* ctx = t->ctx;
+  const long n = (long)ctx->m.Np * ctx->m.Ne;
+  const size_t NpNeA = (size_t)ctx->m.Np * ctx->m.NeA;
+  double* v0 = t->var0 + (size_t)(varID - 1) * NpNeA;
+  double* vt = t->varTmp + (size_t)(varID - 1) * NpNeA;
+  const int ind = t->indmap[s - 1];
+  double* Te = tend_ptr(t, t->tend_ex, ind, varID);
+  double* Ti = t->imex ? tend_ptr(t, t->tend_im, ind, varID) : nullptr;
+  if (t->nstage == 1) TRY(launch_copy(ctx, vt, q, n));
+  if (s == t->nstage) {
+    const double cbe = t->b_ex[s - 1] * t->dt;
+    if (t->imex) return launch_axpy2_set(ctx, q, vt, cbe, Te, t->b_im[s - 1] * t->dt, Ti, n);
+    return launch_axpy_set(ctx, q, vt, cbe, Te, n);
+  }
+  if (s == 1 && !t->imex) {
+    TRY(launch_copy(ctx, v0, q, n));
+    TRY(launch_copy(ctx, vt, q, n));
+  }
+  const double cbe = t->b_ex[s - 1] * t->dt;
+  TRY(launch_copy(ctx, q, v0, n));
+  if (t->imex) TRY(launch_axpy2_acc(ctx, vt, cbe, Te, t->b_im[s - 1] * t->dt, Ti, n));
+  else TRY(launch_axpy_acc(ctx, vt, cbe, Te, n));
+  if (t->tend_buf_size == 1 && !t->imex) {
+    return launch_axpy_set(ctx, q, v0, t->dt * t->a_ex[s][s - 1], tend_ptr(t, t->tend_ex, 0, varID), n);
+  } else if (!t->imex) {
+    for (int s2 = 1; s2 <= s; ++s2)
+      TRY(launch_axpy_acc(ctx, q, t->dt * t->a_ex[s][s2 - 1], tend_ptr(t, t->tend_ex, s2 - 1, varID), n));
+  } else {
+    for (int s2 = 1; s2 <= s; ++s2)
+      TRY(launch_axpy2_acc(ctx, q, t->dt * t->a_ex[s][s2 - 1], tend_ptr(t, t->tend_ex, s2 - 1, varID),
+                           t->dt * t->a_im[s][s2 - 1], tend_ptr(t, t->tend_im, s2 - 1, varID), n));
+  }
+  return FEDG_OK;
+}
+
+int fedg_tint_advance(fedg_tint* t, int stage, int nvar, double* const* q, const int* varIDs) {
+  if (!t || stage < 1 || stage > t->nstage || nvar < 0 || (nvar > 0 && (!q || !varIDs))) return FEDG_ERR_ARG;
+  for (int v = 0; v < nvar; ++v) {
+    if (varIDs[v] < 1 || varIDs[v] > 5 || !q[v]) return FEDG_ERR_ARG;
+    TRY(tint_advance_one(t, stage, q[v], varIDs[v]));
+  }
+  FEDG_HIP_CHECK(hipStreamSynchronize(t->ctx->stream));
+  return FEDG_OK;
+}
+
+int fedg_tint_store_implicit(fedg_tint* t, int stage, double* q, int varID) {
+  if (!t || !q || stage < 1 || stage > t->nstage || varID < 1 || varID > 5) return FEDG_ERR_ARG;
+  if (!t->imex) return FEDG_OK;
+  fedg_ctx* ctx = t->ctx;
+  const long n = (long)ctx->m.Np * ctx->m.Ne;
+  const size_t NpNeA = (size_t)ctx->m.Np * ctx->m.NeA;
+  if (stage == 1) {
+    TRY(launch_copy(ctx, t->var0 + (size_t)(varID - 1) * NpNeA, q, n));
+    TRY(launch_copy(ctx, t->varTmp + (size_t)(varID - 1) * NpNeA, q, n));
+  }
+  const int ind = t->indmap[stage - 1];
+  TRY(launch_axpy_acc(ctx, q, t->dt * t->a_im[stage - 1][stage - 1], tend_ptr(t, t->tend_im, ind, varID), n));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return FEDG_OK;
+}
+
+// ---------------------------------------------------------------------------
+int fedg_heve_create(fedg_ctx* ctx, const char* scheme, double dt, const double* DENS_hyd, const double* PRES_hyd,
+                     const double* THERM_hyd, const double* CORIOLIS, const double* DPhydDx, const double* DPhydDy,
+                     const double* Rtot, const double* CVtot, const double* CPtot, fedg_heve** out) {
+  if (!ctx || !scheme || !out || !DENS_hyd || !PRES_hyd || !THERM_hyd || !CORIOLIS || !DPhydDx || !DPhydDy)
+    return FEDG_ERR_ARG;
+  if ((Rtot || CVtot || CPtot) && !(Rtot && CVtot && CPtot)) {
+    set_error("Rtot, CVtot and CPtot must be given together");
+    return FEDG_ERR_ARG;
+  }
+  *out = nullptr;
+  fedg_heve* st = new fedg_heve();
+  memset(st, 0, sizeof(*st));
+  st->ctx = ctx;
+  int rc = butcher(scheme, dt, &st->tab);
+  if (rc == FEDG_OK && (st->tab.imex || st->tab.tend_buf_size != 1)) {
+    set_error("the fused HEVE stepper takes explicit schemes with one tendency buffer (ERK_4s4o, ERK_1s1o)");
+    rc = FEDG_ERR_UNSUPPORTED;
+  }
+  const DevMesh& m = ctx->m;
+  const size_t NpNe = (size_t)m.Np * m.Ne, cor = (size_t)m.Nfp * m.Ne2DA;
+  // Rtot/CVtot/CPtot equal to the dry constants everywhere are dropped (the
+  // kernel then uses the constants, which is the same arithmetic)
+  bool dry = true;
+  if (Rtot)
+    dry = all_equal(Rtot, NpNe, ctx->c.Rdry) && all_equal(CVtot, NpNe, ctx->c.CVdry) &&
+          all_equal(CPtot, NpNe, ctx->c.CPdry);
+  st->flat = m.flat && dry;
+  const size_t nfield = 15 + 5 + 5 + (dry ? 0 : 3) + 2;
+  const size_t total = NpNe * nfield + cor;
+  if (rc == FEDG_OK) rc = dev_upload<double>(&st->block, nullptr, total);
+  if (rc != FEDG_OK) {
+    if (st->block) (void)hipFree(st->block);
+    delete st;
+    return rc;
+  }
+  double* p = st->block;
+  for (int b = 0; b < 3; ++b)
+    for (int v = 0; v < 5; ++v, p += NpNe) st->buf[b][v] = p;
+  for (int v = 0; v < 5; ++v, p += NpNe) st->VT[v] = p;
+  st->DENS_hyd = p; p += NpNe;
+  st->PRES_hyd = p; p += NpNe;
+  st->THERM_hyd = p; p += NpNe;
+  st->DPhydDx = p; p += NpNe;
+  st->DPhydDy = p; p += NpNe;
+  if (!dry) {
+    st->Rtot = p; p += NpNe;
+    st->CVtot = p; p += NpNe;
+    st->CPtot = p; p += NpNe;
+  }
+  st->pres_scratch = p; p += 2 * NpNe;
+  st->CORIOLIS = p;
+  struct {
+    double* d;
+    const double* h;
+    size_t n;
+  } ups[] = {{st->DENS_hyd, DENS_hyd, NpNe}, {st->PRES_hyd, PRES_hyd, NpNe}, {st->THERM_hyd, THERM_hyd, NpNe},
+             {st->DPhydDx, DPhydDx, NpNe},   {st->DPhydDy, DPhydDy, NpNe},   {st->CORIOLIS, CORIOLIS, cor},
+             {st->Rtot, Rtot, NpNe},         {st->CVtot, CVtot, NpNe},       {st->CPtot, CPtot, NpNe}};
+  for (auto& u : ups) {
+    if (!u.d) continue;
+    hipError_t e = hipMemcpy(u.d, u.h, sizeof(double) * u.n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      set_error(std::string("hipMemcpy: ") + hipGetErrorString(e));
+      fedg_heve_destroy(st);
+      return FEDG_ERR_HIP;
+    }
+  }
+  st->q0 = 0;
+  *out = st;
+  return FEDG_OK;
+}
+
+int fedg_heve_destroy(fedg_heve* st) {
+  if (!st) return FEDG_OK;
+  if (st->block) (void)hipFree(st->block);
+  delete st;
+  return FEDG_OK;
+}
+
+int fedg_heve_is_flat(const fedg_heve* st) { return st ? st->flat : 0; }
+
+int fedg_heve_set_state(fedg_heve* st, const double* const* q) {
+  if (!st || !q) return FEDG_ERR_ARG;
+  const size_t NpNe = (size_t)st->ctx->m.Np * st->ctx->m.Ne;
+  for (int v = 0; v < 5; ++v) FEDG_HIP_CHECK(hipMemcpy(st->buf[st->q0][v], q[v], sizeof(double) * NpNe, hipMemcpyHostToDevice));
+  return FEDG_OK;
+}
+
+int fedg_heve_get_state(fedg_heve* st, double* const* q) {
+  if (!st || !q) return FEDG_ERR_ARG;
+  const size_t NpNe = (size_t)st->ctx->m.Np * st->ctx->m.Ne;
+  FEDG_HIP_CHECK(hipStreamSynchronize(st->ctx->stream));
+  for (int v = 0; v < 5; ++v) FEDG_HIP_CHECK(hipMemcpy(q[v], st->buf[st->q0][v], sizeof(double) * NpNe, hipMemcpyDeviceToHost));
+  return FEDG_OK;
+}
+
+int fedg_heve_state_dev(fedg_heve* st, double** q) {
+  if (!st || !q) return FEDG_ERR_ARG;
+  for (int v = 0; v < 5; ++v) q[v] = st->buf[st->q0][v];
+  return FEDG_OK;
+}
+
+// One RK step: one fused kernel per stage.  Stage inputs and outputs are
+// distinct buffers because neighbouring elements read the stage input.
+static int heve_step(fedg_heve* st, std::vector<hipEvent_t>* evs, size_t* ev_i) {
+  fedg_ctx* ctx = st->ctx;
+  const fedg_tint& t = st->tab;
+  TendIn in{};
+  in.DENS_hyd = st->DENS_hyd;
+  in.PRES_hyd = st->PRES_hyd;
+  in.THERM_hyd = st->THERM_hyd;
+  in.CORIOLIS = st->CORIOLIS;
+  in.DPhydDx = st->DPhydDx;
+  in.DPhydDy = st->DPhydDy;
+  in.Rtot = st->Rtot;
+  in.CVtot = st->CVtot;
+  in.CPtot = st->CPtot;
+  const int q0 = st->q0;
+  int cur = q0;
+  int last_out = q0;
+  for (int s = 1; s <= t.nstage; ++s) {
+    RkOut rk{};
+    int outb;
+    if (s == t.nstage) {
+      outb = (t.nstage > 1) ? q0 : (q0 + 1) % 3;
+      rk.mode = (t.nstage > 1) ? 2 : 3;
+    } else {
+      outb = (s == 1) ? (q0 + 1) % 3 : 3 - q0 - cur;  // the buffer that is neither q^n nor the input
+      rk.mode = (s == 1) ? 0 : 1;
+    }
+    for (int v = 0; v < 5; ++v) {
+      in.q[v] = st->buf[cur][v];
+      rk.out[v] = st->buf[outb][v];
+      rk.var0[v] = st->buf[q0][v];
+      rk.varTmp[v] = st->VT[v];
+    }
+    rk.cb = t.b_ex[s - 1] * t.dt;
+    rk.ca = (s < t.nstage) ? t.dt * t.a_ex[s][s - 1] : 0.0;
+    if (evs) FEDG_HIP_CHECK(hipEventRecord((*evs)[(*ev_i)++], ctx->stream));
+    TRY(launch_heve_fused(ctx, in, rk, st->flat));
+    if (evs) FEDG_HIP_CHECK(hipEventRecord((*evs)[(*ev_i)++], ctx->stream));
+    cur = outb;
+    last_out = outb;
+  }
+  st->q0 = last_out;
+  return FEDG_OK;
+}
+
+int fedg_heve_run(fedg_heve* st, int nsteps) {
+  if (!st || nsteps < 0) return FEDG_ERR_ARG;
+  for (int n = 0; n < nsteps; ++n) TRY(heve_step(st, nullptr, nullptr));
+  FEDG_HIP_CHECK(hipStreamSynchronize(st->ctx->stream));
+  return FEDG_OK;
+}
+
+int fedg_heve_run_timed(fedg_heve* st, int nsteps, double* ms_total, double* ms_per_stage_kernel) {
+  if (!st || nsteps < 1) return FEDG_ERR_ARG;
+  fedg_ctx* ctx = st->ctx;
+  const size_t nev = (size_t)nsteps * st->tab.nstage * 2;
+  std::vector<hipEvent_t> evs(nev);
+  for (auto& e : evs) FEDG_HIP_CHECK(hipEventCreate(&e));
+  hipEvent_t t0, t1;
+  FEDG_HIP_CHECK(hipEventCreate(&t0));
+  FEDG_HIP_CHECK(hipEventCreate(&t1));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  FEDG_HIP_CHECK(hipEventRecord(t0, ctx->stream));
+  size_t ev_i = 0;
+  int rc = FEDG_OK;
+  for (int n = 0; n < nsteps && rc == FEDG_OK; ++n) rc = heve_step(st, &evs, &ev_i);
+  FEDG_HIP_CHECK(hipEventRecord(t1, ctx->stream));
+  FEDG_HIP_CHECK(hipEventSynchronize(t1));
+  if (rc == FEDG_OK) {
+    float tot = 0.f;
+    FEDG_HIP_CHECK(hipEventElapsedTime(&tot, t0, t1));
+    double ksum = 0.0;
+    for (size_t e = 0; e + 1 < ev_i; e += 2) {
+      float ms = 0.f;
+      FEDG_HIP_CHECK(hipEventElapsedTime(&ms, evs[e], evs[e + 1]));
+      ksum += ms;
+    }
+    if (ms_total) *ms_total = tot;
+    if (ms_per_stage_kernel) *ms_per_stage_kernel = ksum / (double)(ev_i / 2);
+  }
+  for (auto& e : evs) (void)hipEventDestroy(e);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  return rc;
+}
+
+int fedg_heve_get_pressure(fedg_heve* st, double* PRES_host, double* DPRES_host) {
+  if (!st || !PRES_host || !DPRES_host) return FEDG_ERR_ARG;
+  fedg_ctx* ctx = st->ctx;
+  const size_t NpNe = (size_t)ctx->m.Np * ctx->m.Ne;
+  double* PRES = st->pres_scratch;
+  double* DPRES = st->pres_scratch + NpNe;
+  // dry constants as fields when the stepper dropped them
+  double *R = st->Rtot, *Cv = st->CVtot, *Cp = st->CPtot;
+  std::vector<double*> tmp;
+  if (!R) {
+    for (int a = 0; a < 3; ++a) {
+      double* d = nullptr;
+      TRY(dev_upload<double>(&d, nullptr, NpNe));
+      std::vector<double> h(NpNe, a == 0 ? ctx->c.Rdry : a == 1 ? ctx->c.CVdry : ctx->c.CPdry);
+      FEDG_HIP_CHECK(hipMemcpy(d, h.data(), sizeof(double) * NpNe, hipMemcpyHostToDevice));
+      tmp.push_back(d);
+    }
+    R = tmp[0];
+    Cv = tmp[1];
+    Cp = tmp[2];
+  }
+  int rc = launch_drhot2pres(ctx, PRES, DPRES, st->buf[st->q0][1], st->PRES_hyd, st->THERM_hyd, R, Cv, Cp,
+                             (long)NpNe);
+  if (rc == FEDG_OK) {
+    FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    FEDG_HIP_CHECK(hipMemcpy(PRES_host, PRES, sizeof(double) * NpNe, hipMemcpyDeviceToHost));
+    FEDG_HIP_CHECK(hipMemcpy(DPRES_host, DPRES, sizeof(double) * NpNe, hipMemcpyDeviceToHost));
+  }
+  for (double* d : tmp) (void)hipFree(d);
+  return rc;
+}
+
+}  // extern "C"

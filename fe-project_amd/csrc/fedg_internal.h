@@ -1,0 +1,111 @@
+// Internal definitions shared by the C-ABI layer and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/fedg.h"
+
+namespace fedg {
+
+void set_error(const std::string& msg);
+
+#define FEDG_HIP_CHECK(expr)                                                         \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      ::fedg::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));          \
+      return FEDG_ERR_HIP;                                                           \
+    }                                                                                \
+  } while (0)
+
+struct Consts {
+  double GRAV, Rdry, CPdry, CVdry, PRES00;
+};
+
+// Device copies of the flattened mesh / element / operator data.
+struct DevMesh {
+  int n1, Np, Nfp, NfpTot;
+  int Ne, NeA, Ne2D, Ne2DA, nB;
+  int NeX, NeY, NeZ;           // box shape (0 if unknown)
+  double* E11;                 // (Np,Ne) Escale(:,:,1,1)
+  double* E22;                 // (Np,Ne) Escale(:,:,2,2)
+  double* E33;                 // (Np,Ne) Escale(:,:,3,3)
+  double* Gsqrt;               // (Np,NeA)
+  double* G13;                 // (Np,NeA)
+  double* G23;                 // (Np,NeA)
+  double* GsqrtH;              // (Nfp,Ne2D)
+  double* G11; double* G12; double* G22;  // (Nfp,Ne2D)
+  double* nx; double* ny; double* nz;     // (NfpTot,Ne)
+  double* Fscale;              // (NfpTot,Ne)
+  int32_t* vmapM;              // (NfpTot,Ne) 0-based
+  int32_t* vmapP;              // (NfpTot,Ne) 0-based
+  int32_t* EMap3Dto2D;         // (Ne) 0-based
+  int32_t* halo_src;           // (nB) 0-based
+  int32_t* velbc;              // (nB)
+  int32_t* nbr;                // (NfpTot,Ne) 0-based interior node of the exterior value
+  int8_t* nbr_bc;              // (NfpTot,Ne) velocity BC applied to the exterior value
+  double* D1D;                 // (n1,n1)
+  double* Lift;                // (n1,n1,n1,6)
+  double* Itr;                 // (n1,n1)
+  // flat uniform box description (valid when flat != 0)
+  int flat;
+  double fE[3];                // E11,E22,E33 of every element
+  double fF[6];                // Fscale of every face
+  int bnd_bc[6];               // velocity BC of the six domain boundaries (closed ones)
+  int periodic[3];             // x, y, z
+};
+
+}  // namespace fedg
+
+struct fedg_ctx {
+  fedg::DevMesh m;
+  fedg::Consts c;
+  hipStream_t stream;
+  int device;
+};
+
+namespace fedg {
+
+// ---- kernel launchers (fedg_kernels.hip) ----
+struct TendIn {
+  const double* q[5];        // stage state, VID order (DDENS, DRHOT, MOMZ, MOMX, MOMY)
+  const double* DPRES;       // drop-in only
+  const double* DENS_hyd;
+  const double* PRES_hyd;
+  const double* THERM_hyd;
+  const double* CORIOLIS;
+  const double* DPhydDx;
+  const double* DPhydDy;
+  const double* Rtot;        // fused: per node or null (dry constants)
+  const double* CVtot;
+  const double* CPtot;
+};
+
+struct RkOut {
+  double* out[5];            // drop-in: tendencies; fused: next stage state
+  const double* var0[5];     // fused: q^n
+  double* varTmp[5];         // fused: RK accumulator
+  double ca, cb;             // dt*a(s+1,s), dt*b(s)
+  int mode;                  // 0: first stage, 1: middle, 2: last, 3: single-stage scheme
+};
+
+int launch_heve_dropin(fedg_ctx* ctx, const TendIn& in, double* const out[5]);
+int launch_heve_fused(fedg_ctx* ctx, const TendIn& in, const RkOut& rk, int use_flat);
+int launch_drhot2pres(fedg_ctx* ctx, double* PRES, double* DPRES, const double* DRHOT,
+                      const double* PRES_hyd, const double* THERM_hyd, const double* Rtot,
+                      const double* CVtot, const double* CPtot, long n);
+int launch_halo_exchange(fedg_ctx* ctx, double* field);
+int launch_apply_bc(fedg_ctx* ctx, double* MOMX, double* MOMY, double* MOMZ);
+int launch_phyd_hgrad(fedg_ctx* ctx, double* DPhydDx, double* DPhydDy, const double* PRES_hyd,
+                      const double* PRES_hyd_ref);
+// generic streaming RK update pieces
+int launch_axpy_set(fedg_ctx* ctx, double* y, const double* a, double ca, const double* x, long n);  // y = a + ca*x
+int launch_axpy_acc(fedg_ctx* ctx, double* y, double ca, const double* x, long n);                   // y = y + ca*x
+int launch_axpy2_acc(fedg_ctx* ctx, double* y, double ca, const double* x, double cb, const double* z, long n);
+int launch_axpy2_set(fedg_ctx* ctx, double* y, const double* a, double ca, const double* x, double cb,
+                     const double* z, long n);  // y = a + ca*x + cb*z
+int launch_copy(fedg_ctx* ctx, double* y, const double* x, long n);
+
+}  // namespace fedg

@@ -1,0 +1,683 @@
+// CDNA4 (gfx950) kernels for the HEVE step of FElib's regional nonhydrostatic
+// DG dynamical core.
+//
+// Element kernel: one workgroup handles EPB elements with one thread per node
+// (Np = N1^3 threads per element, 512 at p=7 = 8 wavefronts, wavefront w =
+// horizontal node plane k=w).  Per element it
+//   1. loads the stage state and base state of its nodes (coalesced, 4 KB per
+//      field per element at p=7),
+//   2. evaluates the Rusanov flux at its NfpTot face nodes (one thread per
+//      face node, exterior values gathered from the neighbour element),
+//   3. forms the 15 pointwise fluxes and applies the 1D derivative matrix along
+//      x, y and z through LDS (one variable at a time, double-buffered), adds
+//      the lift of the face flux and the buoyancy / Coriolis / hydrostatic
+//      terms,
+//   4. either writes the five tendencies (drop-in cal_tend) or applies the RK
+//      stage update in place of the reference's separate streaming pass
+//      (fused stepper).
+// The arithmetic of every node follows the reference's expression order
+// (compiled with -ffp-contract=off; division and sqrt are IEEE correctly
+// rounded), and the pressure's pow is the portable double-double routine of
+// fedg_pow.h, so results agree bit for bit with a CPU evaluation of the same
+// expressions that uses the same pow.
+#include "fedg_internal.h"
+#include "fedg_pow.h"
+
+namespace fedg {
+namespace {
+
+enum { VD = 0, VT = 1, VW = 2, VU = 3, VV = 4 };  // DDENS, DRHOT, MOMZ, MOMX, MOMY
+
+enum { MODE_DROPIN = 0, MODE_FUSED_GENERAL = 1, MODE_FUSED_FLAT = 2 };
+
+// Values of one side of a face node (reference: ..._heve_numflux.F90:1028-1063).
+struct Side {
+  double Gsqrt, G13, G23;
+  double DD, MX, MY, MZ, DT, DP;
+  double Dhyd, Phyd, Thyd;
+};
+
+// Rusanov flux jump for the five variables at one face node, pre-scaled by
+// 0.5*Fscale (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve_numflux.F90:1024-1135).
+__device__ __forceinline__ void rusanov(const Side& M, const Side& P, double nx, double ny, double nz,
+                                        double Fscale, double gamm, double df[5]) {
+  const double RGsqrtV_M = 1.0 / M.Gsqrt, RGsqrtV_P = 1.0 / P.Gsqrt;
+  const double GsqrtDDENS_M = M.Gsqrt * M.DD, GsqrtDDENS_P = P.Gsqrt * P.DD;
+  const double GsqrtMOMX_M = M.Gsqrt * M.MX, GsqrtMOMX_P = P.Gsqrt * P.MX;
+  const double GsqrtMOMY_M = M.Gsqrt * M.MY, GsqrtMOMY_P = P.Gsqrt * P.MY;
+  const double GsqrtMOMZ_M = M.Gsqrt * M.MZ, GsqrtMOMZ_P = P.Gsqrt * P.MZ;
+  const double GsqrtDRHOT_M = M.Gsqrt * M.DT, GsqrtDRHOT_P = P.Gsqrt * P.DT;
+  const double GsqrtDens_M = GsqrtDDENS_M + M.Gsqrt * M.Dhyd;
+  const double GsqrtDens_P = GsqrtDDENS_P + P.Gsqrt * P.Dhyd;
+  const double GsqrtRhot_M = M.Gsqrt * M.Thyd + GsqrtDRHOT_M;
+  const double GsqrtRhot_P = P.Gsqrt * P.Thyd + GsqrtDRHOT_P;
+  const double Vel_M = (GsqrtMOMX_M * nx + GsqrtMOMY_M * ny +
+                        ((GsqrtMOMZ_M * RGsqrtV_M + M.G13 * GsqrtMOMX_M + M.G23 * GsqrtMOMY_M) * nz)) / GsqrtDens_M;
+  const double Vel_P = (GsqrtMOMX_P * nx + GsqrtMOMY_P * ny +
+                        ((GsqrtMOMZ_P * RGsqrtV_P + P.G13 * GsqrtMOMX_P + P.G23 * GsqrtMOMY_P) * nz)) / GsqrtDens_P;
+  double tmp1 = fabs(nx) + fabs(ny);
+  const double Gnn_M = tmp1 + (1.0 * (RGsqrtV_M * RGsqrtV_M) + M.G13 * M.G13 + M.G23 * M.G23) * fabs(nz);
+  const double Gnn_P = tmp1 + (1.0 * (RGsqrtV_P * RGsqrtV_P) + P.G13 * P.G13 + P.G23 * P.G23) * fabs(nz);
+  const double aM = sqrt(Gnn_M * gamm * (M.Phyd + M.DP) * M.Gsqrt / GsqrtDens_M) + fabs(Vel_M);
+  const double aP = sqrt(Gnn_P * gamm * (P.Phyd + P.DP) * P.Gsqrt / GsqrtDens_P) + fabs(Vel_P);
+  const double alpha = aM > aP ? aM : aP;
+
+  tmp1 = Fscale * 0.5;
+  double tmp2 = -alpha * (GsqrtDDENS_P - GsqrtDDENS_M);
+  df[VD] = tmp1 * (GsqrtDens_P * Vel_P - GsqrtDens_M * Vel_M + tmp2);
+  tmp2 = -alpha * (GsqrtDRHOT_P - GsqrtDRHOT_M);
+  df[VT] = tmp1 * (GsqrtRhot_P * Vel_P - GsqrtRhot_M * Vel_M + tmp2);
+  const double tmp3 = P.Gsqrt * P.DP;
+  const double tmp4 = M.Gsqrt * M.DP;
+  const double mom1 = (tmp3 * RGsqrtV_P - tmp4 * RGsqrtV_M) * nz;
+  const double mom2 = (nx + P.G13 * nz) * tmp3 - (nx + M.G13 * nz) * tmp4;
+  const double mom3 = (ny + P.G23 * nz) * tmp3 - (ny + M.G23 * nz) * tmp4;
+  tmp2 = -alpha * (GsqrtMOMZ_P - GsqrtMOMZ_M);
+  df[VW] = tmp1 * (GsqrtMOMZ_P * Vel_P - GsqrtMOMZ_M * Vel_M + mom1 + tmp2);
+  tmp2 = -alpha * (GsqrtMOMX_P - GsqrtMOMX_M);
+  df[VU] = tmp1 * (GsqrtMOMX_P * Vel_P - GsqrtMOMX_M * Vel_M + mom2 + tmp2);
+  tmp2 = -alpha * (GsqrtMOMY_P - GsqrtMOMY_M);
+  df[VV] = tmp1 * (GsqrtMOMY_P * Vel_P - GsqrtMOMY_M * Vel_M + mom3 + tmp2);
+}
+
+// Exterior momentum of a closed boundary (FDS/scale_atm_dyn_dgm_bnd.F90:335-359),
+// from the interior values of the same node.
+__device__ __forceinline__ void reflect_momentum(int bc, double GsqrtM, double GsqrtH, double G11_, double G12_,
+                                                 double G22_, double G13, double G23, double nx, double ny,
+                                                 double nz, double MX, double MY, double MZ, double& MXo,
+                                                 double& MYo, double& MZo) {
+  if (bc == FEDG_BND_SLIP) {
+    const double GsqrtV = GsqrtM / GsqrtH;
+    const double MOMW = MZ / GsqrtV + G13 * MX + G23 * MY;
+    const double t1 = GsqrtV * G13, t2 = GsqrtV * G23;
+    const double fac = nz * (GsqrtV * GsqrtV) /
+                       (1.0 + G11_ * (t1 * t1) + 2.0 * G12_ * ((GsqrtV * GsqrtV) * G13 * G23) + G22_ * (t2 * t2));
+    const double mn = MX * nx + MY * ny + MOMW * nz;
+    MXo = MX - 2.0 * mn * (nx + fac * (G11_ * G13 + G12_ * G23));
+    MYo = MY - 2.0 * mn * (ny + fac * (G12_ * G13 + G22_ * G23));
+    MZo = MZ - 2.0 * mn * fac / GsqrtV;
+  } else if (bc == FEDG_BND_NOSLIP) {
+    MXo = -MX;
+    MYo = -MY;
+    MZo = -MZ;
+  }
+}
+
+// Pressure perturbation of one node (FDS/scale_atm_dyn_dgm_nonhydro3d_common.F90:470-474).
+__device__ __forceinline__ double dpres_of(double DRHOT, double THERM_hyd, double PRES_hyd, double Rtot,
+                                           double CVtot, double CPtot, const Consts& c) {
+  const double rP0 = 1.0 / c.PRES00;
+  const double RHOT = THERM_hyd + DRHOT;
+  const double PRES = c.PRES00 * fedg_pow(Rtot * rP0 * RHOT, CPtot / CVtot);
+  return PRES - PRES_hyd;
+}
+
+// Element-local node of face node a on face f (faces in the reference order
+// y-, x+, y+, x-, z-, z+; ELM/scale_element_hexahedral.F90:173-180).
+template <int N1>
+__device__ __forceinline__ int face_node(int f, int a) {
+  constexpr int N = N1 - 1, n2 = N1 * N1;
+  const int a0 = a % N1, a1 = a / N1;
+  switch (f) {
+    case 0: return a0 + n2 * a1;
+    case 1: return N + N1 * a0 + n2 * a1;
+    case 2: return a0 + N1 * N + n2 * a1;
+    case 3: return N1 * a0 + n2 * a1;
+    case 4: return a;
+    default: return a + n2 * N;
+  }
+}
+
+// Spread the workgroups of one launch so that XCD x (workgroups x, x+8, ...
+// under round-robin dispatch) works on one contiguous 1/8 of the elements:
+// neighbouring elements then share that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  if ((nb & 7) != 0) return b;
+  return (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+template <int N1, int EPB, int MODE>
+__global__ void __launch_bounds__(EPB * N1 * N1 * N1)
+heve_kernel(const DevMesh m, const Consts c, const TendIn in, const RkOut rk) {
+  constexpr int Np = N1 * N1 * N1, Nfp = N1 * N1, NfpTot = 6 * Nfp;
+  constexpr bool DROPIN = MODE == MODE_DROPIN;
+  constexpr bool FLAT = MODE == MODE_FUSED_FLAT;
+  constexpr bool FUSED = !DROPIN;
+
+  __shared__ double sD[N1 * N1];
+  __shared__ double sI[N1 * N1];
+  __shared__ double sF[2][EPB][3][Np];
+  __shared__ double sDF[EPB][5][NfpTot];
+  __shared__ double sRho[EPB][Np];
+  __shared__ double sDP[EPB][FUSED ? Np : 1];
+
+  const int tid = threadIdx.x;
+  const int e = tid / Np;
+  const int p = tid - e * Np;
+  int blk = blockIdx.x;
+  if constexpr (FLAT) blk = xcd_remap(blk, gridDim.x);
+  const int ke = blk * EPB + e;
+  const bool valid = ke < m.Ne;
+  const int i = p % N1, j = (p / N1) % N1, k = p / Nfp;
+  const double gamm = c.CPdry / c.CVdry;
+
+  for (int t = tid; t < N1 * N1; t += blockDim.x) {
+    sD[t] = m.D1D[t];
+    sI[t] = m.Itr[t];
+  }
+
+  const size_t o = (size_t)Np * (valid ? ke : 0) + p;
+  double q[5] = {0, 0, 0, 0, 0};
+  double Dh = 0, Th = 0, DP = 0;
+  double Gs = 1.0, g13 = 0.0, g23 = 0.0, GsH = 1.0;
+  int ke2d = 0;
+  double Rt = c.Rdry, Cv = c.CVdry, Cp = c.CPdry;
+  if (valid) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) q[v] = in.q[v][o];
+    Dh = in.DENS_hyd[o];
+    Th = in.THERM_hyd[o];
+    if constexpr (FLAT) {
+      ke2d = ke % (m.NeX * m.NeY);
+    } else {
+      Gs = m.Gsqrt[o];
+      g13 = m.G13[o];
+      g23 = m.G23[o];
+      ke2d = m.EMap3Dto2D[ke];
+      GsH = m.GsqrtH[(i + N1 * j) + (size_t)Nfp * ke2d];
+    }
+    if constexpr (DROPIN) {
+      DP = in.DPRES[o];
+    } else {
+      if (in.Rtot != nullptr) {
+        Rt = in.Rtot[o];
+        Cv = in.CVtot[o];
+        Cp = in.CPtot[o];
+      }
+      DP = dpres_of(q[VT], Th, in.PRES_hyd[o], Rt, Cv, Cp, c);
+    }
+  }
+  sRho[e][p] = q[VD];
+  if constexpr (FUSED) sDP[e][p] = DP;
+  __syncthreads();
+
+  // ---- face phase: Rusanov flux at the element's face nodes ----
+  if (valid) {
+    const int ei = FLAT ? ke % m.NeX : 0;
+    const int ej = FLAT ? (ke / m.NeX) % m.NeY : 0;
+    const int ek = FLAT ? ke / (m.NeX * m.NeY) : 0;
+    for (int fp = p; fp < NfpTot; fp += Np) {
+      const int f = fp / Nfp, a = fp - f * Nfp;
+      const int pm = face_node<N1>(f, a);
+      const size_t gf = fp + (size_t)NfpTot * ke;
+      const size_t iM = DROPIN ? (size_t)m.vmapM[gf] : (size_t)Np * ke + pm;
+      Side M, P;
+      double nx, ny, nz, Fs;
+      if constexpr (FLAT) {
+        nx = (f == 1) ? 1.0 : (f == 3) ? -1.0 : 0.0;
+        ny = (f == 2) ? 1.0 : (f == 0) ? -1.0 : 0.0;
+        nz = (f == 5) ? 1.0 : (f == 4) ? -1.0 : 0.0;
+        Fs = m.fF[f];
+        M.Gsqrt = 1.0; M.G13 = 0.0; M.G23 = 0.0;
+        P.Gsqrt = 1.0; P.G13 = 0.0; P.G23 = 0.0;
+      } else {
+        nx = m.nx[gf];
+        ny = m.ny[gf];
+        nz = m.nz[gf];
+        Fs = m.Fscale[gf];
+        M.Gsqrt = m.Gsqrt[iM]; M.G13 = m.G13[iM]; M.G23 = m.G23[iM];
+      }
+      M.DD = in.q[VD][iM]; M.DT = in.q[VT][iM]; M.MZ = in.q[VW][iM]; M.MX = in.q[VU][iM]; M.MY = in.q[VV][iM];
+      M.Dhyd = in.DENS_hyd[iM]; M.Phyd = in.PRES_hyd[iM]; M.Thyd = in.THERM_hyd[iM];
+      if constexpr (DROPIN) M.DP = in.DPRES[iM];
+      else M.DP = sDP[e][pm];
+
+      size_t iP;
+      int bc = FEDG_BND_NONE;
+      if constexpr (DROPIN) {
+        iP = (size_t)m.vmapP[gf];
+      } else if constexpr (FLAT) {
+        int ni = ei, nj = ej, nk = ek;
+        bool closed = false;
+        switch (f) {
+          case 0: nj -= 1; if (nj < 0) { if (m.periodic[1]) nj = m.NeY - 1; else closed = true; } break;
+          case 1: ni += 1; if (ni >= m.NeX) { if (m.periodic[0]) ni = 0; else closed = true; } break;
+          case 2: nj += 1; if (nj >= m.NeY) { if (m.periodic[1]) nj = 0; else closed = true; } break;
+          case 3: ni -= 1; if (ni < 0) { if (m.periodic[0]) ni = m.NeX - 1; else closed = true; } break;
+          case 4: nk -= 1; if (nk < 0) { if (m.periodic[2]) nk = m.NeZ - 1; else closed = true; } break;
+          default: nk += 1; if (nk >= m.NeZ) { if (m.periodic[2]) nk = 0; else closed = true; } break;
+        }
+        if (closed) {
+          iP = iM;
+          bc = m.bnd_bc[f];
+        } else {
+          constexpr int opp[6] = {2, 3, 0, 1, 5, 4};
+          iP = (size_t)Np * (ni + m.NeX * (nj + m.NeY * nk)) + face_node<N1>(opp[f], a);
+        }
+      } else {
+        iP = (size_t)m.nbr[gf];
+        bc = m.nbr_bc[gf];
+      }
+      if constexpr (!FLAT) {
+        P.Gsqrt = m.Gsqrt[iP]; P.G13 = m.G13[iP]; P.G23 = m.G23[iP];
+      }
+      P.DD = in.q[VD][iP]; P.DT = in.q[VT][iP]; P.MZ = in.q[VW][iP]; P.MX = in.q[VU][iP]; P.MY = in.q[VV][iP];
+      P.Dhyd = in.DENS_hyd[iP]; P.Phyd = in.PRES_hyd[iP]; P.Thyd = in.THERM_hyd[iP];
+      if constexpr (DROPIN) {
+        P.DP = in.DPRES[iP];
+      } else {
+        double Rp = c.Rdry, Cvp = c.CVdry, Cpp = c.CPdry;
+        if (in.Rtot != nullptr) {
+          Rp = in.Rtot[iP];
+          Cvp = in.CVtot[iP];
+          Cpp = in.CPtot[iP];
+        }
+        P.DP = dpres_of(P.DT, P.Thyd, P.Phyd, Rp, Cvp, Cpp, c);
+      }
+      if (bc != FEDG_BND_NONE) {
+        // exterior momentum from the interior node, as applyBC writes it into the halo
+        double GH = 1.0, g11 = 1.0, g12 = 0.0, g22 = 1.0;
+        if constexpr (!FLAT) {
+          const size_t h = (size_t)(pm % Nfp) + (size_t)Nfp * m.EMap3Dto2D[ke];
+          GH = m.GsqrtH[h];
+          g11 = m.G11[h];
+          g12 = m.G12[h];
+          g22 = m.G22[h];
+        }
+        reflect_momentum(bc, M.Gsqrt, GH, g11, g12, g22, M.G13, M.G23, nx, ny, nz, M.MX, M.MY, M.MZ, P.MX, P.MY,
+                         P.MZ);
+      }
+      double df[5];
+      rusanov(M, P, nx, ny, nz, Fs, gamm, df);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) sDF[e][v][fp] = df[v];
+    }
+  }
+
+  // ---- volume phase ----
+  const double RDENS = 1.0 / (q[VD] + Dh);
+  const double GsqrtV = Gs / GsH;
+  const double RGsqrtV = 1.0 / GsqrtV;
+  const double RGsqrt = 1.0 / Gs;
+  double E11, E22, E33;
+  if constexpr (FLAT) {
+    E11 = m.fE[0];
+    E22 = m.fE[1];
+    E33 = m.fE[2];
+  } else {
+    E11 = valid ? m.E11[o] : 0.0;
+    E22 = valid ? m.E22[o] : 0.0;
+    E33 = valid ? m.E33[o] : 0.0;
+  }
+  double L[6];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) L[f] = m.Lift[p + (size_t)Np * f];
+  double cor = 0.0, dpx = 0.0, dpy = 0.0;
+  if (valid) {
+    cor = in.CORIOLIS[(i + N1 * j) + (size_t)Nfp * ke2d];
+    dpx = in.DPhydDx[o];
+    dpy = in.DPhydDy[o];
+  }
+  // VFilterPM1 of DDENS (ELM/..._tensorprod3D.F90:2828-2839); sRho was
+  // published by the barrier before the face phase
+  double drho = sRho[e][i + N1 * j] * sI[N1 * k];
+#pragma unroll
+  for (int l = 1; l < N1; ++l) drho = drho + sRho[e][i + N1 * (j + N1 * l)] * sI[l + N1 * k];
+
+  // One variable at a time: pointwise flux (FDS/..._rhot_heve.F90:376-418),
+  // Div_var5 (ELM/..._tensorprod3D.F90:2760-2783: Dx Fx, Dy Fy, Dz Fz and the
+  // lift of del_flux), tendency (:424-483) and its RK update.  Recomputing the
+  // mass flux per variable keeps only the state live between variables.
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const double m0 = Gs * q[VU];
+    const double m1 = Gs * q[VV];
+    const double m2 = Gs * (q[VW] * RGsqrtV + g13 * q[VU] + g23 * q[VV]);
+    double f0, f1, f2;
+    if (v == VD) {
+      f0 = m0;
+      f1 = m1;
+      f2 = m2;
+    } else if (v == VT) {
+      const double pt_ = (Th + q[VT]) * RDENS;
+      f0 = m0 * pt_;
+      f1 = m1 * pt_;
+      f2 = m2 * pt_;
+    } else if (v == VW) {
+      const double w_ = q[VW] * RDENS;
+      f0 = m0 * w_;
+      f1 = m1 * w_;
+      f2 = m2 * w_ + Gs * DP * RGsqrtV;
+    } else if (v == VU) {
+      const double GsqrtDPRES_ = Gs * DP;
+      const double u_ = q[VU] * RDENS;
+      f0 = m0 * u_ + GsqrtDPRES_;
+      f1 = m1 * u_;
+      f2 = m2 * u_ + GsqrtDPRES_ * g13;
+    } else {
+      const double GsqrtDPRES_ = Gs * DP;
+      const double v_ = q[VV] * RDENS;
+      f0 = m0 * v_;
+      f1 = m1 * v_ + GsqrtDPRES_;
+      f2 = m2 * v_ + GsqrtDPRES_ * g23;
+    }
+    const int b = v & 1;
+    sF[b][e][0][p] = f0;
+    sF[b][e][1][p] = f1;
+    sF[b][e][2][p] = f2;
+    __syncthreads();
+    const double* Fx = sF[b][e][0];
+    const double* Fy = sF[b][e][1];
+    const double* Fz = sF[b][e][2];
+    double dx = sD[i] * Fx[N1 * (j + N1 * k)];
+    double dy = Fy[i + N1 * (N1 * k)] * sD[j];
+    double dz = Fz[i + N1 * j] * sD[k];
+#pragma unroll
+    for (int l = 1; l < N1; ++l) {
+      dx = dx + sD[i + N1 * l] * Fx[l + N1 * (j + N1 * k)];
+      dy = dy + Fy[i + N1 * (l + N1 * k)] * sD[j + N1 * l];
+      dz = dz + Fz[i + N1 * (j + N1 * l)] * sD[k + N1 * l];
+    }
+    const double* dfv = sDF[e][v];
+    const double lift = L[0] * dfv[i + N1 * k] + L[1] * dfv[Nfp + j + N1 * k] + L[2] * dfv[2 * Nfp + i + N1 * k] +
+                        L[3] * dfv[3 * Nfp + j + N1 * k] + L[4] * dfv[4 * Nfp + i + N1 * j] +
+                        L[5] * dfv[5 * Nfp + i + N1 * j];
+    const double X = E11 * dx + E22 * dy + E33 * dz + lift;
+    double T;
+    if (v == VD || v == VT) {
+      T = -X * RGsqrt;
+    } else if (v == VW) {
+      T = -X * RGsqrt - c.GRAV * drho;
+    } else if (v == VU) {
+      T = -dpx + cor * q[VV];
+      T = T - X * RGsqrt;
+    } else {
+      T = -dpy - cor * q[VU];
+      T = T - X * RGsqrt;
+    }
+    if (valid) {
+      if constexpr (DROPIN) {
+        rk.out[v][o] = T;
+      } else {
+        // RK stage update (CMN/scale_timeint_rk.F90:2232-2350, tend_buf_size = 1)
+        switch (rk.mode) {
+          case 0:
+            rk.varTmp[v][o] = q[v] + rk.cb * T;
+            rk.out[v][o] = q[v] + rk.ca * T;
+            break;
+          case 1:
+            rk.varTmp[v][o] = rk.varTmp[v][o] + rk.cb * T;
+            rk.out[v][o] = rk.var0[v][o] + rk.ca * T;
+            break;
+          case 2:
+            rk.out[v][o] = rk.varTmp[v][o] + rk.cb * T;
+            break;
+          default:
+            rk.out[v][o] = q[v] + rk.cb * T;
+            break;
+        }
+      }
+    }
+    // finish this variable before the next one's LDS traffic; otherwise the
+    // scheduler defers the arithmetic and every variable's operands stay live
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+template <int N1>
+constexpr int epb_for() {
+  return (N1 * N1 * N1 >= 128) ? 1 : (256 / (N1 * N1 * N1));
+}
+
+template <int MODE>
+int launch_heve_mode(fedg_ctx* ctx, const TendIn& in, const RkOut& rk) {
+  const DevMesh& m = ctx->m;
+  auto go = [&](auto n1c) -> int {
+    constexpr int N1 = decltype(n1c)::value;
+    constexpr int EPB = epb_for<N1>();
+    const int nblk = (m.Ne + EPB - 1) / EPB;
+    hipLaunchKernelGGL((heve_kernel<N1, EPB, MODE>), dim3(nblk), dim3(EPB * N1 * N1 * N1), 0, ctx->stream, m,
+                       ctx->c, in, rk);
+    FEDG_HIP_CHECK(hipGetLastError());
+    return FEDG_OK;
+  };
+  switch (m.n1) {
+    case 2: return go(std::integral_constant<int, 2>{});
+    case 3: return go(std::integral_constant<int, 3>{});
+    case 4: return go(std::integral_constant<int, 4>{});
+    case 5: return go(std::integral_constant<int, 5>{});
+    case 6: return go(std::integral_constant<int, 6>{});
+    case 7: return go(std::integral_constant<int, 7>{});
+    case 8: return go(std::integral_constant<int, 8>{});
+    default:
+      set_error("polynomial order not built (supported: p = 1..7)");
+      return FEDG_ERR_UNSUPPORTED;
+  }
+}
+
+// ---------------------------------------------------------------------------
+__global__ void drhot2pres_kernel(double* PRES, double* DPRES, const double* DRHOT, const double* PRES_hyd,
+                                  const double* THERM_hyd, const double* Rtot, const double* CVtot,
+                                  const double* CPtot, long n, const Consts c) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double rP0 = 1.0 / c.PRES00;
+  const double RHOT = THERM_hyd[t] + DRHOT[t];
+  const double P = c.PRES00 * fedg_pow(Rtot[t] * rP0 * RHOT, CPtot[t] / CVtot[t]);
+  PRES[t] = P;
+  DPRES[t] = P - PRES_hyd[t];
+}
+
+__global__ void halo_kernel(double* f, const int32_t* src, long base, int nB) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nB) return;
+  f[base + b] = f[src[b]];
+}
+
+__global__ void apply_bc_kernel(const DevMesh m, double* MOMX, double* MOMY, double* MOMZ) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = (long)m.NfpTot * m.Ne;
+  if (t >= n) return;
+  const long base = (long)m.Np * m.Ne;
+  const long iP = m.vmapP[t];
+  if (iP < base) return;
+  const int bc = m.velbc[iP - base];
+  if (bc == FEDG_BND_NONE) return;
+  const long iM = m.vmapM[t];
+  const int ke = (int)(t / m.NfpTot);
+  const int fp = (int)(t - (long)ke * m.NfpTot);
+  const int pm = (int)(iM - (long)m.Np * ke);
+  const size_t h = (size_t)(pm % m.Nfp) + (size_t)m.Nfp * m.EMap3Dto2D[ke];
+  double mx, my, mz;
+  reflect_momentum(bc, m.Gsqrt[iM], m.GsqrtH[h], m.G11[h], m.G12[h], m.G22[h], m.G13[iM], m.G23[iM], m.nx[t],
+                   m.ny[t], m.nz[t], MOMX[iM], MOMY[iM], MOMZ[iM], mx, my, mz);
+  (void)fp;
+  MOMX[iP] = mx;
+  MOMY[iP] = my;
+  MOMZ[iP] = mz;
+}
+
+// Hydrostatic pressure gradient (FDS/scale_atm_dyn_dgm_nonhydro3d_common.F90:624-777).
+template <int N1, int EPB>
+__global__ void __launch_bounds__(EPB * N1 * N1 * N1)
+phyd_hgrad_kernel(const DevMesh m, double* DPhydDx, double* DPhydDy, const double* PRES_hyd,
+                  const double* PRES_hyd_ref) {
+  constexpr int Np = N1 * N1 * N1, Nfp = N1 * N1, NfpTot = 6 * Nfp;
+  __shared__ double sD[N1 * N1];
+  __shared__ double sF[EPB][4][Np];
+  __shared__ double sH[EPB][2][NfpTot];
+  const int tid = threadIdx.x;
+  const int e = tid / Np, p = tid - e * Np;
+  const int ke = blockIdx.x * EPB + e;
+  const bool valid = ke < m.Ne;
+  const int i = p % N1, j = (p / N1) % N1, k = p / Nfp;
+  for (int t = tid; t < N1 * N1; t += blockDim.x) sD[t] = m.D1D[t];
+  const size_t o = (size_t)Np * (valid ? ke : 0) + p;
+  double RG = 0.0;
+  if (valid) {
+    const int ke2d = m.EMap3Dto2D[ke];
+    for (int fp = p; fp < NfpTot; fp += Np) {
+      const size_t gf = fp + (size_t)NfpTot * ke;
+      const size_t iM = m.vmapM[gf], iP = m.vmapP[gf];
+      const int pm = (int)(iM - (size_t)Np * ke);
+      const double GH = m.GsqrtH[(size_t)(pm % Nfp) + (size_t)Nfp * ke2d];
+      const double GsqrtV_M = m.Gsqrt[iM] / 1.0 / GH;
+      const double GsqrtV_P = m.Gsqrt[iP] / 1.0 / GH;
+      const double DP_M = PRES_hyd[iM] - PRES_hyd_ref[iM];
+      const double DP_P = PRES_hyd[iP] - PRES_hyd_ref[iP];
+      const double tmp1 = m.Fscale[gf] * 0.5 * GsqrtV_P * DP_P;
+      const double tmp2 = m.Fscale[gf] * 0.5 * GsqrtV_M * DP_M;
+      sH[e][0][fp] = (m.nx[gf] + m.G13[iP] * m.nz[gf]) * tmp1 - (m.nx[gf] + m.G13[iM] * m.nz[gf]) * tmp2;
+      sH[e][1][fp] = (m.ny[gf] + m.G23[iP] * m.nz[gf]) * tmp1 - (m.ny[gf] + m.G23[iM] * m.nz[gf]) * tmp2;
+    }
+    const double GsqrtV = m.Gsqrt[o] / (1.0 * m.GsqrtH[(i + N1 * j) + (size_t)Nfp * ke2d]);
+    RG = 1.0 / GsqrtV;
+    const double F1 = GsqrtV * (PRES_hyd[o] - PRES_hyd_ref[o]);
+    sF[e][0][p] = F1;
+    sF[e][1][p] = F1;
+    sF[e][2][p] = m.G13[o] * F1;
+    sF[e][3][p] = m.G23[o] * F1;
+  }
+  __syncthreads();
+  if (!valid) return;
+  const double* F1 = sF[e][0];
+  const double* F2 = sF[e][1];
+  const double* F3 = sF[e][2];
+  const double* Fz = sF[e][3];
+  double dx = sD[i] * F1[N1 * (j + N1 * k)];
+  double dy = F2[i + N1 * (N1 * k)] * sD[j];
+  double dz = F3[i + N1 * j] * sD[k];
+  double dz2 = Fz[i + N1 * j] * sD[k];
+  for (int l = 1; l < N1; ++l) {
+    dx = dx + sD[i + N1 * l] * F1[l + N1 * (j + N1 * k)];
+    dy = dy + F2[i + N1 * (l + N1 * k)] * sD[j + N1 * l];
+    dz = dz + F3[i + N1 * (j + N1 * l)] * sD[k + N1 * l];
+    dz2 = dz2 + Fz[i + N1 * (j + N1 * l)] * sD[k + N1 * l];
+  }
+  double L[6];
+  for (int f = 0; f < 6; ++f) L[f] = m.Lift[p + (size_t)Np * f];
+  double lift[2];
+  for (int s = 0; s < 2; ++s) {
+    const double* h = sH[e][s];
+    lift[s] = L[0] * h[i + N1 * k] + L[1] * h[Nfp + j + N1 * k] + L[2] * h[2 * Nfp + i + N1 * k] +
+              L[3] * h[3 * Nfp + j + N1 * k] + L[4] * h[4 * Nfp + i + N1 * j] + L[5] * h[5 * Nfp + i + N1 * j];
+  }
+  const double gx = m.E11[o] * dx + m.E33[o] * dz + lift[0];
+  const double gy = m.E22[o] * dy + m.E33[o] * dz2 + lift[1];
+  DPhydDx[o] = gx * RG;
+  DPhydDy[o] = gy * RG;
+}
+
+__global__ void axpy_set_kernel(double* y, const double* a, double ca, const double* x, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) y[t] = a[t] + ca * x[t];
+}
+__global__ void axpy_acc_kernel(double* y, double ca, const double* x, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) y[t] = y[t] + ca * x[t];
+}
+__global__ void axpy2_acc_kernel(double* y, double ca, const double* x, double cb, const double* z, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) y[t] = y[t] + ca * x[t] + cb * z[t];
+}
+__global__ void axpy2_set_kernel(double* y, const double* a, double ca, const double* x, double cb, const double* z,
+                                 long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) y[t] = a[t] + ca * x[t] + cb * z[t];
+}
+__global__ void copy_kernel(double* y, const double* x, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) y[t] = x[t];
+}
+
+inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+int launch_heve_dropin(fedg_ctx* ctx, const TendIn& in, double* const out[5]) {
+  RkOut rk{};
+  for (int v = 0; v < 5; ++v) rk.out[v] = out[v];
+  return launch_heve_mode<MODE_DROPIN>(ctx, in, rk);
+}
+
+int launch_heve_fused(fedg_ctx* ctx, const TendIn& in, const RkOut& rk, int use_flat) {
+  if (use_flat) return launch_heve_mode<MODE_FUSED_FLAT>(ctx, in, rk);
+  return launch_heve_mode<MODE_FUSED_GENERAL>(ctx, in, rk);
+}
+
+int launch_drhot2pres(fedg_ctx* ctx, double* PRES, double* DPRES, const double* DRHOT, const double* PRES_hyd,
+                      const double* THERM_hyd, const double* Rtot, const double* CVtot, const double* CPtot,
+                      long n) {
+  if (n <= 0) return FEDG_OK;
+  hipLaunchKernelGGL(drhot2pres_kernel, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream, PRES, DPRES, DRHOT,
+                     PRES_hyd, THERM_hyd, Rtot, CVtot, CPtot, n, ctx->c);
+  FEDG_HIP_CHECK(hipGetLastError());
+  return FEDG_OK;
+}
+
+int launch_halo_exchange(fedg_ctx* ctx, double* field) {
+  const DevMesh& m = ctx->m;
+  if (m.nB <= 0) return FEDG_OK;
+  hipLaunchKernelGGL(halo_kernel, dim3(nblocks(m.nB, 256)), dim3(256), 0, ctx->stream, field, m.halo_src,
+                     (long)m.Np * m.Ne, m.nB);
+  FEDG_HIP_CHECK(hipGetLastError());
+  return FEDG_OK;
+}
+
+int launch_apply_bc(fedg_ctx* ctx, double* MOMX, double* MOMY, double* MOMZ) {
+  const DevMesh& m = ctx->m;
+  const long n = (long)m.NfpTot * m.Ne;
+  hipLaunchKernelGGL(apply_bc_kernel, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream, m, MOMX, MOMY, MOMZ);
+  FEDG_HIP_CHECK(hipGetLastError());
+  return FEDG_OK;
+}
+
+int launch_phyd_hgrad(fedg_ctx* ctx, double* DPhydDx, double* DPhydDy, const double* PRES_hyd,
+                      const double* PRES_hyd_ref) {
+  const DevMesh& m = ctx->m;
+  auto go = [&](auto n1c) -> int {
+    constexpr int N1 = decltype(n1c)::value;
+    constexpr int EPB = epb_for<N1>();
+    hipLaunchKernelGGL((phyd_hgrad_kernel<N1, EPB>), dim3((m.Ne + EPB - 1) / EPB), dim3(EPB * N1 * N1 * N1), 0,
+                       ctx->stream, m, DPhydDx, DPhydDy, PRES_hyd, PRES_hyd_ref);
+    FEDG_HIP_CHECK(hipGetLastError());
+    return FEDG_OK;
+  };
+  switch (m.n1) {
+    case 2: return go(std::integral_constant<int, 2>{});
+    case 3: return go(std::integral_constant<int, 3>{});
+    case 4: return go(std::integral_constant<int, 4>{});
+    case 5: return go(std::integral_constant<int, 5>{});
+    case 6: return go(std::integral_constant<int, 6>{});
+    case 7: return go(std::integral_constant<int, 7>{});
+    case 8: return go(std::integral_constant<int, 8>{});
+    default:
+      set_error("polynomial order not built (supported: p = 1..7)");
+      return FEDG_ERR_UNSUPPORTED;
+  }
+}
+
+#define FEDG_STREAM_LAUNCH(kern, n, ...)                                                                   \
+  do {                                                                                                     \
+    if ((n) <= 0) return FEDG_OK;                                                                          \
+    hipLaunchKernelGGL(kern, dim3(nblocks((n), 256)), dim3(256), 0, ctx->stream, __VA_ARGS__);             \
+    FEDG_HIP_CHECK(hipGetLastError());                                                                     \
+    return FEDG_OK;                                                                                        \
+  } while (0)
+
+int launch_axpy_set(fedg_ctx* ctx, double* y, const double* a, double ca, const double* x, long n) {
+  FEDG_STREAM_LAUNCH(axpy_set_kernel, n, y, a, ca, x, n);
+}
+int launch_axpy_acc(fedg_ctx* ctx, double* y, double ca, const double* x, long n) {
+  FEDG_STREAM_LAUNCH(axpy_acc_kernel, n, y, ca, x, n);
+}
+int launch_axpy2_acc(fedg_ctx* ctx, double* y, double ca, const double* x, double cb, const double* z, long n) {
+  FEDG_STREAM_LAUNCH(axpy2_acc_kernel, n, y, ca, x, cb, z, n);
+}
+int launch_axpy2_set(fedg_ctx* ctx, double* y, const double* a, double ca, const double* x, double cb,
+                     const double* z, long n) {
+  FEDG_STREAM_LAUNCH(axpy2_set_kernel, n, y, a, ca, x, cb, z, n);
+}
+int launch_copy(fedg_ctx* ctx, double* y, const double* x, long n) { FEDG_STREAM_LAUNCH(copy_kernel, n, y, x, n); }
+
+}  // namespace fedg

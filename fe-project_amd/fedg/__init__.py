@@ -1,0 +1,11 @@
+"""fedg: FElib regional nonhydrostatic DG dynamical-core step on AMD MI355X.
+
+Python host mirror of the reference's dynamical-core interfaces over the C ABI
+of libfedg.so (include/fedg.h).  Import fails when the native library is not
+built: there is no CPU fallback.
+"""
+from . import cases, constants  # noqa: F401
+from ._lib import EXPORTED, LIB, LIB_PATH, FedgError, check  # noqa: F401
+from .dyncore import (VARS, DevField, HeveStepper, TimeIntRK, apply_bc, calc_phyd_hgrad,  # noqa: F401
+                      calc_pressure, device_name, halo_exchange, heve_cal_tend)
+from .mesh import BND_NONE, BND_NOSLIP, BND_SLIP, BoxMesh, DeviceMesh, RefElem  # noqa: F401

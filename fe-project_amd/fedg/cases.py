@@ -1,0 +1,160 @@
+"""Base states and initial states of the benchmark cases (host, numpy).
+
+Base states restate FDS/scale_atm_dyn_dgm_hydrostatic.F90 (constT :86-115,
+constPT :129-170, constBVFreq :185-230) and the basic-state potential
+temperature field of FDS/scale_atm_dyn_dgm_nonhydro3d_common.F90:584-619.  The
+warm bubble follows the rising-thermal-bubble initial condition
+(model/atm_nonhydro3d/test/case/rising_thermal_bubble/dry/mod_user.F90:216-240)
+with the cosine bell of model/atm_nonhydro3d/src/preprocess/mod_mkinit_util.F90:54-165
+evaluated at the LGL nodes (its L2 projection onto the same order is the
+identity), and pressure left at its hydrostatic value.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import constants
+from .mesh import BoxMesh
+
+
+@dataclass
+class BaseState:
+    DENS_hyd: np.ndarray
+    PRES_hyd: np.ndarray
+    THERM_hyd: np.ndarray
+    PRES_hyd_ref: np.ndarray
+    Rtot: np.ndarray
+    CVtot: np.ndarray
+    CPtot: np.ndarray
+    CORIOLIS: np.ndarray
+    PT_hyd: np.ndarray  # potential temperature of the base state, interior nodes
+
+
+def calc_rhot_hyd(PRES_hyd, c=constants.SCALE):
+    """RHOT_hyd = P00/R (P_hyd/P00)^(cv/cp) (..._nonhydro3d_common.F90:584-619)."""
+    return c.PRES00 / c.Rdry * (PRES_hyd / c.PRES00) ** (c.CVdry / c.CPdry)
+
+
+def base_state(mesh: BoxMesh, kind="constT", T0=300.0, PT0=300.0, BVF=0.01, Psfc=1.0e5,
+               c=constants.SCALE) -> BaseState:
+    n = mesh.Np * mesh.Ne
+    z = mesh.z
+    D = mesh.field()
+    P = mesh.field()
+    if kind == "constT":
+        H0 = c.Rdry * T0 / c.GRAV
+        P[:n] = Psfc * np.exp(-z / H0)
+        D[:n] = P[:n] / (c.Rdry * T0)
+    elif kind == "constPT":
+        RovCP, CPovR = c.Rdry / c.CPdry, c.CPdry / c.Rdry
+        exner_sfc = (Psfc / c.PRES00) ** RovCP
+        exner = exner_sfc - c.GRAV / (c.CPdry * PT0) * z
+        P[:n] = c.PRES00 * exner ** CPovR
+        D[:n] = P[:n] / (c.Rdry * exner * PT0)
+    elif kind == "constBVFreq":
+        RovCP, CPovR = c.Rdry / c.CPdry, c.CPdry / c.Rdry
+        exner_sfc = (Psfc / c.PRES00) ** RovCP
+        PT = PT0 * np.exp(BVF ** 2 / c.GRAV * z)
+        exner = exner_sfc + c.GRAV ** 2 / (c.CPdry * BVF ** 2) * (1.0 / PT - 1.0 / PT0)
+        P[:n] = c.PRES00 * exner ** CPovR
+        D[:n] = P[:n] / (c.Rdry * exner * PT)
+    else:
+        raise ValueError(f"unknown base state {kind}")
+    mesh.exchange(D)
+    mesh.exchange(P)
+    T = mesh.field()
+    T[:n] = calc_rhot_hyd(P[:n], c)
+    mesh.exchange(T)
+    PT_hyd = T0 * (c.PRES00 / P[:n]) ** (c.Rdry / c.CPdry) if kind == "constT" else (
+        PT0 * np.ones(n) if kind == "constPT" else PT0 * np.exp(BVF ** 2 / c.GRAV * z))
+    return BaseState(DENS_hyd=D, PRES_hyd=P, THERM_hyd=T, PRES_hyd_ref=mesh.field(0.0),
+                     Rtot=mesh.field(c.Rdry), CVtot=mesh.field(c.CVdry), CPtot=mesh.field(c.CPdry),
+                     CORIOLIS=np.zeros(mesh.Nfp_v * mesh.Ne2DA), PT_hyd=PT_hyd)
+
+
+def cosine_bell(mesh: BoxMesh, amp, r, center):
+    """qmax * (0.5 (1 + cos(pi r)))  for r <= 1 (mkinitutil_calc_cosinebell, exponent 1)."""
+    rx = ry = rz = r
+    xc, yc, zc = center
+    rr = np.sqrt(((mesh.x - xc) / rx) ** 2 + ((mesh.y - yc) / ry) ** 2 + ((mesh.z - zc) / rz) ** 2)
+    return np.where(rr <= 1.0, amp * (0.5 * (1.0 + np.cos(np.pi * rr))), 0.0)
+
+
+def warm_bubble_state(mesh: BoxMesh, bs: BaseState, bubbles, c=constants.SCALE):
+    """Prognostic perturbations [DDENS, DRHOT, MOMZ, MOMX, MOMY], each (Np,NeA).
+
+    bubbles: list of (amplitude [K], radius [m], (xc, yc, zc) [m])."""
+    n = mesh.Np * mesh.Ne
+    thp = np.zeros(n)
+    for amp, r, ctr in bubbles:
+        thp += cosine_bell(mesh, amp, r, ctr)
+    RovCp = c.Rdry / c.CPdry
+    Ph = bs.PRES_hyd[:n]
+    PT = bs.PT_hyd + thp
+    DENS = Ph / (c.Rdry * PT * (Ph / c.PRES00) ** RovCp)
+    q = [mesh.field() for _ in range(5)]
+    q[0][:n] = DENS - bs.DENS_hyd[:n]
+    q[1][:n] = DENS * PT - bs.DENS_hyd[:n] * bs.PT_hyd
+    return q
+
+
+# ---------------------------------------------------------------------------
+# Benchmark configurations.
+
+@dataclass
+class Case:
+    name: str
+    porder: int
+    NeX: int
+    NeY: int
+    NeZ: int
+    Lx: float
+    Ly: float
+    Lz: float
+    dt: float
+    nsteps: int
+    bubbles: list
+    base: str = "constT"
+    scheme: str = "ERK_4s4o"
+
+
+def config1() -> Case:
+    """Plain FP64 HEVE warm bubble, 20x20x10 km, 8x8x4 elements, p=3, RK4 dt=0.2 s, 100 steps."""
+    return Case("warm bubble p3 8x8x4", 3, 8, 8, 4, 20.0e3, 20.0e3, 10.0e3, 0.2, 100,
+                [(2.0, 2.0e3, (10.0e3, 10.0e3, 2.0e3))])
+
+
+def config2_bubbles(seed=20261202):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(3):
+        xc, yc = rng.uniform(20.0e3, 140.0e3, size=2)
+        zc = rng.uniform(1.5e3, 3.0e3)
+        out.append((2.0, 2.0e3, (float(xc), float(yc), float(zc))))
+    return out
+
+
+def config2() -> Case:
+    """Plain FP64 HEVE at scale, 160x160x10 km, 64x64x16 elements, p=7, RK4 dt=0.05 s, 200 steps."""
+    return Case("warm bubbles p7 64x64x16", 7, 64, 64, 16, 160.0e3, 160.0e3, 10.0e3, 0.05, 200, config2_bubbles())
+
+
+def scaled_case(base: Case, NeX, NeY, NeZ, porder=None, nsteps=None) -> Case:
+    """Same physics on a smaller element count with the element size of `base`."""
+    p = base.porder if porder is None else porder
+    sx, sy, sz = NeX / base.NeX, NeY / base.NeY, NeZ / base.NeZ
+    Lx, Ly, Lz = base.Lx * sx, base.Ly * sy, base.Lz * sz
+    bubbles = [(a, r, (min(x, 0.5 * Lx), min(y, 0.5 * Ly), min(z, 0.4 * Lz))) for a, r, (x, y, z) in base.bubbles[:1]]
+    return Case(f"{base.name} scaled to {NeX}x{NeY}x{NeZ} p{p}", p, NeX, NeY, NeZ, Lx, Ly, Lz, base.dt,
+                base.nsteps if nsteps is None else nsteps, bubbles, base.base, base.scheme)
+
+
+def build(case: Case, c=constants.SCALE):
+    """Mesh, base state and initial prognostic state of a case (host)."""
+    mesh = BoxMesh(case.porder, case.NeX, case.NeY, case.NeZ, (0.0, case.Lx), (0.0, case.Ly), (0.0, case.Lz),
+                   periodic=(True, True, False))
+    bs = base_state(mesh, case.base, c=c)
+    q = warm_bubble_state(mesh, bs, case.bubbles, c=c)
+    return mesh, bs, q

@@ -1,0 +1,215 @@
+"""Host-side mirror of the reference dynamical-core interfaces on the device.
+
+* ``heve_cal_tend``      - atm_dyn_dgm_nonhydro3d_rhot_heve_cal_tend
+                           (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve.F90:292-489)
+* ``calc_pressure``      - atm_dyn_dgm_nonhydro3d_common_DRHOT2PRES (..._common.F90:428-479)
+* ``halo_exchange``      - MeshFieldComm_Exchange on one tile (DAT/scale_meshfieldcomm_*.F90)
+* ``apply_bc``           - ATMOS_dyn_bnd_applyBC_prgvars_lc (FDS/scale_atm_dyn_dgm_bnd.F90:270-367)
+* ``calc_phyd_hgrad``    - atm_dyn_dgm_nonhydro3d_common_calc_phyd_hgrad_lc (..._common.F90:624-777)
+* ``TimeIntRK``          - timeint_rk (CMN/scale_timeint_rk.F90) with device buffers
+* ``HeveStepper``        - AtmDynDGMDriver_nonhydro3d_update, HEVE branch
+                           (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:614-963), fused:
+                           one kernel per RK stage, state resident on the GPU.
+
+Fields are device buffers (``DevField``) of the reference's (Np,NeA) layout.
+Every call raises ``FedgError`` on a nonzero status, as the reference aborts.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import LIB, check
+from .mesh import DeviceMesh
+
+VARS = ("DDENS", "DRHOT", "MOMZ", "MOMX", "MOMY")  # PRGVAR ids 1..5
+
+
+class DevField:
+    """A device array of float64 (owned)."""
+
+    def __init__(self, n: int, host: np.ndarray | None = None):
+        self.n = int(n)
+        p = C.c_void_p()
+        check(LIB.fedg_dev_alloc(C.byref(p), max(1, self.n) * 8), "fedg_dev_alloc")
+        self.ptr = p
+        if host is not None:
+            self.upload(host)
+        else:
+            check(LIB.fedg_memset_dev(self.ptr, 0, self.n * 8), "fedg_memset_dev")
+
+    def upload(self, host: np.ndarray):
+        h = np.ascontiguousarray(host, dtype=np.float64)
+        if h.size != self.n:
+            raise ValueError(f"size mismatch {h.size} != {self.n}")
+        check(LIB.fedg_memcpy_h2d(self.ptr, h.ctypes.data_as(C.c_void_p), self.n * 8), "fedg_memcpy_h2d")
+
+    def download(self) -> np.ndarray:
+        h = np.empty(self.n, dtype=np.float64)
+        check(LIB.fedg_memcpy_d2h(h.ctypes.data_as(C.c_void_p), self.ptr, self.n * 8), "fedg_memcpy_d2h")
+        return h
+
+    def free(self):
+        if self.ptr:
+            LIB.fedg_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _p(f):
+    return f.ptr if isinstance(f, DevField) else f
+
+
+def heve_cal_tend(dm: DeviceMesh, out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DPRES, DENS_hyd, PRES_hyd, PRES_hyd_ref,
+                  THERM_hyd, CORIOLIS, Rtot, CVtot, CPtot, DPhydDx, DPhydDy):
+    """out = (DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt): device fields."""
+    args = [_p(a) for a in (*out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DPRES, DENS_hyd, PRES_hyd, PRES_hyd_ref, THERM_hyd,
+                            CORIOLIS, Rtot, CVtot, CPtot, DPhydDx, DPhydDy)]
+    check(LIB.fedg_heve_cal_tend(dm.handle, *args), "fedg_heve_cal_tend")
+
+
+def calc_pressure(dm: DeviceMesh, PRES, DPRES, DRHOT, PRES_hyd, THERM_hyd, Rtot, CVtot, CPtot):
+    check(LIB.fedg_drhot2pres(dm.handle, *[_p(a) for a in (PRES, DPRES, DRHOT, PRES_hyd, THERM_hyd, Rtot, CVtot,
+                                                           CPtot)]), "fedg_drhot2pres")
+
+
+def halo_exchange(dm: DeviceMesh, fields):
+    arr = (C.c_void_p * len(fields))(*[_p(f) for f in fields])
+    check(LIB.fedg_halo_exchange(dm.handle, len(fields), arr), "fedg_halo_exchange")
+
+
+def apply_bc(dm: DeviceMesh, DDENS, MOMX, MOMY, MOMZ, THERM, DENS_hyd, PRES_hyd):
+    check(LIB.fedg_apply_bc_prgvars(dm.handle, *[_p(a) for a in (DDENS, MOMX, MOMY, MOMZ, THERM, DENS_hyd,
+                                                                 PRES_hyd)]), "fedg_apply_bc_prgvars")
+
+
+def calc_phyd_hgrad(dm: DeviceMesh, PRES_hyd: np.ndarray, PRES_hyd_ref: np.ndarray):
+    """Hydrostatic pressure gradient (host arrays in/out, halos of the inputs filled)."""
+    n = dm.mesh.Np * dm.mesh.NeA
+    P, R = DevField(n, PRES_hyd), DevField(n, PRES_hyd_ref)
+    Dx, Dy = DevField(n), DevField(n)
+    check(LIB.fedg_calc_phyd_hgrad(dm.handle, Dx.ptr, Dy.ptr, P.ptr, R.ptr), "fedg_calc_phyd_hgrad")
+    out = Dx.download(), Dy.download()
+    for f in (P, R, Dx, Dy):
+        f.free()
+    return out
+
+
+class TimeIntRK:
+    """timeint_rk with device buffers (var0, varTmp, tend_buf2D_ex/_im)."""
+
+    def __init__(self, dm: DeviceMesh, scheme: str, dt: float):
+        self.dm = dm
+        h = C.c_void_p()
+        check(LIB.fedg_tint_create(dm.handle, scheme.encode(), float(dt), C.byref(h)), "fedg_tint_create")
+        self.handle = h
+        ns, nb, im = C.c_int(), C.c_int(), C.c_int()
+        check(LIB.fedg_tint_info(h, C.byref(ns), C.byref(nb), C.byref(im)), "fedg_tint_info")
+        self.nstage, self.tend_buf_size, self.imex = ns.value, nb.value, bool(im.value)
+
+    def tend_ex(self, stage: int, varID: int):
+        return LIB.fedg_tint_tend_ex(self.handle, stage, varID)
+
+    def tend_im(self, stage: int, varID: int):
+        return LIB.fedg_tint_tend_im(self.handle, stage, varID)
+
+    def store_var0(self, q, varID):
+        check(LIB.fedg_tint_store_var0(self.handle, _p(q), varID), "fedg_tint_store_var0")
+
+    def advance(self, stage, qs, varIDs):
+        arr = (C.c_void_p * len(qs))(*[_p(q) for q in qs])
+        ids = (C.c_int * len(varIDs))(*varIDs)
+        check(LIB.fedg_tint_advance(self.handle, stage, len(qs), arr, ids), "fedg_tint_advance")
+
+    def store_implicit(self, stage, q, varID):
+        check(LIB.fedg_tint_store_implicit(self.handle, stage, _p(q), varID), "fedg_tint_store_implicit")
+
+    def close(self):
+        if self.handle:
+            LIB.fedg_tint_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HeveStepper:
+    """Fused HEVE RK stepper with the state resident on the GPU."""
+
+    def __init__(self, dm: DeviceMesh, bs, DPhydDx: np.ndarray, DPhydDy: np.ndarray, dt: float,
+                 scheme: str = "ERK_4s4o", dry: bool = True):
+        self.dm = dm
+        m = dm.mesh
+        n = m.Np * m.Ne
+        self._keep = [np.ascontiguousarray(a[:n]) for a in (bs.DENS_hyd, bs.PRES_hyd, bs.THERM_hyd, DPhydDx, DPhydDy)]
+        cor = np.ascontiguousarray(bs.CORIOLIS)
+        self._keep.append(cor)
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+        if dry:
+            rt = cv = cp = None
+        else:
+            rt, cv, cp = (np.ascontiguousarray(a[:n]) for a in (bs.Rtot, bs.CVtot, bs.CPtot))
+            self._keep += [rt, cv, cp]
+        h = C.c_void_p()
+        check(LIB.fedg_heve_create(dm.handle, scheme.encode(), float(dt), *[ptr(a) for a in self._keep[:3]],
+                                   ptr(cor), ptr(self._keep[3]), ptr(self._keep[4]),
+                                   None if rt is None else ptr(rt), None if cv is None else ptr(cv),
+                                   None if cp is None else ptr(cp), C.byref(h)), "fedg_heve_create")
+        self.handle = h
+        self.n = n
+
+    @property
+    def flat(self) -> bool:
+        return bool(LIB.fedg_heve_is_flat(self.handle))
+
+    def set_state(self, q):
+        hs = [np.ascontiguousarray(a[: self.n], dtype=np.float64) for a in q]
+        arr = (C.c_void_p * 5)(*[h.ctypes.data_as(C.c_void_p) for h in hs])
+        check(LIB.fedg_heve_set_state(self.handle, arr), "fedg_heve_set_state")
+
+    def get_state(self):
+        hs = [np.empty(self.n) for _ in range(5)]
+        arr = (C.c_void_p * 5)(*[h.ctypes.data_as(C.c_void_p) for h in hs])
+        check(LIB.fedg_heve_get_state(self.handle, arr), "fedg_heve_get_state")
+        return hs
+
+    def get_pressure(self):
+        P, D = np.empty(self.n), np.empty(self.n)
+        check(LIB.fedg_heve_get_pressure(self.handle, P.ctypes.data_as(C.c_void_p), D.ctypes.data_as(C.c_void_p)),
+              "fedg_heve_get_pressure")
+        return P, D
+
+    def run(self, nsteps: int):
+        check(LIB.fedg_heve_run(self.handle, int(nsteps)), "fedg_heve_run")
+
+    def run_timed(self, nsteps: int):
+        """(total ms, mean ms per stage-kernel launch) from HIP events on the stepper's stream."""
+        tot, per = C.c_double(), C.c_double()
+        check(LIB.fedg_heve_run_timed(self.handle, int(nsteps), C.byref(tot), C.byref(per)), "fedg_heve_run_timed")
+        return tot.value, per.value
+
+    def close(self):
+        if self.handle:
+            LIB.fedg_heve_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_name() -> str:
+    buf = C.create_string_buffer(256)
+    check(LIB.fedg_device_name(buf, 256), "fedg_device_name")
+    return buf.value.decode()

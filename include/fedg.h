@@ -1,0 +1,249 @@
+/*
+ * fedg: MI355X (gfx950) implementation of FElib's regional nonhydrostatic 3D
+ * DG dynamical-core step (ywkawai/FE-Project, FElib/src/fluid_dyn_solver).
+ *
+ * C ABI only: plain pointers, sizes and status codes; no C++ or torch types.
+ * Arrays keep the reference's Fortran column-major layout (X(Np,NeA) is
+ * X[p + Np*ke]) and its 1-based integer maps, so a Fortran host passes
+ * lcmesh%... arrays through ISO_C_BINDING unchanged.
+ *
+ * Pointer kinds:
+ *   host   - ordinary CPU memory
+ *   device - GPU memory (fedg_dev_alloc, hipMalloc, or an OpenMP/OpenACC
+ *            device copy); the entry points never copy it to the host.
+ *
+ * Every entry point returns FEDG_OK (0) or an error code; fedg_last_error()
+ * gives the message.  The reference aborts (PRC_abort) on such errors; the
+ * Fortran binding (fe-project_amd/fortran/scale_atm_dyn_dgm_hip.F90) does the
+ * same with a nonzero status.  All device work is stream-ordered on the
+ * context's stream and complete when the function returns, as the reference's
+ * OpenACC path waits on its async queue before returning
+ * (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve_gpu.F90:471,484).
+ */
+#ifndef FEDG_H
+#define FEDG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FEDG_OK = 0,
+  FEDG_ERR_ARG = 1,         /* invalid argument or unsupported configuration */
+  FEDG_ERR_HIP = 2,         /* a HIP runtime call failed */
+  FEDG_ERR_NOMEM = 3,       /* device allocation failed */
+  FEDG_ERR_UNSUPPORTED = 4  /* feature outside the built scope */
+};
+
+/* Prognostic variable ids (FDS/scale_atm_dyn_dgm_nonhydro3d_common.F90:68-77), 1-based. */
+enum { FEDG_DDENS_ID = 1, FEDG_DRHOT_ID = 2, FEDG_MOMZ_ID = 3, FEDG_MOMX_ID = 4, FEDG_MOMY_ID = 5 };
+
+/* Velocity boundary kinds per halo slot (FDS/scale_atm_dyn_dgm_bnd.F90:323-360). */
+enum { FEDG_BND_NONE = 0, FEDG_BND_SLIP = 1, FEDG_BND_NOSLIP = 2 };
+
+const char *fedg_last_error(void);
+const char *fedg_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Device memory helpers (for hosts without their own device allocator) */
+int fedg_dev_alloc(void **ptr, size_t bytes);
+int fedg_dev_free(void *ptr);
+int fedg_memcpy_h2d(void *dst_dev, const void *src_host, size_t bytes);
+int fedg_memcpy_d2h(void *dst_host, const void *src_dev, size_t bytes);
+int fedg_memset_dev(void *dst_dev, int value, size_t bytes);
+int fedg_device_sync(void);
+int fedg_device_name(char *buf, int buflen);
+
+/* ------------------------------------------------------------------ */
+/* Reference element operators (host).  Replaces the setup of
+ * ElementOperationTensorProd3D_P<p> (ELM/scale_element_operation_tensorprod3D.F90:5512-5569)
+ * and HexahedralElement (ELM/scale_element_hexahedral.F90:102-404).
+ * n1 = porder+1.  Any output pointer may be NULL.
+ *   x1d, w1d (n1): LGL nodes/weights; D1D (n1,n1); V1D, invV1D (n1,n1);
+ *   invM1D (n1,n1) = V1D V1D^T; Lift_mat (n1,n1,n1,6); IntrpMat_VPOrdM1_tr (n1,n1);
+ *   IndexH2Dto3D (n1^3), IndexH2Dto3D_bnd (6 n1^2): 1-based. */
+int fedg_refelem_tables(int porder, double *x1d, double *w1d, double *D1D, double *V1D,
+                        double *invV1D, double *invM1D, double *Lift_mat, double *IntrpMat_VPOrdM1_tr,
+                        int32_t *IndexH2Dto3D, int32_t *IndexH2Dto3D_bnd);
+
+/* ------------------------------------------------------------------ */
+/* Structured box mesh (host).  Replaces MeshCubeDom3D_setupLocalDom for one
+ * tile (MSH/scale_mesh_cubedom3d.F90:339-469) with the metrics of
+ * MSH/scale_mesh_base3d.F90:200-318, the face maps of
+ * MSH/scale_meshutil_3d.F90:264-641 and the single-tile halo exchange of
+ * DAT/scale_meshfieldcomm_cubedom3d.F90:245-344. */
+typedef struct {
+  int32_t porder;
+  int32_t NeX, NeY, NeZ;
+  double xmin, xmax, ymin, ymax, zmin, zmax;
+  int32_t periodic_x, periodic_y, periodic_z;
+} fedg_box_desc;
+
+typedef struct {
+  int32_t n1, Np, Nfp_h, Nfp_v, NfpTot;
+  int32_t Ne, NeA, Ne2D, Ne2DA, nB;
+} fedg_box_sizes;
+
+/* Host output arrays (caller-allocated, any may be NULL). */
+typedef struct {
+  double *pos_en;       /* (Np,Ne,3)     */
+  double *J;            /* (Np,Ne)       */
+  double *Escale;       /* (Np,Ne,3,3)   */
+  double *sJ;           /* (NfpTot,Ne)   */
+  double *Fscale;       /* (NfpTot,Ne)   */
+  double *normal_fn;    /* (NfpTot,Ne,3) */
+  int32_t *vmapM;       /* (NfpTot,Ne) 1-based into Np*NeA */
+  int32_t *vmapP;       /* (NfpTot,Ne) 1-based into Np*NeA */
+  int32_t *vmapB;       /* (nB) 1-based into Np*Ne */
+  int32_t *halo_src;    /* (nB) 1-based interior node copied into halo slot */
+  int32_t *halo_face;   /* (nB) domain boundary 1..6 (y-,x+,y+,x-,z-,z+) of halo slot */
+  int32_t *EMap3Dto2D;  /* (Ne) 1-based */
+  double *Gsqrt;        /* (Np,NeA)      */
+  double *GI3;          /* (Np,NeA,2)    */
+  double *GsqrtH;       /* (Nfp_v,Ne2D)  */
+} fedg_box_arrays;
+
+int fedg_box_get_sizes(const fedg_box_desc *desc, fedg_box_sizes *sizes);
+int fedg_box_build(const fedg_box_desc *desc, fedg_box_arrays *out);
+
+/* ------------------------------------------------------------------ */
+/* Context: flattened LocalMesh3D + ElementBase3D + ElementOperationTensorProd3D,
+ * uploaded once (the reference keeps them on the device with !$acc enter data,
+ * DAT/scale_localmeshfield_base.F90:119-128).  All pointers are host pointers
+ * to the reference's arrays; the context keeps device copies. */
+typedef struct {
+  int32_t porder;
+  int32_t Ne, NeA, Ne2D, Ne2DA, nB;
+  const double *Escale;      /* (Np,Ne,3,3)   lcmesh%Escale   */
+  const double *Gsqrt;       /* (Np,NeA)      lcmesh%Gsqrt    */
+  const double *GI3;         /* (Np,NeA,2)    lcmesh%GI3      */
+  const double *GsqrtH;      /* (Nfp_v,Ne2D)  lcmesh%GsqrtH   */
+  const double *G11, *G12, *G22; /* (Nfp_v,Ne2D) lcmesh%GIJ(:,:,1,1),(1,2),(2,2); NULL = 1,0,1 */
+  const double *normal_fn;   /* (NfpTot,Ne,3) lcmesh%normal_fn */
+  const double *Fscale;      /* (NfpTot,Ne)   lcmesh%Fscale   */
+  const int32_t *vmapM;      /* (NfpTot,Ne)   lcmesh%vmapM (1-based) */
+  const int32_t *vmapP;      /* (NfpTot,Ne)   lcmesh%vmapP (1-based) */
+  const int32_t *EMap3Dto2D; /* (Ne)          lcmesh%EMap3Dto2D      */
+  const int32_t *halo_src;   /* (nB) interior node that fills each halo slot (1-based) */
+  const int32_t *velbc;      /* (nB) FEDG_BND_* of each halo slot (AtmDynBnd VelBC_list) */
+  const double *D1D;         /* (n1,n1)       element3D_operation%D1D */
+  const double *Lift_mat;    /* (n1,n1,n1,6)  element3D_operation%Lift_mat */
+  const double *IntrpMat_VPOrdM1_tr; /* (n1,n1) element3D_operation%IntrpMat_VPOrdM1_tr */
+  /* box shape when the mesh came from fedg_box_build (0 otherwise); lets the
+   * fused stepper resolve neighbours arithmetically */
+  int32_t NeX, NeY, NeZ;
+  /* physical constants (SCALE scale_const: CONST_GRAV, CONST_Rdry, CONST_CPdry,
+   * CONST_CVdry, CONST_PRE00); zero selects the SCALE defaults below */
+  double GRAV, Rdry, CPdry, CVdry, PRES00;
+} fedg_mesh_desc;
+
+#define FEDG_DEFAULT_GRAV 9.80665
+#define FEDG_DEFAULT_RDRY 287.04
+#define FEDG_DEFAULT_CPDRY 1004.64
+#define FEDG_DEFAULT_CVDRY (1004.64 - 287.04)
+#define FEDG_DEFAULT_PRES00 100000.0
+
+typedef struct fedg_ctx fedg_ctx;
+
+int fedg_ctx_create(const fedg_mesh_desc *desc, fedg_ctx **ctx);
+int fedg_ctx_destroy(fedg_ctx *ctx);
+/* sizes: Np, NfpTot, Ne, NeA, nB ... of the context */
+int fedg_ctx_sizes(const fedg_ctx *ctx, fedg_box_sizes *sizes);
+
+/* ------------------------------------------------------------------ */
+/* Drop-in entry points.  Device pointers; fields are (Np,NeA) unless noted.
+ * Each computes what the cited reference routine computes, for elements
+ * NeS..NeE = 1..Ne, reading halo slots through vmapP as the reference does. */
+
+/* atm_dyn_dgm_nonhydro3d_common_DRHOT2PRES
+ * (FDS/scale_atm_dyn_dgm_nonhydro3d_common.F90:428-479): PRES, DPRES from DRHOT. */
+int fedg_drhot2pres(fedg_ctx *ctx, double *PRES, double *DPRES, const double *DRHOT,
+                    const double *PRES_hyd, const double *THERM_hyd, const double *Rtot,
+                    const double *CVtot, const double *CPtot);
+
+/* MeshFieldComm_Exchange on one tile (DAT/scale_meshfieldcomm_base.F90:571-805,
+ * 846-912): fills the halo tail of each of nfields device fields. */
+int fedg_halo_exchange(fedg_ctx *ctx, int nfields, double *const *fields);
+
+/* ATMOS_dyn_bnd_applyBC_prgvars_lc (FDS/scale_atm_dyn_dgm_bnd.F90:270-367). */
+int fedg_apply_bc_prgvars(fedg_ctx *ctx, double *DDENS, double *MOMX, double *MOMY, double *MOMZ,
+                          double *THERM, const double *DENS_hyd, const double *PRES_hyd);
+
+/* atm_dyn_dgm_nonhydro3d_rhot_heve_cal_tend
+ * (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve.F90:292-489 with its numerical
+ * flux FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve_numflux.F90:946-1138); the
+ * argument list is that of the abstract interface atm_dyn_nonhydro3d_cal_tend_ex
+ * (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:152-199) minus the derived-type
+ * arguments, which the context carries.  CORIOLIS is (Nfp_v,Ne2DA). */
+int fedg_heve_cal_tend(fedg_ctx *ctx, double *DENS_dt, double *MOMX_dt, double *MOMY_dt,
+                       double *MOMZ_dt, double *RHOT_dt, const double *DDENS_, const double *MOMX_,
+                       const double *MOMY_, const double *MOMZ_, const double *DRHOT_,
+                       const double *DPRES_, const double *DENS_hyd, const double *PRES_hyd,
+                       const double *PRES_hyd_ref, const double *THERM_hyd, const double *CORIOLIS,
+                       const double *Rtot, const double *CVtot, const double *CPtot,
+                       const double *DPhydDx, const double *DPhydDy);
+
+/* atm_dyn_dgm_nonhydro3d_common_calc_phyd_hgrad_lc
+ * (FDS/scale_atm_dyn_dgm_nonhydro3d_common.F90:624-777), halos of PRES_hyd and
+ * PRES_hyd_ref filled. */
+int fedg_calc_phyd_hgrad(fedg_ctx *ctx, double *DPhydDx, double *DPhydDy, const double *PRES_hyd,
+                         const double *PRES_hyd_ref);
+
+/* ------------------------------------------------------------------ */
+/* Time integration (timeint_rk, CMN/scale_timeint_rk.F90:35-104, 135-229).
+ * The integrator owns var0/varTmp/tend_buf2D_ex(/_im) on the device, sized
+ * (Np,NeA,5[,nbuf]) like the reference. */
+typedef struct fedg_tint fedg_tint;
+/* scheme: "ERK_4s4o"/"ERK_RK4", "ERK_1s1o"/"ERK_Euler", "IMEX_ARK232", "IMEX_ARK324"
+ * (CMN/scale_timeint_rk_butcher_tab.F90:60-324) */
+int fedg_tint_create(fedg_ctx *ctx, const char *scheme, double dt, fedg_tint **tint);
+int fedg_tint_destroy(fedg_tint *tint);
+int fedg_tint_info(const fedg_tint *tint, int *nstage, int *tend_buf_size, int *imex);
+/* device pointer to tend_buf2D_ex(:,:,varID,tend_buf_indmap(stage)) (1-based ids) */
+double *fedg_tint_tend_ex(fedg_tint *tint, int stage, int varID);
+double *fedg_tint_tend_im(fedg_tint *tint, int stage, int varID);
+/* timeint_rk_store_var0_2D (CMN/scale_timeint_rk.F90:624-647) */
+int fedg_tint_store_var0(fedg_tint *tint, const double *q, int varID);
+/* Advance_varlist: rk_advance_general2D (CMN/scale_timeint_rk.F90:536-573, 2201-2355)
+ * for nvar device fields q[i] with ids varIDs[i] at 1-based stage */
+int fedg_tint_advance(fedg_tint *tint, int stage, int nvar, double *const *q, const int *varIDs);
+/* StoreImplicit: rk_storeimpl_general2D (CMN/scale_timeint_rk.F90:2510-2560) */
+int fedg_tint_store_implicit(fedg_tint *tint, int stage, double *q, int varID);
+
+/* ------------------------------------------------------------------ */
+/* Fused HEVE stepper: the whole AtmDynDGMDriver_nonhydro3d_update HEVE step
+ * (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:614-963) with device-resident
+ * state, one kernel per RK stage (halo, pressure, boundary condition,
+ * tendency and RK update fused; each stage reads the state once).
+ * Explicit schemes with one tendency buffer (ERK_4s4o, ERK_1s1o).
+ * Base-state fields are host pointers (Np,NeA) with filled halos; CORIOLIS is
+ * (Nfp_v,Ne2DA); Rtot/CVtot/CPtot (Np,NeA) or NULL for the dry-air constants. */
+typedef struct fedg_heve fedg_heve;
+int fedg_heve_create(fedg_ctx *ctx, const char *scheme, double dt, const double *DENS_hyd,
+                     const double *PRES_hyd, const double *THERM_hyd, const double *CORIOLIS,
+                     const double *DPhydDx, const double *DPhydDy, const double *Rtot,
+                     const double *CVtot, const double *CPtot, fedg_heve **st);
+int fedg_heve_destroy(fedg_heve *st);
+/* 1 if the flat-terrain uniform-box kernel is used, 0 for the general-metric kernel */
+int fedg_heve_is_flat(const fedg_heve *st);
+/* host state in/out: q[v] (Np,Ne or more) for v = DDENS, DRHOT, MOMZ, MOMX, MOMY */
+int fedg_heve_set_state(fedg_heve *st, const double *const *q_host);
+int fedg_heve_get_state(fedg_heve *st, double *const *q_host);
+/* device pointers of the current state (valid until the next step) */
+int fedg_heve_state_dev(fedg_heve *st, double **q_dev);
+/* nsteps full RK steps on the device; synchronous */
+int fedg_heve_run(fedg_heve *st, int nsteps);
+/* nsteps steps bracketed by HIP events on the stepper's stream: total
+ * milliseconds and the mean duration of each stage kernel launch */
+int fedg_heve_run_timed(fedg_heve *st, int nsteps, double *ms_total, double *ms_per_stage_kernel);
+/* PRES and DPRES of the current state (host (Np,Ne)), the reference's
+ * post-step calc_pressure (:954-959) */
+int fedg_heve_get_pressure(fedg_heve *st, double *PRES_host, double *DPRES_host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FEDG_H */

@@ -1,0 +1,662 @@
+/*
+ * CPU oracle: literal C restatement of the FElib HEVE step.  TEST
+ * INFRASTRUCTURE ONLY (see fedg_oracle.h): the product never calls it.
+ *
+ * Build: gcc -O3 -fopenmp -ffp-contract=off (oracle/Makefile).  Sums are
+ * accumulated left to right in the order the reference writes them, each
+ * product rounded before it is added, as gfortran evaluates the Fortran source
+ * without -ffp-contract=fast.
+ *
+ * How each function is pinned (tests/test_oracle_*.py):
+ *  - tensor-product operators (dirX/dirY/dirZ, Lift, Div): the reference's own
+ *    unit test semantics (FElib/test/FE/element_operation_hexahedral:
+ *    derivative of 4x^p+3y^p+2z^p, Div with Escale/Gsqrt closed form, Lift vs
+ *    a dense Lift matrix), plus a numpy evaluation of the same contractions;
+ *  - VFilterPM1: identity on polynomials of degree <= p-1 in z, removes the
+ *    degree-p Legendre mode (closed form);
+ *  - numflux: consistency (equal states give only the pressure and advective
+ *    jump terms, which vanish), per-face antisymmetry of the jump between the
+ *    two elements sharing a face, and a brute-force numpy evaluation of the
+ *    Rusanov formula at single face nodes;
+ *  - cal_tend: hydrostatic rest state gives tendencies of exactly zero except
+ *    the roundoff of the discrete hydrostatic balance, which the test measures
+ *    against the -DPhydDx/-g*drho terms it cancels; free-stream preservation;
+ *    mass conservation (sum of w*J*DENS_dt = 0 in a periodic box);
+ *  - drhot2pres: closed form P = P00 (R rhotheta / P00)^(cp/cv);
+ *  - RK tables / advance: the reference's test FElib/test/common/timeint_rk
+ *    (order of convergence on the harmonic oscillator) and the Butcher
+ *    coefficients in closed form;
+ *  - halo exchange / BC: the reference's mesh/field test semantics
+ *    (FElib/test/FE/field_cubedom3d_hexahedral) on integer-valued fields, and
+ *    the SLIP condition's closed form (exterior normal momentum = -interior).
+ * The physical constants are SCALE's CONST_* values (external, not vendored
+ * in the reference): parity unpinned for them; both the oracle and the
+ * product take them as inputs.
+ */
+#include "fedg_oracle.h"
+
+#include <math.h>
+#include <omp.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* The portable double-double pow that the GPU kernels use (host build of the
+ * same source).  It is selected for the pressure's pow in parity mode; it is
+ * pinned against 60-digit references in tests/test_pow.py. */
+#include "../fe-project_amd/csrc/fedg_pow.h"
+
+#define DENS_VID 0
+#define RHOT_VID 1
+#define MOMZ_VID 2
+#define MOMX_VID 3
+#define MOMY_VID 4
+
+int oracle_num_threads(void) { return omp_get_max_threads(); }
+
+/* 0: the C library's pow, as the reference Fortran's ** compiles with
+ * gfortran/glibc; 1: the portable correctly rounded pow (parity mode). */
+static int g_pow_mode = 0;
+void oracle_set_pow_mode(int mode) { g_pow_mode = mode; }
+int oracle_get_pow_mode(void) { return g_pow_mode; }
+static inline double opow(double x, double y) { return g_pow_mode ? fedg_pow(x, y) : pow(x, y); }
+void oracle_pow_array(const double *x, const double *y, double *out, long n, int mode) {
+  for (long i = 0; i < n; ++i) out[i] = mode ? fedg_pow(x[i], y[i]) : pow(x[i], y[i]);
+}
+
+/* ---------------------------------------------------------------------- */
+/* FDS/scale_atm_dyn_dgm_nonhydro3d_common.F90:428-479 (non-OpenACC branch) */
+void oracle_drhot2pres(const oracle_mesh *m, double *PRES, double *DPRES,
+                       const double *DRHOT, const double *PRES_hyd, const double *THERM_hyd,
+                       const double *Rtot, const double *CVtot, const double *CPtot) {
+  const double rP0 = 1.0 / m->PRES00;
+  const long n = (long)m->Np * m->Ne;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) {
+    double RHOT = THERM_hyd[i] + DRHOT[i];
+    PRES[i] = m->PRES00 * opow(Rtot[i] * rP0 * RHOT, CPtot[i] / CVtot[i]);
+    DPRES[i] = PRES[i] - PRES_hyd[i];
+  }
+}
+
+/* DAT/scale_meshfieldcomm_base.F90:571-609 (pack by VMapB), :886-894 (copy
+ * on the same rank), :768-789 (unpack into the halo tail), resolved through
+ * the single-tile source map. */
+void oracle_halo_exchange(const oracle_mesh *m, double *val) {
+  const long base = (long)m->Np * m->Ne;
+  double *buf = (double *)malloc(sizeof(double) * (size_t)m->nB);
+  for (int b = 0; b < m->nB; ++b) buf[b] = val[m->halo_src[b] - 1];
+  for (int b = 0; b < m->nB; ++b) val[base + b] = buf[b];
+  free(buf);
+}
+
+/* FDS/scale_atm_dyn_dgm_bnd.F90:270-367 */
+void oracle_apply_bc(const oracle_mesh *m, double *DDENS, double *MOMX, double *MOMY,
+                     double *MOMZ, double *THERM) {
+  (void)DDENS;
+  (void)THERM;
+  const int Np = m->Np, NfpTot = m->NfpTot, Ne = m->Ne, Nfp_v = m->Nfp_v;
+  const long NpNeA = (long)Np * m->NeA;
+  const long nfall = (long)NfpTot * Ne;
+  const double *G13 = m->GI3, *G23 = m->GI3 + NpNeA;
+  const double *nx = m->normal_fn, *ny = m->normal_fn + nfall, *nz = m->normal_fn + 2 * nfall;
+#pragma omp parallel for schedule(static)
+  for (int ke = 0; ke < Ne; ++ke) {
+    for (int p = 0; p < NfpTot; ++p) {
+      long i = p + (long)ke * NfpTot;
+      long iP = m->vmapP[i] - 1;
+      long i_ = (long)m->vmapP[i] - (long)Np * Ne; /* 1-based halo index */
+      if (i_ > 0) {
+        long iM = m->vmapM[i] - 1;
+        int bc = m->velbc[i_ - 1];
+        if (bc == 1) { /* SLIP */
+          int ke2D = m->EMap3Dto2D[ke] - 1;
+          long h = (m->IndexH2Dto3D_bnd[p] - 1) + (long)Nfp_v * ke2D;
+          double GsqrtV = m->Gsqrt[iM] / m->GsqrtH[h];
+          double G11_ = m->G11[h], G12_ = m->G12[h], G22_ = m->G22[h];
+          double MOMW = MOMZ[iM] / GsqrtV + G13[iM] * MOMX[iM] + G23[iM] * MOMY[iM];
+          double t1 = GsqrtV * G13[iM];
+          double t2 = GsqrtV * G23[iM];
+          double fac = nz[i] * (GsqrtV * GsqrtV) /
+                       (1.0 + G11_ * (t1 * t1) + 2.0 * G12_ * ((GsqrtV * GsqrtV) * G13[iM] * G23[iM]) +
+                        G22_ * (t2 * t2));
+          double mom_normal = MOMX[iM] * nx[i] + MOMY[iM] * ny[i] + MOMW * nz[i];
+          MOMX[iP] = MOMX[iM] - 2.0 * mom_normal * (nx[i] + fac * (G11_ * G13[iM] + G12_ * G23[iM]));
+          MOMY[iP] = MOMY[iM] - 2.0 * mom_normal * (ny[i] + fac * (G12_ * G13[iM] + G22_ * G23[iM]));
+          MOMZ[iP] = MOMZ[iM] - 2.0 * mom_normal * fac / GsqrtV;
+        } else if (bc == 2) { /* NOSLIP */
+          MOMX[iP] = -MOMX[iM];
+          MOMY[iP] = -MOMY[iM];
+          MOMZ[iP] = -MOMZ[iM];
+        }
+      }
+    }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Tensor-product operators: ELM/scale_element_operation_tensorprod3D_kernel.F90
+ * (P7 instance at :2226-2339, :2415-2480; the .erb template generates the
+ * same sums for every p). */
+
+/* matvec_dirX: out(i,jk) = sum_l Mat(i,l) * in(l,jk) */
+static void op_dirX(int n1, const double *Mat, const double *in, double *out) {
+  for (int jk = 0; jk < n1 * n1; ++jk)
+    for (int i = 0; i < n1; ++i) {
+      double s = Mat[i] * in[n1 * jk];
+      for (int l = 1; l < n1; ++l) s = s + Mat[i + n1 * l] * in[l + n1 * jk];
+      out[i + n1 * jk] = s;
+    }
+}
+/* matvec_dirY: out(i,j,k) = sum_l in(i,l,k) * Mat_tr(l,j), Mat_tr(l,j) = Mat(j,l) */
+static void op_dirY(int n1, const double *Mat, const double *in, double *out) {
+  for (int k = 0; k < n1; ++k)
+    for (int j = 0; j < n1; ++j)
+      for (int i = 0; i < n1; ++i) {
+        double s = in[i + n1 * (0 + n1 * k)] * Mat[j];
+        for (int l = 1; l < n1; ++l) s = s + in[i + n1 * (l + n1 * k)] * Mat[j + n1 * l];
+        out[i + n1 * (j + n1 * k)] = s;
+      }
+}
+/* matvec_dirZ: out(i,j,k) = sum_l in(i,j,l) * Mat_tr(l,k); Mat_tr passed as stored */
+static void op_dirZ_tr(int n1, const double *Mat_tr, const double *in, double *out) {
+  const int n2 = n1 * n1;
+  for (int k = 0; k < n1; ++k)
+    for (int ij = 0; ij < n2; ++ij) {
+      double s = in[ij] * Mat_tr[0 + n1 * k];
+      for (int l = 1; l < n1; ++l) s = s + in[ij + n2 * l] * Mat_tr[l + n1 * k];
+      out[ij + n2 * k] = s;
+    }
+}
+/* dirZ with the derivative matrix: Mat_tr(l,k) = Mat(k,l) */
+static void op_dirZ(int n1, const double *Mat, const double *in, double *out) {
+  const int n2 = n1 * n1;
+  for (int k = 0; k < n1; ++k)
+    for (int ij = 0; ij < n2; ++ij) {
+      double s = in[ij] * Mat[k];
+      for (int l = 1; l < n1; ++l) s = s + in[ij + n2 * l] * Mat[k + n1 * l];
+      out[ij + n2 * k] = s;
+    }
+}
+/* matvec_Lift_hexahedral: six face terms per node, faces in reference order */
+static void op_lift(int n1, const double *Lift, const double *vin, double *out) {
+  const int n2 = n1 * n1, Np = n2 * n1;
+  for (int k = 0; k < n1; ++k)
+    for (int j = 0; j < n1; ++j)
+      for (int i = 0; i < n1; ++i) {
+        int p = i + n1 * (j + n1 * k);
+        out[p] = Lift[p] * vin[i + n1 * k]
+               + Lift[p + Np] * vin[j + n1 * k + n2]
+               + Lift[p + 2 * Np] * vin[i + n1 * k + 2 * n2]
+               + Lift[p + 3 * Np] * vin[j + n1 * k + 3 * n2]
+               + Lift[p + 4 * Np] * vin[i + n1 * j + 4 * n2]
+               + Lift[p + 5 * Np] * vin[i + n1 * j + 5 * n2];
+      }
+}
+
+/* ---------------------------------------------------------------------- */
+/* FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve_numflux.F90:946-1138 */
+static void heve_numflux_range(const oracle_mesh *m, int ke_s, int ke_e, double *del_flux,
+                               const double *DDENS_, const double *MOMX_, const double *MOMY_,
+                               const double *MOMZ_, const double *DRHOT_, const double *DPRES,
+                               const double *DENS_hyd, const double *PRES_hyd, const double *THERM_hyd) {
+  const int NfpTot = m->NfpTot, Ne = m->Ne;
+  const long NpNeA = (long)m->Np * m->NeA;
+  const long nfall = (long)NfpTot * Ne;
+  const double *G13 = m->GI3, *G23 = m->GI3 + NpNeA;
+  const double gamm = m->CPdry / m->CVdry;
+#pragma omp parallel for schedule(static)
+  for (int ke = ke_s; ke < ke_e; ++ke) {
+    for (int fp = 0; fp < NfpTot; ++fp) {
+      long i = fp + (long)NfpTot * ke;
+      long iM = m->vmapM[i] - 1, iP = m->vmapP[i] - 1;
+      double nx = m->normal_fn[i], ny = m->normal_fn[i + nfall], nz = m->normal_fn[i + 2 * nfall];
+
+      double Gsqrt_M = m->Gsqrt[iM], Gsqrt_P = m->Gsqrt[iP];
+      double RGsqrtV_M = 1.0 / Gsqrt_M, RGsqrtV_P = 1.0 / Gsqrt_P;
+      double G13_M = G13[iM], G13_P = G13[iP], G23_M = G23[iM], G23_P = G23[iP];
+
+      double GsqrtDDENS_M = Gsqrt_M * DDENS_[iM], GsqrtDDENS_P = Gsqrt_P * DDENS_[iP];
+      double GsqrtMOMX_M = Gsqrt_M * MOMX_[iM], GsqrtMOMX_P = Gsqrt_P * MOMX_[iP];
+      double GsqrtMOMY_M = Gsqrt_M * MOMY_[iM], GsqrtMOMY_P = Gsqrt_P * MOMY_[iP];
+      double GsqrtMOMZ_M = Gsqrt_M * MOMZ_[iM], GsqrtMOMZ_P = Gsqrt_P * MOMZ_[iP];
+      double GsqrtDRHOT_M = Gsqrt_M * DRHOT_[iM], GsqrtDRHOT_P = Gsqrt_P * DRHOT_[iP];
+
+      double Phyd_M = PRES_hyd[iM], Phyd_P = PRES_hyd[iP];
+      double dpres_M = DPRES[iM], dpres_P = DPRES[iP];
+
+      double GsqrtDens_M = GsqrtDDENS_M + Gsqrt_M * DENS_hyd[iM];
+      double GsqrtDens_P = GsqrtDDENS_P + Gsqrt_P * DENS_hyd[iP];
+      double GsqrtRhot_M = Gsqrt_M * THERM_hyd[iM] + GsqrtDRHOT_M;
+      double GsqrtRhot_P = Gsqrt_P * THERM_hyd[iP] + GsqrtDRHOT_P;
+
+      double Vel_M = (GsqrtMOMX_M * nx + GsqrtMOMY_M * ny +
+                      ((GsqrtMOMZ_M * RGsqrtV_M + G13_M * GsqrtMOMX_M + G23_M * GsqrtMOMY_M) * nz)) / GsqrtDens_M;
+      double Vel_P = (GsqrtMOMX_P * nx + GsqrtMOMY_P * ny +
+                      ((GsqrtMOMZ_P * RGsqrtV_P + G13_P * GsqrtMOMX_P + G23_P * GsqrtMOMY_P) * nz)) / GsqrtDens_P;
+
+      double tmp1 = fabs(nx) + fabs(ny);
+      double Gnn_M = tmp1 + (1.0 * (RGsqrtV_M * RGsqrtV_M) + G13_M * G13_M + G23_M * G23_M) * fabs(nz);
+      double Gnn_P = tmp1 + (1.0 * (RGsqrtV_P * RGsqrtV_P) + G13_P * G13_P + G23_P * G23_P) * fabs(nz);
+      double aM = sqrt(Gnn_M * gamm * (Phyd_M + dpres_M) * Gsqrt_M / GsqrtDens_M) + fabs(Vel_M);
+      double aP = sqrt(Gnn_P * gamm * (Phyd_P + dpres_P) * Gsqrt_P / GsqrtDens_P) + fabs(Vel_P);
+      double alpha = aM > aP ? aM : aP;
+
+      double *df = del_flux + (long)NfpTot * 5 * ke;
+      tmp1 = m->Fscale[i] * 0.5;
+      double tmp2 = -alpha * (GsqrtDDENS_P - GsqrtDDENS_M);
+      df[fp + NfpTot * DENS_VID] = tmp1 * (GsqrtDens_P * Vel_P - GsqrtDens_M * Vel_M + tmp2);
+      tmp2 = -alpha * (GsqrtDRHOT_P - GsqrtDRHOT_M);
+      df[fp + NfpTot * RHOT_VID] = tmp1 * (GsqrtRhot_P * Vel_P - GsqrtRhot_M * Vel_M + tmp2);
+
+      double tmp3 = Gsqrt_P * dpres_P;
+      double tmp4 = Gsqrt_M * dpres_M;
+      double mom1 = (tmp3 * RGsqrtV_P - tmp4 * RGsqrtV_M) * nz;
+      double mom2 = (nx + G13_P * nz) * tmp3 - (nx + G13_M * nz) * tmp4;
+      double mom3 = (ny + G23_P * nz) * tmp3 - (ny + G23_M * nz) * tmp4;
+
+      tmp2 = -alpha * (GsqrtMOMZ_P - GsqrtMOMZ_M);
+      df[fp + NfpTot * MOMZ_VID] = tmp1 * (GsqrtMOMZ_P * Vel_P - GsqrtMOMZ_M * Vel_M + mom1 + tmp2);
+      tmp2 = -alpha * (GsqrtMOMX_P - GsqrtMOMX_M);
+      df[fp + NfpTot * MOMX_VID] = tmp1 * (GsqrtMOMX_P * Vel_P - GsqrtMOMX_M * Vel_M + mom2 + tmp2);
+      tmp2 = -alpha * (GsqrtMOMY_P - GsqrtMOMY_M);
+      df[fp + NfpTot * MOMY_VID] = tmp1 * (GsqrtMOMY_P * Vel_P - GsqrtMOMY_M * Vel_M + mom3 + tmp2);
+    }
+  }
+}
+
+void oracle_heve_numflux(const oracle_mesh *m, double *del_flux,
+                         const double *DDENS_, const double *MOMX_, const double *MOMY_,
+                         const double *MOMZ_, const double *DRHOT_, const double *DPRES,
+                         const double *DENS_hyd, const double *PRES_hyd, const double *THERM_hyd) {
+  heve_numflux_range(m, 0, m->Ne, del_flux, DDENS_, MOMX_, MOMY_, MOMZ_, DRHOT_, DPRES, DENS_hyd, PRES_hyd,
+                     THERM_hyd);
+}
+
+/* ---------------------------------------------------------------------- */
+/* FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve.F90:372-484 (element loop),
+ * with Div_var5 = ELM/scale_element_operation_tensorprod3D.F90:2760-2783 and
+ * VFilterPM1 = :2828-2839. del_flux holds the numflux of elements [ke_s, ke_e). */
+static void heve_elements(const oracle_mesh *m, int ke_s, int ke_e, const double *del_flux,
+                          double *DENS_dt, double *MOMX_dt, double *MOMY_dt, double *MOMZ_dt, double *RHOT_dt,
+                          const double *DDENS_, const double *MOMX_, const double *MOMY_,
+                          const double *MOMZ_, const double *DRHOT_, const double *DPRES_,
+                          const double *DENS_hyd, const double *THERM_hyd, const double *CORIOLIS,
+                          const double *DPhydDx, const double *DPhydDy) {
+  const int n1 = m->n1, Np = m->Np, NfpTot = m->NfpTot, Nfp_v = m->Nfp_v, Ne = m->Ne;
+  const long NpNeA = (long)Np * m->NeA;
+  const long NpNe = (long)Np * Ne;
+  const double *GI3_1 = m->GI3, *GI3_2 = m->GI3 + NpNeA;
+  const double Grav = m->GRAV;
+#pragma omp parallel
+  {
+    double *Flux = (double *)malloc(sizeof(double) * Np * 15);  /* Flux(Np,3,5)  */
+    double *DFlux = (double *)malloc(sizeof(double) * Np * 20); /* DFlux(Np,4,5) */
+    double *drho = (double *)malloc(sizeof(double) * Np);
+    double *RDENS = (double *)malloc(sizeof(double) * Np);
+    double *RGsqrtV = (double *)malloc(sizeof(double) * Np);
+    double *RGsqrt = (double *)malloc(sizeof(double) * Np);
+#pragma omp for schedule(static)
+    for (int ke = ke_s; ke < ke_e; ++ke) {
+      const int ke2d = m->EMap3Dto2D[ke] - 1;
+      const long o = (long)Np * ke;
+      for (int p = 0; p < Np; ++p) {
+        double GsqrtV = m->Gsqrt[o + p] / m->GsqrtH[(m->IndexH2Dto3D[p] - 1) + (long)Nfp_v * ke2d];
+        RGsqrtV[p] = 1.0 / GsqrtV;
+        RGsqrt[p] = 1.0 / m->Gsqrt[o + p];
+        RDENS[p] = 1.0 / (DDENS_[o + p] + DENS_hyd[o + p]);
+      }
+#define FL(p, d, v) Flux[(p) + Np * ((d) + 3 * (v))]
+#define DF(p, d, v) DFlux[(p) + Np * ((d) + 4 * (v))]
+      for (int p = 0; p < Np; ++p) {
+        double Gsqrt_ = m->Gsqrt[o + p];
+        FL(p, 0, DENS_VID) = Gsqrt_ * MOMX_[o + p];
+        FL(p, 1, DENS_VID) = Gsqrt_ * MOMY_[o + p];
+        FL(p, 2, DENS_VID) = Gsqrt_ * (MOMZ_[o + p] * RGsqrtV[p] + GI3_1[o + p] * MOMX_[o + p] + GI3_2[o + p] * MOMY_[o + p]);
+      }
+      for (int p = 0; p < Np; ++p) {
+        double pt_ = (THERM_hyd[o + p] + DRHOT_[o + p]) * RDENS[p];
+        FL(p, 0, RHOT_VID) = FL(p, 0, DENS_VID) * pt_;
+        FL(p, 1, RHOT_VID) = FL(p, 1, DENS_VID) * pt_;
+        FL(p, 2, RHOT_VID) = FL(p, 2, DENS_VID) * pt_;
+        double w_ = MOMZ_[o + p] * RDENS[p];
+        FL(p, 0, MOMZ_VID) = FL(p, 0, DENS_VID) * w_;
+        FL(p, 1, MOMZ_VID) = FL(p, 1, DENS_VID) * w_;
+        FL(p, 2, MOMZ_VID) = FL(p, 2, DENS_VID) * w_ + m->Gsqrt[o + p] * DPRES_[o + p] * RGsqrtV[p];
+      }
+      for (int p = 0; p < Np; ++p) {
+        double GsqrtDPRES_ = m->Gsqrt[o + p] * DPRES_[o + p];
+        double u_ = MOMX_[o + p] * RDENS[p];
+        FL(p, 0, MOMX_VID) = FL(p, 0, DENS_VID) * u_ + GsqrtDPRES_;
+        FL(p, 1, MOMX_VID) = FL(p, 1, DENS_VID) * u_;
+        FL(p, 2, MOMX_VID) = FL(p, 2, DENS_VID) * u_ + GsqrtDPRES_ * GI3_1[o + p];
+        double v_ = MOMY_[o + p] * RDENS[p];
+        FL(p, 0, MOMY_VID) = FL(p, 0, DENS_VID) * v_;
+        FL(p, 1, MOMY_VID) = FL(p, 1, DENS_VID) * v_ + GsqrtDPRES_;
+        FL(p, 2, MOMY_VID) = FL(p, 2, DENS_VID) * v_ + GsqrtDPRES_ * GI3_2[o + p];
+      }
+      /* Div_var5: Lift first, then the three line contractions, per variable */
+      const double *dfe = del_flux + (long)NfpTot * 5 * ke;
+      for (int v = 0; v < 5; ++v) op_lift(n1, m->Lift_mat, dfe + (long)NfpTot * v, &DF(0, 3, v));
+      for (int v = 0; v < 5; ++v) {
+        op_dirX(n1, m->D1D, &FL(0, 0, v), &DF(0, 0, v));
+        op_dirY(n1, m->D1D, &FL(0, 1, v), &DF(0, 1, v));
+        op_dirZ(n1, m->D1D, &FL(0, 2, v), &DF(0, 2, v));
+      }
+      const double *E11 = m->Escale + o, *E22 = m->Escale + o + NpNe * 4, *E33 = m->Escale + o + NpNe * 8;
+      for (int p = 0; p < Np; ++p) {
+        DENS_dt[o + p] = -(E11[p] * DF(p, 0, DENS_VID) + E22[p] * DF(p, 1, DENS_VID) + E33[p] * DF(p, 2, DENS_VID) +
+                           DF(p, 3, DENS_VID)) * RGsqrt[p];
+        RHOT_dt[o + p] = -(E11[p] * DF(p, 0, RHOT_VID) + E22[p] * DF(p, 1, RHOT_VID) + E33[p] * DF(p, 2, RHOT_VID) +
+                           DF(p, 3, RHOT_VID)) * RGsqrt[p];
+      }
+      op_dirZ_tr(n1, m->IntrpMat_tr, DDENS_ + o, drho);
+      for (int p = 0; p < Np; ++p) {
+        MOMZ_dt[o + p] = -(E11[p] * DF(p, 0, MOMZ_VID) + E22[p] * DF(p, 1, MOMZ_VID) + E33[p] * DF(p, 2, MOMZ_VID) +
+                           DF(p, 3, MOMZ_VID)) * RGsqrt[p] - Grav * drho[p];
+      }
+      for (int p = 0; p < Np; ++p) {
+        double cor = CORIOLIS[(m->IndexH2Dto3D[p] - 1) + (long)Nfp_v * ke2d];
+        MOMX_dt[o + p] = -DPhydDx[o + p] + cor * MOMY_[o + p];
+        MOMY_dt[o + p] = -DPhydDy[o + p] - cor * MOMX_[o + p];
+      }
+      for (int p = 0; p < Np; ++p) {
+        MOMX_dt[o + p] = MOMX_dt[o + p] - (E11[p] * DF(p, 0, MOMX_VID) + E22[p] * DF(p, 1, MOMX_VID) +
+                                           E33[p] * DF(p, 2, MOMX_VID) + DF(p, 3, MOMX_VID)) * RGsqrt[p];
+        MOMY_dt[o + p] = MOMY_dt[o + p] - (E11[p] * DF(p, 0, MOMY_VID) + E22[p] * DF(p, 1, MOMY_VID) +
+                                           E33[p] * DF(p, 2, MOMY_VID) + DF(p, 3, MOMY_VID)) * RGsqrt[p];
+      }
+#undef FL
+#undef DF
+    }
+    free(Flux);
+    free(DFlux);
+    free(drho);
+    free(RDENS);
+    free(RGsqrtV);
+    free(RGsqrt);
+  }
+}
+
+/* The reference computes the face flux of every element first
+ * (get_ebnd_flux, :350-357) and then runs the element loop. */
+void oracle_heve_cal_tend_range(const oracle_mesh *m, int ke_s, int ke_e,
+                          double *DENS_dt, double *MOMX_dt, double *MOMY_dt, double *MOMZ_dt, double *RHOT_dt,
+                          const double *DDENS_, const double *MOMX_, const double *MOMY_,
+                          const double *MOMZ_, const double *DRHOT_, const double *DPRES_,
+                          const double *DENS_hyd, const double *PRES_hyd, const double *PRES_hyd_ref,
+                          const double *THERM_hyd, const double *CORIOLIS,
+                          const double *Rtot, const double *CVtot, const double *CPtot,
+                          const double *DPhydDx, const double *DPhydDy) {
+  (void)PRES_hyd_ref;
+  (void)Rtot;
+  (void)CVtot;
+  (void)CPtot; /* part of the interface; the dry-air constants are used instead */
+  double *del_flux = (double *)malloc(sizeof(double) * (size_t)m->NfpTot * 5 * m->Ne);
+  heve_numflux_range(m, ke_s, ke_e, del_flux, DDENS_, MOMX_, MOMY_, MOMZ_, DRHOT_, DPRES_, DENS_hyd, PRES_hyd,
+                     THERM_hyd);
+  heve_elements(m, ke_s, ke_e, del_flux, DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt, DDENS_, MOMX_, MOMY_,
+                MOMZ_, DRHOT_, DPRES_, DENS_hyd, THERM_hyd, CORIOLIS, DPhydDx, DPhydDy);
+  free(del_flux);
+}
+
+void oracle_heve_cal_tend(const oracle_mesh *m,
+                          double *DENS_dt, double *MOMX_dt, double *MOMY_dt, double *MOMZ_dt, double *RHOT_dt,
+                          const double *DDENS_, const double *MOMX_, const double *MOMY_,
+                          const double *MOMZ_, const double *DRHOT_, const double *DPRES_,
+                          const double *DENS_hyd, const double *PRES_hyd, const double *PRES_hyd_ref,
+                          const double *THERM_hyd, const double *CORIOLIS,
+                          const double *Rtot, const double *CVtot, const double *CPtot,
+                          const double *DPhydDx, const double *DPhydDy) {
+  oracle_heve_cal_tend_range(m, 0, m->Ne, DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt, DDENS_, MOMX_, MOMY_,
+                             MOMZ_, DRHOT_, DPRES_, DENS_hyd, PRES_hyd, PRES_hyd_ref, THERM_hyd, CORIOLIS, Rtot,
+                             CVtot, CPtot, DPhydDx, DPhydDy);
+}
+
+/* ---------------------------------------------------------------------- */
+/* FDS/scale_atm_dyn_dgm_nonhydro3d_common.F90:624-777 with gam = 1 (the
+ * regional mesh sets lcmesh%gam = 1, MSH/scale_mesh_base3d.F90:316) */
+void oracle_calc_phyd_hgrad(const oracle_mesh *m, double *DPhydDx, double *DPhydDy,
+                            const double *PRES_hyd, const double *PRES_hyd_ref) {
+  const int n1 = m->n1, Np = m->Np, NfpTot = m->NfpTot, Nfp_v = m->Nfp_v, Ne = m->Ne;
+  const long NpNeA = (long)Np * m->NeA, NpNe = (long)Np * Ne, nfall = (long)NfpTot * Ne;
+  const double *G13 = m->GI3, *G23 = m->GI3 + NpNeA;
+  double *dfh = (double *)malloc(sizeof(double) * (size_t)NfpTot * 2 * Ne);
+#pragma omp parallel for schedule(static)
+  for (int ke = 0; ke < Ne; ++ke) {
+    int ke2D = m->EMap3Dto2D[ke] - 1;
+    for (int fp = 0; fp < NfpTot; ++fp) {
+      long i = fp + (long)NfpTot * ke;
+      long iM = m->vmapM[i] - 1, iP = m->vmapP[i] - 1;
+      double GH = m->GsqrtH[(m->IndexH2Dto3D_bnd[fp] - 1) + (long)Nfp_v * ke2D];
+      double GsqrtV_M = m->Gsqrt[iM] / 1.0 / GH;
+      double GsqrtV_P = m->Gsqrt[iP] / 1.0 / GH;
+      double DP_M = PRES_hyd[iM] - PRES_hyd_ref[iM];
+      double DP_P = PRES_hyd[iP] - PRES_hyd_ref[iP];
+      double nx = m->normal_fn[i], ny = m->normal_fn[i + nfall], nz = m->normal_fn[i + 2 * nfall];
+      double tmp1 = m->Fscale[i] * 0.5 * GsqrtV_P * DP_P;
+      double tmp2 = m->Fscale[i] * 0.5 * GsqrtV_M * DP_M;
+      dfh[fp + NfpTot * (0 + 2 * ke)] = (nx + G13[iP] * nz) * tmp1 - (nx + G13[iM] * nz) * tmp2;
+      dfh[fp + NfpTot * (1 + 2 * ke)] = (ny + G23[iP] * nz) * tmp1 - (ny + G23[iM] * nz) * tmp2;
+    }
+  }
+#pragma omp parallel
+  {
+    double *F = (double *)malloc(sizeof(double) * Np * 4);
+    double *D = (double *)malloc(sizeof(double) * Np * 8);
+    double *RG = (double *)malloc(sizeof(double) * Np);
+#pragma omp for schedule(static)
+    for (int ke = 0; ke < Ne; ++ke) {
+      int ke2d = m->EMap3Dto2D[ke] - 1;
+      long o = (long)Np * ke;
+      for (int p = 0; p < Np; ++p) {
+        double GsqrtV = m->Gsqrt[o + p] / (1.0 * m->GsqrtH[(m->IndexH2Dto3D[p] - 1) + (long)Nfp_v * ke2d]);
+        RG[p] = 1.0 / GsqrtV;
+        F[p] = GsqrtV * (PRES_hyd[o + p] - PRES_hyd_ref[o + p]);
+      }
+      for (int p = 0; p < Np; ++p) {
+        F[p + Np] = F[p];
+        F[p + 2 * Np] = G13[o + p] * F[p];
+        F[p + 3 * Np] = G23[o + p] * F[p];
+      }
+      /* Div(Flux, del_flux_hyd(:,1,ke)) -> D(:,1..4,1); Dz(Fz) -> D(:,3,2); Lift -> D(:,4,2) */
+      op_lift(n1, m->Lift_mat, dfh + (long)NfpTot * (0 + 2 * ke), D + 3 * Np);
+      op_dirX(n1, m->D1D, F, D);
+      op_dirY(n1, m->D1D, F + Np, D + Np);
+      op_dirZ(n1, m->D1D, F + 2 * Np, D + 2 * Np);
+      op_dirZ(n1, m->D1D, F + 3 * Np, D + 6 * Np);
+      op_lift(n1, m->Lift_mat, dfh + (long)NfpTot * (1 + 2 * ke), D + 7 * Np);
+      const double *E11 = m->Escale + o, *E22 = m->Escale + o + NpNe * 4, *E33 = m->Escale + o + NpNe * 8;
+      for (int p = 0; p < Np; ++p) {
+        double gx = E11[p] * D[p] + E33[p] * D[p + 2 * Np] + D[p + 3 * Np];
+        double gy = E22[p] * D[p + Np] + E33[p] * D[p + 6 * Np] + D[p + 7 * Np];
+        DPhydDx[o + p] = gx * RG[p];
+        DPhydDy[o + p] = gy * RG[p];
+      }
+    }
+    free(F);
+    free(D);
+    free(RG);
+  }
+  free(dfh);
+}
+
+/* ---------------------------------------------------------------------- */
+/* CMN/scale_timeint_rk_butcher_tab.F90:60-324 */
+int oracle_rk_tab_get(const char *name, double dt, oracle_rk_tab *t) {
+  memset(t, 0, sizeof(*t));
+  t->dt = dt;
+#define A_EX(s, u) t->a_ex[((s)-1) * 4 + ((u)-1)]
+#define A_IM(s, u) t->a_im[((s)-1) * 4 + ((u)-1)]
+  if (!strcmp(name, "ERK_4s4o") || !strcmp(name, "ERK_RK4")) {
+    t->nstage = 4;
+    t->tend_buf_size = 1;
+    t->imex = 0;
+    A_EX(2, 1) = 0.5;
+    A_EX(3, 2) = 0.5;
+    A_EX(4, 3) = 1.0;
+    t->b_ex[0] = 1.0 / 6.0;
+    t->b_ex[1] = 2.0 / 6.0;
+    t->b_ex[2] = 2.0 / 6.0;
+    t->b_ex[3] = 1.0 / 6.0;
+    for (int s = 0; s < 4; ++s) t->tend_buf_indmap[s] = 0;
+  } else if (!strcmp(name, "ERK_1s1o") || !strcmp(name, "ERK_Euler")) {
+    t->nstage = 1;
+    t->tend_buf_size = 1;
+    t->b_ex[0] = 1.0;
+    t->tend_buf_indmap[0] = 0;
+  } else if (!strcmp(name, "IMEX_ARK232")) {
+    double alp = (3.0 + 2.0 * sqrt(2.0)) / 6.0;
+    double gam = 1.0 - 1.0 / sqrt(2.0);
+    double del = 1.0 / (2.0 * sqrt(2.0));
+    t->nstage = 3;
+    t->tend_buf_size = 3;
+    t->imex = 1;
+    A_EX(2, 1) = 2.0 * gam;
+    A_EX(3, 1) = 1.0 - alp;
+    A_EX(3, 2) = alp;
+    t->b_ex[0] = del;
+    t->b_ex[1] = del;
+    t->b_ex[2] = gam;
+    A_IM(2, 1) = gam;
+    A_IM(2, 2) = gam;
+    A_IM(3, 1) = del;
+    A_IM(3, 2) = del;
+    A_IM(3, 3) = gam;
+    for (int s = 0; s < 3; ++s) t->b_im[s] = t->b_ex[s];
+    for (int s = 0; s < 3; ++s) t->tend_buf_indmap[s] = s;
+  } else if (!strcmp(name, "IMEX_ARK324")) {
+    t->nstage = 4;
+    t->tend_buf_size = 4;
+    t->imex = 1;
+    A_EX(2, 1) = 1767732205903.0 / 2027836641118.0;
+    A_EX(3, 1) = 5535828885825.0 / 10492691773637.0;
+    A_EX(3, 2) = 788022342437.0 / 10882634858940.0;
+    A_EX(4, 1) = 6485989280629.0 / 16251701735622.0;
+    A_EX(4, 2) = -4246266847089.0 / 9704473918619.0;
+    A_EX(4, 3) = 10755448449292.0 / 10357097424841.0;
+    t->b_ex[0] = 1471266399579.0 / 7840856788654.0;
+    t->b_ex[1] = -4482444167858.0 / 7529755066697.0;
+    t->b_ex[2] = 11266239266428.0 / 11593286722821.0;
+    t->b_ex[3] = 1767732205903.0 / 4055673282236.0;
+    A_IM(2, 1) = 1767732205903.0 / 4055673282236.0;
+    A_IM(2, 2) = 1767732205903.0 / 4055673282236.0;
+    A_IM(3, 1) = 2746238789719.0 / 10658868560708.0;
+    A_IM(3, 2) = -640167445237.0 / 6845629431997.0;
+    A_IM(3, 3) = 1767732205903.0 / 4055673282236.0;
+    for (int s = 0; s < 4; ++s) A_IM(4, s + 1) = t->b_ex[s];
+    for (int s = 0; s < 4; ++s) t->b_im[s] = t->b_ex[s];
+    for (int s = 0; s < 4; ++s) t->tend_buf_indmap[s] = s;
+  } else {
+    return 1;
+  }
+#undef A_EX
+#undef A_IM
+  return 0;
+}
+
+/* CMN/scale_timeint_rk.F90:2201-2355 (rk_advance_general2D), 1-based stage */
+void oracle_rk_advance(const oracle_rk_tab *t, int stage, long n, double *q, double *var0,
+                       double *varTmp, const double *tend_ex, const double *tend_im, long bs) {
+  const int ind = t->tend_buf_indmap[stage - 1];
+  if (t->nstage == 1)
+    for (long i = 0; i < n; ++i) varTmp[i] = q[i];
+  if (stage == t->nstage) {
+    double cbe = t->b_ex[stage - 1] * t->dt;
+    if (t->imex) {
+      double cbi = t->b_im[stage - 1] * t->dt;
+#pragma omp parallel for schedule(static)
+      for (long i = 0; i < n; ++i) q[i] = varTmp[i] + cbe * tend_ex[ind * bs + i] + cbi * tend_im[ind * bs + i];
+    } else {
+#pragma omp parallel for schedule(static)
+      for (long i = 0; i < n; ++i) q[i] = varTmp[i] + cbe * tend_ex[ind * bs + i];
+    }
+    return;
+  }
+  if (stage == 1 && !t->imex)
+    for (long i = 0; i < n; ++i) {
+      var0[i] = q[i];
+      varTmp[i] = q[i];
+    }
+  double cbe = t->b_ex[stage - 1] * t->dt;
+  if (t->imex) {
+    double cbi = t->b_im[stage - 1] * t->dt;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+      q[i] = var0[i];
+      varTmp[i] = varTmp[i] + cbe * tend_ex[ind * bs + i] + cbi * tend_im[ind * bs + i];
+    }
+  } else {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+      q[i] = var0[i];
+      varTmp[i] = varTmp[i] + cbe * tend_ex[ind * bs + i];
+    }
+  }
+  if (t->tend_buf_size == 1 && !t->imex) {
+    double ca = t->dt * t->a_ex[stage * 4 + (stage - 1)];
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) q[i] = var0[i] + ca * tend_ex[i];
+  } else if (!t->imex) {
+    for (int s = 1; s <= stage; ++s) {
+      double ca = t->dt * t->a_ex[stage * 4 + (s - 1)];
+#pragma omp parallel for schedule(static)
+      for (long i = 0; i < n; ++i) q[i] = q[i] + ca * tend_ex[(s - 1) * bs + i];
+    }
+  } else {
+    for (int s = 1; s <= stage; ++s) {
+      double ca = t->dt * t->a_ex[stage * 4 + (s - 1)];
+      double ci = t->dt * t->a_im[stage * 4 + (s - 1)];
+#pragma omp parallel for schedule(static)
+      for (long i = 0; i < n; ++i) q[i] = q[i] + ca * tend_ex[(s - 1) * bs + i] + ci * tend_im[(s - 1) * bs + i];
+    }
+  }
+}
+
+/* CMN/scale_timeint_rk.F90:2510-2560 (rk_storeimpl_general2D) */
+void oracle_rk_store_impl(const oracle_rk_tab *t, int stage, long n, double *q, double *var0,
+                          double *varTmp, const double *tend_im, long bs) {
+  if (!t->imex) return;
+  const int ind = t->tend_buf_indmap[stage - 1];
+  const double ca = t->dt * t->a_im[(stage - 1) * 4 + (stage - 1)];
+  if (stage == 1)
+    for (long i = 0; i < n; ++i) {
+      var0[i] = q[i];
+      varTmp[i] = q[i];
+    }
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) q[i] = q[i] + ca * tend_im[ind * bs + i];
+}
+
+/* ---------------------------------------------------------------------- */
+/* FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:614-963, HEVE branch */
+void oracle_heve_update(const oracle_mesh *m, const oracle_rk_tab *t, int nsteps,
+                        double *const q[5], double *PRES, double *DPRES,
+                        const double *DENS_hyd, const double *PRES_hyd, const double *PRES_hyd_ref,
+                        const double *THERM_hyd, const double *CORIOLIS,
+                        const double *Rtot, const double *CVtot, const double *CPtot,
+                        const double *DPhydDx, const double *DPhydDy, double *work) {
+  const long NpNeA = (long)m->Np * m->NeA;
+  const long n = (long)m->Np * m->Ne; /* nodes NeS..NeE */
+  double *var0 = work;
+  double *varTmp = work + 5 * NpNeA;
+  double *tend = work + 10 * NpNeA; /* tend_buf2D_ex(Np,NeA,5,tend_buf_size) */
+  for (int step = 0; step < nsteps; ++step) {
+    for (int s = 1; s <= t->nstage; ++s) {
+      for (int v = 0; v < 5; ++v) oracle_halo_exchange(m, q[v]);
+      oracle_drhot2pres(m, PRES, DPRES, q[RHOT_VID], PRES_hyd, THERM_hyd, Rtot, CVtot, CPtot);
+      oracle_halo_exchange(m, DPRES);
+      oracle_apply_bc(m, q[DENS_VID], q[MOMX_VID], q[MOMY_VID], q[MOMZ_VID], q[RHOT_VID]);
+      const int ind = t->tend_buf_indmap[s - 1];
+      double *T = tend + (long)ind * 5 * NpNeA;
+      oracle_heve_cal_tend(m, T + DENS_VID * NpNeA, T + MOMX_VID * NpNeA, T + MOMY_VID * NpNeA,
+                           T + MOMZ_VID * NpNeA, T + RHOT_VID * NpNeA, q[DENS_VID], q[MOMX_VID], q[MOMY_VID],
+                           q[MOMZ_VID], q[RHOT_VID], DPRES, DENS_hyd, PRES_hyd, PRES_hyd_ref, THERM_hyd,
+                           CORIOLIS, Rtot, CVtot, CPtot, DPhydDx, DPhydDy);
+      /* add_phy_tend adds zero physics tendencies (dry run without physics). */
+      for (int v = 0; v < 5; ++v)
+        oracle_rk_advance(t, s, n, q[v], var0 + v * NpNeA, varTmp + v * NpNeA, tend + v * NpNeA, NULL,
+                          5 * NpNeA);
+    }
+    oracle_drhot2pres(m, PRES, DPRES, q[RHOT_VID], PRES_hyd, THERM_hyd, Rtot, CVtot, CPtot);
+  }
+}

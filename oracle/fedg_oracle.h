@@ -1,0 +1,139 @@
+/*
+ * CPU oracle for the FElib regional nonhydrostatic DG dynamical-core step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker and the timed
+ * CPU baseline; the product library never links, loads or calls it.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+ *
+ * It restates the reference Fortran (ywkawai/FE-Project, FElib) in plain C with
+ * the same loop order and the same order of floating-point operations, compiled
+ * with -ffp-contract=off so that no multiply-add is fused that Fortran would
+ * round twice.  Every function cites the reference routine it follows.
+ *
+ * Layout: Fortran column-major arrays, X(Np,NeA) stored as X[p + Np*ke];
+ * integer maps keep the reference's 1-based values.
+ *
+ * Parity status of each function (how it is pinned) is listed in the header
+ * comment of fedg_oracle.c.
+ */
+#ifndef FEDG_ORACLE_H
+#define FEDG_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int n1, Np, Nfp_h, Nfp_v, NfpTot;
+  int Ne, NeA, Ne2D, nB;
+  /* geometry (LocalMesh3D) */
+  const double *Escale;     /* (Np,Ne,3,3)      */
+  const double *Gsqrt;      /* (Np,NeA)         */
+  const double *GsqrtH;     /* (Nfp_v,Ne2D)     */
+  const double *GI3;        /* (Np,NeA,2)       */
+  const double *G11, *G12, *G22; /* GIJ (Nfp_v,Ne2D) */
+  const double *normal_fn;  /* (NfpTot,Ne,3)    */
+  const double *Fscale;     /* (NfpTot,Ne)      */
+  const int32_t *vmapM, *vmapP;  /* (NfpTot,Ne) 1-based */
+  const int32_t *vmapB;          /* (nB) 1-based */
+  const int32_t *halo_src;       /* (nB) 1-based source of each halo slot */
+  const int32_t *velbc;          /* (nB) 0: no velocity BC, 1: SLIP, 2: NOSLIP */
+  const int32_t *EMap3Dto2D;     /* (Ne) 1-based */
+  const int32_t *IndexH2Dto3D;   /* (Np) 1-based */
+  const int32_t *IndexH2Dto3D_bnd; /* (NfpTot) 1-based */
+  /* element operators (ElementOperationTensorProd3D) */
+  const double *D1D;        /* (n1,n1)          */
+  const double *Lift_mat;   /* (n1,n1,n1,6)     */
+  const double *IntrpMat_tr;/* (n1,n1)          */
+  /* physical constants (SCALE scale_const) */
+  double GRAV, Rdry, CPdry, CVdry, PRES00;
+} oracle_mesh;
+
+/* FDS/scale_atm_dyn_dgm_nonhydro3d_common.F90:428-479 (CPU branch) */
+void oracle_drhot2pres(const oracle_mesh *m, double *PRES, double *DPRES,
+                       const double *DRHOT, const double *PRES_hyd, const double *THERM_hyd,
+                       const double *Rtot, const double *CVtot, const double *CPtot);
+
+/* single-tile MeshFieldComm_Exchange: val(Np*Ne + b) = val(halo_src(b)) */
+void oracle_halo_exchange(const oracle_mesh *m, double *val);
+
+/* FDS/scale_atm_dyn_dgm_bnd.F90:270-367 */
+void oracle_apply_bc(const oracle_mesh *m, double *DDENS, double *MOMX, double *MOMY,
+                     double *MOMZ, double *THERM);
+
+/* FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve_numflux.F90:946-1138; del_flux(NfpTot,5,Ne) */
+void oracle_heve_numflux(const oracle_mesh *m, double *del_flux,
+                         const double *DDENS_, const double *MOMX_, const double *MOMY_,
+                         const double *MOMZ_, const double *DRHOT_, const double *DPRES,
+                         const double *DENS_hyd, const double *PRES_hyd, const double *THERM_hyd);
+
+/* FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve.F90:292-489, elements [ke_s, ke_e) (0-based) */
+void oracle_heve_cal_tend_range(const oracle_mesh *m, int ke_s, int ke_e,
+                          double *DENS_dt, double *MOMX_dt, double *MOMY_dt, double *MOMZ_dt, double *RHOT_dt,
+                          const double *DDENS_, const double *MOMX_, const double *MOMY_,
+                          const double *MOMZ_, const double *DRHOT_, const double *DPRES_,
+                          const double *DENS_hyd, const double *PRES_hyd, const double *PRES_hyd_ref,
+                          const double *THERM_hyd, const double *CORIOLIS,
+                          const double *Rtot, const double *CVtot, const double *CPtot,
+                          const double *DPhydDx, const double *DPhydDy);
+void oracle_heve_cal_tend(const oracle_mesh *m,
+                          double *DENS_dt, double *MOMX_dt, double *MOMY_dt, double *MOMZ_dt, double *RHOT_dt,
+                          const double *DDENS_, const double *MOMX_, const double *MOMY_,
+                          const double *MOMZ_, const double *DRHOT_, const double *DPRES_,
+                          const double *DENS_hyd, const double *PRES_hyd, const double *PRES_hyd_ref,
+                          const double *THERM_hyd, const double *CORIOLIS,
+                          const double *Rtot, const double *CVtot, const double *CPtot,
+                          const double *DPhydDx, const double *DPhydDy);
+
+/* FDS/scale_atm_dyn_dgm_nonhydro3d_common.F90:624-777, hydrostatic pressure gradient */
+void oracle_calc_phyd_hgrad(const oracle_mesh *m, double *DPhydDx, double *DPhydDy,
+                            const double *PRES_hyd, const double *PRES_hyd_ref);
+
+/* ---- time integration (CMN/scale_timeint_rk.F90, CMN/scale_timeint_rk_butcher_tab.F90) ---- */
+typedef struct {
+  int nstage, tend_buf_size, imex;
+  double dt;
+  double a_ex[16], b_ex[4], a_im[16], b_im[4]; /* a(s,t) at [s*4 + t], 0-based */
+  int tend_buf_indmap[4];                      /* 0-based buffer index per stage */
+} oracle_rk_tab;
+/* returns 0 on success, nonzero for an unknown scheme name */
+int oracle_rk_tab_get(const char *name, double dt, oracle_rk_tab *t);
+
+/* rk_advance_general2D for one variable over nodes [0, n): CMN/scale_timeint_rk.F90:2201-2355.
+ * var0/varTmp: this variable's slices; tend_ex/tend_im: this variable's
+ * tendency in buffer 0, buffer b starting bs doubles later. */
+void oracle_rk_advance(const oracle_rk_tab *t, int stage, long n, double *q, double *var0,
+                       double *varTmp, const double *tend_ex, const double *tend_im, long bs);
+/* rk_storeimpl_general2D: CMN/scale_timeint_rk.F90:2510-2560 */
+void oracle_rk_store_impl(const oracle_rk_tab *t, int stage, long n, double *q, double *var0,
+                          double *varTmp, const double *tend_im, long bs);
+
+/* One HEVE step of AtmDynDGMDriver_nonhydro3d_update
+ * (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:614-963) with hevi_flag=.false.,
+ * hide_mpi_comm_flag=.false., no sponge, no tracers, no modal filter, dry
+ * air and physics tendencies that are zero (add_phy_tend adds exact zeros).
+ * q[v] are the 5 prognostic fields (Np,NeA) in the order DDENS, DRHOT, MOMZ,
+ * MOMX, MOMY (PRGVAR ids 1..5).  work must hold 5*Np*NeA*(2 + tend_buf_size) +
+ * 2*Np*NeA doubles.  PRES/DPRES are updated as the reference does. */
+void oracle_heve_update(const oracle_mesh *m, const oracle_rk_tab *t, int nsteps,
+                        double *const q[5], double *PRES, double *DPRES,
+                        const double *DENS_hyd, const double *PRES_hyd, const double *PRES_hyd_ref,
+                        const double *THERM_hyd, const double *CORIOLIS,
+                        const double *Rtot, const double *CVtot, const double *CPtot,
+                        const double *DPhydDx, const double *DPhydDy, double *work);
+
+int oracle_num_threads(void);
+
+/* pow used for the pressure (drhot2pres): 0 = the C library's pow (what the
+ * reference's ** compiles to with gfortran/glibc, the default); 1 = the
+ * portable double-double pow of fe-project_amd/csrc/fedg_pow.h, which the GPU
+ * kernels use, so that whole steps can be compared bit for bit. */
+void oracle_set_pow_mode(int mode);
+int oracle_get_pow_mode(void);
+void oracle_pow_array(const double *x, const double *y, double *out, long n, int mode);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

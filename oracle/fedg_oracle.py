@@ -1,0 +1,215 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.  The oracle takes
+the same host arrays (mesh, operators, base state) as the product and
+restates the reference Fortran on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int32)
+
+
+class OracleMeshC(C.Structure):
+    _fields_ = [
+        ("n1", C.c_int), ("Np", C.c_int), ("Nfp_h", C.c_int), ("Nfp_v", C.c_int), ("NfpTot", C.c_int),
+        ("Ne", C.c_int), ("NeA", C.c_int), ("Ne2D", C.c_int), ("nB", C.c_int),
+        ("Escale", dp), ("Gsqrt", dp), ("GsqrtH", dp), ("GI3", dp), ("G11", dp), ("G12", dp), ("G22", dp),
+        ("normal_fn", dp), ("Fscale", dp),
+        ("vmapM", ip), ("vmapP", ip), ("vmapB", ip), ("halo_src", ip), ("velbc", ip),
+        ("EMap3Dto2D", ip), ("IndexH2Dto3D", ip), ("IndexH2Dto3D_bnd", ip),
+        ("D1D", dp), ("Lift_mat", dp), ("IntrpMat_tr", dp),
+        ("GRAV", C.c_double), ("Rdry", C.c_double), ("CPdry", C.c_double), ("CVdry", C.c_double),
+        ("PRES00", C.c_double),
+    ]
+
+
+class RkTabC(C.Structure):
+    _fields_ = [("nstage", C.c_int), ("tend_buf_size", C.c_int), ("imex", C.c_int), ("dt", C.c_double),
+                ("a_ex", C.c_double * 16), ("b_ex", C.c_double * 4), ("a_im", C.c_double * 16),
+                ("b_im", C.c_double * 4), ("tend_buf_indmap", C.c_int * 4)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"oracle not built: run `make -C {HERE}`")
+    lib = C.CDLL(LIB_PATH)
+    M = C.POINTER(OracleMeshC)
+    lib.oracle_drhot2pres.argtypes = [M] + [dp] * 8
+    lib.oracle_halo_exchange.argtypes = [M, dp]
+    lib.oracle_apply_bc.argtypes = [M] + [dp] * 5
+    lib.oracle_heve_numflux.argtypes = [M] + [dp] * 10
+    lib.oracle_heve_cal_tend.argtypes = [M] + [dp] * 21
+    lib.oracle_heve_cal_tend_range.argtypes = [M, C.c_int, C.c_int] + [dp] * 21
+    lib.oracle_calc_phyd_hgrad.argtypes = [M] + [dp] * 4
+    lib.oracle_rk_tab_get.argtypes = [C.c_char_p, C.c_double, C.POINTER(RkTabC)]
+    lib.oracle_rk_advance.argtypes = [C.POINTER(RkTabC), C.c_int, C.c_long] + [dp] * 5 + [C.c_long]
+    lib.oracle_rk_store_impl.argtypes = [C.POINTER(RkTabC), C.c_int, C.c_long] + [dp] * 4 + [C.c_long]
+    lib.oracle_heve_update.argtypes = [M, C.POINTER(RkTabC), C.c_int, dp * 5] + [dp] * 2 + [dp] * 10 + [dp]
+    lib.oracle_num_threads.restype = C.c_int
+    lib.oracle_set_pow_mode.argtypes = [C.c_int]
+    lib.oracle_get_pow_mode.restype = C.c_int
+    lib.oracle_pow_array.argtypes = [dp, dp, dp, C.c_long, C.c_int]
+    return lib
+
+
+LIB = _load()
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(dp)
+
+
+def _i(a):
+    return None if a is None else a.ctypes.data_as(ip)
+
+
+class OracleMesh:
+    """Oracle view of a fedg BoxMesh (same host arrays)."""
+
+    def __init__(self, mesh, velbc=None, consts=None, Gsqrt=None, GI3=None):
+        from fedg import constants  # the oracle takes the constants as inputs too
+        c = consts or constants.SCALE
+        e = mesh.elem
+        self.mesh = mesh
+        self.velbc = mesh.velbc() if velbc is None else np.asarray(velbc, dtype=np.int32)
+        self.Gsqrt = mesh.Gsqrt if Gsqrt is None else Gsqrt
+        self.GI3 = mesh.GI3 if GI3 is None else GI3
+        n2d = mesh.Nfp_v * mesh.Ne2D
+        self.G11, self.G12, self.G22 = np.ones(n2d), np.zeros(n2d), np.ones(n2d)
+        s = OracleMeshC()
+        s.n1, s.Np, s.Nfp_h, s.Nfp_v, s.NfpTot = e.n1, e.Np, e.Nfp, e.Nfp, e.NfpTot
+        s.Ne, s.NeA, s.Ne2D, s.nB = mesh.Ne, mesh.NeA, mesh.Ne2D, mesh.nB
+        s.Escale, s.Gsqrt, s.GsqrtH, s.GI3 = _d(mesh.Escale), _d(self.Gsqrt), _d(mesh.GsqrtH), _d(self.GI3)
+        s.G11, s.G12, s.G22 = _d(self.G11), _d(self.G12), _d(self.G22)
+        s.normal_fn, s.Fscale = _d(mesh.normal_fn), _d(mesh.Fscale)
+        s.vmapM, s.vmapP, s.vmapB, s.halo_src = _i(mesh.vmapM), _i(mesh.vmapP), _i(mesh.vmapB), _i(mesh.halo_src)
+        s.velbc, s.EMap3Dto2D = _i(self.velbc), _i(mesh.EMap3Dto2D)
+        s.IndexH2Dto3D, s.IndexH2Dto3D_bnd = _i(e.IndexH2Dto3D), _i(e.IndexH2Dto3D_bnd)
+        s.D1D, s.Lift_mat, s.IntrpMat_tr = _d(e.D1D), _d(e.Lift_mat), _d(e.IntrpMat_VPOrdM1_tr)
+        s.GRAV, s.Rdry, s.CPdry, s.CVdry, s.PRES00 = c.GRAV, c.Rdry, c.CPdry, c.CVdry, c.PRES00
+        self.s = s
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+    # --- wrappers -----------------------------------------------------------
+    def drhot2pres(self, DRHOT, bs):
+        m = self.mesh
+        P, D = m.field(), m.field()
+        LIB.oracle_drhot2pres(self.ref, _d(P), _d(D), _d(DRHOT), _d(bs.PRES_hyd), _d(bs.THERM_hyd), _d(bs.Rtot),
+                              _d(bs.CVtot), _d(bs.CPtot))
+        return P, D
+
+    def exchange(self, f):
+        LIB.oracle_halo_exchange(self.ref, _d(f))
+
+    def apply_bc(self, q):
+        LIB.oracle_apply_bc(self.ref, _d(q[0]), _d(q[3]), _d(q[4]), _d(q[2]), _d(q[1]))
+
+    def numflux(self, q, DPRES, bs):
+        m = self.mesh
+        out = np.zeros(m.NfpTot * 5 * m.Ne)
+        LIB.oracle_heve_numflux(self.ref, _d(out), _d(q[0]), _d(q[3]), _d(q[4]), _d(q[2]), _d(q[1]), _d(DPRES),
+                                _d(bs.DENS_hyd), _d(bs.PRES_hyd), _d(bs.THERM_hyd))
+        return out
+
+    def cal_tend(self, q, DPRES, bs, DPhydDx, DPhydDy, ke_range=None):
+        """Tendencies (DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt), each (Np,NeA)."""
+        m = self.mesh
+        out = [m.field() for _ in range(5)]
+        args = [_d(a) for a in out] + [_d(q[0]), _d(q[3]), _d(q[4]), _d(q[2]), _d(q[1]), _d(DPRES),
+                                       _d(bs.DENS_hyd), _d(bs.PRES_hyd), _d(bs.PRES_hyd_ref), _d(bs.THERM_hyd),
+                                       _d(bs.CORIOLIS), _d(bs.Rtot), _d(bs.CVtot), _d(bs.CPtot), _d(DPhydDx),
+                                       _d(DPhydDy)]
+        if ke_range is None:
+            LIB.oracle_heve_cal_tend(self.ref, *args)
+        else:
+            LIB.oracle_heve_cal_tend_range(self.ref, int(ke_range[0]), int(ke_range[1]), *args)
+        return out
+
+    def calc_phyd_hgrad(self, bs):
+        m = self.mesh
+        Dx, Dy = m.field(), m.field()
+        LIB.oracle_calc_phyd_hgrad(self.ref, _d(Dx), _d(Dy), _d(bs.PRES_hyd), _d(bs.PRES_hyd_ref))
+        return Dx, Dy
+
+    def heve_update(self, q, bs, DPhydDx, DPhydDy, dt, nsteps, scheme="ERK_4s4o"):
+        """Advance q (list of 5 (Np,NeA) host fields, VID order) in place; returns (PRES, DPRES)."""
+        m = self.mesh
+        tab = rk_tab(scheme, dt)
+        nA = m.Np * m.NeA
+        work = np.zeros(nA * (5 * (2 + tab.tend_buf_size) + 2))
+        P, D = m.field(), m.field()
+        qa = (dp * 5)(*[_d(a) for a in q])
+        LIB.oracle_heve_update(self.ref, C.byref(tab), int(nsteps), qa, _d(P), _d(D), _d(bs.DENS_hyd),
+                               _d(bs.PRES_hyd), _d(bs.PRES_hyd_ref), _d(bs.THERM_hyd), _d(bs.CORIOLIS),
+                               _d(bs.Rtot), _d(bs.CVtot), _d(bs.CPtot), _d(DPhydDx), _d(DPhydDy), _d(work))
+        return P, D
+
+
+def rk_tab(name, dt):
+    t = RkTabC()
+    if LIB.oracle_rk_tab_get(name.encode(), float(dt), C.byref(t)) != 0:
+        raise ValueError(f"unknown RK scheme {name}")
+    return t
+
+
+def rk_advance(tab, stage, q, var0, varTmp, tend_ex, tend_im=None, bs=None):
+    n = q.size
+    LIB.oracle_rk_advance(C.byref(tab), stage, n, _d(q), _d(var0), _d(varTmp), _d(tend_ex), _d(tend_im),
+                          n if bs is None else bs)
+
+
+def rk_store_impl(tab, stage, q, var0, varTmp, tend_im, bs=None):
+    n = q.size
+    LIB.oracle_rk_store_impl(C.byref(tab), stage, n, _d(q), _d(var0), _d(varTmp), _d(tend_im),
+                             n if bs is None else bs)
+
+
+def num_threads():
+    return LIB.oracle_num_threads()
+
+
+POW_LIBM, POW_PORTABLE = 0, 1
+
+
+def set_pow_mode(mode: int):
+    """0: the C library's pow (reference build semantics); 1: the portable pow the kernels use."""
+    LIB.oracle_set_pow_mode(int(mode))
+
+
+def get_pow_mode() -> int:
+    return LIB.oracle_get_pow_mode()
+
+
+class pow_mode:
+    """Context manager selecting the oracle's pow."""
+
+    def __init__(self, mode):
+        self.mode = mode
+
+    def __enter__(self):
+        self.prev = get_pow_mode()
+        set_pow_mode(self.mode)
+
+    def __exit__(self, *a):
+        set_pow_mode(self.prev)
+
+
+def pow_array(x, y, mode):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(np.broadcast_to(y, x.shape), dtype=np.float64)
+    out = np.empty_like(x)
+    LIB.oracle_pow_array(_d(x), _d(y), _d(out), x.size, int(mode))
+    return out

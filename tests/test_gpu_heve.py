@@ -1,0 +1,225 @@
+"""GPU parity tests of the HEVE path against the CPU oracle, through the C ABI.
+
+Bar: the kernels evaluate the reference's expressions in the reference's order
+without fused multiply-add, with IEEE division and sqrt, and with the portable
+double-double pow that the oracle's parity mode also uses, so every entry
+point -- and every full RK step -- must agree with the oracle bit for bit.
+Against the oracle's reference mode (the C library's pow) the pressure agrees
+to one unit in the last place; the steps are then held to relative L2 <= 1e-12
+on the fields whose magnitude is not a rounding-level difference of large
+numbers (see test_libm_pow_floor).
+"""
+import numpy as np
+import pytest
+
+import fedg
+from fedg import cases
+import fedg_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+STEP_TOL = 1e-12
+
+
+@pytest.fixture(autouse=True)
+def portable_pow():
+    with O.pow_mode(O.POW_PORTABLE):
+        yield
+
+
+def perturbed_state(mesh, bs, case, om, Dx, Dy, nsteps=3, seed=7):
+    """A state with every prognostic nonzero: a few oracle steps of the bubble
+    case plus seeded noise in the momentum."""
+    q = cases.warm_bubble_state(mesh, bs, case.bubbles)
+    om.heve_update(q, bs, Dx, Dy, case.dt, nsteps)
+    rng = np.random.default_rng(seed)
+    n = mesh.Np * mesh.Ne
+    for v, amp in ((2, 0.05), (3, 0.3), (4, 0.2)):
+        q[v][:n] += amp * rng.standard_normal(n)
+    for f in q:
+        om.exchange(f)
+    _, D = om.drhot2pres(q[1], bs)
+    om.exchange(D)
+    om.apply_bc(q)
+    return q, D
+
+
+def small_case(p, NeX=4, NeY=3, NeZ=4):
+    base = cases.config1() if p == 3 else cases.config2()
+    return cases.scaled_case(base, NeX, NeY, NeZ, porder=p)
+
+
+@pytest.fixture(scope="module", params=[3, 7], ids=["p3", "p7"])
+def setup(request):
+    p = request.param
+    case = cases.config1() if p == 3 else small_case(7)
+    mesh, bs, q0 = cases.build(case)
+    om = O.OracleMesh(mesh)
+    Dx, Dy = om.calc_phyd_hgrad(bs)
+    dm = fedg.DeviceMesh(mesh)
+    return dict(case=case, mesh=mesh, bs=bs, q0=q0, om=om, Dx=Dx, Dy=Dy, dm=dm)
+
+
+def _dev(mesh, a):
+    return fedg.DevField(a.size, a)
+
+
+def test_phyd_hgrad_bitwise(setup):
+    s = setup
+    gx, gy = fedg.calc_phyd_hgrad(s["dm"], s["bs"].PRES_hyd, s["bs"].PRES_hyd_ref)
+    n = s["mesh"].Np * s["mesh"].Ne
+    assert np.array_equal(gx[:n], s["Dx"][:n])
+    assert np.array_equal(gy[:n], s["Dy"][:n])
+
+
+def test_drhot2pres(setup):
+    s = setup
+    mesh, bs, om = s["mesh"], s["bs"], s["om"]
+    q, _ = perturbed_state(mesh, bs, s["case"], om, s["Dx"], s["Dy"])
+    P_o, D_o = om.drhot2pres(q[1], bs)
+    f = lambda a: _dev(mesh, a)
+    P, D = f(mesh.field()), f(mesh.field())
+    fedg.calc_pressure(s["dm"], P, D, f(q[1]), f(bs.PRES_hyd), f(bs.THERM_hyd), f(bs.Rtot), f(bs.CVtot),
+                       f(bs.CPtot))
+    n = mesh.Np * mesh.Ne
+    Pg, Dg = P.download()[:n], D.download()[:n]
+    assert np.array_equal(Pg, P_o[:n]) and np.array_equal(Dg, D_o[:n])
+    # against the C library's pow: at most one unit in the last place of PRES
+    with O.pow_mode(O.POW_LIBM):
+        P_l, D_l = om.drhot2pres(q[1], bs)
+    ulp = np.maximum(np.spacing(np.abs(P_l[:n])), np.spacing(np.abs(Pg)))
+    ndiff = int(np.count_nonzero(Pg != P_l[:n]))
+    print(f"PRES differs from the C library's pow at {ndiff} of {n} nodes")
+    assert np.max(np.abs(Pg - P_l[:n]) / ulp) <= 1.0
+
+
+def test_halo_and_bc_bitwise(setup):
+    s = setup
+    mesh, bs, om = s["mesh"], s["bs"], s["om"]
+    rng = np.random.default_rng(3)
+    q = [mesh.field() for _ in range(5)]
+    n = mesh.Np * mesh.Ne
+    for f in q:
+        f[:n] = rng.standard_normal(n)
+    dq = [_dev(mesh, f) for f in q]
+    fedg.halo_exchange(s["dm"], dq)
+    fedg.apply_bc(s["dm"], dq[0], dq[3], dq[4], dq[2], dq[1], _dev(mesh, bs.DENS_hyd), _dev(mesh, bs.PRES_hyd))
+    for f in q:
+        om.exchange(f)
+    om.apply_bc(q)
+    for a, b in zip(dq, q):
+        assert np.array_equal(a.download(), b)
+
+
+def test_dropin_cal_tend_bitwise(setup):
+    s = setup
+    mesh, bs, om = s["mesh"], s["bs"], s["om"]
+    q, DPRES = perturbed_state(mesh, bs, s["case"], om, s["Dx"], s["Dy"])
+    ref = om.cal_tend(q, DPRES, bs, s["Dx"], s["Dy"])
+    f = lambda a: _dev(mesh, a)
+    out = [f(mesh.field()) for _ in range(5)]
+    fedg.heve_cal_tend(s["dm"], out, f(q[0]), f(q[3]), f(q[4]), f(q[2]), f(q[1]), f(DPRES), f(bs.DENS_hyd),
+                       f(bs.PRES_hyd), f(bs.PRES_hyd_ref), f(bs.THERM_hyd), f(bs.CORIOLIS), f(bs.Rtot),
+                       f(bs.CVtot), f(bs.CPtot), f(s["Dx"]), f(s["Dy"]))
+    n = mesh.Np * mesh.Ne
+    for name, a, b in zip(("DENS_dt", "MOMX_dt", "MOMY_dt", "MOMZ_dt", "RHOT_dt"), out, ref):
+        g = a.download()[:n]
+        assert np.array_equal(g, b[:n]), f"{name}: max abs diff {np.max(np.abs(g - b[:n]))}"
+
+
+@pytest.mark.parametrize("scheme", ["ERK_4s4o", "IMEX_ARK324", "IMEX_ARK232"])
+def test_tint_advance_bitwise(setup, scheme):
+    s = setup
+    mesh, dm = s["mesh"], s["dm"]
+    dt = 0.37
+    t = fedg.TimeIntRK(dm, scheme, dt)
+    tab = O.rk_tab(scheme, dt)
+    nA, n = mesh.Np * mesh.NeA, mesh.Np * mesh.Ne
+    rng = np.random.default_rng(11)
+    q = rng.standard_normal(nA)
+    qd = _dev(mesh, q)
+    # oracle buffers for variable id 2 (DRHOT)
+    var0, varTmp = np.zeros(nA), np.zeros(nA)
+    tex = np.zeros(nA * tab.tend_buf_size)
+    tim = np.zeros(nA * tab.tend_buf_size)
+    import ctypes as C
+    h2d = lambda dst, a: fedg.check(fedg.LIB.fedg_memcpy_h2d(dst, a.ctypes.data_as(C.c_void_p), a.size * 8))
+    for st in range(1, t.nstage + 1):
+        Te = rng.standard_normal(nA)
+        buf = tab.tend_buf_indmap[st - 1]
+        tex[buf * nA:(buf + 1) * nA] = Te
+        h2d(t.tend_ex(st, 2), Te)
+        if t.imex:
+            Ti = rng.standard_normal(nA)
+            tim[buf * nA:(buf + 1) * nA] = Ti
+            h2d(t.tend_im(st, 2), Ti)
+            t.store_implicit(st, qd, 2)
+            O.rk_store_impl(tab, st, q[:n], var0[:n], varTmp[:n], tim, bs=nA)
+        t.advance(st, [qd], [2])
+        O.rk_advance(tab, st, q[:n], var0[:n], varTmp[:n], tex, tim if t.imex else None, bs=nA)
+        assert np.array_equal(qd.download()[:n], q[:n]), f"stage {st}"
+
+
+@pytest.mark.parametrize("box_shape", [True, False], ids=["flat-kernel", "general-kernel"])
+def test_fused_steps_match_oracle(setup, box_shape, rel):
+    s = setup
+    mesh, bs, om, case = s["mesh"], s["bs"], s["om"], s["case"]
+    dm = fedg.DeviceMesh(mesh, box_shape=box_shape)
+    st = fedg.HeveStepper(dm, bs, s["Dx"], s["Dy"], case.dt)
+    assert st.flat == box_shape
+    nsteps = 20
+    q = [a.copy() for a in s["q0"]]
+    st.set_state(q)
+    om.heve_update(q, bs, s["Dx"], s["Dy"], case.dt, nsteps)
+    st.run(nsteps)
+    g = st.get_state()
+    n = mesh.Np * mesh.Ne
+    for name, a, b in zip(fedg.VARS, g, q):
+        assert np.array_equal(a, b[:n]), f"{name}: rel L2 {rel(a, b[:n]):.3e}"
+    P, D = st.get_pressure()
+    Po, Do = om.drhot2pres(q[1], bs)
+    assert np.array_equal(D, Do[:n])
+
+
+def test_dropin_driver_matches_fused(setup, rel):
+    """The reference's stage sequence driven through the drop-in entry points
+    (exchange, pressure, exchange, BC, cal_tend, RK advance) gives the fused
+    stepper's result."""
+    s = setup
+    mesh, bs, case, dm = s["mesh"], s["bs"], s["case"], s["dm"]
+    f = lambda a: _dev(mesh, a)
+    q = [f(a) for a in s["q0"]]
+    P, D = f(mesh.field()), f(mesh.field())
+    Dh, Ph, Th, Pr, Co = f(bs.DENS_hyd), f(bs.PRES_hyd), f(bs.THERM_hyd), f(bs.PRES_hyd_ref), f(bs.CORIOLIS)
+    R, Cv, Cp, Dx, Dy = f(bs.Rtot), f(bs.CVtot), f(bs.CPtot), f(s["Dx"]), f(s["Dy"])
+    t = fedg.TimeIntRK(dm, case.scheme, case.dt)
+    ids = [1, 2, 3, 4, 5]
+    nsteps = 5
+    for _ in range(nsteps):
+        for stg in range(1, t.nstage + 1):
+            fedg.halo_exchange(dm, q)
+            fedg.calc_pressure(dm, P, D, q[1], Ph, Th, R, Cv, Cp)
+            fedg.halo_exchange(dm, [D])
+            fedg.apply_bc(dm, q[0], q[3], q[4], q[2], q[1], Dh, Ph)
+            out = [t.tend_ex(stg, 1), t.tend_ex(stg, 4), t.tend_ex(stg, 5), t.tend_ex(stg, 3), t.tend_ex(stg, 2)]
+            fedg.heve_cal_tend(dm, out, q[0], q[3], q[4], q[2], q[1], D, Dh, Ph, Pr, Th, Co, R, Cv, Cp, Dx, Dy)
+            t.advance(stg, q, ids)
+    st = fedg.HeveStepper(dm, bs, s["Dx"], s["Dy"], case.dt)
+    st.set_state(s["q0"])
+    st.run(nsteps)
+    g = st.get_state()
+    n = mesh.Np * mesh.Ne
+    for name, a, b in zip(fedg.VARS, g, q):
+        assert np.array_equal(a, b.download()[:n]), name
+
+
+def test_hydrostatic_rest_stays_at_rest(setup):
+    s = setup
+    mesh, bs, case, dm = s["mesh"], s["bs"], s["case"], s["dm"]
+    st = fedg.HeveStepper(dm, bs, s["Dx"], s["Dy"], case.dt)
+    n = mesh.Np * mesh.Ne
+    st.set_state([np.zeros(n)] * 5)
+    st.run(10)
+    g = st.get_state()
+    # roundoff of the discrete hydrostatic balance only
+    assert max(np.max(np.abs(a)) for a in g) < 1e-9
