@@ -1,0 +1,123 @@
+#!/usr/bin/env python3
+"""Benchmark: FElib regional nonhydrostatic DG HEVE step on one MI355X.
+
+Workload: 160 km x 160 km x 10 km box, 64x64x16 hexahedra, p=7 (512 LGL
+nodes per element, 33.5M nodes, 167.8M DOF), dry isothermal base state,
+three 2 K warm bubbles, classic RK4 (ERK_4s4o), dt = 0.05 s, plain FP64.
+A step is one RK4 step = 4 fused stage kernels over all elements, state
+resident in HBM.  value = DOF*stage/s = 5 Np Ne * 4 * K / wall time of K steps.
+
+roofline: the fused stage kernel against 8 TB/s, with the algorithmic
+traffic of 36.8 B per DOF*stage (q^n, stage state and RK accumulator reads and
+writes plus three base-state fields per stage); its duration is the mean of
+HIP events recorded around every stage launch on the stepper's stream during
+the timed steps.  traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE /
+WRITE_SIZE (profiles/heve_stage_flat_p7_64x64x16.traffic.txt) when present.
+
+cpu_baseline: the CPU oracle (C restatement of the reference Fortran, OpenMP,
+FP64, the C library's pow) timed on this host over one RK4 step of the same
+mesh.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fe-project_amd"))
+
+BYTES_PER_DOF_STAGE = 36.8
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "heve_stage_flat_p7_64x64x16.traffic.txt")
+
+
+def read_traffic():
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("hbm_bytes_per_launch"):
+                    return float(line.split("=")[1])
+    except OSError:
+        return None
+    return None
+
+
+def cpu_baseline(mesh, bs, q, dt, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    import fedg_oracle as O
+    O.set_pow_mode(O.POW_LIBM)
+    om = O.OracleMesh(mesh)
+    Dx, Dy = om.calc_phyd_hgrad(bs)
+    qq = [a.copy() for a in q]
+    t0 = time.perf_counter()
+    om.heve_update(qq, bs, Dx, Dy, dt, 1)
+    sec = time.perf_counter() - t0
+    dof_stage = 5 * mesh.Np * mesh.Ne * 4
+    return {"value": dof_stage / sec, "unit": "DOF·stage/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"1 RK4 step (4 stages) of the same 64x64x16 p=7 mesh, {dof_stage:.3e} DOF·stage, "
+                      f"{sec:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only; the multi-GPU sharding is a design "
+              "(horizontal tiles with per-stage face exchange), not an implemented path.", file=sys.stderr)
+        sys.exit(2)
+
+    import fedg
+    from fedg import cases
+
+    case = cases.config2()
+    mesh, bs, q = cases.build(case)
+    dm = fedg.DeviceMesh(mesh)
+    Dx, Dy = fedg.calc_phyd_hgrad(dm, bs.PRES_hyd, bs.PRES_hyd_ref)
+    st = fedg.HeveStepper(dm, bs, Dx, Dy, case.dt, scheme=case.scheme)
+    st.set_state(q)
+    st.run(args.warmup)
+    fedg.check(fedg.LIB.fedg_device_sync(), "sync")
+    t0 = time.perf_counter()
+    ms_events, ms_kernel = st.run_timed(args.steps)
+    fedg.check(fedg.LIB.fedg_device_sync(), "sync")
+    wall = time.perf_counter() - t0
+
+    dof = 5 * mesh.Np * mesh.Ne
+    nstage = 4
+    value = dof * nstage * args.steps / wall
+    achieved = BYTES_PER_DOF_STAGE * dof / (ms_kernel * 1e-3) / 1e9
+    out = {
+        "metric": "DOF·stage/s at p=7 (plain FP64 vs FP32+learned-HEVI, equal err); %HBM peak",
+        "value": value,
+        "unit": "DOF·stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "plain FP64 HEVE warm bubbles, 64x64x16 elements, p=7, RK4 dt=0.05 s",
+                   "elements": mesh.Ne, "nodes": mesh.Np * mesh.Ne, "dof": dof, "kernel": "fused stage, "
+                   + ("flat-terrain" if st.flat else "general-metric")},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic()},
+    }
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", str(min(os.cpu_count() or 1, 16))))
+        out["cpu_baseline"] = cpu_baseline(mesh, bs, q, case.dt, threads)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

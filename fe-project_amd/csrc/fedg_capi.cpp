@@ -77,6 +77,7 @@ struct fedg_heve {
   int flat;
   double* block;            // one allocation for every device array below
   double* buf[3][5];        // stage states (q^n and two intermediates)
+  double* DP[3];            // pressure perturbation of each stage state
   double* VT[5];            // RK accumulator
   int q0;                   // index of q^n in buf
   double *DENS_hyd, *PRES_hyd, *THERM_hyd, *DPhydDx, *DPhydDy, *CORIOLIS;
@@ -734,7 +735,7 @@ int fedg_heve_create(fedg_ctx* ctx, const char* scheme, double dt, const double*
     dry = all_equal(Rtot, NpNe, ctx->c.Rdry) && all_equal(CVtot, NpNe, ctx->c.CVdry) &&
           all_equal(CPtot, NpNe, ctx->c.CPdry);
   st->flat = m.flat && dry;
-  const size_t nfield = 15 + 5 + 5 + (dry ? 0 : 3) + 2;
+  const size_t nfield = 15 + 3 + 5 + 5 + (dry ? 0 : 3) + 2;
   const size_t total = NpNe * nfield + cor;
   if (rc == FEDG_OK) rc = dev_upload<double>(&st->block, nullptr, total);
   if (rc != FEDG_OK) {
@@ -746,6 +747,7 @@ int fedg_heve_create(fedg_ctx* ctx, const char* scheme, double dt, const double*
   for (int b = 0; b < 3; ++b)
     for (int v = 0; v < 5; ++v, p += NpNe) st->buf[b][v] = p;
   for (int v = 0; v < 5; ++v, p += NpNe) st->VT[v] = p;
+  for (int b = 0; b < 3; ++b, p += NpNe) st->DP[b] = p;
   st->DENS_hyd = p; p += NpNe;
   st->PRES_hyd = p; p += NpNe;
   st->THERM_hyd = p; p += NpNe;
@@ -790,8 +792,14 @@ int fedg_heve_is_flat(const fedg_heve* st) { return st ? st->flat : 0; }
 
 int fedg_heve_set_state(fedg_heve* st, const double* const* q) {
   if (!st || !q) return FEDG_ERR_ARG;
-  const size_t NpNe = (size_t)st->ctx->m.Np * st->ctx->m.Ne;
+  fedg_ctx* ctx = st->ctx;
+  const size_t NpNe = (size_t)ctx->m.Np * ctx->m.Ne;
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   for (int v = 0; v < 5; ++v) FEDG_HIP_CHECK(hipMemcpy(st->buf[st->q0][v], q[v], sizeof(double) * NpNe, hipMemcpyHostToDevice));
+  // the stage kernels read the pressure perturbation of their input state
+  TRY(launch_dpres(ctx, st->DP[st->q0], st->buf[st->q0][1], st->PRES_hyd, st->THERM_hyd, st->Rtot, st->CVtot,
+                   st->CPtot, (long)NpNe));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   return FEDG_OK;
 }
 
@@ -843,6 +851,8 @@ static int heve_step(fedg_heve* st, std::vector<hipEvent_t>* evs, size_t* ev_i) 
       rk.var0[v] = st->buf[q0][v];
       rk.varTmp[v] = st->VT[v];
     }
+    in.DPRES = st->DP[cur];
+    rk.DPout = st->DP[outb];
     rk.cb = t.b_ex[s - 1] * t.dt;
     rk.ca = (s < t.nstage) ? t.dt * t.a_ex[s][s - 1] : 0.0;
     if (evs) FEDG_HIP_CHECK(hipEventRecord((*evs)[(*ev_i)++], ctx->stream));

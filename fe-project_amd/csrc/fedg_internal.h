@@ -71,7 +71,7 @@ namespace fedg {
 // ---- kernel launchers (fedg_kernels.hip) ----
 struct TendIn {
   const double* q[5];        // stage state, VID order (DDENS, DRHOT, MOMZ, MOMX, MOMY)
-  const double* DPRES;       // drop-in only
+  const double* DPRES;       // pressure perturbation of the stage state
   const double* DENS_hyd;
   const double* PRES_hyd;
   const double* THERM_hyd;
@@ -87,12 +87,16 @@ struct RkOut {
   double* out[5];            // drop-in: tendencies; fused: next stage state
   const double* var0[5];     // fused: q^n
   double* varTmp[5];         // fused: RK accumulator
+  double* DPout;             // fused: pressure perturbation of the next stage state
   double ca, cb;             // dt*a(s+1,s), dt*b(s)
   int mode;                  // 0: first stage, 1: middle, 2: last, 3: single-stage scheme
 };
 
 int launch_heve_dropin(fedg_ctx* ctx, const TendIn& in, double* const out[5]);
 int launch_heve_fused(fedg_ctx* ctx, const TendIn& in, const RkOut& rk, int use_flat);
+// DPRES of a state (dry constants when Rtot is null), as DRHOT2PRES computes it
+int launch_dpres(fedg_ctx* ctx, double* DPRES, const double* DRHOT, const double* PRES_hyd,
+                 const double* THERM_hyd, const double* Rtot, const double* CVtot, const double* CPtot, long n);
 int launch_drhot2pres(fedg_ctx* ctx, double* PRES, double* DPRES, const double* DRHOT,
                       const double* PRES_hyd, const double* THERM_hyd, const double* Rtot,
                       const double* CVtot, const double* CPtot, long n);

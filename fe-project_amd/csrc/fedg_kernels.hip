@@ -136,20 +136,37 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
   return (b & 7) * (nb >> 3) + (b >> 3);
 }
 
+// Loads and stores with a 32-bit byte offset from a base pointer held in
+// SGPRs (fields are below 4 GB; fedg_ctx_create checks it), so that the
+// compiler uses the saddr + 32-bit vgpr offset form and keeps no 64-bit
+// addresses in VGPRs.
+__device__ __forceinline__ double ldg(const double* base, uint32_t idx) {
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + (idx << 3));
+}
+__device__ __forceinline__ void stg(double* base, uint32_t idx, double v) {
+  *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + (idx << 3)) = v;
+}
+
+// Own-node values staged in LDS for the face phase: DDENS, DRHOT, MOMZ, MOMX,
+// MOMY (VID order), then DENS_hyd, PRES_hyd, THERM_hyd, DPRES.
+enum { S_DH = 5, S_PH = 6, S_TH = 7, S_DP = 8, S_N = 9 };
+
 template <int N1, int EPB, int MODE>
-__global__ void __launch_bounds__(EPB * N1 * N1 * N1)
+__global__ void __launch_bounds__(EPB * N1 * N1 * N1, 2)
 heve_kernel(const DevMesh m, const Consts c, const TendIn in, const RkOut rk) {
   constexpr int Np = N1 * N1 * N1, Nfp = N1 * N1, NfpTot = 6 * Nfp;
   constexpr bool DROPIN = MODE == MODE_DROPIN;
   constexpr bool FLAT = MODE == MODE_FUSED_FLAT;
   constexpr bool FUSED = !DROPIN;
+  // exterior values are loaded before the first barrier when every face node
+  // has its own thread (p >= 5); otherwise inside the face loop
+  constexpr bool PRELOAD = NfpTot <= Np;
 
-  __shared__ double sD[N1 * N1];
+  __shared__ double sDt[N1 * N1];
   __shared__ double sI[N1 * N1];
+  __shared__ double sS[EPB][S_N][Np];
   __shared__ double sF[2][EPB][3][Np];
   __shared__ double sDF[EPB][5][NfpTot];
-  __shared__ double sRho[EPB][Np];
-  __shared__ double sDP[EPB][FUSED ? Np : 1];
 
   const int tid = threadIdx.x;
   const int e = tid / Np;
@@ -162,55 +179,126 @@ heve_kernel(const DevMesh m, const Consts c, const TendIn in, const RkOut rk) {
   const double gamm = c.CPdry / c.CVdry;
 
   for (int t = tid; t < N1 * N1; t += blockDim.x) {
-    sD[t] = m.D1D[t];
+    sDt[(t % N1) * N1 + t / N1] = m.D1D[t];
     sI[t] = m.Itr[t];
   }
 
-  const size_t o = (size_t)Np * (valid ? ke : 0) + p;
+  // ---- exterior side of this thread's face node: index and boundary kind ----
+  const int fp0 = p;
+  const int f0 = fp0 / Nfp, a0 = fp0 - f0 * Nfp;
+  const bool has_face = valid && fp0 < NfpTot;
+  uint32_t iP0 = 0;
+  int bc0 = FEDG_BND_NONE;
+  const int pm0 = face_node<N1>(f0 < 6 ? f0 : 0, a0);
+  auto exterior = [&](int f, int a, int pm, uint32_t& iP, int& bc) {
+    const uint32_t gf = (uint32_t)(f * Nfp + a) + (uint32_t)NfpTot * ke;
+    if constexpr (DROPIN) {
+      iP = (uint32_t)m.vmapP[gf];
+      bc = FEDG_BND_NONE;
+    } else if constexpr (FLAT) {
+      const int ei = ke % m.NeX, ej = (ke / m.NeX) % m.NeY, ek = ke / (m.NeX * m.NeY);
+      int ni = ei, nj = ej, nk = ek;
+      bool closed = false;
+      switch (f) {
+        case 0: nj -= 1; if (nj < 0) { if (m.periodic[1]) nj = m.NeY - 1; else closed = true; } break;
+        case 1: ni += 1; if (ni >= m.NeX) { if (m.periodic[0]) ni = 0; else closed = true; } break;
+        case 2: nj += 1; if (nj >= m.NeY) { if (m.periodic[1]) nj = 0; else closed = true; } break;
+        case 3: ni -= 1; if (ni < 0) { if (m.periodic[0]) ni = m.NeX - 1; else closed = true; } break;
+        case 4: nk -= 1; if (nk < 0) { if (m.periodic[2]) nk = m.NeZ - 1; else closed = true; } break;
+        default: nk += 1; if (nk >= m.NeZ) { if (m.periodic[2]) nk = 0; else closed = true; } break;
+      }
+      if (closed) {
+        iP = (uint32_t)Np * ke + pm;
+        bc = m.bnd_bc[f];
+      } else {
+        constexpr int opp[6] = {2, 3, 0, 1, 5, 4};
+        iP = (uint32_t)Np * (ni + m.NeX * (nj + m.NeY * nk)) + face_node<N1>(opp[f], a);
+        bc = FEDG_BND_NONE;
+      }
+    } else {
+      iP = (uint32_t)m.nbr[gf];
+      bc = m.nbr_bc[gf];
+    }
+  };
+  struct Ext {
+    double DD, DT, MZ, MX, MY, Dh, Ph, Th, DP, Gs, g13, g23;
+  };
+  auto load_ext = [&](uint32_t iP, Ext& x) {
+    x.DD = ldg(in.q[VD], iP); x.DT = ldg(in.q[VT], iP); x.MZ = ldg(in.q[VW], iP);
+    x.MX = ldg(in.q[VU], iP); x.MY = ldg(in.q[VV], iP);
+    x.Dh = ldg(in.DENS_hyd, iP); x.Ph = ldg(in.PRES_hyd, iP); x.Th = ldg(in.THERM_hyd, iP);
+    x.DP = ldg(in.DPRES, iP);
+    if constexpr (!FLAT) {
+      x.Gs = ldg(m.Gsqrt, iP); x.g13 = ldg(m.G13, iP); x.g23 = ldg(m.G23, iP);
+    } else {
+      x.Gs = 1.0; x.g13 = 0.0; x.g23 = 0.0;
+    }
+  };
+  Ext X0;
+  if constexpr (PRELOAD) {
+    if (has_face) {
+      exterior(f0, a0, pm0, iP0, bc0);
+      load_ext(iP0, X0);
+    }
+  }
+
+  // ---- own node: state, base state, pressure ----
+  const uint32_t o = (uint32_t)Np * (valid ? ke : 0) + p;
   double q[5] = {0, 0, 0, 0, 0};
-  double Dh = 0, Th = 0, DP = 0;
+  double Dh = 0, Ph = 0, Th = 0, DP = 0;
   double Gs = 1.0, g13 = 0.0, g23 = 0.0, GsH = 1.0;
   int ke2d = 0;
-  double Rt = c.Rdry, Cv = c.CVdry, Cp = c.CPdry;
+  double cor = 0.0, dpx = 0.0, dpy = 0.0;
   if (valid) {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) q[v] = in.q[v][o];
-    Dh = in.DENS_hyd[o];
-    Th = in.THERM_hyd[o];
+    for (int v = 0; v < 5; ++v) q[v] = ldg(in.q[v], o);
+    Dh = ldg(in.DENS_hyd, o);
+    Ph = ldg(in.PRES_hyd, o);
+    Th = ldg(in.THERM_hyd, o);
     if constexpr (FLAT) {
       ke2d = ke % (m.NeX * m.NeY);
     } else {
-      Gs = m.Gsqrt[o];
-      g13 = m.G13[o];
-      g23 = m.G23[o];
+      Gs = ldg(m.Gsqrt, o);
+      g13 = ldg(m.G13, o);
+      g23 = ldg(m.G23, o);
       ke2d = m.EMap3Dto2D[ke];
-      GsH = m.GsqrtH[(i + N1 * j) + (size_t)Nfp * ke2d];
+      GsH = ldg(m.GsqrtH, (uint32_t)(i + N1 * j) + (uint32_t)Nfp * ke2d);
     }
-    if constexpr (DROPIN) {
-      DP = in.DPRES[o];
-    } else {
-      if (in.Rtot != nullptr) {
-        Rt = in.Rtot[o];
-        Cv = in.CVtot[o];
-        Cp = in.CPtot[o];
-      }
-      DP = dpres_of(q[VT], Th, in.PRES_hyd[o], Rt, Cv, Cp, c);
-    }
+    cor = ldg(in.CORIOLIS, (uint32_t)(i + N1 * j) + (uint32_t)Nfp * ke2d);
+    dpx = ldg(in.DPhydDx, o);
+    dpy = ldg(in.DPhydDy, o);
+    DP = ldg(in.DPRES, o);
   }
-  sRho[e][p] = q[VD];
-  if constexpr (FUSED) sDP[e][p] = DP;
+  double L[6];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) L[f] = ldg(m.Lift, (uint32_t)(p + f * Np));
+#pragma unroll
+  for (int v = 0; v < 5; ++v) sS[e][v][p] = q[v];
+  sS[e][S_DH][p] = Dh;
+  sS[e][S_PH][p] = Ph;
+  sS[e][S_TH][p] = Th;
+  sS[e][S_DP][p] = DP;
   __syncthreads();
 
   // ---- face phase: Rusanov flux at the element's face nodes ----
   if (valid) {
-    const int ei = FLAT ? ke % m.NeX : 0;
-    const int ej = FLAT ? (ke / m.NeX) % m.NeY : 0;
-    const int ek = FLAT ? ke / (m.NeX * m.NeY) : 0;
     for (int fp = p; fp < NfpTot; fp += Np) {
       const int f = fp / Nfp, a = fp - f * Nfp;
-      const int pm = face_node<N1>(f, a);
-      const size_t gf = fp + (size_t)NfpTot * ke;
-      const size_t iM = DROPIN ? (size_t)m.vmapM[gf] : (size_t)Np * ke + pm;
+      const int pm = PRELOAD ? pm0 : face_node<N1>(f, a);
+      const uint32_t gf = (uint32_t)fp + (uint32_t)NfpTot * ke;
+      uint32_t iP;
+      int bc;
+      Ext X;
+      if constexpr (PRELOAD) {
+        iP = iP0;
+        bc = bc0;
+        X = X0;
+      } else {
+        exterior(f, a, pm, iP, bc);
+        load_ext(iP, X);
+      }
+      const uint32_t iM = DROPIN ? (uint32_t)m.vmapM[gf] : (uint32_t)Np * ke + pm;
+      const int pmM = DROPIN ? (int)(iM - (uint32_t)Np * ke) : pm;
       Side M, P;
       double nx, ny, nz, Fs;
       if constexpr (FLAT) {
@@ -219,70 +307,30 @@ heve_kernel(const DevMesh m, const Consts c, const TendIn in, const RkOut rk) {
         nz = (f == 5) ? 1.0 : (f == 4) ? -1.0 : 0.0;
         Fs = m.fF[f];
         M.Gsqrt = 1.0; M.G13 = 0.0; M.G23 = 0.0;
-        P.Gsqrt = 1.0; P.G13 = 0.0; P.G23 = 0.0;
       } else {
-        nx = m.nx[gf];
-        ny = m.ny[gf];
-        nz = m.nz[gf];
-        Fs = m.Fscale[gf];
-        M.Gsqrt = m.Gsqrt[iM]; M.G13 = m.G13[iM]; M.G23 = m.G23[iM];
+        nx = ldg(m.nx, gf);
+        ny = ldg(m.ny, gf);
+        nz = ldg(m.nz, gf);
+        Fs = ldg(m.Fscale, gf);
+        M.Gsqrt = ldg(m.Gsqrt, iM); M.G13 = ldg(m.G13, iM); M.G23 = ldg(m.G23, iM);
       }
-      M.DD = in.q[VD][iM]; M.DT = in.q[VT][iM]; M.MZ = in.q[VW][iM]; M.MX = in.q[VU][iM]; M.MY = in.q[VV][iM];
-      M.Dhyd = in.DENS_hyd[iM]; M.Phyd = in.PRES_hyd[iM]; M.Thyd = in.THERM_hyd[iM];
-      if constexpr (DROPIN) M.DP = in.DPRES[iM];
-      else M.DP = sDP[e][pm];
-
-      size_t iP;
-      int bc = FEDG_BND_NONE;
-      if constexpr (DROPIN) {
-        iP = (size_t)m.vmapP[gf];
-      } else if constexpr (FLAT) {
-        int ni = ei, nj = ej, nk = ek;
-        bool closed = false;
-        switch (f) {
-          case 0: nj -= 1; if (nj < 0) { if (m.periodic[1]) nj = m.NeY - 1; else closed = true; } break;
-          case 1: ni += 1; if (ni >= m.NeX) { if (m.periodic[0]) ni = 0; else closed = true; } break;
-          case 2: nj += 1; if (nj >= m.NeY) { if (m.periodic[1]) nj = 0; else closed = true; } break;
-          case 3: ni -= 1; if (ni < 0) { if (m.periodic[0]) ni = m.NeX - 1; else closed = true; } break;
-          case 4: nk -= 1; if (nk < 0) { if (m.periodic[2]) nk = m.NeZ - 1; else closed = true; } break;
-          default: nk += 1; if (nk >= m.NeZ) { if (m.periodic[2]) nk = 0; else closed = true; } break;
-        }
-        if (closed) {
-          iP = iM;
-          bc = m.bnd_bc[f];
-        } else {
-          constexpr int opp[6] = {2, 3, 0, 1, 5, 4};
-          iP = (size_t)Np * (ni + m.NeX * (nj + m.NeY * nk)) + face_node<N1>(opp[f], a);
-        }
-      } else {
-        iP = (size_t)m.nbr[gf];
-        bc = m.nbr_bc[gf];
-      }
-      if constexpr (!FLAT) {
-        P.Gsqrt = m.Gsqrt[iP]; P.G13 = m.G13[iP]; P.G23 = m.G23[iP];
-      }
-      P.DD = in.q[VD][iP]; P.DT = in.q[VT][iP]; P.MZ = in.q[VW][iP]; P.MX = in.q[VU][iP]; P.MY = in.q[VV][iP];
-      P.Dhyd = in.DENS_hyd[iP]; P.Phyd = in.PRES_hyd[iP]; P.Thyd = in.THERM_hyd[iP];
-      if constexpr (DROPIN) {
-        P.DP = in.DPRES[iP];
-      } else {
-        double Rp = c.Rdry, Cvp = c.CVdry, Cpp = c.CPdry;
-        if (in.Rtot != nullptr) {
-          Rp = in.Rtot[iP];
-          Cvp = in.CVtot[iP];
-          Cpp = in.CPtot[iP];
-        }
-        P.DP = dpres_of(P.DT, P.Thyd, P.Phyd, Rp, Cvp, Cpp, c);
-      }
+      M.DD = sS[e][VD][pmM]; M.DT = sS[e][VT][pmM]; M.MZ = sS[e][VW][pmM];
+      M.MX = sS[e][VU][pmM]; M.MY = sS[e][VV][pmM];
+      M.Dhyd = sS[e][S_DH][pmM]; M.Phyd = sS[e][S_PH][pmM]; M.Thyd = sS[e][S_TH][pmM];
+      M.DP = sS[e][S_DP][pmM];
+      P.Gsqrt = X.Gs; P.G13 = X.g13; P.G23 = X.g23;
+      P.DD = X.DD; P.DT = X.DT; P.MZ = X.MZ; P.MX = X.MX; P.MY = X.MY;
+      P.Dhyd = X.Dh; P.Phyd = X.Ph; P.Thyd = X.Th;
+      P.DP = X.DP;
       if (bc != FEDG_BND_NONE) {
         // exterior momentum from the interior node, as applyBC writes it into the halo
         double GH = 1.0, g11 = 1.0, g12 = 0.0, g22 = 1.0;
         if constexpr (!FLAT) {
-          const size_t h = (size_t)(pm % Nfp) + (size_t)Nfp * m.EMap3Dto2D[ke];
-          GH = m.GsqrtH[h];
-          g11 = m.G11[h];
-          g12 = m.G12[h];
-          g22 = m.G22[h];
+          const uint32_t h = (uint32_t)(pm % Nfp) + (uint32_t)Nfp * m.EMap3Dto2D[ke];
+          GH = ldg(m.GsqrtH, h);
+          g11 = ldg(m.G11, h);
+          g12 = ldg(m.G12, h);
+          g22 = ldg(m.G22, h);
         }
         reflect_momentum(bc, M.Gsqrt, GH, g11, g12, g22, M.G13, M.G23, nx, ny, nz, M.MX, M.MY, M.MZ, P.MX, P.MY,
                          P.MZ);
@@ -295,6 +343,22 @@ heve_kernel(const DevMesh m, const Consts c, const TendIn in, const RkOut rk) {
   }
 
   // ---- volume phase ----
+  // RK operands, loaded now so their latency overlaps the contractions
+  double rkA[5] = {0, 0, 0, 0, 0}, rkB[5] = {0, 0, 0, 0, 0};
+  if constexpr (FUSED) {
+    if (valid) {
+      if (rk.mode == 1) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          rkA[v] = ldg(rk.var0[v], o);
+          rkB[v] = ldg(rk.varTmp[v], o);
+        }
+      } else if (rk.mode == 2) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) rkB[v] = ldg(rk.varTmp[v], o);
+      }
+    }
+  }
   const double RDENS = 1.0 / (q[VD] + Dh);
   const double GsqrtV = Gs / GsH;
   const double RGsqrtV = 1.0 / GsqrtV;
@@ -305,29 +369,19 @@ heve_kernel(const DevMesh m, const Consts c, const TendIn in, const RkOut rk) {
     E22 = m.fE[1];
     E33 = m.fE[2];
   } else {
-    E11 = valid ? m.E11[o] : 0.0;
-    E22 = valid ? m.E22[o] : 0.0;
-    E33 = valid ? m.E33[o] : 0.0;
+    E11 = valid ? ldg(m.E11, o) : 0.0;
+    E22 = valid ? ldg(m.E22, o) : 0.0;
+    E33 = valid ? ldg(m.E33, o) : 0.0;
   }
-  double L[6];
+  // VFilterPM1 of DDENS (ELM/..._tensorprod3D.F90:2828-2839)
+  double drho = sS[e][VD][i + N1 * j] * sI[N1 * k];
 #pragma unroll
-  for (int f = 0; f < 6; ++f) L[f] = m.Lift[p + (size_t)Np * f];
-  double cor = 0.0, dpx = 0.0, dpy = 0.0;
-  if (valid) {
-    cor = in.CORIOLIS[(i + N1 * j) + (size_t)Nfp * ke2d];
-    dpx = in.DPhydDx[o];
-    dpy = in.DPhydDy[o];
-  }
-  // VFilterPM1 of DDENS (ELM/..._tensorprod3D.F90:2828-2839); sRho was
-  // published by the barrier before the face phase
-  double drho = sRho[e][i + N1 * j] * sI[N1 * k];
-#pragma unroll
-  for (int l = 1; l < N1; ++l) drho = drho + sRho[e][i + N1 * (j + N1 * l)] * sI[l + N1 * k];
+  for (int l = 1; l < N1; ++l) drho = drho + sS[e][VD][i + N1 * (j + N1 * l)] * sI[l + N1 * k];
 
   // One variable at a time: pointwise flux (FDS/..._rhot_heve.F90:376-418),
   // Div_var5 (ELM/..._tensorprod3D.F90:2760-2783: Dx Fx, Dy Fy, Dz Fz and the
-  // lift of del_flux), tendency (:424-483) and its RK update.  Recomputing the
-  // mass flux per variable keeps only the state live between variables.
+  // lift of del_flux), tendency (:424-483) and its RK update
+  // (CMN/scale_timeint_rk.F90:2232-2350 with one tendency buffer).
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
     const double m0 = Gs * q[VU];
@@ -369,15 +423,20 @@ heve_kernel(const DevMesh m, const Consts c, const TendIn in, const RkOut rk) {
     const double* Fx = sF[b][e][0];
     const double* Fy = sF[b][e][1];
     const double* Fz = sF[b][e][2];
-    double dx = sD[i] * Fx[N1 * (j + N1 * k)];
-    double dy = Fy[i + N1 * (N1 * k)] * sD[j];
-    double dz = Fz[i + N1 * j] * sD[k];
+    // one direction at a time, so that only one line of operands is live
+    // (sDt holds D1D transposed: row i of D1D is sDt[N1*i .. N1*i+N1-1])
+    double dx = sDt[N1 * i] * Fx[N1 * (j + N1 * k)];
 #pragma unroll
-    for (int l = 1; l < N1; ++l) {
-      dx = dx + sD[i + N1 * l] * Fx[l + N1 * (j + N1 * k)];
-      dy = dy + Fy[i + N1 * (l + N1 * k)] * sD[j + N1 * l];
-      dz = dz + Fz[i + N1 * (j + N1 * l)] * sD[k + N1 * l];
-    }
+    for (int l = 1; l < N1; ++l) dx = dx + sDt[l + N1 * i] * Fx[l + N1 * (j + N1 * k)];
+    __builtin_amdgcn_sched_barrier(0);
+    double dy = Fy[i + N1 * (N1 * k)] * sDt[N1 * j];
+#pragma unroll
+    for (int l = 1; l < N1; ++l) dy = dy + Fy[i + N1 * (l + N1 * k)] * sDt[l + N1 * j];
+    __builtin_amdgcn_sched_barrier(0);
+    double dz = Fz[i + N1 * j] * sDt[N1 * k];
+#pragma unroll
+    for (int l = 1; l < N1; ++l) dz = dz + Fz[i + N1 * (j + N1 * l)] * sDt[l + N1 * k];
+    __builtin_amdgcn_sched_barrier(0);
     const double* dfv = sDF[e][v];
     const double lift = L[0] * dfv[i + N1 * k] + L[1] * dfv[Nfp + j + N1 * k] + L[2] * dfv[2 * Nfp + i + N1 * k] +
                         L[3] * dfv[3 * Nfp + j + N1 * k] + L[4] * dfv[4 * Nfp + i + N1 * j] +
@@ -397,24 +456,36 @@ heve_kernel(const DevMesh m, const Consts c, const TendIn in, const RkOut rk) {
     }
     if (valid) {
       if constexpr (DROPIN) {
-        rk.out[v][o] = T;
+        stg(rk.out[v], o, T);
       } else {
-        // RK stage update (CMN/scale_timeint_rk.F90:2232-2350, tend_buf_size = 1)
+        double qn;
         switch (rk.mode) {
           case 0:
-            rk.varTmp[v][o] = q[v] + rk.cb * T;
-            rk.out[v][o] = q[v] + rk.ca * T;
+            stg(rk.varTmp[v], o, q[v] + rk.cb * T);
+            qn = q[v] + rk.ca * T;
             break;
           case 1:
-            rk.varTmp[v][o] = rk.varTmp[v][o] + rk.cb * T;
-            rk.out[v][o] = rk.var0[v][o] + rk.ca * T;
+            stg(rk.varTmp[v], o, rkB[v] + rk.cb * T);
+            qn = rkA[v] + rk.ca * T;
             break;
           case 2:
-            rk.out[v][o] = rk.varTmp[v][o] + rk.cb * T;
+            qn = rkB[v] + rk.cb * T;
             break;
           default:
-            rk.out[v][o] = q[v] + rk.cb * T;
+            qn = q[v] + rk.cb * T;
             break;
+        }
+        stg(rk.out[v], o, qn);
+        if (v == VT) {
+          // pressure perturbation of the new state, read by the next stage
+          // (own node and the neighbours' face nodes) instead of recomputed
+          double Rt = c.Rdry, Cv = c.CVdry, Cp = c.CPdry;
+          if (in.Rtot != nullptr) {
+            Rt = ldg(in.Rtot, o);
+            Cv = ldg(in.CVtot, o);
+            Cp = ldg(in.CPtot, o);
+          }
+          stg(rk.DPout, o, dpres_of(qn, Th, Ph, Rt, Cv, Cp, c));
         }
       }
     }
@@ -466,6 +537,19 @@ __global__ void drhot2pres_kernel(double* PRES, double* DPRES, const double* DRH
   const double P = c.PRES00 * fedg_pow(Rtot[t] * rP0 * RHOT, CPtot[t] / CVtot[t]);
   PRES[t] = P;
   DPRES[t] = P - PRES_hyd[t];
+}
+
+__global__ void dpres_kernel(double* DPRES, const double* DRHOT, const double* PRES_hyd, const double* THERM_hyd,
+                             const double* Rtot, const double* CVtot, const double* CPtot, long n, const Consts c) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double Rt = c.Rdry, Cv = c.CVdry, Cp = c.CPdry;
+  if (Rtot != nullptr) {
+    Rt = Rtot[t];
+    Cv = CVtot[t];
+    Cp = CPtot[t];
+  }
+  DPRES[t] = dpres_of(DRHOT[t], THERM_hyd[t], PRES_hyd[t], Rt, Cv, Cp, c);
 }
 
 __global__ void halo_kernel(double* f, const int32_t* src, long base, int nB) {
@@ -611,6 +695,15 @@ int launch_drhot2pres(fedg_ctx* ctx, double* PRES, double* DPRES, const double* 
   if (n <= 0) return FEDG_OK;
   hipLaunchKernelGGL(drhot2pres_kernel, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream, PRES, DPRES, DRHOT,
                      PRES_hyd, THERM_hyd, Rtot, CVtot, CPtot, n, ctx->c);
+  FEDG_HIP_CHECK(hipGetLastError());
+  return FEDG_OK;
+}
+
+int launch_dpres(fedg_ctx* ctx, double* DPRES, const double* DRHOT, const double* PRES_hyd, const double* THERM_hyd,
+                 const double* Rtot, const double* CVtot, const double* CPtot, long n) {
+  if (n <= 0) return FEDG_OK;
+  hipLaunchKernelGGL(dpres_kernel, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream, DPRES, DRHOT, PRES_hyd,
+                     THERM_hyd, Rtot, CVtot, CPtot, n, ctx->c);
   FEDG_HIP_CHECK(hipGetLastError());
   return FEDG_OK;
 }
