@@ -42,7 +42,7 @@ int dev_upload(T** dst, const T* src, size_t count) {
 void free_mesh(DevMesh& m) {
   void* ptrs[] = {m.E11, m.E22, m.E33, m.Gsqrt, m.G13, m.G23, m.GsqrtH, m.G11, m.G12, m.G22, m.nx, m.ny, m.nz,
                   m.Fscale, m.vmapM, m.vmapP, m.EMap3Dto2D, m.halo_src, m.velbc, m.nbr, m.nbr_bc, m.D1D,
-                  m.Lift, m.Itr};
+                  m.Lift, m.Itr, m.liftT};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   memset(&m, 0, sizeof(m));
@@ -401,6 +401,30 @@ int fedg_ctx_create(const fedg_mesh_desc* d, fedg_ctx** out) {
   up(&m.D1D, d->D1D, (size_t)n1 * n1);
   up(&m.Lift, d->Lift_mat, (size_t)Np * 6);
   up(&m.Itr, d->IntrpMat_VPOrdM1_tr, (size_t)n1 * n1);
+  {
+    // tensor-product structure of Lift_mat: face f's entry depends only on
+    // the node's index along the face's normal axis (y for faces 1,3; x for
+    // faces 2,4; z for faces 5,6)
+    std::vector<double> lt((size_t)n1 * 6);
+    const double* L = d->Lift_mat;
+    auto line = [&](int f, int i, int j, int k) { return f == 0 || f == 2 ? j : f == 1 || f == 3 ? i : k; };
+    for (int f = 0; f < 6; ++f)
+      for (int x = 0; x < n1; ++x) {
+        const int i = (f == 1 || f == 3) ? x : 0, j = (f == 0 || f == 2) ? x : 0, k = (f >= 4) ? x : 0;
+        lt[x + (size_t)n1 * f] = L[i + n1 * (j + n1 * k) + (size_t)Np * f];
+      }
+    bool tensor = true;
+    for (int f = 0; f < 6 && tensor; ++f)
+      for (int k = 0; k < n1 && tensor; ++k)
+        for (int j = 0; j < n1 && tensor; ++j)
+          for (int i = 0; i < n1; ++i)
+            if (L[i + n1 * (j + n1 * k) + (size_t)Np * f] != lt[line(f, i, j, k) + (size_t)n1 * f]) {
+              tensor = false;
+              break;
+            }
+    m.lift_tensor = tensor ? 1 : 0;
+    up(&m.liftT, lt.data(), lt.size());
+  }
   if (rc != FEDG_OK) {
     free_mesh(m);
     (void)hipStreamDestroy(ctx->stream);

@@ -49,6 +49,11 @@ struct DevMesh {
   double* D1D;                 // (n1,n1)
   double* Lift;                // (n1,n1,n1,6)
   double* Itr;                 // (n1,n1)
+  // Lift_mat as six 1D columns, when it has the tensor-product structure of
+  // the exact-mass-matrix lift: Lift(i,j,k,f) = liftT(x,f) with x = j (f = 1,3),
+  // i (f = 2,4), k (f = 5,6); lift_tensor = 0 otherwise
+  int lift_tensor;
+  double* liftT;               // (n1,6)
   // flat uniform box description (valid when flat != 0)
   int flat;
   double fE[3];                // E11,E22,E33 of every element
