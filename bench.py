@@ -12,7 +12,7 @@ traffic of 36.8 B per DOF*stage (q^n, stage state and RK accumulator reads and
 writes plus three base-state fields per stage); its duration is the mean of
 HIP events recorded around every stage launch on the stepper's stream during
 the timed steps.  traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE /
-WRITE_SIZE (profiles/heve_stage_flat_p7_64x64x16.traffic.txt) when present.
+WRITE_SIZE (profiles/heve_stage_flat_p7_64x64x16.md), scaled by a copy kernel.
 
 cpu_baseline: the CPU oracle (C restatement of the reference Fortran, OpenMP,
 FP64, the C library's pow) timed on this host over one RK4 step of the same
@@ -29,14 +29,15 @@ sys.path.insert(0, os.path.join(ROOT, "fe-project_amd"))
 
 BYTES_PER_DOF_STAGE = 36.8
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "heve_stage_flat_p7_64x64x16.traffic.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "heve_stage_flat_p7_64x64x16.md")
 
 
 def read_traffic():
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
-                if line.startswith("hbm_bytes_per_launch"):
+                line = line.strip()
+                if line.startswith("hbm_bytes_per_launch="):
                     return float(line.split("=")[1])
     except OSError:
         return None
@@ -66,6 +67,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the timed steps, run one device copy of Np*Ne doubles (copy_kernel), the "
+                         "known-size reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU sharding is a design "
@@ -111,6 +115,13 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic()},
     }
+    if args.copy_probe:
+        t = fedg.TimeIntRK(dm, "ERK_4s4o", case.dt)
+        src = fedg.DevField(mesh.Np * mesh.NeA)
+        t.store_var0(src, 1)
+        out["copy_probe_bytes"] = {"read": 8 * mesh.Np * mesh.Ne, "write": 8 * mesh.Np * mesh.Ne}
+        src.free()
+        t.close()
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", str(min(os.cpu_count() or 1, 16))))
         out["cpu_baseline"] = cpu_baseline(mesh, bs, q, case.dt, threads)
