@@ -3,7 +3,7 @@
 
 Workload: 160 km x 160 km x 10 km box, 64x64x16 hexahedra, p=7 (512 LGL
 nodes per element, 33.5M nodes, 167.8M DOF), dry isothermal base state,
-three 2 K warm bubbles, classic RK4 (ERK_4s4o), dt = 0.05 s, plain FP64.
+three 2 K warm bubbles, classic RK4 (ERK_4s4o), dt = 0.04 s, plain FP64.
 A step is one RK4 step = 4 fused stage kernels over all elements, state
 resident in HBM.  value = DOF*stage/s = 5 Np Ne * 4 * K / wall time of K steps.
 
@@ -109,7 +109,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "plain FP64 HEVE warm bubbles, 64x64x16 elements, p=7, RK4 dt=0.05 s",
+        "config": {"workload": "plain FP64 HEVE warm bubbles, 64x64x16 elements, p=7, RK4 dt=0.04 s",
                    "elements": mesh.Ne, "nodes": mesh.Np * mesh.Ne, "dof": dof, "kernel": "fused stage, "
                    + ("flat-terrain" if st.flat else "general-metric")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
