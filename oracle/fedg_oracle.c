@@ -58,7 +58,8 @@ int oracle_num_threads(void) { return omp_get_max_threads(); }
 static int g_pow_mode = 0;
 void oracle_set_pow_mode(int mode) { g_pow_mode = mode; }
 int oracle_get_pow_mode(void) { return g_pow_mode; }
-static inline double opow(double x, double y) { return g_pow_mode ? fedg_pow(x, y) : pow(x, y); }
+double oracle_opow(double x, double y) { return g_pow_mode ? fedg_pow(x, y) : pow(x, y); }
+#define opow oracle_opow
 void oracle_pow_array(const double *x, const double *y, double *out, long n, int mode) {
   for (long i = 0; i < n; ++i) out[i] = mode ? fedg_pow(x[i], y[i]) : pow(x[i], y[i]);
 }
@@ -133,65 +134,7 @@ void oracle_apply_bc(const oracle_mesh *m, double *DDENS, double *MOMX, double *
   }
 }
 
-/* ---------------------------------------------------------------------- */
-/* Tensor-product operators: ELM/scale_element_operation_tensorprod3D_kernel.F90
- * (P7 instance at :2226-2339, :2415-2480; the .erb template generates the
- * same sums for every p). */
-
-/* matvec_dirX: out(i,jk) = sum_l Mat(i,l) * in(l,jk) */
-static void op_dirX(int n1, const double *Mat, const double *in, double *out) {
-  for (int jk = 0; jk < n1 * n1; ++jk)
-    for (int i = 0; i < n1; ++i) {
-      double s = Mat[i] * in[n1 * jk];
-      for (int l = 1; l < n1; ++l) s = s + Mat[i + n1 * l] * in[l + n1 * jk];
-      out[i + n1 * jk] = s;
-    }
-}
-/* matvec_dirY: out(i,j,k) = sum_l in(i,l,k) * Mat_tr(l,j), Mat_tr(l,j) = Mat(j,l) */
-static void op_dirY(int n1, const double *Mat, const double *in, double *out) {
-  for (int k = 0; k < n1; ++k)
-    for (int j = 0; j < n1; ++j)
-      for (int i = 0; i < n1; ++i) {
-        double s = in[i + n1 * (0 + n1 * k)] * Mat[j];
-        for (int l = 1; l < n1; ++l) s = s + in[i + n1 * (l + n1 * k)] * Mat[j + n1 * l];
-        out[i + n1 * (j + n1 * k)] = s;
-      }
-}
-/* matvec_dirZ: out(i,j,k) = sum_l in(i,j,l) * Mat_tr(l,k); Mat_tr passed as stored */
-static void op_dirZ_tr(int n1, const double *Mat_tr, const double *in, double *out) {
-  const int n2 = n1 * n1;
-  for (int k = 0; k < n1; ++k)
-    for (int ij = 0; ij < n2; ++ij) {
-      double s = in[ij] * Mat_tr[0 + n1 * k];
-      for (int l = 1; l < n1; ++l) s = s + in[ij + n2 * l] * Mat_tr[l + n1 * k];
-      out[ij + n2 * k] = s;
-    }
-}
-/* dirZ with the derivative matrix: Mat_tr(l,k) = Mat(k,l) */
-static void op_dirZ(int n1, const double *Mat, const double *in, double *out) {
-  const int n2 = n1 * n1;
-  for (int k = 0; k < n1; ++k)
-    for (int ij = 0; ij < n2; ++ij) {
-      double s = in[ij] * Mat[k];
-      for (int l = 1; l < n1; ++l) s = s + in[ij + n2 * l] * Mat[k + n1 * l];
-      out[ij + n2 * k] = s;
-    }
-}
-/* matvec_Lift_hexahedral: six face terms per node, faces in reference order */
-static void op_lift(int n1, const double *Lift, const double *vin, double *out) {
-  const int n2 = n1 * n1, Np = n2 * n1;
-  for (int k = 0; k < n1; ++k)
-    for (int j = 0; j < n1; ++j)
-      for (int i = 0; i < n1; ++i) {
-        int p = i + n1 * (j + n1 * k);
-        out[p] = Lift[p] * vin[i + n1 * k]
-               + Lift[p + Np] * vin[j + n1 * k + n2]
-               + Lift[p + 2 * Np] * vin[i + n1 * k + 2 * n2]
-               + Lift[p + 3 * Np] * vin[j + n1 * k + 3 * n2]
-               + Lift[p + 4 * Np] * vin[i + n1 * j + 4 * n2]
-               + Lift[p + 5 * Np] * vin[i + n1 * j + 5 * n2];
-      }
-}
+#include "oracle_ops.h"
 
 /* ---------------------------------------------------------------------- */
 /* FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve_numflux.F90:946-1138 */

@@ -133,6 +133,48 @@ void oracle_set_pow_mode(int mode);
 int oracle_get_pow_mode(void);
 void oracle_pow_array(const double *x, const double *y, double *out, long n, int mode);
 
+
+/* ---------------------------------------------------------------------- */
+/* HEVI (fedg_oracle_hevi.c) */
+
+/* FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi_numflux.F90:232-416; del_flux (NfpTot,5,Ne) */
+void oracle_hevi_numflux(const oracle_mesh *m, double *del_flux, const double *DDENS_, const double *MOMX_,
+                         const double *MOMY_, const double *MOMZ_, const double *DRHOT_, const double *DPRES,
+                         const double *DENS_hyd, const double *PRES_hyd, const double *THERM_hyd);
+/* FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi.F90:289-482 */
+void oracle_hevi_cal_tend(const oracle_mesh *m, double *DENS_dt, double *MOMX_dt, double *MOMY_dt,
+                          double *MOMZ_dt, double *RHOT_dt, const double *DDENS_, const double *MOMX_,
+                          const double *MOMY_, const double *MOMZ_, const double *DRHOT_, const double *DPRES_,
+                          const double *DENS_hyd, const double *PRES_hyd, const double *PRES_hyd_ref,
+                          const double *THERM_hyd, const double *CORIOLIS, const double *Rtot,
+                          const double *CVtot, const double *CPtot, const double *DPhydDx, const double *DPhydDy);
+/* Newton iterations of FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi.F90:858-917 (reference: max_it = 1,
+ * rtol <= 0); fields in VID order, pv (Np,Ne) in/out, qs/q0 (Np,NeA). Returns iterations done. */
+int oracle_hevi_newton(const oracle_mesh *m, double *const pv[5], const double *const qs[5],
+                       const double *const q0[5], const double *DENS_hyd, const double *PRES_hyd,
+                       const double *Rtot, const double *CVtot, const double *CPtot, double impl_fac, double dt,
+                       int max_it, double rtol, double *resid);
+/* FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi.F90:772-965; tend, q, var0 in VID order */
+int oracle_hevi_cal_vi(const oracle_mesh *m, double *const tend[5], const double *const q[5],
+                       const double *const var0[5], const double *DENS_hyd, const double *PRES_hyd,
+                       const double *Rtot, const double *CVtot, const double *CPtot, double impl_fac, double dt,
+                       int max_it, double rtol, double *resid);
+void oracle_hevi_linear_system(const oracle_mesh *m, double *const pv[5], const double *const qs[5],
+                               const double *const q0[5], const double *DENS_hyd, const double *PRES_hyd,
+                               const double *Rtot, const double *CVtot, const double *CPtot, double impl_fac,
+                               double dt, double *D, double *L, double *U, double *b, double *Duv, double *Luv,
+                               double *Uuv, double *buv);
+void oracle_hevi_residual(const oracle_mesh *m, double *const pv[5], const double *const qs[5],
+                          const double *const q0[5], const double *DENS_hyd, const double *PRES_hyd,
+                          const double *Rtot, const double *CVtot, const double *CPtot, double impl_fac, double dt,
+                          double *const G[5]);
+/* FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:694-870, HEVI branch; work: (10 + 10*tend_buf_size)*Np*NeA */
+void oracle_hevi_update(const oracle_mesh *m, const oracle_rk_tab *t, int nsteps, double *const q[5],
+                        double *PRES, double *DPRES, const double *DENS_hyd, const double *PRES_hyd,
+                        const double *PRES_hyd_ref, const double *THERM_hyd, const double *CORIOLIS,
+                        const double *Rtot, const double *CVtot, const double *CPtot, const double *DPhydDx,
+                        const double *DPhydDy, int max_it, double rtol, double *work);
+
 #ifdef __cplusplus
 }
 #endif

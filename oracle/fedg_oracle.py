@@ -59,6 +59,19 @@ def _load():
     lib.oracle_set_pow_mode.argtypes = [C.c_int]
     lib.oracle_get_pow_mode.restype = C.c_int
     lib.oracle_pow_array.argtypes = [dp, dp, dp, C.c_long, C.c_int]
+    V5 = dp * 5
+    lib.oracle_hevi_numflux.argtypes = [M] + [dp] * 10
+    lib.oracle_hevi_cal_tend.argtypes = [M] + [dp] * 21
+    lib.oracle_hevi_newton.argtypes = [M, V5, V5, V5] + [dp] * 5 + [C.c_double, C.c_double, C.c_int,
+                                                                     C.c_double, dp]
+    lib.oracle_hevi_newton.restype = C.c_int
+    lib.oracle_hevi_cal_vi.argtypes = [M, V5, V5, V5] + [dp] * 5 + [C.c_double, C.c_double, C.c_int,
+                                                                     C.c_double, dp]
+    lib.oracle_hevi_cal_vi.restype = C.c_int
+    lib.oracle_hevi_linear_system.argtypes = [M, V5, V5, V5] + [dp] * 5 + [C.c_double, C.c_double] + [dp] * 8
+    lib.oracle_hevi_residual.argtypes = [M, V5, V5, V5] + [dp] * 5 + [C.c_double, C.c_double, V5]
+    lib.oracle_hevi_update.argtypes = [M, C.POINTER(RkTabC), C.c_int, V5] + [dp] * 2 + [dp] * 10 + [
+        C.c_int, C.c_double, dp]
     return lib
 
 
@@ -137,6 +150,86 @@ class OracleMesh:
         else:
             LIB.oracle_heve_cal_tend_range(self.ref, int(ke_range[0]), int(ke_range[1]), *args)
         return out
+
+    # --- HEVI ---------------------------------------------------------------
+    def hevi_numflux(self, q, DPRES, bs):
+        m = self.mesh
+        out = np.zeros(m.NfpTot * 5 * m.Ne)
+        LIB.oracle_hevi_numflux(self.ref, _d(out), _d(q[0]), _d(q[3]), _d(q[4]), _d(q[2]), _d(q[1]), _d(DPRES),
+                                _d(bs.DENS_hyd), _d(bs.PRES_hyd), _d(bs.THERM_hyd))
+        return out
+
+    def hevi_cal_tend(self, q, DPRES, bs, DPhydDx, DPhydDy):
+        """Explicit HEVI tendencies (DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt), each (Np,NeA)."""
+        m = self.mesh
+        out = [m.field() for _ in range(5)]
+        LIB.oracle_hevi_cal_tend(self.ref, *[_d(a) for a in out], _d(q[0]), _d(q[3]), _d(q[4]), _d(q[2]),
+                                 _d(q[1]), _d(DPRES), _d(bs.DENS_hyd), _d(bs.PRES_hyd), _d(bs.PRES_hyd_ref),
+                                 _d(bs.THERM_hyd), _d(bs.CORIOLIS), _d(bs.Rtot), _d(bs.CVtot), _d(bs.CPtot),
+                                 _d(DPhydDx), _d(DPhydDy))
+        return out
+
+    def _v5(self, fields):
+        return (dp * 5)(*[_d(a) for a in fields])
+
+    def hevi_cal_vi(self, q, var0, bs, impl_fac, dt, max_it=1, rtol=0.0):
+        """Implicit tendencies in VID order (each (Np,NeA)), iterations done, residual history."""
+        m = self.mesh
+        tend = [m.field() for _ in range(5)]
+        resid = np.zeros(max(max_it, 1))
+        it = LIB.oracle_hevi_cal_vi(self.ref, self._v5(tend), self._v5(q), self._v5(var0), _d(bs.DENS_hyd),
+                                    _d(bs.PRES_hyd), _d(bs.Rtot), _d(bs.CVtot), _d(bs.CPtot), float(impl_fac),
+                                    float(dt), int(max_it), float(rtol), _d(resid))
+        return tend, it, resid[:min(it + 1, max(max_it, 1))]
+
+    def hevi_newton(self, pv, qs, q0, bs, impl_fac, dt, max_it=1, rtol=0.0):
+        """Newton iterations on pv (VID order, updated in place); returns (iterations, residual history)."""
+        resid = np.zeros(max(max_it, 1))
+        it = LIB.oracle_hevi_newton(self.ref, self._v5(pv), self._v5(qs), self._v5(q0), _d(bs.DENS_hyd),
+                                    _d(bs.PRES_hyd), _d(bs.Rtot), _d(bs.CVtot), _d(bs.CPtot), float(impl_fac),
+                                    float(dt), int(max_it), float(rtol), _d(resid))
+        return it, resid[:min(it + 1, max(max_it, 1))]
+
+    def hevi_linear_system(self, pv, qs, q0, bs, impl_fac, dt):
+        m = self.mesh
+        N = m.elem.n1
+        NeZ = m.Ne // m.Ne2D
+        nc = NeZ * m.Nfp_v * m.Ne2D
+        n3 = 3 * N
+        D, L, U, b = np.zeros(nc * n3 * n3), np.zeros(nc * n3 * 3), np.zeros(nc * n3 * 3), np.zeros(nc * n3)
+        Duv, Luv, Uuv, buv = np.zeros(nc * N * N), np.zeros(nc * N), np.zeros(nc * N), np.zeros(nc * N * 2)
+        LIB.oracle_hevi_linear_system(self.ref, self._v5(pv), self._v5(qs), self._v5(q0), _d(bs.DENS_hyd),
+                                      _d(bs.PRES_hyd), _d(bs.Rtot), _d(bs.CVtot), _d(bs.CPtot), float(impl_fac),
+                                      float(dt), _d(D), _d(L), _d(U), _d(b), _d(Duv), _d(Luv), _d(Uuv), _d(buv))
+        sh = (m.Ne2D, m.Nfp_v, NeZ)
+        return {"D": D.reshape(sh + (n3, n3)).transpose(0, 1, 2, 4, 3),
+                "L": L.reshape(sh + (3, n3)).transpose(0, 1, 2, 4, 3),
+                "U": U.reshape(sh + (3, n3)).transpose(0, 1, 2, 4, 3),
+                "b": b.reshape(sh + (n3,)),
+                "Duv": Duv.reshape(sh + (N, N)).transpose(0, 1, 2, 4, 3),
+                "Luv": Luv.reshape(sh + (N,)), "Uuv": Uuv.reshape(sh + (N,)),
+                "buv": buv.reshape(sh + (2, N)).transpose(0, 1, 2, 4, 3)}
+
+    def hevi_residual(self, pv, qs, q0, bs, impl_fac, dt):
+        m = self.mesh
+        G = [np.zeros(m.Np * m.Ne) for _ in range(5)]
+        LIB.oracle_hevi_residual(self.ref, self._v5(pv), self._v5(qs), self._v5(q0), _d(bs.DENS_hyd),
+                                 _d(bs.PRES_hyd), _d(bs.Rtot), _d(bs.CVtot), _d(bs.CPtot), float(impl_fac),
+                                 float(dt), self._v5(G))
+        return G
+
+    def hevi_update(self, q, bs, DPhydDx, DPhydDy, dt, nsteps, scheme="IMEX_ARK324", max_it=1, rtol=0.0):
+        """Advance q (VID order, (Np,NeA) host fields) in place by HEVI IMEX steps; returns (PRES, DPRES)."""
+        m = self.mesh
+        tab = rk_tab(scheme, dt)
+        nA = m.Np * m.NeA
+        work = np.zeros(nA * (10 + 10 * tab.tend_buf_size))
+        P, D = m.field(), m.field()
+        LIB.oracle_hevi_update(self.ref, C.byref(tab), int(nsteps), self._v5(q), _d(P), _d(D), _d(bs.DENS_hyd),
+                               _d(bs.PRES_hyd), _d(bs.PRES_hyd_ref), _d(bs.THERM_hyd), _d(bs.CORIOLIS),
+                               _d(bs.Rtot), _d(bs.CVtot), _d(bs.CPtot), _d(DPhydDx), _d(DPhydDy), int(max_it),
+                               float(rtol), _d(work))
+        return P, D
 
     def calc_phyd_hgrad(self, bs):
         m = self.mesh
