@@ -11,6 +11,7 @@
 
 #include "fedg_builder.h"
 #include "fedg_internal.h"
+#include "fedg_hevi.h"
 
 namespace fedg {
 
@@ -511,6 +512,7 @@ int fedg_ctx_create(const fedg_mesh_desc* d, fedg_ctx** out) {
 
 int fedg_ctx_destroy(fedg_ctx* ctx) {
   if (!ctx) return FEDG_OK;
+  hevi_work_free(ctx);
   free_mesh(ctx->m);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -593,6 +595,138 @@ int fedg_heve_cal_tend(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, double* 
   TRY(launch_heve_dropin(ctx, in, out));
   FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   return FEDG_OK;
+}
+
+int fedg_hevi_cal_tend(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, double* MOMY_dt, double* MOMZ_dt,
+                       double* RHOT_dt, const double* DDENS_, const double* MOMX_, const double* MOMY_,
+                       const double* MOMZ_, const double* DRHOT_, const double* DPRES_, const double* DENS_hyd,
+                       const double* PRES_hyd, const double* PRES_hyd_ref, const double* THERM_hyd,
+                       const double* CORIOLIS, const double* Rtot, const double* CVtot, const double* CPtot,
+                       const double* DPhydDx, const double* DPhydDy) {
+  (void)PRES_hyd_ref;
+  if (!ctx || !DENS_dt || !MOMX_dt || !MOMY_dt || !MOMZ_dt || !RHOT_dt || !DDENS_ || !MOMX_ || !MOMY_ || !MOMZ_ ||
+      !DRHOT_ || !DPRES_ || !DENS_hyd || !PRES_hyd || !THERM_hyd || !CORIOLIS || !DPhydDx || !DPhydDy)
+    return FEDG_ERR_ARG;
+  TendIn in{};
+  in.q[0] = DDENS_;
+  in.q[1] = DRHOT_;
+  in.q[2] = MOMZ_;
+  in.q[3] = MOMX_;
+  in.q[4] = MOMY_;
+  in.DPRES = DPRES_;
+  in.DENS_hyd = DENS_hyd;
+  in.PRES_hyd = PRES_hyd;
+  in.THERM_hyd = THERM_hyd;
+  in.CORIOLIS = CORIOLIS;
+  in.DPhydDx = DPhydDx;
+  in.DPhydDy = DPhydDy;
+  in.Rtot = Rtot;
+  in.CVtot = CVtot;
+  in.CPtot = CPtot;
+  double* out[5] = {DENS_dt, RHOT_dt, MOMZ_dt, MOMX_dt, MOMY_dt};
+  TRY(launch_hevi_explicit(ctx, in, out));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return FEDG_OK;
+}
+
+}  // extern "C"
+
+namespace fedg {
+int hevi_work_get(fedg_ctx* ctx, HeviWork** w) {
+  if (ctx->hevi) {
+    *w = ctx->hevi;
+    return FEDG_OK;
+  }
+  const DevMesh& m = ctx->m;
+  if (m.Ne2D <= 0 || m.Ne % m.Ne2D != 0) {
+    set_error("HEVI: the mesh is not a stack of Ne2D-element horizontal layers");
+    return FEDG_ERR_UNSUPPORTED;
+  }
+  HeviWork* h = new HeviWork();
+  const size_t n = (size_t)m.Np * m.Ne;
+  int rc = FEDG_OK;
+  auto al = [&](double** p, size_t cnt) {
+    if (rc == FEDG_OK && hipMalloc((void**)p, sizeof(double) * cnt) != hipSuccess) {
+      set_error("HEVI: device allocation of the implicit-solve workspace failed");
+      rc = FEDG_ERR_NOMEM;
+    }
+  };
+  for (int v = 0; v < 5; ++v) al(&h->pv[v], n);
+  al(&h->alph, (size_t)2 * m.Nfp * m.Ne);
+  al(&h->b3, 3 * n);
+  al(&h->buv, 2 * n);
+  al(&h->coef, 3 * n);
+  al(&h->partial, 256);
+  ctx->hevi = h;
+  if (rc != FEDG_OK) {
+    hevi_work_free(ctx);
+    return rc;
+  }
+  *w = h;
+  return FEDG_OK;
+}
+
+void hevi_work_free(fedg_ctx* ctx) {
+  HeviWork* h = ctx->hevi;
+  if (!h) return;
+  for (int v = 0; v < 5; ++v)
+    if (h->pv[v]) (void)hipFree(h->pv[v]);
+  double* bufs[] = {h->alph, h->b3, h->buv, h->coef, h->partial};
+  for (double* b : bufs)
+    if (b) (void)hipFree(b);
+  delete h;
+  ctx->hevi = nullptr;
+}
+}  // namespace fedg
+
+extern "C" {
+
+static int hevi_cal_vi_common(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, double* MOMY_dt, double* MOMZ_dt,
+                              double* RHOT_dt, const double* DDENS_, const double* MOMX_, const double* MOMY_,
+                              const double* MOMZ_, const double* DRHOT_, const double* DENS_hyd,
+                              const double* PRES_hyd, const double* DDENS0_, const double* MOMX0_,
+                              const double* MOMY0_, const double* MOMZ0_, const double* DRHOT0_, const double* Rtot,
+                              const double* CVtot, const double* CPtot, double impl_fac, int max_it, double rtol,
+                              double* resid, int* iterations) {
+  if (!ctx || !DENS_dt || !MOMX_dt || !MOMY_dt || !MOMZ_dt || !RHOT_dt || !DDENS_ || !MOMX_ || !MOMY_ || !MOMZ_ ||
+      !DRHOT_ || !DENS_hyd || !PRES_hyd || !DDENS0_ || !MOMX0_ || !MOMY0_ || !MOMZ0_ || !DRHOT0_ || !Rtot ||
+      !CVtot || !CPtot || max_it < 1 || !std::isfinite(impl_fac))
+    return FEDG_ERR_ARG;
+  HeviWork* w = nullptr;
+  TRY(hevi_work_get(ctx, &w));
+  double* tend[5] = {DENS_dt, RHOT_dt, MOMZ_dt, MOMX_dt, MOMY_dt};
+  const double* q[5] = {DDENS_, DRHOT_, MOMZ_, MOMX_, MOMY_};
+  const double* q0[5] = {DDENS0_, DRHOT0_, MOMZ0_, MOMX0_, MOMY0_};
+  int it = 0;
+  TRY(hevi_cal_vi(ctx, w, tend, q, q0, DENS_hyd, PRES_hyd, Rtot, CVtot, CPtot, impl_fac, max_it, rtol, resid, &it));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  if (iterations) *iterations = it;
+  return FEDG_OK;
+}
+
+int fedg_hevi_cal_vi(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, double* MOMY_dt, double* MOMZ_dt,
+                     double* RHOT_dt, const double* DDENS_, const double* MOMX_, const double* MOMY_,
+                     const double* MOMZ_, const double* DRHOT_, const double* DENS_hyd, const double* PRES_hyd,
+                     const double* DDENS0_, const double* MOMX0_, const double* MOMY0_, const double* MOMZ0_,
+                     const double* DRHOT0_, const double* Rtot, const double* CVtot, const double* CPtot,
+                     double impl_fac, double dt) {
+  (void)dt;
+  return hevi_cal_vi_common(ctx, DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt, DDENS_, MOMX_, MOMY_, MOMZ_, DRHOT_,
+                            DENS_hyd, PRES_hyd, DDENS0_, MOMX0_, MOMY0_, MOMZ0_, DRHOT0_, Rtot, CVtot, CPtot,
+                            impl_fac, 1, 0.0, nullptr, nullptr);
+}
+
+int fedg_hevi_cal_vi_newton(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, double* MOMY_dt, double* MOMZ_dt,
+                            double* RHOT_dt, const double* DDENS_, const double* MOMX_, const double* MOMY_,
+                            const double* MOMZ_, const double* DRHOT_, const double* DENS_hyd,
+                            const double* PRES_hyd, const double* DDENS0_, const double* MOMX0_,
+                            const double* MOMY0_, const double* MOMZ0_, const double* DRHOT0_, const double* Rtot,
+                            const double* CVtot, const double* CPtot, double impl_fac, double dt, int max_it,
+                            double rtol, double* resid, int* iterations) {
+  (void)dt;
+  return hevi_cal_vi_common(ctx, DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt, DDENS_, MOMX_, MOMY_, MOMZ_, DRHOT_,
+                            DENS_hyd, PRES_hyd, DDENS0_, MOMX0_, MOMY0_, MOMZ0_, DRHOT0_, Rtot, CVtot, CPtot,
+                            impl_fac, max_it, rtol, resid, iterations);
 }
 
 int fedg_calc_phyd_hgrad(fedg_ctx* ctx, double* DPhydDx, double* DPhydDy, const double* PRES_hyd,

@@ -1,0 +1,709 @@
+// CDNA4 (gfx950) kernels for the vertically implicit part of FElib's HEVI
+// scheme (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi.F90:772-965 and
+// FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi_common_2.F90:111-1328).
+//
+// One Newton iteration is four launches:
+//   1. hevi_rhs_uv   (element, one thread per node): the damping coefficient
+//      alph at every vertical face node from q^n, the horizontal-momentum
+//      face jumps of the iterate, their lift, MOMX_t / MOMY_t and the uv rhs;
+//   2. hevi_solve_uv (one wavefront per vertical node line): assembles the
+//      N x N blocks of construct_matbnd_uv level by level, block-Thomas
+//      elimination with the pivoted LU of solve_NnodeN_uv, back substitution,
+//      MOMX, MOMY += increment;
+//   3. hevi_rhs3     (element): vertical fluxes of DDENS, MOMZ, DRHOT, their
+//      face jumps, Dz and lift, VFilterPM1 buoyancy, the var3 rhs and the
+//      linearisation coefficients POT, WT, DPDRHOT;
+//   4. hevi_solve3   (one wavefront per line): the 3N x 3N blocks of
+//      construct_matbnd, block-Thomas with the LU of solve_NnodeN_var3,
+//      DDENS, MOMZ, DRHOT += increment.
+// A line is the column of one horizontal node (ij) of one horizontal element
+// through all NeZ elements; everything the implicit operator couples lies on
+// it.  The LU runs on the line's block in LDS with all 64 lanes: pivot by a
+// wavefront max-reduction and ballot, row swap, scaling and the rank-1 update
+// spread over the lanes; every entry sees the same operations in the same
+// order as the reference's serial loops, so results are bit for bit those of
+// the CPU restatement.
+#include "fedg_device.h"
+#include "fedg_hevi.h"
+
+namespace fedg {
+namespace {
+
+struct HeviArgs {
+  DevMesh m;
+  Consts c;
+  const double* qs[5];     // stage state q* (VID order), (Np,NeA)
+  const double* q0[5];     // var0 = q^n
+  double* pv[5];           // Newton iterate, (Np,Ne)
+  const double* DENS_hyd;
+  const double* PRES_hyd;
+  const double* Rtot;
+  const double* CVtot;
+  const double* CPtot;
+  double* alph;            // (2*Nfp, Ne): z- then z+ face nodes
+  double* b3;              // (3, Np*Ne): DENS, MOMZ, RHOT rhs
+  double* buv;             // (2, Np*Ne)
+  double* coef;            // (3, Np*Ne): POT, WT, DPDRHOT
+  double* T[5];            // implicit-operator tendencies (VID order), written when want_T
+  double impl_fac;
+  int want_b;              // form the rhs (Newton) or only the operator (impl_fac = 0)
+  int NeZ;
+};
+
+__device__ __forceinline__ double gsqrtv(const HeviArgs& a, int ke, int p, int N1) {
+  const int ke2d = a.m.EMap3Dto2D[ke];
+  const int ij = p % (N1 * N1);
+  return ldg(a.m.Gsqrt, (uint32_t)a.m.Np * ke + p) / ldg(a.m.GsqrtH, (uint32_t)ij + (uint32_t)a.m.Nfp * ke2d);
+}
+
+// vertical neighbour node of the face node (bottom: f = 0, top: f = 1); self at the column ends
+__device__ __forceinline__ uint32_t vz_nbr(const HeviArgs& a, int ke, int f, int ij, int N1) {
+  const int Ne2D = a.m.Ne2D;
+  const int kz = ke / Ne2D;
+  const uint32_t Np = a.m.Np, Nfp = a.m.Nfp;
+  if (f == 0) {
+    if (kz > 0) return Np * (ke - Ne2D) + ij + Nfp * (N1 - 1);
+    return Np * ke + ij;
+  }
+  if (kz < a.NeZ - 1) return Np * (ke + Ne2D) + ij;
+  return Np * ke + ij + Nfp * (N1 - 1);
+}
+
+// ---------------------------------------------------------------------------
+// 1. vi_cal_del_flux_dyn_uv (:1015-1158) + eval_Ax_uv (:482-586)
+template <int N1>
+__global__ void __launch_bounds__(N1* N1* N1) hevi_rhs_uv_kernel(const HeviArgs a) {
+  constexpr int Np = N1 * N1 * N1, Nfp = N1 * N1;
+  __shared__ double sdf[2][2][Nfp];  // face (bottom, top), var (MOMX, MOMY), node
+  const int p = threadIdx.x;
+  const int ke = blockIdx.x;
+  const int ij = p % Nfp, k = p / Nfp;
+  const DevMesh& m = a.m;
+  const Consts& c = a.c;
+  const uint32_t o = (uint32_t)Np * ke + p;
+  const double gamm = c.CPdry / c.CVdry, rgamm = c.CVdry / c.CPdry;
+  const double PRES0ovRdry = c.PRES00 / c.Rdry, rP0 = 1.0 / c.PRES00;
+  if (k == 0 || k == N1 - 1) {
+    for (int f = 0; f < 2; ++f) {
+      if ((f == 0) != (k == 0)) continue;
+      const int fp = (4 + f) * Nfp + ij;  // reference face-node index
+      const uint32_t gfp = (uint32_t)fp + (uint32_t)(6 * Nfp) * ke;
+      const uint32_t iM = o, iP = vz_nbr(a, ke, f, ij, N1);
+      const double nz = ldg(m.nz, gfp);
+      const int ke2d = m.EMap3Dto2D[ke];
+      const double G11_ = ldg(m.G11, (uint32_t)ij + (uint32_t)Nfp * ke2d);
+      const double G12_ = ldg(m.G12, (uint32_t)ij + (uint32_t)Nfp * ke2d);
+      const double G22_ = ldg(m.G22, (uint32_t)ij + (uint32_t)Nfp * ke2d);
+      double wt0[2], pres0[2], rdens0[2], Gnn[2];
+      const uint32_t ix[2] = {iM, iP};
+      for (int s = 0; s < 2; ++s) {
+        const uint32_t n = ix[s];
+        const int ken = n / Np;
+        const double gam = 1.0;
+        const double rgam2 = 1.0 / (gam * gam);
+        const double RGsqrtV = 1.0 / gsqrtv(a, ken, n - (uint32_t)Np * ken, N1);
+        const double MOMX0 = ldg(a.q0[VU], n), MOMY0 = ldg(a.q0[VV], n), DRHOT0 = ldg(a.q0[VT], n);
+        const double Rt = ldg(a.Rtot, n), CPovCV = ldg(a.CPtot, n) / ldg(a.CVtot, n);
+        const double g13 = ldg(m.G13, n), g23 = ldg(m.G23, n);
+        rdens0[s] = 1.0 / (ldg(a.DENS_hyd, n) + ldg(a.q0[VD], n));
+        wt0[s] = (ldg(a.q0[VW], n) * RGsqrtV + g13 * MOMX0 + g23 * MOMY0) * rdens0[s];
+        const double rhot_hyd = PRES0ovRdry * fedg_pow(ldg(a.PRES_hyd, n) / c.PRES00, rgamm);
+        pres0[s] = c.PRES00 * fedg_pow(Rt * rP0 * (rhot_hyd + DRHOT0), CPovCV);
+        const double Gxz = rgam2 * (G11_ * g13 + G12_ * g23);
+        const double Gyz = rgam2 * (G12_ * g13 + G22_ * g23);
+        Gnn[s] = (1.0 * (RGsqrtV * RGsqrtV) + g13 * Gxz + g23 * Gyz) * fabs(nz);
+      }
+      const double sM = fabs(wt0[0]) + sqrt(Gnn[0] * gamm * pres0[0] * rdens0[0]);
+      const double sP = fabs(wt0[1]) + sqrt(Gnn[1] * gamm * pres0[1] * rdens0[1]);
+      const double al = (nz * nz) * (sM > sP ? sM : sP);
+      stg(a.alph, (uint32_t)(f * Nfp + ij) + (uint32_t)(2 * Nfp) * ke, al);
+      const double tmp1 = -0.5 * ldg(m.Fscale, gfp) * al;
+      sdf[f][0][ij] = tmp1 * (ldg(a.pv[VU], iP) - ldg(a.pv[VU], iM));
+      sdf[f][1][ij] = tmp1 * (ldg(a.pv[VV], iP) - ldg(a.pv[VV], iM));
+    }
+  }
+  __syncthreads();
+  // Lift of the vertical-face jumps (the lateral faces carry none)
+  const double L4 = ldg(m.Lift, (uint32_t)(p + 4 * Np)), L5 = ldg(m.Lift, (uint32_t)(p + 5 * Np));
+  const double LDx = L4 * sdf[0][0][ij] + L5 * sdf[1][0][ij];
+  const double LDy = L4 * sdf[0][1][ij] + L5 * sdf[1][1][ij];
+  const double RGsqrtV = 1.0 / gsqrtv(a, ke, p, N1);
+  const double Tx = -LDx * RGsqrtV, Ty = -LDy * RGsqrtV;
+  if (a.want_b) {
+    const uint32_t n = (uint32_t)Np * m.Ne;
+    stg(a.buv, o, a.impl_fac * Tx - ldg(a.pv[VU], o) + ldg(a.qs[VU], o));
+    stg(a.buv, o + n, a.impl_fac * Ty - ldg(a.pv[VV], o) + ldg(a.qs[VV], o));
+  } else {
+    stg(a.T[VU], o, Tx);
+    stg(a.T[VV], o, Ty);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3. vi_cal_del_flux_dyn (:1161-1328) + eval_Ax (:111-326)
+template <int N1>
+__global__ void __launch_bounds__(N1* N1* N1) hevi_rhs3_kernel(const HeviArgs a) {
+  constexpr int Np = N1 * N1 * N1, Nfp = N1 * N1;
+  __shared__ double sF[3][Np];      // DENS, MOMZ, RHOT vertical fluxes
+  __shared__ double sRho[Np];
+  __shared__ double sdf[2][3][Nfp];  // face (bottom, top), var (DENS, MOMZ, RHOT), node
+  __shared__ double sD[N1 * N1];
+  __shared__ double sI[N1 * N1];
+  const int p = threadIdx.x;
+  const int ke = blockIdx.x;
+  const int ij = p % Nfp, k = p / Nfp;
+  const DevMesh& m = a.m;
+  const Consts& c = a.c;
+  const uint32_t o = (uint32_t)Np * ke + p;
+  const int kz = ke / m.Ne2D;
+  const double rgamm = c.CVdry / c.CPdry;
+  const double PRES0ovRdry = c.PRES00 / c.Rdry, rP0 = 1.0 / c.PRES00, P0ovR = c.PRES00 / c.Rdry;
+  if (p < N1 * N1) {
+    sD[p] = m.D1D[p];
+    sI[p] = m.Itr[p];
+  }
+  // vertical face jumps of the iterate
+  if (k == 0 || k == N1 - 1) {
+    for (int f = 0; f < 2; ++f) {
+      if ((f == 0) != (k == 0)) continue;
+      const int fp = (4 + f) * Nfp + ij;
+      const uint32_t gfp = (uint32_t)fp + (uint32_t)(6 * Nfp) * ke;
+      const uint32_t iM = o, iP = vz_nbr(a, ke, f, ij, N1);
+      const uint32_t ix[2] = {iM, iP};
+      double DD[2], MZ[2], MW[2], DT[2], pott[2], dpres[2], GV[2];
+      for (int s = 0; s < 2; ++s) {
+        const uint32_t n = ix[s];
+        const int ken = n / Np;
+        GV[s] = gsqrtv(a, ken, n - (uint32_t)Np * ken, N1);
+        DD[s] = ldg(a.pv[VD], n);
+        MZ[s] = ldg(a.pv[VW], n);
+        DT[s] = ldg(a.pv[VT], n);
+        const double dens = ldg(a.DENS_hyd, n) + DD[s];
+        const double PH = ldg(a.PRES_hyd, n);
+        const double rhot_hyd = PRES0ovRdry * fedg_pow(PH / c.PRES00, rgamm);
+        pott[s] = (rhot_hyd + DT[s]) / dens;
+        dpres[s] = c.PRES00 * fedg_pow(ldg(a.Rtot, n) * rP0 * dens * pott[s], ldg(a.CPtot, n) / ldg(a.CVtot, n));
+        dpres[s] = dpres[s] - PH;
+        MW[s] = MZ[s] + GV[s] * ldg(m.G13, n) * ldg(a.pv[VU], n) + GV[s] * ldg(m.G23, n) * ldg(a.pv[VV], n);
+      }
+      if ((kz == 0 || kz == a.NeZ - 1) && iM == iP) {
+        MZ[1] = -MZ[0] - 2.0 * GV[0] * (ldg(m.G13, iM) * ldg(a.pv[VU], iM) + ldg(m.G23, iM) * ldg(a.pv[VV], iM));
+        MW[1] = -MW[0];
+      }
+      const double nz = ldg(m.nz, gfp);
+      const double al = ldg(a.alph, (uint32_t)(f * Nfp + ij) + (uint32_t)(2 * Nfp) * ke);
+      const double tmp1 = 0.5 * ldg(m.Fscale, gfp);
+      sdf[f][0][ij] = tmp1 * (+(MW[1] - MW[0]) * nz - al * (DD[1] - DD[0]));
+      sdf[f][1][ij] = tmp1 * (+(dpres[1] - dpres[0]) * nz - al * (MZ[1] - MZ[0]));
+      sdf[f][2][ij] = tmp1 * (+(pott[1] * MW[1] - pott[0] * MW[0]) * nz - al * (DT[1] - DT[0]));
+    }
+  }
+  // vertical fluxes of the node
+  const double GV = gsqrtv(a, ke, p, N1);
+  const double PVD = ldg(a.pv[VD], o), PVW = ldg(a.pv[VW], o), PVT = ldg(a.pv[VT], o);
+  const double FD = PVW + GV * ldg(m.G13, o) * ldg(a.pv[VU], o) + GV * ldg(m.G23, o) * ldg(a.pv[VV], o);
+  const double PH = ldg(a.PRES_hyd, o), DH = ldg(a.DENS_hyd, o);
+  const double RHOT = P0ovR * fedg_pow(PH / c.PRES00, rgamm) + PVT;
+  const double pt_ = RHOT / (PVD + DH);
+  const double Rt = ldg(a.Rtot, o), CPt = ldg(a.CPtot, o), CVt = ldg(a.CVtot, o);
+  const double FW = c.PRES00 * fedg_pow(Rt * rP0 * RHOT, CPt / CVt) - PH;
+  sF[0][p] = FD;
+  sF[1][p] = FW;
+  sF[2][p] = pt_ * FD;
+  sRho[p] = PVD;
+  __syncthreads();
+  // Dz (matvec_dirZ), VFilterPM1 and the lift of the vertical-face jumps
+  double dz[3];
+#pragma unroll
+  for (int v = 0; v < 3; ++v) {
+    double s = sF[v][ij] * sD[k];
+#pragma unroll
+    for (int l = 1; l < N1; ++l) s = s + sF[v][ij + Nfp * l] * sD[k + N1 * l];
+    dz[v] = s;
+  }
+  double drho = sRho[ij] * sI[N1 * k];
+#pragma unroll
+  for (int l = 1; l < N1; ++l) drho = drho + sRho[ij + Nfp * l] * sI[l + N1 * k];
+  const double L4 = ldg(m.Lift, (uint32_t)(p + 4 * Np)), L5 = ldg(m.Lift, (uint32_t)(p + 5 * Np));
+  const double E33 = ldg(m.E33, o);
+  const double RGsqrtV = 1.0 / GV;
+  const double TD = -(E33 * dz[0] + (L4 * sdf[0][0][ij] + L5 * sdf[1][0][ij])) * RGsqrtV;
+  const double TT = -(E33 * dz[2] + (L4 * sdf[0][2][ij] + L5 * sdf[1][2][ij])) * RGsqrtV;
+  const double TW = -(E33 * dz[1] + (L4 * sdf[0][1][ij] + L5 * sdf[1][1][ij])) * RGsqrtV - c.GRAV * drho;
+  if (a.want_b) {
+    const uint32_t n = (uint32_t)Np * m.Ne;
+    const double dens = DH + PVD;
+    stg(a.coef, o, RHOT / dens);
+    stg(a.coef, o + n, FD / dens);
+    const double CPovCV = CPt / CVt;
+    stg(a.coef, o + 2 * n, CPovCV * c.PRES00 * fedg_pow(Rt / c.PRES00 * RHOT, CPovCV) / RHOT);
+    stg(a.b3, o, a.impl_fac * TD - PVD + ldg(a.qs[VD], o));
+    stg(a.b3, o + n, a.impl_fac * TW - PVW + ldg(a.qs[VW], o));
+    stg(a.b3, o + 2 * n, a.impl_fac * TT - PVT + ldg(a.qs[VT], o));
+  } else {
+    stg(a.T[VD], o, TD);
+    stg(a.T[VW], o, TW);
+    stg(a.T[VT], o, TT);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront-cooperative pivoted LU + solve of solve_NnodeN_var3 / _uv
+// (FDS/scale_atm_dyn_dgm_hevi_common_linalgebra.F90:2142-2445) for an n x n
+// block A (column-major in LDS) and nr right-hand sides X (n x nr in LDS).
+// Must be called by all 64 lanes of a single-wavefront workgroup.
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) {
+    const double o = __shfl_xor(v, s, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+__device__ void lu_solve_wave(int n, double* A, int* ipiv, int nr, double* X) {
+  const int lane = threadIdx.x;
+  for (int k = 0; k < n; ++k) {
+    // pivot: the first row of max |A(i,k)|, i >= k, keeping k on ties; a NaN
+    // on the diagonal keeps k, NaN below it is never chosen
+    const double akk = fabs(A[k + n * k]);
+    double v = -1.0;
+    if (lane >= k && lane < n) {
+      const double t = fabs(A[lane + n * k]);
+      v = (t == t) ? t : -1.0;
+    }
+    int piv = k;
+    if (akk == akk) {
+      const double mx = wave_max(v);
+      const unsigned long long hit = __ballot(lane >= k && lane < n && v == mx);
+      piv = __ffsll((long long)hit) - 1;
+    }
+    if (lane == 0) ipiv[k] = piv;
+    if (piv != k && lane < n) {
+      const double t = A[k + n * lane];
+      A[k + n * lane] = A[piv + n * lane];
+      A[piv + n * lane] = t;
+    }
+    __syncthreads();
+    const double inv = 1.0 / A[k + n * k];
+    __syncthreads();
+    if (lane == k) A[k + n * k] = inv;
+    if (lane > k && lane < n) A[lane + n * k] = A[lane + n * k] * inv;
+    __syncthreads();
+    const int w = n - k - 1;
+    for (int e = lane; e < w * w; e += 64) {
+      const int i = k + 1 + e % w, j = k + 1 + e / w;
+      A[i + n * j] = A[i + n * j] - A[i + n * k] * A[k + n * j];
+    }
+    __syncthreads();
+  }
+  if (lane < nr) {
+    double* x = X + n * lane;
+    for (int i = 0; i < n - 1; ++i) {
+      const int ip = ipiv[i];
+      if (ip != i) {
+        const double t = x[i];
+        x[i] = x[ip];
+        x[ip] = t;
+      }
+    }
+    for (int i = 0; i < n; ++i) {
+      double t = x[i];
+      for (int j = 0; j < i; ++j) t = t - x[j] * A[i + n * j];
+      x[i] = t;
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double t = x[i];
+      for (int j = i + 1; j < n; ++j) t = t - A[i + n * j] * x[j];
+      x[i] = t * A[i + n * i];
+    }
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// 2. construct_matbnd_uv (:905-1011) + solve_uv (:589-689), one line per workgroup
+template <int N1>
+__global__ void __launch_bounds__(64) hevi_solve_uv_kernel(const HeviArgs a) {
+  constexpr int N = N1, Nfp = N1 * N1, Np = N1 * N1 * N1;
+  extern __shared__ double lds[];
+  double* A = lds;                  // N*N
+  double* Lv = A + N * N;           // N
+  double* X = Lv + N;               // NeZ * 3 * N: b(:,1), b(:,2), G per level
+  int* ipiv = reinterpret_cast<int*>(X + (size_t)a.NeZ * 3 * N);
+  const int lane = threadIdx.x;
+  const DevMesh& m = a.m;
+  const int nline = m.Ne2D * Nfp;
+  const int NeZ = a.NeZ;
+  const uint32_t nn = (uint32_t)Np * m.Ne;
+  for (int line = blockIdx.x; line < nline; line += gridDim.x) {
+    const int ke2D = line / Nfp, ij = line % Nfp;
+    for (int kz = 0; kz < NeZ; ++kz) {
+      const int ke = ke2D + kz * m.Ne2D;
+      double* Xk = X + (size_t)kz * 3 * N;
+      // face damping of the rows: t[f][pv] for the two vertical faces
+      for (int e = lane; e < N * N; e += 64) A[e] = (e % N == e / N) ? 1.0 : 0.0;
+      if (lane < N) {
+        const uint32_t q = (uint32_t)Np * ke + ij + Nfp * lane;
+        Xk[lane] = ldg(a.buv, q);
+        Xk[lane + N] = ldg(a.buv, q + nn);
+      }
+      __syncthreads();
+      for (int f1 = 1; f1 <= 2; ++f1) {
+        const int kz2 = (f1 == 1) ? (kz - 1 > 0 ? kz - 1 : 0) : (kz + 1 < NeZ - 1 ? kz + 1 : NeZ - 1);
+        const int pv1 = (f1 == 1) ? 0 : N - 1;
+        int f2 = (f1 == 1) ? 2 : 1;
+        const bool bc = (kz == 0 && f1 == 1) || (kz == NeZ - 1 && f1 == 2);
+        if (bc) continue;
+        const int ke_2 = ke2D + kz2 * m.Ne2D;
+        if (lane < N) {
+          const int pv = lane;
+          const int p = ij + Nfp * pv;
+          const uint32_t gfp = (uint32_t)((3 + f1) * Nfp + ij) + (uint32_t)(6 * Nfp) * ke;
+          const double fac = 0.5 * a.impl_fac / gsqrtv(a, ke, p, N1);
+          const double am = ldg(a.alph, (uint32_t)((f1 - 1) * Nfp + ij) + (uint32_t)(2 * Nfp) * ke);
+          const double ap = ldg(a.alph, (uint32_t)((f2 - 1) * Nfp + ij) + (uint32_t)(2 * Nfp) * ke_2);
+          const double t = fac * ldg(m.Lift, (uint32_t)(p + (3 + f1) * Np)) * ldg(m.Fscale, gfp) * (am > ap ? am : ap);
+          A[pv + N * pv1] = A[pv + N * pv1] + t;
+          if (f1 == 1) Lv[pv] = -t;
+          else Xk[pv + 2 * N] = -t;  // U lands in the G slot
+        }
+        __syncthreads();
+      }
+      if (kz > 0) {
+        const double* Xp = X + (size_t)(kz - 1) * 3 * N;
+        if (lane < N) {
+          const double tmp = Lv[lane];
+          A[lane] = A[lane] - tmp * Xp[(N - 1) + 2 * N];
+          Xk[lane] = Xk[lane] - tmp * Xp[N - 1];
+          Xk[lane + N] = Xk[lane + N] - tmp * Xp[(N - 1) + N];
+        }
+        __syncthreads();
+      }
+      lu_solve_wave(N, A, ipiv, kz == NeZ - 1 ? 2 : 3, Xk);
+    }
+    for (int kz = NeZ - 2; kz >= 0; --kz) {
+      double* Xk = X + (size_t)kz * 3 * N;
+      const double* Xn = X + (size_t)(kz + 1) * 3 * N;
+      if (lane < N) {
+        const double tmp = Xk[lane + 2 * N];
+        Xk[lane] = Xk[lane] - tmp * Xn[0];
+        Xk[lane + N] = Xk[lane + N] - tmp * Xn[N];
+      }
+      __syncthreads();
+    }
+    for (int e = lane; e < NeZ * N; e += 64) {
+      const int kz = e / N, pv = e % N;
+      const uint32_t q = (uint32_t)Np * (ke2D + kz * m.Ne2D) + ij + Nfp * pv;
+      const double* Xk = X + (size_t)kz * 3 * N;
+      stg(a.pv[VU], q, ldg(a.pv[VU], q) + Xk[pv]);
+      stg(a.pv[VV], q, ldg(a.pv[VV], q) + Xk[pv + N]);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 4. construct_matbnd (:693-901) + solve (:329-479), one line per workgroup.
+// Unknowns of an element: (var, pv), var fastest: DENS, MOMZ, RHOT.
+template <int N1>
+__global__ void __launch_bounds__(64) hevi_solve3_kernel(const HeviArgs a) {
+  constexpr int N = N1, Nfp = N1 * N1, Np = N1 * N1 * N1, n3 = 3 * N1;
+  extern __shared__ double lds[];
+  double* A = lds;                    // n3*n3
+  double* Lm = A + n3 * n3;           // n3*3
+  double* t12 = Lm + n3 * 3;          // [face][tmp1/tmp2][pv]: 4*N
+  double* cf = t12 + 4 * N;           // coefficients of the level and its neighbours: [3 levels][3][N]
+  double* X = cf + 9 * N;             // NeZ * 4 * n3: b, G(:,1..3) per level
+  int* ipiv = reinterpret_cast<int*>(X + (size_t)a.NeZ * 4 * n3);
+  const int lane = threadIdx.x;
+  const DevMesh& m = a.m;
+  const int nline = m.Ne2D * Nfp;
+  const int NeZ = a.NeZ;
+  const uint32_t nn = (uint32_t)Np * m.Ne;
+  const double impl_fac = a.impl_fac;
+  for (int line = blockIdx.x; line < nline; line += gridDim.x) {
+    const int ke2D = line / Nfp, ij = line % Nfp;
+    for (int kz = 0; kz < NeZ; ++kz) {
+      const int ke = ke2D + kz * m.Ne2D;
+      double* Xk = X + (size_t)kz * 4 * n3;
+      // POT, WT, DPDRHOT of the line in this level (slot 1) and the levels below (0) / above (2)
+      for (int e = lane; e < 9 * N; e += 64) {
+        const int lv = e / (3 * N), r = e % (3 * N), c = r / N, pv = r % N;
+        int kzz = kz + lv - 1;
+        kzz = kzz < 0 ? 0 : (kzz > NeZ - 1 ? NeZ - 1 : kzz);
+        const uint32_t q = (uint32_t)Np * (ke2D + kzz * m.Ne2D) + ij + Nfp * pv;
+        cf[e] = ldg(a.coef, q + (uint32_t)c * nn);
+      }
+      if (lane < n3) {
+        const int pv = lane / 3, v = lane % 3;
+        Xk[lane] = ldg(a.b3, (uint32_t)Np * ke + ij + Nfp * pv + (uint32_t)v * nn);
+      }
+      // tmp1, tmp2 of both vertical faces
+      if (lane < 2 * N) {
+        const int f1 = 1 + lane / N, pv = lane % N;
+        const int kz2 = (f1 == 1) ? (kz - 1 > 0 ? kz - 1 : 0) : (kz + 1 < NeZ - 1 ? kz + 1 : NeZ - 1);
+        const bool bc = (kz == 0 && f1 == 1) || (kz == NeZ - 1 && f1 == 2);
+        const int f2 = bc ? f1 : ((f1 == 1) ? 2 : 1);
+        const int ke_2 = ke2D + kz2 * m.Ne2D;
+        const int p = ij + Nfp * pv;
+        const uint32_t gfp = (uint32_t)((3 + f1) * Nfp + ij) + (uint32_t)(6 * Nfp) * ke;
+        const double Lift = ldg(m.Lift, (uint32_t)(p + (3 + f1) * Np));
+        const double fac = 0.5 * impl_fac / gsqrtv(a, ke, p, N1) * Lift * ldg(m.Fscale, gfp);
+        const double am = ldg(a.alph, (uint32_t)((f1 - 1) * Nfp + ij) + (uint32_t)(2 * Nfp) * ke);
+        const double ap = ldg(a.alph, (uint32_t)((f2 - 1) * Nfp + ij) + (uint32_t)(2 * Nfp) * ke_2);
+        t12[(f1 - 1) * 2 * N + pv] = fac * (am > ap ? am : ap);
+        t12[(f1 - 1) * 2 * N + N + pv] = fac * ldg(m.nz, gfp);
+      }
+      __syncthreads();
+      const double* POT0 = cf + 3 * N;  // this level
+      const double* WT0 = POT0 + N;
+      const double* DPD0 = WT0 + N;
+      // D block with the face terms of its first / last node column
+      for (int e = lane; e < n3 * n3; e += 64) {
+        const int r = e % n3, cc = e / n3;
+        const int v = r % 3, pv = r / 3, v2 = cc % 3, pv2 = cc / 3;
+        const uint32_t prow = (uint32_t)Np * ke + ij + Nfp * pv;
+        const double Id = (pv == pv2) ? 1.0 : 0.0;
+        const double fdz = ldg(m.E33, prow) / gsqrtv(a, ke, ij + Nfp * pv, N1) * (impl_fac * m.D1D[pv + N * pv2]);
+        const double POT = POT0[pv2], WT = WT0[pv2], DPD = DPD0[pv2];
+        double d;
+        if (v == 0) d = (v2 == 0) ? Id : (v2 == 1 ? fdz : 0.0);
+        else if (v == 1) d = (v2 == 1) ? Id : (v2 == 0 ? impl_fac * a.c.GRAV * m.Itr[pv2 + N * pv] : fdz * DPD);
+        else d = (v2 == 0) ? -fdz * POT * WT : (v2 == 1 ? fdz * POT : Id + fdz * WT);
+        for (int f1 = 1; f1 <= 2; ++f1) {
+          const int pv1 = (f1 == 1) ? 0 : N - 1;
+          if (pv2 != pv1) continue;
+          const bool bc = (kz == 0 && f1 == 1) || (kz == NeZ - 1 && f1 == 2);
+          const double tmp1 = t12[(f1 - 1) * 2 * N + pv], tmp2 = t12[(f1 - 1) * 2 * N + N + pv];
+          if (bc) {
+            if (v == 2 && v2 == 0) d = d + 2.0 * tmp2 * POT * WT;
+            else if (v == 0 && v2 == 1) d = d - 2.0 * tmp2;
+            else if (v == 1 && v2 == 1) d = d + 2.0 * tmp1;
+            else if (v == 2 && v2 == 1) d = d - 2.0 * tmp2 * POT;
+            else if (v == 2 && v2 == 2) d = d - 2.0 * tmp2 * WT;
+          } else {
+            if (v == 0 && v2 == 0) d = d + tmp1;
+            else if (v == 2 && v2 == 0) d = d + tmp2 * POT * WT;
+            else if (v == 0 && v2 == 1) d = d - tmp2;
+            else if (v == 1 && v2 == 1) d = d + tmp1;
+            else if (v == 2 && v2 == 1) d = d - tmp2 * POT;
+            else if (v == 1 && v2 == 2) d = d - tmp2 * DPD;
+            else if (v == 2 && v2 == 2) d = d + tmp1 - tmp2 * WT;
+          }
+        }
+        A[e] = d;
+      }
+      // L (rows couple to the top node of the level below) and U (bottom node
+      // of the level above, stored in the G slots of this level)
+      for (int e = lane; e < 2 * n3 * 3; e += 64) {
+        const int f1 = 1 + e / (n3 * 3), r = (e % (n3 * 3)) % n3, v2 = (e % (n3 * 3)) / n3;
+        const bool bc = (kz == 0 && f1 == 1) || (kz == NeZ - 1 && f1 == 2);
+        if (bc) continue;
+        const int v = r % 3, pv = r / 3;
+        const int pv2 = (f1 == 1) ? N - 1 : 0;
+        const double* cfo = cf + (f1 == 1 ? 0 : 6 * N);  // level below / above
+        const double POT = cfo[pv2], WT = cfo[N + pv2], DPD = cfo[2 * N + pv2];
+        const double tmp1 = t12[(f1 - 1) * 2 * N + pv], tmp2 = t12[(f1 - 1) * 2 * N + N + pv];
+        double l;
+        if (v2 == 0) l = (v == 0) ? -tmp1 : (v == 1 ? 0.0 : -tmp2 * POT * WT);
+        else if (v2 == 1) l = (v == 0) ? +tmp2 : (v == 1 ? -tmp1 : +tmp2 * POT);
+        else l = (v == 0) ? 0.0 : (v == 1 ? +tmp2 * DPD : -tmp1 + tmp2 * WT);
+        if (f1 == 1) Lm[r + n3 * v2] = l;
+        else Xk[r + n3 * (1 + v2)] = l;
+      }
+      __syncthreads();
+      if (kz > 0) {
+        const double* Xp = X + (size_t)(kz - 1) * 4 * n3;
+        const int p1 = 3 * (N - 1), p2 = p1 + 1, p3 = p1 + 2;
+        if (lane < n3) {
+          const int r = lane;
+          const double t1 = Lm[r], t2 = Lm[r + n3], t3 = Lm[r + 2 * n3];
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            A[r + n3 * c] = A[r + n3 * c] - t1 * Xp[p1 + n3 * (1 + c)] - t2 * Xp[p2 + n3 * (1 + c)] -
+                            t3 * Xp[p3 + n3 * (1 + c)];
+          Xk[r] = Xk[r] - t1 * Xp[p1] - t2 * Xp[p2] - t3 * Xp[p3];
+        }
+        __syncthreads();
+      }
+      lu_solve_wave(n3, A, ipiv, kz == NeZ - 1 ? 1 : 4, Xk);
+    }
+    for (int kz = NeZ - 2; kz >= 0; --kz) {
+      double* Xk = X + (size_t)kz * 4 * n3;
+      const double* Xn = X + (size_t)(kz + 1) * 4 * n3;
+      if (lane < n3) {
+        const int r = lane;
+        Xk[r] = Xk[r] - Xk[r + n3] * Xn[0] - Xk[r + 2 * n3] * Xn[1] - Xk[r + 3 * n3] * Xn[2];
+      }
+      __syncthreads();
+    }
+    for (int e = lane; e < NeZ * N; e += 64) {
+      const int kz = e / N, pv = e % N;
+      const uint32_t q = (uint32_t)Np * (ke2D + kz * m.Ne2D) + ij + Nfp * pv;
+      const double* Xk = X + (size_t)kz * 4 * n3;
+      stg(a.pv[VD], q, ldg(a.pv[VD], q) + Xk[0 + 3 * pv]);
+      stg(a.pv[VW], q, ldg(a.pv[VW], q) + Xk[1 + 3 * pv]);
+      stg(a.pv[VT], q, ldg(a.pv[VT], q) + Xk[2 + 3 * pv]);
+    }
+    __syncthreads();
+  }
+}
+
+// tendency = (iterate - q*) / impl_fac  (..._rhot_hevi.F90:937-947)
+__global__ void hevi_tend_kernel(double* T0, double* T1, double* T2, double* T3, double* T4, const double* p0,
+                                 const double* p1, const double* p2, const double* p3, const double* p4,
+                                 const double* q0, const double* q1, const double* q2, const double* q3,
+                                 const double* q4, double impl_fac, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  T0[t] = (p0[t] - q0[t]) / impl_fac;
+  T1[t] = (p1[t] - q1[t]) / impl_fac;
+  T2[t] = (p2[t] - q2[t]) / impl_fac;
+  T3[t] = (p3[t] - q3[t]) / impl_fac;
+  T4[t] = (p4[t] - q4[t]) / impl_fac;
+}
+
+// sum of squares of a vector, per-block partial sums (residual norms)
+__global__ void sumsq_kernel(const double* x, long n, double* partial) {
+  __shared__ double s[256];
+  double acc = 0.0;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+    acc += x[t] * x[t];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+template <int N1>
+size_t lds_uv(int NeZ) { return sizeof(double) * (N1 * N1 + N1 + (size_t)NeZ * 3 * N1) + sizeof(int) * 64; }
+template <int N1>
+size_t lds_3(int NeZ) {
+  constexpr int n3 = 3 * N1;
+  return sizeof(double) * (n3 * n3 + n3 * 3 + 4 * N1 + 9 * N1 + (size_t)NeZ * 4 * n3) + sizeof(int) * 64;
+}
+
+template <int N1>
+int run_newton(fedg_ctx* ctx, HeviArgs& a, int max_it, double rtol, double* resid, double* partial,
+               double* partial_h, int* it_done) {
+  const DevMesh& m = ctx->m;
+  const int nline = m.Ne2D * N1 * N1;
+  const int nb = nline < 4096 ? nline : 4096;
+  const size_t luv = lds_uv<N1>(a.NeZ), l3 = lds_3<N1>(a.NeZ);
+  if (l3 > 64 * 1024) {
+    set_error("HEVI: too many elements in the vertical for the line solve's LDS (NeZ)");
+    return FEDG_ERR_UNSUPPORTED;
+  }
+  const long n = (long)m.Np * m.Ne;
+  double r0 = 0.0;
+  int it = 0;
+  for (; it < max_it; ++it) {
+    hipLaunchKernelGGL(hevi_rhs_uv_kernel<N1>, dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
+    FEDG_HIP_CHECK(hipGetLastError());
+    double r2 = 0.0;
+    const bool track = rtol > 0.0 || resid != nullptr;
+    if (track) {
+      hipLaunchKernelGGL(sumsq_kernel, dim3(256), dim3(256), 0, ctx->stream, a.buv, 2 * n, partial);
+      FEDG_HIP_CHECK(hipGetLastError());
+      FEDG_HIP_CHECK(hipMemcpyAsync(partial_h, partial, 256 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+      for (int b = 0; b < 256; ++b) r2 += partial_h[b];
+    }
+    hipLaunchKernelGGL(hevi_solve_uv_kernel<N1>, dim3(nb), dim3(64), luv, ctx->stream, a);
+    FEDG_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(hevi_rhs3_kernel<N1>, dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
+    FEDG_HIP_CHECK(hipGetLastError());
+    if (track) {
+      hipLaunchKernelGGL(sumsq_kernel, dim3(256), dim3(256), 0, ctx->stream, a.b3, 3 * n, partial);
+      FEDG_HIP_CHECK(hipGetLastError());
+      FEDG_HIP_CHECK(hipMemcpyAsync(partial_h, partial, 256 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+      for (int b = 0; b < 256; ++b) r2 += partial_h[b];
+      const double r = sqrt(r2);
+      if (resid) resid[it] = r;
+      if (it == 0) r0 = r;
+      if (rtol > 0.0 && it > 0 && r <= rtol * r0) break;
+    }
+    hipLaunchKernelGGL(hevi_solve3_kernel<N1>, dim3(nb), dim3(64), l3, ctx->stream, a);
+    FEDG_HIP_CHECK(hipGetLastError());
+  }
+  *it_done = it;
+  return FEDG_OK;
+}
+
+template <int N1>
+int run_operator(fedg_ctx* ctx, HeviArgs& a) {
+  const DevMesh& m = ctx->m;
+  hipLaunchKernelGGL(hevi_rhs_uv_kernel<N1>, dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
+  FEDG_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(hevi_rhs3_kernel<N1>, dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
+  FEDG_HIP_CHECK(hipGetLastError());
+  return FEDG_OK;
+}
+
+}  // namespace
+
+int hevi_cal_vi(fedg_ctx* ctx, HeviWork* w, double* const tend[5], const double* const q[5],
+                const double* const var0[5], const double* DENS_hyd, const double* PRES_hyd, const double* Rtot,
+                const double* CVtot, const double* CPtot, double impl_fac, int max_it, double rtol, double* resid,
+                int* it_done) {
+  const DevMesh& m = ctx->m;
+  HeviArgs a{};
+  a.m = m;
+  a.c = ctx->c;
+  for (int v = 0; v < 5; ++v) {
+    a.qs[v] = q[v];
+    a.q0[v] = var0[v];
+    a.pv[v] = w->pv[v];
+    a.T[v] = tend[v];
+  }
+  a.DENS_hyd = DENS_hyd;
+  a.PRES_hyd = PRES_hyd;
+  a.Rtot = Rtot;
+  a.CVtot = CVtot;
+  a.CPtot = CPtot;
+  a.alph = w->alph;
+  a.b3 = w->b3;
+  a.buv = w->buv;
+  a.coef = w->coef;
+  a.impl_fac = impl_fac;
+  a.NeZ = m.Ne / m.Ne2D;
+  const long n = (long)m.Np * m.Ne;
+  *it_done = 0;
+  for (int v = 0; v < 5; ++v)
+    FEDG_HIP_CHECK(hipMemcpyAsync(w->pv[v], var0[v], sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+  auto dispatch = [&](auto n1c) -> int {
+    constexpr int N1 = decltype(n1c)::value;
+    if (fabs(impl_fac) > 0.0) {
+      a.want_b = 1;
+      const int rc = run_newton<N1>(ctx, a, max_it, rtol, resid, w->partial, w->partial_h, it_done);
+      if (rc != FEDG_OK) return rc;
+      const int bs = 256;
+      hipLaunchKernelGGL(hevi_tend_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, ctx->stream, tend[VD], tend[VT],
+                         tend[VW], tend[VU], tend[VV], w->pv[VD], w->pv[VT], w->pv[VW], w->pv[VU], w->pv[VV], q[VD],
+                         q[VT], q[VW], q[VU], q[VV], impl_fac, n);
+      FEDG_HIP_CHECK(hipGetLastError());
+      return FEDG_OK;
+    }
+    a.want_b = 0;
+    return run_operator<N1>(ctx, a);
+  };
+  switch (m.n1) {
+    case 2: return dispatch(std::integral_constant<int, 2>{});
+    case 3: return dispatch(std::integral_constant<int, 3>{});
+    case 4: return dispatch(std::integral_constant<int, 4>{});
+    case 5: return dispatch(std::integral_constant<int, 5>{});
+    case 6: return dispatch(std::integral_constant<int, 6>{});
+    case 7: return dispatch(std::integral_constant<int, 7>{});
+    case 8: return dispatch(std::integral_constant<int, 8>{});
+    default:
+      set_error("polynomial order not built (supported: p = 1..7)");
+      return FEDG_ERR_UNSUPPORTED;
+  }
+}
+
+}  // namespace fedg

@@ -64,11 +64,16 @@ struct DevMesh {
 
 }  // namespace fedg
 
+namespace fedg {
+struct HeviWork;
+}
+
 struct fedg_ctx {
   fedg::DevMesh m;
   fedg::Consts c;
   hipStream_t stream;
   int device;
+  fedg::HeviWork* hevi;  // implicit-solve workspace, allocated on first use
 };
 
 namespace fedg {
@@ -98,6 +103,7 @@ struct RkOut {
 };
 
 int launch_heve_dropin(fedg_ctx* ctx, const TendIn& in, double* const out[5]);
+int launch_hevi_explicit(fedg_ctx* ctx, const TendIn& in, double* const out[5]);
 int launch_heve_fused(fedg_ctx* ctx, const TendIn& in, const RkOut& rk, int use_flat);
 // DPRES of a state (dry constants when Rtot is null), as DRHOT2PRES computes it
 int launch_dpres(fedg_ctx* ctx, double* DPRES, const double* DRHOT, const double* PRES_hyd,

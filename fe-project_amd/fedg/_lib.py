@@ -83,6 +83,10 @@ def _load():
         "fedg_apply_bc_prgvars": (C.c_int, [vp] + [vp] * 7),
         "fedg_heve_cal_tend": (C.c_int, [vp] + [vp] * 21),
         "fedg_calc_phyd_hgrad": (C.c_int, [vp] + [vp] * 4),
+        "fedg_hevi_cal_tend": (C.c_int, [vp] + [vp] * 21),
+        "fedg_hevi_cal_vi": (C.c_int, [vp] + [vp] * 20 + [C.c_double, C.c_double]),
+        "fedg_hevi_cal_vi_newton": (C.c_int, [vp] + [vp] * 20 + [C.c_double, C.c_double, C.c_int, C.c_double,
+                                                                 C.POINTER(C.c_double), C.POINTER(C.c_int)]),
         "fedg_tint_create": (C.c_int, [vp, C.c_char_p, C.c_double, C.POINTER(vp)]),
         "fedg_tint_destroy": (C.c_int, [vp]),
         "fedg_tint_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -114,7 +118,8 @@ EXPORTED = (
     "fedg_last_error fedg_version fedg_dev_alloc fedg_dev_free fedg_memcpy_h2d fedg_memcpy_d2h "
     "fedg_memset_dev fedg_device_sync fedg_device_name fedg_refelem_tables fedg_box_get_sizes "
     "fedg_box_build fedg_ctx_create fedg_ctx_destroy fedg_ctx_sizes fedg_drhot2pres fedg_halo_exchange "
-    "fedg_apply_bc_prgvars fedg_heve_cal_tend fedg_calc_phyd_hgrad fedg_tint_create fedg_tint_destroy "
+    "fedg_apply_bc_prgvars fedg_heve_cal_tend fedg_hevi_cal_tend fedg_hevi_cal_vi fedg_hevi_cal_vi_newton "
+    "fedg_calc_phyd_hgrad fedg_tint_create fedg_tint_destroy "
     "fedg_tint_info fedg_tint_tend_ex fedg_tint_tend_im fedg_tint_store_var0 fedg_tint_advance "
     "fedg_tint_store_implicit fedg_heve_create fedg_heve_destroy fedg_heve_is_flat fedg_heve_set_state "
     "fedg_heve_get_state fedg_heve_state_dev fedg_heve_run fedg_heve_run_timed fedg_heve_get_pressure"

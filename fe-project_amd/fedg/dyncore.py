@@ -74,6 +74,33 @@ def heve_cal_tend(dm: DeviceMesh, out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DPRES, DE
     check(LIB.fedg_heve_cal_tend(dm.handle, *args), "fedg_heve_cal_tend")
 
 
+def hevi_cal_tend(dm: DeviceMesh, out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DPRES, DENS_hyd, PRES_hyd, PRES_hyd_ref,
+                  THERM_hyd, CORIOLIS, Rtot, CVtot, CPtot, DPhydDx, DPhydDy):
+    """Explicit HEVI tendencies; out = (DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt): device fields."""
+    args = [_p(a) for a in (*out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DPRES, DENS_hyd, PRES_hyd, PRES_hyd_ref, THERM_hyd,
+                            CORIOLIS, Rtot, CVtot, CPtot, DPhydDx, DPhydDy)]
+    check(LIB.fedg_hevi_cal_tend(dm.handle, *args), "fedg_hevi_cal_tend")
+
+
+def hevi_cal_vi(dm: DeviceMesh, out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DENS_hyd, PRES_hyd, DDENS0, MOMX0, MOMY0,
+                MOMZ0, DRHOT0, Rtot, CVtot, CPtot, impl_fac, dt, max_it=None, rtol=0.0):
+    """Implicit HEVI tendencies into out = (DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt) (device fields).
+
+    max_it None: the reference's single Newton iteration (fedg_hevi_cal_vi).
+    Otherwise Newton to rtol (fedg_hevi_cal_vi_newton); returns (iterations, residual norms)."""
+    args = [_p(a) for a in (*out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DENS_hyd, PRES_hyd, DDENS0, MOMX0, MOMY0, MOMZ0,
+                            DRHOT0, Rtot, CVtot, CPtot)]
+    if max_it is None:
+        check(LIB.fedg_hevi_cal_vi(dm.handle, *args, float(impl_fac), float(dt)), "fedg_hevi_cal_vi")
+        return 1, None
+    resid = (C.c_double * max(int(max_it), 1))()
+    it = C.c_int(0)
+    check(LIB.fedg_hevi_cal_vi_newton(dm.handle, *args, float(impl_fac), float(dt), int(max_it), float(rtol), resid,
+                                      C.byref(it)), "fedg_hevi_cal_vi_newton")
+    n = min(it.value + 1, int(max_it))
+    return it.value, np.array(resid[:n])
+
+
 def calc_pressure(dm: DeviceMesh, PRES, DPRES, DRHOT, PRES_hyd, THERM_hyd, Rtot, CVtot, CPtot):
     check(LIB.fedg_drhot2pres(dm.handle, *[_p(a) for a in (PRES, DPRES, DRHOT, PRES_hyd, THERM_hyd, Rtot, CVtot,
                                                            CPtot)]), "fedg_drhot2pres")

@@ -186,6 +186,51 @@ int fedg_heve_cal_tend(fedg_ctx *ctx, double *DENS_dt, double *MOMX_dt, double *
                        const double *Rtot, const double *CVtot, const double *CPtot,
                        const double *DPhydDx, const double *DPhydDy);
 
+/* atm_dyn_dgm_nonhydro3d_rhot_hevi_cal_tend
+ * (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi.F90:289-482 with its numerical
+ * flux FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi_numflux.F90:232-416): the
+ * explicit tendency of the HEVI scheme; same arguments as fedg_heve_cal_tend
+ * (the cal_tend_ex interface, FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:152-199). */
+int fedg_hevi_cal_tend(fedg_ctx *ctx, double *DENS_dt, double *MOMX_dt, double *MOMY_dt,
+                       double *MOMZ_dt, double *RHOT_dt, const double *DDENS_, const double *MOMX_,
+                       const double *MOMY_, const double *MOMZ_, const double *DRHOT_,
+                       const double *DPRES_, const double *DENS_hyd, const double *PRES_hyd,
+                       const double *PRES_hyd_ref, const double *THERM_hyd, const double *CORIOLIS,
+                       const double *Rtot, const double *CVtot, const double *CPtot,
+                       const double *DPhydDx, const double *DPhydDy);
+
+/* atm_dyn_dgm_nonhydro3d_rhot_hevi_cal_vi
+ * (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi.F90:772-965, with
+ * FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi_common_2.F90:111-1328 and the
+ * block LU of FDS/scale_atm_dyn_dgm_hevi_common_linalgebra.F90:2142-2445): the
+ * implicit tendencies of the vertical terms, one Newton iteration from var0
+ * (DDENS0_ .. DRHOT0_ = q^n) towards q = q* + impl_fac T_v(q), q* the stage
+ * state; impl_fac = 0 returns T_v(var0).  Arguments in the order of the
+ * abstract interface atm_dyn_nonhydro3d_cal_vi
+ * (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:201-250) minus the derived
+ * types; fields (Np,NeA) on the device; the mesh must be one box column
+ * stack (elements numbered horizontally first). */
+int fedg_hevi_cal_vi(fedg_ctx *ctx, double *DENS_dt, double *MOMX_dt, double *MOMY_dt, double *MOMZ_dt,
+                     double *RHOT_dt, const double *DDENS_, const double *MOMX_, const double *MOMY_,
+                     const double *MOMZ_, const double *DRHOT_, const double *DENS_hyd,
+                     const double *PRES_hyd, const double *DDENS0_, const double *MOMX0_,
+                     const double *MOMY0_, const double *MOMZ0_, const double *DRHOT0_,
+                     const double *Rtot, const double *CVtot, const double *CPtot, double impl_fac,
+                     double dt);
+
+/* No counterpart in the reference (which stops after one iteration,
+ * ..._rhot_hevi.F90:885): fedg_hevi_cal_vi iterated until the residual norm
+ * ||G_k|| <= rtol ||G_0|| or max_it iterations; resid (host, max_it) gets the
+ * norms, *iterations the number of Newton updates applied. */
+int fedg_hevi_cal_vi_newton(fedg_ctx *ctx, double *DENS_dt, double *MOMX_dt, double *MOMY_dt,
+                            double *MOMZ_dt, double *RHOT_dt, const double *DDENS_, const double *MOMX_,
+                            const double *MOMY_, const double *MOMZ_, const double *DRHOT_,
+                            const double *DENS_hyd, const double *PRES_hyd, const double *DDENS0_,
+                            const double *MOMX0_, const double *MOMY0_, const double *MOMZ0_,
+                            const double *DRHOT0_, const double *Rtot, const double *CVtot,
+                            const double *CPtot, double impl_fac, double dt, int max_it, double rtol,
+                            double *resid, int *iterations);
+
 /* atm_dyn_dgm_nonhydro3d_common_calc_phyd_hgrad_lc
  * (FDS/scale_atm_dyn_dgm_nonhydro3d_common.F90:624-777), halos of PRES_hyd and
  * PRES_hyd_ref filled. */
