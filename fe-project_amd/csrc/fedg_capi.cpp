@@ -1096,4 +1096,215 @@ int fedg_heve_get_pressure(fedg_heve* st, double* PRES_host, double* DPRES_host)
   return rc;
 }
 
+
+// ---------------------------------------------------------------------------
+// HEVI step driver (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:694-870, the
+// hevi_flag branch): per stage the implicit tendency (cal_vi), StoreImplicit,
+// halo exchange, pressure, boundary conditions, the explicit HEVI tendency
+// and the IMEX advance, all on the context's stream.
+struct fedg_hevi {
+  fedg_ctx* ctx;
+  fedg_tint* tint;
+  double* block;
+  double* q[5];  // (Np,NeA) state, VID order
+  double *PRES, *DPRES;
+  double *DENS_hyd, *PRES_hyd, *THERM_hyd, *DPhydDx, *DPhydDy, *Rtot, *CVtot, *CPtot, *CORIOLIS;
+  int max_it;
+  double rtol;
+  long newton_iters, newton_calls;
+};
+
+int fedg_hevi_destroy(fedg_hevi* st) {
+  if (!st) return FEDG_OK;
+  if (st->tint) fedg_tint_destroy(st->tint);
+  if (st->block) (void)hipFree(st->block);
+  delete st;
+  return FEDG_OK;
+}
+
+int fedg_hevi_create(fedg_ctx* ctx, const char* scheme, double dt, const double* DENS_hyd, const double* PRES_hyd,
+                     const double* THERM_hyd, const double* CORIOLIS, const double* DPhydDx, const double* DPhydDy,
+                     const double* Rtot, const double* CVtot, const double* CPtot, int max_it, double rtol,
+                     fedg_hevi** out) {
+  if (!ctx || !scheme || !out || !DENS_hyd || !PRES_hyd || !THERM_hyd || !CORIOLIS || !DPhydDx || !DPhydDy ||
+      !Rtot || !CVtot || !CPtot || max_it < 1)
+    return FEDG_ERR_ARG;
+  *out = nullptr;
+  fedg_hevi* st = new fedg_hevi();
+  memset(st, 0, sizeof(*st));
+  st->ctx = ctx;
+  st->max_it = max_it;
+  st->rtol = rtol;
+  int rc = fedg_tint_create(ctx, scheme, dt, &st->tint);
+  if (rc == FEDG_OK && !st->tint->imex) {
+    set_error("the HEVI step takes an IMEX scheme (IMEX_ARK232, IMEX_ARK324)");
+    rc = FEDG_ERR_ARG;
+  }
+  HeviWork* w = nullptr;
+  if (rc == FEDG_OK) rc = hevi_work_get(ctx, &w);
+  const DevMesh& m = ctx->m;
+  const size_t nA = (size_t)m.Np * m.NeA, cor = (size_t)m.Nfp * m.Ne2DA;
+  if (rc == FEDG_OK) rc = dev_upload<double>(&st->block, nullptr, nA * 15 + cor);
+  if (rc != FEDG_OK) {
+    fedg_hevi_destroy(st);
+    return rc;
+  }
+  double* p = st->block;
+  for (int v = 0; v < 5; ++v, p += nA) st->q[v] = p;
+  st->PRES = p; p += nA;
+  st->DPRES = p; p += nA;
+  double** dst[] = {&st->DENS_hyd, &st->PRES_hyd, &st->THERM_hyd, &st->DPhydDx, &st->DPhydDy,
+                    &st->Rtot, &st->CVtot, &st->CPtot};
+  const double* src[] = {DENS_hyd, PRES_hyd, THERM_hyd, DPhydDx, DPhydDy, Rtot, CVtot, CPtot};
+  for (int a = 0; a < 8; ++a, p += nA) {
+    *dst[a] = p;
+    hipError_t e = hipMemcpy(p, src[a], sizeof(double) * nA, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      set_error(std::string("hipMemcpy: ") + hipGetErrorString(e));
+      fedg_hevi_destroy(st);
+      return FEDG_ERR_HIP;
+    }
+  }
+  st->CORIOLIS = p;
+  if (hipMemcpy(p, CORIOLIS, sizeof(double) * cor, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("hipMemcpy of CORIOLIS failed");
+    fedg_hevi_destroy(st);
+    return FEDG_ERR_HIP;
+  }
+  *out = st;
+  return FEDG_OK;
+}
+
+int fedg_hevi_set_state(fedg_hevi* st, const double* const* q) {
+  if (!st || !q) return FEDG_ERR_ARG;
+  const size_t n = (size_t)st->ctx->m.Np * st->ctx->m.Ne;
+  FEDG_HIP_CHECK(hipStreamSynchronize(st->ctx->stream));
+  for (int v = 0; v < 5; ++v) FEDG_HIP_CHECK(hipMemcpy(st->q[v], q[v], sizeof(double) * n, hipMemcpyHostToDevice));
+  return FEDG_OK;
+}
+
+int fedg_hevi_get_state(fedg_hevi* st, double* const* q) {
+  if (!st || !q) return FEDG_ERR_ARG;
+  const size_t n = (size_t)st->ctx->m.Np * st->ctx->m.Ne;
+  FEDG_HIP_CHECK(hipStreamSynchronize(st->ctx->stream));
+  for (int v = 0; v < 5; ++v) FEDG_HIP_CHECK(hipMemcpy(q[v], st->q[v], sizeof(double) * n, hipMemcpyDeviceToHost));
+  return FEDG_OK;
+}
+
+int fedg_hevi_newton_stats(const fedg_hevi* st, long* iterations, long* calls) {
+  if (!st) return FEDG_ERR_ARG;
+  if (iterations) *iterations = st->newton_iters;
+  if (calls) *calls = st->newton_calls;
+  return FEDG_OK;
+}
+
+static int hevi_step(fedg_hevi* st, hipEvent_t* ev) {
+  fedg_ctx* ctx = st->ctx;
+  fedg_tint* t = st->tint;
+  const DevMesh& m = ctx->m;
+  const long n = (long)m.Np * m.Ne;
+  const size_t nA = (size_t)m.Np * m.NeA;
+  HeviWork* w = nullptr;
+  TRY(hevi_work_get(ctx, &w));
+  for (int v = 0; v < 5; ++v) TRY(launch_copy(ctx, t->var0 + v * nA, st->q[v], n));  // StoreVar0
+  for (int s = 1; s <= t->nstage; ++s) {
+    const int ind = t->indmap[s - 1];
+    const double impl_fac = t->a_im[s - 1][s - 1] * t->dt;
+    double* Ti[5];
+    double* Te[5];
+    const double* qv[5];
+    const double* v0[5];
+    for (int v = 0; v < 5; ++v) {
+      Ti[v] = tend_ptr(t, t->tend_im, ind, v + 1);
+      Te[v] = tend_ptr(t, t->tend_ex, ind, v + 1);
+      qv[v] = st->q[v];
+      v0[v] = t->var0 + v * nA;
+    }
+    int it = 0;
+    if (ev) FEDG_HIP_CHECK(hipEventRecord(ev[0], ctx->stream));
+    TRY(hevi_cal_vi(ctx, w, Ti, qv, v0, st->DENS_hyd, st->PRES_hyd, st->Rtot, st->CVtot, st->CPtot, impl_fac,
+                    st->max_it, st->rtol, nullptr, &it));
+    if (ev) FEDG_HIP_CHECK(hipEventRecord(ev[1], ctx->stream));
+    if (fabs(impl_fac) > 0.0) {
+      st->newton_iters += it;
+      st->newton_calls += 1;
+    }
+    for (int v = 0; v < 5; ++v) {  // StoreImplicit
+      if (s == 1) {
+        TRY(launch_copy(ctx, t->var0 + v * nA, st->q[v], n));
+        TRY(launch_copy(ctx, t->varTmp + v * nA, st->q[v], n));
+      }
+      TRY(launch_axpy_acc(ctx, st->q[v], impl_fac, Ti[v], n));
+    }
+    for (int v = 0; v < 5; ++v) TRY(launch_halo_exchange(ctx, st->q[v]));
+    TRY(launch_drhot2pres(ctx, st->PRES, st->DPRES, st->q[1], st->PRES_hyd, st->THERM_hyd, st->Rtot, st->CVtot,
+                          st->CPtot, n));
+    TRY(launch_halo_exchange(ctx, st->DPRES));
+    TRY(launch_apply_bc(ctx, st->q[3], st->q[4], st->q[2]));
+    TendIn in{};
+    for (int v = 0; v < 5; ++v) in.q[v] = st->q[v];
+    in.DPRES = st->DPRES;
+    in.DENS_hyd = st->DENS_hyd;
+    in.PRES_hyd = st->PRES_hyd;
+    in.THERM_hyd = st->THERM_hyd;
+    in.CORIOLIS = st->CORIOLIS;
+    in.DPhydDx = st->DPhydDx;
+    in.DPhydDy = st->DPhydDy;
+    in.Rtot = st->Rtot;
+    in.CVtot = st->CVtot;
+    in.CPtot = st->CPtot;
+    if (ev) FEDG_HIP_CHECK(hipEventRecord(ev[2], ctx->stream));
+    TRY(launch_hevi_explicit(ctx, in, Te));
+    if (ev) FEDG_HIP_CHECK(hipEventRecord(ev[3], ctx->stream));
+    for (int v = 0; v < 5; ++v) TRY(tint_advance_one(t, s, st->q[v], v + 1));
+    if (ev) ev += 4;
+  }
+  return FEDG_OK;
+}
+
+int fedg_hevi_run(fedg_hevi* st, int nsteps) {
+  if (!st || nsteps < 0) return FEDG_ERR_ARG;
+  for (int k = 0; k < nsteps; ++k) TRY(hevi_step(st, nullptr));
+  FEDG_HIP_CHECK(hipStreamSynchronize(st->ctx->stream));
+  return FEDG_OK;
+}
+
+int fedg_hevi_run_timed(fedg_hevi* st, int nsteps, double* ms_total, double* ms_implicit_per_stage,
+                        double* ms_explicit_per_stage) {
+  if (!st || nsteps < 1) return FEDG_ERR_ARG;
+  fedg_ctx* ctx = st->ctx;
+  const int ns = st->tint->nstage;
+  std::vector<hipEvent_t> evs((size_t)nsteps * ns * 4);
+  for (auto& e : evs) FEDG_HIP_CHECK(hipEventCreate(&e));
+  hipEvent_t t0, t1;
+  FEDG_HIP_CHECK(hipEventCreate(&t0));
+  FEDG_HIP_CHECK(hipEventCreate(&t1));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  FEDG_HIP_CHECK(hipEventRecord(t0, ctx->stream));
+  int rc = FEDG_OK;
+  for (int k = 0; k < nsteps && rc == FEDG_OK; ++k) rc = hevi_step(st, evs.data() + (size_t)k * ns * 4);
+  FEDG_HIP_CHECK(hipEventRecord(t1, ctx->stream));
+  FEDG_HIP_CHECK(hipEventSynchronize(t1));
+  if (rc == FEDG_OK) {
+    float tot = 0.f;
+    FEDG_HIP_CHECK(hipEventElapsedTime(&tot, t0, t1));
+    double im = 0.0, ex = 0.0;
+    for (size_t e = 0; e < evs.size(); e += 4) {
+      float a = 0.f, b = 0.f;
+      FEDG_HIP_CHECK(hipEventElapsedTime(&a, evs[e], evs[e + 1]));
+      FEDG_HIP_CHECK(hipEventElapsedTime(&b, evs[e + 2], evs[e + 3]));
+      im += a;
+      ex += b;
+    }
+    const double cnt = (double)nsteps * ns;
+    if (ms_total) *ms_total = tot;
+    if (ms_implicit_per_stage) *ms_implicit_per_stage = im / cnt;
+    if (ms_explicit_per_stage) *ms_explicit_per_stage = ex / cnt;
+  }
+  for (auto& e : evs) (void)hipEventDestroy(e);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  return rc;
+}
+
 }  // extern "C"

@@ -145,6 +145,42 @@ def config2() -> Case:
     return Case("warm bubbles p7 64x64x16", 7, 64, 64, 16, 160.0e3, 160.0e3, 10.0e3, 0.04, 200, config2_bubbles())
 
 
+def config4(dt=None) -> Case:
+    """HEVI at scale: the inertia-gravity wave of Skamarock & Klemp (1994) in a
+    300 x 20 x 10 km periodic channel, 240x16x8 elements, p=7, constant
+    Brunt-Vaisala frequency 0.01 1/s, theta0 = 300 K, mean wind 20 m/s,
+    theta' = 0.01 K sin(pi z/H) / (1 + ((x - 100 km)/5 km)^2), IMEX_ARK324."""
+    return Case("inertia-gravity wave p7 240x16x8", 7, 240, 16, 8, 300.0e3, 20.0e3, 10.0e3,
+                CONFIG4_DT if dt is None else dt, 900, [], base="constBVFreq", scheme="IMEX_ARK324")
+
+
+# Time step of config4: the horizontal acoustics stay explicit in HEVI, and
+# with 1.25 km p=7 elements IMEX_ARK324 is unstable at 0.1 s and stable at
+# 0.05 s (CPU oracle runs of 60 simulated seconds on an 8x2x8 piece of the
+# channel); 2 s, the step a vertical-only acoustic limit would allow, is far
+# beyond the horizontal limit.
+CONFIG4_DT = 0.05
+
+
+def igw_state(mesh: BoxMesh, bs: BaseState, U=20.0, dtheta=0.01, xc=100.0e3, a=5.0e3, theta0=300.0, BVF=0.01,
+              c=constants.SCALE):
+    """Inertia-gravity-wave initial state (model/atm_nonhydro3d/test/case/inertia_gravity_wave/mod_user.F90:
+    231-274 with the bell in x only and a uniform mean wind): pressure left hydrostatic (DRHOT = 0), density
+    from the perturbed potential temperature, MOMX = DENS * U."""
+    n = mesh.Np * mesh.Ne
+    x, z = mesh.x, mesh.z
+    H = float(np.max(z) - np.min(z))
+    PT = theta0 * np.exp(BVF ** 2 / c.GRAV * z) + dtheta * np.sin(np.pi * z / H) / (1.0 + ((x - xc) / a) ** 2)
+    P = bs.PRES_hyd[:n]
+    DENS = P / (c.Rdry * PT * (P / c.PRES00) ** (c.Rdry / c.CPdry))
+    q = [mesh.field() for _ in range(5)]
+    q[0][:n] = DENS - bs.DENS_hyd[:n]
+    q[3][:n] = DENS * U
+    for f in q:
+        mesh.exchange(f)
+    return q
+
+
 def scaled_case(base: Case, NeX, NeY, NeZ, porder=None, nsteps=None) -> Case:
     """Same physics on a smaller element count with the element size of `base`."""
     p = base.porder if porder is None else porder
@@ -160,5 +196,8 @@ def build(case: Case, c=constants.SCALE):
     mesh = BoxMesh(case.porder, case.NeX, case.NeY, case.NeZ, (0.0, case.Lx), (0.0, case.Ly), (0.0, case.Lz),
                    periodic=(True, True, False))
     bs = base_state(mesh, case.base, c=c)
-    q = warm_bubble_state(mesh, bs, case.bubbles, c=c)
+    if case.scheme.startswith("IMEX"):
+        q = igw_state(mesh, bs, xc=min(100.0e3, 0.5 * case.Lx), c=c)
+    else:
+        q = warm_bubble_state(mesh, bs, case.bubbles, c=c)
     return mesh, bs, q

@@ -236,6 +236,64 @@ class HeveStepper:
             pass
 
 
+class HeviStepper:
+    """HEVI IMEX step driver with the state resident on the GPU (fedg_hevi_*).
+
+    max_it / rtol: Newton iterations of the implicit solve per stage (the
+    reference runs one, rtol 0)."""
+
+    def __init__(self, dm: DeviceMesh, bs, DPhydDx: np.ndarray, DPhydDy: np.ndarray, dt: float,
+                 scheme: str = "IMEX_ARK324", max_it: int = 1, rtol: float = 0.0):
+        self.dm = dm
+        m = dm.mesh
+        self.n = m.Np * m.Ne
+        self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                      (bs.DENS_hyd, bs.PRES_hyd, bs.THERM_hyd, bs.CORIOLIS, DPhydDx, DPhydDy, bs.Rtot, bs.CVtot,
+                       bs.CPtot)]
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+        h = C.c_void_p()
+        check(LIB.fedg_hevi_create(dm.handle, scheme.encode(), float(dt), *[ptr(a) for a in self._keep],
+                                   int(max_it), float(rtol), C.byref(h)), "fedg_hevi_create")
+        self.handle = h
+
+    def set_state(self, q):
+        hs = [np.ascontiguousarray(a[: self.n], dtype=np.float64) for a in q]
+        arr = (C.c_void_p * 5)(*[h.ctypes.data_as(C.c_void_p) for h in hs])
+        check(LIB.fedg_hevi_set_state(self.handle, arr), "fedg_hevi_set_state")
+
+    def get_state(self):
+        hs = [np.empty(self.n) for _ in range(5)]
+        arr = (C.c_void_p * 5)(*[h.ctypes.data_as(C.c_void_p) for h in hs])
+        check(LIB.fedg_hevi_get_state(self.handle, arr), "fedg_hevi_get_state")
+        return hs
+
+    def run(self, nsteps: int):
+        check(LIB.fedg_hevi_run(self.handle, int(nsteps)), "fedg_hevi_run")
+
+    def run_timed(self, nsteps: int):
+        """(total ms, mean ms per stage of the implicit solve, of the explicit tendency)."""
+        tot, im, ex = C.c_double(), C.c_double(), C.c_double()
+        check(LIB.fedg_hevi_run_timed(self.handle, int(nsteps), C.byref(tot), C.byref(im), C.byref(ex)),
+              "fedg_hevi_run_timed")
+        return tot.value, im.value, ex.value
+
+    def newton_stats(self):
+        it, calls = C.c_long(), C.c_long()
+        check(LIB.fedg_hevi_newton_stats(self.handle, C.byref(it), C.byref(calls)), "fedg_hevi_newton_stats")
+        return it.value, calls.value
+
+    def close(self):
+        if self.handle:
+            LIB.fedg_hevi_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def device_name() -> str:
     buf = C.create_string_buffer(256)
     check(LIB.fedg_device_name(buf, 256), "fedg_device_name")

@@ -288,6 +288,31 @@ int fedg_heve_run_timed(fedg_heve *st, int nsteps, double *ms_total, double *ms_
  * post-step calc_pressure (:954-959) */
 int fedg_heve_get_pressure(fedg_heve *st, double *PRES_host, double *DPRES_host);
 
+/* ------------------------------------------------------------------ */
+/* HEVI step driver: AtmDynDGMDriver_nonhydro3d_update with hevi_flag
+ * (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:694-870): StoreVar0, then per
+ * IMEX stage cal_vi (fedg_hevi_cal_vi; Newton iterations max_it, tolerance
+ * rtol, the reference: 1 and 0), StoreImplicit, halo exchange, DRHOT2PRES,
+ * boundary conditions, the explicit HEVI tendency and Advance, on the
+ * context's stream with state resident on the device.  Base-state arguments
+ * are host (Np,NeA) arrays with filled halos (CORIOLIS (Nfp_v,Ne2DA)). */
+typedef struct fedg_hevi fedg_hevi;
+int fedg_hevi_create(fedg_ctx *ctx, const char *scheme, double dt, const double *DENS_hyd,
+                     const double *PRES_hyd, const double *THERM_hyd, const double *CORIOLIS,
+                     const double *DPhydDx, const double *DPhydDy, const double *Rtot, const double *CVtot,
+                     const double *CPtot, int max_it, double rtol, fedg_hevi **out);
+int fedg_hevi_destroy(fedg_hevi *st);
+/* q: 5 host arrays (Np,Ne) in variable-id order */
+int fedg_hevi_set_state(fedg_hevi *st, const double *const *q);
+int fedg_hevi_get_state(fedg_hevi *st, double *const *q);
+int fedg_hevi_run(fedg_hevi *st, int nsteps);
+/* nsteps steps bracketed by HIP events: total milliseconds and the mean
+ * duration per stage of the implicit solve and of the explicit tendency */
+int fedg_hevi_run_timed(fedg_hevi *st, int nsteps, double *ms_total, double *ms_implicit_per_stage,
+                        double *ms_explicit_per_stage);
+/* Newton iterations summed over the implicit stages run so far, and their number */
+int fedg_hevi_newton_stats(const fedg_hevi *st, long *iterations, long *calls);
+
 #ifdef __cplusplus
 }
 #endif

@@ -115,3 +115,29 @@ def test_hevi_newton_to_tolerance(setup):
     n = mesh.Np * mesh.Ne
     for v in range(5):
         assert np.array_equal(got[v][:n], ref[v][:n])
+
+
+@pytest.mark.parametrize("max_it,rtol", [(1, 0.0), (6, 1e-10)], ids=["one-iteration", "newton"])
+@pytest.mark.parametrize("p", [3, 7])
+def test_hevi_steps_match_oracle(p, max_it, rtol):
+    """Whole IMEX_ARK324 HEVI steps of the inertia-gravity wave on the device
+    against the oracle's driver restatement."""
+    base = cases.config4()
+    case = cases.scaled_case(base, 4, 2, 4, porder=p)
+    mesh, bs, q = cases.build(case)
+    om = O.OracleMesh(mesh)
+    Dx, Dy = om.calc_phyd_hgrad(bs)
+    dm = fedg.DeviceMesh(mesh)
+    st = fedg.HeviStepper(dm, bs, Dx, Dy, case.dt, scheme=case.scheme, max_it=max_it, rtol=rtol)
+    st.set_state(q)
+    st.run(3)
+    g = st.get_state()
+    qq = [a.copy() for a in q]
+    om.hevi_update(qq, bs, Dx, Dy, case.dt, 3, scheme=case.scheme, max_it=max_it, rtol=rtol)
+    n = mesh.Np * mesh.Ne
+    for name, a, b in zip(fedg.VARS, g, qq):
+        assert np.array_equal(a, b[:n]), f"{name}: max abs diff {np.max(np.abs(a - b[:n]))}"
+    it, calls = st.newton_stats()
+    assert calls == 3 * 3  # three implicit stages per ARK324 step
+    assert (it == calls) if max_it == 1 else (calls <= it <= max_it * calls)
+    st.close()
