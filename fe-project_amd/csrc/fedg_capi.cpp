@@ -84,6 +84,7 @@ struct fedg_heve {
   double *DENS_hyd, *PRES_hyd, *THERM_hyd, *DPhydDx, *DPhydDy, *CORIOLIS;
   double *Rtot, *CVtot, *CPtot;
   double* pres_scratch;     // (Np,Ne) x2 for PRES/DPRES
+  int fp32;                 // stage tendency in the FP32 perturbation form
 };
 
 namespace {
@@ -948,6 +949,16 @@ int fedg_heve_destroy(fedg_heve* st) {
 
 int fedg_heve_is_flat(const fedg_heve* st) { return st ? st->flat : 0; }
 
+int fedg_heve_set_precision(fedg_heve* st, int bits) {
+  if (!st || (bits != 32 && bits != 64)) return FEDG_ERR_ARG;
+  if (bits == 32 && !st->flat) {
+    set_error("the FP32 perturbation form is built for the flat-terrain box with dry air");
+    return FEDG_ERR_UNSUPPORTED;
+  }
+  st->fp32 = bits == 32;
+  return FEDG_OK;
+}
+
 int fedg_heve_set_state(fedg_heve* st, const double* const* q) {
   if (!st || !q) return FEDG_ERR_ARG;
   fedg_ctx* ctx = st->ctx;
@@ -1014,7 +1025,8 @@ static int heve_step(fedg_heve* st, std::vector<hipEvent_t>* evs, size_t* ev_i) 
     rk.cb = t.b_ex[s - 1] * t.dt;
     rk.ca = (s < t.nstage) ? t.dt * t.a_ex[s][s - 1] : 0.0;
     if (evs) FEDG_HIP_CHECK(hipEventRecord((*evs)[(*ev_i)++], ctx->stream));
-    TRY(launch_heve_fused(ctx, in, rk, st->flat));
+    if (st->fp32) TRY(launch_heve_fused32(ctx, in, rk));
+    else TRY(launch_heve_fused(ctx, in, rk, st->flat));
     if (evs) FEDG_HIP_CHECK(hipEventRecord((*evs)[(*ev_i)++], ctx->stream));
     cur = outb;
     last_out = outb;

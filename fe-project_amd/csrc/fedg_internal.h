@@ -105,6 +105,8 @@ struct RkOut {
 int launch_heve_dropin(fedg_ctx* ctx, const TendIn& in, double* const out[5]);
 int launch_hevi_explicit(fedg_ctx* ctx, const TendIn& in, double* const out[5]);
 int launch_heve_fused(fedg_ctx* ctx, const TendIn& in, const RkOut& rk, int use_flat);
+// FP32 perturbation form of the fused stage (flat box, dry air; RkOut.DPout unused)
+int launch_heve_fused32(fedg_ctx* ctx, const TendIn& in, const RkOut& rk);
 // DPRES of a state (dry constants when Rtot is null), as DRHOT2PRES computes it
 int launch_dpres(fedg_ctx* ctx, double* DPRES, const double* DRHOT, const double* PRES_hyd,
                  const double* THERM_hyd, const double* Rtot, const double* CVtot, const double* CPtot, long n);

@@ -735,6 +735,226 @@ __global__ void __launch_bounds__(256, 2) heve7_flat_kernel(const DevMesh m, con
   }
 }
 
+// ---------------------------------------------------------------------------
+// FP32 perturbation form of the fused HEVE stage (flat-terrain box, dry air).
+// The tendency -- pointwise fluxes, face flux, derivative contractions, lift,
+// buoyancy -- is evaluated in FP32 on the perturbation variables about the
+// FP64 base state; the state, the base state and the RK update stay FP64.
+// The pressure perturbation, which FP64 forms as the small difference
+// P(rhotheta) - P_hyd, is formed without that cancellation:
+//   DPRES = P_hyd ((1 + DRHOT/THERM_hyd)^(cp/cv) - 1)
+//         = P_hyd expm1((cp/cv) log1p(DRHOT/THERM_hyd)),
+// exact for THERM_hyd = P00/R (P_hyd/P00)^(cv/cp) (the base state's
+// definition, ..._nonhydro3d_common.F90:584-619).
+__device__ __forceinline__ float dpres32(float DT, float Th, float Ph, float gamm) {
+  return Ph * expm1f(gamm * log1pf(DT / Th));
+}
+
+// Rusanov flux jump in FP32 for the flat box (Gsqrt = 1, G13 = G23 = 0,
+// axis-aligned unit normals; same formula as rusanov()).
+struct Side32 {
+  float DD, DT, MZ, MX, MY, Dh, Ph, Th, DP;
+};
+__device__ __forceinline__ void rusanov32(const Side32& M, const Side32& P, float nx, float ny, float nz, float Fs,
+                                          float gamm, float df[5]) {
+  const float DensM = M.DD + M.Dh, DensP = P.DD + P.Dh;
+  const float RhotM = M.Th + M.DT, RhotP = P.Th + P.DT;
+  const float VelM = (M.MX * nx + M.MY * ny + M.MZ * nz) / DensM;
+  const float VelP = (P.MX * nx + P.MY * ny + P.MZ * nz) / DensP;
+  const float aM = sqrtf(gamm * (M.Ph + M.DP) / DensM) + fabsf(VelM);
+  const float aP = sqrtf(gamm * (P.Ph + P.DP) / DensP) + fabsf(VelP);
+  const float alpha = aM > aP ? aM : aP;
+  const float h = 0.5f * Fs;
+  df[VD] = h * (DensP * VelP - DensM * VelM - alpha * (P.DD - M.DD));
+  df[VT] = h * (RhotP * VelP - RhotM * VelM - alpha * (P.DT - M.DT));
+  const float dp = P.DP - M.DP;
+  df[VW] = h * (P.MZ * VelP - M.MZ * VelM + dp * nz - alpha * (P.MZ - M.MZ));
+  df[VU] = h * (P.MX * VelP - M.MX * VelM + dp * nx - alpha * (P.MX - M.MX));
+  df[VV] = h * (P.MY * VelP - M.MY * VelM + dp * ny - alpha * (P.MY - M.MY));
+}
+
+template <int N1, int EPB>
+__global__ void __launch_bounds__(EPB* N1* N1* N1, 2) heve32_kernel(const DevMesh m, const Consts c, const TendIn in,
+                                                                    const RkOut rk) {
+  constexpr int Np = N1 * N1 * N1, Nfp = N1 * N1, NfpTot = 6 * Nfp;
+  __shared__ float sDt[N1 * N1];
+  __shared__ float sI[N1 * N1];
+  __shared__ float sS[EPB][S_N][Np];
+  __shared__ float sF[2][EPB][3][Np];
+  __shared__ float sDF[EPB][5][NfpTot];
+
+  const int tid = threadIdx.x;
+  const int e = tid / Np;
+  const int p = tid - e * Np;
+  const int ke = xcd_remap(blockIdx.x, gridDim.x) * EPB + e;
+  const bool valid = ke < m.Ne;
+  const int i = p % N1, j = (p / N1) % N1, k = p / Nfp;
+  const float gamm = (float)(c.CPdry / c.CVdry);
+  for (int t = tid; t < N1 * N1; t += blockDim.x) {
+    sDt[(t % N1) * N1 + t / N1] = (float)m.D1D[t];
+    sI[t] = (float)m.Itr[t];
+  }
+  const uint32_t o = (uint32_t)Np * (valid ? ke : 0) + p;
+  double q[5] = {0, 0, 0, 0, 0};
+  float qf[5] = {0, 0, 0, 0, 0};
+  float Dh = 1.f, Ph = 1.f, Th = 1.f, DP = 0.f, cor = 0.f, dpx = 0.f, dpy = 0.f;
+  if (valid) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      q[v] = ldg(in.q[v], o);
+      qf[v] = (float)q[v];
+    }
+    Dh = (float)ldg(in.DENS_hyd, o);
+    Ph = (float)ldg(in.PRES_hyd, o);
+    Th = (float)ldg(in.THERM_hyd, o);
+    DP = dpres32(qf[VT], Th, Ph, gamm);
+    cor = (float)ldg(in.CORIOLIS, (uint32_t)(i + N1 * j) + (uint32_t)Nfp * (ke % (m.NeX * m.NeY)));
+    dpx = (float)ldg(in.DPhydDx, o);
+    dpy = (float)ldg(in.DPhydDy, o);
+  }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) sS[e][v][p] = qf[v];
+  sS[e][S_DH][p] = Dh;
+  sS[e][S_PH][p] = Ph;
+  sS[e][S_TH][p] = Th;
+  sS[e][S_DP][p] = DP;
+  float L[6];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) L[f] = (float)ldg(m.Lift, (uint32_t)(p + f * Np));
+  __syncthreads();
+
+  // face phase (flat box: arithmetic neighbours, closed boundaries reflect)
+  if (valid) {
+    const int ei = ke % m.NeX, ej = (ke / m.NeX) % m.NeY, ek = ke / (m.NeX * m.NeY);
+    for (int fp = p; fp < NfpTot; fp += Np) {
+      const int f = fp / Nfp, a = fp - f * Nfp;
+      const int pm = face_node<N1>(f, a);
+      int ni = ei, nj = ej, nk = ek;
+      bool closed = false;
+      switch (f) {
+        case 0: nj -= 1; if (nj < 0) { if (m.periodic[1]) nj = m.NeY - 1; else closed = true; } break;
+        case 1: ni += 1; if (ni >= m.NeX) { if (m.periodic[0]) ni = 0; else closed = true; } break;
+        case 2: nj += 1; if (nj >= m.NeY) { if (m.periodic[1]) nj = 0; else closed = true; } break;
+        case 3: ni -= 1; if (ni < 0) { if (m.periodic[0]) ni = m.NeX - 1; else closed = true; } break;
+        case 4: nk -= 1; if (nk < 0) { if (m.periodic[2]) nk = m.NeZ - 1; else closed = true; } break;
+        default: nk += 1; if (nk >= m.NeZ) { if (m.periodic[2]) nk = 0; else closed = true; } break;
+      }
+      const float nx = (f == 1) ? 1.f : (f == 3) ? -1.f : 0.f;
+      const float ny = (f == 2) ? 1.f : (f == 0) ? -1.f : 0.f;
+      const float nz = (f == 5) ? 1.f : (f == 4) ? -1.f : 0.f;
+      Side32 M, P;
+      M.DD = sS[e][VD][pm]; M.DT = sS[e][VT][pm]; M.MZ = sS[e][VW][pm]; M.MX = sS[e][VU][pm];
+      M.MY = sS[e][VV][pm]; M.Dh = sS[e][S_DH][pm]; M.Ph = sS[e][S_PH][pm]; M.Th = sS[e][S_TH][pm];
+      M.DP = sS[e][S_DP][pm];
+      if (closed) {
+        P = M;
+        const int bc = m.bnd_bc[f];
+        if (bc == FEDG_BND_SLIP) {
+          const float mn = M.MX * nx + M.MY * ny + M.MZ * nz;
+          P.MX = M.MX - 2.f * mn * nx;
+          P.MY = M.MY - 2.f * mn * ny;
+          P.MZ = M.MZ - 2.f * mn * nz;
+        } else if (bc == FEDG_BND_NOSLIP) {
+          P.MX = -M.MX; P.MY = -M.MY; P.MZ = -M.MZ;
+        }
+      } else {
+        constexpr int opp[6] = {2, 3, 0, 1, 5, 4};
+        const uint32_t iP = (uint32_t)Np * (ni + m.NeX * (nj + m.NeY * nk)) + face_node<N1>(opp[f], a);
+        P.DD = (float)ldg(in.q[VD], iP); P.DT = (float)ldg(in.q[VT], iP); P.MZ = (float)ldg(in.q[VW], iP);
+        P.MX = (float)ldg(in.q[VU], iP); P.MY = (float)ldg(in.q[VV], iP);
+        P.Dh = (float)ldg(in.DENS_hyd, iP); P.Ph = (float)ldg(in.PRES_hyd, iP); P.Th = (float)ldg(in.THERM_hyd, iP);
+        P.DP = dpres32(P.DT, P.Th, P.Ph, gamm);
+      }
+      float df[5];
+      rusanov32(M, P, nx, ny, nz, (float)m.fF[f], gamm, df);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) sDF[e][v][fp] = df[v];
+    }
+  }
+
+  // RK operands (FP64)
+  double rkA[5] = {0, 0, 0, 0, 0}, rkB[5] = {0, 0, 0, 0, 0};
+  if (valid) {
+    if (rk.mode == 1) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        rkA[v] = ldg(rk.var0[v], o);
+        rkB[v] = ldg(rk.varTmp[v], o);
+      }
+    } else if (rk.mode == 2) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) rkB[v] = ldg(rk.varTmp[v], o);
+    }
+  }
+  const float RD = 1.f / (qf[VD] + Dh);
+  const float E11 = (float)m.fE[0], E22 = (float)m.fE[1], E33 = (float)m.fE[2];
+  const float grav = (float)c.GRAV;
+  float drho = sS[e][VD][i + N1 * j] * sI[N1 * k];
+#pragma unroll
+  for (int l = 1; l < N1; ++l) drho += sS[e][VD][i + N1 * (j + N1 * l)] * sI[l + N1 * k];
+
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    float f0, f1, f2;
+    if (v == VD) {
+      f0 = qf[VU]; f1 = qf[VV]; f2 = qf[VW];
+    } else if (v == VT) {
+      const float pt = (Th + qf[VT]) * RD;
+      f0 = qf[VU] * pt; f1 = qf[VV] * pt; f2 = qf[VW] * pt;
+    } else {
+      const float u = qf[v] * RD;
+      f0 = qf[VU] * u + (v == VU ? DP : 0.f);
+      f1 = qf[VV] * u + (v == VV ? DP : 0.f);
+      f2 = qf[VW] * u + (v == VW ? DP : 0.f);
+    }
+    const int b = v & 1;
+    sF[b][e][0][p] = f0;
+    sF[b][e][1][p] = f1;
+    sF[b][e][2][p] = f2;
+    __syncthreads();
+    const float* Fx = sF[b][e][0];
+    const float* Fy = sF[b][e][1];
+    const float* Fz = sF[b][e][2];
+    float dx = 0.f, dy = 0.f, dz = 0.f;
+#pragma unroll
+    for (int l = 0; l < N1; ++l) dx += sDt[l + N1 * i] * Fx[l + N1 * (j + N1 * k)];
+#pragma unroll
+    for (int l = 0; l < N1; ++l) dy += Fy[i + N1 * (l + N1 * k)] * sDt[l + N1 * j];
+#pragma unroll
+    for (int l = 0; l < N1; ++l) dz += Fz[i + N1 * (j + N1 * l)] * sDt[l + N1 * k];
+    const float* dfv = sDF[e][v];
+    const float lift = L[0] * dfv[i + N1 * k] + L[1] * dfv[Nfp + j + N1 * k] + L[2] * dfv[2 * Nfp + i + N1 * k] +
+                       L[3] * dfv[3 * Nfp + j + N1 * k] + L[4] * dfv[4 * Nfp + i + N1 * j] +
+                       L[5] * dfv[5 * Nfp + i + N1 * j];
+    float T = -(E11 * dx + E22 * dy + E33 * dz + lift);
+    if (v == VW) T -= grav * drho;
+    else if (v == VU) T += -dpx + cor * qf[VV];
+    else if (v == VV) T += -dpy - cor * qf[VU];
+    const double Td = (double)T;
+    if (valid) {
+      double qn;
+      switch (rk.mode) {
+        case 0:
+          stg(rk.varTmp[v], o, q[v] + rk.cb * Td);
+          qn = q[v] + rk.ca * Td;
+          break;
+        case 1:
+          stg(rk.varTmp[v], o, rkB[v] + rk.cb * Td);
+          qn = rkA[v] + rk.ca * Td;
+          break;
+        case 2:
+          qn = rkB[v] + rk.cb * Td;
+          break;
+        default:
+          qn = q[v] + rk.cb * Td;
+          break;
+      }
+      stg(rk.out[v], o, qn);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 template <int N1>
 constexpr int epb_for() {
   return (N1 * N1 * N1 >= 128) ? 1 : (256 / (N1 * N1 * N1));
@@ -928,6 +1148,35 @@ int launch_hevi_explicit(fedg_ctx* ctx, const TendIn& in, double* const out[5]) 
   RkOut rk{};
   for (int v = 0; v < 5; ++v) rk.out[v] = out[v];
   return launch_heve_mode<MODE_DROPIN_HEVI>(ctx, in, rk);
+}
+
+int launch_heve_fused32(fedg_ctx* ctx, const TendIn& in, const RkOut& rk) {
+  const DevMesh& m = ctx->m;
+  if (!m.flat || in.Rtot != nullptr) {
+    set_error("the FP32 perturbation form is built for the flat-terrain box with dry air");
+    return FEDG_ERR_UNSUPPORTED;
+  }
+  auto go = [&](auto n1c) -> int {
+    constexpr int N1 = decltype(n1c)::value;
+    constexpr int EPB = epb_for<N1>();
+    const int nblk = (m.Ne + EPB - 1) / EPB;
+    hipLaunchKernelGGL((heve32_kernel<N1, EPB>), dim3(nblk), dim3(EPB * N1 * N1 * N1), 0, ctx->stream, m, ctx->c,
+                       in, rk);
+    FEDG_HIP_CHECK(hipGetLastError());
+    return FEDG_OK;
+  };
+  switch (m.n1) {
+    case 2: return go(std::integral_constant<int, 2>{});
+    case 3: return go(std::integral_constant<int, 3>{});
+    case 4: return go(std::integral_constant<int, 4>{});
+    case 5: return go(std::integral_constant<int, 5>{});
+    case 6: return go(std::integral_constant<int, 6>{});
+    case 7: return go(std::integral_constant<int, 7>{});
+    case 8: return go(std::integral_constant<int, 8>{});
+    default:
+      set_error("polynomial order not built (supported: p = 1..7)");
+      return FEDG_ERR_UNSUPPORTED;
+  }
 }
 
 int launch_heve_fused(fedg_ctx* ctx, const TendIn& in, const RkOut& rk, int use_flat) {

@@ -84,6 +84,7 @@ def _load():
         "fedg_heve_cal_tend": (C.c_int, [vp] + [vp] * 21),
         "fedg_calc_phyd_hgrad": (C.c_int, [vp] + [vp] * 4),
         "fedg_hevi_cal_tend": (C.c_int, [vp] + [vp] * 21),
+        "fedg_heve_set_precision": (C.c_int, [vp, C.c_int]),
         "fedg_hevi_create": (C.c_int, [vp, C.c_char_p, C.c_double] + [vp] * 9 + [C.c_int, C.c_double,
                                                                                  C.POINTER(vp)]),
         "fedg_hevi_destroy": (C.c_int, [vp]),
@@ -128,7 +129,7 @@ EXPORTED = (
     "fedg_memset_dev fedg_device_sync fedg_device_name fedg_refelem_tables fedg_box_get_sizes "
     "fedg_box_build fedg_ctx_create fedg_ctx_destroy fedg_ctx_sizes fedg_drhot2pres fedg_halo_exchange "
     "fedg_apply_bc_prgvars fedg_heve_cal_tend fedg_hevi_cal_tend fedg_hevi_cal_vi fedg_hevi_cal_vi_newton "
-    "fedg_hevi_create fedg_hevi_destroy fedg_hevi_set_state fedg_hevi_get_state fedg_hevi_run "
+    "fedg_heve_set_precision fedg_hevi_create fedg_hevi_destroy fedg_hevi_set_state fedg_hevi_get_state fedg_hevi_run "
     "fedg_hevi_run_timed fedg_hevi_newton_stats fedg_calc_phyd_hgrad fedg_tint_create fedg_tint_destroy "
     "fedg_tint_info fedg_tint_tend_ex fedg_tint_tend_im fedg_tint_store_var0 fedg_tint_advance "
     "fedg_tint_store_implicit fedg_heve_create fedg_heve_destroy fedg_heve_is_flat fedg_heve_set_state "

@@ -198,6 +198,10 @@ class HeveStepper:
     def flat(self) -> bool:
         return bool(LIB.fedg_heve_is_flat(self.handle))
 
+    def set_precision(self, bits: int):
+        """64: the reference's FP64 stage; 32: the FP32 perturbation-form tendency (FP64 state and RK)."""
+        check(LIB.fedg_heve_set_precision(self.handle, int(bits)), "fedg_heve_set_precision")
+
     def set_state(self, q):
         hs = [np.ascontiguousarray(a[: self.n], dtype=np.float64) for a in q]
         arr = (C.c_void_p * 5)(*[h.ctypes.data_as(C.c_void_p) for h in hs])

@@ -14,7 +14,7 @@
  *
  * Every entry point returns FEDG_OK (0) or an error code; fedg_last_error()
  * gives the message.  The reference aborts (PRC_abort) on such errors; the
- * Fortran binding (fe-project_amd/fortran/scale_atm_dyn_dgm_hip.F90) does the
+ * Fortran binding (fe-project_amd/fortran/fedg_c_binding.F90) lets the host do the
  * same with a nonzero status.  All device work is stream-ordered on the
  * context's stream and complete when the function returns, as the reference's
  * OpenACC path waits on its async queue before returning
@@ -274,6 +274,14 @@ int fedg_heve_create(fedg_ctx *ctx, const char *scheme, double dt, const double 
 int fedg_heve_destroy(fedg_heve *st);
 /* 1 if the flat-terrain uniform-box kernel is used, 0 for the general-metric kernel */
 int fedg_heve_is_flat(const fedg_heve *st);
+/* No counterpart in the reference (one global precision switch, Mkinclude:34-35):
+ * bits = 32 evaluates each stage's tendency -- fluxes, face flux, derivative
+ * contractions, lift, buoyancy -- in FP32 on the perturbation variables about
+ * the FP64 base state, with the pressure perturbation formed without
+ * cancellation, P_hyd expm1(cp/cv log1p(DRHOT/THERM_hyd)); the state, the
+ * base state and the RK update stay FP64.  Flat-terrain box, dry air.
+ * bits = 64 (default): the reference's FP64 arithmetic. */
+int fedg_heve_set_precision(fedg_heve *st, int bits);
 /* host state in/out: q[v] (Np,Ne or more) for v = DDENS, DRHOT, MOMZ, MOMX, MOMY */
 int fedg_heve_set_state(fedg_heve *st, const double *const *q_host);
 int fedg_heve_get_state(fedg_heve *st, double *const *q_host);

@@ -1,0 +1,53 @@
+"""FP32 perturbation form of the HEVE stage (fedg_heve_set_precision(32))
+against the FP64 path, which is bit for bit the oracle (test_gpu_heve.py).
+
+The FP32 form changes results by design; its bar is the error against the FP64
+run: relative L2 and Linf of every prognostic after 100 steps of the p=3 warm
+bubble, and after one simulated hour (18000 steps of 0.2 s).
+"""
+import numpy as np
+import pytest
+
+import fedg
+from fedg import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def run(case, nsteps, bits):
+    mesh, bs, q = cases.build(case)
+    dm = fedg.DeviceMesh(mesh)
+    Dx, Dy = fedg.calc_phyd_hgrad(dm, bs.PRES_hyd, bs.PRES_hyd_ref)
+    st = fedg.HeveStepper(dm, bs, Dx, Dy, case.dt, scheme=case.scheme)
+    st.set_precision(bits)
+    st.set_state(q)
+    st.run(nsteps)
+    out = st.get_state()
+    st.close()
+    return out, q
+
+
+def errors(a, b):
+    rl2 = np.linalg.norm(a - b) / np.linalg.norm(b)
+    rli = np.max(np.abs(a - b)) / np.max(np.abs(b))
+    return rl2, rli
+
+
+@pytest.mark.parametrize("nsteps,tol", [(100, 1e-5), (18000, 1e-4)], ids=["100-steps", "one-hour"])
+def test_fp32_form_error_against_fp64(nsteps, tol):
+    case = cases.config1()
+    a, q0 = run(case, nsteps, 32)
+    b, _ = run(case, nsteps, 64)
+    for v, name in enumerate(fedg.VARS):
+        assert np.all(np.isfinite(a[v])), name
+        rl2, rli = errors(a[v], b[v])
+        print(f"{nsteps} steps {name}: FP32 form vs FP64 relative L2 {rl2:.2e}, Linf {rli:.2e}")
+        assert rl2 <= tol, name
+
+
+def test_fp32_form_keeps_rest_state():
+    case = cases.config1()
+    case.bubbles = []
+    out, q = run(case, 50, 32)
+    for v in range(5):
+        assert np.max(np.abs(out[v])) <= 1e-6, fedg.VARS[v]
