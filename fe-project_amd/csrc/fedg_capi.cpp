@@ -3,6 +3,7 @@
 // citations are in include/fedg.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -12,6 +13,7 @@
 #include "fedg_builder.h"
 #include "fedg_internal.h"
 #include "fedg_hevi.h"
+#include "fedg_mlp.h"
 
 namespace fedg {
 
@@ -672,9 +674,11 @@ void hevi_work_free(fedg_ctx* ctx) {
   if (!h) return;
   for (int v = 0; v < 5; ++v)
     if (h->pv[v]) (void)hipFree(h->pv[v]);
-  double* bufs[] = {h->alph, h->b3, h->buv, h->coef, h->partial};
+  double* bufs[] = {h->alph, h->b3, h->buv, h->coef, h->partial, h->r_plain, h->r_guess, h->flag};
   for (double* b : bufs)
     if (b) (void)hipFree(b);
+  if (h->X) (void)hipFree(h->X);
+  if (h->Y) (void)hipFree(h->Y);
   delete h;
   ctx->hevi = nullptr;
 }
@@ -1124,6 +1128,14 @@ struct fedg_hevi {
   int max_it;
   double rtol;
   long newton_iters, newton_calls;
+  // learned initial guess and training-pair harvest
+  fedg::MlpImpl* mlp;
+  double scale[5];
+  long guess_lines, guess_fallbacks;
+  float* hx;
+  float* hy;
+  int hcap, hcount, hper;
+  unsigned hseed;
 };
 
 int fedg_hevi_destroy(fedg_hevi* st) {
@@ -1233,10 +1245,34 @@ static int hevi_step(fedg_hevi* st, hipEvent_t* ev) {
       v0[v] = t->var0 + v * nA;
     }
     int it = 0;
+    // explicit tendency of the previous stage: a feature of the learned guess
+    const double* Tprev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (s > 1)
+      for (int v = 0; v < 5; ++v) Tprev[v] = tend_ptr(t, t->tend_ex, t->indmap[s - 2], v + 1);
+    HeviGuess g{};
+    g.mlp = st->mlp;
+    for (int v = 0; v < 5; ++v) {
+      g.Tex[v] = Tprev[v];
+      g.scale[v] = st->scale[v];
+    }
+    g.dt = t->dt;
     if (ev) FEDG_HIP_CHECK(hipEventRecord(ev[0], ctx->stream));
     TRY(hevi_cal_vi(ctx, w, Ti, qv, v0, st->DENS_hyd, st->PRES_hyd, st->Rtot, st->CVtot, st->CPtot, impl_fac,
-                    st->max_it, st->rtol, nullptr, &it));
+                    st->max_it, st->rtol, nullptr, &it, st->mlp ? &g : nullptr));
     if (ev) FEDG_HIP_CHECK(hipEventRecord(ev[1], ctx->stream));
+    st->guess_lines += g.lines;
+    st->guess_fallbacks += g.fallbacks;
+    if (st->hx && fabs(impl_fac) > 0.0 && st->hcount < st->hcap) {
+      const long nlines = (long)m.Ne2D * m.Nfp;
+      const int L = (m.Ne / m.Ne2D) * m.n1;
+      const int rows = std::min(st->hper, st->hcap - st->hcount);
+      st->hseed = st->hseed * 1664525u + 1013904223u;
+      const long line0 = (long)(st->hseed % (unsigned)nlines);
+      long stride = std::max(1L, nlines / rows) | 1L;  // odd stride spreads the lines
+      TRY(hevi_harvest(ctx, w, qv, v0, Tprev, st->scale, impl_fac, t->dt, line0, stride, rows,
+                       st->hx + (size_t)st->hcount * (10 * L + 1), st->hy + (size_t)st->hcount * (5 * L)));
+      st->hcount += rows;
+    }
     if (fabs(impl_fac) > 0.0) {
       st->newton_iters += it;
       st->newton_calls += 1;
@@ -1317,6 +1353,94 @@ int fedg_hevi_run_timed(fedg_hevi* st, int nsteps, double* ms_total, double* ms_
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
   return rc;
+}
+
+
+// ---------------------------------------------------------------------------
+// Per-column MLP and its use as the HEVI Newton initial guess
+struct fedg_mlp {
+  fedg::MlpImpl* impl;
+};
+
+int fedg_mlp_create(int n_in, int n_hidden, int n_hidden_layers, int n_out, unsigned seed, fedg_mlp** out) {
+  if (!out) return FEDG_ERR_ARG;
+  *out = nullptr;
+  MlpImpl* p = nullptr;
+  TRY(mlp_create(n_in, n_hidden, n_hidden_layers, n_out, seed, &p));
+  *out = new fedg_mlp{p};
+  return FEDG_OK;
+}
+
+int fedg_mlp_destroy(fedg_mlp* m) {
+  if (!m) return FEDG_OK;
+  mlp_destroy(m->impl);
+  delete m;
+  return FEDG_OK;
+}
+
+int fedg_mlp_param_count(const fedg_mlp* m, long* n) {
+  if (!m || !n) return FEDG_ERR_ARG;
+  *n = mlp_param_count(m->impl);
+  return FEDG_OK;
+}
+
+int fedg_mlp_get_params(fedg_mlp* m, float* params_host, float* grads_host) {
+  if (!m) return FEDG_ERR_ARG;
+  return mlp_get(m->impl, params_host, grads_host);
+}
+
+int fedg_mlp_set_params(fedg_mlp* m, const float* params_host) {
+  if (!m || !params_host) return FEDG_ERR_ARG;
+  return mlp_set(m->impl, params_host);
+}
+
+int fedg_mlp_forward(fedg_mlp* m, const float* X, float* Y, int batch) {
+  if (!m || !X || !Y || batch < 1) return FEDG_ERR_ARG;
+  return mlp_forward(m->impl, X, Y, batch);
+}
+
+int fedg_mlp_loss_grad(fedg_mlp* m, const float* X, const float* T, int batch, float* loss) {
+  if (!m || !X || !T || batch < 1) return FEDG_ERR_ARG;
+  return mlp_loss_grad(m->impl, X, T, batch, loss);
+}
+
+int fedg_mlp_adam_step(fedg_mlp* m, float lr, float beta1, float beta2, float eps) {
+  if (!m || !(lr > 0.f)) return FEDG_ERR_ARG;
+  return mlp_adam(m->impl, lr, beta1, beta2, eps);
+}
+
+int fedg_hevi_set_guess(fedg_hevi* st, fedg_mlp* mlp, const double* scale) {
+  if (!st || !scale) return FEDG_ERR_ARG;
+  for (int v = 0; v < 5; ++v) {
+    if (!(scale[v] > 0.0)) return FEDG_ERR_ARG;
+    st->scale[v] = scale[v];
+  }
+  st->mlp = mlp ? mlp->impl : nullptr;
+  return FEDG_OK;
+}
+
+int fedg_hevi_guess_stats(const fedg_hevi* st, long* lines, long* fallbacks) {
+  if (!st) return FEDG_ERR_ARG;
+  if (lines) *lines = st->guess_lines;
+  if (fallbacks) *fallbacks = st->guess_fallbacks;
+  return FEDG_OK;
+}
+
+int fedg_hevi_harvest(fedg_hevi* st, float* X_dev, float* Y_dev, int capacity, int per_stage, unsigned seed) {
+  if (!st || capacity < 0 || (capacity > 0 && (!X_dev || !Y_dev || per_stage < 1))) return FEDG_ERR_ARG;
+  st->hx = capacity > 0 ? X_dev : nullptr;
+  st->hy = capacity > 0 ? Y_dev : nullptr;
+  st->hcap = capacity;
+  st->hcount = 0;
+  st->hper = per_stage;
+  st->hseed = seed;
+  return FEDG_OK;
+}
+
+int fedg_hevi_harvest_count(const fedg_hevi* st, int* count) {
+  if (!st || !count) return FEDG_ERR_ARG;
+  *count = st->hcount;
+  return FEDG_OK;
 }
 
 }  // extern "C"

@@ -13,6 +13,28 @@ struct HeviWork {
   double* coef;       // (3, Np*Ne) POT, WT, DPDRHOT
   double* partial;    // 256 partial sums (residual norms)
   double partial_h[256];
+  // learned initial guess (allocated on first use)
+  float* X;           // (chunk, n_in) features
+  float* Y;           // (chunk, n_out) predictions
+  int chunk, n_in, n_out;
+  double* r_plain;    // (nlines) residual norm^2 of each line at q^n
+  double* r_guess;    // (nlines) at the predicted iterate
+  double* flag;       // (nlines) 1 where the line fell back to q^n
+};
+
+struct MlpImpl;
+
+// Learned initial iterate of the Newton solve: the network maps a line's
+// (q* - q^n, impl_fac T_ex) / scale and impl_fac/dt to (q - q*) / scale of
+// the converged iterate q.  Lines whose predicted iterate has a larger
+// residual than q^n start from q^n (fallback).
+struct HeviGuess {
+  MlpImpl* mlp;
+  const double* Tex[5];  // explicit tendency of the previous stage (VID order), or null
+  double scale[5];
+  double dt;
+  long lines;            // out: lines solved
+  long fallbacks;        // out: lines that fell back to q^n
 };
 
 int hevi_work_get(fedg_ctx* ctx, HeviWork** w);
@@ -26,6 +48,13 @@ void hevi_work_free(fedg_ctx* ctx);
 int hevi_cal_vi(fedg_ctx* ctx, HeviWork* w, double* const tend[5], const double* const q[5],
                 const double* const var0[5], const double* DENS_hyd, const double* PRES_hyd, const double* Rtot,
                 const double* CVtot, const double* CPtot, double impl_fac, int max_it, double rtol, double* resid,
-                int* it_done);
+                int* it_done, HeviGuess* guess = nullptr);
+
+// Features and targets of nrows lines (line = (line0 + r*stride) mod nlines)
+// of a converged solve (iterate in w->pv) into X (nrows x n_in) and Y
+// (nrows x n_out), device fp32; the network's input/output layout.
+int hevi_harvest(fedg_ctx* ctx, HeviWork* w, const double* const q[5], const double* const var0[5],
+                 const double* const Tex[5], const double scale[5], double impl_fac, double dt, long line0,
+                 long stride, int nrows, float* X, float* Y);
 
 }  // namespace fedg

@@ -25,6 +25,7 @@
 // the CPU restatement.
 #include "fedg_device.h"
 #include "fedg_hevi.h"
+#include "fedg_mlp.h"
 
 namespace fedg {
 namespace {
@@ -579,6 +580,136 @@ __global__ void sumsq_kernel(const double* x, long n, double* partial) {
   if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
 }
 
+// ---------------------------------------------------------------------------
+// Learned initial iterate: features, targets, application, per-line residual
+// and fallback.  One workgroup per line, one thread per node of the line
+// (L = NeZ*N1 threads); node t of a line is (kz = t / N1, pv = t % N1).
+struct LineArgs {
+  const double* qs[5];
+  const double* q0[5];
+  const double* Tex[5];
+  double* pv[5];
+  double inv_s[5], s[5];
+  double impl_fac, dt;
+  int Np, Nfp, Ne2D, N1, L, nlines;
+  long line0, stride;
+};
+
+__device__ __forceinline__ uint32_t line_node(const LineArgs& a, long line, int t) {
+  const int ke2D = (int)(line / a.Nfp), ij = (int)(line % a.Nfp);
+  const int kz = t / a.N1, pv = t % a.N1;
+  return (uint32_t)a.Np * (ke2D + kz * a.Ne2D) + ij + a.Nfp * pv;
+}
+
+__global__ void line_features_kernel(const LineArgs a, float* X, int n_in, float* Y, int n_out) {
+  const int row = blockIdx.x, t = threadIdx.x;
+  const long line = (a.line0 + (long)row * a.stride) % a.nlines;
+  const uint32_t n = line_node(a, line, t);
+  float* x = X + (size_t)row * n_in;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const double qs = ldg(a.qs[v], n);
+    x[v * a.L + t] = (float)((qs - ldg(a.q0[v], n)) * a.inv_s[v]);
+    const double te = a.Tex[v] ? ldg(a.Tex[v], n) : 0.0;
+    x[(5 + v) * a.L + t] = (float)(a.impl_fac * te * a.inv_s[v]);
+    if (Y) Y[(size_t)row * n_out + v * a.L + t] = (float)((ldg(a.pv[v], n) - qs) * a.inv_s[v]);
+  }
+  if (t == 0) x[10 * a.L] = (float)(a.impl_fac / a.dt);
+}
+
+__global__ void line_apply_kernel(const LineArgs a, const float* Y, int n_out) {
+  const int row = blockIdx.x, t = threadIdx.x;
+  const long line = (a.line0 + (long)row * a.stride) % a.nlines;
+  const uint32_t n = line_node(a, line, t);
+#pragma unroll
+  for (int v = 0; v < 5; ++v)
+    stg(a.pv[v], n, ldg(a.qs[v], n) + a.s[v] * (double)Y[(size_t)row * n_out + v * a.L + t]);
+}
+
+// residual norm^2 of every line from the uv and var3 right-hand sides
+__global__ void line_resid_kernel(const LineArgs a, const double* buv, const double* b3, long nn, double* r) {
+  __shared__ double sh[512];
+  const long line = blockIdx.x;
+  const int t = threadIdx.x;
+  const uint32_t n = line_node(a, line, t);
+  double acc = 0.0;
+  for (int c = 0; c < 2; ++c) {
+    const double x = ldg(buv, n + (uint32_t)(c * nn));
+    acc += x * x;
+  }
+  for (int c = 0; c < 3; ++c) {
+    const double x = ldg(b3, n + (uint32_t)(c * nn));
+    acc += x * x;
+  }
+  sh[t] = acc;
+  __syncthreads();
+  for (int w = 256; w > 0; w >>= 1) {
+    if (t < w && t + w < a.L) sh[t] += sh[t + w];
+    __syncthreads();
+  }
+  if (t == 0) r[line] = sh[0];
+}
+
+// lines whose predicted iterate has the larger residual restart from q^n
+__global__ void line_select_kernel(const LineArgs a, const double* r_guess, const double* r_plain, double* flag) {
+  const long line = blockIdx.x;
+  const int t = threadIdx.x;
+  const bool fb = !(r_guess[line] <= r_plain[line]);
+  if (fb) {
+    const uint32_t n = line_node(a, line, t);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) stg(a.pv[v], n, ldg(a.q0[v], n));
+  }
+  if (t == 0) flag[line] = fb ? 1.0 : 0.0;
+}
+
+__global__ void sum_kernel(const double* x, long n, double* partial) {
+  __shared__ double s[256];
+  double acc = 0.0;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) acc += x[t];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+int host_sum(fedg_ctx* ctx, const double* x, long n, double* partial, double* partial_h, double* out) {
+  hipLaunchKernelGGL(sum_kernel, dim3(256), dim3(256), 0, ctx->stream, x, n, partial);
+  FEDG_HIP_CHECK(hipGetLastError());
+  FEDG_HIP_CHECK(hipMemcpyAsync(partial_h, partial, 256 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  double s = 0.0;
+  for (int b = 0; b < 256; ++b) s += partial_h[b];
+  *out = s;
+  return FEDG_OK;
+}
+
+LineArgs line_args(const DevMesh& m, const HeviArgs& a, const double* const Tex[5], const double scale[5], double dt) {
+  LineArgs l{};
+  for (int v = 0; v < 5; ++v) {
+    l.qs[v] = a.qs[v];
+    l.q0[v] = a.q0[v];
+    l.Tex[v] = Tex ? Tex[v] : nullptr;
+    l.pv[v] = a.pv[v];
+    l.s[v] = scale[v];
+    l.inv_s[v] = 1.0 / scale[v];
+  }
+  l.impl_fac = a.impl_fac;
+  l.dt = dt;
+  l.Np = m.Np;
+  l.Nfp = m.Nfp;
+  l.Ne2D = m.Ne2D;
+  l.N1 = m.n1;
+  l.L = a.NeZ * m.n1;
+  l.nlines = m.Ne2D * m.Nfp;
+  l.line0 = 0;
+  l.stride = 1;
+  return l;
+}
+
 template <int N1>
 size_t lds_uv(int NeZ) { return sizeof(double) * (N1 * N1 + N1 + (size_t)NeZ * 3 * N1) + sizeof(int) * 64; }
 template <int N1>
@@ -589,7 +720,7 @@ size_t lds_3(int NeZ) {
 
 template <int N1>
 int run_newton(fedg_ctx* ctx, HeviArgs& a, int max_it, double rtol, double* resid, double* partial,
-               double* partial_h, int* it_done) {
+               double* partial_h, int* it_done, double r0_ext = 0.0) {
   const DevMesh& m = ctx->m;
   const int nline = m.Ne2D * N1 * N1;
   const int nb = nline < 4096 ? nline : 4096;
@@ -599,7 +730,9 @@ int run_newton(fedg_ctx* ctx, HeviArgs& a, int max_it, double rtol, double* resi
     return FEDG_ERR_UNSUPPORTED;
   }
   const long n = (long)m.Np * m.Ne;
-  double r0 = 0.0;
+  // r0_ext > 0: the residual norm at q^n, the reference of the tolerance when
+  // the iteration starts from a predicted iterate (which may already meet it)
+  double r0 = r0_ext;
   int it = 0;
   for (; it < max_it; ++it) {
     hipLaunchKernelGGL(hevi_rhs_uv_kernel<N1>, dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
@@ -625,13 +758,95 @@ int run_newton(fedg_ctx* ctx, HeviArgs& a, int max_it, double rtol, double* resi
       for (int b = 0; b < 256; ++b) r2 += partial_h[b];
       const double r = sqrt(r2);
       if (resid) resid[it] = r;
-      if (it == 0) r0 = r;
-      if (rtol > 0.0 && it > 0 && r <= rtol * r0) break;
+      if (it == 0 && r0_ext <= 0.0) r0 = r;
+      if (rtol > 0.0 && (it > 0 || r0_ext > 0.0) && r <= rtol * r0) break;
     }
     hipLaunchKernelGGL(hevi_solve3_kernel<N1>, dim3(nb), dim3(64), l3, ctx->stream, a);
     FEDG_HIP_CHECK(hipGetLastError());
   }
   *it_done = it;
+  return FEDG_OK;
+}
+
+// Predicted Newton iterate from the network, with the per-line fallback to
+// q^n; returns ||G(q^n)|| (the tolerance's reference) in *r0.
+int guess_buffers(fedg_ctx* ctx, HeviWork* w, int n_in, int n_out) {
+  const DevMesh& m = ctx->m;
+  const long nlines = (long)m.Ne2D * m.Nfp;
+  if (w->X && w->n_in == n_in && w->n_out == n_out) return FEDG_OK;
+  if (w->X) (void)hipFree(w->X);
+  if (w->Y) (void)hipFree(w->Y);
+  if (w->r_plain) (void)hipFree(w->r_plain);
+  if (w->r_guess) (void)hipFree(w->r_guess);
+  if (w->flag) (void)hipFree(w->flag);
+  w->X = w->Y = nullptr;
+  w->r_plain = w->r_guess = w->flag = nullptr;
+  w->chunk = (int)(nlines < 16384 ? nlines : 16384);
+  w->n_in = n_in;
+  w->n_out = n_out;
+  FEDG_HIP_CHECK(hipMalloc((void**)&w->X, sizeof(float) * (size_t)w->chunk * n_in));
+  FEDG_HIP_CHECK(hipMalloc((void**)&w->Y, sizeof(float) * (size_t)w->chunk * n_out));
+  FEDG_HIP_CHECK(hipMalloc((void**)&w->r_plain, sizeof(double) * nlines));
+  FEDG_HIP_CHECK(hipMalloc((void**)&w->r_guess, sizeof(double) * nlines));
+  FEDG_HIP_CHECK(hipMalloc((void**)&w->flag, sizeof(double) * nlines));
+  return FEDG_OK;
+}
+
+template <int N1>
+int predict_iterate(fedg_ctx* ctx, HeviWork* w, HeviArgs& a, HeviGuess* g, double* r0) {
+  const DevMesh& m = ctx->m;
+  const long nlines = (long)m.Ne2D * m.Nfp;
+  const int L = a.NeZ * N1;
+  int n_in = 0, n_out = 0;
+  mlp_dims(g->mlp, &n_in, &n_out);
+  if (n_in != 10 * L + 1 || n_out != 5 * L || L > 512) {
+    set_error("HEVI guess: the network's sizes do not match 10 NeZ (p+1) + 1 inputs and 5 NeZ (p+1) outputs");
+    return FEDG_ERR_ARG;
+  }
+  const int rcb = guess_buffers(ctx, w, n_in, n_out);
+  if (rcb != FEDG_OK) return rcb;
+  LineArgs la = line_args(m, a, g->Tex, g->scale, g->dt);
+  const long nn = (long)m.Np * m.Ne;
+  // residual of every line at q^n (pv holds q^n)
+  hipLaunchKernelGGL(hevi_rhs_uv_kernel<N1>, dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
+  FEDG_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(hevi_rhs3_kernel<N1>, dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
+  FEDG_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(line_resid_kernel, dim3(nlines), dim3(L), 0, ctx->stream, la, a.buv, a.b3, nn, w->r_plain);
+  FEDG_HIP_CHECK(hipGetLastError());
+  double r2 = 0.0;
+  const int rcs = host_sum(ctx, w->r_plain, nlines, w->partial, w->partial_h, &r2);
+  if (rcs != FEDG_OK) return rcs;
+  *r0 = sqrt(r2);
+  // network prediction, chunk by chunk
+  for (long l0 = 0; l0 < nlines; l0 += w->chunk) {
+    const int rows = (int)((nlines - l0) < w->chunk ? (nlines - l0) : w->chunk);
+    la.line0 = l0;
+    la.stride = 1;
+    hipLaunchKernelGGL(line_features_kernel, dim3(rows), dim3(L), 0, ctx->stream, la, w->X, n_in, (float*)nullptr,
+                       n_out);
+    FEDG_HIP_CHECK(hipGetLastError());
+    FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    const int rcf = mlp_forward(g->mlp, w->X, w->Y, rows);
+    if (rcf != FEDG_OK) return rcf;
+    hipLaunchKernelGGL(line_apply_kernel, dim3(rows), dim3(L), 0, ctx->stream, la, w->Y, n_out);
+    FEDG_HIP_CHECK(hipGetLastError());
+  }
+  // residual at the prediction; lines where it is not smaller restart from q^n
+  la.line0 = 0;
+  hipLaunchKernelGGL(hevi_rhs_uv_kernel<N1>, dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
+  FEDG_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(hevi_rhs3_kernel<N1>, dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
+  FEDG_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(line_resid_kernel, dim3(nlines), dim3(L), 0, ctx->stream, la, a.buv, a.b3, nn, w->r_guess);
+  FEDG_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(line_select_kernel, dim3(nlines), dim3(L), 0, ctx->stream, la, w->r_guess, w->r_plain, w->flag);
+  FEDG_HIP_CHECK(hipGetLastError());
+  double nfb = 0.0;
+  const int rcs2 = host_sum(ctx, w->flag, nlines, w->partial, w->partial_h, &nfb);
+  if (rcs2 != FEDG_OK) return rcs2;
+  g->lines += nlines;
+  g->fallbacks += (long)(nfb + 0.5);
   return FEDG_OK;
 }
 
@@ -650,7 +865,7 @@ int run_operator(fedg_ctx* ctx, HeviArgs& a) {
 int hevi_cal_vi(fedg_ctx* ctx, HeviWork* w, double* const tend[5], const double* const q[5],
                 const double* const var0[5], const double* DENS_hyd, const double* PRES_hyd, const double* Rtot,
                 const double* CVtot, const double* CPtot, double impl_fac, int max_it, double rtol, double* resid,
-                int* it_done) {
+                int* it_done, HeviGuess* guess) {
   const DevMesh& m = ctx->m;
   HeviArgs a{};
   a.m = m;
@@ -680,7 +895,12 @@ int hevi_cal_vi(fedg_ctx* ctx, HeviWork* w, double* const tend[5], const double*
     constexpr int N1 = decltype(n1c)::value;
     if (fabs(impl_fac) > 0.0) {
       a.want_b = 1;
-      const int rc = run_newton<N1>(ctx, a, max_it, rtol, resid, w->partial, w->partial_h, it_done);
+      double r0 = 0.0;
+      if (guess && guess->mlp) {
+        const int rc = predict_iterate<N1>(ctx, w, a, guess, &r0);
+        if (rc != FEDG_OK) return rc;
+      }
+      const int rc = run_newton<N1>(ctx, a, max_it, rtol, resid, w->partial, w->partial_h, it_done, r0);
       if (rc != FEDG_OK) return rc;
       const int bs = 256;
       hipLaunchKernelGGL(hevi_tend_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, ctx->stream, tend[VD], tend[VT],
@@ -704,6 +924,29 @@ int hevi_cal_vi(fedg_ctx* ctx, HeviWork* w, double* const tend[5], const double*
       set_error("polynomial order not built (supported: p = 1..7)");
       return FEDG_ERR_UNSUPPORTED;
   }
+}
+
+int hevi_harvest(fedg_ctx* ctx, HeviWork* w, const double* const q[5], const double* const var0[5],
+                 const double* const Tex[5], const double scale[5], double impl_fac, double dt, long line0,
+                 long stride, int nrows, float* X, float* Y) {
+  const DevMesh& m = ctx->m;
+  HeviArgs a{};
+  a.m = m;
+  for (int v = 0; v < 5; ++v) {
+    a.qs[v] = q[v];
+    a.q0[v] = var0[v];
+    a.pv[v] = w->pv[v];
+  }
+  a.impl_fac = impl_fac;
+  a.NeZ = m.Ne / m.Ne2D;
+  LineArgs la = line_args(m, a, Tex, scale, dt);
+  if (la.L > 512 || nrows < 1) return FEDG_ERR_ARG;
+  la.line0 = line0;
+  la.stride = stride;
+  hipLaunchKernelGGL(line_features_kernel, dim3(nrows), dim3(la.L), 0, ctx->stream, la, X, 10 * la.L + 1, Y,
+                     5 * la.L);
+  FEDG_HIP_CHECK(hipGetLastError());
+  return FEDG_OK;
 }
 
 }  // namespace fedg

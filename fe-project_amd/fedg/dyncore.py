@@ -63,7 +63,7 @@ class DevField:
 
 
 def _p(f):
-    return f.ptr if isinstance(f, DevField) else f
+    return f.ptr if isinstance(f, (DevField, DevField32)) else f
 
 
 def heve_cal_tend(dm: DeviceMesh, out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DPRES, DENS_hyd, PRES_hyd, PRES_hyd_ref,
@@ -281,6 +281,26 @@ class HeviStepper:
               "fedg_hevi_run_timed")
         return tot.value, im.value, ex.value
 
+    def set_guess(self, mlp, scale):
+        """Initial Newton iterate from mlp (None: q^n); scale: 5 per-variable scales (VID order)."""
+        sc = (C.c_double * 5)(*[float(x) for x in scale])
+        check(LIB.fedg_hevi_set_guess(self.handle, None if mlp is None else mlp.handle, sc), "fedg_hevi_set_guess")
+
+    def guess_stats(self):
+        lines, fb = C.c_long(), C.c_long()
+        check(LIB.fedg_hevi_guess_stats(self.handle, C.byref(lines), C.byref(fb)), "fedg_hevi_guess_stats")
+        return lines.value, fb.value
+
+    def harvest(self, X: DevField, Y: DevField, capacity: int, per_stage: int, seed: int = 1):
+        """Collect (features, converged targets) rows into the fp32 device buffers X, Y during later steps."""
+        check(LIB.fedg_hevi_harvest(self.handle, _p(X), _p(Y), int(capacity), int(per_stage), int(seed)),
+              "fedg_hevi_harvest")
+
+    def harvest_count(self):
+        c = C.c_int()
+        check(LIB.fedg_hevi_harvest_count(self.handle, C.byref(c)), "fedg_hevi_harvest_count")
+        return c.value
+
     def newton_stats(self):
         it, calls = C.c_long(), C.c_long()
         check(LIB.fedg_hevi_newton_stats(self.handle, C.byref(it), C.byref(calls)), "fedg_hevi_newton_stats")
@@ -294,6 +314,94 @@ class HeviStepper:
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+class Mlp:
+    """Per-column MLP on the GPU (fedg_mlp_*): BF16 matrix-core products, FP32 accumulation and masters."""
+
+    def __init__(self, n_in, n_hidden, n_hidden_layers, n_out, seed=1):
+        h = C.c_void_p()
+        check(LIB.fedg_mlp_create(int(n_in), int(n_hidden), int(n_hidden_layers), int(n_out), int(seed), C.byref(h)),
+              "fedg_mlp_create")
+        self.handle = h
+        self.n_in, self.n_out = n_in, n_out
+        n = C.c_long()
+        check(LIB.fedg_mlp_param_count(h, C.byref(n)), "fedg_mlp_param_count")
+        self.nparam = n.value
+
+    def params(self):
+        p = np.empty(self.nparam, dtype=np.float32)
+        check(LIB.fedg_mlp_get_params(self.handle, p.ctypes.data_as(C.c_void_p), None), "fedg_mlp_get_params")
+        return p
+
+    def grads(self):
+        g = np.empty(self.nparam, dtype=np.float32)
+        check(LIB.fedg_mlp_get_params(self.handle, None, g.ctypes.data_as(C.c_void_p)), "fedg_mlp_get_params")
+        return g
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check(LIB.fedg_mlp_set_params(self.handle, p.ctypes.data_as(C.c_void_p)), "fedg_mlp_set_params")
+
+    def forward(self, X: "DevField32", Y: "DevField32", batch: int):
+        check(LIB.fedg_mlp_forward(self.handle, _p(X), _p(Y), int(batch)), "fedg_mlp_forward")
+
+    def loss_grad(self, X, T, batch: int) -> float:
+        loss = C.c_float()
+        check(LIB.fedg_mlp_loss_grad(self.handle, _p(X), _p(T), int(batch), C.byref(loss)), "fedg_mlp_loss_grad")
+        return loss.value
+
+    def adam_step(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        check(LIB.fedg_mlp_adam_step(self.handle, float(lr), float(beta1), float(beta2), float(eps)),
+              "fedg_mlp_adam_step")
+
+    def close(self):
+        if self.handle:
+            LIB.fedg_mlp_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DevField32:
+    """A device float32 array (the MLP's inputs and outputs)."""
+
+    def __init__(self, n: int, host: np.ndarray | None = None):
+        self.n = int(n)
+        self.ptr = C.c_void_p()
+        check(LIB.fedg_dev_alloc(C.byref(self.ptr), 4 * max(self.n, 1)), "fedg_dev_alloc")
+        if host is not None:
+            self.upload(host)
+
+    def upload(self, host: np.ndarray):
+        h = np.ascontiguousarray(host, dtype=np.float32).ravel()
+        assert h.size <= self.n
+        check(LIB.fedg_memcpy_h2d(self.ptr, h.ctypes.data_as(C.c_void_p), 4 * h.size), "fedg_memcpy_h2d")
+
+    def download(self, n=None) -> np.ndarray:
+        n = self.n if n is None else int(n)
+        h = np.empty(n, dtype=np.float32)
+        check(LIB.fedg_memcpy_d2h(h.ctypes.data_as(C.c_void_p), self.ptr, 4 * n), "fedg_memcpy_d2h")
+        return h
+
+    @property
+    def _as_parameter_(self):
+        return self.ptr
+
+    def free(self):
+        if self.ptr:
+            LIB.fedg_dev_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
         except Exception:
             pass
 

@@ -321,6 +321,39 @@ int fedg_hevi_run_timed(fedg_hevi *st, int nsteps, double *ms_total, double *ms_
 /* Newton iterations summed over the implicit stages run so far, and their number */
 int fedg_hevi_newton_stats(const fedg_hevi *st, long *iterations, long *calls);
 
+
+/* ------------------------------------------------------------------ */
+/* Learned initial iterate of the HEVI Newton solve.  No counterpart in the
+ * reference (its Newton loop starts from q^n and stops after one iteration,
+ * FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi.F90:868-917).  A dense ReLU
+ * network n_in -> n_hidden (x n_hidden_layers) -> n_out, BF16 products on
+ * the matrix cores with FP32 accumulation, FP32 master weights, Adam.  For a
+ * vertical node line of L = NeZ (p+1) nodes: n_in = 10 L + 1 (the line's
+ * (q* - q^n) / scale and impl_fac T_ex / scale per variable, then
+ * impl_fac/dt), n_out = 5 L ((q - q*) / scale of the converged iterate q).
+ * Device arrays are fp32, row-major (batch x features). */
+typedef struct fedg_mlp fedg_mlp;
+int fedg_mlp_create(int n_in, int n_hidden, int n_hidden_layers, int n_out, unsigned seed, fedg_mlp **out);
+int fedg_mlp_destroy(fedg_mlp *m);
+int fedg_mlp_param_count(const fedg_mlp *m, long *n);
+/* host copies: every layer's W (n_out x n_in row-major), then every bias */
+int fedg_mlp_get_params(fedg_mlp *m, float *params_host, float *grads_host);
+int fedg_mlp_set_params(fedg_mlp *m, const float *params_host);
+int fedg_mlp_forward(fedg_mlp *m, const float *X_dev, float *Y_dev, int batch);
+/* mean squared error against T_dev and the gradient of every parameter */
+int fedg_mlp_loss_grad(fedg_mlp *m, const float *X_dev, const float *T_dev, int batch, float *loss);
+int fedg_mlp_adam_step(fedg_mlp *m, float lr, float beta1, float beta2, float eps);
+/* Use mlp (or none: NULL) for the initial iterate of every implicit stage of
+ * the HEVI stepper; lines whose predicted iterate has a larger residual than
+ * q^n start from q^n.  scale: the 5 per-variable scales (VID order). */
+int fedg_hevi_set_guess(fedg_hevi *st, fedg_mlp *mlp, const double *scale);
+int fedg_hevi_guess_stats(const fedg_hevi *st, long *lines, long *fallbacks);
+/* Collect training pairs from the following steps: per implicit stage,
+ * per_stage lines' (features, converged targets) appended to X_dev / Y_dev
+ * until capacity rows are stored. */
+int fedg_hevi_harvest(fedg_hevi *st, float *X_dev, float *Y_dev, int capacity, int per_stage, unsigned seed);
+int fedg_hevi_harvest_count(const fedg_hevi *st, int *count);
+
 #ifdef __cplusplus
 }
 #endif
