@@ -45,7 +45,7 @@ int dev_upload(T** dst, const T* src, size_t count) {
 void free_mesh(DevMesh& m) {
   void* ptrs[] = {m.E11, m.E22, m.E33, m.Gsqrt, m.G13, m.G23, m.GsqrtH, m.G11, m.G12, m.G22, m.nx, m.ny, m.nz,
                   m.Fscale, m.vmapM, m.vmapP, m.EMap3Dto2D, m.halo_src, m.velbc, m.nbr, m.nbr_bc, m.D1D,
-                  m.Lift, m.Itr, m.liftT};
+                  m.Lift, m.Itr, m.liftT, m.MFh, m.MFv};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   memset(&m, 0, sizeof(m));
@@ -87,6 +87,7 @@ struct fedg_heve {
   double *Rtot, *CVtot, *CPtot;
   double* pres_scratch;     // (Np,Ne) x2 for PRES/DPRES
   int fp32;                 // stage tendency in the FP32 perturbation form
+  int mfilter;              // modal filter after every step
 };
 
 namespace {
@@ -235,6 +236,44 @@ int fedg_refelem_tables(int porder, double* x1d, double* w1d, double* D1D, doubl
   cp(IntrpMat_VPOrdM1_tr, e.IntrpMat_VPOrdM1_tr);
   cp(IndexH2Dto3D, e.IndexH2Dto3D);
   cp(IndexH2Dto3D_bnd, e.IndexH2Dto3D_bnd);
+  return FEDG_OK;
+}
+
+// ModalFilter_Init_line with get_exp_filter (ELM/scale_element_modalfilter.F90:60-90, 196-236):
+// FilterMat = V diag(sigma) V^-1, sigma_p = exp(-alpha ((eta - etac)/(1 - etac))^ord)
+// for eta = (p-1)/porder > etac (p > 1), else 1.  F is (n1,n1) column-major.
+int fedg_modalfilter_matrix(int porder, double etac, double alpha, int ord, double* F) {
+  if (!F || ord < 0 || !(etac < 1.0)) {
+    set_error("modal filter: bad arguments (F, ord >= 0, etac < 1)");
+    return FEDG_ERR_ARG;
+  }
+  RefElem e;
+  try {
+    build_refelem(porder, e);
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+    return FEDG_ERR_ARG;
+  }
+  const int n1 = e.n1;
+  std::vector<double> sigma(n1, 1.0);
+  for (int p = 0; p < n1; ++p) {
+    const double eta = (double)p / (double)porder;
+    if (eta > etac && p != 0) {
+      const double x = (eta - etac) / (1.0 - etac);
+      double r = 1.0, b = x;  // x**ord by binary powering
+      for (int k = ord; k > 0; k >>= 1) {
+        if (k & 1) r *= b;
+        b *= b;
+      }
+      sigma[p] = std::exp(-alpha * r);
+    }
+  }
+  for (int j = 0; j < n1; ++j)
+    for (int i = 0; i < n1; ++i) {
+      double acc = 0.0;
+      for (int l = 0; l < n1; ++l) acc += e.V1D[i + n1 * l] * (sigma[l] * e.invV1D[l + n1 * j]);
+      F[i + n1 * j] = acc;
+    }
   return FEDG_OK;
 }
 
@@ -513,6 +552,27 @@ int fedg_ctx_create(const fedg_mesh_desc* d, fedg_ctx** out) {
   return FEDG_OK;
 }
 
+int fedg_ctx_setup_modalfilter(fedg_ctx* ctx, const double* MFilter_h1D, const double* MFilter_v1D) {
+  if (!ctx || !MFilter_h1D || !MFilter_v1D) return FEDG_ERR_ARG;
+  DevMesh& m = ctx->m;
+  const size_t nn = (size_t)m.n1 * m.n1;
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  if (m.MFh) (void)hipFree(m.MFh);
+  if (m.MFv) (void)hipFree(m.MFv);
+  m.MFh = m.MFv = nullptr;
+  int rc = dev_upload(&m.MFh, MFilter_h1D, nn);
+  if (rc == FEDG_OK) rc = dev_upload(&m.MFv, MFilter_v1D, nn);
+  return rc;
+}
+
+int fedg_modalfilter_apply(fedg_ctx* ctx, double* DDENS, double* MOMX, double* MOMY, double* MOMZ, double* DRHOT,
+                           int do_weight_Gsqrt) {
+  if (!ctx || !DDENS || !MOMX || !MOMY || !MOMZ || !DRHOT) return FEDG_ERR_ARG;
+  TRY(launch_modalfilter(ctx, DDENS, MOMX, MOMY, MOMZ, DRHOT, do_weight_Gsqrt));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return FEDG_OK;
+}
+
 int fedg_ctx_destroy(fedg_ctx* ctx) {
   if (!ctx) return FEDG_OK;
   hevi_work_free(ctx);
@@ -692,7 +752,7 @@ static int hevi_cal_vi_common(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, d
                               const double* PRES_hyd, const double* DDENS0_, const double* MOMX0_,
                               const double* MOMY0_, const double* MOMZ0_, const double* DRHOT0_, const double* Rtot,
                               const double* CVtot, const double* CPtot, double impl_fac, int max_it, double rtol,
-                              double* resid, int* iterations) {
+                              double* resid, int* iterations, HeviGuess* guess = nullptr) {
   if (!ctx || !DENS_dt || !MOMX_dt || !MOMY_dt || !MOMZ_dt || !RHOT_dt || !DDENS_ || !MOMX_ || !MOMY_ || !MOMZ_ ||
       !DRHOT_ || !DENS_hyd || !PRES_hyd || !DDENS0_ || !MOMX0_ || !MOMY0_ || !MOMZ0_ || !DRHOT0_ || !Rtot ||
       !CVtot || !CPtot || max_it < 1 || !std::isfinite(impl_fac))
@@ -703,7 +763,8 @@ static int hevi_cal_vi_common(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, d
   const double* q[5] = {DDENS_, DRHOT_, MOMZ_, MOMX_, MOMY_};
   const double* q0[5] = {DDENS0_, DRHOT0_, MOMZ0_, MOMX0_, MOMY0_};
   int it = 0;
-  TRY(hevi_cal_vi(ctx, w, tend, q, q0, DENS_hyd, PRES_hyd, Rtot, CVtot, CPtot, impl_fac, max_it, rtol, resid, &it));
+  TRY(hevi_cal_vi(ctx, w, tend, q, q0, DENS_hyd, PRES_hyd, Rtot, CVtot, CPtot, impl_fac, max_it, rtol, resid, &it,
+                  guess));
   FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   if (iterations) *iterations = it;
   return FEDG_OK;
@@ -732,6 +793,33 @@ int fedg_hevi_cal_vi_newton(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, dou
   return hevi_cal_vi_common(ctx, DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt, DDENS_, MOMX_, MOMY_, MOMZ_, DRHOT_,
                             DENS_hyd, PRES_hyd, DDENS0_, MOMX0_, MOMY0_, MOMZ0_, DRHOT0_, Rtot, CVtot, CPtot,
                             impl_fac, max_it, rtol, resid, iterations);
+}
+
+struct fedg_mlp {
+  fedg::MlpImpl* impl;
+};
+
+int fedg_hevi_cal_vi_guess(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, double* MOMY_dt, double* MOMZ_dt,
+                           double* RHOT_dt, const double* DDENS_, const double* MOMX_, const double* MOMY_,
+                           const double* MOMZ_, const double* DRHOT_, const double* DENS_hyd, const double* PRES_hyd,
+                           const double* DDENS0_, const double* MOMX0_, const double* MOMY0_, const double* MOMZ0_,
+                           const double* DRHOT0_, const double* Rtot, const double* CVtot, const double* CPtot,
+                           double impl_fac, double dt, int max_it, double rtol, fedg_mlp* mlp, const double* scale,
+                           const double* const* Tex, double* resid, int* iterations, long* fallbacks) {
+  if (!mlp || !scale || !(dt > 0.0) || !(fabs(impl_fac) > 0.0)) return FEDG_ERR_ARG;
+  HeviGuess g{};
+  g.mlp = mlp->impl;
+  for (int v = 0; v < 10; ++v) {
+    if (!(scale[v] > 0.0)) return FEDG_ERR_ARG;
+    g.scale[v] = scale[v];
+  }
+  for (int v = 0; v < 5; ++v) g.Tex[v] = Tex ? Tex[v] : nullptr;
+  g.dt = dt;
+  TRY(hevi_cal_vi_common(ctx, DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt, DDENS_, MOMX_, MOMY_, MOMZ_, DRHOT_,
+                         DENS_hyd, PRES_hyd, DDENS0_, MOMX0_, MOMY0_, MOMZ0_, DRHOT0_, Rtot, CVtot, CPtot, impl_fac,
+                         max_it, rtol, resid, iterations, &g));
+  if (fallbacks) *fallbacks = g.fallbacks;
+  return FEDG_OK;
 }
 
 int fedg_calc_phyd_hgrad(fedg_ctx* ctx, double* DPhydDx, double* DPhydDy, const double* PRES_hyd,
@@ -963,6 +1051,16 @@ int fedg_heve_set_precision(fedg_heve* st, int bits) {
   return FEDG_OK;
 }
 
+int fedg_heve_set_modalfilter(fedg_heve* st, int on) {
+  if (!st) return FEDG_ERR_ARG;
+  if (on && !st->ctx->m.MFh) {
+    set_error("modal filter: matrices not set up (fedg_ctx_setup_modalfilter)");
+    return FEDG_ERR_ARG;
+  }
+  st->mfilter = on ? 1 : 0;
+  return FEDG_OK;
+}
+
 int fedg_heve_set_state(fedg_heve* st, const double* const* q) {
   if (!st || !q) return FEDG_ERR_ARG;
   fedg_ctx* ctx = st->ctx;
@@ -1036,6 +1134,13 @@ static int heve_step(fedg_heve* st, std::vector<hipEvent_t>* evs, size_t* ev_i) 
     last_out = outb;
   }
   st->q0 = last_out;
+  if (st->mfilter) {  // FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:940-950
+    double* const* q = st->buf[st->q0];
+    TRY(launch_modalfilter(ctx, q[0], q[3], q[4], q[2], q[1], 1));
+    const DevMesh& m = ctx->m;
+    TRY(launch_dpres(ctx, st->DP[st->q0], q[1], st->PRES_hyd, st->THERM_hyd, st->Rtot, st->CVtot, st->CPtot,
+                     (long)m.Np * m.Ne));
+  }
   return FEDG_OK;
 }
 
@@ -1127,10 +1232,11 @@ struct fedg_hevi {
   double *DENS_hyd, *PRES_hyd, *THERM_hyd, *DPhydDx, *DPhydDy, *Rtot, *CVtot, *CPtot, *CORIOLIS;
   int max_it;
   double rtol;
+  int mfilter;  // modal filter after every step
   long newton_iters, newton_calls;
   // learned initial guess and training-pair harvest
   fedg::MlpImpl* mlp;
-  double scale[5];
+  double scale[10];  // 5 state scales, then 5 increment scales (VID order)
   long guess_lines, guess_fallbacks;
   float* hx;
   float* hy;
@@ -1215,6 +1321,16 @@ int fedg_hevi_get_state(fedg_hevi* st, double* const* q) {
   return FEDG_OK;
 }
 
+int fedg_hevi_set_modalfilter(fedg_hevi* st, int on) {
+  if (!st) return FEDG_ERR_ARG;
+  if (on && !st->ctx->m.MFh) {
+    set_error("modal filter: matrices not set up (fedg_ctx_setup_modalfilter)");
+    return FEDG_ERR_ARG;
+  }
+  st->mfilter = on ? 1 : 0;
+  return FEDG_OK;
+}
+
 int fedg_hevi_newton_stats(const fedg_hevi* st, long* iterations, long* calls) {
   if (!st) return FEDG_ERR_ARG;
   if (iterations) *iterations = st->newton_iters;
@@ -1251,10 +1367,8 @@ static int hevi_step(fedg_hevi* st, hipEvent_t* ev) {
       for (int v = 0; v < 5; ++v) Tprev[v] = tend_ptr(t, t->tend_ex, t->indmap[s - 2], v + 1);
     HeviGuess g{};
     g.mlp = st->mlp;
-    for (int v = 0; v < 5; ++v) {
-      g.Tex[v] = Tprev[v];
-      g.scale[v] = st->scale[v];
-    }
+    for (int v = 0; v < 5; ++v) g.Tex[v] = Tprev[v];
+    for (int v = 0; v < 10; ++v) g.scale[v] = st->scale[v];
     g.dt = t->dt;
     if (ev) FEDG_HIP_CHECK(hipEventRecord(ev[0], ctx->stream));
     TRY(hevi_cal_vi(ctx, w, Ti, qv, v0, st->DENS_hyd, st->PRES_hyd, st->Rtot, st->CVtot, st->CPtot, impl_fac,
@@ -1307,6 +1421,7 @@ static int hevi_step(fedg_hevi* st, hipEvent_t* ev) {
     for (int v = 0; v < 5; ++v) TRY(tint_advance_one(t, s, st->q[v], v + 1));
     if (ev) ev += 4;
   }
+  if (st->mfilter) TRY(launch_modalfilter(ctx, st->q[0], st->q[3], st->q[4], st->q[2], st->q[1], 1));
   return FEDG_OK;
 }
 
@@ -1358,9 +1473,6 @@ int fedg_hevi_run_timed(fedg_hevi* st, int nsteps, double* ms_total, double* ms_
 
 // ---------------------------------------------------------------------------
 // Per-column MLP and its use as the HEVI Newton initial guess
-struct fedg_mlp {
-  fedg::MlpImpl* impl;
-};
 
 int fedg_mlp_create(int n_in, int n_hidden, int n_hidden_layers, int n_out, unsigned seed, fedg_mlp** out) {
   if (!out) return FEDG_ERR_ARG;
@@ -1411,10 +1523,9 @@ int fedg_mlp_adam_step(fedg_mlp* m, float lr, float beta1, float beta2, float ep
 
 int fedg_hevi_set_guess(fedg_hevi* st, fedg_mlp* mlp, const double* scale) {
   if (!st || !scale) return FEDG_ERR_ARG;
-  for (int v = 0; v < 5; ++v) {
+  for (int v = 0; v < 10; ++v)
     if (!(scale[v] > 0.0)) return FEDG_ERR_ARG;
-    st->scale[v] = scale[v];
-  }
+  for (int v = 0; v < 10; ++v) st->scale[v] = scale[v];
   st->mlp = mlp ? mlp->impl : nullptr;
   return FEDG_OK;
 }

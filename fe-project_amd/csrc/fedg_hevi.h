@@ -25,13 +25,14 @@ struct HeviWork {
 struct MlpImpl;
 
 // Learned initial iterate of the Newton solve: the network maps a line's
-// (q* - q^n, impl_fac T_ex) / scale and impl_fac/dt to (q - q*) / scale of
-// the converged iterate q.  Lines whose predicted iterate has a larger
-// residual than q^n start from q^n (fallback).
+// stage state q* / s, impl_fac T_ex / d and impl_fac/dt to (q - q*) / d of
+// the converged iterate q (s: state scales, d: increment scales, per
+// variable).  Lines whose predicted iterate has a larger residual than q^n
+// start from q^n (fallback).
 struct HeviGuess {
   MlpImpl* mlp;
   const double* Tex[5];  // explicit tendency of the previous stage (VID order), or null
-  double scale[5];
+  double scale[10];      // s (VID order), then d
   double dt;
   long lines;            // out: lines solved
   long fallbacks;        // out: lines that fell back to q^n
@@ -54,7 +55,7 @@ int hevi_cal_vi(fedg_ctx* ctx, HeviWork* w, double* const tend[5], const double*
 // of a converged solve (iterate in w->pv) into X (nrows x n_in) and Y
 // (nrows x n_out), device fp32; the network's input/output layout.
 int hevi_harvest(fedg_ctx* ctx, HeviWork* w, const double* const q[5], const double* const var0[5],
-                 const double* const Tex[5], const double scale[5], double impl_fac, double dt, long line0,
+                 const double* const Tex[5], const double scale[10], double impl_fac, double dt, long line0,
                  long stride, int nrows, float* X, float* Y);
 
 }  // namespace fedg

@@ -589,7 +589,8 @@ struct LineArgs {
   const double* q0[5];
   const double* Tex[5];
   double* pv[5];
-  double inv_s[5], s[5];
+  double inv_s[5];      // 1 / state scale
+  double inv_d[5], d[5];  // increment scale
   double impl_fac, dt;
   int Np, Nfp, Ne2D, N1, L, nlines;
   long line0, stride;
@@ -609,10 +610,10 @@ __global__ void line_features_kernel(const LineArgs a, float* X, int n_in, float
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
     const double qs = ldg(a.qs[v], n);
-    x[v * a.L + t] = (float)((qs - ldg(a.q0[v], n)) * a.inv_s[v]);
+    x[v * a.L + t] = (float)(qs * a.inv_s[v]);
     const double te = a.Tex[v] ? ldg(a.Tex[v], n) : 0.0;
-    x[(5 + v) * a.L + t] = (float)(a.impl_fac * te * a.inv_s[v]);
-    if (Y) Y[(size_t)row * n_out + v * a.L + t] = (float)((ldg(a.pv[v], n) - qs) * a.inv_s[v]);
+    x[(5 + v) * a.L + t] = (float)(a.impl_fac * te * a.inv_d[v]);
+    if (Y) Y[(size_t)row * n_out + v * a.L + t] = (float)((ldg(a.pv[v], n) - qs) * a.inv_d[v]);
   }
   if (t == 0) x[10 * a.L] = (float)(a.impl_fac / a.dt);
 }
@@ -623,7 +624,7 @@ __global__ void line_apply_kernel(const LineArgs a, const float* Y, int n_out) {
   const uint32_t n = line_node(a, line, t);
 #pragma unroll
   for (int v = 0; v < 5; ++v)
-    stg(a.pv[v], n, ldg(a.qs[v], n) + a.s[v] * (double)Y[(size_t)row * n_out + v * a.L + t]);
+    stg(a.pv[v], n, ldg(a.qs[v], n) + a.d[v] * (double)Y[(size_t)row * n_out + v * a.L + t]);
 }
 
 // residual norm^2 of every line from the uv and var3 right-hand sides
@@ -687,15 +688,16 @@ int host_sum(fedg_ctx* ctx, const double* x, long n, double* partial, double* pa
   return FEDG_OK;
 }
 
-LineArgs line_args(const DevMesh& m, const HeviArgs& a, const double* const Tex[5], const double scale[5], double dt) {
+LineArgs line_args(const DevMesh& m, const HeviArgs& a, const double* const Tex[5], const double scale[10], double dt) {
   LineArgs l{};
   for (int v = 0; v < 5; ++v) {
     l.qs[v] = a.qs[v];
     l.q0[v] = a.q0[v];
     l.Tex[v] = Tex ? Tex[v] : nullptr;
     l.pv[v] = a.pv[v];
-    l.s[v] = scale[v];
     l.inv_s[v] = 1.0 / scale[v];
+    l.d[v] = scale[5 + v];
+    l.inv_d[v] = 1.0 / scale[5 + v];
   }
   l.impl_fac = a.impl_fac;
   l.dt = dt;
@@ -927,7 +929,7 @@ int hevi_cal_vi(fedg_ctx* ctx, HeviWork* w, double* const tend[5], const double*
 }
 
 int hevi_harvest(fedg_ctx* ctx, HeviWork* w, const double* const q[5], const double* const var0[5],
-                 const double* const Tex[5], const double scale[5], double impl_fac, double dt, long line0,
+                 const double* const Tex[5], const double scale[10], double impl_fac, double dt, long line0,
                  long stride, int nrows, float* X, float* Y) {
   const DevMesh& m = ctx->m;
   HeviArgs a{};

@@ -54,6 +54,8 @@ struct DevMesh {
   // i (f = 2,4), k (f = 5,6); lift_tensor = 0 otherwise
   int lift_tensor;
   double* liftT;               // (n1,6)
+  double* MFh;                 // (n1,n1) modal filter MFilter_h1D, or null
+  double* MFv;                 // (n1,n1) MFilter_v1D
   // flat uniform box description (valid when flat != 0)
   int flat;
   double fE[3];                // E11,E22,E33 of every element
@@ -124,5 +126,8 @@ int launch_axpy2_acc(fedg_ctx* ctx, double* y, double ca, const double* x, doubl
 int launch_axpy2_set(fedg_ctx* ctx, double* y, const double* a, double ca, const double* x, double cb,
                      const double* z, long n);  // y = a + ca*x + cb*z
 int launch_copy(fedg_ctx* ctx, double* y, const double* x, long n);
+// atm_dyn_dgm_modalfilter_apply on elements 1..Ne (needs m.MFh, m.MFv)
+int launch_modalfilter(fedg_ctx* ctx, double* DDENS, double* MOMX, double* MOMY, double* MOMZ, double* DRHOT,
+                       int do_weight_Gsqrt);
 
 }  // namespace fedg

@@ -1251,6 +1251,91 @@ int launch_phyd_hgrad(fedg_ctx* ctx, double* DPhydDx, double* DPhydDy, const dou
   }
 }
 
+// Modal filter of atm_dyn_dgm_modalfilter_apply (FDS/scale_atm_dyn_dgm_modalfilter.F90:48-130):
+// per element and variable, the x pass (MFilter_h1D), y pass (its transpose
+// applied from the right) and z pass (MFilter_v1D) of
+// element_operation_kernel_matvec_ModalFilter_P* (ELM/..._tensorprod3D_kernel.F90:1061-1112),
+// each sum over l in order, on Gsqrt-weighted values when weight != 0.
+struct Fields5 {
+  double* f[5];
+};
+
+template <int N1, int EPB>
+__global__ void __launch_bounds__(EPB* N1* N1* N1) modalfilter_kernel(Fields5 F, const double* __restrict__ Gsqrt,
+                                                                     const double* __restrict__ MFh,
+                                                                     const double* __restrict__ MFv, int Ne,
+                                                                     int weight) {
+  constexpr int Np = N1 * N1 * N1;
+  __shared__ double sh[2 * N1 * N1];
+  __shared__ double a[EPB * Np], b[EPB * Np];
+  const int le = threadIdx.x / Np, t = threadIdx.x % Np;
+  const int ke = blockIdx.x * EPB + le;
+  for (int u = threadIdx.x; u < 2 * N1 * N1; u += blockDim.x) sh[u] = u < N1 * N1 ? MFh[u] : MFv[u - N1 * N1];
+  const bool act = ke < Ne;
+  const int i = t % N1, j = (t / N1) % N1, k = t / (N1 * N1);
+  const size_t o = (size_t)ke * Np + t;
+  const double G = (act && weight) ? Gsqrt[o] : 1.0;
+  const double* Fh = sh;
+  const double* Fv = sh + N1 * N1;
+  double* A = a + le * Np;
+  double* B = b + le * Np;
+  for (int v = 0; v < 5; ++v) {
+    double x = act ? F.f[v][o] : 0.0;
+    if (weight) x = G * x;
+    A[t] = x;
+    __syncthreads();
+    double s = Fh[i] * A[N1 * (j + N1 * k)];
+#pragma unroll
+    for (int l = 1; l < N1; ++l) s = s + Fh[i + N1 * l] * A[l + N1 * (j + N1 * k)];
+    B[t] = s;
+    __syncthreads();
+    s = B[i + N1 * N1 * k] * Fh[j];
+#pragma unroll
+    for (int l = 1; l < N1; ++l) s = s + B[i + N1 * (l + N1 * k)] * Fh[j + N1 * l];
+    A[t] = s;
+    __syncthreads();
+    s = A[i + N1 * j] * Fv[k];
+#pragma unroll
+    for (int l = 1; l < N1; ++l) s = s + A[i + N1 * (j + N1 * l)] * Fv[k + N1 * l];
+    if (weight) {
+      const double RG = 1.0 / G;
+      s = s * RG;
+    }
+    if (act) F.f[v][o] = s;
+    __syncthreads();
+  }
+}
+
+int launch_modalfilter(fedg_ctx* ctx, double* DDENS, double* MOMX, double* MOMY, double* MOMZ, double* DRHOT,
+                       int do_weight_Gsqrt) {
+  const DevMesh& m = ctx->m;
+  if (!m.MFh || !m.MFv) {
+    set_error("modal filter: matrices not set up (fedg_ctx_setup_modalfilter)");
+    return FEDG_ERR_ARG;
+  }
+  Fields5 F{{DDENS, MOMX, MOMY, MOMZ, DRHOT}};
+  auto go = [&](auto n1c) -> int {
+    constexpr int N1 = decltype(n1c)::value;
+    constexpr int EPB = epb_for<N1>();
+    hipLaunchKernelGGL((modalfilter_kernel<N1, EPB>), dim3((m.Ne + EPB - 1) / EPB), dim3(EPB * N1 * N1 * N1), 0,
+                       ctx->stream, F, m.Gsqrt, m.MFh, m.MFv, m.Ne, do_weight_Gsqrt ? 1 : 0);
+    FEDG_HIP_CHECK(hipGetLastError());
+    return FEDG_OK;
+  };
+  switch (m.n1) {
+    case 2: return go(std::integral_constant<int, 2>{});
+    case 3: return go(std::integral_constant<int, 3>{});
+    case 4: return go(std::integral_constant<int, 4>{});
+    case 5: return go(std::integral_constant<int, 5>{});
+    case 6: return go(std::integral_constant<int, 6>{});
+    case 7: return go(std::integral_constant<int, 7>{});
+    case 8: return go(std::integral_constant<int, 8>{});
+    default:
+      set_error("polynomial order not built (supported: p = 1..7)");
+      return FEDG_ERR_UNSUPPORTED;
+  }
+}
+
 #define FEDG_STREAM_LAUNCH(kern, n, ...)                                                                   \
   do {                                                                                                     \
     if ((n) <= 0) return FEDG_OK;                                                                          \

@@ -85,6 +85,14 @@ def _load():
         "fedg_calc_phyd_hgrad": (C.c_int, [vp] + [vp] * 4),
         "fedg_hevi_cal_tend": (C.c_int, [vp] + [vp] * 21),
         "fedg_heve_set_precision": (C.c_int, [vp, C.c_int]),
+        "fedg_modalfilter_matrix": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_int, vp]),
+        "fedg_ctx_setup_modalfilter": (C.c_int, [vp, vp, vp]),
+        "fedg_modalfilter_apply": (C.c_int, [vp, vp, vp, vp, vp, vp, C.c_int]),
+        "fedg_heve_set_modalfilter": (C.c_int, [vp, C.c_int]),
+        "fedg_hevi_set_modalfilter": (C.c_int, [vp, C.c_int]),
+        "fedg_hevi_cal_vi_guess": (C.c_int, [vp] + [vp] * 20 + [C.c_double, C.c_double, C.c_int, C.c_double, vp,
+                                                               C.POINTER(C.c_double), C.POINTER(vp), vp,
+                                                               C.POINTER(C.c_int), C.POINTER(C.c_long)]),
         "fedg_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint, C.POINTER(vp)]),
         "fedg_mlp_destroy": (C.c_int, [vp]),
         "fedg_mlp_param_count": (C.c_int, [vp, C.POINTER(C.c_long)]),
@@ -141,7 +149,9 @@ EXPORTED = (
     "fedg_memset_dev fedg_device_sync fedg_device_name fedg_refelem_tables fedg_box_get_sizes "
     "fedg_box_build fedg_ctx_create fedg_ctx_destroy fedg_ctx_sizes fedg_drhot2pres fedg_halo_exchange "
     "fedg_apply_bc_prgvars fedg_heve_cal_tend fedg_hevi_cal_tend fedg_hevi_cal_vi fedg_hevi_cal_vi_newton "
-    "fedg_mlp_create fedg_mlp_destroy fedg_mlp_param_count fedg_mlp_get_params fedg_mlp_set_params "
+    "fedg_modalfilter_matrix fedg_ctx_setup_modalfilter fedg_modalfilter_apply fedg_heve_set_modalfilter "
+    "fedg_hevi_set_modalfilter "
+    "fedg_hevi_cal_vi_guess fedg_mlp_create fedg_mlp_destroy fedg_mlp_param_count fedg_mlp_get_params fedg_mlp_set_params "
     "fedg_mlp_forward fedg_mlp_loss_grad fedg_mlp_adam_step fedg_hevi_set_guess fedg_hevi_guess_stats "
     "fedg_hevi_harvest fedg_hevi_harvest_count "
     "fedg_heve_set_precision fedg_hevi_create fedg_hevi_destroy fedg_hevi_set_state fedg_hevi_get_state fedg_hevi_run "

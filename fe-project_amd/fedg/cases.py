@@ -118,6 +118,9 @@ class Case:
     bubbles: list
     base: str = "constT"
     scheme: str = "ERK_4s4o"
+    # modal filter after every step (MF_ETAC, MF_ALPHA, MF_ORDER, the same
+    # horizontally and vertically), or None: MODALFILTER_FLAG = .false.
+    modalfilter: tuple | None = None
 
 
 def config1() -> Case:
@@ -143,6 +146,35 @@ def config2() -> Case:
     RK4 lies between 0.04 s (stable over 750 steps) and 0.05 s (blows up
     near step 180)."""
     return Case("warm bubbles p7 64x64x16", 7, 64, 64, 16, 160.0e3, 160.0e3, 10.0e3, 0.04, 200, config2_bubbles())
+
+
+# Modal filter of the reference's dry rising-thermal-bubble case
+# (model/atm_nonhydro3d/test/case/rising_thermal_bubble/dry/run.conf:53,71-76).
+RISING_BUBBLE_FILTER = (0.0, 0.25, 24)
+
+
+def config3() -> Case:
+    """Precision form on the config-1 shape for one simulated hour (18000 steps of 0.2 s).
+
+    Without a filter the unfiltered FP64 warm bubble itself blows up after
+    about 5000 steps (aliasing; the oracle's run is finite at step 4000 and
+    NaN by 5000), so the hour runs with the modal filter the reference's
+    rising-thermal-bubble case uses, in the FP64 and the FP32 run alike."""
+    c = config1()
+    c.name = "warm bubble p3 8x8x4, one hour, modal filter"
+    c.nsteps = 18000
+    c.modalfilter = RISING_BUBBLE_FILTER
+    return c
+
+
+def setup_modalfilter(case: Case, dm):
+    """Upload the case's filter matrices to a DeviceMesh (no-op without a filter); returns whether set."""
+    if case.modalfilter is None:
+        return False
+    from .dyncore import modalfilter_matrix
+    F = modalfilter_matrix(case.porder, *case.modalfilter)
+    dm.setup_modalfilter(F, F)
+    return True
 
 
 def config4(dt=None) -> Case:
@@ -188,7 +220,7 @@ def scaled_case(base: Case, NeX, NeY, NeZ, porder=None, nsteps=None) -> Case:
     Lx, Ly, Lz = base.Lx * sx, base.Ly * sy, base.Lz * sz
     bubbles = [(a, r, (min(x, 0.5 * Lx), min(y, 0.5 * Ly), min(z, 0.4 * Lz))) for a, r, (x, y, z) in base.bubbles[:1]]
     return Case(f"{base.name} scaled to {NeX}x{NeY}x{NeZ} p{p}", p, NeX, NeY, NeZ, Lx, Ly, Lz, base.dt,
-                base.nsteps if nsteps is None else nsteps, bubbles, base.base, base.scheme)
+                base.nsteps if nsteps is None else nsteps, bubbles, base.base, base.scheme, base.modalfilter)
 
 
 def build(case: Case, c=constants.SCALE):

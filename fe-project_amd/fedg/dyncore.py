@@ -6,6 +6,7 @@
 * ``halo_exchange``      - MeshFieldComm_Exchange on one tile (DAT/scale_meshfieldcomm_*.F90)
 * ``apply_bc``           - ATMOS_dyn_bnd_applyBC_prgvars_lc (FDS/scale_atm_dyn_dgm_bnd.F90:270-367)
 * ``calc_phyd_hgrad``    - atm_dyn_dgm_nonhydro3d_common_calc_phyd_hgrad_lc (..._common.F90:624-777)
+* ``modalfilter_apply``  - atm_dyn_dgm_modalfilter_apply (FDS/scale_atm_dyn_dgm_modalfilter.F90:48-130)
 * ``TimeIntRK``          - timeint_rk (CMN/scale_timeint_rk.F90) with device buffers
 * ``HeveStepper``        - AtmDynDGMDriver_nonhydro3d_update, HEVE branch
                            (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:614-963), fused:
@@ -101,6 +102,26 @@ def hevi_cal_vi(dm: DeviceMesh, out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DENS_hyd, P
     return it.value, np.array(resid[:n])
 
 
+def hevi_cal_vi_guess(dm: DeviceMesh, out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DENS_hyd, PRES_hyd, DDENS0, MOMX0, MOMY0,
+                      MOMZ0, DRHOT0, Rtot, CVtot, CPtot, impl_fac, dt, max_it, rtol, mlp, scale, Tex=None):
+    """hevi_cal_vi to rtol started from mlp's prediction (fedg_hevi_cal_vi_guess).
+
+    Tex: the previous stage's explicit tendency (5 device fields, VID order) or None.
+    Returns (iterations, residual norms, lines that fell back to q^n)."""
+    args = [_p(a) for a in (*out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DENS_hyd, PRES_hyd, DDENS0, MOMX0, MOMY0, MOMZ0,
+                            DRHOT0, Rtot, CVtot, CPtot)]
+    if len(scale) != 10:
+        raise ValueError("scale takes 10 values: 5 state scales, then 5 increment scales")
+    sc = (C.c_double * 10)(*[float(x) for x in scale])
+    tex = None if Tex is None else (C.c_void_p * 5)(*[_p(a) for a in Tex])
+    resid = (C.c_double * max(int(max_it), 1))()
+    it, fb = C.c_int(0), C.c_long(0)
+    check(LIB.fedg_hevi_cal_vi_guess(dm.handle, *args, float(impl_fac), float(dt), int(max_it), float(rtol),
+                                     mlp.handle, sc, tex, resid, C.byref(it), C.byref(fb)), "fedg_hevi_cal_vi_guess")
+    n = min(it.value + 1, int(max_it))
+    return it.value, np.array(resid[:n]), fb.value
+
+
 def calc_pressure(dm: DeviceMesh, PRES, DPRES, DRHOT, PRES_hyd, THERM_hyd, Rtot, CVtot, CPtot):
     check(LIB.fedg_drhot2pres(dm.handle, *[_p(a) for a in (PRES, DPRES, DRHOT, PRES_hyd, THERM_hyd, Rtot, CVtot,
                                                            CPtot)]), "fedg_drhot2pres")
@@ -114,6 +135,21 @@ def halo_exchange(dm: DeviceMesh, fields):
 def apply_bc(dm: DeviceMesh, DDENS, MOMX, MOMY, MOMZ, THERM, DENS_hyd, PRES_hyd):
     check(LIB.fedg_apply_bc_prgvars(dm.handle, *[_p(a) for a in (DDENS, MOMX, MOMY, MOMZ, THERM, DENS_hyd,
                                                                  PRES_hyd)]), "fedg_apply_bc_prgvars")
+
+
+def modalfilter_matrix(porder: int, etac: float = 2.0 / 3.0, alpha: float = 36.0, order: int = 16) -> np.ndarray:
+    """ModalFilter%FilterMat of a 1D element (ELM/scale_element_modalfilter.F90:60-90), F[i, j] = F(i+1, j+1)."""
+    n1 = porder + 1
+    F = np.zeros(n1 * n1)
+    check(LIB.fedg_modalfilter_matrix(int(porder), float(etac), float(alpha), int(order),
+                                      F.ctypes.data_as(C.c_void_p)), "fedg_modalfilter_matrix")
+    return F.reshape(n1, n1, order="F")
+
+
+def modalfilter_apply(dm: DeviceMesh, DDENS, MOMX, MOMY, MOMZ, DRHOT, do_weight_Gsqrt=True):
+    """atm_dyn_dgm_modalfilter_apply (FDS/scale_atm_dyn_dgm_modalfilter.F90:48-130) on device fields."""
+    check(LIB.fedg_modalfilter_apply(dm.handle, *[_p(a) for a in (DDENS, MOMX, MOMY, MOMZ, DRHOT)],
+                                     int(bool(do_weight_Gsqrt))), "fedg_modalfilter_apply")
 
 
 def calc_phyd_hgrad(dm: DeviceMesh, PRES_hyd: np.ndarray, PRES_hyd_ref: np.ndarray):
@@ -202,6 +238,10 @@ class HeveStepper:
         """64: the reference's FP64 stage; 32: the FP32 perturbation-form tendency (FP64 state and RK)."""
         check(LIB.fedg_heve_set_precision(self.handle, int(bits)), "fedg_heve_set_precision")
 
+    def set_modalfilter(self, on: bool = True):
+        """MODALFILTER_FLAG: filter after every step (DeviceMesh.setup_modalfilter first)."""
+        check(LIB.fedg_heve_set_modalfilter(self.handle, int(bool(on))), "fedg_heve_set_modalfilter")
+
     def set_state(self, q):
         hs = [np.ascontiguousarray(a[: self.n], dtype=np.float64) for a in q]
         arr = (C.c_void_p * 5)(*[h.ctypes.data_as(C.c_void_p) for h in hs])
@@ -281,9 +321,15 @@ class HeviStepper:
               "fedg_hevi_run_timed")
         return tot.value, im.value, ex.value
 
+    def set_modalfilter(self, on: bool = True):
+        """MODALFILTER_FLAG: filter after every step (DeviceMesh.setup_modalfilter first)."""
+        check(LIB.fedg_hevi_set_modalfilter(self.handle, int(bool(on))), "fedg_hevi_set_modalfilter")
+
     def set_guess(self, mlp, scale):
-        """Initial Newton iterate from mlp (None: q^n); scale: 5 per-variable scales (VID order)."""
-        sc = (C.c_double * 5)(*[float(x) for x in scale])
+        """Initial Newton iterate from mlp (None: q^n); scale: 5 state then 5 increment scales (VID order)."""
+        if len(scale) != 10:
+            raise ValueError("scale takes 10 values: 5 state scales, then 5 increment scales")
+        sc = (C.c_double * 10)(*[float(x) for x in scale])
         check(LIB.fedg_hevi_set_guess(self.handle, None if mlp is None else mlp.handle, sc), "fedg_hevi_set_guess")
 
     def guess_stats(self):

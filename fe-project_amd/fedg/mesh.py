@@ -147,6 +147,12 @@ class DeviceMesh:
         check(LIB.fedg_ctx_create(C.byref(d), C.byref(h)), "fedg_ctx_create")
         self.handle = h
 
+    def setup_modalfilter(self, MF_h, MF_v):
+        """Setup_ModalFilter: (n1,n1) filter matrices (e.g. from ``modalfilter_matrix``)."""
+        h = np.asfortranarray(MF_h, dtype=np.float64).ravel(order="F")
+        v = np.asfortranarray(MF_v, dtype=np.float64).ravel(order="F")
+        check(LIB.fedg_ctx_setup_modalfilter(self.handle, _dp(h), _dp(v)), "fedg_ctx_setup_modalfilter")
+
     def close(self):
         if self.handle:
             LIB.fedg_ctx_destroy(self.handle)

@@ -69,6 +69,14 @@ int fedg_refelem_tables(int porder, double *x1d, double *w1d, double *D1D, doubl
                         double *invV1D, double *invM1D, double *Lift_mat, double *IntrpMat_VPOrdM1_tr,
                         int32_t *IndexH2Dto3D, int32_t *IndexH2Dto3D_bnd);
 
+/* Modal filter matrix of a 1D LGL element of order porder: ModalFilter%FilterMat
+ * of ModalFilter_Init_line (ELM/scale_element_modalfilter.F90:60-90) with the
+ * exponential filter of get_exp_filter (:196-236), F = V diag(sigma) V^-1,
+ * sigma_p = exp(-alpha ((eta_p - etac)/(1 - etac))^ord) for eta_p = p/porder
+ * > etac, p > 0, else 1.  F: (n1,n1) column-major.  The driver's defaults are
+ * etac 2/3, alpha 36, ord 16 (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:1249-1294). */
+int fedg_modalfilter_matrix(int porder, double etac, double alpha, int ord, double *F);
+
 /* ------------------------------------------------------------------ */
 /* Structured box mesh (host).  Replaces MeshCubeDom3D_setupLocalDom for one
  * tile (MSH/scale_mesh_cubedom3d.F90:339-469) with the metrics of
@@ -150,6 +158,15 @@ typedef struct fedg_ctx fedg_ctx;
 
 int fedg_ctx_create(const fedg_mesh_desc *desc, fedg_ctx **ctx);
 int fedg_ctx_destroy(fedg_ctx *ctx);
+
+/* Modal filter.  Setup_ModalFilter (ELM/scale_element_operation_tensorprod3D.F90:5464-5509):
+ * the host's MFilter_h1D and MFilter_v1D ((n1,n1) column-major host arrays,
+ * e.g. from fedg_modalfilter_matrix).  Apply: atm_dyn_dgm_modalfilter_apply
+ * (FDS/scale_atm_dyn_dgm_modalfilter.F90:48-130) on device fields (Np,NeA),
+ * elements 1..Ne, in place; returns after the work is complete. */
+int fedg_ctx_setup_modalfilter(fedg_ctx *ctx, const double *MFilter_h1D, const double *MFilter_v1D);
+int fedg_modalfilter_apply(fedg_ctx *ctx, double *DDENS, double *MOMX, double *MOMY, double *MOMZ, double *DRHOT,
+                           int do_weight_Gsqrt);
 /* sizes: Np, NfpTot, Ne, NeA, nB ... of the context */
 int fedg_ctx_sizes(const fedg_ctx *ctx, fedg_box_sizes *sizes);
 
@@ -282,6 +299,10 @@ int fedg_heve_is_flat(const fedg_heve *st);
  * base state and the RK update stay FP64.  Flat-terrain box, dry air.
  * bits = 64 (default): the reference's FP64 arithmetic. */
 int fedg_heve_set_precision(fedg_heve *st, int bits);
+/* MODALFILTER_FLAG: filter the state after every step, with Gsqrt weighting
+ * (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:940-950); needs
+ * fedg_ctx_setup_modalfilter first */
+int fedg_heve_set_modalfilter(fedg_heve *st, int on);
 /* host state in/out: q[v] (Np,Ne or more) for v = DDENS, DRHOT, MOMZ, MOMX, MOMY */
 int fedg_heve_set_state(fedg_heve *st, const double *const *q_host);
 int fedg_heve_get_state(fedg_heve *st, double *const *q_host);
@@ -320,6 +341,8 @@ int fedg_hevi_run_timed(fedg_hevi *st, int nsteps, double *ms_total, double *ms_
                         double *ms_explicit_per_stage);
 /* Newton iterations summed over the implicit stages run so far, and their number */
 int fedg_hevi_newton_stats(const fedg_hevi *st, long *iterations, long *calls);
+/* as fedg_heve_set_modalfilter */
+int fedg_hevi_set_modalfilter(fedg_hevi *st, int on);
 
 
 /* ------------------------------------------------------------------ */
@@ -329,8 +352,11 @@ int fedg_hevi_newton_stats(const fedg_hevi *st, long *iterations, long *calls);
  * network n_in -> n_hidden (x n_hidden_layers) -> n_out, BF16 products on
  * the matrix cores with FP32 accumulation, FP32 master weights, Adam.  For a
  * vertical node line of L = NeZ (p+1) nodes: n_in = 10 L + 1 (the line's
- * (q* - q^n) / scale and impl_fac T_ex / scale per variable, then
- * impl_fac/dt), n_out = 5 L ((q - q*) / scale of the converged iterate q).
+ * stage state q* / s per variable, then impl_fac T_ex / d per variable, then
+ * impl_fac/dt; T_ex is the previous stage's explicit tendency, zero at the
+ * first stage), n_out = 5 L ((q - q*) / d of the converged iterate q).
+ * Variables in VID order, line node t = kz (p+1) + k (k the node level in
+ * element kz); s, d: per-variable state and increment scales.
  * Device arrays are fp32, row-major (batch x features). */
 typedef struct fedg_mlp fedg_mlp;
 int fedg_mlp_create(int n_in, int n_hidden, int n_hidden_layers, int n_out, unsigned seed, fedg_mlp **out);
@@ -345,8 +371,21 @@ int fedg_mlp_loss_grad(fedg_mlp *m, const float *X_dev, const float *T_dev, int 
 int fedg_mlp_adam_step(fedg_mlp *m, float lr, float beta1, float beta2, float eps);
 /* Use mlp (or none: NULL) for the initial iterate of every implicit stage of
  * the HEVI stepper; lines whose predicted iterate has a larger residual than
- * q^n start from q^n.  scale: the 5 per-variable scales (VID order). */
+ * q^n start from q^n.  scale: 10 values, s then d (VID order each).  The
+ * tolerance stays relative to the residual norm at q^n. */
 int fedg_hevi_set_guess(fedg_hevi *st, fedg_mlp *mlp, const double *scale);
+/* fedg_hevi_cal_vi_newton started from the network's prediction for a host
+ * that drives its own stage loop: Tex (5 device fields in VID order, or NULL)
+ * is the previous stage's explicit tendency; *fallbacks receives the number
+ * of lines that started from q^n; the tolerance stays relative to ||G(q^n)||. */
+int fedg_hevi_cal_vi_guess(fedg_ctx *ctx, double *DENS_dt, double *MOMX_dt, double *MOMY_dt, double *MOMZ_dt,
+                           double *RHOT_dt, const double *DDENS_, const double *MOMX_, const double *MOMY_,
+                           const double *MOMZ_, const double *DRHOT_, const double *DENS_hyd,
+                           const double *PRES_hyd, const double *DDENS0_, const double *MOMX0_,
+                           const double *MOMY0_, const double *MOMZ0_, const double *DRHOT0_,
+                           const double *Rtot, const double *CVtot, const double *CPtot, double impl_fac,
+                           double dt, int max_it, double rtol, fedg_mlp *mlp, const double *scale,
+                           const double *const *Tex, double *resid, int *iterations, long *fallbacks);
 int fedg_hevi_guess_stats(const fedg_hevi *st, long *lines, long *fallbacks);
 /* Collect training pairs from the following steps: per implicit stage,
  * per_stage lines' (features, converged targets) appended to X_dev / Y_dev

@@ -600,6 +600,64 @@ void oracle_heve_update(const oracle_mesh *m, const oracle_rk_tab *t, int nsteps
         oracle_rk_advance(t, s, n, q[v], var0 + v * NpNeA, varTmp + v * NpNeA, tend + v * NpNeA, NULL,
                           5 * NpNeA);
     }
+    if (m->MF_h)
+      oracle_modalfilter_apply(m, q[DENS_VID], q[MOMX_VID], q[MOMY_VID], q[MOMZ_VID], q[RHOT_VID], 1);
     oracle_drhot2pres(m, PRES, DPRES, q[RHOT_VID], PRES_hyd, THERM_hyd, Rtot, CVtot, CPtot);
+  }
+}
+
+/* element_operation_kernel_matvec_ModalFilter (x: Mat_h1D, y: Mat_h1D_tr, z:
+ * Mat_v1D_tr), each sum taken over l = 1..n1 in order */
+static void modalfilter_elem(int N, const double *Fh, const double *Fv, const double *in, double *work,
+                             double *out) {
+  for (int k = 0; k < N; ++k)
+    for (int j = 0; j < N; ++j)
+      for (int i = 0; i < N; ++i) {
+        double s = Fh[i] * in[N * (j + N * k)];
+        for (int l = 1; l < N; ++l) s = s + Fh[i + N * l] * in[l + N * (j + N * k)];
+        out[i + N * (j + N * k)] = s;
+      }
+  for (int k = 0; k < N; ++k)
+    for (int j = 0; j < N; ++j)
+      for (int i = 0; i < N; ++i) {
+        double s = out[i + N * N * k] * Fh[j];
+        for (int l = 1; l < N; ++l) s = s + out[i + N * (l + N * k)] * Fh[j + N * l];
+        work[i + N * (j + N * k)] = s;
+      }
+  for (int k = 0; k < N; ++k)
+    for (int j = 0; j < N; ++j)
+      for (int i = 0; i < N; ++i) {
+        double s = work[i + N * j] * Fv[k];
+        for (int l = 1; l < N; ++l) s = s + work[i + N * (j + N * l)] * Fv[k + N * l];
+        out[i + N * (j + N * k)] = s;
+      }
+}
+
+void oracle_modalfilter_apply(const oracle_mesh *m, double *DDENS, double *MOMX, double *MOMY, double *MOMZ,
+                              double *DRHOT, int do_weight_Gsqrt) {
+  const int N = m->n1, Np = m->Np;
+  double *f[5] = {DDENS, MOMX, MOMY, MOMZ, DRHOT};
+#pragma omp parallel
+  {
+    double *tmp = (double *)malloc(sizeof(double) * Np * 3);
+    double *work = tmp + Np, *out = tmp + 2 * Np;
+#pragma omp for schedule(static)
+    for (int ke = 0; ke < m->Ne; ++ke) {
+      const double *G = m->Gsqrt + (long)Np * ke;
+      for (int v = 0; v < 5; ++v) {
+        double *x = f[v] + (long)Np * ke;
+        for (int kk = 0; kk < Np; ++kk) tmp[kk] = do_weight_Gsqrt ? G[kk] * x[kk] : x[kk];
+        modalfilter_elem(N, m->MF_h, m->MF_v, tmp, work, out);
+        for (int kk = 0; kk < Np; ++kk) {
+          if (do_weight_Gsqrt) {
+            const double RGsqrt = 1.0 / G[kk];
+            x[kk] = out[kk] * RGsqrt;
+          } else {
+            x[kk] = out[kk];
+          }
+        }
+      }
+    }
+    free(tmp);
   }
 }

@@ -48,7 +48,18 @@ typedef struct {
   const double *IntrpMat_tr;/* (n1,n1)          */
   /* physical constants (SCALE scale_const) */
   double GRAV, Rdry, CPdry, CVdry, PRES00;
+  /* modal filter (ElementOperationTensorProd3D%MFilter_h1D / MFilter_v1D as
+   * ModalFilter%FilterMat, (n1,n1)); NULL: MODALFILTER_FLAG = .false. */
+  const double *MF_h, *MF_v;
 } oracle_mesh;
+
+/* atm_dyn_dgm_modalfilter_apply (FDS/scale_atm_dyn_dgm_modalfilter.F90:48-130)
+ * with element_operation_kernel_matvec_ModalFilter_P* (ELM/..._tensorprod3D_kernel.F90:1061-1112):
+ * per element x, y then z pass of the 1D filter matrices, weighted by Gsqrt
+ * when do_weight_Gsqrt.  The driver applies it after every step
+ * (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:940-950). */
+void oracle_modalfilter_apply(const oracle_mesh *m, double *DDENS, double *MOMX, double *MOMY, double *MOMZ,
+                              double *DRHOT, int do_weight_Gsqrt);
 
 /* FDS/scale_atm_dyn_dgm_nonhydro3d_common.F90:428-479 (CPU branch) */
 void oracle_drhot2pres(const oracle_mesh *m, double *PRES, double *DPRES,
@@ -111,8 +122,9 @@ void oracle_rk_store_impl(const oracle_rk_tab *t, int stage, long n, double *q, 
 
 /* One HEVE step of AtmDynDGMDriver_nonhydro3d_update
  * (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:614-963) with hevi_flag=.false.,
- * hide_mpi_comm_flag=.false., no sponge, no tracers, no modal filter, dry
- * air and physics tendencies that are zero (add_phy_tend adds exact zeros).
+ * hide_mpi_comm_flag=.false., no sponge, no tracers, the modal filter when
+ * m->MF_h is set (MODALFILTER_FLAG, do_weight_Gsqrt = .true.), dry air and
+ * physics tendencies that are zero (add_phy_tend adds exact zeros).
  * q[v] are the 5 prognostic fields (Np,NeA) in the order DDENS, DRHOT, MOMZ,
  * MOMX, MOMY (PRGVAR ids 1..5).  work must hold 5*Np*NeA*(2 + tend_buf_size) +
  * 2*Np*NeA doubles.  PRES/DPRES are updated as the reference does. */
@@ -154,6 +166,11 @@ int oracle_hevi_newton(const oracle_mesh *m, double *const pv[5], const double *
                        const double *const q0[5], const double *DENS_hyd, const double *PRES_hyd,
                        const double *Rtot, const double *CVtot, const double *CPtot, double impl_fac, double dt,
                        int max_it, double rtol, double *resid);
+/* r0_ext > 0: the residual norm the tolerance is relative to */
+int oracle_hevi_newton_r0(const oracle_mesh *m, double *const pv[5], const double *const qs[5],
+                          const double *const q0[5], const double *DENS_hyd, const double *PRES_hyd,
+                          const double *Rtot, const double *CVtot, const double *CPtot, double impl_fac, double dt,
+                          int max_it, double rtol, double r0_ext, double *resid);
 /* FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi.F90:772-965; tend, q, var0 in VID order */
 int oracle_hevi_cal_vi(const oracle_mesh *m, double *const tend[5], const double *const q[5],
                        const double *const var0[5], const double *DENS_hyd, const double *PRES_hyd,

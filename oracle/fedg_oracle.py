@@ -29,7 +29,7 @@ class OracleMeshC(C.Structure):
         ("EMap3Dto2D", ip), ("IndexH2Dto3D", ip), ("IndexH2Dto3D_bnd", ip),
         ("D1D", dp), ("Lift_mat", dp), ("IntrpMat_tr", dp),
         ("GRAV", C.c_double), ("Rdry", C.c_double), ("CPdry", C.c_double), ("CVdry", C.c_double),
-        ("PRES00", C.c_double),
+        ("PRES00", C.c_double), ("MF_h", dp), ("MF_v", dp),
     ]
 
 
@@ -65,6 +65,10 @@ def _load():
     lib.oracle_hevi_newton.argtypes = [M, V5, V5, V5] + [dp] * 5 + [C.c_double, C.c_double, C.c_int,
                                                                      C.c_double, dp]
     lib.oracle_hevi_newton.restype = C.c_int
+    lib.oracle_hevi_newton_r0.argtypes = [M, V5, V5, V5] + [dp] * 5 + [C.c_double, C.c_double, C.c_int,
+                                                                        C.c_double, C.c_double, dp]
+    lib.oracle_hevi_newton_r0.restype = C.c_int
+    lib.oracle_modalfilter_apply.argtypes = [M] + [dp] * 5 + [C.c_int]
     lib.oracle_hevi_cal_vi.argtypes = [M, V5, V5, V5] + [dp] * 5 + [C.c_double, C.c_double, C.c_int,
                                                                      C.c_double, dp]
     lib.oracle_hevi_cal_vi.restype = C.c_int
@@ -115,6 +119,23 @@ class OracleMesh:
     @property
     def ref(self):
         return C.byref(self.s)
+
+    def set_modalfilter(self, MF_h=None, MF_v=None):
+        """Filter matrices ((n1,n1) Fortran order) applied after every step by the update loops; None: off."""
+        if MF_h is None:
+            self._mf = None
+            self.s.MF_h = self.s.MF_v = None
+            return
+        self._mf = (np.asfortranarray(MF_h, dtype=np.float64).ravel(order="F"),
+                    np.asfortranarray(MF_v, dtype=np.float64).ravel(order="F"))
+        self.s.MF_h, self.s.MF_v = _d(self._mf[0]), _d(self._mf[1])
+
+    def modalfilter_apply(self, q, do_weight_Gsqrt=True):
+        """Filter q (VID order) in place with the matrices of set_modalfilter."""
+        if getattr(self, "_mf", None) is None:
+            raise ValueError("set_modalfilter first")
+        LIB.oracle_modalfilter_apply(self.ref, _d(q[0]), _d(q[3]), _d(q[4]), _d(q[2]), _d(q[1]),
+                                     int(bool(do_weight_Gsqrt)))
 
     # --- wrappers -----------------------------------------------------------
     def drhot2pres(self, DRHOT, bs):
@@ -182,12 +203,14 @@ class OracleMesh:
                                     float(dt), int(max_it), float(rtol), _d(resid))
         return tend, it, resid[:min(it + 1, max(max_it, 1))]
 
-    def hevi_newton(self, pv, qs, q0, bs, impl_fac, dt, max_it=1, rtol=0.0):
-        """Newton iterations on pv (VID order, updated in place); returns (iterations, residual history)."""
+    def hevi_newton(self, pv, qs, q0, bs, impl_fac, dt, max_it=1, rtol=0.0, r0_ref=0.0):
+        """Newton iterations on pv (VID order, updated in place); returns (iterations, residual history).
+
+        r0_ref > 0: the residual norm the tolerance is relative to (else the first iteration's)."""
         resid = np.zeros(max(max_it, 1))
-        it = LIB.oracle_hevi_newton(self.ref, self._v5(pv), self._v5(qs), self._v5(q0), _d(bs.DENS_hyd),
-                                    _d(bs.PRES_hyd), _d(bs.Rtot), _d(bs.CVtot), _d(bs.CPtot), float(impl_fac),
-                                    float(dt), int(max_it), float(rtol), _d(resid))
+        it = LIB.oracle_hevi_newton_r0(self.ref, self._v5(pv), self._v5(qs), self._v5(q0), _d(bs.DENS_hyd),
+                                       _d(bs.PRES_hyd), _d(bs.Rtot), _d(bs.CVtot), _d(bs.CPtot), float(impl_fac),
+                                       float(dt), int(max_it), float(rtol), float(r0_ref), _d(resid))
         return it, resid[:min(it + 1, max(max_it, 1))]
 
     def hevi_linear_system(self, pv, qs, q0, bs, impl_fac, dt):

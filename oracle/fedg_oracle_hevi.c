@@ -785,13 +785,24 @@ int oracle_hevi_newton(const oracle_mesh *m, double *const pv[5], const double *
                        const double *const q0[5], const double *DENS_hyd, const double *PRES_hyd,
                        const double *Rtot, const double *CVtot, const double *CPtot, double impl_fac, double dt,
                        int max_it, double rtol, double *resid) {
+  return oracle_hevi_newton_r0(m, pv, qs, q0, DENS_hyd, PRES_hyd, Rtot, CVtot, CPtot, impl_fac, dt, max_it, rtol,
+                               0.0, resid);
+}
+
+/* As oracle_hevi_newton; r0_ext > 0 is the norm the tolerance is relative to
+ * (the residual at q^n when the iterate starts from a prediction, which may
+ * already meet the tolerance at the first check). */
+int oracle_hevi_newton_r0(const oracle_mesh *m, double *const pv[5], const double *const qs[5],
+                          const double *const q0[5], const double *DENS_hyd, const double *PRES_hyd,
+                          const double *Rtot, const double *CVtot, const double *CPtot, double impl_fac, double dt,
+                          int max_it, double rtol, double r0_ext, double *resid) {
   hevi_ctx h;
   hevi_ctx_init(&h, m, impl_fac, dt);
   h.qs = qs; h.q0 = q0; h.pv = pv;
   h.DENS_hyd = DENS_hyd; h.PRES_hyd = PRES_hyd; h.Rtot = Rtot; h.CVtot = CVtot; h.CPtot = CPtot;
   const long n = (long)m->Np * m->Ne;
   double *T = (double *)malloc(sizeof(double) * n * 5);
-  double r0 = 0.0;
+  double r0 = r0_ext;
   int it = 0;
   for (; it < max_it; ++it) {
     hevi_eval_Ax_uv(&h, T + 3 * n, T + 4 * n, 1);
@@ -803,8 +814,8 @@ int oracle_hevi_newton(const oracle_mesh *m, double *const pv[5], const double *
     r2 += sumsq(h.b, 3 * n);
     const double r = sqrt(r2);
     if (resid) resid[it] = r;
-    if (it == 0) r0 = r;
-    if (rtol > 0.0 && it > 0 && r <= rtol * r0) break;
+    if (it == 0 && r0_ext <= 0.0) r0 = r;
+    if (rtol > 0.0 && (it > 0 || r0_ext > 0.0) && r <= rtol * r0) break;
     hevi_solve_var3(&h);
   }
   free(T);
@@ -970,6 +981,8 @@ void oracle_hevi_update(const oracle_mesh *m, const oracle_rk_tab *t, int nsteps
         oracle_rk_advance(t, s, n, q[v], var0 + v * NpNeA, varTmp + v * NpNeA, tend_ex + v * NpNeA,
                           tend_im + v * NpNeA, 5 * NpNeA);
     }
+    if (m->MF_h)
+      oracle_modalfilter_apply(m, q[DENS_VID], q[MOMX_VID], q[MOMY_VID], q[MOMZ_VID], q[RHOT_VID], 1);
     oracle_drhot2pres(m, PRES, DPRES, q[RHOT_VID], PRES_hyd, THERM_hyd, Rtot, CVtot, CPtot);
   }
 }
