@@ -6,23 +6,24 @@
 //   1. hevi_rhs_uv   (element, one thread per node): the damping coefficient
 //      alph at every vertical face node from q^n, the horizontal-momentum
 //      face jumps of the iterate, their lift, MOMX_t / MOMY_t and the uv rhs;
-//   2. hevi_solve_uv (one wavefront per vertical node line): assembles the
-//      N x N blocks of construct_matbnd_uv level by level, block-Thomas
-//      elimination with the pivoted LU of solve_NnodeN_uv, back substitution,
-//      MOMX, MOMY += increment;
+//   2. hevi_solve_uv (64/8 lines per wavefront, a lane per block row):
+//      assembles the N x N blocks of construct_matbnd_uv level by level,
+//      block-Thomas elimination with the pivoted LU of solve_NnodeN_uv, back
+//      substitution, MOMX, MOMY += increment;
 //   3. hevi_rhs3     (element): vertical fluxes of DDENS, MOMZ, DRHOT, their
 //      face jumps, Dz and lift, VFilterPM1 buoyancy, the var3 rhs and the
 //      linearisation coefficients POT, WT, DPDRHOT;
-//   4. hevi_solve3   (one wavefront per line): the 3N x 3N blocks of
-//      construct_matbnd, block-Thomas with the LU of solve_NnodeN_var3,
-//      DDENS, MOMZ, DRHOT += increment.
+//   4. hevi_solve3   (two lines per wavefront at p = 7, a lane per block
+//      row): the 3N x 3N blocks of construct_matbnd, block-Thomas with the LU
+//      of solve_NnodeN_var3, DDENS, MOMZ, DRHOT += increment.
 // A line is the column of one horizontal node (ij) of one horizontal element
 // through all NeZ elements; everything the implicit operator couples lies on
-// it.  The LU runs on the line's block in LDS with all 64 lanes: pivot by a
-// wavefront max-reduction and ballot, row swap, scaling and the rank-1 update
-// spread over the lanes; every entry sees the same operations in the same
-// order as the reference's serial loops, so results are bit for bit those of
-// the CPU restatement.
+// it.  A block's rows stay in registers of their lanes; pivoting exchanges
+// row positions, the pivot row and the solved entries travel by cross-lane
+// shuffles, and the levels' solutions wait in LDS for the block back
+// substitution.  Every entry sees the same operations in the same order as
+// the reference's serial loops, so results are bit for bit those of the CPU
+// restatement.
 #include "fedg_device.h"
 #include "fedg_hevi.h"
 #include "fedg_mlp.h"
@@ -252,302 +253,362 @@ __global__ void __launch_bounds__(N1* N1* N1) hevi_rhs3_kernel(const HeviArgs a)
 // Wavefront-cooperative pivoted LU + solve of solve_NnodeN_var3 / _uv
 // (FDS/scale_atm_dyn_dgm_hevi_common_linalgebra.F90:2142-2445) for an n x n
 // block A (column-major in LDS) and nr right-hand sides X (n x nr in LDS).
-// Must be called by all 64 lanes of a single-wavefront workgroup.
-__device__ __forceinline__ double wave_max(double v) {
+// ---------------------------------------------------------------------------
+// Line solves.  A wavefront holds 64/SEG lines, one per segment of SEG lanes;
+// lane r of a segment owns row r of the level's n x n block in registers and
+// the rows' right-hand-side entries.  Partial pivoting moves rows by
+// exchanging their positions (pos), not their data: the operations every
+// entry sees -- pivot choice, reciprocal of the pivot, scaling of the column,
+// rank-1 update, forward substitution by columns, back substitution in
+// ascending column order -- are those of the reference's serial loops
+// (FDS/scale_atm_dyn_dgm_hevi_common_linalgebra.F90:2142-2445), so results
+// are bit for bit the CPU restatement's.
+
+template <int SEG>
+__device__ __forceinline__ double seg_max(double v) {
 #pragma unroll
-  for (int s = 32; s > 0; s >>= 1) {
+  for (int s = SEG / 2; s > 0; s >>= 1) {
     const double o = __shfl_xor(v, s, 64);
     v = o > v ? o : v;
   }
   return v;
 }
 
-__device__ void lu_solve_wave(int n, double* A, int* ipiv, int nr, double* X) {
-  const int lane = threadIdx.x;
-  for (int k = 0; k < n; ++k) {
-    // pivot: the first row of max |A(i,k)|, i >= k, keeping k on ties; a NaN
-    // on the diagonal keeps k, NaN below it is never chosen
-    const double akk = fabs(A[k + n * k]);
-    double v = -1.0;
-    if (lane >= k && lane < n) {
-      const double t = fabs(A[lane + n * k]);
-      v = (t == t) ? t : -1.0;
-    }
-    int piv = k;
-    if (akk == akk) {
-      const double mx = wave_max(v);
-      const unsigned long long hit = __ballot(lane >= k && lane < n && v == mx);
-      piv = __ffsll((long long)hit) - 1;
-    }
-    if (lane == 0) ipiv[k] = piv;
-    if (piv != k && lane < n) {
-      const double t = A[k + n * lane];
-      A[k + n * lane] = A[piv + n * lane];
-      A[piv + n * lane] = t;
-    }
-    __syncthreads();
-    const double inv = 1.0 / A[k + n * k];
-    __syncthreads();
-    if (lane == k) A[k + n * k] = inv;
-    if (lane > k && lane < n) A[lane + n * k] = A[lane + n * k] * inv;
-    __syncthreads();
-    const int w = n - k - 1;
-    for (int e = lane; e < w * w; e += 64) {
-      const int i = k + 1 + e % w, j = k + 1 + e / w;
-      A[i + n * j] = A[i + n * j] - A[i + n * k] * A[k + n * j];
-    }
-    __syncthreads();
+template <int SEG>
+__device__ __forceinline__ int seg_min(int v) {
+#pragma unroll
+  for (int s = SEG / 2; s > 0; s >>= 1) {
+    const int o = __shfl_xor(v, s, 64);
+    v = o < v ? o : v;
   }
-  if (lane < nr) {
-    double* x = X + n * lane;
-    for (int i = 0; i < n - 1; ++i) {
-      const int ip = ipiv[i];
-      if (ip != i) {
-        const double t = x[i];
-        x[i] = x[ip];
-        x[ip] = t;
-      }
-    }
-    for (int i = 0; i < n; ++i) {
-      double t = x[i];
-      for (int j = 0; j < i; ++j) t = t - x[j] * A[i + n * j];
-      x[i] = t;
-    }
-    for (int i = n - 1; i >= 0; --i) {
-      double t = x[i];
-      for (int j = i + 1; j < n; ++j) t = t - A[i + n * j] * x[j];
-      x[i] = t * A[i + n * i];
-    }
-  }
-  __syncthreads();
+  return v;
 }
 
+// the lane of the segment where pred holds (exactly one)
+__device__ __forceinline__ int seg_lane(bool pred, unsigned long long segmask) {
+  return __ffsll((long long)(__ballot(pred) & segmask)) - 1;
+}
+
+// LU with partial pivoting of the segment's n x n block (row r in A of lane
+// r), then nr <= NR right-hand sides (x[q] of lane r: entry r of rhs q).
+// The solution goes to X (LDS, column-major n x NR).  Every lane of the
+// single-wavefront workgroup calls it.
+template <int n, int SEG, int NR>
+__device__ __forceinline__ void lu_solve_rows(double (&A)[n], double (&x)[NR], int nr, bool act,
+                                              unsigned long long segmask, double* X) {
+  const int lane = threadIdx.x;
+  int pos = act ? (lane % SEG) : (1 << 20);
+#pragma unroll
+  for (int k = 0; k < n; ++k) {
+    // pivot: the first position >= k of max |A(i,k)|; a NaN on the diagonal
+    // keeps k, a NaN below it is never chosen (the reference's strict '>')
+    double av = -1.0;
+    if (act && pos >= k) {
+      const double t = fabs(A[k]);
+      av = (t == t) ? t : -1.0;
+    }
+    const int lk = seg_lane(act && pos == k, segmask);
+    const double akk = fabs(__shfl(A[k], lk, 64));
+    const double mx = seg_max<SEG>(av);
+    const int cand = seg_min<SEG>((act && pos >= k && av == mx) ? pos : (1 << 20));
+    const int ppos = (akk == akk) ? cand : k;
+    const int lp = seg_lane(act && pos == ppos, segmask);
+    if (act) {
+      if (pos == k) pos = ppos;
+      else if (pos == ppos) pos = k;
+    }
+    const double dinv = 1.0 / __shfl(A[k], lp, 64);
+    const bool below = act && pos > k;
+    const double l = A[k] * dinv;
+    // the pivot row's entries one at a time (the pivot lane's row is not
+    // changed in this step)
+#pragma unroll
+    for (int j = k + 1; j < n; ++j) {
+      const double pj = __shfl(A[j], lp, 64);
+      if (below) A[j] = A[j] - l * pj;
+    }
+    if (lane == lp) A[k] = dinv;
+    else if (below) A[k] = l;
+  }
+  // forward substitution with the unit lower factor, column by column
+#pragma unroll
+  for (int j = 0; j < n - 1; ++j) {
+    const int lj = seg_lane(act && pos == j, segmask);
+    double xj[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) xj[q] = __shfl(x[q], lj, 64);
+    if (act && pos > j) {
+#pragma unroll
+      for (int q = 0; q < NR; ++q)
+        if (q < nr) x[q] = x[q] - xj[q] * A[j];
+    }
+  }
+  // back substitution, row by row (each row's sum in ascending column order)
+#pragma clang loop unroll(full)
+  for (int i = n - 1; i >= 0; --i) {
+    if (act && pos == i) {
+      double t[NR];
+#pragma clang loop unroll(full)
+      for (int q = 0; q < NR; ++q) t[q] = x[q];
+#pragma clang loop unroll(full)
+      for (int j = i + 1; j < n; ++j) {
+        const double aj = A[j];
+#pragma clang loop unroll(full)
+        for (int q = 0; q < NR; ++q)
+          if (q < nr) t[q] = t[q] - aj * X[j + n * q];
+      }
+#pragma clang loop unroll(full)
+      for (int q = 0; q < NR; ++q)
+        if (q < nr) X[i + n * q] = t[q] * A[i];
+    }
+    __syncthreads();
+  }
+}
+
+template <int N1>
+struct LineSeg {
+  static constexpr int n3 = 3 * N1;
+  static constexpr int SEG3 = n3 <= 8 ? 8 : (n3 <= 16 ? 16 : (n3 <= 32 ? 32 : 64));
+  static constexpr int SEGU = N1 <= 8 ? 8 : (N1 <= 16 ? 16 : 32);
+};
+
 // ---------------------------------------------------------------------------
-// 2. construct_matbnd_uv (:905-1011) + solve_uv (:589-689), one line per workgroup
+// 2. construct_matbnd_uv (:905-1011) + solve_uv (:589-689)
 template <int N1>
 __global__ void __launch_bounds__(64) hevi_solve_uv_kernel(const HeviArgs a) {
   constexpr int N = N1, Nfp = N1 * N1, Np = N1 * N1 * N1;
+  constexpr int SEG = LineSeg<N1>::SEGU, LPW = 64 / SEG;
   extern __shared__ double lds[];
-  double* A = lds;                  // N*N
-  double* Lv = A + N * N;           // N
-  double* X = Lv + N;               // NeZ * 3 * N: b(:,1), b(:,2), G per level
-  int* ipiv = reinterpret_cast<int*>(X + (size_t)a.NeZ * 3 * N);
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x, sg = lane / SEG, r = lane % SEG;
+  const bool act = r < N;
+  const unsigned long long segmask = (SEG == 64 ? ~0ull : ((1ull << SEG) - 1ull)) << (sg * SEG);
   const DevMesh& m = a.m;
   const int nline = m.Ne2D * Nfp;
   const int NeZ = a.NeZ;
   const uint32_t nn = (uint32_t)Np * m.Ne;
-  for (int line = blockIdx.x; line < nline; line += gridDim.x) {
-    const int ke2D = line / Nfp, ij = line % Nfp;
+  double* X = lds + (size_t)sg * NeZ * 3 * N;  // per level: b(:,1), b(:,2), G
+  const int pv = act ? r : 0;
+  for (int g = blockIdx.x; g * LPW < nline; g += gridDim.x) {
+    const int line = g * LPW + sg;
+    const bool lv = line < nline;
+    const int ke2D = lv ? line / Nfp : 0, ij = lv ? line % Nfp : 0;
     for (int kz = 0; kz < NeZ; ++kz) {
       const int ke = ke2D + kz * m.Ne2D;
-      double* Xk = X + (size_t)kz * 3 * N;
-      // face damping of the rows: t[f][pv] for the two vertical faces
-      for (int e = lane; e < N * N; e += 64) A[e] = (e % N == e / N) ? 1.0 : 0.0;
-      if (lane < N) {
-        const uint32_t q = (uint32_t)Np * ke + ij + Nfp * lane;
-        Xk[lane] = ldg(a.buv, q);
-        Xk[lane + N] = ldg(a.buv, q + nn);
-      }
-      __syncthreads();
-      for (int f1 = 1; f1 <= 2; ++f1) {
-        const int kz2 = (f1 == 1) ? (kz - 1 > 0 ? kz - 1 : 0) : (kz + 1 < NeZ - 1 ? kz + 1 : NeZ - 1);
-        const int pv1 = (f1 == 1) ? 0 : N - 1;
-        int f2 = (f1 == 1) ? 2 : 1;
-        const bool bc = (kz == 0 && f1 == 1) || (kz == NeZ - 1 && f1 == 2);
-        if (bc) continue;
-        const int ke_2 = ke2D + kz2 * m.Ne2D;
-        if (lane < N) {
-          const int pv = lane;
+      double A[N];
+#pragma unroll
+      for (int c = 0; c < N; ++c) A[c] = (c == pv) ? 1.0 : 0.0;
+      double x[3] = {0.0, 0.0, 0.0};
+      double lvv = 0.0;
+      if (lv && act) {
+        const uint32_t q = (uint32_t)Np * ke + ij + Nfp * pv;
+        x[0] = ldg(a.buv, q);
+        x[1] = ldg(a.buv, q + nn);
+        // face damping of the row: f1 = 1 couples to the level below, 2 above
+#pragma unroll
+        for (int f1 = 1; f1 <= 2; ++f1) {
+          const int kz2 = (f1 == 1) ? (kz - 1 > 0 ? kz - 1 : 0) : (kz + 1 < NeZ - 1 ? kz + 1 : NeZ - 1);
+          const int f2 = (f1 == 1) ? 2 : 1;
+          const bool bc = (kz == 0 && f1 == 1) || (kz == NeZ - 1 && f1 == 2);
+          if (bc) continue;
+          const int ke_2 = ke2D + kz2 * m.Ne2D;
           const int p = ij + Nfp * pv;
           const uint32_t gfp = (uint32_t)((3 + f1) * Nfp + ij) + (uint32_t)(6 * Nfp) * ke;
           const double fac = 0.5 * a.impl_fac / gsqrtv(a, ke, p, N1);
           const double am = ldg(a.alph, (uint32_t)((f1 - 1) * Nfp + ij) + (uint32_t)(2 * Nfp) * ke);
           const double ap = ldg(a.alph, (uint32_t)((f2 - 1) * Nfp + ij) + (uint32_t)(2 * Nfp) * ke_2);
           const double t = fac * ldg(m.Lift, (uint32_t)(p + (3 + f1) * Np)) * ldg(m.Fscale, gfp) * (am > ap ? am : ap);
-          A[pv + N * pv1] = A[pv + N * pv1] + t;
-          if (f1 == 1) Lv[pv] = -t;
-          else Xk[pv + 2 * N] = -t;  // U lands in the G slot
+          if (f1 == 1) {
+            A[0] = A[0] + t;
+            lvv = -t;
+          } else {
+            A[N - 1] = A[N - 1] + t;
+            x[2] = -t;  // U lands in the G slot
+          }
         }
-        __syncthreads();
       }
       if (kz > 0) {
         const double* Xp = X + (size_t)(kz - 1) * 3 * N;
-        if (lane < N) {
-          const double tmp = Lv[lane];
-          A[lane] = A[lane] - tmp * Xp[(N - 1) + 2 * N];
-          Xk[lane] = Xk[lane] - tmp * Xp[N - 1];
-          Xk[lane + N] = Xk[lane + N] - tmp * Xp[(N - 1) + N];
-        }
-        __syncthreads();
+        A[0] = A[0] - lvv * Xp[(N - 1) + 2 * N];
+        x[0] = x[0] - lvv * Xp[N - 1];
+        x[1] = x[1] - lvv * Xp[(N - 1) + N];
       }
-      lu_solve_wave(N, A, ipiv, kz == NeZ - 1 ? 2 : 3, Xk);
+      lu_solve_rows<N, SEG, 3>(A, x, kz == NeZ - 1 ? 2 : 3, act, segmask, X + (size_t)kz * 3 * N);
     }
     for (int kz = NeZ - 2; kz >= 0; --kz) {
       double* Xk = X + (size_t)kz * 3 * N;
       const double* Xn = X + (size_t)(kz + 1) * 3 * N;
-      if (lane < N) {
-        const double tmp = Xk[lane + 2 * N];
-        Xk[lane] = Xk[lane] - tmp * Xn[0];
-        Xk[lane + N] = Xk[lane + N] - tmp * Xn[N];
+      if (act) {
+        const double tmp = Xk[r + 2 * N];
+        Xk[r] = Xk[r] - tmp * Xn[0];
+        Xk[r + N] = Xk[r + N] - tmp * Xn[N];
       }
       __syncthreads();
     }
-    for (int e = lane; e < NeZ * N; e += 64) {
-      const int kz = e / N, pv = e % N;
-      const uint32_t q = (uint32_t)Np * (ke2D + kz * m.Ne2D) + ij + Nfp * pv;
-      const double* Xk = X + (size_t)kz * 3 * N;
-      stg(a.pv[VU], q, ldg(a.pv[VU], q) + Xk[pv]);
-      stg(a.pv[VV], q, ldg(a.pv[VV], q) + Xk[pv + N]);
+    if (lv && act) {
+      for (int kz = 0; kz < NeZ; ++kz) {
+        const uint32_t q = (uint32_t)Np * (ke2D + kz * m.Ne2D) + ij + Nfp * pv;
+        const double* Xk = X + (size_t)kz * 3 * N;
+        stg(a.pv[VU], q, ldg(a.pv[VU], q) + Xk[pv]);
+        stg(a.pv[VV], q, ldg(a.pv[VV], q) + Xk[pv + N]);
+      }
     }
     __syncthreads();
   }
 }
 
 // ---------------------------------------------------------------------------
-// 4. construct_matbnd (:693-901) + solve (:329-479), one line per workgroup.
+// 4. construct_matbnd (:693-901) + solve (:329-479).
 // Unknowns of an element: (var, pv), var fastest: DENS, MOMZ, RHOT.
 template <int N1>
 __global__ void __launch_bounds__(64) hevi_solve3_kernel(const HeviArgs a) {
   constexpr int N = N1, Nfp = N1 * N1, Np = N1 * N1 * N1, n3 = 3 * N1;
+  constexpr int SEG = LineSeg<N1>::SEG3, LPW = 64 / SEG;
   extern __shared__ double lds[];
-  double* A = lds;                    // n3*n3
-  double* Lm = A + n3 * n3;           // n3*3
-  double* t12 = Lm + n3 * 3;          // [face][tmp1/tmp2][pv]: 4*N
-  double* cf = t12 + 4 * N;           // coefficients of the level and its neighbours: [3 levels][3][N]
-  double* X = cf + 9 * N;             // NeZ * 4 * n3: b, G(:,1..3) per level
-  int* ipiv = reinterpret_cast<int*>(X + (size_t)a.NeZ * 4 * n3);
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x, sg = lane / SEG, r = lane % SEG;
+  const bool act = r < n3;
+  const unsigned long long segmask = (SEG == 64 ? ~0ull : ((1ull << SEG) - 1ull)) << (sg * SEG);
   const DevMesh& m = a.m;
   const int nline = m.Ne2D * Nfp;
   const int NeZ = a.NeZ;
   const uint32_t nn = (uint32_t)Np * m.Ne;
   const double impl_fac = a.impl_fac;
-  for (int line = blockIdx.x; line < nline; line += gridDim.x) {
-    const int ke2D = line / Nfp, ij = line % Nfp;
+  double* sD = lds;      // D1D
+  double* sI = sD + N * N;  // IntrpMat_VPOrdM1_tr
+  double* cf = sI + N * N + (size_t)sg * (13 * N + (size_t)NeZ * 4 * n3);  // [3 levels][POT, WT, DPDRHOT][N]
+  double* t12 = cf + 9 * N;  // [face][tmp1, tmp2][N]
+  double* X = t12 + 4 * N;   // NeZ * 4 * n3: b, G(:,1..3) per level
+  for (int e = lane; e < N * N; e += 64) {
+    sD[e] = m.D1D[e];
+    sI[e] = m.Itr[e];
+  }
+  const int pv = act ? r / 3 : 0, v = r % 3;
+  for (int g = blockIdx.x; g * LPW < nline; g += gridDim.x) {
+    const int line = g * LPW + sg;
+    const bool lv = line < nline;
+    const int ke2D = lv ? line / Nfp : 0, ij = lv ? line % Nfp : 0;
     for (int kz = 0; kz < NeZ; ++kz) {
       const int ke = ke2D + kz * m.Ne2D;
-      double* Xk = X + (size_t)kz * 4 * n3;
-      // POT, WT, DPDRHOT of the line in this level (slot 1) and the levels below (0) / above (2)
-      for (int e = lane; e < 9 * N; e += 64) {
-        const int lv = e / (3 * N), r = e % (3 * N), c = r / N, pv = r % N;
-        int kzz = kz + lv - 1;
-        kzz = kzz < 0 ? 0 : (kzz > NeZ - 1 ? NeZ - 1 : kzz);
-        const uint32_t q = (uint32_t)Np * (ke2D + kzz * m.Ne2D) + ij + Nfp * pv;
-        cf[e] = ldg(a.coef, q + (uint32_t)c * nn);
-      }
-      if (lane < n3) {
-        const int pv = lane / 3, v = lane % 3;
-        Xk[lane] = ldg(a.b3, (uint32_t)Np * ke + ij + Nfp * pv + (uint32_t)v * nn);
-      }
-      // tmp1, tmp2 of both vertical faces
-      if (lane < 2 * N) {
-        const int f1 = 1 + lane / N, pv = lane % N;
-        const int kz2 = (f1 == 1) ? (kz - 1 > 0 ? kz - 1 : 0) : (kz + 1 < NeZ - 1 ? kz + 1 : NeZ - 1);
-        const bool bc = (kz == 0 && f1 == 1) || (kz == NeZ - 1 && f1 == 2);
-        const int f2 = bc ? f1 : ((f1 == 1) ? 2 : 1);
-        const int ke_2 = ke2D + kz2 * m.Ne2D;
-        const int p = ij + Nfp * pv;
-        const uint32_t gfp = (uint32_t)((3 + f1) * Nfp + ij) + (uint32_t)(6 * Nfp) * ke;
-        const double Lift = ldg(m.Lift, (uint32_t)(p + (3 + f1) * Np));
-        const double fac = 0.5 * impl_fac / gsqrtv(a, ke, p, N1) * Lift * ldg(m.Fscale, gfp);
-        const double am = ldg(a.alph, (uint32_t)((f1 - 1) * Nfp + ij) + (uint32_t)(2 * Nfp) * ke);
-        const double ap = ldg(a.alph, (uint32_t)((f2 - 1) * Nfp + ij) + (uint32_t)(2 * Nfp) * ke_2);
-        t12[(f1 - 1) * 2 * N + pv] = fac * (am > ap ? am : ap);
-        t12[(f1 - 1) * 2 * N + N + pv] = fac * ldg(m.nz, gfp);
+      __syncthreads();
+      if (lv) {
+        // POT, WT, DPDRHOT of the line in this level (slot 1) and the levels below (0) / above (2)
+        for (int e = r; e < 9 * N; e += SEG) {
+          const int lvl = e / (3 * N), rr = e % (3 * N), c = rr / N, pq = rr % N;
+          int kzz = kz + lvl - 1;
+          kzz = kzz < 0 ? 0 : (kzz > NeZ - 1 ? NeZ - 1 : kzz);
+          const uint32_t q = (uint32_t)Np * (ke2D + kzz * m.Ne2D) + ij + Nfp * pq;
+          cf[e] = ldg(a.coef, q + (uint32_t)c * nn);
+        }
+        // tmp1, tmp2 of both vertical faces
+        if (r < 2 * N) {
+          const int f1 = 1 + r / N, pq = r % N;
+          const int kz2 = (f1 == 1) ? (kz - 1 > 0 ? kz - 1 : 0) : (kz + 1 < NeZ - 1 ? kz + 1 : NeZ - 1);
+          const bool bc = (kz == 0 && f1 == 1) || (kz == NeZ - 1 && f1 == 2);
+          const int f2 = bc ? f1 : ((f1 == 1) ? 2 : 1);
+          const int ke_2 = ke2D + kz2 * m.Ne2D;
+          const int p = ij + Nfp * pq;
+          const uint32_t gfp = (uint32_t)((3 + f1) * Nfp + ij) + (uint32_t)(6 * Nfp) * ke;
+          const double Lift = ldg(m.Lift, (uint32_t)(p + (3 + f1) * Np));
+          const double fac = 0.5 * impl_fac / gsqrtv(a, ke, p, N1) * Lift * ldg(m.Fscale, gfp);
+          const double am = ldg(a.alph, (uint32_t)((f1 - 1) * Nfp + ij) + (uint32_t)(2 * Nfp) * ke);
+          const double ap = ldg(a.alph, (uint32_t)((f2 - 1) * Nfp + ij) + (uint32_t)(2 * Nfp) * ke_2);
+          t12[(f1 - 1) * 2 * N + pq] = fac * (am > ap ? am : ap);
+          t12[(f1 - 1) * 2 * N + N + pq] = fac * ldg(m.nz, gfp);
+        }
       }
       __syncthreads();
-      const double* POT0 = cf + 3 * N;  // this level
-      const double* WT0 = POT0 + N;
-      const double* DPD0 = WT0 + N;
-      // D block with the face terms of its first / last node column
-      for (int e = lane; e < n3 * n3; e += 64) {
-        const int r = e % n3, cc = e / n3;
-        const int v = r % 3, pv = r / 3, v2 = cc % 3, pv2 = cc / 3;
+      double A[n3];
+      double x[4] = {0.0, 0.0, 0.0, 0.0};
+      double lm[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int cc = 0; cc < n3; ++cc) A[cc] = 0.0;
+      if (lv && act) {
+        const double* POT0 = cf + 3 * N;  // this level
+        const double* WT0 = POT0 + N;
+        const double* DPD0 = WT0 + N;
         const uint32_t prow = (uint32_t)Np * ke + ij + Nfp * pv;
-        const double Id = (pv == pv2) ? 1.0 : 0.0;
-        const double fdz = ldg(m.E33, prow) / gsqrtv(a, ke, ij + Nfp * pv, N1) * (impl_fac * m.D1D[pv + N * pv2]);
-        const double POT = POT0[pv2], WT = WT0[pv2], DPD = DPD0[pv2];
-        double d;
-        if (v == 0) d = (v2 == 0) ? Id : (v2 == 1 ? fdz : 0.0);
-        else if (v == 1) d = (v2 == 1) ? Id : (v2 == 0 ? impl_fac * a.c.GRAV * m.Itr[pv2 + N * pv] : fdz * DPD);
-        else d = (v2 == 0) ? -fdz * POT * WT : (v2 == 1 ? fdz * POT : Id + fdz * WT);
+        const double Eg = ldg(m.E33, prow) / gsqrtv(a, ke, ij + Nfp * pv, N1);
+        x[0] = ldg(a.b3, prow + (uint32_t)v * nn);
+        const double tm1[2] = {t12[pv], t12[2 * N + pv]};      // tmp1 of faces 1, 2
+        const double tm2[2] = {t12[N + pv], t12[3 * N + pv]};  // tmp2
+        // D block with the face terms of its first / last node column
+#pragma unroll
+        for (int cc = 0; cc < n3; ++cc) {
+          const int v2 = cc % 3, pv2 = cc / 3;
+          const double Id = (pv == pv2) ? 1.0 : 0.0;
+          const double fdz = Eg * (impl_fac * sD[pv + N * pv2]);
+          const double POT = POT0[pv2], WT = WT0[pv2], DPD = DPD0[pv2];
+          double d;
+          if (v == 0) d = (v2 == 0) ? Id : (v2 == 1 ? fdz : 0.0);
+          else if (v == 1) d = (v2 == 1) ? Id : (v2 == 0 ? impl_fac * a.c.GRAV * sI[pv2 + N * pv] : fdz * DPD);
+          else d = (v2 == 0) ? -fdz * POT * WT : (v2 == 1 ? fdz * POT : Id + fdz * WT);
+#pragma unroll
+          for (int f1 = 1; f1 <= 2; ++f1) {
+            const int pv1 = (f1 == 1) ? 0 : N - 1;
+            if (pv2 != pv1) continue;
+            const bool bc = (kz == 0 && f1 == 1) || (kz == NeZ - 1 && f1 == 2);
+            const double tmp1 = tm1[f1 - 1], tmp2 = tm2[f1 - 1];
+            if (bc) {
+              if (v == 2 && v2 == 0) d = d + 2.0 * tmp2 * POT * WT;
+              else if (v == 0 && v2 == 1) d = d - 2.0 * tmp2;
+              else if (v == 1 && v2 == 1) d = d + 2.0 * tmp1;
+              else if (v == 2 && v2 == 1) d = d - 2.0 * tmp2 * POT;
+              else if (v == 2 && v2 == 2) d = d - 2.0 * tmp2 * WT;
+            } else {
+              if (v == 0 && v2 == 0) d = d + tmp1;
+              else if (v == 2 && v2 == 0) d = d + tmp2 * POT * WT;
+              else if (v == 0 && v2 == 1) d = d - tmp2;
+              else if (v == 1 && v2 == 1) d = d + tmp1;
+              else if (v == 2 && v2 == 1) d = d - tmp2 * POT;
+              else if (v == 1 && v2 == 2) d = d - tmp2 * DPD;
+              else if (v == 2 && v2 == 2) d = d + tmp1 - tmp2 * WT;
+            }
+          }
+          A[cc] = d;
+        }
+        // L (coupling to the top node of the level below) and U (bottom node
+        // of the level above, the G right-hand sides of this level)
+#pragma unroll
         for (int f1 = 1; f1 <= 2; ++f1) {
-          const int pv1 = (f1 == 1) ? 0 : N - 1;
-          if (pv2 != pv1) continue;
           const bool bc = (kz == 0 && f1 == 1) || (kz == NeZ - 1 && f1 == 2);
-          const double tmp1 = t12[(f1 - 1) * 2 * N + pv], tmp2 = t12[(f1 - 1) * 2 * N + N + pv];
-          if (bc) {
-            if (v == 2 && v2 == 0) d = d + 2.0 * tmp2 * POT * WT;
-            else if (v == 0 && v2 == 1) d = d - 2.0 * tmp2;
-            else if (v == 1 && v2 == 1) d = d + 2.0 * tmp1;
-            else if (v == 2 && v2 == 1) d = d - 2.0 * tmp2 * POT;
-            else if (v == 2 && v2 == 2) d = d - 2.0 * tmp2 * WT;
-          } else {
-            if (v == 0 && v2 == 0) d = d + tmp1;
-            else if (v == 2 && v2 == 0) d = d + tmp2 * POT * WT;
-            else if (v == 0 && v2 == 1) d = d - tmp2;
-            else if (v == 1 && v2 == 1) d = d + tmp1;
-            else if (v == 2 && v2 == 1) d = d - tmp2 * POT;
-            else if (v == 1 && v2 == 2) d = d - tmp2 * DPD;
-            else if (v == 2 && v2 == 2) d = d + tmp1 - tmp2 * WT;
+          if (bc) continue;
+          const int pv2 = (f1 == 1) ? N - 1 : 0;
+          const double* cfo = cf + (f1 == 1 ? 0 : 6 * N);
+          const double POT = cfo[pv2], WT = cfo[N + pv2], DPD = cfo[2 * N + pv2];
+          const double tmp1 = tm1[f1 - 1], tmp2 = tm2[f1 - 1];
+#pragma unroll
+          for (int v2 = 0; v2 < 3; ++v2) {
+            double l;
+            if (v2 == 0) l = (v == 0) ? -tmp1 : (v == 1 ? 0.0 : -tmp2 * POT * WT);
+            else if (v2 == 1) l = (v == 0) ? +tmp2 : (v == 1 ? -tmp1 : +tmp2 * POT);
+            else l = (v == 0) ? 0.0 : (v == 1 ? +tmp2 * DPD : -tmp1 + tmp2 * WT);
+            if (f1 == 1) lm[v2] = l;
+            else x[1 + v2] = l;
           }
         }
-        A[e] = d;
       }
-      // L (rows couple to the top node of the level below) and U (bottom node
-      // of the level above, stored in the G slots of this level)
-      for (int e = lane; e < 2 * n3 * 3; e += 64) {
-        const int f1 = 1 + e / (n3 * 3), r = (e % (n3 * 3)) % n3, v2 = (e % (n3 * 3)) / n3;
-        const bool bc = (kz == 0 && f1 == 1) || (kz == NeZ - 1 && f1 == 2);
-        if (bc) continue;
-        const int v = r % 3, pv = r / 3;
-        const int pv2 = (f1 == 1) ? N - 1 : 0;
-        const double* cfo = cf + (f1 == 1 ? 0 : 6 * N);  // level below / above
-        const double POT = cfo[pv2], WT = cfo[N + pv2], DPD = cfo[2 * N + pv2];
-        const double tmp1 = t12[(f1 - 1) * 2 * N + pv], tmp2 = t12[(f1 - 1) * 2 * N + N + pv];
-        double l;
-        if (v2 == 0) l = (v == 0) ? -tmp1 : (v == 1 ? 0.0 : -tmp2 * POT * WT);
-        else if (v2 == 1) l = (v == 0) ? +tmp2 : (v == 1 ? -tmp1 : +tmp2 * POT);
-        else l = (v == 0) ? 0.0 : (v == 1 ? +tmp2 * DPD : -tmp1 + tmp2 * WT);
-        if (f1 == 1) Lm[r + n3 * v2] = l;
-        else Xk[r + n3 * (1 + v2)] = l;
-      }
-      __syncthreads();
       if (kz > 0) {
         const double* Xp = X + (size_t)(kz - 1) * 4 * n3;
-        const int p1 = 3 * (N - 1), p2 = p1 + 1, p3 = p1 + 2;
-        if (lane < n3) {
-          const int r = lane;
-          const double t1 = Lm[r], t2 = Lm[r + n3], t3 = Lm[r + 2 * n3];
+        constexpr int p1 = 3 * (N - 1), p2 = p1 + 1, p3 = p1 + 2;
 #pragma unroll
-          for (int c = 0; c < 3; ++c)
-            A[r + n3 * c] = A[r + n3 * c] - t1 * Xp[p1 + n3 * (1 + c)] - t2 * Xp[p2 + n3 * (1 + c)] -
-                            t3 * Xp[p3 + n3 * (1 + c)];
-          Xk[r] = Xk[r] - t1 * Xp[p1] - t2 * Xp[p2] - t3 * Xp[p3];
-        }
-        __syncthreads();
+        for (int c = 0; c < 3; ++c)
+          A[c] = A[c] - lm[0] * Xp[p1 + n3 * (1 + c)] - lm[1] * Xp[p2 + n3 * (1 + c)] - lm[2] * Xp[p3 + n3 * (1 + c)];
+        x[0] = x[0] - lm[0] * Xp[p1] - lm[1] * Xp[p2] - lm[2] * Xp[p3];
       }
-      lu_solve_wave(n3, A, ipiv, kz == NeZ - 1 ? 1 : 4, Xk);
+      lu_solve_rows<n3, SEG, 4>(A, x, kz == NeZ - 1 ? 1 : 4, act, segmask, X + (size_t)kz * 4 * n3);
     }
     for (int kz = NeZ - 2; kz >= 0; --kz) {
       double* Xk = X + (size_t)kz * 4 * n3;
       const double* Xn = X + (size_t)(kz + 1) * 4 * n3;
-      if (lane < n3) {
-        const int r = lane;
-        Xk[r] = Xk[r] - Xk[r + n3] * Xn[0] - Xk[r + 2 * n3] * Xn[1] - Xk[r + 3 * n3] * Xn[2];
-      }
+      if (act) Xk[r] = Xk[r] - Xk[r + n3] * Xn[0] - Xk[r + 2 * n3] * Xn[1] - Xk[r + 3 * n3] * Xn[2];
       __syncthreads();
     }
-    for (int e = lane; e < NeZ * N; e += 64) {
-      const int kz = e / N, pv = e % N;
-      const uint32_t q = (uint32_t)Np * (ke2D + kz * m.Ne2D) + ij + Nfp * pv;
-      const double* Xk = X + (size_t)kz * 4 * n3;
-      stg(a.pv[VD], q, ldg(a.pv[VD], q) + Xk[0 + 3 * pv]);
-      stg(a.pv[VW], q, ldg(a.pv[VW], q) + Xk[1 + 3 * pv]);
-      stg(a.pv[VT], q, ldg(a.pv[VT], q) + Xk[2 + 3 * pv]);
+    if (lv && act) {
+      double* dst = v == 0 ? a.pv[VD] : (v == 1 ? a.pv[VW] : a.pv[VT]);
+      for (int kz = 0; kz < NeZ; ++kz) {
+        const uint32_t q = (uint32_t)Np * (ke2D + kz * m.Ne2D) + ij + Nfp * pv;
+        stg(dst, q, ldg(dst, q) + X[(size_t)kz * 4 * n3 + r]);
+      }
     }
-    __syncthreads();
   }
 }
 
@@ -713,11 +774,13 @@ LineArgs line_args(const DevMesh& m, const HeviArgs& a, const double* const Tex[
 }
 
 template <int N1>
-size_t lds_uv(int NeZ) { return sizeof(double) * (N1 * N1 + N1 + (size_t)NeZ * 3 * N1) + sizeof(int) * 64; }
+size_t lds_uv(int NeZ) {
+  return sizeof(double) * (size_t)(64 / LineSeg<N1>::SEGU) * NeZ * 3 * N1;
+}
 template <int N1>
 size_t lds_3(int NeZ) {
   constexpr int n3 = 3 * N1;
-  return sizeof(double) * (n3 * n3 + n3 * 3 + 4 * N1 + 9 * N1 + (size_t)NeZ * 4 * n3) + sizeof(int) * 64;
+  return sizeof(double) * (2 * N1 * N1 + (size_t)(64 / LineSeg<N1>::SEG3) * (13 * N1 + (size_t)NeZ * 4 * n3));
 }
 
 template <int N1>
@@ -725,9 +788,11 @@ int run_newton(fedg_ctx* ctx, HeviArgs& a, int max_it, double rtol, double* resi
                double* partial_h, int* it_done, double r0_ext = 0.0) {
   const DevMesh& m = ctx->m;
   const int nline = m.Ne2D * N1 * N1;
-  const int nb = nline < 4096 ? nline : 4096;
+  const int guv = (nline + 64 / LineSeg<N1>::SEGU - 1) / (64 / LineSeg<N1>::SEGU);
+  const int g3 = (nline + 64 / LineSeg<N1>::SEG3 - 1) / (64 / LineSeg<N1>::SEG3);
+  const int nbuv = guv < 65536 ? guv : 65536, nb3 = g3 < 65536 ? g3 : 65536;
   const size_t luv = lds_uv<N1>(a.NeZ), l3 = lds_3<N1>(a.NeZ);
-  if (l3 > 64 * 1024) {
+  if (l3 > 64 * 1024 || luv > 64 * 1024) {
     set_error("HEVI: too many elements in the vertical for the line solve's LDS (NeZ)");
     return FEDG_ERR_UNSUPPORTED;
   }
@@ -748,7 +813,7 @@ int run_newton(fedg_ctx* ctx, HeviArgs& a, int max_it, double rtol, double* resi
       FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
       for (int b = 0; b < 256; ++b) r2 += partial_h[b];
     }
-    hipLaunchKernelGGL(hevi_solve_uv_kernel<N1>, dim3(nb), dim3(64), luv, ctx->stream, a);
+    hipLaunchKernelGGL(hevi_solve_uv_kernel<N1>, dim3(nbuv), dim3(64), luv, ctx->stream, a);
     FEDG_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(hevi_rhs3_kernel<N1>, dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
     FEDG_HIP_CHECK(hipGetLastError());
@@ -763,7 +828,7 @@ int run_newton(fedg_ctx* ctx, HeviArgs& a, int max_it, double rtol, double* resi
       if (it == 0 && r0_ext <= 0.0) r0 = r;
       if (rtol > 0.0 && (it > 0 || r0_ext > 0.0) && r <= rtol * r0) break;
     }
-    hipLaunchKernelGGL(hevi_solve3_kernel<N1>, dim3(nb), dim3(64), l3, ctx->stream, a);
+    hipLaunchKernelGGL(hevi_solve3_kernel<N1>, dim3(nb3), dim3(64), l3, ctx->stream, a);
     FEDG_HIP_CHECK(hipGetLastError());
   }
   *it_done = it;
