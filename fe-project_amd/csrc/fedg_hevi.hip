@@ -264,55 +264,68 @@ __global__ void __launch_bounds__(N1* N1* N1) hevi_rhs3_kernel(const HeviArgs a)
 // (FDS/scale_atm_dyn_dgm_hevi_common_linalgebra.F90:2142-2445), so results
 // are bit for bit the CPU restatement's.
 
-template <int SEG>
-__device__ __forceinline__ double seg_max(double v) {
-#pragma unroll
-  for (int s = SEG / 2; s > 0; s >>= 1) {
-    const double o = __shfl_xor(v, s, 64);
-    v = o > v ? o : v;
-  }
-  return v;
+// Cross-lane exchanges inside a segment: DPP permutations within 16-lane
+// rows, ds_swizzle between the two rows of a 32-lane segment.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
 }
 
-template <int SEG>
-__device__ __forceinline__ int seg_min(int v) {
-#pragma unroll
-  for (int s = SEG / 2; s > 0; s >>= 1) {
-    const int o = __shfl_xor(v, s, 64);
-    v = o < v ? o : v;
-  }
-  return v;
+// pivot candidate: larger value wins, equal values go to the smaller key
+// (position * 64 + lane)
+__device__ __forceinline__ void piv_take(double& v, int& k, double ov, int ok) {
+  const bool t = (ov > v) || (ov == v && ok < k);
+  v = t ? ov : v;
+  k = t ? ok : k;
 }
 
-// the lane of the segment where pred holds (exactly one)
-__device__ __forceinline__ int seg_lane(bool pred, unsigned long long segmask) {
-  return __ffsll((long long)(__ballot(pred) & segmask)) - 1;
+template <int CTRL>
+__device__ __forceinline__ void piv_dpp(double& v, int& k) {
+  const int lo = dpp_i<CTRL>(__double2loint(v));
+  const int hi = dpp_i<CTRL>(__double2hiint(v));
+  piv_take(v, k, __hiloint2double(hi, lo), dpp_i<CTRL>(k));
+}
+
+// all-reduce of (value, key) over the segment
+template <int SEG>
+__device__ __forceinline__ void piv_reduce(double& v, int& k) {
+  piv_dpp<0xB1>(v, k);   // quad_perm [1,0,3,2]: partner lane ^ 1
+  piv_dpp<0x4E>(v, k);   // quad_perm [2,3,0,1]: partner lane ^ 2
+  piv_dpp<0x141>(v, k);  // row_half_mirror: the other quad of the 8
+  if constexpr (SEG >= 16) piv_dpp<0x140>(v, k);  // row_mirror: the other 8 of the row
+  if constexpr (SEG >= 32) {
+    constexpr int X16 = (0x10 << 10) | 0x1F;  // swizzle bitmask mode: lane ^ 16 within 32
+    const int lo = __builtin_amdgcn_ds_swizzle(__double2loint(v), X16);
+    const int hi = __builtin_amdgcn_ds_swizzle(__double2hiint(v), X16);
+    piv_take(v, k, __hiloint2double(hi, lo), __builtin_amdgcn_ds_swizzle(k, X16));
+  }
+  if constexpr (SEG >= 64) piv_take(v, k, __shfl_xor(v, 32, 64), __shfl_xor(k, 32, 64));
 }
 
 // LU with partial pivoting of the segment's n x n block (row r in A of lane
-// r), then nr <= NR right-hand sides (x[q] of lane r: entry r of rhs q).
-// The solution goes to X (LDS, column-major n x NR).  Every lane of the
+// r), applied to nr <= NR right-hand sides as they go (x[q] of lane r: entry
+// r of rhs q; the row operations on them are the forward substitution), then
+// back substitution through the factors in U (LDS, n x n by position).  The
+// solution goes to X (LDS, column-major n x NR).  Every lane of the
 // single-wavefront workgroup calls it.
 template <int n, int SEG, int NR>
-__device__ __forceinline__ void lu_solve_rows(double (&A)[n], double (&x)[NR], int nr, bool act,
-                                              unsigned long long segmask, double* X) {
+__device__ __forceinline__ void lu_solve_rows(double (&A)[n], double (&x)[NR], int nr, bool act, double* U,
+                                              double* X) {
   const int lane = threadIdx.x;
   int pos = act ? (lane % SEG) : (1 << 20);
 #pragma unroll
   for (int k = 0; k < n; ++k) {
     // pivot: the first position >= k of max |A(i,k)|; a NaN on the diagonal
     // keeps k, a NaN below it is never chosen (the reference's strict '>')
-    double av = -1.0;
+    double av = -2.0;
+    int key = 1 << 30;
     if (act && pos >= k) {
       const double t = fabs(A[k]);
-      av = (t == t) ? t : -1.0;
+      av = (t == t) ? t : (pos == k ? __builtin_huge_val() : -1.0);
+      key = pos * 64 + lane;
     }
-    const int lk = seg_lane(act && pos == k, segmask);
-    const double akk = fabs(__shfl(A[k], lk, 64));
-    const double mx = seg_max<SEG>(av);
-    const int cand = seg_min<SEG>((act && pos >= k && av == mx) ? pos : (1 << 20));
-    const int ppos = (akk == akk) ? cand : k;
-    const int lp = seg_lane(act && pos == ppos, segmask);
+    piv_reduce<SEG>(av, key);
+    const int ppos = key >> 6, lp = key & 63;
     if (act) {
       if (pos == k) pos = ppos;
       else if (pos == ppos) pos = k;
@@ -327,42 +340,39 @@ __device__ __forceinline__ void lu_solve_rows(double (&A)[n], double (&x)[NR], i
       const double pj = __shfl(A[j], lp, 64);
       if (below) A[j] = A[j] - l * pj;
     }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      const double pq = __shfl(x[q], lp, 64);
+      if (below && q < nr) x[q] = x[q] - pq * l;
+    }
     if (lane == lp) A[k] = dinv;
     else if (below) A[k] = l;
   }
-  // forward substitution with the unit lower factor, column by column
+  // the factors' rows to LDS by position, the forward-reduced right-hand
+  // sides to X
+  if (act) {
 #pragma unroll
-  for (int j = 0; j < n - 1; ++j) {
-    const int lj = seg_lane(act && pos == j, segmask);
-    double xj[NR];
+    for (int j = 0; j < n; ++j) U[pos * n + j] = A[j];
 #pragma unroll
-    for (int q = 0; q < NR; ++q) xj[q] = __shfl(x[q], lj, 64);
-    if (act && pos > j) {
-#pragma unroll
-      for (int q = 0; q < NR; ++q)
-        if (q < nr) x[q] = x[q] - xj[q] * A[j];
+    for (int q = 0; q < NR; ++q)
+      if (q < nr) X[pos + n * q] = x[q];
+  }
+  __syncthreads();
+  // back substitution, one lane per right-hand side, each row's sum in
+  // ascending column order
+  const int rq = lane % SEG;
+  if (rq < nr) {
+    double xs[n];
+#pragma clang loop unroll(full)
+    for (int i = n - 1; i >= 0; --i) {
+      double t = X[i + n * rq];
+#pragma clang loop unroll(full)
+      for (int j = i + 1; j < n; ++j) t = t - U[i * n + j] * xs[j];
+      xs[i] = t * U[i * n + i];
+      X[i + n * rq] = xs[i];
     }
   }
-  // back substitution, row by row (each row's sum in ascending column order)
-#pragma clang loop unroll(full)
-  for (int i = n - 1; i >= 0; --i) {
-    if (act && pos == i) {
-      double t[NR];
-#pragma clang loop unroll(full)
-      for (int q = 0; q < NR; ++q) t[q] = x[q];
-#pragma clang loop unroll(full)
-      for (int j = i + 1; j < n; ++j) {
-        const double aj = A[j];
-#pragma clang loop unroll(full)
-        for (int q = 0; q < NR; ++q)
-          if (q < nr) t[q] = t[q] - aj * X[j + n * q];
-      }
-#pragma clang loop unroll(full)
-      for (int q = 0; q < NR; ++q)
-        if (q < nr) X[i + n * q] = t[q] * A[i];
-    }
-    __syncthreads();
-  }
+  __syncthreads();
 }
 
 template <int N1>
@@ -381,12 +391,12 @@ __global__ void __launch_bounds__(64) hevi_solve_uv_kernel(const HeviArgs a) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x, sg = lane / SEG, r = lane % SEG;
   const bool act = r < N;
-  const unsigned long long segmask = (SEG == 64 ? ~0ull : ((1ull << SEG) - 1ull)) << (sg * SEG);
   const DevMesh& m = a.m;
   const int nline = m.Ne2D * Nfp;
   const int NeZ = a.NeZ;
   const uint32_t nn = (uint32_t)Np * m.Ne;
-  double* X = lds + (size_t)sg * NeZ * 3 * N;  // per level: b(:,1), b(:,2), G
+  double* U = lds + (size_t)sg * (N * N + (size_t)NeZ * 3 * N);  // LU factors
+  double* X = U + N * N;  // per level: b(:,1), b(:,2), G
   const int pv = act ? r : 0;
   for (int g = blockIdx.x; g * LPW < nline; g += gridDim.x) {
     const int line = g * LPW + sg;
@@ -432,7 +442,7 @@ __global__ void __launch_bounds__(64) hevi_solve_uv_kernel(const HeviArgs a) {
         x[0] = x[0] - lvv * Xp[N - 1];
         x[1] = x[1] - lvv * Xp[(N - 1) + N];
       }
-      lu_solve_rows<N, SEG, 3>(A, x, kz == NeZ - 1 ? 2 : 3, act, segmask, X + (size_t)kz * 3 * N);
+      lu_solve_rows<N, SEG, 3>(A, x, kz == NeZ - 1 ? 2 : 3, act, U, X + (size_t)kz * 3 * N);
     }
     for (int kz = NeZ - 2; kz >= 0; --kz) {
       double* Xk = X + (size_t)kz * 3 * N;
@@ -466,7 +476,6 @@ __global__ void __launch_bounds__(64) hevi_solve3_kernel(const HeviArgs a) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x, sg = lane / SEG, r = lane % SEG;
   const bool act = r < n3;
-  const unsigned long long segmask = (SEG == 64 ? ~0ull : ((1ull << SEG) - 1ull)) << (sg * SEG);
   const DevMesh& m = a.m;
   const int nline = m.Ne2D * Nfp;
   const int NeZ = a.NeZ;
@@ -474,9 +483,10 @@ __global__ void __launch_bounds__(64) hevi_solve3_kernel(const HeviArgs a) {
   const double impl_fac = a.impl_fac;
   double* sD = lds;      // D1D
   double* sI = sD + N * N;  // IntrpMat_VPOrdM1_tr
-  double* cf = sI + N * N + (size_t)sg * (13 * N + (size_t)NeZ * 4 * n3);  // [3 levels][POT, WT, DPDRHOT][N]
+  double* cf = sI + N * N + (size_t)sg * (13 * N + n3 * n3 + (size_t)NeZ * 4 * n3);  // [3 levels][POT, WT, DPDRHOT][N]
   double* t12 = cf + 9 * N;  // [face][tmp1, tmp2][N]
-  double* X = t12 + 4 * N;   // NeZ * 4 * n3: b, G(:,1..3) per level
+  double* U = t12 + 4 * N;   // LU factors
+  double* X = U + n3 * n3;   // NeZ * 4 * n3: b, G(:,1..3) per level
   for (int e = lane; e < N * N; e += 64) {
     sD[e] = m.D1D[e];
     sI[e] = m.Itr[e];
@@ -594,7 +604,7 @@ __global__ void __launch_bounds__(64) hevi_solve3_kernel(const HeviArgs a) {
           A[c] = A[c] - lm[0] * Xp[p1 + n3 * (1 + c)] - lm[1] * Xp[p2 + n3 * (1 + c)] - lm[2] * Xp[p3 + n3 * (1 + c)];
         x[0] = x[0] - lm[0] * Xp[p1] - lm[1] * Xp[p2] - lm[2] * Xp[p3];
       }
-      lu_solve_rows<n3, SEG, 4>(A, x, kz == NeZ - 1 ? 1 : 4, act, segmask, X + (size_t)kz * 4 * n3);
+      lu_solve_rows<n3, SEG, 4>(A, x, kz == NeZ - 1 ? 1 : 4, act, U, X + (size_t)kz * 4 * n3);
     }
     for (int kz = NeZ - 2; kz >= 0; --kz) {
       double* Xk = X + (size_t)kz * 4 * n3;
@@ -775,12 +785,13 @@ LineArgs line_args(const DevMesh& m, const HeviArgs& a, const double* const Tex[
 
 template <int N1>
 size_t lds_uv(int NeZ) {
-  return sizeof(double) * (size_t)(64 / LineSeg<N1>::SEGU) * NeZ * 3 * N1;
+  return sizeof(double) * (size_t)(64 / LineSeg<N1>::SEGU) * (N1 * N1 + (size_t)NeZ * 3 * N1);
 }
 template <int N1>
 size_t lds_3(int NeZ) {
   constexpr int n3 = 3 * N1;
-  return sizeof(double) * (2 * N1 * N1 + (size_t)(64 / LineSeg<N1>::SEG3) * (13 * N1 + (size_t)NeZ * 4 * n3));
+  return sizeof(double) *
+         (2 * N1 * N1 + (size_t)(64 / LineSeg<N1>::SEG3) * (13 * N1 + n3 * n3 + (size_t)NeZ * 4 * n3));
 }
 
 template <int N1>
