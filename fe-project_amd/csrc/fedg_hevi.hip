@@ -305,11 +305,11 @@ __device__ __forceinline__ void piv_reduce(double& v, int& k) {
 // LU with partial pivoting of the segment's n x n block (row r in A of lane
 // r), applied to nr <= NR right-hand sides as they go (x[q] of lane r: entry
 // r of rhs q; the row operations on them are the forward substitution), then
-// back substitution through the factors in U (LDS, n x n by position).  The
-// solution goes to X (LDS, column-major n x NR).  Every lane of the
-// single-wavefront workgroup calls it.
+// back substitution.  The solution goes to X (LDS, column-major n x NR);
+// P2L (LDS, n ints) receives the lane of every row position.  Every lane of
+// the single-wavefront workgroup calls it.
 template <int n, int SEG, int NR>
-__device__ __forceinline__ void lu_solve_rows(double (&A)[n], double (&x)[NR], int nr, bool act, double* U,
+__device__ __forceinline__ void lu_solve_rows(double (&A)[n], double (&x)[NR], int nr, bool act, int* P2L,
                                               double* X) {
   const int lane = threadIdx.x;
   int pos = act ? (lane % SEG) : (1 << 20);
@@ -348,29 +348,27 @@ __device__ __forceinline__ void lu_solve_rows(double (&A)[n], double (&x)[NR], i
     if (lane == lp) A[k] = dinv;
     else if (below) A[k] = l;
   }
-  // the factors' rows to LDS by position, the forward-reduced right-hand
-  // sides to X
+  // the forward-reduced right-hand sides to X, the lane of every position
   if (act) {
-#pragma unroll
-    for (int j = 0; j < n; ++j) U[pos * n + j] = A[j];
 #pragma unroll
     for (int q = 0; q < NR; ++q)
       if (q < nr) X[pos + n * q] = x[q];
+    P2L[pos] = lane;
   }
   __syncthreads();
   // back substitution, one lane per right-hand side, each row's sum in
-  // ascending column order
+  // ascending column order; the factors' rows come from their lanes
   const int rq = lane % SEG;
-  if (rq < nr) {
-    double xs[n];
+  const bool chain = rq < nr;
+  double xs[n];
 #pragma clang loop unroll(full)
-    for (int i = n - 1; i >= 0; --i) {
-      double t = X[i + n * rq];
+  for (int i = n - 1; i >= 0; --i) {
+    const int li = P2L[i];
+    double t = chain ? X[i + n * rq] : 0.0;
 #pragma clang loop unroll(full)
-      for (int j = i + 1; j < n; ++j) t = t - U[i * n + j] * xs[j];
-      xs[i] = t * U[i * n + i];
-      X[i + n * rq] = xs[i];
-    }
+    for (int j = i + 1; j < n; ++j) t = t - __shfl(A[j], li, 64) * xs[j];
+    xs[i] = t * __shfl(A[i], li, 64);
+    if (chain) X[i + n * rq] = xs[i];
   }
   __syncthreads();
 }
@@ -395,8 +393,8 @@ __global__ void __launch_bounds__(64) hevi_solve_uv_kernel(const HeviArgs a) {
   const int nline = m.Ne2D * Nfp;
   const int NeZ = a.NeZ;
   const uint32_t nn = (uint32_t)Np * m.Ne;
-  double* U = lds + (size_t)sg * (N * N + (size_t)NeZ * 3 * N);  // LU factors
-  double* X = U + N * N;  // per level: b(:,1), b(:,2), G
+  double* X = lds + (size_t)sg * (N + (size_t)NeZ * 3 * N);  // per level: b(:,1), b(:,2), G
+  int* P2L = reinterpret_cast<int*>(X + (size_t)NeZ * 3 * N);
   const int pv = act ? r : 0;
   for (int g = blockIdx.x; g * LPW < nline; g += gridDim.x) {
     const int line = g * LPW + sg;
@@ -442,7 +440,7 @@ __global__ void __launch_bounds__(64) hevi_solve_uv_kernel(const HeviArgs a) {
         x[0] = x[0] - lvv * Xp[N - 1];
         x[1] = x[1] - lvv * Xp[(N - 1) + N];
       }
-      lu_solve_rows<N, SEG, 3>(A, x, kz == NeZ - 1 ? 2 : 3, act, U, X + (size_t)kz * 3 * N);
+      lu_solve_rows<N, SEG, 3>(A, x, kz == NeZ - 1 ? 2 : 3, act, P2L, X + (size_t)kz * 3 * N);
     }
     for (int kz = NeZ - 2; kz >= 0; --kz) {
       double* Xk = X + (size_t)kz * 3 * N;
@@ -483,10 +481,10 @@ __global__ void __launch_bounds__(64) hevi_solve3_kernel(const HeviArgs a) {
   const double impl_fac = a.impl_fac;
   double* sD = lds;      // D1D
   double* sI = sD + N * N;  // IntrpMat_VPOrdM1_tr
-  double* cf = sI + N * N + (size_t)sg * (13 * N + n3 * n3 + (size_t)NeZ * 4 * n3);  // [3 levels][POT, WT, DPDRHOT][N]
+  double* cf = sI + N * N + (size_t)sg * (13 * N + n3 + (size_t)NeZ * 4 * n3);  // [3 levels][POT, WT, DPDRHOT][N]
   double* t12 = cf + 9 * N;  // [face][tmp1, tmp2][N]
-  double* U = t12 + 4 * N;   // LU factors
-  double* X = U + n3 * n3;   // NeZ * 4 * n3: b, G(:,1..3) per level
+  int* P2L = reinterpret_cast<int*>(t12 + 4 * N);  // lane of each row position
+  double* X = t12 + 4 * N + n3;  // NeZ * 4 * n3: b, G(:,1..3) per level
   for (int e = lane; e < N * N; e += 64) {
     sD[e] = m.D1D[e];
     sI[e] = m.Itr[e];
@@ -604,7 +602,7 @@ __global__ void __launch_bounds__(64) hevi_solve3_kernel(const HeviArgs a) {
           A[c] = A[c] - lm[0] * Xp[p1 + n3 * (1 + c)] - lm[1] * Xp[p2 + n3 * (1 + c)] - lm[2] * Xp[p3 + n3 * (1 + c)];
         x[0] = x[0] - lm[0] * Xp[p1] - lm[1] * Xp[p2] - lm[2] * Xp[p3];
       }
-      lu_solve_rows<n3, SEG, 4>(A, x, kz == NeZ - 1 ? 1 : 4, act, U, X + (size_t)kz * 4 * n3);
+      lu_solve_rows<n3, SEG, 4>(A, x, kz == NeZ - 1 ? 1 : 4, act, P2L, X + (size_t)kz * 4 * n3);
     }
     for (int kz = NeZ - 2; kz >= 0; --kz) {
       double* Xk = X + (size_t)kz * 4 * n3;
@@ -785,13 +783,13 @@ LineArgs line_args(const DevMesh& m, const HeviArgs& a, const double* const Tex[
 
 template <int N1>
 size_t lds_uv(int NeZ) {
-  return sizeof(double) * (size_t)(64 / LineSeg<N1>::SEGU) * (N1 * N1 + (size_t)NeZ * 3 * N1);
+  return sizeof(double) * (size_t)(64 / LineSeg<N1>::SEGU) * (N1 + (size_t)NeZ * 3 * N1);
 }
 template <int N1>
 size_t lds_3(int NeZ) {
   constexpr int n3 = 3 * N1;
   return sizeof(double) *
-         (2 * N1 * N1 + (size_t)(64 / LineSeg<N1>::SEG3) * (13 * N1 + n3 * n3 + (size_t)NeZ * 4 * n3));
+         (2 * N1 * N1 + (size_t)(64 / LineSeg<N1>::SEG3) * (13 * N1 + n3 + (size_t)NeZ * 4 * n3));
 }
 
 template <int N1>
