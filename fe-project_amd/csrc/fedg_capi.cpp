@@ -378,6 +378,9 @@ int fedg_ctx_create(const fedg_mesh_desc* d, fedg_ctx** out) {
     delete ctx;
     return FEDG_ERR_HIP;
   }
+  if (hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess ||
+      ctx->ncu < 1)
+    ctx->ncu = 256;
   const size_t NpNe = (size_t)Np * m.Ne, NpNeA = (size_t)Np * m.NeA, nf = (size_t)NfpTot * m.Ne;
   const size_t n2d = (size_t)Nfp * m.Ne2D;
   int rc = FEDG_OK;

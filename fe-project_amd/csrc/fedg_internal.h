@@ -75,6 +75,7 @@ struct fedg_ctx {
   fedg::Consts c;
   hipStream_t stream;
   int device;
+  int ncu;               // compute units of the device
   fedg::HeviWork* hevi;  // implicit-solve workspace, allocated on first use
 };
 
