@@ -812,7 +812,7 @@ int fedg_hevi_cal_vi_guess(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, doub
   if (!mlp || !scale || !(dt > 0.0) || !(fabs(impl_fac) > 0.0)) return FEDG_ERR_ARG;
   HeviGuess g{};
   g.mlp = mlp->impl;
-  for (int v = 0; v < 10; ++v) {
+  for (int v = 0; v < 15; ++v) {
     if (!(scale[v] > 0.0)) return FEDG_ERR_ARG;
     g.scale[v] = scale[v];
   }
@@ -1239,8 +1239,9 @@ struct fedg_hevi {
   long newton_iters, newton_calls;
   // learned initial guess and training-pair harvest
   fedg::MlpImpl* mlp;
-  double scale[10];  // 5 state scales, then 5 increment scales (VID order)
+  double scale[15];  // state, tendency-feature and increment scales (5 each, VID order)
   long guess_lines, guess_fallbacks;
+  double guess_r2_plain, guess_r2_start;
   float* hx;
   float* hy;
   int hcap, hcount, hper;
@@ -1268,6 +1269,7 @@ int fedg_hevi_create(fedg_ctx* ctx, const char* scheme, double dt, const double*
   st->ctx = ctx;
   st->max_it = max_it;
   st->rtol = rtol;
+  for (int v = 0; v < 15; ++v) st->scale[v] = 1.0;
   int rc = fedg_tint_create(ctx, scheme, dt, &st->tint);
   if (rc == FEDG_OK && !st->tint->imex) {
     set_error("the HEVI step takes an IMEX scheme (IMEX_ARK232, IMEX_ARK324)");
@@ -1371,7 +1373,7 @@ static int hevi_step(fedg_hevi* st, hipEvent_t* ev) {
     HeviGuess g{};
     g.mlp = st->mlp;
     for (int v = 0; v < 5; ++v) g.Tex[v] = Tprev[v];
-    for (int v = 0; v < 10; ++v) g.scale[v] = st->scale[v];
+    for (int v = 0; v < 15; ++v) g.scale[v] = st->scale[v];
     g.dt = t->dt;
     if (ev) FEDG_HIP_CHECK(hipEventRecord(ev[0], ctx->stream));
     TRY(hevi_cal_vi(ctx, w, Ti, qv, v0, st->DENS_hyd, st->PRES_hyd, st->Rtot, st->CVtot, st->CPtot, impl_fac,
@@ -1379,6 +1381,8 @@ static int hevi_step(fedg_hevi* st, hipEvent_t* ev) {
     if (ev) FEDG_HIP_CHECK(hipEventRecord(ev[1], ctx->stream));
     st->guess_lines += g.lines;
     st->guess_fallbacks += g.fallbacks;
+    st->guess_r2_plain += g.r2_plain;
+    st->guess_r2_start += g.r2_start;
     if (st->hx && fabs(impl_fac) > 0.0 && st->hcount < st->hcap) {
       const long nlines = (long)m.Ne2D * m.Nfp;
       const int L = (m.Ne / m.Ne2D) * m.n1;
@@ -1526,9 +1530,9 @@ int fedg_mlp_adam_step(fedg_mlp* m, float lr, float beta1, float beta2, float ep
 
 int fedg_hevi_set_guess(fedg_hevi* st, fedg_mlp* mlp, const double* scale) {
   if (!st || !scale) return FEDG_ERR_ARG;
-  for (int v = 0; v < 10; ++v)
+  for (int v = 0; v < 15; ++v)
     if (!(scale[v] > 0.0)) return FEDG_ERR_ARG;
-  for (int v = 0; v < 10; ++v) st->scale[v] = scale[v];
+  for (int v = 0; v < 15; ++v) st->scale[v] = scale[v];
   st->mlp = mlp ? mlp->impl : nullptr;
   return FEDG_OK;
 }
@@ -1537,6 +1541,13 @@ int fedg_hevi_guess_stats(const fedg_hevi* st, long* lines, long* fallbacks) {
   if (!st) return FEDG_ERR_ARG;
   if (lines) *lines = st->guess_lines;
   if (fallbacks) *fallbacks = st->guess_fallbacks;
+  return FEDG_OK;
+}
+
+int fedg_hevi_guess_residuals(const fedg_hevi* st, double* r2_plain, double* r2_start) {
+  if (!st) return FEDG_ERR_ARG;
+  if (r2_plain) *r2_plain = st->guess_r2_plain;
+  if (r2_start) *r2_start = st->guess_r2_start;
   return FEDG_OK;
 }
 

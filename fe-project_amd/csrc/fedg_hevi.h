@@ -25,17 +25,19 @@ struct HeviWork {
 struct MlpImpl;
 
 // Learned initial iterate of the Newton solve: the network maps a line's
-// stage state q* / s, impl_fac T_ex / d and impl_fac/dt to (q - q*) / d of
-// the converged iterate q (s: state scales, d: increment scales, per
-// variable).  Lines whose predicted iterate has a larger residual than q^n
+// stage state q* / s, impl_fac T_ex / t and impl_fac/dt to (q - q*) / d of
+// the converged iterate q (s, t, d: per-variable scales of the state, the
+// tendency feature and the increment).  Lines whose predicted iterate has a larger residual than q^n
 // start from q^n (fallback).
 struct HeviGuess {
   MlpImpl* mlp;
   const double* Tex[5];  // explicit tendency of the previous stage (VID order), or null
-  double scale[10];      // s (VID order), then d
+  double scale[15];      // s (VID order), then t, then d
   double dt;
   long lines;            // out: lines solved
   long fallbacks;        // out: lines that fell back to q^n
+  double r2_plain;       // out: ||G(q^n)||^2 summed over the solves
+  double r2_start;       // out: ||G||^2 at the iterates the lines started from
 };
 
 int hevi_work_get(fedg_ctx* ctx, HeviWork** w);
@@ -55,7 +57,7 @@ int hevi_cal_vi(fedg_ctx* ctx, HeviWork* w, double* const tend[5], const double*
 // of a converged solve (iterate in w->pv) into X (nrows x n_in) and Y
 // (nrows x n_out), device fp32; the network's input/output layout.
 int hevi_harvest(fedg_ctx* ctx, HeviWork* w, const double* const q[5], const double* const var0[5],
-                 const double* const Tex[5], const double scale[10], double impl_fac, double dt, long line0,
+                 const double* const Tex[5], const double scale[15], double impl_fac, double dt, long line0,
                  long stride, int nrows, float* X, float* Y);
 
 }  // namespace fedg

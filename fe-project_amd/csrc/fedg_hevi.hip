@@ -24,6 +24,8 @@
 // substitution.  Every entry sees the same operations in the same order as
 // the reference's serial loops, so results are bit for bit those of the CPU
 // restatement.
+#include <cfloat>
+
 #include "fedg_device.h"
 #include "fedg_hevi.h"
 #include "fedg_mlp.h"
@@ -659,6 +661,7 @@ struct LineArgs {
   const double* Tex[5];
   double* pv[5];
   double inv_s[5];      // 1 / state scale
+  double inv_t[5];      // 1 / tendency-feature scale
   double inv_d[5], d[5];  // increment scale
   double impl_fac, dt;
   int Np, Nfp, Ne2D, N1, L, nlines;
@@ -681,7 +684,7 @@ __global__ void line_features_kernel(const LineArgs a, float* X, int n_in, float
     const double qs = ldg(a.qs[v], n);
     x[v * a.L + t] = (float)(qs * a.inv_s[v]);
     const double te = a.Tex[v] ? ldg(a.Tex[v], n) : 0.0;
-    x[(5 + v) * a.L + t] = (float)(a.impl_fac * te * a.inv_d[v]);
+    x[(5 + v) * a.L + t] = (float)(a.impl_fac * te * a.inv_t[v]);
     if (Y) Y[(size_t)row * n_out + v * a.L + t] = (float)((ldg(a.pv[v], n) - qs) * a.inv_d[v]);
   }
   if (t == 0) x[10 * a.L] = (float)(a.impl_fac / a.dt);
@@ -720,17 +723,23 @@ __global__ void line_resid_kernel(const LineArgs a, const double* buv, const dou
   if (t == 0) r[line] = sh[0];
 }
 
-// lines whose predicted iterate has the larger residual restart from q^n
-__global__ void line_select_kernel(const LineArgs a, const double* r_guess, const double* r_plain, double* flag) {
+// lines whose predicted iterate has the larger residual restart from q^n;
+// r_guess becomes the residual of the start the line keeps
+__global__ void line_select_kernel(const LineArgs a, double* r_guess, const double* r_plain, double* flag) {
   const long line = blockIdx.x;
   const int t = threadIdx.x;
-  const bool fb = !(r_guess[line] <= r_plain[line]);
+  const double rg = r_guess[line], rp = r_plain[line];
+  const bool fb = !(rg <= rp);
   if (fb) {
     const uint32_t n = line_node(a, line, t);
 #pragma unroll
     for (int v = 0; v < 5; ++v) stg(a.pv[v], n, ldg(a.q0[v], n));
   }
-  if (t == 0) flag[line] = fb ? 1.0 : 0.0;
+  __syncthreads();  // every thread has read r_guess[line]
+  if (t == 0) {
+    flag[line] = fb ? 1.0 : 0.0;
+    r_guess[line] = fb ? rp : rg;
+  }
 }
 
 __global__ void sum_kernel(const double* x, long n, double* partial) {
@@ -757,7 +766,7 @@ int host_sum(fedg_ctx* ctx, const double* x, long n, double* partial, double* pa
   return FEDG_OK;
 }
 
-LineArgs line_args(const DevMesh& m, const HeviArgs& a, const double* const Tex[5], const double scale[10], double dt) {
+LineArgs line_args(const DevMesh& m, const HeviArgs& a, const double* const Tex[5], const double scale[15], double dt) {
   LineArgs l{};
   for (int v = 0; v < 5; ++v) {
     l.qs[v] = a.qs[v];
@@ -765,8 +774,9 @@ LineArgs line_args(const DevMesh& m, const HeviArgs& a, const double* const Tex[
     l.Tex[v] = Tex ? Tex[v] : nullptr;
     l.pv[v] = a.pv[v];
     l.inv_s[v] = 1.0 / scale[v];
-    l.d[v] = scale[5 + v];
-    l.inv_d[v] = 1.0 / scale[5 + v];
+    l.inv_t[v] = 1.0 / scale[5 + v];
+    l.d[v] = scale[10 + v];
+    l.inv_d[v] = 1.0 / scale[10 + v];
   }
   l.impl_fac = a.impl_fac;
   l.dt = dt;
@@ -809,6 +819,23 @@ int run_newton(fedg_ctx* ctx, HeviArgs& a, int max_it, double rtol, double* resi
   // r0_ext > 0: the residual norm at q^n, the reference of the tolerance when
   // the iteration starts from a predicted iterate (which may already meet it)
   double r0 = r0_ext;
+  // rounding floor of the residual: G = impl_fac T - q + q* cancels terms of
+  // the size of the state, so ||G|| cannot drop much below eps ||q*||; the
+  // iteration also stops at 4 eps ||q*||, and once an update has not halved
+  // the residual (it has reached its floor)
+  double atol = 0.0;
+  if (rtol > 0.0) {
+    double q2 = 0.0;
+    for (int v = 0; v < 5; ++v) {
+      hipLaunchKernelGGL(sumsq_kernel, dim3(256), dim3(256), 0, ctx->stream, a.qs[v], n, partial);
+      FEDG_HIP_CHECK(hipGetLastError());
+      FEDG_HIP_CHECK(hipMemcpyAsync(partial_h, partial, 256 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+      for (int b = 0; b < 256; ++b) q2 += partial_h[b];
+    }
+    atol = 4.0 * DBL_EPSILON * sqrt(q2);
+  }
+  double r_prev = 0.0;
   int it = 0;
   for (; it < max_it; ++it) {
     hipLaunchKernelGGL(hevi_rhs_uv_kernel<N1>, dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
@@ -835,7 +862,9 @@ int run_newton(fedg_ctx* ctx, HeviArgs& a, int max_it, double rtol, double* resi
       const double r = sqrt(r2);
       if (resid) resid[it] = r;
       if (it == 0 && r0_ext <= 0.0) r0 = r;
-      if (rtol > 0.0 && (it > 0 || r0_ext > 0.0) && r <= rtol * r0) break;
+      if (rtol > 0.0 && (it > 0 || r0_ext > 0.0) && (r <= rtol * r0 || r <= atol)) break;
+      if (rtol > 0.0 && it > 0 && r > 0.5 * r_prev) break;
+      r_prev = r;
     }
     hipLaunchKernelGGL(hevi_solve3_kernel<N1>, dim3(nb3), dim3(64), l3, ctx->stream, a);
     FEDG_HIP_CHECK(hipGetLastError());
@@ -918,11 +947,15 @@ int predict_iterate(fedg_ctx* ctx, HeviWork* w, HeviArgs& a, HeviGuess* g, doubl
   FEDG_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(line_select_kernel, dim3(nlines), dim3(L), 0, ctx->stream, la, w->r_guess, w->r_plain, w->flag);
   FEDG_HIP_CHECK(hipGetLastError());
-  double nfb = 0.0;
+  double nfb = 0.0, r2s = 0.0;
   const int rcs2 = host_sum(ctx, w->flag, nlines, w->partial, w->partial_h, &nfb);
   if (rcs2 != FEDG_OK) return rcs2;
+  const int rcs3 = host_sum(ctx, w->r_guess, nlines, w->partial, w->partial_h, &r2s);
+  if (rcs3 != FEDG_OK) return rcs3;
   g->lines += nlines;
   g->fallbacks += (long)(nfb + 0.5);
+  g->r2_plain += r2;
+  g->r2_start += r2s;
   return FEDG_OK;
 }
 
@@ -1003,7 +1036,7 @@ int hevi_cal_vi(fedg_ctx* ctx, HeviWork* w, double* const tend[5], const double*
 }
 
 int hevi_harvest(fedg_ctx* ctx, HeviWork* w, const double* const q[5], const double* const var0[5],
-                 const double* const Tex[5], const double scale[10], double impl_fac, double dt, long line0,
+                 const double* const Tex[5], const double scale[15], double impl_fac, double dt, long line0,
                  long stride, int nrows, float* X, float* Y) {
   const DevMesh& m = ctx->m;
   HeviArgs a{};

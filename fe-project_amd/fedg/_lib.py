@@ -103,6 +103,7 @@ def _load():
         "fedg_mlp_adam_step": (C.c_int, [vp, C.c_float, C.c_float, C.c_float, C.c_float]),
         "fedg_hevi_set_guess": (C.c_int, [vp, vp, C.POINTER(C.c_double)]),
         "fedg_hevi_guess_stats": (C.c_int, [vp, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
+        "fedg_hevi_guess_residuals": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "fedg_hevi_harvest": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.c_uint]),
         "fedg_hevi_harvest_count": (C.c_int, [vp, C.POINTER(C.c_int)]),
         "fedg_hevi_create": (C.c_int, [vp, C.c_char_p, C.c_double] + [vp] * 9 + [C.c_int, C.c_double,
@@ -153,7 +154,7 @@ EXPORTED = (
     "fedg_hevi_set_modalfilter "
     "fedg_hevi_cal_vi_guess fedg_mlp_create fedg_mlp_destroy fedg_mlp_param_count fedg_mlp_get_params fedg_mlp_set_params "
     "fedg_mlp_forward fedg_mlp_loss_grad fedg_mlp_adam_step fedg_hevi_set_guess fedg_hevi_guess_stats "
-    "fedg_hevi_harvest fedg_hevi_harvest_count "
+    "fedg_hevi_harvest fedg_hevi_harvest_count fedg_hevi_guess_residuals "
     "fedg_heve_set_precision fedg_hevi_create fedg_hevi_destroy fedg_hevi_set_state fedg_hevi_get_state fedg_hevi_run "
     "fedg_hevi_run_timed fedg_hevi_newton_stats fedg_calc_phyd_hgrad fedg_tint_create fedg_tint_destroy "
     "fedg_tint_info fedg_tint_tend_ex fedg_tint_tend_im fedg_tint_store_var0 fedg_tint_advance "

@@ -110,9 +110,9 @@ def hevi_cal_vi_guess(dm: DeviceMesh, out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DENS_
     Returns (iterations, residual norms, lines that fell back to q^n)."""
     args = [_p(a) for a in (*out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DENS_hyd, PRES_hyd, DDENS0, MOMX0, MOMY0, MOMZ0,
                             DRHOT0, Rtot, CVtot, CPtot)]
-    if len(scale) != 10:
-        raise ValueError("scale takes 10 values: 5 state scales, then 5 increment scales")
-    sc = (C.c_double * 10)(*[float(x) for x in scale])
+    if len(scale) != 15:
+        raise ValueError("scale takes 15 values: state, tendency-feature and increment scales")
+    sc = (C.c_double * 15)(*[float(x) for x in scale])
     tex = None if Tex is None else (C.c_void_p * 5)(*[_p(a) for a in Tex])
     resid = (C.c_double * max(int(max_it), 1))()
     it, fb = C.c_int(0), C.c_long(0)
@@ -326,16 +326,22 @@ class HeviStepper:
         check(LIB.fedg_hevi_set_modalfilter(self.handle, int(bool(on))), "fedg_hevi_set_modalfilter")
 
     def set_guess(self, mlp, scale):
-        """Initial Newton iterate from mlp (None: q^n); scale: 5 state then 5 increment scales (VID order)."""
-        if len(scale) != 10:
-            raise ValueError("scale takes 10 values: 5 state scales, then 5 increment scales")
-        sc = (C.c_double * 10)(*[float(x) for x in scale])
+        """Initial Newton iterate from mlp (None: q^n); scale: 5 state, 5 tendency-feature, 5 increment scales."""
+        if len(scale) != 15:
+            raise ValueError("scale takes 15 values: state, tendency-feature and increment scales")
+        sc = (C.c_double * 15)(*[float(x) for x in scale])
         check(LIB.fedg_hevi_set_guess(self.handle, None if mlp is None else mlp.handle, sc), "fedg_hevi_set_guess")
 
     def guess_stats(self):
         lines, fb = C.c_long(), C.c_long()
         check(LIB.fedg_hevi_guess_stats(self.handle, C.byref(lines), C.byref(fb)), "fedg_hevi_guess_stats")
         return lines.value, fb.value
+
+    def guess_residuals(self):
+        """(||G(q^n)||, ||G(start)||), root of the sums of squares over the solves so far."""
+        a, b = C.c_double(), C.c_double()
+        check(LIB.fedg_hevi_guess_residuals(self.handle, C.byref(a), C.byref(b)), "fedg_hevi_guess_residuals")
+        return float(np.sqrt(a.value)), float(np.sqrt(b.value))
 
     def harvest(self, X: DevField, Y: DevField, capacity: int, per_stage: int, seed: int = 1):
         """Collect (features, converged targets) rows into the fp32 device buffers X, Y during later steps."""

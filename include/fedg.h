@@ -237,7 +237,9 @@ int fedg_hevi_cal_vi(fedg_ctx *ctx, double *DENS_dt, double *MOMX_dt, double *MO
 
 /* No counterpart in the reference (which stops after one iteration,
  * ..._rhot_hevi.F90:885): fedg_hevi_cal_vi iterated until the residual norm
- * ||G_k|| <= rtol ||G_0|| or max_it iterations; resid (host, max_it) gets the
+ * ||G_k|| <= rtol ||G_0||, or at the rounding floor of G = impl_fac T - q +
+ * q* (||G_k|| <= 4 eps ||q*||, q* the stage state, or an update that has
+ * not halved ||G||), or after max_it iterations; resid (host, max_it) gets the
  * norms, *iterations the number of Newton updates applied. */
 int fedg_hevi_cal_vi_newton(fedg_ctx *ctx, double *DENS_dt, double *MOMX_dt, double *MOMY_dt,
                             double *MOMZ_dt, double *RHOT_dt, const double *DDENS_, const double *MOMX_,
@@ -352,11 +354,12 @@ int fedg_hevi_set_modalfilter(fedg_hevi *st, int on);
  * network n_in -> n_hidden (x n_hidden_layers) -> n_out, BF16 products on
  * the matrix cores with FP32 accumulation, FP32 master weights, Adam.  For a
  * vertical node line of L = NeZ (p+1) nodes: n_in = 10 L + 1 (the line's
- * stage state q* / s per variable, then impl_fac T_ex / d per variable, then
+ * stage state q* / s per variable, then impl_fac T_ex / t per variable, then
  * impl_fac/dt; T_ex is the previous stage's explicit tendency, zero at the
  * first stage), n_out = 5 L ((q - q*) / d of the converged iterate q).
  * Variables in VID order, line node t = kz (p+1) + k (k the node level in
- * element kz); s, d: per-variable state and increment scales.
+ * element kz); s, t, d: per-variable scales of the state, the tendency
+ * feature and the increment.
  * Device arrays are fp32, row-major (batch x features). */
 typedef struct fedg_mlp fedg_mlp;
 int fedg_mlp_create(int n_in, int n_hidden, int n_hidden_layers, int n_out, unsigned seed, fedg_mlp **out);
@@ -371,7 +374,7 @@ int fedg_mlp_loss_grad(fedg_mlp *m, const float *X_dev, const float *T_dev, int 
 int fedg_mlp_adam_step(fedg_mlp *m, float lr, float beta1, float beta2, float eps);
 /* Use mlp (or none: NULL) for the initial iterate of every implicit stage of
  * the HEVI stepper; lines whose predicted iterate has a larger residual than
- * q^n start from q^n.  scale: 10 values, s then d (VID order each).  The
+ * q^n start from q^n.  scale: 15 values, s, t, then d (VID order each).  The
  * tolerance stays relative to the residual norm at q^n. */
 int fedg_hevi_set_guess(fedg_hevi *st, fedg_mlp *mlp, const double *scale);
 /* fedg_hevi_cal_vi_newton started from the network's prediction for a host
@@ -387,6 +390,9 @@ int fedg_hevi_cal_vi_guess(fedg_ctx *ctx, double *DENS_dt, double *MOMX_dt, doub
                            double dt, int max_it, double rtol, fedg_mlp *mlp, const double *scale,
                            const double *const *Tex, double *resid, int *iterations, long *fallbacks);
 int fedg_hevi_guess_stats(const fedg_hevi *st, long *lines, long *fallbacks);
+/* ||G(q^n)||^2 and ||G||^2 at the iterates the lines started from (the
+ * prediction or, where it was worse, q^n), each summed over the solves */
+int fedg_hevi_guess_residuals(const fedg_hevi *st, double *r2_plain, double *r2_start);
 /* Collect training pairs from the following steps: per implicit stage,
  * per_stage lines' (features, converged targets) appended to X_dev / Y_dev
  * until capacity rows are stored. */

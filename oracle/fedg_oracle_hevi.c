@@ -27,6 +27,7 @@
  * its last bits are not reproduced (parity unpinned at the ulp level for that
  * one coefficient; the tests pin the assembled matrix against the Jacobian).
  */
+#include <float.h>
 #include <math.h>
 #include <omp.h>
 #include <stdlib.h>
@@ -778,7 +779,8 @@ static void hevi_ctx_free(hevi_ctx *h) {
 /* Newton iterations of cal_vi (..._rhot_hevi.F90:858-917) on the iterate
  * pv (VID order, (Np,Ne), initialised by the caller).  The reference runs
  * exactly one (max_it = 1, rtol <= 0).  With rtol > 0 the loop stops once
- * ||G||_2 / ||G_0||_2 <= rtol, G = (b_uv, b) being the residual that eval_Ax_uv
+ * ||G||_2 / ||G_0||_2 <= rtol, ||G||_2 <= 4 eps ||q*||_2 or an update has
+ * not halved ||G|| (the rounding floor), G = (b_uv, b) being the residual that eval_Ax_uv
  * and eval_Ax form at the start of an iteration; resid[k] receives ||G_k||.
  * Returns the number of iterations done. */
 int oracle_hevi_newton(const oracle_mesh *m, double *const pv[5], const double *const qs[5],
@@ -803,6 +805,16 @@ int oracle_hevi_newton_r0(const oracle_mesh *m, double *const pv[5], const doubl
   const long n = (long)m->Np * m->Ne;
   double *T = (double *)malloc(sizeof(double) * n * 5);
   double r0 = r0_ext;
+  /* rounding floor: stop also at ||G|| <= 4 eps ||q*||, and once an update
+   * has not halved the residual */
+  double atol = 0.0;
+  if (rtol > 0.0) {
+    double q2 = 0.0;
+    for (int v = 0; v < 5; ++v)
+      for (long i = 0; i < n; ++i) q2 += qs[v][i] * qs[v][i];
+    atol = 4.0 * DBL_EPSILON * sqrt(q2);
+  }
+  double r_prev = 0.0;
   int it = 0;
   for (; it < max_it; ++it) {
     hevi_eval_Ax_uv(&h, T + 3 * n, T + 4 * n, 1);
@@ -815,7 +827,9 @@ int oracle_hevi_newton_r0(const oracle_mesh *m, double *const pv[5], const doubl
     const double r = sqrt(r2);
     if (resid) resid[it] = r;
     if (it == 0 && r0_ext <= 0.0) r0 = r;
-    if (rtol > 0.0 && (it > 0 || r0_ext > 0.0) && r <= rtol * r0) break;
+    if (rtol > 0.0 && (it > 0 || r0_ext > 0.0) && (r <= rtol * r0 || r <= atol)) break;
+    if (rtol > 0.0 && it > 0 && r > 0.5 * r_prev) break;
+    r_prev = r;
     hevi_solve_var3(&h);
   }
   free(T);

@@ -135,8 +135,8 @@ def features(mesh, qs, Tex, scale, impl_fac, dt, lines, pv=None):
     """Network inputs (and targets when pv is given) of the given lines.
 
     qs, Tex, pv: 5 fields (VID order); Tex may be None (first stage).
-    scale: 5 state scales then 5 increment scales."""
-    s, d = np.asarray(scale[:5]), np.asarray(scale[5:])
+    scale: 5 state, 5 tendency-feature and 5 increment scales."""
+    s, tf, d = np.asarray(scale[:5]), np.asarray(scale[5:10]), np.asarray(scale[10:15])
     idx = np.stack([line_nodes(mesh, l) for l in lines])
     L = idx.shape[1]
     X = np.zeros((len(lines), 10 * L + 1))
@@ -144,7 +144,7 @@ def features(mesh, qs, Tex, scale, impl_fac, dt, lines, pv=None):
     for v in range(5):
         X[:, v * L:(v + 1) * L] = qs[v][idx] / s[v]
         if Tex is not None:
-            X[:, (5 + v) * L:(6 + v) * L] = impl_fac * Tex[v][idx] / d[v]
+            X[:, (5 + v) * L:(6 + v) * L] = impl_fac * Tex[v][idx] / tf[v]
         if pv is not None:
             Y[:, v * L:(v + 1) * L] = (pv[v][idx] - qs[v][idx]) / d[v]
     X[:, 10 * L] = impl_fac / dt
@@ -178,7 +178,7 @@ def cal_vi_guess(om, lay, params, q, var0, bs, impl_fac, dt, Tex, scale, max_it,
     L = idx.shape[1]
     pv = [a.copy() for a in v0]
     for v in range(5):
-        pv[v][idx] = qv[v][idx] + scale[5 + v] * Y[:, v * L:(v + 1) * L]
+        pv[v][idx] = qv[v][idx] + scale[10 + v] * Y[:, v * L:(v + 1) * L]
     Gg = om.hevi_residual([a.copy() for a in pv], qv, v0, bs, impl_fac, dt)
     r_guess = line_residual2(m, Gg)
     fb = ~(r_guess <= r_plain)

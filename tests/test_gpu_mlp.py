@@ -150,7 +150,7 @@ def dev(a):
     return fedg.DevField(a.size, a)
 
 
-SCALE = [1e-3, 0.3, 1.0, 10.0, 1.0, 1e-4, 1e-2, 0.1, 0.1, 0.1]
+SCALE = [1e-3, 0.3, 1.0, 10.0, 1.0, 1e-5, 1e-3, 0.05, 0.05, 0.05, 1e-4, 1e-2, 0.1, 0.1, 0.1]
 
 
 def run_guess(dm, mesh, bs, q, q0, impl_fac, dt, net, Tex=None, max_it=10, rtol=1e-10):

@@ -101,7 +101,7 @@ def test_features_layout():
     qs = [rng.standard_normal(n) for _ in range(5)]
     Tex = [rng.standard_normal(n) for _ in range(5)]
     pv = [a + 0.1 * rng.standard_normal(n) for a in qs]
-    scale = [1.0, 2.0, 3.0, 4.0, 5.0, 0.5, 0.25, 0.125, 2.0, 4.0]
+    scale = [1.0, 2.0, 3.0, 4.0, 5.0, 0.5, 0.25, 0.125, 2.0, 4.0, 8.0, 0.1, 0.2, 0.3, 0.4]
     lines = [0, 3, 7]
     X, Y = M.features(mesh, qs, Tex, scale, 0.7, 0.35, lines, pv)
     L = 2 * 3
@@ -111,6 +111,6 @@ def test_features_layout():
         for v in range(5):
             assert np.allclose(X[r, v * L:(v + 1) * L], qs[v][nodes] / scale[v], rtol=1e-6)
             assert np.allclose(X[r, (5 + v) * L:(6 + v) * L], 0.7 * Tex[v][nodes] / scale[5 + v], rtol=1e-6)
-            assert np.allclose(Y[r, v * L:(v + 1) * L], (pv[v][nodes] - qs[v][nodes]) / scale[5 + v], rtol=1e-5,
+            assert np.allclose(Y[r, v * L:(v + 1) * L], (pv[v][nodes] - qs[v][nodes]) / scale[10 + v], rtol=1e-5,
                                atol=1e-7)
         assert X[r, -1] == np.float32(2.0)
