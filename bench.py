@@ -17,6 +17,14 @@ WRITE_SIZE (profiles/heve_stage_flat_p7_64x64x16.md), scaled by a copy kernel.
 cpu_baseline: the CPU oracle (C restatement of the reference Fortran, OpenMP,
 FP64, the C library's pow) timed on this host over one RK4 step of the same
 mesh.
+
+forms: the same step with the FP32 perturbation-form tendency (FP64 state and
+RK), and the HEVI step of the 240x16x8 p=7 inertia-gravity-wave channel
+(IMEX_ARK324, dt = 0.05 s) with the reference's single Newton iteration and
+with Newton to a relative residual of 1e-10.
+
+--workload selects what is timed alone (for profiling): heve (default),
+heve-fp32 or hevi (the HEVI step with the reference's single iteration).
 """
 import argparse
 import json
@@ -61,12 +69,54 @@ def cpu_baseline(mesh, bs, q, dt, threads):
                       f"{sec:.2f} s"}
 
 
+def time_heve(fedg, cases, bits, steps, warmup):
+    case = cases.config2()
+    mesh, bs, q = cases.build(case)
+    dm = fedg.DeviceMesh(mesh)
+    Dx, Dy = fedg.calc_phyd_hgrad(dm, bs.PRES_hyd, bs.PRES_hyd_ref)
+    st = fedg.HeveStepper(dm, bs, Dx, Dy, case.dt, scheme=case.scheme)
+    st.set_precision(bits)
+    st.set_state(q)
+    st.run(warmup)
+    fedg.check(fedg.LIB.fedg_device_sync(), "sync")
+    t0 = time.perf_counter()
+    ms_events, ms_kernel = st.run_timed(steps)
+    fedg.check(fedg.LIB.fedg_device_sync(), "sync")
+    wall = time.perf_counter() - t0
+    flat = st.flat
+    st.close()
+    return dict(case=case, mesh=mesh, bs=bs, q=q, dm=dm, wall=wall, ms_kernel=ms_kernel, flat=flat)
+
+
+def time_hevi(fedg, cases, steps, warmup, max_it, rtol, case=None, dm=None):
+    case = case or cases.config4()
+    mesh, bs, q = cases.build(case)
+    dm = dm or fedg.DeviceMesh(mesh)
+    Dx, Dy = fedg.calc_phyd_hgrad(dm, bs.PRES_hyd, bs.PRES_hyd_ref)
+    st = fedg.HeviStepper(dm, bs, Dx, Dy, case.dt, scheme=case.scheme, max_it=max_it, rtol=rtol)
+    st.set_state(q)
+    st.run(warmup)
+    it0, calls0 = st.newton_stats()
+    fedg.check(fedg.LIB.fedg_device_sync(), "sync")
+    t0 = time.perf_counter()
+    tot, im, ex = st.run_timed(steps)
+    wall = time.perf_counter() - t0
+    it, calls = st.newton_stats()
+    st.close()
+    dof = 5 * mesh.Np * mesh.Ne
+    return {"ms_per_step": wall * 1e3 / steps, "value": dof * 4 * steps / wall,
+            "ms_implicit_per_stage": im, "ms_explicit_per_stage": ex,
+            "newton_iterations_per_solve": (it - it0) / max(calls - calls0, 1), "max_it": max_it, "rtol": rtol}, dm
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=("heve", "heve-fp32", "hevi"), default="heve")
+    ap.add_argument("--no-forms", action="store_true", help="skip the FP32 and HEVI lines of the default run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed steps, run one device copy of Np*Ne doubles (copy_kernel), the "
                          "known-size reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -79,18 +129,15 @@ def main():
     import fedg
     from fedg import cases
 
-    case = cases.config2()
-    mesh, bs, q = cases.build(case)
-    dm = fedg.DeviceMesh(mesh)
-    Dx, Dy = fedg.calc_phyd_hgrad(dm, bs.PRES_hyd, bs.PRES_hyd_ref)
-    st = fedg.HeveStepper(dm, bs, Dx, Dy, case.dt, scheme=case.scheme)
-    st.set_state(q)
-    st.run(args.warmup)
-    fedg.check(fedg.LIB.fedg_device_sync(), "sync")
-    t0 = time.perf_counter()
-    ms_events, ms_kernel = st.run_timed(args.steps)
-    fedg.check(fedg.LIB.fedg_device_sync(), "sync")
-    wall = time.perf_counter() - t0
+    if args.workload == "hevi":
+        r, _ = time_hevi(fedg, cases, args.steps, args.warmup, 1, 0.0)
+        print(json.dumps({"metric": "DOF·stage/s", "workload": "HEVI inertia-gravity wave 240x16x8 p=7, "
+                          "IMEX_ARK324 dt=0.05 s, one Newton iteration", **r}))
+        return
+    bits = 32 if args.workload == "heve-fp32" else 64
+    h = time_heve(fedg, cases, bits, args.steps, args.warmup)
+    case, mesh, bs, q, dm = h["case"], h["mesh"], h["bs"], h["q"], h["dm"]
+    wall, ms_kernel = h["wall"], h["ms_kernel"]
 
     dof = 5 * mesh.Np * mesh.Ne
     nstage = 4
@@ -107,11 +154,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if bits == 64 else "f32 tendency, f64 state",
         "data": "synthetic",
-        "config": {"workload": "plain FP64 HEVE warm bubbles, 64x64x16 elements, p=7, RK4 dt=0.04 s",
+        "config": {"workload": ("plain FP64" if bits == 64 else "FP32 perturbation-form") +
+                   " HEVE warm bubbles, 64x64x16 elements, p=7, RK4 dt=0.04 s",
                    "elements": mesh.Ne, "nodes": mesh.Np * mesh.Ne, "dof": dof, "kernel": "fused stage, "
-                   + ("flat-terrain" if st.flat else "general-metric")},
+                   + ("flat-terrain" if h["flat"] else "general-metric")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic()},
     }
@@ -122,6 +170,17 @@ def main():
         out["copy_probe_bytes"] = {"read": 8 * mesh.Np * mesh.Ne, "write": 8 * mesh.Np * mesh.Ne}
         src.free()
         t.close()
+    if args.workload == "heve" and not args.no_forms:
+        f32 = time_heve(fedg, cases, 32, args.steps, args.warmup)
+        f32["dm"].close()
+        hevi1, hdm = time_hevi(fedg, cases, 5, 1, 1, 0.0)
+        hevin, _ = time_hevi(fedg, cases, 5, 1, 4, 1e-10, dm=hdm)
+        hdm.close()
+        out["forms"] = {
+            "heve_fp32": {"ms_per_step": f32["wall"] * 1e3 / args.steps, "value": dof * 4 * args.steps / f32["wall"],
+                          "ms_stage_kernel": f32["ms_kernel"]},
+            "hevi_240x16x8_p7": {"one_iteration": hevi1, "newton_1e-10": hevin},
+        }
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", str(min(os.cpu_count() or 1, 16))))
         out["cpu_baseline"] = cpu_baseline(mesh, bs, q, case.dt, threads)

@@ -131,13 +131,33 @@ __global__ void relu_back_kernel(float* __restrict__ dH, const float* __restrict
   if (t < n && !(H[t] > 0.f)) dH[t] = 0.f;
 }
 
-// column sums of dZ [B x N] -> db[N]
-__global__ void colsum_kernel(const float* __restrict__ dZ, float* __restrict__ db, int Bn, int N) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= N) return;
+// column sums of dZ [B x N] -> db[N]: 64 columns x 16 row groups per
+// 1024-thread block, the groups' partial sums added in group order
+__global__ void __launch_bounds__(1024) colsum_kernel(const float* __restrict__ dZ, float* __restrict__ db, int Bn,
+                                                      int N) {
+  __shared__ float part[16][64];
+  const int tx = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + tx;
   float s = 0.f;
-  for (int b = 0; b < Bn; ++b) s += dZ[(size_t)b * N + c];
-  db[c] = s;
+  if (c < N) {
+    int b = g;
+    for (; b + 48 < Bn; b += 64) {
+      const float a0 = dZ[(size_t)b * N + c], a1 = dZ[(size_t)(b + 16) * N + c];
+      const float a2 = dZ[(size_t)(b + 32) * N + c], a3 = dZ[(size_t)(b + 48) * N + c];
+      s += a0;
+      s += a1;
+      s += a2;
+      s += a3;
+    }
+    for (; b < Bn; b += 16) s += dZ[(size_t)b * N + c];
+  }
+  part[g][tx] = s;
+  __syncthreads();
+  if (g == 0 && c < N) {
+    float t = part[0][tx];
+    for (int k = 1; k < 16; ++k) t += part[k][tx];
+    db[c] = t;
+  }
 }
 
 // mean squared error: dY = 2 (Y - T) / (B N), per-block partial sums of (Y-T)^2
@@ -358,7 +378,7 @@ static int mlp_bwd(MlpImpl* p, int B) {
                          (long)B * N);
       MLP_TRY(mlp_hip(hipGetLastError(), "relu_back_kernel"));
     }
-    hipLaunchKernelGGL(colsum_kernel, dim3(nblk(N, 256)), dim3(256), 0, p->stream, dZ, p->grads + p->boff[l], B, N);
+    hipLaunchKernelGGL(colsum_kernel, dim3(nblk(N, 64)), dim3(1024), 0, p->stream, dZ, p->grads + p->boff[l], B, N);
     MLP_TRY(mlp_hip(hipGetLastError(), "colsum_kernel"));
     // dW = dZ^T X : A = dZ^T [N x Bp], B = X^T [K x Bp]
     const int Bp = pad16(B);
