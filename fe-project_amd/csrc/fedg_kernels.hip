@@ -579,33 +579,44 @@ __global__ void __launch_bounds__(256, 2) heve7_flat_kernel(const DevMesh m, con
     }
   };
 
+  // a variable's pointwise operands (RK operands, DPhydDx/Dy and Coriolis,
+  // PRES_hyd), loaded one variable ahead so that their latency overlaps the
+  // previous variable's contractions
+  struct Ops {
+    double rA0, rA1, rB0, rB1, sa, sb, cor, Pha, Phb;
+  };
+  auto load_ops = [&](int v) {
+    Ops o{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (rk.mode == 1) {
+      o.rA0 = ldg(rk.var0[v], o0);
+      o.rA1 = ldg(rk.var0[v], o1);
+    }
+    if (rk.mode == 1 || rk.mode == 2) {
+      o.rB0 = ldg(rk.varTmp[v], o0);
+      o.rB1 = ldg(rk.varTmp[v], o1);
+    }
+    if (v == VU) {
+      o.sa = ldg(in.DPhydDx, o0);
+      o.sb = ldg(in.DPhydDx, o1);
+      o.cor = ldg(in.CORIOLIS, o2d);
+    } else if (v == VV) {
+      o.sa = ldg(in.DPhydDy, o0);
+      o.sb = ldg(in.DPhydDy, o1);
+      o.cor = ldg(in.CORIOLIS, o2d);
+    } else if (v == VT) {
+      o.Pha = ldg(in.PRES_hyd, o0);
+      o.Phb = ldg(in.PRES_hyd, o1);
+    }
+    return o;
+  };
+  Ops nxt = load_ops(0);
   double drhoa = 0.0, drhob = 0.0;
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
-    // this variable's pointwise operands, issued before its contractions so
-    // that their latency overlaps them
-    double rA0 = 0.0, rA1 = 0.0, rB0 = 0.0, rB1 = 0.0;
-    if (rk.mode == 1) {
-      rA0 = ldg(rk.var0[v], o0);
-      rA1 = ldg(rk.var0[v], o1);
-    }
-    if (rk.mode == 1 || rk.mode == 2) {
-      rB0 = ldg(rk.varTmp[v], o0);
-      rB1 = ldg(rk.varTmp[v], o1);
-    }
-    double sa = 0.0, sb = 0.0, cor = 0.0, Pha = 0.0, Phb = 0.0;
-    if (v == VU) {
-      sa = ldg(in.DPhydDx, o0);
-      sb = ldg(in.DPhydDx, o1);
-      cor = ldg(in.CORIOLIS, o2d);
-    } else if (v == VV) {
-      sa = ldg(in.DPhydDy, o0);
-      sb = ldg(in.DPhydDy, o1);
-      cor = ldg(in.CORIOLIS, o2d);
-    } else if (v == VT) {
-      Pha = ldg(in.PRES_hyd, o0);
-      Phb = ldg(in.PRES_hyd, o1);
-    }
+    const Ops cur = nxt;
+    if (v < 4) nxt = load_ops(v + 1);
+    const double rA0 = cur.rA0, rA1 = cur.rA1, rB0 = cur.rB0, rB1 = cur.rB1;
+    const double sa = cur.sa, sb = cur.sb, cor = cur.cor, Pha = cur.Pha, Phb = cur.Phb;
     const int b = v & 1;
     double f0, f1, f2;
     flux(v, qa, RDa, Tha, DPa, f0, f1, f2);
