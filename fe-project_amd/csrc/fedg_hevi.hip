@@ -264,7 +264,8 @@ __global__ void __launch_bounds__(N1* N1* N1) hevi_rhs3_kernel(const HeviArgs a)
 // rank-1 update, forward substitution by columns, back substitution in
 // ascending column order -- are those of the reference's serial loops
 // (FDS/scale_atm_dyn_dgm_hevi_common_linalgebra.F90:2142-2445), so results
-// are bit for bit the CPU restatement's.
+// are bit for bit the CPU restatement's (rows outside the update subtract an
+// exact zero, which can only turn a -0 entry into +0).
 
 // Cross-lane exchanges inside a segment: DPP permutations within 16-lane
 // rows, ds_swizzle between the two rows of a 32-lane segment.
@@ -335,17 +336,21 @@ __device__ __forceinline__ void lu_solve_rows(double (&A)[n], double (&x)[NR], i
     const double dinv = 1.0 / __shfl(A[k], lp, 64);
     const bool below = act && pos > k;
     const double l = A[k] * dinv;
+    // rows that are not below the pivot subtract l0 * p = 0, which leaves
+    // every finite entry unchanged (up to the sign of a zero entry) and saves
+    // the per-entry select
+    const double l0 = below ? l : 0.0;
     // the pivot row's entries one at a time (the pivot lane's row is not
     // changed in this step)
 #pragma unroll
     for (int j = k + 1; j < n; ++j) {
       const double pj = __shfl(A[j], lp, 64);
-      if (below) A[j] = A[j] - l * pj;
+      A[j] = A[j] - l0 * pj;
     }
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
       const double pq = __shfl(x[q], lp, 64);
-      if (below && q < nr) x[q] = x[q] - pq * l;
+      if (q < nr) x[q] = x[q] - pq * l0;
     }
     if (lane == lp) A[k] = dinv;
     else if (below) A[k] = l;
