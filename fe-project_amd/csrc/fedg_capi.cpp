@@ -88,6 +88,7 @@ struct fedg_heve {
   double* pres_scratch;     // (Np,Ne) x2 for PRES/DPRES
   int fp32;                 // stage tendency in the FP32 perturbation form
   int mfilter;              // modal filter after every step
+  double* lvl;              // per-level base-state tables (horizontally homogeneous base state) or null
 };
 
 namespace {
@@ -1030,6 +1031,30 @@ int fedg_heve_create(fedg_ctx* ctx, const char* scheme, double dt, const double*
       return FEDG_ERR_HIP;
     }
   }
+  // horizontally homogeneous base state (every element of a level holds the
+  // same values, bit for bit): the flat p = 7 stage reads it from per-level
+  // tables, which stay in L2, instead of five full fields
+  if (st->flat && m.n1 == 8 && m.NeX > 0 && m.NeY > 0 && m.NeZ > 0 && m.Ne == m.NeX * m.NeY * m.NeZ) {
+    const double* f[5] = {DENS_hyd, PRES_hyd, THERM_hyd, DPhydDx, DPhydDy};
+    const size_t Np = (size_t)m.Np, per = (size_t)m.NeX * m.NeY;
+    bool hom = true;
+    for (int a = 0; a < 5 && hom; ++a)
+      for (size_t ke = 0; ke < (size_t)m.Ne && hom; ++ke) {
+        const size_t rep = (ke / per) * per;
+        hom = memcmp(f[a] + ke * Np, f[a] + rep * Np, Np * sizeof(double)) == 0;
+      }
+    if (hom) {
+      std::vector<double> tab((size_t)5 * m.NeZ * Np);
+      for (int a = 0; a < 5; ++a)
+        for (int kz = 0; kz < m.NeZ; ++kz)
+          memcpy(&tab[((size_t)a * m.NeZ + kz) * Np], f[a] + (size_t)kz * per * Np, Np * sizeof(double));
+      rc = dev_upload<double>(&st->lvl, tab.data(), tab.size());
+      if (rc != FEDG_OK) {
+        fedg_heve_destroy(st);
+        return rc;
+      }
+    }
+  }
   st->q0 = 0;
   *out = st;
   return FEDG_OK;
@@ -1038,6 +1063,7 @@ int fedg_heve_create(fedg_ctx* ctx, const char* scheme, double dt, const double*
 int fedg_heve_destroy(fedg_heve* st) {
   if (!st) return FEDG_OK;
   if (st->block) (void)hipFree(st->block);
+  if (st->lvl) (void)hipFree(st->lvl);
   delete st;
   return FEDG_OK;
 }
@@ -1051,6 +1077,11 @@ int fedg_heve_set_precision(fedg_heve* st, int bits) {
     return FEDG_ERR_UNSUPPORTED;
   }
   st->fp32 = bits == 32;
+  // the pressure perturbation of q^n as the FP64 stage forms it (the FP32
+  // stages below p = 7 do not keep it)
+  const DevMesh& m = st->ctx->m;
+  TRY(launch_dpres(st->ctx, st->DP[st->q0], st->buf[st->q0][1], st->PRES_hyd, st->THERM_hyd, st->Rtot, st->CVtot,
+                   st->CPtot, (long)m.Np * m.Ne));
   return FEDG_OK;
 }
 
@@ -1106,6 +1137,7 @@ static int heve_step(fedg_heve* st, std::vector<hipEvent_t>* evs, size_t* ev_i) 
   in.Rtot = st->Rtot;
   in.CVtot = st->CVtot;
   in.CPtot = st->CPtot;
+  in.lvl = st->lvl;
   const int q0 = st->q0;
   int cur = q0;
   int last_out = q0;

@@ -94,6 +94,10 @@ struct TendIn {
   const double* Rtot;        // fused: per node or null (dry constants)
   const double* CVtot;
   const double* CPtot;
+  // fused p = 7 flat stage: DENS_hyd, PRES_hyd, THERM_hyd, DPhydDx, DPhydDy as
+  // (5, NeZ, Np) tables by element level when every element of a level holds
+  // the same values (horizontally homogeneous base state), else null
+  const double* lvl;
 };
 
 struct RkOut {
@@ -108,7 +112,9 @@ struct RkOut {
 int launch_heve_dropin(fedg_ctx* ctx, const TendIn& in, double* const out[5]);
 int launch_hevi_explicit(fedg_ctx* ctx, const TendIn& in, double* const out[5]);
 int launch_heve_fused(fedg_ctx* ctx, const TendIn& in, const RkOut& rk, int use_flat);
-// FP32 perturbation form of the fused stage (flat box, dry air; RkOut.DPout unused)
+// FP32 perturbation form of the fused stage (flat box, dry air; at p = 7 the
+// stage reads DPRES and writes RkOut.DPout like the FP64 stage, otherwise it
+// forms DPRES itself and leaves DPout)
 int launch_heve_fused32(fedg_ctx* ctx, const TendIn& in, const RkOut& rk);
 // DPRES of a state (dry constants when Rtot is null), as DRHOT2PRES computes it
 int launch_dpres(fedg_ctx* ctx, double* DPRES, const double* DRHOT, const double* PRES_hyd,

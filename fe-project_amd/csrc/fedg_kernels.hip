@@ -20,6 +20,8 @@
 // rounded), and the pressure's pow is the portable double-double routine of
 // fedg_pow.h, so results agree bit for bit with a CPU evaluation of the same
 // expressions that uses the same pow.
+#include <type_traits>
+
 #include "fedg_device.h"
 
 namespace fedg {
@@ -383,370 +385,6 @@ heve_kernel(const DevMesh m, const Consts c, const TendIn in, const RkOut rk) {
 }
 
 // ---------------------------------------------------------------------------
-// p = 7 fused stage kernel on the flat-terrain uniform box: one element per
-// 256-thread workgroup, thread (i,j,kk) owning the two nodes (i,j,kk) and
-// (i,j,kk+4), so that wavefront kk holds two horizontal node planes.
-//  - each thread's D row i, D row j and the Lift entries along x and y stay
-//    lane-varying; the z rows D(kk,:), D(kk+4,:), the VFilterPM1 columns and
-//    the vertical Lift entries are wavefront-uniform and come from scalar loads;
-//  - face work: wavefront w evaluates lateral face w (y-, x+, y+, x-), one face
-//    node per lane, so its neighbour logic is uniform; wavefronts 0 and 3 also
-//    evaluate the bottom / top face, whose interior node is their own;
-//  - every global load of the stage (own state, both sides of the face nodes,
-//    RK operands) is issued before the first barrier, and the face fluxes need
-//    no barrier (their interior values are loaded, not exchanged);
-//  - 44 KB of LDS and at most 256 VGPRs without spills: two elements are
-//    resident per CU (a tighter register bound spills and runs 2x slower).
-// Arithmetic is identical, operation for operation, to heve_kernel.
-__global__ void __launch_bounds__(256, 2) heve7_flat_kernel(const DevMesh m, const Consts c, const TendIn in,
-                                                            const RkOut rk) {
-  constexpr int N1 = 8, Np = 512, Nfp = 64, NfpTot = 384;
-  __shared__ double sDt[N1 * N1];
-  __shared__ double sI[N1 * N1];
-  __shared__ double sF[2][3][Np];
-  __shared__ double sDF[5][NfpTot];
-  __shared__ double sRho[Np];
-
-  const int t = threadIdx.x;
-  const int i = t & 7, j = (t >> 3) & 7;
-  const int kk = __builtin_amdgcn_readfirstlane(t >> 6);  // wavefront index
-  const int ke = xcd_remap(blockIdx.x, gridDim.x);
-  const int p0 = i + N1 * j + Nfp * kk, p1 = p0 + 4 * Nfp;
-  const uint32_t base = (uint32_t)Np * ke;
-  const uint32_t o0 = base + p0, o1 = base + p1;
-  const double gamm = c.CPdry / c.CVdry;
-  if (t < N1 * N1) {
-    sDt[(t % N1) * N1 + t / N1] = m.D1D[t];
-    sI[t] = m.Itr[t];
-  }
-
-  // ---- batch 1: own state of both nodes ----
-  double qa[5], qb[5];
-#pragma unroll
-  for (int v = 0; v < 5; ++v) {
-    qa[v] = ldg(in.q[v], o0);
-    qb[v] = ldg(in.q[v], o1);
-  }
-  const double Dha = ldg(in.DENS_hyd, o0), Dhb = ldg(in.DENS_hyd, o1);
-  const double Tha = ldg(in.THERM_hyd, o0), Thb = ldg(in.THERM_hyd, o1);
-  const double DPa = ldg(in.DPRES, o0), DPb = ldg(in.DPRES, o1);
-
-  // ---- face nodes of this thread: lateral face kk (node lane = t & 63) and,
-  // for wavefronts 0 and 3, the bottom / top face node (i,j) ----
-  const int ei = ke % m.NeX, ej = (ke / m.NeX) % m.NeY, ek = ke / (m.NeX * m.NeY);
-  const int lane = t & 63;
-  const int fL = kk;  // 0: y-, 1: x+, 2: y+, 3: x-
-  const int pmL = face_node<N1>(fL, lane);
-  uint32_t iPL;
-  int bcL = FEDG_BND_NONE;
-  {
-    int ni = ei, nj = ej;
-    bool closed = false;
-    switch (fL) {
-      case 0: nj -= 1; if (nj < 0) { if (m.periodic[1]) nj = m.NeY - 1; else closed = true; } break;
-      case 1: ni += 1; if (ni >= m.NeX) { if (m.periodic[0]) ni = 0; else closed = true; } break;
-      case 2: nj += 1; if (nj >= m.NeY) { if (m.periodic[1]) nj = 0; else closed = true; } break;
-      default: ni -= 1; if (ni < 0) { if (m.periodic[0]) ni = m.NeX - 1; else closed = true; } break;
-    }
-    constexpr int opp[4] = {2, 3, 0, 1};
-    if (closed) {
-      iPL = base + pmL;
-      bcL = m.bnd_bc[fL];
-    } else {
-      iPL = (uint32_t)Np * (ni + m.NeX * (nj + m.NeY * ek)) + face_node<N1>(opp[fL], lane);
-    }
-  }
-  const uint32_t iML = base + pmL;
-  double ML[9], PL[9];  // DD, DT, MZ, MX, MY, Dhyd, Phyd, Thyd, DP
-#pragma unroll
-  for (int v = 0; v < 5; ++v) {
-    ML[v] = ldg(in.q[v], iML);
-    PL[v] = ldg(in.q[v], iPL);
-  }
-  ML[5] = ldg(in.DENS_hyd, iML); ML[6] = ldg(in.PRES_hyd, iML); ML[7] = ldg(in.THERM_hyd, iML);
-  ML[8] = ldg(in.DPRES, iML);
-  PL[5] = ldg(in.DENS_hyd, iPL); PL[6] = ldg(in.PRES_hyd, iPL); PL[7] = ldg(in.THERM_hyd, iPL);
-  PL[8] = ldg(in.DPRES, iPL);
-  // vertical face (wavefronts 0 and 3): interior node is this thread's node
-  const bool hasV = (kk == 0) || (kk == 3);
-  const int fV = (kk == 0) ? 4 : 5;
-  double PV[9];
-  double PhV = 0.0;  // PRES_hyd of the interior node of the vertical face
-  int bcV = FEDG_BND_NONE;
-  if (hasV) {
-    int nk = ek + (fV == 4 ? -1 : 1);
-    bool closed = false;
-    if (nk < 0 || nk >= m.NeZ) {
-      if (m.periodic[2]) nk = (nk + m.NeZ) % m.NeZ;
-      else closed = true;
-    }
-    const uint32_t own = (fV == 4) ? o0 : o1;
-    uint32_t iPV;
-    if (closed) {
-      iPV = own;
-      bcV = m.bnd_bc[fV];
-    } else {
-      iPV = (uint32_t)Np * (ei + m.NeX * (ej + m.NeY * nk)) + (uint32_t)(i + N1 * j + Nfp * (fV == 4 ? 7 : 0));
-    }
-#pragma unroll
-    for (int v = 0; v < 5; ++v) PV[v] = ldg(in.q[v], iPV);
-    PV[5] = ldg(in.DENS_hyd, iPV); PV[6] = ldg(in.PRES_hyd, iPV); PV[7] = ldg(in.THERM_hyd, iPV);
-    PV[8] = ldg(in.DPRES, iPV);
-    PhV = ldg(in.PRES_hyd, own);
-  }
-
-  // VFilterPM1 input
-  sRho[p0] = qa[VD];
-  sRho[p1] = qb[VD];
-
-  // ---- face fluxes ----
-  auto face = [&](int f, const double* Mv, double* Pv, int bc, int fp) {
-    Side M, P;
-    const double nx = (f == 1) ? 1.0 : (f == 3) ? -1.0 : 0.0;
-    const double ny = (f == 2) ? 1.0 : (f == 0) ? -1.0 : 0.0;
-    const double nz = (f == 5) ? 1.0 : (f == 4) ? -1.0 : 0.0;
-    M.Gsqrt = 1.0; M.G13 = 0.0; M.G23 = 0.0;
-    P.Gsqrt = 1.0; P.G13 = 0.0; P.G23 = 0.0;
-    M.DD = Mv[0]; M.DT = Mv[1]; M.MZ = Mv[2]; M.MX = Mv[3]; M.MY = Mv[4];
-    M.Dhyd = Mv[5]; M.Phyd = Mv[6]; M.Thyd = Mv[7]; M.DP = Mv[8];
-    P.DD = Pv[0]; P.DT = Pv[1]; P.MZ = Pv[2]; P.MX = Pv[3]; P.MY = Pv[4];
-    P.Dhyd = Pv[5]; P.Phyd = Pv[6]; P.Thyd = Pv[7]; P.DP = Pv[8];
-    if (bc != FEDG_BND_NONE)
-      reflect_momentum(bc, 1.0, 1.0, 1.0, 0.0, 1.0, 0.0, 0.0, nx, ny, nz, M.MX, M.MY, M.MZ, P.MX, P.MY, P.MZ);
-    double df[5];
-    rusanov(M, P, nx, ny, nz, m.fF[f], gamm, df);
-#pragma unroll
-    for (int v = 0; v < 5; ++v) sDF[v][fp] = df[v];
-  };
-  face(fL, ML, PL, bcL, fL * Nfp + lane);
-  __builtin_amdgcn_sched_barrier(0);
-  if (hasV) {
-    double MV[9];
-#pragma unroll
-    for (int v = 0; v < 5; ++v) MV[v] = (fV == 4) ? qa[v] : qb[v];
-    MV[5] = (fV == 4) ? Dha : Dhb;
-    MV[6] = PhV;
-    MV[7] = (fV == 4) ? Tha : Thb;
-    MV[8] = (fV == 4) ? DPa : DPb;
-    face(fV, MV, PV, bcV, fV * Nfp + i + N1 * j);
-  }
-  __builtin_amdgcn_sched_barrier(0);
-
-  const int ke2d = ke % (m.NeX * m.NeY);
-  const uint32_t o2d = (uint32_t)(i + N1 * j) + (uint32_t)Nfp * ke2d;
-
-  // D(kk,:) and D(kk+4,:), wavefront-uniform (scalar loads)
-  double Dka[N1], Dkb[N1];
-#pragma unroll
-  for (int l = 0; l < N1; ++l) {
-    Dka[l] = m.D1D[kk + N1 * l];
-    Dkb[l] = m.D1D[kk + 4 + N1 * l];
-  }
-  // Lift entries: lane-varying along x and y, wavefront-uniform along z
-  const double* LT = m.liftT;  // (N1,6): entry of face f on the line through the node
-  const double Ly0 = LT[j + N1 * 0], Lx1 = LT[i + N1 * 1], Ly2 = LT[j + N1 * 2], Lx3 = LT[i + N1 * 3];
-  const double Lz4a = LT[kk + N1 * 4], Lz5a = LT[kk + N1 * 5];
-  const double Lz4b = LT[kk + 4 + N1 * 4], Lz5b = LT[kk + 4 + N1 * 5];
-
-  const double E11 = m.fE[0], E22 = m.fE[1], E33 = m.fE[2];
-  const double RDa = 1.0 / (qa[VD] + Dha), RDb = 1.0 / (qb[VD] + Dhb);
-  // Gsqrt = GsqrtH = 1: GsqrtV = 1/1, RGsqrtV = 1/GsqrtV, RGsqrt = 1/1
-  const double GsqrtV = 1.0 / 1.0;
-  const double RGsqrtV = 1.0 / GsqrtV;
-  const double RGsqrt = 1.0 / 1.0;
-  const double Gs = 1.0, g13 = 0.0, g23 = 0.0;
-
-  auto flux = [&](int v, const double* q, double RD, double Th, double DP, double& f0, double& f1, double& f2) {
-    const double m0 = Gs * q[VU];
-    const double m1 = Gs * q[VV];
-    const double m2 = Gs * (q[VW] * RGsqrtV + g13 * q[VU] + g23 * q[VV]);
-    if (v == VD) {
-      f0 = m0; f1 = m1; f2 = m2;
-    } else if (v == VT) {
-      const double pt_ = (Th + q[VT]) * RD;
-      f0 = m0 * pt_; f1 = m1 * pt_; f2 = m2 * pt_;
-    } else if (v == VW) {
-      const double w_ = q[VW] * RD;
-      f0 = m0 * w_; f1 = m1 * w_; f2 = m2 * w_ + Gs * DP * RGsqrtV;
-    } else if (v == VU) {
-      const double G = Gs * DP;
-      const double u_ = q[VU] * RD;
-      f0 = m0 * u_ + G; f1 = m1 * u_; f2 = m2 * u_ + G * g13;
-    } else {
-      const double G = Gs * DP;
-      const double v_ = q[VV] * RD;
-      f0 = m0 * v_; f1 = m1 * v_ + G; f2 = m2 * v_ + G * g23;
-    }
-  };
-
-  // a variable's pointwise operands (RK operands, DPhydDx/Dy and Coriolis,
-  // PRES_hyd), loaded one variable ahead so that their latency overlaps the
-  // previous variable's contractions
-  struct Ops {
-    double rA0, rA1, rB0, rB1, sa, sb, cor, Pha, Phb;
-  };
-  auto load_ops = [&](int v) {
-    Ops o{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (rk.mode == 1) {
-      o.rA0 = ldg(rk.var0[v], o0);
-      o.rA1 = ldg(rk.var0[v], o1);
-    }
-    if (rk.mode == 1 || rk.mode == 2) {
-      o.rB0 = ldg(rk.varTmp[v], o0);
-      o.rB1 = ldg(rk.varTmp[v], o1);
-    }
-    if (v == VU) {
-      o.sa = ldg(in.DPhydDx, o0);
-      o.sb = ldg(in.DPhydDx, o1);
-      o.cor = ldg(in.CORIOLIS, o2d);
-    } else if (v == VV) {
-      o.sa = ldg(in.DPhydDy, o0);
-      o.sb = ldg(in.DPhydDy, o1);
-      o.cor = ldg(in.CORIOLIS, o2d);
-    } else if (v == VT) {
-      o.Pha = ldg(in.PRES_hyd, o0);
-      o.Phb = ldg(in.PRES_hyd, o1);
-    }
-    return o;
-  };
-  Ops nxt = load_ops(0);
-  double drhoa = 0.0, drhob = 0.0;
-#pragma unroll
-  for (int v = 0; v < 5; ++v) {
-    const Ops cur = nxt;
-    if (v < 4) nxt = load_ops(v + 1);
-    const double rA0 = cur.rA0, rA1 = cur.rA1, rB0 = cur.rB0, rB1 = cur.rB1;
-    const double sa = cur.sa, sb = cur.sb, cor = cur.cor, Pha = cur.Pha, Phb = cur.Phb;
-    const int b = v & 1;
-    double f0, f1, f2;
-    flux(v, qa, RDa, Tha, DPa, f0, f1, f2);
-    sF[b][0][p0] = f0; sF[b][1][p0] = f1; sF[b][2][p0] = f2;
-    flux(v, qb, RDb, Thb, DPb, f0, f1, f2);
-    sF[b][0][p1] = f0; sF[b][1][p1] = f1; sF[b][2][p1] = f2;
-    __syncthreads();
-    if (v == 0) {
-      // VFilterPM1 of DDENS (ELM/..._tensorprod3D.F90:2828-2839)
-      double zl[N1];
-#pragma unroll
-      for (int l = 0; l < N1; ++l) zl[l] = sRho[i + N1 * (j + N1 * l)];
-      drhoa = zl[0] * sI[N1 * kk];
-      drhob = zl[0] * sI[N1 * (kk + 4)];
-#pragma unroll
-      for (int l = 1; l < N1; ++l) {
-        drhoa = drhoa + zl[l] * sI[l + N1 * kk];
-        drhob = drhob + zl[l] * sI[l + N1 * (kk + 4)];
-      }
-    }
-    const double* Fx = sF[b][0];
-    const double* Fy = sF[b][1];
-    const double* Fz = sF[b][2];
-    double dxa, dxb, dya, dyb, dza, dzb;
-    {
-      double Di[N1];
-#pragma unroll
-      for (int l = 0; l < N1; ++l) Di[l] = sDt[l + N1 * i];
-      dxa = Di[0] * Fx[N1 * (j + N1 * kk)];
-      dxb = Di[0] * Fx[N1 * (j + N1 * (kk + 4))];
-#pragma unroll
-      for (int l = 1; l < N1; ++l) {
-        dxa = dxa + Di[l] * Fx[l + N1 * (j + N1 * kk)];
-        dxb = dxb + Di[l] * Fx[l + N1 * (j + N1 * (kk + 4))];
-      }
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    {
-      double Dj[N1];
-#pragma unroll
-      for (int l = 0; l < N1; ++l) Dj[l] = sDt[l + N1 * j];
-      dya = Fy[i + N1 * (N1 * kk)] * Dj[0];
-      dyb = Fy[i + N1 * (N1 * (kk + 4))] * Dj[0];
-#pragma unroll
-      for (int l = 1; l < N1; ++l) {
-        dya = dya + Fy[i + N1 * (l + N1 * kk)] * Dj[l];
-        dyb = dyb + Fy[i + N1 * (l + N1 * (kk + 4))] * Dj[l];
-      }
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    {
-      double zl[N1];
-#pragma unroll
-      for (int l = 0; l < N1; ++l) zl[l] = Fz[i + N1 * (j + N1 * l)];
-      dza = zl[0] * Dka[0];
-      dzb = zl[0] * Dkb[0];
-#pragma unroll
-      for (int l = 1; l < N1; ++l) {
-        dza = dza + zl[l] * Dka[l];
-        dzb = dzb + zl[l] * Dkb[l];
-      }
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    const double* dfv = sDF[v];
-    const double fz4 = dfv[4 * Nfp + i + N1 * j], fz5 = dfv[5 * Nfp + i + N1 * j];
-    const double lifta = Ly0 * dfv[i + N1 * kk] + Lx1 * dfv[Nfp + j + N1 * kk] + Ly2 * dfv[2 * Nfp + i + N1 * kk] +
-                         Lx3 * dfv[3 * Nfp + j + N1 * kk] + Lz4a * fz4 + Lz5a * fz5;
-    const double liftb = Ly0 * dfv[i + N1 * (kk + 4)] + Lx1 * dfv[Nfp + j + N1 * (kk + 4)] +
-                         Ly2 * dfv[2 * Nfp + i + N1 * (kk + 4)] + Lx3 * dfv[3 * Nfp + j + N1 * (kk + 4)] +
-                         Lz4b * fz4 + Lz5b * fz5;
-    const double Xa = E11 * dxa + E22 * dya + E33 * dza + lifta;
-    const double Xb = E11 * dxb + E22 * dyb + E33 * dzb + liftb;
-    double Ta, Tb;
-    if (v == VD || v == VT) {
-      Ta = -Xa * RGsqrt;
-      Tb = -Xb * RGsqrt;
-    } else if (v == VW) {
-      Ta = -Xa * RGsqrt - c.GRAV * drhoa;
-      Tb = -Xb * RGsqrt - c.GRAV * drhob;
-    } else if (v == VU) {
-      Ta = -sa + cor * qa[VV];
-      Ta = Ta - Xa * RGsqrt;
-      Tb = -sb + cor * qb[VV];
-      Tb = Tb - Xb * RGsqrt;
-    } else {
-      Ta = -sa - cor * qa[VU];
-      Ta = Ta - Xa * RGsqrt;
-      Tb = -sb - cor * qb[VU];
-      Tb = Tb - Xb * RGsqrt;
-    }
-    // RK stage update (CMN/scale_timeint_rk.F90:2232-2350, tend_buf_size = 1)
-    double na, nb;
-    switch (rk.mode) {
-      case 0:
-        stg(rk.varTmp[v], o0, qa[v] + rk.cb * Ta);
-        stg(rk.varTmp[v], o1, qb[v] + rk.cb * Tb);
-        na = qa[v] + rk.ca * Ta;
-        nb = qb[v] + rk.ca * Tb;
-        break;
-      case 1:
-        stg(rk.varTmp[v], o0, rB0 + rk.cb * Ta);
-        stg(rk.varTmp[v], o1, rB1 + rk.cb * Tb);
-        na = rA0 + rk.ca * Ta;
-        nb = rA1 + rk.ca * Tb;
-        break;
-      case 2:
-        na = rB0 + rk.cb * Ta;
-        nb = rB1 + rk.cb * Tb;
-        break;
-      default:
-        na = qa[v] + rk.cb * Ta;
-        nb = qb[v] + rk.cb * Tb;
-        break;
-    }
-    stg(rk.out[v], o0, na);
-    stg(rk.out[v], o1, nb);
-    if (v == VT) {
-      if (in.Rtot != nullptr) {
-        stg(rk.DPout, o0, dpres_of(na, Tha, Pha, ldg(in.Rtot, o0), ldg(in.CVtot, o0), ldg(in.CPtot, o0), c));
-        stg(rk.DPout, o1, dpres_of(nb, Thb, Phb, ldg(in.Rtot, o1), ldg(in.CVtot, o1), ldg(in.CPtot, o1), c));
-      } else {
-        stg(rk.DPout, o0, dpres_of(na, Tha, Pha, c.Rdry, c.CVdry, c.CPdry, c));
-        stg(rk.DPout, o1, dpres_of(nb, Thb, Phb, c.Rdry, c.CVdry, c.CPdry, c));
-      }
-    }
-    __builtin_amdgcn_sched_barrier(0);
-  }
-}
-
-// ---------------------------------------------------------------------------
 // FP32 perturbation form of the fused HEVE stage (flat-terrain box, dry air).
 // The tendency -- pointwise fluxes, face flux, derivative contractions, lift,
 // buoyancy -- is evaluated in FP32 on the perturbation variables about the
@@ -782,6 +420,462 @@ __device__ __forceinline__ void rusanov32(const Side32& M, const Side32& P, floa
   df[VW] = h * (P.MZ * VelP - M.MZ * VelM + dp * nz - alpha * (P.MZ - M.MZ));
   df[VU] = h * (P.MX * VelP - M.MX * VelM + dp * nx - alpha * (P.MX - M.MX));
   df[VV] = h * (P.MY * VelP - M.MY * VelM + dp * ny - alpha * (P.MY - M.MY));
+}
+
+// ---------------------------------------------------------------------------
+// p = 7 fused stage kernel on the flat-terrain uniform box: one element per
+// 256-thread workgroup, thread (i,j,kk) owning the two nodes (i,j,kk) and
+// (i,j,kk+4), so that wavefront kk holds two horizontal node planes.
+//  - each thread's D row i, D row j and the Lift entries along x and y stay
+//    lane-varying; the z rows D(kk,:), D(kk+4,:), the VFilterPM1 columns and
+//    the vertical Lift entries are wavefront-uniform and come from scalar loads;
+//  - face work: wavefront w evaluates lateral face w (y-, x+, y+, x-), one face
+//    node per lane, so its neighbour logic is uniform; wavefronts 0 and 3 also
+//    evaluate the bottom / top face, whose interior node is their own;
+//  - every global load of the stage (own state, both sides of the face nodes,
+//    RK operands) is issued before the first barrier, and the face fluxes need
+//    no barrier (their interior values are loaded, not exchanged);
+//  - 44 KB of LDS and at most 256 VGPRs without spills: two elements are
+//    resident per CU (a tighter register bound spills and runs 2x slower).
+// Arithmetic is identical, operation for operation, to heve_kernel.
+template <typename R, bool LVL>
+__device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, const TendIn& in, const RkOut& rk) {
+  constexpr int N1 = 8, Np = 512, Nfp = 64, NfpTot = 384;
+  constexpr bool F64 = std::is_same<R, double>::value;
+  __shared__ R sDt[N1 * N1];
+  __shared__ R sI[N1 * N1];
+  __shared__ R sF[2][3][Np];
+  __shared__ R sDF[5][NfpTot];
+  __shared__ R sRho[Np];
+
+  const int t = threadIdx.x;
+  const int i = t & 7, j = (t >> 3) & 7;
+  const int kk = __builtin_amdgcn_readfirstlane(t >> 6);  // wavefront index
+  const int ke = xcd_remap(blockIdx.x, gridDim.x);
+  const int p0 = i + N1 * j + Nfp * kk, p1 = p0 + 4 * Nfp;
+  const uint32_t base = (uint32_t)Np * ke;
+  const uint32_t o0 = base + p0, o1 = base + p1;
+  const R gamm = (R)(c.CPdry / c.CVdry);
+  if (t < N1 * N1) {
+    sDt[(t % N1) * N1 + t / N1] = (R)m.D1D[t];
+    sI[t] = (R)m.Itr[t];
+  }
+
+  // ---- batch 1: own state of both nodes ----
+  double qa[5], qb[5];  // FP64 state for the RK update
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    qa[v] = ldg(in.q[v], o0);
+    qb[v] = ldg(in.q[v], o1);
+  }
+  // the state in the tendency's arithmetic (the FP64 state itself at R = double)
+  [[maybe_unused]] R raf[5], rbf[5];
+  if constexpr (!F64) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      raf[v] = (R)qa[v];
+      rbf[v] = (R)qb[v];
+    }
+  }
+  R* const ra = [&]() -> R* {
+    if constexpr (F64) return qa;
+    else return raf;
+  }();
+  R* const rb = [&]() -> R* {
+    if constexpr (F64) return qb;
+    else return rbf;
+  }();
+  // a base-state field (0 DENS_hyd, 1 PRES_hyd, 2 THERM_hyd, 3 DPhydDx, 4
+  // DPhydDy) at global node g = node of an element of level lev: from the
+  // per-level tables when the base state is horizontally homogeneous (LVL),
+  // else from the field
+  const int ek0 = ke / (m.NeX * m.NeY);
+  auto hyd = [&](const double* fld, int fi, uint32_t g, int lev, int node) -> double {
+    if constexpr (LVL) return ldg(in.lvl, (uint32_t)((fi * m.NeZ + lev) * Np + node));
+    else return ldg(fld, g);
+  };
+  const R Dha = (R)hyd(in.DENS_hyd, 0, o0, ek0, p0), Dhb = (R)hyd(in.DENS_hyd, 0, o1, ek0, p1);
+  const R Tha = (R)hyd(in.THERM_hyd, 2, o0, ek0, p0), Thb = (R)hyd(in.THERM_hyd, 2, o1, ek0, p1);
+  const R DPa = (R)ldg(in.DPRES, o0), DPb = (R)ldg(in.DPRES, o1);
+
+  // ---- face nodes of this thread: lateral face kk (node lane = t & 63) and,
+  // for wavefronts 0 and 3, the bottom / top face node (i,j) ----
+  const int ei = ke % m.NeX, ej = (ke / m.NeX) % m.NeY, ek = ke / (m.NeX * m.NeY);
+  const int lane = t & 63;
+  const int fL = kk;  // 0: y-, 1: x+, 2: y+, 3: x-
+  const int pmL = face_node<N1>(fL, lane);
+  uint32_t iPL;
+  int nodePL;  // node of the exterior face value within its element
+  int bcL = FEDG_BND_NONE;
+  {
+    int ni = ei, nj = ej;
+    bool closed = false;
+    switch (fL) {
+      case 0: nj -= 1; if (nj < 0) { if (m.periodic[1]) nj = m.NeY - 1; else closed = true; } break;
+      case 1: ni += 1; if (ni >= m.NeX) { if (m.periodic[0]) ni = 0; else closed = true; } break;
+      case 2: nj += 1; if (nj >= m.NeY) { if (m.periodic[1]) nj = 0; else closed = true; } break;
+      default: ni -= 1; if (ni < 0) { if (m.periodic[0]) ni = m.NeX - 1; else closed = true; } break;
+    }
+    constexpr int opp[4] = {2, 3, 0, 1};
+    if (closed) {
+      iPL = base + pmL;
+      nodePL = pmL;
+      bcL = m.bnd_bc[fL];
+    } else {
+      nodePL = face_node<N1>(opp[fL], lane);
+      iPL = (uint32_t)Np * (ni + m.NeX * (nj + m.NeY * ek)) + nodePL;
+    }
+  }
+  const uint32_t iML = base + pmL;
+  R ML[9], PL[9];  // DD, DT, MZ, MX, MY, Dhyd, Phyd, Thyd, DP
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    ML[v] = (R)ldg(in.q[v], iML);
+    PL[v] = (R)ldg(in.q[v], iPL);
+  }
+  ML[5] = (R)hyd(in.DENS_hyd, 0, iML, ek, pmL); ML[6] = (R)hyd(in.PRES_hyd, 1, iML, ek, pmL);
+  ML[7] = (R)hyd(in.THERM_hyd, 2, iML, ek, pmL);
+  ML[8] = (R)ldg(in.DPRES, iML);
+  PL[5] = (R)hyd(in.DENS_hyd, 0, iPL, ek, nodePL); PL[6] = (R)hyd(in.PRES_hyd, 1, iPL, ek, nodePL);
+  PL[7] = (R)hyd(in.THERM_hyd, 2, iPL, ek, nodePL);
+  PL[8] = (R)ldg(in.DPRES, iPL);
+  // vertical face (wavefronts 0 and 3): interior node is this thread's node
+  const bool hasV = (kk == 0) || (kk == 3);
+  const int fV = (kk == 0) ? 4 : 5;
+  R PV[9];
+  R PhV = R(0);  // PRES_hyd of the interior node of the vertical face
+  int bcV = FEDG_BND_NONE;
+  if (hasV) {
+    int nk = ek + (fV == 4 ? -1 : 1);
+    bool closed = false;
+    if (nk < 0 || nk >= m.NeZ) {
+      if (m.periodic[2]) nk = (nk + m.NeZ) % m.NeZ;
+      else closed = true;
+    }
+    const uint32_t own = (fV == 4) ? o0 : o1;
+    const int nodeOwn = (fV == 4) ? p0 : p1;
+    uint32_t iPV;
+    int nodePV = nodeOwn, levPV = ek;
+    if (closed) {
+      iPV = own;
+      bcV = m.bnd_bc[fV];
+    } else {
+      nodePV = i + N1 * j + Nfp * (fV == 4 ? 7 : 0);
+      levPV = nk;
+      iPV = (uint32_t)Np * (ei + m.NeX * (ej + m.NeY * nk)) + (uint32_t)nodePV;
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) PV[v] = (R)ldg(in.q[v], iPV);
+    PV[5] = (R)hyd(in.DENS_hyd, 0, iPV, levPV, nodePV); PV[6] = (R)hyd(in.PRES_hyd, 1, iPV, levPV, nodePV);
+    PV[7] = (R)hyd(in.THERM_hyd, 2, iPV, levPV, nodePV);
+    PV[8] = (R)ldg(in.DPRES, iPV);
+    PhV = (R)hyd(in.PRES_hyd, 1, own, ek, nodeOwn);
+  }
+
+  // VFilterPM1 input
+  sRho[p0] = ra[VD];
+  sRho[p1] = rb[VD];
+
+  // ---- face fluxes ----
+  auto face = [&](int f, const R* Mv, R* Pv, int bc, int fp) {
+    R df[5];
+    if constexpr (F64) {
+      Side M, P;
+      const double nx = (f == 1) ? 1.0 : (f == 3) ? -1.0 : 0.0;
+      const double ny = (f == 2) ? 1.0 : (f == 0) ? -1.0 : 0.0;
+      const double nz = (f == 5) ? 1.0 : (f == 4) ? -1.0 : 0.0;
+      M.Gsqrt = 1.0; M.G13 = 0.0; M.G23 = 0.0;
+      P.Gsqrt = 1.0; P.G13 = 0.0; P.G23 = 0.0;
+      M.DD = Mv[0]; M.DT = Mv[1]; M.MZ = Mv[2]; M.MX = Mv[3]; M.MY = Mv[4];
+      M.Dhyd = Mv[5]; M.Phyd = Mv[6]; M.Thyd = Mv[7]; M.DP = Mv[8];
+      P.DD = Pv[0]; P.DT = Pv[1]; P.MZ = Pv[2]; P.MX = Pv[3]; P.MY = Pv[4];
+      P.Dhyd = Pv[5]; P.Phyd = Pv[6]; P.Thyd = Pv[7]; P.DP = Pv[8];
+      if (bc != FEDG_BND_NONE)
+        reflect_momentum(bc, 1.0, 1.0, 1.0, 0.0, 1.0, 0.0, 0.0, nx, ny, nz, M.MX, M.MY, M.MZ, P.MX, P.MY, P.MZ);
+      rusanov(M, P, nx, ny, nz, m.fF[f], gamm, df);
+    } else {
+      const float nx = (f == 1) ? 1.f : (f == 3) ? -1.f : 0.f;
+      const float ny = (f == 2) ? 1.f : (f == 0) ? -1.f : 0.f;
+      const float nz = (f == 5) ? 1.f : (f == 4) ? -1.f : 0.f;
+      Side32 M, P;
+      M.DD = Mv[0]; M.DT = Mv[1]; M.MZ = Mv[2]; M.MX = Mv[3]; M.MY = Mv[4];
+      M.Dh = Mv[5]; M.Ph = Mv[6]; M.Th = Mv[7]; M.DP = Mv[8];
+      P.DD = Pv[0]; P.DT = Pv[1]; P.MZ = Pv[2]; P.MX = Pv[3]; P.MY = Pv[4];
+      P.Dh = Pv[5]; P.Ph = Pv[6]; P.Th = Pv[7]; P.DP = Pv[8];
+      if (bc == FEDG_BND_SLIP) {  // closed boundary: the exterior side is the interior one, reflected
+        const float mn = M.MX * nx + M.MY * ny + M.MZ * nz;
+        P.MX = M.MX - 2.f * mn * nx;
+        P.MY = M.MY - 2.f * mn * ny;
+        P.MZ = M.MZ - 2.f * mn * nz;
+      } else if (bc == FEDG_BND_NOSLIP) {
+        P.MX = -M.MX; P.MY = -M.MY; P.MZ = -M.MZ;
+      }
+      rusanov32(M, P, nx, ny, nz, (float)m.fF[f], gamm, df);
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) sDF[v][fp] = df[v];
+  };
+  face(fL, ML, PL, bcL, fL * Nfp + lane);
+  __builtin_amdgcn_sched_barrier(0);
+  if (hasV) {
+    R MV[9];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) MV[v] = (fV == 4) ? ra[v] : rb[v];
+    MV[5] = (fV == 4) ? Dha : Dhb;
+    MV[6] = PhV;
+    MV[7] = (fV == 4) ? Tha : Thb;
+    MV[8] = (fV == 4) ? DPa : DPb;
+    face(fV, MV, PV, bcV, fV * Nfp + i + N1 * j);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+
+  const int ke2d = ke % (m.NeX * m.NeY);
+  const uint32_t o2d = (uint32_t)(i + N1 * j) + (uint32_t)Nfp * ke2d;
+
+  // D(kk,:) and D(kk+4,:), wavefront-uniform (scalar loads)
+  double Dka[N1], Dkb[N1];
+#pragma unroll
+  for (int l = 0; l < N1; ++l) {
+    Dka[l] = m.D1D[kk + N1 * l];
+    Dkb[l] = m.D1D[kk + 4 + N1 * l];
+  }
+  // Lift entries: lane-varying along x and y, wavefront-uniform along z
+  const double* LT = m.liftT;  // (N1,6): entry of face f on the line through the node
+  const double Ly0 = LT[j + N1 * 0], Lx1 = LT[i + N1 * 1], Ly2 = LT[j + N1 * 2], Lx3 = LT[i + N1 * 3];
+  const double Lz4a = LT[kk + N1 * 4], Lz5a = LT[kk + N1 * 5];
+  const double Lz4b = LT[kk + 4 + N1 * 4], Lz5b = LT[kk + 4 + N1 * 5];
+
+  const R E11 = (R)m.fE[0], E22 = (R)m.fE[1], E33 = (R)m.fE[2];
+  const R RDa = R(1) / (ra[VD] + Dha), RDb = R(1) / (rb[VD] + Dhb);
+  // Gsqrt = GsqrtH = 1: GsqrtV = 1/1, RGsqrtV = 1/GsqrtV, RGsqrt = 1/1
+  const R GsqrtV = R(1) / R(1);
+  const R RGsqrtV = R(1) / GsqrtV;
+  const R RGsqrt = R(1) / R(1);
+  const R Gs = R(1), g13 = R(0), g23 = R(0);
+  const R GRAV = (R)c.GRAV;
+
+  auto flux = [&](int v, const R* q, R RD, R Th, R DP, R& f0, R& f1, R& f2) {
+    const R m0 = Gs * q[VU];
+    const R m1 = Gs * q[VV];
+    const R m2 = Gs * (q[VW] * RGsqrtV + g13 * q[VU] + g23 * q[VV]);
+    if (v == VD) {
+      f0 = m0; f1 = m1; f2 = m2;
+    } else if (v == VT) {
+      const R pt_ = (Th + q[VT]) * RD;
+      f0 = m0 * pt_; f1 = m1 * pt_; f2 = m2 * pt_;
+    } else if (v == VW) {
+      const R w_ = q[VW] * RD;
+      f0 = m0 * w_; f1 = m1 * w_; f2 = m2 * w_ + Gs * DP * RGsqrtV;
+    } else if (v == VU) {
+      const R G = Gs * DP;
+      const R u_ = q[VU] * RD;
+      f0 = m0 * u_ + G; f1 = m1 * u_; f2 = m2 * u_ + G * g13;
+    } else {
+      const R G = Gs * DP;
+      const R v_ = q[VV] * RD;
+      f0 = m0 * v_; f1 = m1 * v_ + G; f2 = m2 * v_ + G * g23;
+    }
+  };
+
+  // a variable's pointwise operands (RK operands, DPhydDx/Dy and Coriolis,
+  // PRES_hyd), loaded one variable ahead so that their latency overlaps the
+  // previous variable's contractions
+  struct Ops {
+    double rA0, rA1, rB0, rB1, sa, sb, cor, Pha, Phb;
+  };
+  auto load_ops = [&](int v) {
+    Ops o{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (rk.mode == 1) {
+      o.rA0 = ldg(rk.var0[v], o0);
+      o.rA1 = ldg(rk.var0[v], o1);
+    }
+    if (rk.mode == 1 || rk.mode == 2) {
+      o.rB0 = ldg(rk.varTmp[v], o0);
+      o.rB1 = ldg(rk.varTmp[v], o1);
+    }
+    if (v == VU) {
+      o.sa = hyd(in.DPhydDx, 3, o0, ek0, p0);
+      o.sb = hyd(in.DPhydDx, 3, o1, ek0, p1);
+      o.cor = ldg(in.CORIOLIS, o2d);
+    } else if (v == VV) {
+      o.sa = hyd(in.DPhydDy, 4, o0, ek0, p0);
+      o.sb = hyd(in.DPhydDy, 4, o1, ek0, p1);
+      o.cor = ldg(in.CORIOLIS, o2d);
+    } else if (v == VT) {
+      o.Pha = hyd(in.PRES_hyd, 1, o0, ek0, p0);
+      o.Phb = hyd(in.PRES_hyd, 1, o1, ek0, p1);
+    }
+    return o;
+  };
+  Ops nxt = load_ops(0);
+  R drhoa = R(0), drhob = R(0);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const Ops cur = nxt;
+    if (v < 4) nxt = load_ops(v + 1);
+    const double rA0 = cur.rA0, rA1 = cur.rA1, rB0 = cur.rB0, rB1 = cur.rB1;
+    const R sa = (R)cur.sa, sb = (R)cur.sb, cor = (R)cur.cor;
+    const double Pha = cur.Pha, Phb = cur.Phb;
+    const int b = v & 1;
+    R f0, f1, f2;
+    flux(v, ra, RDa, Tha, DPa, f0, f1, f2);
+    sF[b][0][p0] = f0; sF[b][1][p0] = f1; sF[b][2][p0] = f2;
+    flux(v, rb, RDb, Thb, DPb, f0, f1, f2);
+    sF[b][0][p1] = f0; sF[b][1][p1] = f1; sF[b][2][p1] = f2;
+    __syncthreads();
+    if (v == 0) {
+      // VFilterPM1 of DDENS (ELM/..._tensorprod3D.F90:2828-2839)
+      R zl[N1];
+#pragma unroll
+      for (int l = 0; l < N1; ++l) zl[l] = sRho[i + N1 * (j + N1 * l)];
+      drhoa = zl[0] * sI[N1 * kk];
+      drhob = zl[0] * sI[N1 * (kk + 4)];
+#pragma unroll
+      for (int l = 1; l < N1; ++l) {
+        drhoa = drhoa + zl[l] * sI[l + N1 * kk];
+        drhob = drhob + zl[l] * sI[l + N1 * (kk + 4)];
+      }
+    }
+    const R* Fx = sF[b][0];
+    const R* Fy = sF[b][1];
+    const R* Fz = sF[b][2];
+    R dxa, dxb, dya, dyb, dza, dzb;
+    {
+      R Di[N1];
+#pragma unroll
+      for (int l = 0; l < N1; ++l) Di[l] = sDt[l + N1 * i];
+      dxa = Di[0] * Fx[N1 * (j + N1 * kk)];
+      dxb = Di[0] * Fx[N1 * (j + N1 * (kk + 4))];
+#pragma unroll
+      for (int l = 1; l < N1; ++l) {
+        dxa = dxa + Di[l] * Fx[l + N1 * (j + N1 * kk)];
+        dxb = dxb + Di[l] * Fx[l + N1 * (j + N1 * (kk + 4))];
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    {
+      R Dj[N1];
+#pragma unroll
+      for (int l = 0; l < N1; ++l) Dj[l] = sDt[l + N1 * j];
+      dya = Fy[i + N1 * (N1 * kk)] * Dj[0];
+      dyb = Fy[i + N1 * (N1 * (kk + 4))] * Dj[0];
+#pragma unroll
+      for (int l = 1; l < N1; ++l) {
+        dya = dya + Fy[i + N1 * (l + N1 * kk)] * Dj[l];
+        dyb = dyb + Fy[i + N1 * (l + N1 * (kk + 4))] * Dj[l];
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    {
+      R zl[N1];
+#pragma unroll
+      for (int l = 0; l < N1; ++l) zl[l] = Fz[i + N1 * (j + N1 * l)];
+      dza = zl[0] * Dka[0];
+      dzb = zl[0] * Dkb[0];
+#pragma unroll
+      for (int l = 1; l < N1; ++l) {
+        dza = dza + zl[l] * Dka[l];
+        dzb = dzb + zl[l] * Dkb[l];
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    const R* dfv = sDF[v];
+    const R fz4 = dfv[4 * Nfp + i + N1 * j], fz5 = dfv[5 * Nfp + i + N1 * j];
+    const R lifta = Ly0 * dfv[i + N1 * kk] + Lx1 * dfv[Nfp + j + N1 * kk] + Ly2 * dfv[2 * Nfp + i + N1 * kk] +
+                         Lx3 * dfv[3 * Nfp + j + N1 * kk] + Lz4a * fz4 + Lz5a * fz5;
+    const R liftb = Ly0 * dfv[i + N1 * (kk + 4)] + Lx1 * dfv[Nfp + j + N1 * (kk + 4)] +
+                         Ly2 * dfv[2 * Nfp + i + N1 * (kk + 4)] + Lx3 * dfv[3 * Nfp + j + N1 * (kk + 4)] +
+                         Lz4b * fz4 + Lz5b * fz5;
+    const R Xa = E11 * dxa + E22 * dya + E33 * dza + lifta;
+    const R Xb = E11 * dxb + E22 * dyb + E33 * dzb + liftb;
+    R Ta, Tb;
+    if (v == VD || v == VT) {
+      Ta = -Xa * RGsqrt;
+      Tb = -Xb * RGsqrt;
+    } else if (v == VW) {
+      Ta = -Xa * RGsqrt - GRAV * drhoa;
+      Tb = -Xb * RGsqrt - GRAV * drhob;
+    } else if (v == VU) {
+      Ta = -sa + cor * ra[VV];
+      Ta = Ta - Xa * RGsqrt;
+      Tb = -sb + cor * rb[VV];
+      Tb = Tb - Xb * RGsqrt;
+    } else {
+      Ta = -sa - cor * ra[VU];
+      Ta = Ta - Xa * RGsqrt;
+      Tb = -sb - cor * rb[VU];
+      Tb = Tb - Xb * RGsqrt;
+    }
+    const double TA = (double)Ta, TB = (double)Tb;
+    // RK stage update (CMN/scale_timeint_rk.F90:2232-2350, tend_buf_size = 1)
+    double na, nb;
+    switch (rk.mode) {
+      case 0:
+        stg(rk.varTmp[v], o0, qa[v] + rk.cb * TA);
+        stg(rk.varTmp[v], o1, qb[v] + rk.cb * TB);
+        na = qa[v] + rk.ca * TA;
+        nb = qb[v] + rk.ca * TB;
+        break;
+      case 1:
+        stg(rk.varTmp[v], o0, rB0 + rk.cb * TA);
+        stg(rk.varTmp[v], o1, rB1 + rk.cb * TB);
+        na = rA0 + rk.ca * TA;
+        nb = rA1 + rk.ca * TB;
+        break;
+      case 2:
+        na = rB0 + rk.cb * TA;
+        nb = rB1 + rk.cb * TB;
+        break;
+      default:
+        na = qa[v] + rk.cb * TA;
+        nb = qb[v] + rk.cb * TB;
+        break;
+    }
+    stg(rk.out[v], o0, na);
+    stg(rk.out[v], o1, nb);
+    if (v == VT) {
+      if constexpr (F64) {
+        if (in.Rtot != nullptr) {
+          stg(rk.DPout, o0, dpres_of(na, Tha, Pha, ldg(in.Rtot, o0), ldg(in.CVtot, o0), ldg(in.CPtot, o0), c));
+          stg(rk.DPout, o1, dpres_of(nb, Thb, Phb, ldg(in.Rtot, o1), ldg(in.CVtot, o1), ldg(in.CPtot, o1), c));
+        } else {
+          stg(rk.DPout, o0, dpres_of(na, Tha, Pha, c.Rdry, c.CVdry, c.CPdry, c));
+          stg(rk.DPout, o1, dpres_of(nb, Thb, Phb, c.Rdry, c.CVdry, c.CPdry, c));
+        }
+      } else {
+        // perturbation form of the pressure (dry air, the base state's gamma)
+        stg(rk.DPout, o0, (double)dpres32((float)na, Tha, (float)Pha, gamm));
+        stg(rk.DPout, o1, (double)dpres32((float)nb, Thb, (float)Phb, gamm));
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+__global__ void __launch_bounds__(256, 2) heve7_flat_kernel(const DevMesh m, const Consts c, const TendIn in,
+                                                            const RkOut rk) {
+  heve7_body<double, false>(m, c, in, rk);
+}
+
+// base state and its horizontal gradient read from per-level tables
+// (horizontally homogeneous base state): the same values, a fraction of the
+// bytes (the tables stay in L2)
+__global__ void __launch_bounds__(256, 2) heve7_lvl_kernel(const DevMesh m, const Consts c, const TendIn in,
+                                                           const RkOut rk) {
+  heve7_body<double, true>(m, c, in, rk);
+}
+
+// The same p = 7 stage with the tendency in FP32 about the FP64 base state
+// (perturbation form; state, base state and RK update FP64; the pressure
+// perturbation of the new state in FP32 as dpres32).  Half the LDS and fewer
+// registers: three workgroups per CU.
+__global__ void __launch_bounds__(256, 3) heve7_f32_kernel(const DevMesh m, const Consts c, const TendIn in,
+                                                           const RkOut rk) {
+  heve7_body<float, false>(m, c, in, rk);
+}
+__global__ void __launch_bounds__(256, 3) heve7_f32_lvl_kernel(const DevMesh m, const Consts c, const TendIn in,
+                                                               const RkOut rk) {
+  heve7_body<float, true>(m, c, in, rk);
 }
 
 template <int N1, int EPB>
@@ -1167,6 +1261,14 @@ int launch_heve_fused32(fedg_ctx* ctx, const TendIn& in, const RkOut& rk) {
     set_error("the FP32 perturbation form is built for the flat-terrain box with dry air");
     return FEDG_ERR_UNSUPPORTED;
   }
+  if (m.n1 == 8 && m.lift_tensor) {
+    if (in.lvl)
+      hipLaunchKernelGGL(heve7_f32_lvl_kernel, dim3(m.Ne), dim3(256), 0, ctx->stream, m, ctx->c, in, rk);
+    else
+      hipLaunchKernelGGL(heve7_f32_kernel, dim3(m.Ne), dim3(256), 0, ctx->stream, m, ctx->c, in, rk);
+    FEDG_HIP_CHECK(hipGetLastError());
+    return FEDG_OK;
+  }
   auto go = [&](auto n1c) -> int {
     constexpr int N1 = decltype(n1c)::value;
     constexpr int EPB = epb_for<N1>();
@@ -1193,7 +1295,10 @@ int launch_heve_fused32(fedg_ctx* ctx, const TendIn& in, const RkOut& rk) {
 int launch_heve_fused(fedg_ctx* ctx, const TendIn& in, const RkOut& rk, int use_flat) {
   const DevMesh& m = ctx->m;
   if (use_flat && m.n1 == 8 && m.lift_tensor) {
-    hipLaunchKernelGGL(heve7_flat_kernel, dim3(m.Ne), dim3(256), 0, ctx->stream, m, ctx->c, in, rk);
+    if (in.lvl)
+      hipLaunchKernelGGL(heve7_lvl_kernel, dim3(m.Ne), dim3(256), 0, ctx->stream, m, ctx->c, in, rk);
+    else
+      hipLaunchKernelGGL(heve7_flat_kernel, dim3(m.Ne), dim3(256), 0, ctx->stream, m, ctx->c, in, rk);
     FEDG_HIP_CHECK(hipGetLastError());
     return FEDG_OK;
   }
