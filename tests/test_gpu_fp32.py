@@ -57,3 +57,16 @@ def test_fp32_form_keeps_rest_state():
     out, q = run(case, 50, 32)
     for v in range(5):
         assert np.max(np.abs(out[v])) <= 1e-6, fedg.VARS[v]
+
+
+def test_fp32_form_p7_error_against_fp64():
+    """The p=7 FP32 stage (the layout of the FP64 p=7 kernel, base state from
+    per-level tables) on the config-2 physics, 4x3x4 elements, 100 steps."""
+    case = cases.scaled_case(cases.config2(), 4, 3, 4, nsteps=100)
+    a, _ = run(case, case.nsteps, 32)
+    b, _ = run(case, case.nsteps, 64)
+    for v, name in enumerate(fedg.VARS):
+        assert np.all(np.isfinite(a[v])), name
+        rl2, rli = errors(a[v], b[v])
+        print(f"p=7 100 steps {name}: FP32 form vs FP64 relative L2 {rl2:.2e}, Linf {rli:.2e}")
+        assert rl2 <= 1e-5, name
