@@ -868,12 +868,13 @@ __global__ void __launch_bounds__(256, 2) heve7_lvl_kernel(const DevMesh m, cons
 // The same p = 7 stage with the tendency in FP32 about the FP64 base state
 // (perturbation form; state, base state and RK update FP64; the pressure
 // perturbation of the new state in FP32 as dpres32).  Half the LDS and fewer
-// registers: three workgroups per CU.
-__global__ void __launch_bounds__(256, 3) heve7_f32_kernel(const DevMesh m, const Consts c, const TendIn in,
+// registers; two workgroups per CU all the same (a bound for three spills and
+// runs 20 % slower).
+__global__ void __launch_bounds__(256, 2) heve7_f32_kernel(const DevMesh m, const Consts c, const TendIn in,
                                                            const RkOut rk) {
   heve7_body<float, false>(m, c, in, rk);
 }
-__global__ void __launch_bounds__(256, 3) heve7_f32_lvl_kernel(const DevMesh m, const Consts c, const TendIn in,
+__global__ void __launch_bounds__(256, 2) heve7_f32_lvl_kernel(const DevMesh m, const Consts c, const TendIn in,
                                                                const RkOut rk) {
   heve7_body<float, true>(m, c, in, rk);
 }
