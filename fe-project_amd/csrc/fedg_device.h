@@ -131,6 +131,30 @@ __device__ __forceinline__ double dpres_of(double DRHOT, double THERM_hyd, doubl
   const double PRES = c.PRES00 * fedg_pow(Rtot * rP0 * RHOT, CPtot / CVtot);
   return PRES - PRES_hyd;
 }
+// The same with the pow tables at `tab` (a copy in LDS): the same bits.
+__device__ __forceinline__ double dpres_of_tab(double DRHOT, double THERM_hyd, double PRES_hyd, double Rtot,
+                                               double CVtot, double CPtot, const Consts& c, const double* tab) {
+  const double rP0 = 1.0 / c.PRES00;
+  const double RHOT = THERM_hyd + DRHOT;
+  const double PRES = c.PRES00 * fedg_pow_tab(Rtot * rP0 * RHOT, CPtot / CVtot, tab);
+  return PRES - PRES_hyd;
+}
+// Copy the 5 x 128 pow tables to `tab` (LDS) with the threads of a workgroup
+// of `nthr` threads; the caller synchronises before the first read.
+__device__ __forceinline__ void pow_tables_to(double* tab, int t, int nthr) {
+  for (int q = t; q < 640; q += nthr) {
+    const int e = q & 127;
+    double v;
+    switch (q >> 7) {
+      case 0: v = fedg_pow_invc[e]; break;
+      case 1: v = fedg_pow_logc_hi[e]; break;
+      case 2: v = fedg_pow_logc_lo[e]; break;
+      case 3: v = fedg_pow_exp_hi[e]; break;
+      default: v = fedg_pow_exp_lo[e]; break;
+    }
+    tab[q] = v;
+  }
+}
 
 // Element-local node of face node a on face f (faces in the reference order
 // y-, x+, y+, x-, z-, z+; ELM/scale_element_hexahedral.F90:173-180).

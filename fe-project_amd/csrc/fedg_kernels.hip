@@ -447,6 +447,9 @@ __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, co
   __shared__ R sF[2][3][Np];
   __shared__ R sDF[5][NfpTot];
   __shared__ R sRho[Np];
+  // tables of the pressure's pow (FP64 form), read by the stage update after
+  // the first barrier
+  __shared__ double sPow[F64 ? 640 : 1];
 
   const int t = threadIdx.x;
   const int i = t & 7, j = (t >> 3) & 7;
@@ -460,6 +463,7 @@ __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, co
     sDt[(t % N1) * N1 + t / N1] = (R)m.D1D[t];
     sI[t] = (R)m.Itr[t];
   }
+  if constexpr (F64) pow_tables_to(sPow, t, 256);
 
   // ---- batch 1: own state of both nodes ----
   double qa[5], qb[5];  // FP64 state for the RK update
@@ -836,11 +840,11 @@ __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, co
     if (v == VT) {
       if constexpr (F64) {
         if (in.Rtot != nullptr) {
-          stg(rk.DPout, o0, dpres_of(na, Tha, Pha, ldg(in.Rtot, o0), ldg(in.CVtot, o0), ldg(in.CPtot, o0), c));
-          stg(rk.DPout, o1, dpres_of(nb, Thb, Phb, ldg(in.Rtot, o1), ldg(in.CVtot, o1), ldg(in.CPtot, o1), c));
+          stg(rk.DPout, o0, dpres_of_tab(na, Tha, Pha, ldg(in.Rtot, o0), ldg(in.CVtot, o0), ldg(in.CPtot, o0), c, sPow));
+          stg(rk.DPout, o1, dpres_of_tab(nb, Thb, Phb, ldg(in.Rtot, o1), ldg(in.CVtot, o1), ldg(in.CPtot, o1), c, sPow));
         } else {
-          stg(rk.DPout, o0, dpres_of(na, Tha, Pha, c.Rdry, c.CVdry, c.CPdry, c));
-          stg(rk.DPout, o1, dpres_of(nb, Thb, Phb, c.Rdry, c.CVdry, c.CPdry, c));
+          stg(rk.DPout, o0, dpres_of_tab(na, Tha, Pha, c.Rdry, c.CVdry, c.CPdry, c, sPow));
+          stg(rk.DPout, o1, dpres_of_tab(nb, Thb, Phb, c.Rdry, c.CVdry, c.CPdry, c, sPow));
         }
       } else {
         // perturbation form of the pressure (dry air, the base state's gamma)

@@ -62,7 +62,11 @@ FEDG_POW_FN double fedg_fast_two_sum(double a, double b, double *err) {
   return s;
 }
 
-FEDG_POW_FN double fedg_pow_core(double x, double y) {
+/* The core with the five 128-entry tables passed in, so that a caller can
+ * read them from a faster memory (the GPU kernels copy them to LDS); the
+ * arithmetic does not depend on where the tables live. */
+FEDG_POW_FN double fedg_pow_core_t(double x, double y, const double *tinvc, const double *tlogc_hi,
+                                   const double *tlogc_lo, const double *texp_hi, const double *texp_lo) {
   /* ---- ln x in double-double ---- */
   uint64_t ux = fedg_pow_bits(x);
   int e = (int)((ux >> 52) & 0x7ff);
@@ -73,7 +77,7 @@ FEDG_POW_FN double fedg_pow_core(double x, double y) {
   e -= 1023;
   const int i = (int)((ux >> 45) & 127);
   const double m = fedg_pow_from_bits((ux & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
-  const double invc = fedg_pow_invc[i];
+  const double invc = tinvc[i];
   const double p_hi = m * invc;
   const double p_lo = fma(m, invc, -p_hi);
   const double r_hi = p_hi - 1.0; /* exact: p_hi in [1-2^-8, 1+2^-8] */
@@ -93,9 +97,9 @@ FEDG_POW_FN double fedg_pow_core(double x, double y) {
   /* e ln2 + (-ln invc) + ln(1+r) */
   const double ed = (double)e;
   double s1_lo, s2_lo;
-  const double s1 = fedg_two_sum(ed * FEDG_POW_LN2_HI, fedg_pow_logc_hi[i], &s1_lo);
+  const double s1 = fedg_two_sum(ed * FEDG_POW_LN2_HI, tlogc_hi[i], &s1_lo);
   const double s2 = fedg_two_sum(s1, a_hi, &s2_lo);
-  const double lo = ed * FEDG_POW_LN2_LO + fedg_pow_logc_lo[i] + s1_lo + s2_lo + l1_lo;
+  const double lo = ed * FEDG_POW_LN2_LO + tlogc_lo[i] + s1_lo + s2_lo + l1_lo;
   double L_lo;
   const double L_hi = fedg_two_sum(s2, lo, &L_lo);
 
@@ -121,7 +125,7 @@ FEDG_POW_FN double fedg_pow_core(double x, double y) {
   const double em_lo = em_lo0 + s_lo + ((t * t) * t) * q + (t_lo2 + t_lo2 * t);
   const int j = k & 127;
   const int qexp = (k - j) / 128;
-  const double Eh = fedg_pow_exp_hi[j], El = fedg_pow_exp_lo[j];
+  const double Eh = texp_hi[j], El = texp_lo[j];
   const double pr_hi = Eh * em_hi;
   const double pr_lo = fma(Eh, em_hi, -pr_hi);
   double u_lo;
@@ -131,17 +135,35 @@ FEDG_POW_FN double fedg_pow_core(double x, double y) {
   return res * fedg_pow_from_bits((uint64_t)(qexp + 1023) << 52);
 }
 
-/* pow(x, y): the portable path for x > 0 finite and moderate results, the
- * platform pow otherwise (never reached by the dynamical core's pressure). */
-FEDG_POW_FN double fedg_pow(double x, double y) {
+FEDG_POW_FN double fedg_pow_core(double x, double y) {
+  return fedg_pow_core_t(x, y, fedg_pow_invc, fedg_pow_logc_hi, fedg_pow_logc_lo, fedg_pow_exp_hi, fedg_pow_exp_lo);
+}
+
+/* Whether (x, y) takes the portable path: x > 0 finite and |y log2 x| well
+ * inside the double range. */
+FEDG_POW_FN int fedg_pow_in_range(double x, double y) {
   if (x > 0.0 && x < 1.0e308 && y == y && y > -1.0e300 && y < 1.0e300) {
     const double lx = 710.0 / 0.6931471805599453; /* bound on |y * log2 x| */
     uint64_t ux = fedg_pow_bits(x);
     int e = (int)((ux >> 52) & 0x7ff) - 1023;
     const double ae = (double)(e < 0 ? -e : e) + 1.0;
     const double ay = y < 0.0 ? -y : y;
-    if (ay * ae < lx - 2.0) return fedg_pow_core(x, y);
+    return ay * ae < lx - 2.0;
   }
+  return 0;
+}
+
+/* pow(x, y): the portable path for x > 0 finite and moderate results, the
+ * platform pow otherwise (never reached by the dynamical core's pressure). */
+FEDG_POW_FN double fedg_pow(double x, double y) {
+  if (fedg_pow_in_range(x, y)) return fedg_pow_core(x, y);
+  return pow(x, y);
+}
+
+/* fedg_pow with the tables at t = {invc, logc_hi, logc_lo, exp_hi, exp_lo},
+ * 128 entries each, contiguous. */
+FEDG_POW_FN double fedg_pow_tab(double x, double y, const double *t) {
+  if (fedg_pow_in_range(x, y)) return fedg_pow_core_t(x, y, t, t + 128, t + 256, t + 384, t + 512);
   return pow(x, y);
 }
 
