@@ -120,6 +120,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed steps, run one device copy of Np*Ne doubles (copy_kernel), the "
                          "known-size reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--no-probe", action="store_true",
+                    help="skip the access-pattern probe (the stage's loads and stores without arithmetic)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU sharding is a design "
@@ -163,6 +165,15 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic()},
     }
+    if not args.no_probe:
+        # the same loads and stores without the arithmetic: the practical HBM
+        # ceiling of this traffic pattern (first, middle x2, last RK4 stage)
+        pm = [fedg.stage_probe(md, mesh.NeX, mesh.NeY, mesh.NeZ, 10) for md in (0, 1, 2)]
+        probe = (pm[0] + 2 * pm[1] + pm[2]) / 4
+        out["access_pattern_probe"] = {
+            "ms_per_stage": probe, "ms_first_middle_last": pm, "kernel_ms_per_stage": ms_kernel,
+            "kernel_over_probe": ms_kernel / probe,
+            "probe_frac_of_peak": BYTES_PER_DOF_STAGE * dof / (probe * 1e-3) / 1e9 / HBM_PEAK_GBS}
     if args.copy_probe:
         t = fedg.TimeIntRK(dm, "ERK_4s4o", case.dt)
         src = fedg.DevField(mesh.Np * mesh.NeA)

@@ -204,6 +204,9 @@ int fedg_device_sync(void) {
   FEDG_HIP_CHECK(hipDeviceSynchronize());
   return FEDG_OK;
 }
+int fedg_stage_probe(int mode, int NeX, int NeY, int NeZ, int reps, double* ms) {
+  return fedg::run_stage_probe(mode, NeX, NeY, NeZ, reps, ms);
+}
 int fedg_device_name(char* buf, int buflen) {
   int dev = 0;
   FEDG_HIP_CHECK(hipGetDevice(&dev));

@@ -112,6 +112,7 @@ struct RkOut {
 int launch_heve_dropin(fedg_ctx* ctx, const TendIn& in, double* const out[5]);
 int launch_hevi_explicit(fedg_ctx* ctx, const TendIn& in, double* const out[5]);
 int launch_heve_fused(fedg_ctx* ctx, const TendIn& in, const RkOut& rk, int use_flat);
+int run_stage_probe(int mode, int NeX, int NeY, int NeZ, int reps, double* ms);
 // FP32 perturbation form of the fused stage (flat box, dry air; at p = 7 the
 // stage reads DPRES and writes RkOut.DPout like the FP64 stage, otherwise it
 // forms DPRES itself and leaves DPout)

@@ -21,6 +21,7 @@
 // fedg_pow.h, so results agree bit for bit with a CPU evaluation of the same
 // expressions that uses the same pow.
 #include <type_traits>
+#include <vector>
 
 #include "fedg_device.h"
 
@@ -1239,6 +1240,48 @@ __global__ void axpy2_set_kernel(double* y, const double* a, double ca, const do
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) y[t] = a[t] + ca * x[t] + cb * z[t];
 }
+// Access-pattern probe (measurement only): the loads and stores of one p = 7
+// fused stage on a periodic NeX x NeY x NeZ box without its arithmetic.  Every
+// workgroup reads NR fields of its element's 512 nodes, the exterior values
+// of NG fields at its 384 face nodes (from the six neighbours) and writes NW
+// fields: the practical HBM ceiling for that traffic mix.
+struct ProbeArgs {
+  const double* r[16];
+  double* w[11];
+  int NeX, NeY, NeZ;
+};
+template <int NR, int NW, int NG>
+__global__ void __launch_bounds__(256) stage_probe_kernel(const ProbeArgs p) {
+  const int e = blockIdx.x, t = threadIdx.x;
+  const size_t base = (size_t)e * 512;
+  double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+  for (int f = 0; f < NR; ++f) {
+    a0 += p.r[f][base + t];
+    a1 += p.r[f][base + t + 256];
+  }
+  const int ei = e % p.NeX, ej = (e / p.NeX) % p.NeY, ek = e / (p.NeX * p.NeY);
+  for (int idx = t; idx < 384; idx += 256) {
+    const int f = idx / 64, a = idx % 64;
+    int ni = ei, nj = ej, nk = ek;
+    if (f == 0) nj = (nj + p.NeY - 1) % p.NeY;
+    else if (f == 1) ni = (ni + 1) % p.NeX;
+    else if (f == 2) nj = (nj + 1) % p.NeY;
+    else if (f == 3) ni = (ni + p.NeX - 1) % p.NeX;
+    else if (f == 4) nk = (nk + p.NeZ - 1) % p.NeZ;
+    else nk = (nk + 1) % p.NeZ;
+    const int opp = (f < 4) ? (f ^ 2) : (9 - f);
+    const size_t g = (size_t)(ni + p.NeX * (nj + p.NeY * nk)) * 512 + face_node<8>(opp, a);
+#pragma unroll
+    for (int q = 0; q < NG; ++q) a0 += p.r[q][g];
+  }
+#pragma unroll
+  for (int f = 0; f < NW; ++f) {
+    p.w[f][base + t] = a0 + f;
+    p.w[f][base + t + 256] = a1 + f;
+  }
+}
+
 __global__ void copy_kernel(double* y, const double* x, long n) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) y[t] = x[t];
@@ -1258,6 +1301,57 @@ int launch_hevi_explicit(fedg_ctx* ctx, const TendIn& in, double* const out[5]) 
   RkOut rk{};
   for (int v = 0; v < 5; ++v) rk.out[v] = out[v];
   return launch_heve_mode<MODE_DROPIN_HEVI>(ctx, in, rk);
+}
+
+// mode 0 / 1 / 2: the traffic of the first / middle / last RK4 stage
+// (reads 6 / 16 / 11 fields, writes 11 / 11 / 6, six fields gathered at the
+// face nodes); arrays of NeX NeY NeZ 512 doubles, allocated here
+int run_stage_probe(int mode, int NeX, int NeY, int NeZ, int reps, double* ms) {
+  if (mode < 0 || mode > 2 || NeX < 1 || NeY < 1 || NeZ < 1 || reps < 1 || !ms) return FEDG_ERR_ARG;
+  const size_t ne = (size_t)NeX * NeY * NeZ;
+  if (ne > (size_t)0x7fffffff) return FEDG_ERR_ARG;
+  const size_t n = ne * 512;
+  std::vector<double*> buf(27, nullptr);
+  int rc = FEDG_OK;
+  for (auto& b : buf) {
+    if (hipMalloc(&b, n * sizeof(double)) != hipSuccess || hipMemset(b, 0, n * sizeof(double)) != hipSuccess) {
+      set_error("stage probe: cannot allocate its 27 fields");
+      rc = FEDG_ERR_NOMEM;
+      break;
+    }
+  }
+  ProbeArgs p;
+  for (int f = 0; f < 16; ++f) p.r[f] = buf[f];
+  for (int f = 0; f < 11; ++f) p.w[f] = buf[16 + f];
+  p.NeX = NeX; p.NeY = NeY; p.NeZ = NeZ;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (rc == FEDG_OK) {
+    auto go = [&]() {
+      if (mode == 0) hipLaunchKernelGGL((stage_probe_kernel<6, 11, 6>), dim3((unsigned)ne), dim3(256), 0, 0, p);
+      else if (mode == 1) hipLaunchKernelGGL((stage_probe_kernel<16, 11, 6>), dim3((unsigned)ne), dim3(256), 0, 0, p);
+      else hipLaunchKernelGGL((stage_probe_kernel<11, 6, 6>), dim3((unsigned)ne), dim3(256), 0, 0, p);
+    };
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = FEDG_ERR_HIP;
+    if (rc == FEDG_OK) {
+      go();
+      go();
+      (void)hipEventRecord(e0, 0);
+      for (int r = 0; r < reps; ++r) go();
+      (void)hipEventRecord(e1, 0);
+      float t = 0.f;
+      if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+        set_error("stage probe: launch failed");
+        rc = FEDG_ERR_HIP;
+      }
+      *ms = (double)t / reps;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  for (auto b : buf)
+    if (b) (void)hipFree(b);
+  return rc;
 }
 
 int launch_heve_fused32(fedg_ctx* ctx, const TendIn& in, const RkOut& rk) {

@@ -72,6 +72,7 @@ def _load():
         "fedg_memset_dev": (C.c_int, [vp, C.c_int, C.c_size_t]),
         "fedg_device_sync": (C.c_int, []),
         "fedg_device_name": (C.c_int, [C.c_char_p, C.c_int]),
+        "fedg_stage_probe": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
         "fedg_refelem_tables": (C.c_int, [C.c_int, dp, dp, dp, dp, dp, dp, dp, dp, ip, ip]),
         "fedg_box_get_sizes": (C.c_int, [C.POINTER(BoxDesc), C.POINTER(BoxSizes)]),
         "fedg_box_build": (C.c_int, [C.POINTER(BoxDesc), C.POINTER(BoxArrays)]),
@@ -147,7 +148,7 @@ LIB = _load()
 
 EXPORTED = (
     "fedg_last_error fedg_version fedg_dev_alloc fedg_dev_free fedg_memcpy_h2d fedg_memcpy_d2h "
-    "fedg_memset_dev fedg_device_sync fedg_device_name fedg_refelem_tables fedg_box_get_sizes "
+    "fedg_memset_dev fedg_device_sync fedg_device_name fedg_stage_probe fedg_refelem_tables fedg_box_get_sizes "
     "fedg_box_build fedg_ctx_create fedg_ctx_destroy fedg_ctx_sizes fedg_drhot2pres fedg_halo_exchange "
     "fedg_apply_bc_prgvars fedg_heve_cal_tend fedg_hevi_cal_tend fedg_hevi_cal_vi fedg_hevi_cal_vi_newton "
     "fedg_modalfilter_matrix fedg_ctx_setup_modalfilter fedg_modalfilter_apply fedg_heve_set_modalfilter "

@@ -462,3 +462,11 @@ def device_name() -> str:
     buf = C.create_string_buffer(256)
     check(LIB.fedg_device_name(buf, 256), "fedg_device_name")
     return buf.value.decode()
+
+
+def stage_probe(mode: int, NeX: int, NeY: int, NeZ: int, reps: int = 10) -> float:
+    """Mean ms of a kernel with the loads and stores of one p=7 RK4 stage and no
+    arithmetic (fedg_stage_probe): mode 0 / 1 / 2 = first / middle / last stage."""
+    ms = C.c_double()
+    check(LIB.fedg_stage_probe(int(mode), int(NeX), int(NeY), int(NeZ), int(reps), C.byref(ms)), "fedg_stage_probe")
+    return ms.value

@@ -56,6 +56,15 @@ int fedg_memcpy_d2h(void *dst_host, const void *src_dev, size_t bytes);
 int fedg_memset_dev(void *dst_dev, int value, size_t bytes);
 int fedg_device_sync(void);
 int fedg_device_name(char *buf, int buflen);
+/* Access-pattern probe (no counterpart in the reference; for measurement).
+ * Times, on the current device, a kernel with the loads and stores of one
+ * p = 7 fused RK4 stage on a periodic NeX x NeY x NeZ box and no arithmetic:
+ * mode 0 / 1 / 2 = first / middle / last stage (reads 6 / 16 / 11 fields of
+ * every node, writes 11 / 11 / 6, gathers six fields at the 384 face nodes of
+ * every element from its neighbours).  Allocates and frees 27 fields of
+ * NeX NeY NeZ 512 doubles.  *ms = mean kernel time over reps launches.  This
+ * is the practical HBM ceiling of a stage with that traffic. */
+int fedg_stage_probe(int mode, int NeX, int NeY, int NeZ, int reps, double *ms);
 
 /* ------------------------------------------------------------------ */
 /* Reference element operators (host).  Replaces the setup of
