@@ -667,6 +667,68 @@ int fedg_heve_cal_tend(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, double* 
   return FEDG_OK;
 }
 
+int fedg_heve_cal_tend_cco(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, double* MOMY_dt, double* MOMZ_dt,
+                           double* RHOT_dt, const double* DDENS_, const double* MOMX_, const double* MOMY_,
+                           const double* MOMZ_, const double* DRHOT_, const double* DPRES_, const double* DENS_hyd,
+                           const double* PRES_hyd, const double* PRES_hyd_ref, const double* THERM_hyd,
+                           const double* CORIOLIS, const double* Rtot, const double* CVtot, const double* CPtot,
+                           const double* DPhydDx, const double* DPhydDy) {
+  (void)PRES_hyd_ref;
+  if (!ctx || !DENS_dt || !MOMX_dt || !MOMY_dt || !MOMZ_dt || !RHOT_dt || !DDENS_ || !MOMX_ || !MOMY_ || !MOMZ_ ||
+      !DRHOT_ || !DPRES_ || !DENS_hyd || !PRES_hyd || !THERM_hyd || !CORIOLIS || !DPhydDx || !DPhydDy)
+    return FEDG_ERR_ARG;
+  TendIn in{};
+  in.q[0] = DDENS_;
+  in.q[1] = DRHOT_;
+  in.q[2] = MOMZ_;
+  in.q[3] = MOMX_;
+  in.q[4] = MOMY_;
+  in.DPRES = DPRES_;
+  in.DENS_hyd = DENS_hyd;
+  in.PRES_hyd = PRES_hyd;
+  in.THERM_hyd = THERM_hyd;
+  in.CORIOLIS = CORIOLIS;
+  in.DPhydDx = DPhydDx;
+  in.DPhydDy = DPhydDy;
+  in.Rtot = Rtot;
+  in.CVtot = CVtot;
+  in.CPtot = CPtot;
+  double* out[5] = {DENS_dt, RHOT_dt, MOMZ_dt, MOMX_dt, MOMY_dt};
+  TRY(launch_heve_cco(ctx, in, out));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return FEDG_OK;
+}
+
+int fedg_heve_add_bnd_contrib(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, double* MOMY_dt, double* MOMZ_dt,
+                              double* RHOT_dt, const double* DDENS_, const double* MOMX_, const double* MOMY_,
+                              const double* MOMZ_, const double* DRHOT_, double* DPRES, const double* DENS_hyd,
+                              const double* PRES_hyd, const double* THERM_hyd, const double* Rtot,
+                              const double* CVtot, const double* CPtot) {
+  if (!ctx || !DENS_dt || !MOMX_dt || !MOMY_dt || !MOMZ_dt || !RHOT_dt || !DDENS_ || !MOMX_ || !MOMY_ || !MOMZ_ ||
+      !DRHOT_ || !DPRES || !DENS_hyd || !PRES_hyd || !THERM_hyd)
+    return FEDG_ERR_ARG;
+  if ((Rtot == nullptr) != (CVtot == nullptr) || (Rtot == nullptr) != (CPtot == nullptr)) {
+    set_error("fedg_heve_add_bnd_contrib: Rtot, CVtot, CPtot are all given or all NULL");
+    return FEDG_ERR_ARG;
+  }
+  TendIn in{};
+  in.q[0] = DDENS_;
+  in.q[1] = DRHOT_;
+  in.q[2] = MOMZ_;
+  in.q[3] = MOMX_;
+  in.q[4] = MOMY_;
+  in.DENS_hyd = DENS_hyd;
+  in.PRES_hyd = PRES_hyd;
+  in.THERM_hyd = THERM_hyd;
+  in.Rtot = Rtot;
+  in.CVtot = CVtot;
+  in.CPtot = CPtot;
+  double* dt[5] = {DENS_dt, RHOT_dt, MOMZ_dt, MOMX_dt, MOMY_dt};
+  TRY(launch_heve_bnd(ctx, in, dt, DPRES));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return FEDG_OK;
+}
+
 int fedg_hevi_cal_tend(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, double* MOMY_dt, double* MOMZ_dt,
                        double* RHOT_dt, const double* DDENS_, const double* MOMX_, const double* MOMY_,
                        const double* MOMZ_, const double* DRHOT_, const double* DPRES_, const double* DENS_hyd,

@@ -19,6 +19,12 @@ struct Side {
 
 // Rusanov flux jump for the five variables at one face node, pre-scaled by
 // 0.5*Fscale (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve_numflux.F90:1024-1135).
+// CS2: the boundary-contribution form of the same flux
+// (..._heve_numflux.F90:242-496), which groups the sound speed as
+// sqrt(Gnn * Cs2) with Cs2 = gamm (P_hyd + DPRES) Gsqrt / GsqrtDens (:351-352,
+// :408-409, :425-431) instead of sqrt(Gnn gamm (P_hyd + DPRES) Gsqrt / GsqrtDens)
+// (:1071-1080); every other operation is the same.
+template <bool CS2 = false>
 __device__ __forceinline__ void rusanov(const Side& M, const Side& P, double nx, double ny, double nz,
                                         double Fscale, double gamm, double df[5]) {
   const double RGsqrtV_M = 1.0 / M.Gsqrt, RGsqrtV_P = 1.0 / P.Gsqrt;
@@ -38,8 +44,16 @@ __device__ __forceinline__ void rusanov(const Side& M, const Side& P, double nx,
   double tmp1 = fabs(nx) + fabs(ny);
   const double Gnn_M = tmp1 + (1.0 * (RGsqrtV_M * RGsqrtV_M) + M.G13 * M.G13 + M.G23 * M.G23) * fabs(nz);
   const double Gnn_P = tmp1 + (1.0 * (RGsqrtV_P * RGsqrtV_P) + P.G13 * P.G13 + P.G23 * P.G23) * fabs(nz);
-  const double aM = sqrt(Gnn_M * gamm * (M.Phyd + M.DP) * M.Gsqrt / GsqrtDens_M) + fabs(Vel_M);
-  const double aP = sqrt(Gnn_P * gamm * (P.Phyd + P.DP) * P.Gsqrt / GsqrtDens_P) + fabs(Vel_P);
+  double aM, aP;
+  if constexpr (CS2) {
+    const double Cs2_M = gamm * (M.Phyd + M.DP) * M.Gsqrt / GsqrtDens_M;
+    const double Cs2_P = gamm * (P.Phyd + P.DP) * P.Gsqrt / GsqrtDens_P;
+    aM = sqrt(Gnn_M * Cs2_M) + fabs(Vel_M);
+    aP = sqrt(Gnn_P * Cs2_P) + fabs(Vel_P);
+  } else {
+    aM = sqrt(Gnn_M * gamm * (M.Phyd + M.DP) * M.Gsqrt / GsqrtDens_M) + fabs(Vel_M);
+    aP = sqrt(Gnn_P * gamm * (P.Phyd + P.DP) * P.Gsqrt / GsqrtDens_P) + fabs(Vel_P);
+  }
   const double alpha = aM > aP ? aM : aP;
 
   tmp1 = Fscale * 0.5;

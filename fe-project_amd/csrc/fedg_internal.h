@@ -111,6 +111,10 @@ struct RkOut {
 
 int launch_heve_dropin(fedg_ctx* ctx, const TendIn& in, double* const out[5]);
 int launch_hevi_explicit(fedg_ctx* ctx, const TendIn& in, double* const out[5]);
+// drop-in tendency split for exchange overlap: interior part, then the face
+// contribution added in place (DPRES: its halo tail is computed first)
+int launch_heve_cco(fedg_ctx* ctx, const TendIn& in, double* const out[5]);
+int launch_heve_bnd(fedg_ctx* ctx, const TendIn& in, double* const dt[5], double* DPRES);
 int launch_heve_fused(fedg_ctx* ctx, const TendIn& in, const RkOut& rk, int use_flat);
 int run_stage_probe(int mode, int NeX, int NeY, int NeZ, int reps, double* ms);
 // FP32 perturbation form of the fused stage (flat box, dry air; at p = 7 the

@@ -32,7 +32,12 @@ namespace {
 // (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi.F90:289-482): horizontal-only
 // dissipation in the face flux, no vertical mass / rhotheta flux, no vertical
 // pressure gradient and no buoyancy (those are the implicit part).
-enum { MODE_DROPIN = 0, MODE_FUSED_GENERAL = 1, MODE_FUSED_FLAT = 2, MODE_DROPIN_HEVI = 3 };
+// MODE_DROPIN_CCO: the interior part of the drop-in tendency
+// (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve.F90:493-678, cal_tend_cco): no
+// face flux and no lift; the face contribution is added afterwards by
+// heve_bnd_kernel (add_bnd_contrib_generalvc), so that a halo exchange can
+// overlap the interior work.
+enum { MODE_DROPIN = 0, MODE_FUSED_GENERAL = 1, MODE_FUSED_FLAT = 2, MODE_DROPIN_HEVI = 3, MODE_DROPIN_CCO = 4 };
 
 // Own-node values staged in LDS for the face phase: DDENS, DRHOT, MOMZ, MOMX,
 // MOMY (VID order), then DENS_hyd, PRES_hyd, THERM_hyd, DPRES.
@@ -43,7 +48,8 @@ __global__ void __launch_bounds__(EPB * N1 * N1 * N1, 2)
 heve_kernel(const DevMesh m, const Consts c, const TendIn in, const RkOut rk) {
   constexpr int Np = N1 * N1 * N1, Nfp = N1 * N1, NfpTot = 6 * Nfp;
   constexpr bool HEVI = MODE == MODE_DROPIN_HEVI;
-  constexpr bool DROPIN = MODE == MODE_DROPIN || HEVI;
+  constexpr bool CCO = MODE == MODE_DROPIN_CCO;
+  constexpr bool DROPIN = MODE == MODE_DROPIN || HEVI || CCO;
   constexpr bool FLAT = MODE == MODE_FUSED_FLAT;
   constexpr bool FUSED = !DROPIN;
   // exterior values are loaded before the first barrier when every face node
@@ -123,7 +129,7 @@ heve_kernel(const DevMesh m, const Consts c, const TendIn in, const RkOut rk) {
     }
   };
   Ext X0;
-  if constexpr (PRELOAD) {
+  if constexpr (PRELOAD && !CCO) {
     if (has_face) {
       exterior(f0, a0, pm0, iP0, bc0);
       load_ext(iP0, X0);
@@ -157,9 +163,11 @@ heve_kernel(const DevMesh m, const Consts c, const TendIn in, const RkOut rk) {
     dpy = ldg(in.DPhydDy, o);
     DP = ldg(in.DPRES, o);
   }
-  double L[6];
+  double L[6] = {0, 0, 0, 0, 0, 0};
+  if constexpr (!CCO) {
 #pragma unroll
-  for (int f = 0; f < 6; ++f) L[f] = ldg(m.Lift, (uint32_t)(p + f * Np));
+    for (int f = 0; f < 6; ++f) L[f] = ldg(m.Lift, (uint32_t)(p + f * Np));
+  }
 #pragma unroll
   for (int v = 0; v < 5; ++v) sS[e][v][p] = q[v];
   sS[e][S_DH][p] = Dh;
@@ -169,7 +177,7 @@ heve_kernel(const DevMesh m, const Consts c, const TendIn in, const RkOut rk) {
   __syncthreads();
 
   // ---- face phase: Rusanov flux at the element's face nodes ----
-  if (valid) {
+  if (valid && !CCO) {
     for (int fp = p; fp < NfpTot; fp += Np) {
       const int f = fp / Nfp, a = fp - f * Nfp;
       const int pm = PRELOAD ? pm0 : face_node<N1>(f, a);
@@ -327,11 +335,17 @@ heve_kernel(const DevMesh m, const Consts c, const TendIn in, const RkOut rk) {
     for (int l = 1; l < N1; ++l) dz = dz + Fz[i + N1 * (j + N1 * l)] * sDt[l + N1 * k];
     __builtin_amdgcn_sched_barrier(0);
     const double* dfv = sDF[e][v];
-    const double lift = L[0] * dfv[i + N1 * k] + L[1] * dfv[Nfp + j + N1 * k] + L[2] * dfv[2 * Nfp + i + N1 * k] +
-                        L[3] * dfv[3 * Nfp + j + N1 * k] + L[4] * dfv[4 * Nfp + i + N1 * j] +
-                        L[5] * dfv[5 * Nfp + i + N1 * j];
-    // HEVI: DENS and RHOT have no vertical flux term, MOMZ no buoyancy
-    const double X = (HEVI && (v == VD || v == VT)) ? E11 * dx + E22 * dy + lift : E11 * dx + E22 * dy + E33 * dz + lift;
+    double X;
+    if constexpr (CCO) {
+      // Div_var5_2: the three line contractions only (..._rhot_heve.F90:607-669)
+      X = E11 * dx + E22 * dy + E33 * dz;
+    } else {
+      const double lift = L[0] * dfv[i + N1 * k] + L[1] * dfv[Nfp + j + N1 * k] + L[2] * dfv[2 * Nfp + i + N1 * k] +
+                          L[3] * dfv[3 * Nfp + j + N1 * k] + L[4] * dfv[4 * Nfp + i + N1 * j] +
+                          L[5] * dfv[5 * Nfp + i + N1 * j];
+      // HEVI: DENS and RHOT have no vertical flux term, MOMZ no buoyancy
+      X = (HEVI && (v == VD || v == VT)) ? E11 * dx + E22 * dy + lift : E11 * dx + E22 * dy + E33 * dz + lift;
+    }
     double T;
     if (v == VD || v == VT) {
       T = -X * RGsqrt;
@@ -1066,6 +1080,61 @@ __global__ void __launch_bounds__(EPB* N1* N1* N1, 2) heve32_kernel(const DevMes
   }
 }
 
+// Face contribution of the drop-in tendency
+// (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve_numflux.F90:242-496,
+// add_bnd_contrib_generalvc): the Rusanov flux jump at every face node from
+// vmapM / vmapP (boundary faces read the halo tail), its lift (Lift_var5,
+// ELM/scale_element_operation_tensorprod3D.F90:2810-2823) and
+// dt -= lift / Gsqrt.  One thread per node; the face nodes are shared out
+// over the element's threads.
+template <int N1, int EPB>
+__global__ void __launch_bounds__(EPB * N1 * N1 * N1) heve_bnd_kernel(const DevMesh m, const Consts c,
+                                                                     const TendIn in, const RkOut rk) {
+  constexpr int Np = N1 * N1 * N1, Nfp = N1 * N1, NfpTot = 6 * Nfp;
+  __shared__ double sDF[EPB][5][NfpTot];
+  const int tid = threadIdx.x;
+  const int e = tid / Np;
+  const int p = tid - e * Np;
+  const int ke = blockIdx.x * EPB + e;
+  const bool valid = ke < m.Ne;
+  const int i = p % N1, j = (p / N1) % N1, k = p / Nfp;
+  const double gamm = c.CPdry / c.CVdry;
+  auto side = [&](uint32_t n, Side& S) {
+    S.Gsqrt = ldg(m.Gsqrt, n); S.G13 = ldg(m.G13, n); S.G23 = ldg(m.G23, n);
+    S.DD = ldg(in.q[VD], n); S.DT = ldg(in.q[VT], n); S.MZ = ldg(in.q[VW], n);
+    S.MX = ldg(in.q[VU], n); S.MY = ldg(in.q[VV], n);
+    S.Dhyd = ldg(in.DENS_hyd, n); S.Phyd = ldg(in.PRES_hyd, n); S.Thyd = ldg(in.THERM_hyd, n);
+    S.DP = ldg(in.DPRES, n);
+  };
+  if (valid) {
+    for (int fp = p; fp < NfpTot; fp += Np) {
+      const uint32_t gf = (uint32_t)fp + (uint32_t)NfpTot * ke;
+      Side M, P;
+      side((uint32_t)m.vmapM[gf], M);
+      side((uint32_t)m.vmapP[gf], P);
+      double df[5];
+      rusanov<true>(M, P, ldg(m.nx, gf), ldg(m.ny, gf), ldg(m.nz, gf), ldg(m.Fscale, gf), gamm, df);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) sDF[e][v][fp] = df[v];
+    }
+  }
+  __syncthreads();
+  if (!valid) return;
+  const uint32_t o = (uint32_t)Np * ke + p;
+  double L[6];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) L[f] = ldg(m.Lift, (uint32_t)(p + f * Np));
+  const double RGsqrt = 1.0 / ldg(m.Gsqrt, o);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const double* dfv = sDF[e][v];
+    const double lift = L[0] * dfv[i + N1 * k] + L[1] * dfv[Nfp + j + N1 * k] + L[2] * dfv[2 * Nfp + i + N1 * k] +
+                        L[3] * dfv[3 * Nfp + j + N1 * k] + L[4] * dfv[4 * Nfp + i + N1 * j] +
+                        L[5] * dfv[5 * Nfp + i + N1 * j];
+    stg(rk.out[v], o, ldg(rk.out[v], o) - RGsqrt * lift);
+  }
+}
+
 template <int N1>
 constexpr int epb_for() {
   return (N1 * N1 * N1 >= 128) ? 1 : (256 / (N1 * N1 * N1));
@@ -1295,6 +1364,50 @@ int launch_heve_dropin(fedg_ctx* ctx, const TendIn& in, double* const out[5]) {
   RkOut rk{};
   for (int v = 0; v < 5; ++v) rk.out[v] = out[v];
   return launch_heve_mode<MODE_DROPIN>(ctx, in, rk);
+}
+
+int launch_heve_cco(fedg_ctx* ctx, const TendIn& in, double* const out[5]) {
+  RkOut rk{};
+  for (int v = 0; v < 5; ++v) rk.out[v] = out[v];
+  return launch_heve_mode<MODE_DROPIN_CCO>(ctx, in, rk);
+}
+
+int launch_heve_bnd(fedg_ctx* ctx, const TendIn& in, double* const dt[5], double* DPRES) {
+  const DevMesh& m = ctx->m;
+  // DPRES of the halo tail from its state and base state (..._heve_numflux.F90:331-335)
+  if (m.nB > 0) {
+    const long base = (long)m.Np * m.Ne;
+    const auto off = [&](const double* f) -> const double* { return f ? f + base : nullptr; };
+    hipLaunchKernelGGL(dpres_kernel, dim3(nblocks(m.nB, 256)), dim3(256), 0, ctx->stream, DPRES + base,
+                       in.q[VT] + base, in.PRES_hyd + base, in.THERM_hyd + base, off(in.Rtot), off(in.CVtot),
+                       off(in.CPtot), (long)m.nB, ctx->c);
+    FEDG_HIP_CHECK(hipGetLastError());
+  }
+  RkOut rk{};
+  for (int v = 0; v < 5; ++v) rk.out[v] = dt[v];
+  TendIn tin = in;
+  tin.DPRES = DPRES;
+  auto go = [&](auto n1c) -> int {
+    constexpr int N1 = decltype(n1c)::value;
+    constexpr int EPB = epb_for<N1>();
+    const int nblk = (m.Ne + EPB - 1) / EPB;
+    hipLaunchKernelGGL((heve_bnd_kernel<N1, EPB>), dim3(nblk), dim3(EPB * N1 * N1 * N1), 0, ctx->stream, m, ctx->c,
+                       tin, rk);
+    FEDG_HIP_CHECK(hipGetLastError());
+    return FEDG_OK;
+  };
+  switch (m.n1) {
+    case 2: return go(std::integral_constant<int, 2>{});
+    case 3: return go(std::integral_constant<int, 3>{});
+    case 4: return go(std::integral_constant<int, 4>{});
+    case 5: return go(std::integral_constant<int, 5>{});
+    case 6: return go(std::integral_constant<int, 6>{});
+    case 7: return go(std::integral_constant<int, 7>{});
+    case 8: return go(std::integral_constant<int, 8>{});
+    default:
+      set_error("polynomial order not built (supported: p = 1..7)");
+      return FEDG_ERR_UNSUPPORTED;
+  }
 }
 
 int launch_hevi_explicit(fedg_ctx* ctx, const TendIn& in, double* const out[5]) {

@@ -75,6 +75,23 @@ def heve_cal_tend(dm: DeviceMesh, out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DPRES, DE
     check(LIB.fedg_heve_cal_tend(dm.handle, *args), "fedg_heve_cal_tend")
 
 
+def heve_cal_tend_cco(dm: DeviceMesh, out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DPRES, DENS_hyd, PRES_hyd, PRES_hyd_ref,
+                      THERM_hyd, CORIOLIS, Rtot, CVtot, CPtot, DPhydDx, DPhydDy):
+    """Interior (volume) part of the HEVE tendency, cal_tend_cco; out as heve_cal_tend."""
+    args = [_p(a) for a in (*out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DPRES, DENS_hyd, PRES_hyd, PRES_hyd_ref, THERM_hyd,
+                            CORIOLIS, Rtot, CVtot, CPtot, DPhydDx, DPhydDy)]
+    check(LIB.fedg_heve_cal_tend_cco(dm.handle, *args), "fedg_heve_cal_tend_cco")
+
+
+def heve_add_bnd_contrib(dm: DeviceMesh, dt, DDENS, MOMX, MOMY, MOMZ, DRHOT, DPRES, DENS_hyd, PRES_hyd, THERM_hyd,
+                         Rtot=None, CVtot=None, CPtot=None):
+    """Face contribution added in place to dt = (DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt)
+    (add_bnd_contrib_generalvc); DPRES's halo tail is recomputed first."""
+    args = [_p(a) for a in (*dt, DDENS, MOMX, MOMY, MOMZ, DRHOT, DPRES, DENS_hyd, PRES_hyd, THERM_hyd, Rtot, CVtot,
+                            CPtot)]
+    check(LIB.fedg_heve_add_bnd_contrib(dm.handle, *args), "fedg_heve_add_bnd_contrib")
+
+
 def hevi_cal_tend(dm: DeviceMesh, out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DPRES, DENS_hyd, PRES_hyd, PRES_hyd_ref,
                   THERM_hyd, CORIOLIS, Rtot, CVtot, CPtot, DPhydDx, DPhydDy):
     """Explicit HEVI tendencies; out = (DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt): device fields."""

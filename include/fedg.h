@@ -212,6 +212,37 @@ int fedg_heve_cal_tend(fedg_ctx *ctx, double *DENS_dt, double *MOMX_dt, double *
                        const double *Rtot, const double *CVtot, const double *CPtot,
                        const double *DPhydDx, const double *DPhydDy);
 
+/* The same tendency split so that the halo exchange can overlap the interior
+ * work (the reference's communication-hiding pair):
+ *  - fedg_heve_cal_tend_cco = atm_dyn_dgm_nonhydro3d_rhot_heve_cal_tend_cco
+ *    (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve.F90:493-678): the volume
+ *    terms only (pointwise fluxes, Div_var5_2, VFilterPM1, Coriolis and
+ *    DPhydDx/Dy), no face flux; needs no halo values.  Same arguments as
+ *    fedg_heve_cal_tend.
+ *  - fedg_heve_add_bnd_contrib = atm_dyn_dgm_nonhydro3d_rhot_heve_add_bnd_contrib_generalvc
+ *    (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve_numflux.F90:242-496): after
+ *    the exchange, DPRES of the halo tail from the halo's DRHOT and base state
+ *    (written to DPRES), then the Rusanov flux at every face node (vmapM /
+ *    vmapP), its lift (Lift_var5) and dt -= lift / Gsqrt for the five
+ *    tendencies in place (PRGVAR_dt(:,:,1..5) of the reference =
+ *    DENS_dt, RHOT_dt, MOMZ_dt, MOMX_dt, MOMY_dt).  Rtot, CVtot, CPtot are all
+ *    given (halo included) or all NULL (dry-air constants).
+ * cal_tend_cco followed by add_bnd_contrib equals fedg_heve_cal_tend up to the
+ * rounding of the regrouped sums (the reference's own two paths differ the
+ * same way). */
+int fedg_heve_cal_tend_cco(fedg_ctx *ctx, double *DENS_dt, double *MOMX_dt, double *MOMY_dt,
+                           double *MOMZ_dt, double *RHOT_dt, const double *DDENS_, const double *MOMX_,
+                           const double *MOMY_, const double *MOMZ_, const double *DRHOT_,
+                           const double *DPRES_, const double *DENS_hyd, const double *PRES_hyd,
+                           const double *PRES_hyd_ref, const double *THERM_hyd, const double *CORIOLIS,
+                           const double *Rtot, const double *CVtot, const double *CPtot,
+                           const double *DPhydDx, const double *DPhydDy);
+int fedg_heve_add_bnd_contrib(fedg_ctx *ctx, double *DENS_dt, double *MOMX_dt, double *MOMY_dt,
+                              double *MOMZ_dt, double *RHOT_dt, const double *DDENS_, const double *MOMX_,
+                              const double *MOMY_, const double *MOMZ_, const double *DRHOT_, double *DPRES,
+                              const double *DENS_hyd, const double *PRES_hyd, const double *THERM_hyd,
+                              const double *Rtot, const double *CVtot, const double *CPtot);
+
 /* atm_dyn_dgm_nonhydro3d_rhot_hevi_cal_tend
  * (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi.F90:289-482 with its numerical
  * flux FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi_numflux.F90:232-416): the

@@ -137,11 +137,14 @@ void oracle_apply_bc(const oracle_mesh *m, double *DDENS, double *MOMX, double *
 #include "oracle_ops.h"
 
 /* ---------------------------------------------------------------------- */
-/* FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve_numflux.F90:946-1138 */
+/* FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve_numflux.F90:946-1138.  cs2 != 0:
+ * the flux of add_bnd_contrib_generalvc (:242-496), identical except that the
+ * sound speed is grouped as sqrt(Gnn * Cs2), Cs2 = gamm (P_hyd + DPRES) Gsqrt
+ * / GsqrtDens (:351-352, :408-409, :425-431). */
 static void heve_numflux_range(const oracle_mesh *m, int ke_s, int ke_e, double *del_flux,
                                const double *DDENS_, const double *MOMX_, const double *MOMY_,
                                const double *MOMZ_, const double *DRHOT_, const double *DPRES,
-                               const double *DENS_hyd, const double *PRES_hyd, const double *THERM_hyd) {
+                               const double *DENS_hyd, const double *PRES_hyd, const double *THERM_hyd, int cs2) {
   const int NfpTot = m->NfpTot, Ne = m->Ne;
   const long NpNeA = (long)m->Np * m->NeA;
   const long nfall = (long)NfpTot * Ne;
@@ -180,8 +183,16 @@ static void heve_numflux_range(const oracle_mesh *m, int ke_s, int ke_e, double 
       double tmp1 = fabs(nx) + fabs(ny);
       double Gnn_M = tmp1 + (1.0 * (RGsqrtV_M * RGsqrtV_M) + G13_M * G13_M + G23_M * G23_M) * fabs(nz);
       double Gnn_P = tmp1 + (1.0 * (RGsqrtV_P * RGsqrtV_P) + G13_P * G13_P + G23_P * G23_P) * fabs(nz);
-      double aM = sqrt(Gnn_M * gamm * (Phyd_M + dpres_M) * Gsqrt_M / GsqrtDens_M) + fabs(Vel_M);
-      double aP = sqrt(Gnn_P * gamm * (Phyd_P + dpres_P) * Gsqrt_P / GsqrtDens_P) + fabs(Vel_P);
+      double aM, aP;
+      if (cs2) {
+        double Cs2_M = gamm * (Phyd_M + dpres_M) * Gsqrt_M / GsqrtDens_M;
+        double Cs2_P = gamm * (Phyd_P + dpres_P) * Gsqrt_P / GsqrtDens_P;
+        aM = sqrt(Gnn_M * Cs2_M) + fabs(Vel_M);
+        aP = sqrt(Gnn_P * Cs2_P) + fabs(Vel_P);
+      } else {
+        aM = sqrt(Gnn_M * gamm * (Phyd_M + dpres_M) * Gsqrt_M / GsqrtDens_M) + fabs(Vel_M);
+        aP = sqrt(Gnn_P * gamm * (Phyd_P + dpres_P) * Gsqrt_P / GsqrtDens_P) + fabs(Vel_P);
+      }
       double alpha = aM > aP ? aM : aP;
 
       double *df = del_flux + (long)NfpTot * 5 * ke;
@@ -212,13 +223,15 @@ void oracle_heve_numflux(const oracle_mesh *m, double *del_flux,
                          const double *MOMZ_, const double *DRHOT_, const double *DPRES,
                          const double *DENS_hyd, const double *PRES_hyd, const double *THERM_hyd) {
   heve_numflux_range(m, 0, m->Ne, del_flux, DDENS_, MOMX_, MOMY_, MOMZ_, DRHOT_, DPRES, DENS_hyd, PRES_hyd,
-                     THERM_hyd);
+                     THERM_hyd, 0);
 }
 
 /* ---------------------------------------------------------------------- */
 /* FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve.F90:372-484 (element loop),
  * with Div_var5 = ELM/scale_element_operation_tensorprod3D.F90:2760-2783 and
- * VFilterPM1 = :2828-2839. del_flux holds the numflux of elements [ke_s, ke_e). */
+ * VFilterPM1 = :2828-2839. del_flux holds the numflux of elements [ke_s, ke_e).
+ * del_flux NULL: the interior form cal_tend_cco (..._rhot_heve.F90:493-678),
+ * Div_var5_2 (ELM/..._tensorprod3D.F90:2788-2805) without the lift term. */
 static void heve_elements(const oracle_mesh *m, int ke_s, int ke_e, const double *del_flux,
                           double *DENS_dt, double *MOMX_dt, double *MOMY_dt, double *MOMZ_dt, double *RHOT_dt,
                           const double *DDENS_, const double *MOMX_, const double *MOMY_,
@@ -278,24 +291,36 @@ static void heve_elements(const oracle_mesh *m, int ke_s, int ke_e, const double
         FL(p, 2, MOMY_VID) = FL(p, 2, DENS_VID) * v_ + GsqrtDPRES_ * GI3_2[o + p];
       }
       /* Div_var5: Lift first, then the three line contractions, per variable */
-      const double *dfe = del_flux + (long)NfpTot * 5 * ke;
-      for (int v = 0; v < 5; ++v) op_lift(n1, m->Lift_mat, dfe + (long)NfpTot * v, &DF(0, 3, v));
+      if (del_flux) {
+        const double *dfe = del_flux + (long)NfpTot * 5 * ke;
+        for (int v = 0; v < 5; ++v) op_lift(n1, m->Lift_mat, dfe + (long)NfpTot * v, &DF(0, 3, v));
+      }
       for (int v = 0; v < 5; ++v) {
         op_dirX(n1, m->D1D, &FL(0, 0, v), &DF(0, 0, v));
         op_dirY(n1, m->D1D, &FL(0, 1, v), &DF(0, 1, v));
         op_dirZ(n1, m->D1D, &FL(0, 2, v), &DF(0, 2, v));
       }
       const double *E11 = m->Escale + o, *E22 = m->Escale + o + NpNe * 4, *E33 = m->Escale + o + NpNe * 8;
-      for (int p = 0; p < Np; ++p) {
-        DENS_dt[o + p] = -(E11[p] * DF(p, 0, DENS_VID) + E22[p] * DF(p, 1, DENS_VID) + E33[p] * DF(p, 2, DENS_VID) +
-                           DF(p, 3, DENS_VID)) * RGsqrt[p];
-        RHOT_dt[o + p] = -(E11[p] * DF(p, 0, RHOT_VID) + E22[p] * DF(p, 1, RHOT_VID) + E33[p] * DF(p, 2, RHOT_VID) +
-                           DF(p, 3, RHOT_VID)) * RGsqrt[p];
+      if (del_flux) {
+        for (int p = 0; p < Np; ++p) {
+          DENS_dt[o + p] = -(E11[p] * DF(p, 0, DENS_VID) + E22[p] * DF(p, 1, DENS_VID) +
+                             E33[p] * DF(p, 2, DENS_VID) + DF(p, 3, DENS_VID)) * RGsqrt[p];
+          RHOT_dt[o + p] = -(E11[p] * DF(p, 0, RHOT_VID) + E22[p] * DF(p, 1, RHOT_VID) +
+                             E33[p] * DF(p, 2, RHOT_VID) + DF(p, 3, RHOT_VID)) * RGsqrt[p];
+        }
+      } else { /* ..._rhot_heve.F90:614-626 */
+        for (int p = 0; p < Np; ++p) {
+          DENS_dt[o + p] = -(E11[p] * DF(p, 0, DENS_VID) + E22[p] * DF(p, 1, DENS_VID) +
+                             E33[p] * DF(p, 2, DENS_VID)) * RGsqrt[p];
+          RHOT_dt[o + p] = -(E11[p] * DF(p, 0, RHOT_VID) + E22[p] * DF(p, 1, RHOT_VID) +
+                             E33[p] * DF(p, 2, RHOT_VID)) * RGsqrt[p];
+        }
       }
       op_dirZ_tr(n1, m->IntrpMat_tr, DDENS_ + o, drho);
       for (int p = 0; p < Np; ++p) {
-        MOMZ_dt[o + p] = -(E11[p] * DF(p, 0, MOMZ_VID) + E22[p] * DF(p, 1, MOMZ_VID) + E33[p] * DF(p, 2, MOMZ_VID) +
-                           DF(p, 3, MOMZ_VID)) * RGsqrt[p] - Grav * drho[p];
+        double X = E11[p] * DF(p, 0, MOMZ_VID) + E22[p] * DF(p, 1, MOMZ_VID) + E33[p] * DF(p, 2, MOMZ_VID);
+        if (del_flux) X = X + DF(p, 3, MOMZ_VID);
+        MOMZ_dt[o + p] = -X * RGsqrt[p] - Grav * drho[p];
       }
       for (int p = 0; p < Np; ++p) {
         double cor = CORIOLIS[(m->IndexH2Dto3D[p] - 1) + (long)Nfp_v * ke2d];
@@ -303,10 +328,14 @@ static void heve_elements(const oracle_mesh *m, int ke_s, int ke_e, const double
         MOMY_dt[o + p] = -DPhydDy[o + p] - cor * MOMX_[o + p];
       }
       for (int p = 0; p < Np; ++p) {
-        MOMX_dt[o + p] = MOMX_dt[o + p] - (E11[p] * DF(p, 0, MOMX_VID) + E22[p] * DF(p, 1, MOMX_VID) +
-                                           E33[p] * DF(p, 2, MOMX_VID) + DF(p, 3, MOMX_VID)) * RGsqrt[p];
-        MOMY_dt[o + p] = MOMY_dt[o + p] - (E11[p] * DF(p, 0, MOMY_VID) + E22[p] * DF(p, 1, MOMY_VID) +
-                                           E33[p] * DF(p, 2, MOMY_VID) + DF(p, 3, MOMY_VID)) * RGsqrt[p];
+        double XU = E11[p] * DF(p, 0, MOMX_VID) + E22[p] * DF(p, 1, MOMX_VID) + E33[p] * DF(p, 2, MOMX_VID);
+        double XV = E11[p] * DF(p, 0, MOMY_VID) + E22[p] * DF(p, 1, MOMY_VID) + E33[p] * DF(p, 2, MOMY_VID);
+        if (del_flux) {
+          XU = XU + DF(p, 3, MOMX_VID);
+          XV = XV + DF(p, 3, MOMY_VID);
+        }
+        MOMX_dt[o + p] = MOMX_dt[o + p] - XU * RGsqrt[p];
+        MOMY_dt[o + p] = MOMY_dt[o + p] - XV * RGsqrt[p];
       }
 #undef FL
 #undef DF
@@ -336,7 +365,7 @@ void oracle_heve_cal_tend_range(const oracle_mesh *m, int ke_s, int ke_e,
   (void)CPtot; /* part of the interface; the dry-air constants are used instead */
   double *del_flux = (double *)malloc(sizeof(double) * (size_t)m->NfpTot * 5 * m->Ne);
   heve_numflux_range(m, ke_s, ke_e, del_flux, DDENS_, MOMX_, MOMY_, MOMZ_, DRHOT_, DPRES_, DENS_hyd, PRES_hyd,
-                     THERM_hyd);
+                     THERM_hyd, 0);
   heve_elements(m, ke_s, ke_e, del_flux, DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt, DDENS_, MOMX_, MOMY_,
                 MOMZ_, DRHOT_, DPRES_, DENS_hyd, THERM_hyd, CORIOLIS, DPhydDx, DPhydDy);
   free(del_flux);
@@ -353,6 +382,64 @@ void oracle_heve_cal_tend(const oracle_mesh *m,
   oracle_heve_cal_tend_range(m, 0, m->Ne, DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt, DDENS_, MOMX_, MOMY_,
                              MOMZ_, DRHOT_, DPRES_, DENS_hyd, PRES_hyd, PRES_hyd_ref, THERM_hyd, CORIOLIS, Rtot,
                              CVtot, CPtot, DPhydDx, DPhydDy);
+}
+
+/* FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve.F90:493-678 (cal_tend_cco):
+ * the interior part of the tendency, no face flux */
+void oracle_heve_cal_tend_cco(const oracle_mesh *m,
+                              double *DENS_dt, double *MOMX_dt, double *MOMY_dt, double *MOMZ_dt, double *RHOT_dt,
+                              const double *DDENS_, const double *MOMX_, const double *MOMY_,
+                              const double *MOMZ_, const double *DRHOT_, const double *DPRES_,
+                              const double *DENS_hyd, const double *THERM_hyd, const double *CORIOLIS,
+                              const double *DPhydDx, const double *DPhydDy) {
+  heve_elements(m, 0, m->Ne, NULL, DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt, DDENS_, MOMX_, MOMY_, MOMZ_, DRHOT_,
+                DPRES_, DENS_hyd, THERM_hyd, CORIOLIS, DPhydDx, DPhydDy);
+}
+
+/* FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve_numflux.F90:242-496
+ * (add_bnd_contrib_generalvc): DPRES of the halo tail (:331-335), the face
+ * flux with the Cs2 grouping, Lift_var5 (ELM/..._tensorprod3D.F90:2810-2823)
+ * and dt -= RGsqrt * Lift (:488-492).  Rtot NULL: the dry-air constants. */
+void oracle_heve_add_bnd_contrib(const oracle_mesh *m,
+                                 double *DENS_dt, double *MOMX_dt, double *MOMY_dt, double *MOMZ_dt, double *RHOT_dt,
+                                 const double *DDENS_, const double *MOMX_, const double *MOMY_,
+                                 const double *MOMZ_, const double *DRHOT_, double *DPRES,
+                                 const double *DENS_hyd, const double *PRES_hyd, const double *THERM_hyd,
+                                 const double *Rtot, const double *CVtot, const double *CPtot) {
+  const int n1 = m->n1, Np = m->Np, NfpTot = m->NfpTot, Ne = m->Ne;
+  const double rP0 = 1.0 / m->PRES00;
+  const long h0 = (long)Np * Ne;
+  for (long i = h0; i < h0 + m->nB; ++i) {
+    double Rt = Rtot ? Rtot[i] : m->Rdry, Cv = CVtot ? CVtot[i] : m->CVdry, Cp = CPtot ? CPtot[i] : m->CPdry;
+    double PRES = m->PRES00 * opow(Rt * rP0 * (THERM_hyd[i] + DRHOT_[i]), Cp / Cv);
+    DPRES[i] = PRES - PRES_hyd[i];
+  }
+  double *del_flux = (double *)malloc(sizeof(double) * (size_t)NfpTot * 5 * Ne);
+  heve_numflux_range(m, 0, Ne, del_flux, DDENS_, MOMX_, MOMY_, MOMZ_, DRHOT_, DPRES, DENS_hyd, PRES_hyd, THERM_hyd,
+                     1);
+  double *dt[5];
+  dt[DENS_VID] = DENS_dt;
+  dt[RHOT_VID] = RHOT_dt;
+  dt[MOMZ_VID] = MOMZ_dt;
+  dt[MOMX_VID] = MOMX_dt;
+  dt[MOMY_VID] = MOMY_dt;
+#pragma omp parallel
+  {
+    double *Lift = (double *)malloc(sizeof(double) * Np);
+#pragma omp for schedule(static)
+    for (int ke = 0; ke < Ne; ++ke) {
+      const long o = (long)Np * ke;
+      for (int v = 0; v < 5; ++v) {
+        op_lift(n1, m->Lift_mat, del_flux + (long)NfpTot * (v + 5 * ke), Lift);
+        for (int p = 0; p < Np; ++p) {
+          double RGsqrt = 1.0 / m->Gsqrt[o + p];
+          dt[v][o + p] = dt[v][o + p] - RGsqrt * Lift[p];
+        }
+      }
+    }
+    free(Lift);
+  }
+  free(del_flux);
 }
 
 /* ---------------------------------------------------------------------- */

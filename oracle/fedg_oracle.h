@@ -97,6 +97,22 @@ void oracle_heve_cal_tend(const oracle_mesh *m,
                           const double *Rtot, const double *CVtot, const double *CPtot,
                           const double *DPhydDx, const double *DPhydDy);
 
+/* cal_tend_cco (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve.F90:493-678) and
+ * add_bnd_contrib_generalvc (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve_numflux.F90:242-496);
+ * the second writes DPRES's halo tail and adds to the five tendencies */
+void oracle_heve_cal_tend_cco(const oracle_mesh *m,
+                              double *DENS_dt, double *MOMX_dt, double *MOMY_dt, double *MOMZ_dt, double *RHOT_dt,
+                              const double *DDENS_, const double *MOMX_, const double *MOMY_,
+                              const double *MOMZ_, const double *DRHOT_, const double *DPRES_,
+                              const double *DENS_hyd, const double *THERM_hyd, const double *CORIOLIS,
+                              const double *DPhydDx, const double *DPhydDy);
+void oracle_heve_add_bnd_contrib(const oracle_mesh *m,
+                                 double *DENS_dt, double *MOMX_dt, double *MOMY_dt, double *MOMZ_dt, double *RHOT_dt,
+                                 const double *DDENS_, const double *MOMX_, const double *MOMY_,
+                                 const double *MOMZ_, const double *DRHOT_, double *DPRES,
+                                 const double *DENS_hyd, const double *PRES_hyd, const double *THERM_hyd,
+                                 const double *Rtot, const double *CVtot, const double *CPtot);
+
 /* FDS/scale_atm_dyn_dgm_nonhydro3d_common.F90:624-777, hydrostatic pressure gradient */
 void oracle_calc_phyd_hgrad(const oracle_mesh *m, double *DPhydDx, double *DPhydDy,
                             const double *PRES_hyd, const double *PRES_hyd_ref);

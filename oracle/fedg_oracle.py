@@ -50,6 +50,8 @@ def _load():
     lib.oracle_heve_numflux.argtypes = [M] + [dp] * 10
     lib.oracle_heve_cal_tend.argtypes = [M] + [dp] * 21
     lib.oracle_heve_cal_tend_range.argtypes = [M, C.c_int, C.c_int] + [dp] * 21
+    lib.oracle_heve_cal_tend_cco.argtypes = [M] + [dp] * 16
+    lib.oracle_heve_add_bnd_contrib.argtypes = [M] + [dp] * 17
     lib.oracle_calc_phyd_hgrad.argtypes = [M] + [dp] * 4
     lib.oracle_rk_tab_get.argtypes = [C.c_char_p, C.c_double, C.POINTER(RkTabC)]
     lib.oracle_rk_advance.argtypes = [C.POINTER(RkTabC), C.c_int, C.c_long] + [dp] * 5 + [C.c_long]
@@ -171,6 +173,23 @@ class OracleMesh:
         else:
             LIB.oracle_heve_cal_tend_range(self.ref, int(ke_range[0]), int(ke_range[1]), *args)
         return out
+
+    def cal_tend_cco(self, q, DPRES, bs, DPhydDx, DPhydDy):
+        """Interior tendencies of cal_tend_cco (DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt)."""
+        m = self.mesh
+        out = [m.field() for _ in range(5)]
+        LIB.oracle_heve_cal_tend_cco(self.ref, *[_d(a) for a in out], _d(q[0]), _d(q[3]), _d(q[4]), _d(q[2]),
+                                     _d(q[1]), _d(DPRES), _d(bs.DENS_hyd), _d(bs.THERM_hyd), _d(bs.CORIOLIS),
+                                     _d(DPhydDx), _d(DPhydDy))
+        return out
+
+    def add_bnd_contrib(self, dt, q, DPRES, bs, moist=False):
+        """add_bnd_contrib_generalvc in place on dt = (DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt); writes
+        DPRES's halo tail.  moist: Rtot/CVtot/CPtot from bs, else the dry constants."""
+        ext = [_d(bs.Rtot), _d(bs.CVtot), _d(bs.CPtot)] if moist else [None, None, None]
+        LIB.oracle_heve_add_bnd_contrib(self.ref, *[_d(a) for a in dt], _d(q[0]), _d(q[3]), _d(q[4]), _d(q[2]),
+                                        _d(q[1]), _d(DPRES), _d(bs.DENS_hyd), _d(bs.PRES_hyd), _d(bs.THERM_hyd),
+                                        *ext)
 
     # --- HEVI ---------------------------------------------------------------
     def hevi_numflux(self, q, DPRES, bs):

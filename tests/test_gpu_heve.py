@@ -127,6 +127,39 @@ def test_dropin_cal_tend_bitwise(setup):
         assert np.array_equal(g, b[:n]), f"{name}: max abs diff {np.max(np.abs(g - b[:n]))}"
 
 
+@pytest.mark.parametrize("moist", [False, True], ids=["dry", "moist-arrays"])
+def test_dropin_cco_and_bnd_contrib_bitwise(setup, moist):
+    """cal_tend_cco, then add_bnd_contrib_generalvc on its output (with the
+    halo DPRES recomputed), against the oracle bit for bit."""
+    s = setup
+    mesh, bs, om = s["mesh"], s["bs"], s["om"]
+    q, DPRES = perturbed_state(mesh, bs, s["case"], om, s["Dx"], s["Dy"], seed=13)
+    n = mesh.Np * mesh.Ne
+    ref = om.cal_tend_cco(q, DPRES, bs, s["Dx"], s["Dy"])
+    f = lambda a: _dev(mesh, a)
+    out = [f(mesh.field()) for _ in range(5)]
+    fedg.heve_cal_tend_cco(s["dm"], out, f(q[0]), f(q[3]), f(q[4]), f(q[2]), f(q[1]), f(DPRES), f(bs.DENS_hyd),
+                           f(bs.PRES_hyd), f(bs.PRES_hyd_ref), f(bs.THERM_hyd), f(bs.CORIOLIS), f(bs.Rtot),
+                           f(bs.CVtot), f(bs.CPtot), f(s["Dx"]), f(s["Dy"]))
+    names = ("DENS_dt", "MOMX_dt", "MOMY_dt", "MOMZ_dt", "RHOT_dt")
+    for name, a, b in zip(names, out, ref):
+        g = a.download()[:n]
+        assert np.array_equal(g, b[:n]), f"cco {name}: max abs diff {np.max(np.abs(g - b[:n]))}"
+    # the halo tail of DPRES is recomputed by both sides: start it from garbage
+    Dh = DPRES.copy()
+    Dh[n:] = 12345.0
+    Dg = f(Dh)
+    Do = Dh.copy()
+    om.add_bnd_contrib(ref, q, Do, bs, moist=moist)
+    ext = (f(bs.Rtot), f(bs.CVtot), f(bs.CPtot)) if moist else (None, None, None)
+    fedg.heve_add_bnd_contrib(s["dm"], out, f(q[0]), f(q[3]), f(q[4]), f(q[2]), f(q[1]), Dg, f(bs.DENS_hyd),
+                              f(bs.PRES_hyd), f(bs.THERM_hyd), *ext)
+    for name, a, b in zip(names, out, ref):
+        g = a.download()[:n]
+        assert np.array_equal(g, b[:n]), f"bnd {name}: max abs diff {np.max(np.abs(g - b[:n]))}"
+    assert np.array_equal(Dg.download()[:n + mesh.nB], Do[:n + mesh.nB])
+
+
 @pytest.mark.parametrize("scheme", ["ERK_4s4o", "IMEX_ARK324", "IMEX_ARK232"])
 def test_tint_advance_bitwise(setup, scheme):
     s = setup
