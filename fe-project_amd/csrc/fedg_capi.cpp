@@ -729,6 +729,39 @@ int fedg_heve_add_bnd_contrib(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, d
   return FEDG_OK;
 }
 
+int fedg_trcadvect_cal_tend(fedg_ctx* ctx, double* QTRC_dt, const double* QTRC_, const double* MOMX_,
+                            const double* MOMY_, const double* MOMZ_, const double* alphDENS_M,
+                            const double* alphDENS_P, const double* fct_coef, const double* RHOQ_tp,
+                            const double* J) {
+  if (!ctx || !QTRC_dt || !QTRC_ || !MOMX_ || !MOMY_ || !MOMZ_ || !alphDENS_M || !alphDENS_P || !fct_coef ||
+      !RHOQ_tp || !J)
+    return FEDG_ERR_ARG;
+  // FaceIntMat (scale_atm_dyn_dgm_trcadvect3d_heve.F90:75-137): the face
+  // node weights are products of the 1D LGL weights of the element's order
+  RefElem e;
+  try {
+    build_refelem(ctx->m.n1 - 1, e);
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+    return FEDG_ERR_ARG;
+  }
+  TrcLaunch t{};
+  t.dt = QTRC_dt;
+  t.Q = QTRC_;
+  t.MX = MOMX_;
+  t.MY = MOMY_;
+  t.MZ = MOMZ_;
+  t.aM = alphDENS_M;
+  t.aP = alphDENS_P;
+  t.R = fct_coef;
+  t.tp = RHOQ_tp;
+  t.J = J;
+  for (int l = 0; l < ctx->m.n1; ++l) t.w[l] = e.w1d[l];
+  TRY(launch_trcadv(ctx, t));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return FEDG_OK;
+}
+
 int fedg_hevi_cal_tend(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, double* MOMY_dt, double* MOMZ_dt,
                        double* RHOT_dt, const double* DDENS_, const double* MOMX_, const double* MOMY_,
                        const double* MOMZ_, const double* DRHOT_, const double* DPRES_, const double* DENS_hyd,

@@ -115,6 +115,13 @@ int launch_hevi_explicit(fedg_ctx* ctx, const TendIn& in, double* const out[5]);
 // contribution added in place (DPRES: its halo tail is computed first)
 int launch_heve_cco(fedg_ctx* ctx, const TendIn& in, double* const out[5]);
 int launch_heve_bnd(fedg_ctx* ctx, const TendIn& in, double* const dt[5], double* DPRES);
+// tracer advection tendency (HEVE)
+struct TrcLaunch {
+  double* dt;
+  const double *Q, *MX, *MY, *MZ, *aM, *aP, *R, *tp, *J;
+  double w[8];  // 1D LGL integration weights
+};
+int launch_trcadv(fedg_ctx* ctx, const TrcLaunch& t);
 int launch_heve_fused(fedg_ctx* ctx, const TendIn& in, const RkOut& rk, int use_flat);
 int run_stage_probe(int mode, int NeX, int NeY, int NeZ, int reps, double* ms);
 // FP32 perturbation form of the fused stage (flat box, dry air; at p = 7 the

@@ -1135,6 +1135,118 @@ __global__ void __launch_bounds__(EPB * N1 * N1 * N1) heve_bnd_kernel(const DevM
   }
 }
 
+// Tracer advection tendency, HEVE (FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:149-231
+// with its face flux get_delflux_generalhvc, :554-674): central flux of the
+// mass flux with the density's dissipation coefficients alphDENS_M/P, the
+// FCT coefficient of the upwind side chosen by the sign of the net outward
+// flux of each face (FaceIntMat of the LGL face weights, :75-137), Div with
+// the lift, then QTRC_dt = -(...)/Gsqrt + RHOQ_tp.  One thread per node.
+struct TrcArgs {
+  double* dt;                       // QTRC_dt (Np,NeA)
+  const double *Q, *MX, *MY, *MZ;   // QTRC_ and the mass fluxes (Np,NeA)
+  const double *aM, *aP;            // alphDENS_M / _P (NfpTot,Ne)
+  const double* R;                  // fct_coef (Np,NeA)
+  const double* tp;                 // RHOQ_tp (Np,NeA)
+  const double* J;                  // (Np,Ne)
+  double w[8];                      // 1D LGL integration weights
+};
+template <int N1, int EPB>
+__global__ void __launch_bounds__(EPB * N1 * N1 * N1) trcadv_kernel(const DevMesh m, const TrcArgs a) {
+  constexpr int Np = N1 * N1 * N1, Nfp = N1 * N1, NfpTot = 6 * Nfp;
+  __shared__ double sDt[N1 * N1];
+  __shared__ double sF[EPB][3][Np];
+  __shared__ double sNF[EPB][NfpTot];  // numflux
+  __shared__ double sQF[EPB][NfpTot];  // QTRC_M * MomFlxM
+  __shared__ double sRP[EPB][NfpTot], sRM[EPB][NfpTot];
+  __shared__ double sJF[EPB][NfpTot];  // J(iM) * Fscale * numflux
+  __shared__ double sOut[EPB][6];      // net outward flux of each face
+  __shared__ double sDF[EPB][NfpTot];  // del_flux
+  const int tid = threadIdx.x;
+  const int e = tid / Np;
+  const int p = tid - e * Np;
+  const int ke = blockIdx.x * EPB + e;
+  const bool valid = ke < m.Ne;
+  const int i = p % N1, j = (p / N1) % N1, k = p / Nfp;
+  for (int t = tid; t < N1 * N1; t += blockDim.x) sDt[(t % N1) * N1 + t / N1] = m.D1D[t];
+  const int ke2d = valid ? m.EMap3Dto2D[ke] : 0;
+  if (valid) {
+    for (int fp = p; fp < NfpTot; fp += Np) {
+      const uint32_t gf = (uint32_t)fp + (uint32_t)NfpTot * ke;
+      const uint32_t iM = (uint32_t)m.vmapM[gf], iP = (uint32_t)m.vmapP[gf];
+      const int f = fp / Nfp, an = fp - f * Nfp;
+      const double GsH = ldg(m.GsqrtH, (uint32_t)(face_node<N1>(f, an) % Nfp) + (uint32_t)Nfp * ke2d);
+      const double Gsqrt_M = ldg(m.Gsqrt, iM), Gsqrt_P = ldg(m.Gsqrt, iP);
+      const double GsqrtV_M = Gsqrt_M / GsH, GsqrtV_P = Gsqrt_P / GsH;
+      const double G13_M = ldg(m.G13, iM), G13_P = ldg(m.G13, iP);
+      const double G23_M = ldg(m.G23, iM), G23_P = ldg(m.G23, iP);
+      const double Q_M = ldg(a.Q, iM), Q_P = ldg(a.Q, iP);
+      const double GMX_M = Gsqrt_M * ldg(a.MX, iM), GMX_P = Gsqrt_P * ldg(a.MX, iP);
+      const double GMY_M = Gsqrt_M * ldg(a.MY, iM), GMY_P = Gsqrt_P * ldg(a.MY, iP);
+      const double GMZ_M = Gsqrt_M * ldg(a.MZ, iM), GMZ_P = Gsqrt_P * ldg(a.MZ, iP);
+      const double nx = ldg(m.nx, gf), ny = ldg(m.ny, gf), nz = ldg(m.nz, gf);
+      const double MF_M = GMX_M * nx + GMY_M * ny + ((GMZ_M / GsqrtV_M + G13_M * GMX_M + G23_M * GMY_M) * nz);
+      const double MF_P = GMX_P * nx + GMY_P * ny + ((GMZ_P / GsqrtV_P + G13_P * GMX_P + G23_P * GMY_P) * nz);
+      const double nf = 0.5 * ((Q_P * MF_P + Q_M * MF_M) - ldg(a.aP, gf) * Q_P + ldg(a.aM, gf) * Q_M);
+      sNF[e][fp] = nf;
+      sQF[e][fp] = Q_M * MF_M;
+      sRP[e][fp] = ldg(a.R, iP);
+      sRM[e][fp] = ldg(a.R, iM);
+      sJF[e][fp] = ldg(a.J, iM) * ldg(m.Fscale, gf) * nf;
+    }
+  }
+  __syncthreads();
+  // FaceIntMat x (J Fscale numflux): row f sums its face's nodes in order from 0
+  if (valid && p < 6) {
+    double s = 0.0;
+    for (int an = 0; an < Nfp; ++an) s = s + (a.w[an % N1] * a.w[an / N1]) * sJF[e][p * Nfp + an];
+    sOut[e][p] = s;
+  }
+  __syncthreads();
+  if (valid) {
+    for (int fp = p; fp < NfpTot; fp += Np) {
+      const uint32_t gf = (uint32_t)fp + (uint32_t)NfpTot * ke;
+      const double RP = sRP[e][fp], RM = sRM[e][fp];
+      const double sg = copysign(1.0, sOut[e][fp / Nfp]);
+      sDF[e][fp] = ldg(m.Fscale, gf) * (sNF[e][fp] * 0.5 * (RP + RM - (RP - RM) * sg) - sQF[e][fp]);
+    }
+  }
+  // volume flux
+  const uint32_t o = (uint32_t)Np * (valid ? ke : 0) + p;
+  double Gs = 1.0;
+  if (valid) {
+    Gs = ldg(m.Gsqrt, o);
+    const double RGsqrt = 1.0 / Gs;
+    const double RGsqrtV = ldg(m.GsqrtH, (uint32_t)(i + N1 * j) + (uint32_t)Nfp * ke2d) * RGsqrt;
+    const double Qo = ldg(a.Q, o), mx = ldg(a.MX, o), my = ldg(a.MY, o), mz = ldg(a.MZ, o);
+    sF[e][0][p] = Gs * mx * Qo;
+    sF[e][1][p] = Gs * my * Qo;
+    sF[e][2][p] = Gs * (mz * RGsqrtV + ldg(m.G13, o) * mx + ldg(m.G23, o) * my) * Qo;
+  }
+  __syncthreads();
+  if (!valid) return;
+  const double* Fx = sF[e][0];
+  const double* Fy = sF[e][1];
+  const double* Fz = sF[e][2];
+  double dx = sDt[N1 * i] * Fx[N1 * (j + N1 * k)];
+#pragma unroll
+  for (int l = 1; l < N1; ++l) dx = dx + sDt[l + N1 * i] * Fx[l + N1 * (j + N1 * k)];
+  double dy = Fy[i + N1 * (N1 * k)] * sDt[N1 * j];
+#pragma unroll
+  for (int l = 1; l < N1; ++l) dy = dy + Fy[i + N1 * (l + N1 * k)] * sDt[l + N1 * j];
+  double dz = Fz[i + N1 * j] * sDt[N1 * k];
+#pragma unroll
+  for (int l = 1; l < N1; ++l) dz = dz + Fz[i + N1 * (j + N1 * l)] * sDt[l + N1 * k];
+  double L[6];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) L[f] = ldg(m.Lift, (uint32_t)(p + f * Np));
+  const double* dfv = sDF[e];
+  const double lift = L[0] * dfv[i + N1 * k] + L[1] * dfv[Nfp + j + N1 * k] + L[2] * dfv[2 * Nfp + i + N1 * k] +
+                      L[3] * dfv[3 * Nfp + j + N1 * k] + L[4] * dfv[4 * Nfp + i + N1 * j] +
+                      L[5] * dfv[5 * Nfp + i + N1 * j];
+  const double X = ldg(m.E11, o) * dx + ldg(m.E22, o) * dy + ldg(m.E33, o) * dz + lift;
+  stg(a.dt, o, -X / Gs + ldg(a.tp, o));
+}
+
 template <int N1>
 constexpr int epb_for() {
   return (N1 * N1 * N1 >= 128) ? 1 : (256 / (N1 * N1 * N1));
@@ -1393,6 +1505,34 @@ int launch_heve_bnd(fedg_ctx* ctx, const TendIn& in, double* const dt[5], double
     const int nblk = (m.Ne + EPB - 1) / EPB;
     hipLaunchKernelGGL((heve_bnd_kernel<N1, EPB>), dim3(nblk), dim3(EPB * N1 * N1 * N1), 0, ctx->stream, m, ctx->c,
                        tin, rk);
+    FEDG_HIP_CHECK(hipGetLastError());
+    return FEDG_OK;
+  };
+  switch (m.n1) {
+    case 2: return go(std::integral_constant<int, 2>{});
+    case 3: return go(std::integral_constant<int, 3>{});
+    case 4: return go(std::integral_constant<int, 4>{});
+    case 5: return go(std::integral_constant<int, 5>{});
+    case 6: return go(std::integral_constant<int, 6>{});
+    case 7: return go(std::integral_constant<int, 7>{});
+    case 8: return go(std::integral_constant<int, 8>{});
+    default:
+      set_error("polynomial order not built (supported: p = 1..7)");
+      return FEDG_ERR_UNSUPPORTED;
+  }
+}
+
+int launch_trcadv(fedg_ctx* ctx, const TrcLaunch& t) {
+  const DevMesh& m = ctx->m;
+  TrcArgs a;
+  a.dt = t.dt; a.Q = t.Q; a.MX = t.MX; a.MY = t.MY; a.MZ = t.MZ;
+  a.aM = t.aM; a.aP = t.aP; a.R = t.R; a.tp = t.tp; a.J = t.J;
+  for (int l = 0; l < 8; ++l) a.w[l] = t.w[l];
+  auto go = [&](auto n1c) -> int {
+    constexpr int N1 = decltype(n1c)::value;
+    constexpr int EPB = epb_for<N1>();
+    const int nblk = (m.Ne + EPB - 1) / EPB;
+    hipLaunchKernelGGL((trcadv_kernel<N1, EPB>), dim3(nblk), dim3(EPB * N1 * N1 * N1), 0, ctx->stream, m, a);
     FEDG_HIP_CHECK(hipGetLastError());
     return FEDG_OK;
   };

@@ -2,6 +2,8 @@
 
 * ``heve_cal_tend``      - atm_dyn_dgm_nonhydro3d_rhot_heve_cal_tend
                            (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve.F90:292-489)
+* ``trcadvect_cal_tend`` - atm_dyn_dgm_trcadvect3d_heve_cal_tend
+                           (FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:149-231)
 * ``calc_pressure``      - atm_dyn_dgm_nonhydro3d_common_DRHOT2PRES (..._common.F90:428-479)
 * ``halo_exchange``      - MeshFieldComm_Exchange on one tile (DAT/scale_meshfieldcomm_*.F90)
 * ``apply_bc``           - ATMOS_dyn_bnd_applyBC_prgvars_lc (FDS/scale_atm_dyn_dgm_bnd.F90:270-367)
@@ -90,6 +92,15 @@ def heve_add_bnd_contrib(dm: DeviceMesh, dt, DDENS, MOMX, MOMY, MOMZ, DRHOT, DPR
     args = [_p(a) for a in (*dt, DDENS, MOMX, MOMY, MOMZ, DRHOT, DPRES, DENS_hyd, PRES_hyd, THERM_hyd, Rtot, CVtot,
                             CPtot)]
     check(LIB.fedg_heve_add_bnd_contrib(dm.handle, *args), "fedg_heve_add_bnd_contrib")
+
+
+def trcadvect_cal_tend(dm: DeviceMesh, QTRC_dt, QTRC, MOMX, MOMY, MOMZ, alphDENS_M, alphDENS_P, fct_coef, RHOQ_tp,
+                       J):
+    """Advection tendency of one tracer, atm_dyn_dgm_trcadvect3d_heve_cal_tend
+    (FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:149-231); alphDENS_M/_P are
+    (NfpTot,Ne), J is (Np,Ne), the rest (Np,NeA) device fields."""
+    args = [_p(a) for a in (QTRC_dt, QTRC, MOMX, MOMY, MOMZ, alphDENS_M, alphDENS_P, fct_coef, RHOQ_tp, J)]
+    check(LIB.fedg_trcadvect_cal_tend(dm.handle, *args), "fedg_trcadvect_cal_tend")
 
 
 def hevi_cal_tend(dm: DeviceMesh, out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DPRES, DENS_hyd, PRES_hyd, PRES_hyd_ref,

@@ -243,6 +243,21 @@ int fedg_heve_add_bnd_contrib(fedg_ctx *ctx, double *DENS_dt, double *MOMX_dt, d
                               const double *DENS_hyd, const double *PRES_hyd, const double *THERM_hyd,
                               const double *Rtot, const double *CVtot, const double *CPtot);
 
+/* atm_dyn_dgm_trcadvect3d_heve_cal_tend
+ * (FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:149-231 with its face flux
+ * atm_dyn_dgm_trcadvect3d_heve_get_delflux_generalhvc, :554-674): the
+ * advection tendency of one tracer, QTRC_dt = -Div(Gsqrt (MOMX, MOMY,
+ * MOMZ/GsqrtV + G13 MOMX + G23 MOMY) QTRC)/Gsqrt + RHOQ_tp, with the central
+ * face flux damped by alphDENS_M/_P (NfpTot,Ne) and weighted by fct_coef on
+ * the upwind side of the face's net outward flux.  The FaceIntMat argument of
+ * the reference (its _Init, :75-137) is built from the context's order; J is
+ * lcmesh%J (Np,Ne).  QTRC_, the momenta, fct_coef and RHOQ_tp are
+ * (Np,NeA) with the halo filled. */
+int fedg_trcadvect_cal_tend(fedg_ctx *ctx, double *QTRC_dt, const double *QTRC_,
+                            const double *MOMX_, const double *MOMY_, const double *MOMZ_,
+                            const double *alphDENS_M, const double *alphDENS_P,
+                            const double *fct_coef, const double *RHOQ_tp, const double *J);
+
 /* atm_dyn_dgm_nonhydro3d_rhot_hevi_cal_tend
  * (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi.F90:289-482 with its numerical
  * flux FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi_numflux.F90:232-416): the

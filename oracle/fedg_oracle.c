@@ -442,6 +442,104 @@ void oracle_heve_add_bnd_contrib(const oracle_mesh *m,
   free(del_flux);
 }
 
+/* FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:149-231 (cal_tend) with
+ * get_delflux_generalhvc (:554-674).  FaceIntMat (:75-137) is applied as the
+ * reference's CSR product (CMN/scale_sparsemat.F90:439-474): row f sums its
+ * face's nodes in storage order from s = 0, weights w(a0)*w(a1) of the 1D
+ * LGL weights w1d.  Div (ELM/..._tensorprod3D.F90) is the three line
+ * contractions plus the lift of del_flux. */
+void oracle_trcadvect_cal_tend(const oracle_mesh *m, const double *w1d, double *QTRC_dt, const double *QTRC_,
+                               const double *MOMX_, const double *MOMY_, const double *MOMZ_,
+                               const double *alphDENS_M, const double *alphDENS_P, const double *fct_coef,
+                               const double *RHOQ_tp, const double *J) {
+  const int n1 = m->n1, Np = m->Np, NfpTot = m->NfpTot, Nfp_h = m->Nfp_h, Nfp_v = m->Nfp_v, Ne = m->Ne;
+  const long NpNeA = (long)Np * m->NeA;
+  const long NpNe = (long)Np * Ne;
+  const double *G13 = m->GI3, *G23 = m->GI3 + NpNeA;
+  const double *nx = m->normal_fn, *ny = m->normal_fn + (long)NfpTot * Ne, *nz = m->normal_fn + 2L * NfpTot * Ne;
+  double *IntW = (double *)malloc(sizeof(double) * NfpTot); /* nonzero of FaceIntMat in each column */
+  for (int f = 0; f < 4; ++f)
+    for (int k = 0; k < n1; ++k)
+      for (int i = 0; i < n1; ++i) IntW[f * Nfp_h + i + n1 * k] = w1d[i] * w1d[k];
+  for (int f = 0; f < 2; ++f)
+    for (int j = 0; j < n1; ++j)
+      for (int i = 0; i < n1; ++i) IntW[4 * Nfp_h + f * Nfp_v + i + n1 * j] = w1d[i] * w1d[j];
+#pragma omp parallel
+  {
+    double *MomFlxM = (double *)malloc(sizeof(double) * NfpTot), *MomFlxP = (double *)malloc(sizeof(double) * NfpTot);
+    double *numflux = (double *)malloc(sizeof(double) * NfpTot), *b = (double *)malloc(sizeof(double) * NfpTot);
+    double *QM = (double *)malloc(sizeof(double) * NfpTot), *RM = (double *)malloc(sizeof(double) * NfpTot);
+    double *RP = (double *)malloc(sizeof(double) * NfpTot), *del_flux = (double *)malloc(sizeof(double) * NfpTot);
+    double *Flux = (double *)malloc(sizeof(double) * Np * 3), *DFlux = (double *)malloc(sizeof(double) * Np * 4);
+    double outward[6];
+#pragma omp for schedule(static)
+    for (int ke = 0; ke < Ne; ++ke) {
+      const int ke2d = m->EMap3Dto2D[ke] - 1;
+      const long of = (long)NfpTot * ke, o = (long)Np * ke;
+      for (int fp = 0; fp < NfpTot; ++fp) {
+        const long iM = m->vmapM[of + fp] - 1, iP = m->vmapP[of + fp] - 1;
+        const double GsH = m->GsqrtH[(m->IndexH2Dto3D_bnd[fp] - 1) + (long)Nfp_v * ke2d];
+        const double Gsqrt_M = m->Gsqrt[iM], Gsqrt_P = m->Gsqrt[iP];
+        const double GsqrtV_M = Gsqrt_M / GsH, GsqrtV_P = Gsqrt_P / GsH;
+        const double GMX_M = Gsqrt_M * MOMX_[iM], GMX_P = Gsqrt_P * MOMX_[iP];
+        const double GMY_M = Gsqrt_M * MOMY_[iM], GMY_P = Gsqrt_P * MOMY_[iP];
+        const double GMZ_M = Gsqrt_M * MOMZ_[iM], GMZ_P = Gsqrt_P * MOMZ_[iP];
+        MomFlxM[fp] = GMX_M * nx[of + fp] + GMY_M * ny[of + fp] +
+                      ((GMZ_M / GsqrtV_M + G13[iM] * GMX_M + G23[iM] * GMY_M) * nz[of + fp]);
+        MomFlxP[fp] = GMX_P * nx[of + fp] + GMY_P * ny[of + fp] +
+                      ((GMZ_P / GsqrtV_P + G13[iP] * GMX_P + G23[iP] * GMY_P) * nz[of + fp]);
+        QM[fp] = QTRC_[iM];
+        RM[fp] = fct_coef[iM];
+        RP[fp] = fct_coef[iP];
+        numflux[fp] = 0.5 * ((QTRC_[iP] * MomFlxP[fp] + QTRC_[iM] * MomFlxM[fp]) - alphDENS_P[of + fp] * QTRC_[iP] +
+                             alphDENS_M[of + fp] * QTRC_[iM]);
+        b[fp] = J[iM] * m->Fscale[of + fp] * numflux[fp];
+      }
+      for (int f = 0; f < 6; ++f) {
+        const int is = f < 4 ? f * Nfp_h : 4 * Nfp_h + (f - 4) * Nfp_v, n = f < 4 ? Nfp_h : Nfp_v;
+        double s = 0.0;
+        for (int c = is; c < is + n; ++c) s = s + IntW[c] * b[c];
+        outward[f] = s;
+      }
+      for (int fp = 0; fp < NfpTot; ++fp) {
+        const int f = fp < 4 * Nfp_h ? fp / Nfp_h : 4 + (fp - 4 * Nfp_h) / Nfp_v;
+        del_flux[fp] = m->Fscale[of + fp] *
+                       (numflux[fp] * 0.5 * (RP[fp] + RM[fp] - (RP[fp] - RM[fp]) * copysign(1.0, outward[f])) -
+                        QM[fp] * MomFlxM[fp]);
+      }
+      for (int p = 0; p < Np; ++p) {
+        const double Gsqrt_ = m->Gsqrt[o + p];
+        const double RGsqrt = 1.0 / Gsqrt_;
+        const double RGsqrtV = m->GsqrtH[(m->IndexH2Dto3D[p] - 1) + (long)Nfp_v * ke2d] * RGsqrt;
+        Flux[p] = Gsqrt_ * MOMX_[o + p] * QTRC_[o + p];
+        Flux[p + Np] = Gsqrt_ * MOMY_[o + p] * QTRC_[o + p];
+        Flux[p + 2 * Np] =
+            Gsqrt_ * (MOMZ_[o + p] * RGsqrtV + G13[o + p] * MOMX_[o + p] + G23[o + p] * MOMY_[o + p]) * QTRC_[o + p];
+      }
+      op_dirX(n1, m->D1D, Flux, DFlux);
+      op_dirY(n1, m->D1D, Flux + Np, DFlux + Np);
+      op_dirZ(n1, m->D1D, Flux + 2 * Np, DFlux + 2 * Np);
+      op_lift(n1, m->Lift_mat, del_flux, DFlux + 3 * Np);
+      const double *E11 = m->Escale + o, *E22 = m->Escale + o + NpNe * 4, *E33 = m->Escale + o + NpNe * 8;
+      for (int p = 0; p < Np; ++p)
+        QTRC_dt[o + p] = -(E11[p] * DFlux[p] + E22[p] * DFlux[p + Np] + E33[p] * DFlux[p + 2 * Np] +
+                           DFlux[p + 3 * Np]) / m->Gsqrt[o + p] +
+                         RHOQ_tp[o + p];
+    }
+    free(MomFlxM);
+    free(MomFlxP);
+    free(numflux);
+    free(b);
+    free(QM);
+    free(RM);
+    free(RP);
+    free(del_flux);
+    free(Flux);
+    free(DFlux);
+  }
+  free(IntW);
+}
+
 /* ---------------------------------------------------------------------- */
 /* FDS/scale_atm_dyn_dgm_nonhydro3d_common.F90:624-777 with gam = 1 (the
  * regional mesh sets lcmesh%gam = 1, MSH/scale_mesh_base3d.F90:316) */

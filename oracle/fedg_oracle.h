@@ -113,6 +113,14 @@ void oracle_heve_add_bnd_contrib(const oracle_mesh *m,
                                  const double *DENS_hyd, const double *PRES_hyd, const double *THERM_hyd,
                                  const double *Rtot, const double *CVtot, const double *CPtot);
 
+/* FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:149-231 with :554-674 (tracer
+ * advection tendency); w1d (n1): 1D LGL weights that build FaceIntMat;
+ * alphDENS_M/_P (NfpTot,Ne); J (Np,Ne) */
+void oracle_trcadvect_cal_tend(const oracle_mesh *m, const double *w1d, double *QTRC_dt, const double *QTRC_,
+                               const double *MOMX_, const double *MOMY_, const double *MOMZ_,
+                               const double *alphDENS_M, const double *alphDENS_P, const double *fct_coef,
+                               const double *RHOQ_tp, const double *J);
+
 /* FDS/scale_atm_dyn_dgm_nonhydro3d_common.F90:624-777, hydrostatic pressure gradient */
 void oracle_calc_phyd_hgrad(const oracle_mesh *m, double *DPhydDx, double *DPhydDy,
                             const double *PRES_hyd, const double *PRES_hyd_ref);

@@ -53,6 +53,7 @@ def _load():
     lib.oracle_heve_cal_tend_cco.argtypes = [M] + [dp] * 16
     lib.oracle_heve_add_bnd_contrib.argtypes = [M] + [dp] * 17
     lib.oracle_calc_phyd_hgrad.argtypes = [M] + [dp] * 4
+    lib.oracle_trcadvect_cal_tend.argtypes = [M] + [dp] * 11
     lib.oracle_rk_tab_get.argtypes = [C.c_char_p, C.c_double, C.POINTER(RkTabC)]
     lib.oracle_rk_advance.argtypes = [C.POINTER(RkTabC), C.c_int, C.c_long] + [dp] * 5 + [C.c_long]
     lib.oracle_rk_store_impl.argtypes = [C.POINTER(RkTabC), C.c_int, C.c_long] + [dp] * 4 + [C.c_long]
@@ -190,6 +191,15 @@ class OracleMesh:
         LIB.oracle_heve_add_bnd_contrib(self.ref, *[_d(a) for a in dt], _d(q[0]), _d(q[3]), _d(q[4]), _d(q[2]),
                                         _d(q[1]), _d(DPRES), _d(bs.DENS_hyd), _d(bs.PRES_hyd), _d(bs.THERM_hyd),
                                         *ext)
+
+    def trcadvect_cal_tend(self, QTRC, MOMX, MOMY, MOMZ, alphDENS_M, alphDENS_P, fct_coef, RHOQ_tp, J=None):
+        """Tracer advection tendency QTRC_dt (Np,NeA); J defaults to the mesh's (Np,Ne) Jacobian."""
+        m = self.mesh
+        out = m.field()
+        J = m.J if J is None else J
+        LIB.oracle_trcadvect_cal_tend(self.ref, _d(m.elem.w1d), _d(out), _d(QTRC), _d(MOMX), _d(MOMY), _d(MOMZ),
+                                      _d(alphDENS_M), _d(alphDENS_P), _d(fct_coef), _d(RHOQ_tp), _d(J))
+        return out
 
     # --- HEVI ---------------------------------------------------------------
     def hevi_numflux(self, q, DPRES, bs):
