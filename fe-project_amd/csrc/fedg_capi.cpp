@@ -729,15 +729,10 @@ int fedg_heve_add_bnd_contrib(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, d
   return FEDG_OK;
 }
 
-int fedg_trcadvect_cal_tend(fedg_ctx* ctx, double* QTRC_dt, const double* QTRC_, const double* MOMX_,
-                            const double* MOMY_, const double* MOMZ_, const double* alphDENS_M,
-                            const double* alphDENS_P, const double* fct_coef, const double* RHOQ_tp,
-                            const double* J) {
-  if (!ctx || !QTRC_dt || !QTRC_ || !MOMX_ || !MOMY_ || !MOMZ_ || !alphDENS_M || !alphDENS_P || !fct_coef ||
-      !RHOQ_tp || !J)
-    return FEDG_ERR_ARG;
-  // FaceIntMat (scale_atm_dyn_dgm_trcadvect3d_heve.F90:75-137): the face
-  // node weights are products of the 1D LGL weights of the element's order
+// FaceIntMat and IntWeight_lgl (scale_atm_dyn_dgm_trcadvect3d_heve.F90:75-137,
+// ELM/scale_element_hexahedral.F90:286-298) are products of the 1D LGL
+// weights of the element's order
+static int trc_weights(fedg_ctx* ctx, TrcW& w) {
   RefElem e;
   try {
     build_refelem(ctx->m.n1 - 1, e);
@@ -745,7 +740,19 @@ int fedg_trcadvect_cal_tend(fedg_ctx* ctx, double* QTRC_dt, const double* QTRC_,
     set_error(ex.what());
     return FEDG_ERR_ARG;
   }
+  for (int l = 0; l < 8; ++l) w.w[l] = l < ctx->m.n1 ? e.w1d[l] : 0.0;
+  return FEDG_OK;
+}
+
+int fedg_trcadvect_cal_tend(fedg_ctx* ctx, double* QTRC_dt, const double* QTRC_, const double* MOMX_,
+                            const double* MOMY_, const double* MOMZ_, const double* alphDENS_M,
+                            const double* alphDENS_P, const double* fct_coef, const double* RHOQ_tp,
+                            const double* J) {
+  if (!ctx || !QTRC_dt || !QTRC_ || !MOMX_ || !MOMY_ || !MOMZ_ || !alphDENS_M || !alphDENS_P || !fct_coef ||
+      !RHOQ_tp || !J)
+    return FEDG_ERR_ARG;
   TrcLaunch t{};
+  TRY(trc_weights(ctx, t.w));
   t.dt = QTRC_dt;
   t.Q = QTRC_;
   t.MX = MOMX_;
@@ -756,8 +763,93 @@ int fedg_trcadvect_cal_tend(fedg_ctx* ctx, double* QTRC_dt, const double* QTRC_,
   t.R = fct_coef;
   t.tp = RHOQ_tp;
   t.J = J;
-  for (int l = 0; l < ctx->m.n1; ++l) t.w[l] = e.w1d[l];
   TRY(launch_trcadv(ctx, t));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return FEDG_OK;
+}
+
+int fedg_trcadvect_calc_fct_coef(fedg_ctx* ctx, double* fct_coef, const double* QTRC_, const double* MOMX_,
+                                 const double* MOMY_, const double* MOMZ_, const double* RHOQ_tp_,
+                                 const double* alphDENS_M, const double* alphDENS_P, const double* DENS_hyd,
+                                 const double* DDENS_, const double* DDENS0_, double rk_c_ssm1, double dt,
+                                 const double* J, int disable_limiter) {
+  if (!ctx || !fct_coef) return FEDG_ERR_ARG;
+  TrcLaunch t{};
+  t.dt = fct_coef;
+  t.fct = 1;
+  t.disable = disable_limiter ? 1 : 0;
+  if (!t.disable) {
+    if (!QTRC_ || !MOMX_ || !MOMY_ || !MOMZ_ || !RHOQ_tp_ || !alphDENS_M || !alphDENS_P || !DENS_hyd || !DDENS_ ||
+        !DDENS0_ || !J)
+      return FEDG_ERR_ARG;
+    if (!(dt != 0.0)) {
+      set_error("fedg_trcadvect_calc_fct_coef: dt must be nonzero");
+      return FEDG_ERR_ARG;
+    }
+    TRY(trc_weights(ctx, t.w));
+    t.Q = QTRC_;
+    t.MX = MOMX_;
+    t.MY = MOMY_;
+    t.MZ = MOMZ_;
+    t.tp = RHOQ_tp_;
+    t.aM = alphDENS_M;
+    t.aP = alphDENS_P;
+    t.Dh = DENS_hyd;
+    t.DD = DDENS_;
+    t.DD0 = DDENS0_;
+    t.c = rk_c_ssm1;
+    t.dtt = dt;
+    t.J = J;
+  }
+  TRY(launch_trcadv(ctx, t));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return FEDG_OK;
+}
+
+int fedg_trcadvect_tmar(fedg_ctx* ctx, double* QTRC_, const double* DENS_hyd, const double* DDENS_,
+                        const double* J) {
+  if (!ctx || !QTRC_ || !DENS_hyd || !DDENS_ || !J) return FEDG_ERR_ARG;
+  TrcW w{};
+  TRY(trc_weights(ctx, w));
+  TRY(launch_tmar(ctx, QTRC_, DENS_hyd, DDENS_, J, w));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return FEDG_OK;
+}
+
+int fedg_trcadvect_save_massflux(fedg_ctx* ctx, double* MFLX_x_tavg, double* MFLX_y_tavg, double* MFLX_z_tavg,
+                                 double* alph_dens_M, double* alph_dens_P, const double* DDENS, const double* MOMX,
+                                 const double* MOMY, const double* MOMZ, const double* DPRES, const double* DENS_hyd,
+                                 const double* PRES_hyd, const double* Rtot, const double* CVtot,
+                                 const double* CPtot, int rkstage, double tavg_weight_h, double tavg_weight_v,
+                                 int is_hevi) {
+  (void)Rtot;  // the reference's cal_alphdens_dyn takes them but uses the dry-air gamma
+  (void)CVtot;
+  (void)CPtot;
+  if (!ctx || !MFLX_x_tavg || !MFLX_y_tavg || !MFLX_z_tavg || !alph_dens_M || !alph_dens_P || !DDENS || !MOMX ||
+      !MOMY || !MOMZ || !DPRES || !DENS_hyd || !PRES_hyd)
+    return FEDG_ERR_ARG;
+  if (rkstage < 1) {
+    set_error("fedg_trcadvect_save_massflux: rkstage counts from 1");
+    return FEDG_ERR_ARG;
+  }
+  MflxArgs a{};
+  a.FX = MFLX_x_tavg;
+  a.FY = MFLX_y_tavg;
+  a.FZ = MFLX_z_tavg;
+  a.alM = alph_dens_M;
+  a.alP = alph_dens_P;
+  a.DD = DDENS;
+  a.MX = MOMX;
+  a.MY = MOMY;
+  a.MZ = MOMZ;
+  a.DP = DPRES;
+  a.Dh = DENS_hyd;
+  a.Ph = PRES_hyd;
+  a.rkstage = rkstage;
+  a.hevi = is_hevi ? 1 : 0;
+  a.twh = tavg_weight_h;
+  a.twv = tavg_weight_v;
+  TRY(launch_save_massflux(ctx, a));
   FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   return FEDG_OK;
 }

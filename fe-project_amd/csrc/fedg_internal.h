@@ -115,13 +115,28 @@ int launch_hevi_explicit(fedg_ctx* ctx, const TendIn& in, double* const out[5]);
 // contribution added in place (DPRES: its halo tail is computed first)
 int launch_heve_cco(fedg_ctx* ctx, const TendIn& in, double* const out[5]);
 int launch_heve_bnd(fedg_ctx* ctx, const TendIn& in, double* const dt[5], double* DPRES);
-// tracer advection tendency (HEVE)
-struct TrcLaunch {
-  double* dt;
-  const double *Q, *MX, *MY, *MZ, *aM, *aP, *R, *tp, *J;
+// tracer advection (HEVE): tendency, FCT coefficient, TMAR, mass-flux save
+struct TrcW {
   double w[8];  // 1D LGL integration weights
 };
+struct TrcLaunch {
+  double* dt;   // QTRC_dt, or fct_coef when fct
+  const double *Q, *MX, *MY, *MZ, *aM, *aP, *R, *tp, *J;
+  const double *Dh, *DD, *DD0;  // fct: DENS_hyd, DDENS_, DDENS0_
+  double c, dtt;                // fct: rk_c_ssm1, dt
+  int fct, disable;
+  TrcW w;
+};
+struct MflxArgs {
+  double *FX, *FY, *FZ;      // MFLX_x/y/z_tavg (Np,NeA)
+  double *alM, *alP;         // alph_dens_M / _P (NfpTot,Ne)
+  const double *DD, *MX, *MY, *MZ, *DP, *Dh, *Ph;
+  int rkstage, hevi;
+  double twh, twv;
+};
 int launch_trcadv(fedg_ctx* ctx, const TrcLaunch& t);
+int launch_tmar(fedg_ctx* ctx, double* Q, const double* Dh, const double* DD, const double* J, const TrcW& w);
+int launch_save_massflux(fedg_ctx* ctx, const MflxArgs& a);
 int launch_heve_fused(fedg_ctx* ctx, const TendIn& in, const RkOut& rk, int use_flat);
 int run_stage_probe(int mode, int NeX, int NeY, int NeZ, int reps, double* ms);
 // FP32 perturbation form of the fused stage (flat box, dry air; at p = 7 the

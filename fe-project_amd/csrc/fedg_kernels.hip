@@ -1142,32 +1142,55 @@ __global__ void __launch_bounds__(EPB * N1 * N1 * N1) heve_bnd_kernel(const DevM
 // flux of each face (FaceIntMat of the LGL face weights, :75-137), Div with
 // the lift, then QTRC_dt = -(...)/Gsqrt + RHOQ_tp.  One thread per node.
 struct TrcArgs {
-  double* dt;                       // QTRC_dt (Np,NeA)
+  double* dt;                       // QTRC_dt (Np,NeA); fct_coef for the FCT kernel
   const double *Q, *MX, *MY, *MZ;   // QTRC_ and the mass fluxes (Np,NeA)
   const double *aM, *aP;            // alphDENS_M / _P (NfpTot,Ne)
   const double* R;                  // fct_coef (Np,NeA)
   const double* tp;                 // RHOQ_tp (Np,NeA)
   const double* J;                  // (Np,Ne)
   double w[8];                      // 1D LGL integration weights
+  // FCT coefficient only: density of the stage s-1, DENS_hyd + (1-c) DDENS0 + c DDENS
+  const double *Dh, *DD, *DD0;
+  double c, dtt;
+  int disable;                      // disable_limiter: fct_coef = 1
 };
-template <int N1, int EPB>
+enum { TRC_TEND = 0, TRC_FCT = 1 };
+// KIND = TRC_TEND: the tendency of cal_tend.  KIND = TRC_FCT: the FCT
+// coefficient of calc_fct_coef (:232-305): the sum of the positive net
+// outward fluxes of the six faces (get_netOutwardFlux_generalhvc, :676-797,
+// the same face flux), against the tracer mass available over dt,
+// sum(Gsqrt J IntWeight_lgl (dens_ssm1 QTRC / dt + RHOQ_tp)), summed node by
+// node in order by one thread.
+template <int N1, int EPB, int KIND>
 __global__ void __launch_bounds__(EPB * N1 * N1 * N1) trcadv_kernel(const DevMesh m, const TrcArgs a) {
   constexpr int Np = N1 * N1 * N1, Nfp = N1 * N1, NfpTot = 6 * Nfp;
+  constexpr bool TEND = KIND == TRC_TEND;
+  constexpr int EF = TEND ? EPB : 1;  // arrays only the tendency uses
   __shared__ double sDt[N1 * N1];
-  __shared__ double sF[EPB][3][Np];
-  __shared__ double sNF[EPB][NfpTot];  // numflux
-  __shared__ double sQF[EPB][NfpTot];  // QTRC_M * MomFlxM
-  __shared__ double sRP[EPB][NfpTot], sRM[EPB][NfpTot];
-  __shared__ double sJF[EPB][NfpTot];  // J(iM) * Fscale * numflux
-  __shared__ double sOut[EPB][6];      // net outward flux of each face
-  __shared__ double sDF[EPB][NfpTot];  // del_flux
+  __shared__ double sF[EF][3][TEND ? Np : 1];
+  __shared__ double sNF[EF][TEND ? NfpTot : 1];  // numflux
+  __shared__ double sQF[EF][TEND ? NfpTot : 1];  // QTRC_M * MomFlxM
+  __shared__ double sRP[EF][TEND ? NfpTot : 1], sRM[EF][TEND ? NfpTot : 1];
+  __shared__ double sDF[EF][TEND ? NfpTot : 1];  // del_flux
+  __shared__ double sJF[EPB][NfpTot];            // J(iM) * Fscale * numflux
+  __shared__ double sOut[EPB][6];                // net outward flux of each face
+  __shared__ double sM[TEND ? 1 : EPB][TEND ? 1 : Np];  // FCT: mass terms of the nodes
+  __shared__ double sC[EPB];                            // FCT: the element's coefficient
   const int tid = threadIdx.x;
   const int e = tid / Np;
   const int p = tid - e * Np;
   const int ke = blockIdx.x * EPB + e;
   const bool valid = ke < m.Ne;
   const int i = p % N1, j = (p / N1) % N1, k = p / Nfp;
-  for (int t = tid; t < N1 * N1; t += blockDim.x) sDt[(t % N1) * N1 + t / N1] = m.D1D[t];
+  const uint32_t o = (uint32_t)Np * (valid ? ke : 0) + p;
+  if constexpr (!TEND) {
+    if (a.disable) {  // uniform over the grid
+      if (valid) stg(a.dt, o, 1.0);
+      return;
+    }
+  }
+  if constexpr (TEND)
+    for (int t = tid; t < N1 * N1; t += blockDim.x) sDt[(t % N1) * N1 + t / N1] = m.D1D[t];
   const int ke2d = valid ? m.EMap3Dto2D[ke] : 0;
   if (valid) {
     for (int fp = p; fp < NfpTot; fp += Np) {
@@ -1187,11 +1210,20 @@ __global__ void __launch_bounds__(EPB * N1 * N1 * N1) trcadv_kernel(const DevMes
       const double MF_M = GMX_M * nx + GMY_M * ny + ((GMZ_M / GsqrtV_M + G13_M * GMX_M + G23_M * GMY_M) * nz);
       const double MF_P = GMX_P * nx + GMY_P * ny + ((GMZ_P / GsqrtV_P + G13_P * GMX_P + G23_P * GMY_P) * nz);
       const double nf = 0.5 * ((Q_P * MF_P + Q_M * MF_M) - ldg(a.aP, gf) * Q_P + ldg(a.aM, gf) * Q_M);
-      sNF[e][fp] = nf;
-      sQF[e][fp] = Q_M * MF_M;
-      sRP[e][fp] = ldg(a.R, iP);
-      sRM[e][fp] = ldg(a.R, iM);
+      if constexpr (TEND) {
+        sNF[e][fp] = nf;
+        sQF[e][fp] = Q_M * MF_M;
+        sRP[e][fp] = ldg(a.R, iP);
+        sRM[e][fp] = ldg(a.R, iM);
+      }
       sJF[e][fp] = ldg(a.J, iM) * ldg(m.Fscale, gf) * nf;
+    }
+  }
+  if constexpr (!TEND) {
+    if (valid) {
+      const double IW = (a.w[i] * a.w[j]) * a.w[k];
+      const double dens = ldg(a.Dh, o) + (1.0 - a.c) * ldg(a.DD0, o) + a.c * ldg(a.DD, o);
+      sM[e][p] = ldg(m.Gsqrt, o) * ldg(a.J, o) * IW * (dens * ldg(a.Q, o) / a.dtt + ldg(a.tp, o));
     }
   }
   __syncthreads();
@@ -1201,50 +1233,146 @@ __global__ void __launch_bounds__(EPB * N1 * N1 * N1) trcadv_kernel(const DevMes
     for (int an = 0; an < Nfp; ++an) s = s + (a.w[an % N1] * a.w[an / N1]) * sJF[e][p * Nfp + an];
     sOut[e][p] = s;
   }
-  __syncthreads();
-  if (valid) {
-    for (int fp = p; fp < NfpTot; fp += Np) {
-      const uint32_t gf = (uint32_t)fp + (uint32_t)NfpTot * ke;
-      const double RP = sRP[e][fp], RM = sRM[e][fp];
-      const double sg = copysign(1.0, sOut[e][fp / Nfp]);
-      sDF[e][fp] = ldg(m.Fscale, gf) * (sNF[e][fp] * 0.5 * (RP + RM - (RP - RM) * sg) - sQF[e][fp]);
+  if constexpr (!TEND) {
+    __syncthreads();
+    if (valid && p == 0) {
+      double net = 0.0;
+      for (int f = 0; f < 6; ++f) net = net + (sOut[e][f] > 0.0 ? sOut[e][f] : 0.0);
+      double q = 0.0;
+      for (int n = 0; n < Np; ++n) q = q + sM[e][n];
+      double r = q / (net + 1.0e-10);
+      r = r < 1.0 ? r : 1.0;
+      sC[e] = r > 0.0 ? r : 0.0;
     }
+    __syncthreads();
+    if (valid) stg(a.dt, o, sC[e]);
+    return;
+  } else {
+    __syncthreads();
+    if (valid) {
+      for (int fp = p; fp < NfpTot; fp += Np) {
+        const uint32_t gf = (uint32_t)fp + (uint32_t)NfpTot * ke;
+        const double RP = sRP[e][fp], RM = sRM[e][fp];
+        const double sg = copysign(1.0, sOut[e][fp / Nfp]);
+        sDF[e][fp] = ldg(m.Fscale, gf) * (sNF[e][fp] * 0.5 * (RP + RM - (RP - RM) * sg) - sQF[e][fp]);
+      }
+    }
+    // volume flux
+    double Gs = 1.0;
+    if (valid) {
+      Gs = ldg(m.Gsqrt, o);
+      const double RGsqrt = 1.0 / Gs;
+      const double RGsqrtV = ldg(m.GsqrtH, (uint32_t)(i + N1 * j) + (uint32_t)Nfp * ke2d) * RGsqrt;
+      const double Qo = ldg(a.Q, o), mx = ldg(a.MX, o), my = ldg(a.MY, o), mz = ldg(a.MZ, o);
+      sF[e][0][p] = Gs * mx * Qo;
+      sF[e][1][p] = Gs * my * Qo;
+      sF[e][2][p] = Gs * (mz * RGsqrtV + ldg(m.G13, o) * mx + ldg(m.G23, o) * my) * Qo;
+    }
+    __syncthreads();
+    if (!valid) return;
+    const double* Fx = sF[e][0];
+    const double* Fy = sF[e][1];
+    const double* Fz = sF[e][2];
+    double dx = sDt[N1 * i] * Fx[N1 * (j + N1 * k)];
+#pragma unroll
+    for (int l = 1; l < N1; ++l) dx = dx + sDt[l + N1 * i] * Fx[l + N1 * (j + N1 * k)];
+    double dy = Fy[i + N1 * (N1 * k)] * sDt[N1 * j];
+#pragma unroll
+    for (int l = 1; l < N1; ++l) dy = dy + Fy[i + N1 * (l + N1 * k)] * sDt[l + N1 * j];
+    double dz = Fz[i + N1 * j] * sDt[N1 * k];
+#pragma unroll
+    for (int l = 1; l < N1; ++l) dz = dz + Fz[i + N1 * (j + N1 * l)] * sDt[l + N1 * k];
+    double L[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) L[f] = ldg(m.Lift, (uint32_t)(p + f * Np));
+    const double* dfv = sDF[e];
+    const double lift = L[0] * dfv[i + N1 * k] + L[1] * dfv[Nfp + j + N1 * k] + L[2] * dfv[2 * Nfp + i + N1 * k] +
+                        L[3] * dfv[3 * Nfp + j + N1 * k] + L[4] * dfv[4 * Nfp + i + N1 * j] +
+                        L[5] * dfv[5 * Nfp + i + N1 * j];
+    const double X = ldg(m.E11, o) * dx + ldg(m.E22, o) * dy + ldg(m.E33, o) * dz + lift;
+    stg(a.dt, o, -X / Gs + ldg(a.tp, o));
   }
-  // volume flux
+}
+
+// TMAR, the second step of the limiter (:308-337): negative values cut to
+// zero, then the element's tracer mass restored by one factor,
+// QTRC = Q0 / (Q1 + 1e-32) * max(0, QTRC), the two masses summed node by node
+// in order by one thread.
+template <int N1, int EPB>
+__global__ void __launch_bounds__(EPB * N1 * N1 * N1)
+    tmar_kernel(const DevMesh m, double* Q, const double* Dh, const double* DD, const double* J, TrcW w) {
+  constexpr int Np = N1 * N1 * N1;
+  __shared__ double s0[EPB][Np], s1[EPB][Np];
+  __shared__ double sr[EPB];
+  const int tid = threadIdx.x;
+  const int e = tid / Np;
+  const int p = tid - e * Np;
+  const int ke = blockIdx.x * EPB + e;
+  const bool valid = ke < m.Ne;
+  const int i = p % N1, j = (p / N1) % N1, k = p / (N1 * N1);
   const uint32_t o = (uint32_t)Np * (valid ? ke : 0) + p;
-  double Gs = 1.0;
+  double qp = 0.0;
   if (valid) {
-    Gs = ldg(m.Gsqrt, o);
-    const double RGsqrt = 1.0 / Gs;
-    const double RGsqrtV = ldg(m.GsqrtH, (uint32_t)(i + N1 * j) + (uint32_t)Nfp * ke2d) * RGsqrt;
-    const double Qo = ldg(a.Q, o), mx = ldg(a.MX, o), my = ldg(a.MY, o), mz = ldg(a.MZ, o);
-    sF[e][0][p] = Gs * mx * Qo;
-    sF[e][1][p] = Gs * my * Qo;
-    sF[e][2][p] = Gs * (mz * RGsqrtV + ldg(m.G13, o) * mx + ldg(m.G23, o) * my) * Qo;
+    const double dens = ldg(Dh, o) + ldg(DD, o);
+    const double q = ldg(Q, o);
+    qp = q > 0.0 ? q : 0.0;
+    const double g = ldg(m.Gsqrt, o) * ldg(J, o) * ((w.w[i] * w.w[j]) * w.w[k]) * dens;
+    s0[e][p] = g * q;
+    s1[e][p] = g * qp;
   }
   __syncthreads();
-  if (!valid) return;
-  const double* Fx = sF[e][0];
-  const double* Fy = sF[e][1];
-  const double* Fz = sF[e][2];
-  double dx = sDt[N1 * i] * Fx[N1 * (j + N1 * k)];
-#pragma unroll
-  for (int l = 1; l < N1; ++l) dx = dx + sDt[l + N1 * i] * Fx[l + N1 * (j + N1 * k)];
-  double dy = Fy[i + N1 * (N1 * k)] * sDt[N1 * j];
-#pragma unroll
-  for (int l = 1; l < N1; ++l) dy = dy + Fy[i + N1 * (l + N1 * k)] * sDt[l + N1 * j];
-  double dz = Fz[i + N1 * j] * sDt[N1 * k];
-#pragma unroll
-  for (int l = 1; l < N1; ++l) dz = dz + Fz[i + N1 * (j + N1 * l)] * sDt[l + N1 * k];
-  double L[6];
-#pragma unroll
-  for (int f = 0; f < 6; ++f) L[f] = ldg(m.Lift, (uint32_t)(p + f * Np));
-  const double* dfv = sDF[e];
-  const double lift = L[0] * dfv[i + N1 * k] + L[1] * dfv[Nfp + j + N1 * k] + L[2] * dfv[2 * Nfp + i + N1 * k] +
-                      L[3] * dfv[3 * Nfp + j + N1 * k] + L[4] * dfv[4 * Nfp + i + N1 * j] +
-                      L[5] * dfv[5 * Nfp + i + N1 * j];
-  const double X = ldg(m.E11, o) * dx + ldg(m.E22, o) * dy + ldg(m.E33, o) * dz + lift;
-  stg(a.dt, o, -X / Gs + ldg(a.tp, o));
+  if (valid && p == 0) {
+    double q0 = 0.0, q1 = 0.0;
+    for (int n = 0; n < Np; ++n) q0 = q0 + s0[e][n];
+    for (int n = 0; n < Np; ++n) q1 = q1 + s1[e][n];
+    sr[e] = q0 / (q1 + 1.0e-32);
+  }
+  __syncthreads();
+  if (valid) stg(Q, o, sr[e] * qp);
+}
+
+// save_massflux (:340-404): time-weighted mass fluxes over the RK stages,
+// and the damping coefficients of cal_alphdens_dyn (:447-536) accumulated
+// per face node.
+__global__ void __launch_bounds__(256) massflux_kernel(const MflxArgs a, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint32_t o = (uint32_t)t;
+  if (a.rkstage == 1) {
+    stg(a.FX, o, a.twh * ldg(a.MX, o));
+    stg(a.FY, o, a.twh * ldg(a.MY, o));
+    stg(a.FZ, o, a.twv * ldg(a.MZ, o));
+  } else {
+    stg(a.FX, o, ldg(a.FX, o) + a.twh * ldg(a.MX, o));
+    stg(a.FY, o, ldg(a.FY, o) + a.twh * ldg(a.MY, o));
+    stg(a.FZ, o, ldg(a.FZ, o) + a.twv * ldg(a.MZ, o));
+  }
+}
+template <int N1>
+__global__ void __launch_bounds__(256) alphdens_kernel(const DevMesh m, const MflxArgs a, double gamm) {
+  constexpr int Nfp = N1 * N1, NfpTot = 6 * Nfp;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= NfpTot * m.Ne) return;
+  const uint32_t gf = (uint32_t)t;
+  const int ke = t / NfpTot, fp = t - ke * NfpTot;
+  const int f = fp / Nfp, an = fp - f * Nfp;
+  const uint32_t h = (uint32_t)(face_node<N1>(f, an) % Nfp) + (uint32_t)Nfp * m.EMap3Dto2D[ke];
+  const uint32_t iM = (uint32_t)m.vmapM[gf], iP = (uint32_t)m.vmapP[gf];
+  const double densM = ldg(a.DD, iM) + ldg(a.Dh, iM);
+  const double densP = ldg(a.DD, iP) + ldg(a.Dh, iP);
+  const double nx = ldg(m.nx, gf), ny = ldg(m.ny, gf), nz = ldg(m.nz, gf);
+  double Gnn = ldg(m.G11, h) * fabs(nx) + ldg(m.G22, h) * fabs(ny);
+  if (!a.hevi) Gnn = Gnn + fabs(nz);
+  const double VelM = (ldg(a.MX, iM) * nx + ldg(a.MY, iM) * ny + ldg(a.MZ, iM) * nz) / densM;
+  const double VelP = (ldg(a.MX, iP) * nx + ldg(a.MY, iP) * ny + ldg(a.MZ, iP) * nz) / densP;
+  const double cM = sqrt(Gnn * gamm * (ldg(a.Ph, iM) + ldg(a.DP, iM)) / densM) + fabs(VelM);
+  const double cP = sqrt(Gnn * gamm * (ldg(a.Ph, iP) + ldg(a.DP, iP)) / densP) + fabs(VelP);
+  const double alpha = cM > cP ? cM : cP;
+  const double tw = a.twh * (fabs(nx) + fabs(ny)) + a.twv * fabs(nz);
+  const double aM0 = a.rkstage == 1 ? 0.0 : ldg(a.alM, gf);
+  const double aP0 = a.rkstage == 1 ? 0.0 : ldg(a.alP, gf);
+  stg(a.alM, gf, aM0 + tw * alpha * densM * ldg(m.Gsqrt, iM));
+  stg(a.alP, gf, aP0 + tw * alpha * densP * ldg(m.Gsqrt, iP));
 }
 
 template <int N1>
@@ -1522,21 +1650,9 @@ int launch_heve_bnd(fedg_ctx* ctx, const TendIn& in, double* const dt[5], double
   }
 }
 
-int launch_trcadv(fedg_ctx* ctx, const TrcLaunch& t) {
-  const DevMesh& m = ctx->m;
-  TrcArgs a;
-  a.dt = t.dt; a.Q = t.Q; a.MX = t.MX; a.MY = t.MY; a.MZ = t.MZ;
-  a.aM = t.aM; a.aP = t.aP; a.R = t.R; a.tp = t.tp; a.J = t.J;
-  for (int l = 0; l < 8; ++l) a.w[l] = t.w[l];
-  auto go = [&](auto n1c) -> int {
-    constexpr int N1 = decltype(n1c)::value;
-    constexpr int EPB = epb_for<N1>();
-    const int nblk = (m.Ne + EPB - 1) / EPB;
-    hipLaunchKernelGGL((trcadv_kernel<N1, EPB>), dim3(nblk), dim3(EPB * N1 * N1 * N1), 0, ctx->stream, m, a);
-    FEDG_HIP_CHECK(hipGetLastError());
-    return FEDG_OK;
-  };
-  switch (m.n1) {
+template <class F>
+static int dispatch_n1(int n1, F&& go) {
+  switch (n1) {
     case 2: return go(std::integral_constant<int, 2>{});
     case 3: return go(std::integral_constant<int, 3>{});
     case 4: return go(std::integral_constant<int, 4>{});
@@ -1548,6 +1664,56 @@ int launch_trcadv(fedg_ctx* ctx, const TrcLaunch& t) {
       set_error("polynomial order not built (supported: p = 1..7)");
       return FEDG_ERR_UNSUPPORTED;
   }
+}
+
+int launch_trcadv(fedg_ctx* ctx, const TrcLaunch& t) {
+  const DevMesh& m = ctx->m;
+  TrcArgs a{};
+  a.dt = t.dt; a.Q = t.Q; a.MX = t.MX; a.MY = t.MY; a.MZ = t.MZ;
+  a.aM = t.aM; a.aP = t.aP; a.R = t.R; a.tp = t.tp; a.J = t.J;
+  a.Dh = t.Dh; a.DD = t.DD; a.DD0 = t.DD0; a.c = t.c; a.dtt = t.dtt; a.disable = t.disable;
+  for (int l = 0; l < 8; ++l) a.w[l] = t.w.w[l];
+  return dispatch_n1(m.n1, [&](auto n1c) -> int {
+    constexpr int N1 = decltype(n1c)::value;
+    constexpr int EPB = epb_for<N1>();
+    const int nblk = (m.Ne + EPB - 1) / EPB;
+    if (t.fct)
+      hipLaunchKernelGGL((trcadv_kernel<N1, EPB, TRC_FCT>), dim3(nblk), dim3(EPB * N1 * N1 * N1), 0, ctx->stream, m,
+                         a);
+    else
+      hipLaunchKernelGGL((trcadv_kernel<N1, EPB, TRC_TEND>), dim3(nblk), dim3(EPB * N1 * N1 * N1), 0, ctx->stream,
+                         m, a);
+    FEDG_HIP_CHECK(hipGetLastError());
+    return FEDG_OK;
+  });
+}
+
+int launch_tmar(fedg_ctx* ctx, double* Q, const double* Dh, const double* DD, const double* J, const TrcW& w) {
+  const DevMesh& m = ctx->m;
+  return dispatch_n1(m.n1, [&](auto n1c) -> int {
+    constexpr int N1 = decltype(n1c)::value;
+    constexpr int EPB = epb_for<N1>();
+    const int nblk = (m.Ne + EPB - 1) / EPB;
+    hipLaunchKernelGGL((tmar_kernel<N1, EPB>), dim3(nblk), dim3(EPB * N1 * N1 * N1), 0, ctx->stream, m, Q, Dh, DD, J,
+                       w);
+    FEDG_HIP_CHECK(hipGetLastError());
+    return FEDG_OK;
+  });
+}
+
+int launch_save_massflux(fedg_ctx* ctx, const MflxArgs& a) {
+  const DevMesh& m = ctx->m;
+  const int n = m.Np * m.Ne;
+  hipLaunchKernelGGL(massflux_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, a, n);
+  FEDG_HIP_CHECK(hipGetLastError());
+  const double gamm = ctx->c.CPdry / ctx->c.CVdry;
+  return dispatch_n1(m.n1, [&](auto n1c) -> int {
+    constexpr int N1 = decltype(n1c)::value;
+    const int nf = 6 * N1 * N1 * m.Ne;
+    hipLaunchKernelGGL((alphdens_kernel<N1>), dim3((nf + 255) / 256), dim3(256), 0, ctx->stream, m, a, gamm);
+    FEDG_HIP_CHECK(hipGetLastError());
+    return FEDG_OK;
+  });
 }
 
 int launch_hevi_explicit(fedg_ctx* ctx, const TendIn& in, double* const out[5]) {

@@ -4,6 +4,8 @@
                            (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve.F90:292-489)
 * ``trcadvect_cal_tend`` - atm_dyn_dgm_trcadvect3d_heve_cal_tend
                            (FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:149-231)
+* ``trcadvect_calc_fct_coef``, ``trcadvect_tmar``, ``trcadvect_save_massflux``
+                         - the limiter and mass-flux routines of the same file (:232-404)
 * ``calc_pressure``      - atm_dyn_dgm_nonhydro3d_common_DRHOT2PRES (..._common.F90:428-479)
 * ``halo_exchange``      - MeshFieldComm_Exchange on one tile (DAT/scale_meshfieldcomm_*.F90)
 * ``apply_bc``           - ATMOS_dyn_bnd_applyBC_prgvars_lc (FDS/scale_atm_dyn_dgm_bnd.F90:270-367)
@@ -101,6 +103,33 @@ def trcadvect_cal_tend(dm: DeviceMesh, QTRC_dt, QTRC, MOMX, MOMY, MOMZ, alphDENS
     (NfpTot,Ne), J is (Np,Ne), the rest (Np,NeA) device fields."""
     args = [_p(a) for a in (QTRC_dt, QTRC, MOMX, MOMY, MOMZ, alphDENS_M, alphDENS_P, fct_coef, RHOQ_tp, J)]
     check(LIB.fedg_trcadvect_cal_tend(dm.handle, *args), "fedg_trcadvect_cal_tend")
+
+
+def trcadvect_calc_fct_coef(dm: DeviceMesh, fct_coef, QTRC, MOMX, MOMY, MOMZ, RHOQ_tp, alphDENS_M, alphDENS_P,
+                            DENS_hyd, DDENS, DDENS0, rk_c_ssm1, dt, J, disable_limiter=False):
+    """FCT coefficient of each element, atm_dyn_dgm_trcadvect3d_heve_calc_fct_coef
+    (FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:232-305); written on the interior elements."""
+    args = [_p(a) for a in (fct_coef, QTRC, MOMX, MOMY, MOMZ, RHOQ_tp, alphDENS_M, alphDENS_P, DENS_hyd, DDENS,
+                            DDENS0)]
+    check(LIB.fedg_trcadvect_calc_fct_coef(dm.handle, *args, float(rk_c_ssm1), float(dt), _p(J),
+                                           int(bool(disable_limiter))), "fedg_trcadvect_calc_fct_coef")
+
+
+def trcadvect_tmar(dm: DeviceMesh, QTRC, DENS_hyd, DDENS, J):
+    """Truncation and mass-aware rescaling in place, atm_dyn_dgm_trcadvect3d_TMAR (..._trcadvect3d_heve.F90:308-337)."""
+    check(LIB.fedg_trcadvect_tmar(dm.handle, _p(QTRC), _p(DENS_hyd), _p(DDENS), _p(J)), "fedg_trcadvect_tmar")
+
+
+def trcadvect_save_massflux(dm: DeviceMesh, MFLX_x, MFLX_y, MFLX_z, alph_dens_M, alph_dens_P, DDENS, MOMX, MOMY, MOMZ,
+                            DPRES, DENS_hyd, PRES_hyd, Rtot, CVtot, CPtot, rkstage, tavg_weight_h, tavg_weight_v,
+                            is_hevi=False):
+    """Time-averaged mass fluxes and face damping coefficients of one RK stage,
+    atm_dyn_dgm_trcadvect3d_save_massflux (..._trcadvect3d_heve.F90:340-404)."""
+    args = [_p(a) for a in (MFLX_x, MFLX_y, MFLX_z, alph_dens_M, alph_dens_P, DDENS, MOMX, MOMY, MOMZ, DPRES,
+                            DENS_hyd, PRES_hyd, Rtot, CVtot, CPtot)]
+    check(LIB.fedg_trcadvect_save_massflux(dm.handle, *args, int(rkstage), float(tavg_weight_h),
+                                           float(tavg_weight_v), int(bool(is_hevi))),
+          "fedg_trcadvect_save_massflux")
 
 
 def hevi_cal_tend(dm: DeviceMesh, out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DPRES, DENS_hyd, PRES_hyd, PRES_hyd_ref,

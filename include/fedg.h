@@ -258,6 +258,47 @@ int fedg_trcadvect_cal_tend(fedg_ctx *ctx, double *QTRC_dt, const double *QTRC_,
                             const double *alphDENS_M, const double *alphDENS_P,
                             const double *fct_coef, const double *RHOQ_tp, const double *J);
 
+/* atm_dyn_dgm_trcadvect3d_heve_calc_fct_coef
+ * (FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:232-305, with
+ * get_netOutwardFlux_generalhvc, :676-797): per element
+ * fct_coef = max(0, min(1, Q / (netOutwardFlux + 1e-10))), Q the tracer mass
+ * available over dt, sum(Gsqrt J IntWeight_lgl (dens_ssm1 QTRC_ / dt + RHOQ_tp_))
+ * with dens_ssm1 = DENS_hyd + (1 - rk_c_ssm1) DDENS0_ + rk_c_ssm1 DDENS_, and
+ * netOutwardFlux the sum over the six faces of the positive LGL-weighted face
+ * integrals of the face flux of fedg_trcadvect_cal_tend.  disable_limiter != 0:
+ * fct_coef = 1 (every other pointer may then be NULL).  Interior elements are
+ * written; the halo is the caller's exchange, as in the reference driver. */
+int fedg_trcadvect_calc_fct_coef(fedg_ctx *ctx, double *fct_coef, const double *QTRC_,
+                                 const double *MOMX_, const double *MOMY_, const double *MOMZ_,
+                                 const double *RHOQ_tp_, const double *alphDENS_M,
+                                 const double *alphDENS_P, const double *DENS_hyd,
+                                 const double *DDENS_, const double *DDENS0_, double rk_c_ssm1,
+                                 double dt, const double *J, int disable_limiter);
+
+/* atm_dyn_dgm_trcadvect3d_TMAR (FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:308-337):
+ * in place, per element, QTRC_ = Q0 / (Q1 + 1e-32) * max(0, QTRC_) with Q0, Q1 the
+ * element masses sum(Gsqrt J IntWeight_lgl dens QTRC_) before and after the cut,
+ * dens = DENS_hyd + DDENS_. */
+int fedg_trcadvect_tmar(fedg_ctx *ctx, double *QTRC_, const double *DENS_hyd, const double *DDENS_,
+                        const double *J);
+
+/* atm_dyn_dgm_trcadvect3d_save_massflux (FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:340-404)
+ * with atm_dyn_dgm_trcadvect3d_heve_cal_alphdens_dyn (:447-536): at RK stage
+ * rkstage (from 1; 1 resets), MFLX_x/y_tavg += tavg_weight_h MOMX/MOMY,
+ * MFLX_z_tavg += tavg_weight_v MOMZ on the interior elements, and
+ * alph_dens_M/_P (NfpTot,Ne) += tavg_weight alpha dens Gsqrt on both sides of
+ * every face node, alpha the larger of the two sides' sqrt(Gnn gamma P / dens)
+ * + |normal velocity| (Gnn without |nz| when is_hevi).  As in the reference,
+ * gamma is the dry-air CPdry / CVdry; Rtot, CVtot, CPtot are accepted and
+ * unused. */
+int fedg_trcadvect_save_massflux(fedg_ctx *ctx, double *MFLX_x_tavg, double *MFLX_y_tavg,
+                                 double *MFLX_z_tavg, double *alph_dens_M, double *alph_dens_P,
+                                 const double *DDENS, const double *MOMX, const double *MOMY,
+                                 const double *MOMZ, const double *DPRES, const double *DENS_hyd,
+                                 const double *PRES_hyd, const double *Rtot, const double *CVtot,
+                                 const double *CPtot, int rkstage, double tavg_weight_h,
+                                 double tavg_weight_v, int is_hevi);
+
 /* atm_dyn_dgm_nonhydro3d_rhot_hevi_cal_tend
  * (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi.F90:289-482 with its numerical
  * flux FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi_numflux.F90:232-416): the

@@ -442,12 +442,65 @@ void oracle_heve_add_bnd_contrib(const oracle_mesh *m,
   free(del_flux);
 }
 
-/* FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:149-231 (cal_tend) with
- * get_delflux_generalhvc (:554-674).  FaceIntMat (:75-137) is applied as the
+/* FaceIntMat of atm_dyn_dgm_trcadvect3d_heve_Init (FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:75-137):
+ * the nonzero of each column, w(a0)*w(a1) of the 1D LGL weights w1d */
+static double *trc_face_weights(const oracle_mesh *m, const double *w1d) {
+  const int n1 = m->n1, Nfp_h = m->Nfp_h, Nfp_v = m->Nfp_v;
+  double *IntW = (double *)malloc(sizeof(double) * m->NfpTot);
+  for (int f = 0; f < 4; ++f)
+    for (int k = 0; k < n1; ++k)
+      for (int i = 0; i < n1; ++i) IntW[f * Nfp_h + i + n1 * k] = w1d[i] * w1d[k];
+  for (int f = 0; f < 2; ++f)
+    for (int j = 0; j < n1; ++j)
+      for (int i = 0; i < n1; ++i) IntW[4 * Nfp_h + f * Nfp_v + i + n1 * j] = w1d[i] * w1d[j];
+  return IntW;
+}
+
+/* The face flux of element ke shared by get_delflux_generalhvc (:554-674)
+ * and get_netOutwardFlux_generalhvc (:676-797): MomFlxM, numflux and the six
+ * LGL-weighted face integrals outward(f) of J(iM) Fscale numflux, applied as the
  * reference's CSR product (CMN/scale_sparsemat.F90:439-474): row f sums its
- * face's nodes in storage order from s = 0, weights w(a0)*w(a1) of the 1D
- * LGL weights w1d.  Div (ELM/..._tensorprod3D.F90) is the three line
- * contractions plus the lift of del_flux. */
+ * face's nodes in storage order from s = 0. */
+static void trc_face_flux(const oracle_mesh *m, const double *IntW, int ke, const double *QTRC_, const double *MOMX_,
+                          const double *MOMY_, const double *MOMZ_, const double *alphDENS_M,
+                          const double *alphDENS_P, const double *J, double *MomFlxM, double *numflux,
+                          double *outward) {
+  const int NfpTot = m->NfpTot, Nfp_h = m->Nfp_h, Nfp_v = m->Nfp_v, Ne = m->Ne;
+  const long NpNeA = (long)m->Np * m->NeA;
+  const double *G13 = m->GI3, *G23 = m->GI3 + NpNeA;
+  const double *nx = m->normal_fn, *ny = m->normal_fn + (long)NfpTot * Ne, *nz = m->normal_fn + 2L * NfpTot * Ne;
+  const int ke2d = m->EMap3Dto2D[ke] - 1;
+  const long of = (long)NfpTot * ke;
+  double b[6 * 64]; /* six faces of at most 8 x 8 nodes (p <= 7) */
+  for (int fp = 0; fp < NfpTot; ++fp) {
+    const long iM = m->vmapM[of + fp] - 1, iP = m->vmapP[of + fp] - 1;
+    const double GsH = m->GsqrtH[(m->IndexH2Dto3D_bnd[fp] - 1) + (long)Nfp_v * ke2d];
+    const double Gsqrt_M = m->Gsqrt[iM], Gsqrt_P = m->Gsqrt[iP];
+    const double GsqrtV_M = Gsqrt_M / GsH, GsqrtV_P = Gsqrt_P / GsH;
+    const double GMX_M = Gsqrt_M * MOMX_[iM], GMX_P = Gsqrt_P * MOMX_[iP];
+    const double GMY_M = Gsqrt_M * MOMY_[iM], GMY_P = Gsqrt_P * MOMY_[iP];
+    const double GMZ_M = Gsqrt_M * MOMZ_[iM], GMZ_P = Gsqrt_P * MOMZ_[iP];
+    MomFlxM[fp] = GMX_M * nx[of + fp] + GMY_M * ny[of + fp] +
+                  ((GMZ_M / GsqrtV_M + G13[iM] * GMX_M + G23[iM] * GMY_M) * nz[of + fp]);
+    const double MomFlxP = GMX_P * nx[of + fp] + GMY_P * ny[of + fp] +
+                           ((GMZ_P / GsqrtV_P + G13[iP] * GMX_P + G23[iP] * GMY_P) * nz[of + fp]);
+    numflux[fp] = 0.5 * ((QTRC_[iP] * MomFlxP + QTRC_[iM] * MomFlxM[fp]) - alphDENS_P[of + fp] * QTRC_[iP] +
+                         alphDENS_M[of + fp] * QTRC_[iM]);
+    b[fp] = J[iM] * m->Fscale[of + fp] * numflux[fp];
+  }
+  for (int f = 0; f < 6; ++f) {
+    const int is = f < 4 ? f * Nfp_h : 4 * Nfp_h + (f - 4) * Nfp_v, n = f < 4 ? Nfp_h : Nfp_v;
+    double s = 0.0;
+    for (int c = is; c < is + n; ++c) s = s + IntW[c] * b[c];
+    outward[f] = s;
+  }
+}
+
+/* FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:149-231 (cal_tend) with
+ * get_delflux_generalhvc (:554-674): del_flux with the FCT coefficient of the
+ * upwind side, sign(1, outward(f)) of the face's net flux; Div
+ * (ELM/..._tensorprod3D.F90) is the three line contractions plus the lift of
+ * del_flux. */
 void oracle_trcadvect_cal_tend(const oracle_mesh *m, const double *w1d, double *QTRC_dt, const double *QTRC_,
                                const double *MOMX_, const double *MOMY_, const double *MOMZ_,
                                const double *alphDENS_M, const double *alphDENS_P, const double *fct_coef,
@@ -456,56 +509,25 @@ void oracle_trcadvect_cal_tend(const oracle_mesh *m, const double *w1d, double *
   const long NpNeA = (long)Np * m->NeA;
   const long NpNe = (long)Np * Ne;
   const double *G13 = m->GI3, *G23 = m->GI3 + NpNeA;
-  const double *nx = m->normal_fn, *ny = m->normal_fn + (long)NfpTot * Ne, *nz = m->normal_fn + 2L * NfpTot * Ne;
-  double *IntW = (double *)malloc(sizeof(double) * NfpTot); /* nonzero of FaceIntMat in each column */
-  for (int f = 0; f < 4; ++f)
-    for (int k = 0; k < n1; ++k)
-      for (int i = 0; i < n1; ++i) IntW[f * Nfp_h + i + n1 * k] = w1d[i] * w1d[k];
-  for (int f = 0; f < 2; ++f)
-    for (int j = 0; j < n1; ++j)
-      for (int i = 0; i < n1; ++i) IntW[4 * Nfp_h + f * Nfp_v + i + n1 * j] = w1d[i] * w1d[j];
+  double *IntW = trc_face_weights(m, w1d);
 #pragma omp parallel
   {
-    double *MomFlxM = (double *)malloc(sizeof(double) * NfpTot), *MomFlxP = (double *)malloc(sizeof(double) * NfpTot);
-    double *numflux = (double *)malloc(sizeof(double) * NfpTot), *b = (double *)malloc(sizeof(double) * NfpTot);
-    double *QM = (double *)malloc(sizeof(double) * NfpTot), *RM = (double *)malloc(sizeof(double) * NfpTot);
-    double *RP = (double *)malloc(sizeof(double) * NfpTot), *del_flux = (double *)malloc(sizeof(double) * NfpTot);
+    double *MomFlxM = (double *)malloc(sizeof(double) * NfpTot), *numflux = (double *)malloc(sizeof(double) * NfpTot);
+    double *del_flux = (double *)malloc(sizeof(double) * NfpTot);
     double *Flux = (double *)malloc(sizeof(double) * Np * 3), *DFlux = (double *)malloc(sizeof(double) * Np * 4);
     double outward[6];
 #pragma omp for schedule(static)
     for (int ke = 0; ke < Ne; ++ke) {
       const int ke2d = m->EMap3Dto2D[ke] - 1;
       const long of = (long)NfpTot * ke, o = (long)Np * ke;
+      trc_face_flux(m, IntW, ke, QTRC_, MOMX_, MOMY_, MOMZ_, alphDENS_M, alphDENS_P, J, MomFlxM, numflux, outward);
       for (int fp = 0; fp < NfpTot; ++fp) {
         const long iM = m->vmapM[of + fp] - 1, iP = m->vmapP[of + fp] - 1;
-        const double GsH = m->GsqrtH[(m->IndexH2Dto3D_bnd[fp] - 1) + (long)Nfp_v * ke2d];
-        const double Gsqrt_M = m->Gsqrt[iM], Gsqrt_P = m->Gsqrt[iP];
-        const double GsqrtV_M = Gsqrt_M / GsH, GsqrtV_P = Gsqrt_P / GsH;
-        const double GMX_M = Gsqrt_M * MOMX_[iM], GMX_P = Gsqrt_P * MOMX_[iP];
-        const double GMY_M = Gsqrt_M * MOMY_[iM], GMY_P = Gsqrt_P * MOMY_[iP];
-        const double GMZ_M = Gsqrt_M * MOMZ_[iM], GMZ_P = Gsqrt_P * MOMZ_[iP];
-        MomFlxM[fp] = GMX_M * nx[of + fp] + GMY_M * ny[of + fp] +
-                      ((GMZ_M / GsqrtV_M + G13[iM] * GMX_M + G23[iM] * GMY_M) * nz[of + fp]);
-        MomFlxP[fp] = GMX_P * nx[of + fp] + GMY_P * ny[of + fp] +
-                      ((GMZ_P / GsqrtV_P + G13[iP] * GMX_P + G23[iP] * GMY_P) * nz[of + fp]);
-        QM[fp] = QTRC_[iM];
-        RM[fp] = fct_coef[iM];
-        RP[fp] = fct_coef[iP];
-        numflux[fp] = 0.5 * ((QTRC_[iP] * MomFlxP[fp] + QTRC_[iM] * MomFlxM[fp]) - alphDENS_P[of + fp] * QTRC_[iP] +
-                             alphDENS_M[of + fp] * QTRC_[iM]);
-        b[fp] = J[iM] * m->Fscale[of + fp] * numflux[fp];
-      }
-      for (int f = 0; f < 6; ++f) {
-        const int is = f < 4 ? f * Nfp_h : 4 * Nfp_h + (f - 4) * Nfp_v, n = f < 4 ? Nfp_h : Nfp_v;
-        double s = 0.0;
-        for (int c = is; c < is + n; ++c) s = s + IntW[c] * b[c];
-        outward[f] = s;
-      }
-      for (int fp = 0; fp < NfpTot; ++fp) {
+        const double RM = fct_coef[iM], RP = fct_coef[iP];
         const int f = fp < 4 * Nfp_h ? fp / Nfp_h : 4 + (fp - 4 * Nfp_h) / Nfp_v;
         del_flux[fp] = m->Fscale[of + fp] *
-                       (numflux[fp] * 0.5 * (RP[fp] + RM[fp] - (RP[fp] - RM[fp]) * copysign(1.0, outward[f])) -
-                        QM[fp] * MomFlxM[fp]);
+                       (numflux[fp] * 0.5 * (RP + RM - (RP - RM) * copysign(1.0, outward[f])) -
+                        QTRC_[iM] * MomFlxM[fp]);
       }
       for (int p = 0; p < Np; ++p) {
         const double Gsqrt_ = m->Gsqrt[o + p];
@@ -527,17 +549,124 @@ void oracle_trcadvect_cal_tend(const oracle_mesh *m, const double *w1d, double *
                          RHOQ_tp[o + p];
     }
     free(MomFlxM);
-    free(MomFlxP);
     free(numflux);
-    free(b);
-    free(QM);
-    free(RM);
-    free(RP);
     free(del_flux);
     free(Flux);
     free(DFlux);
   }
   free(IntW);
+}
+
+/* IntWeight_lgl of the hexahedron (ELM/scale_element_hexahedral.F90:286-298) */
+static double trc_vol_weight(const double *w1d, int n1, int p) {
+  const int i = p % n1, j = (p / n1) % n1, k = p / (n1 * n1);
+  return w1d[i] * w1d[j] * w1d[k];
+}
+
+/* FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:232-305 (calc_fct_coef) with
+ * get_netOutwardFlux_generalhvc (:676-797): net = sum_f max(0, outward(f)),
+ * Q = sum_p Gsqrt J IntWeight_lgl (dens_ssm1 QTRC / dt + RHOQ_tp), both summed
+ * in order from zero as Fortran's sum does. */
+void oracle_trcadvect_calc_fct_coef(const oracle_mesh *m, const double *w1d, double *fct_coef, const double *QTRC_,
+                                    const double *MOMX_, const double *MOMY_, const double *MOMZ_,
+                                    const double *RHOQ_tp_, const double *alphDENS_M, const double *alphDENS_P,
+                                    const double *DENS_hyd, const double *DDENS_, const double *DDENS0_,
+                                    double rk_c_ssm1, double dt, const double *J, int disable_limiter) {
+  const int Np = m->Np, NfpTot = m->NfpTot, Ne = m->Ne;
+  if (disable_limiter) {
+    for (long t = 0; t < (long)Np * Ne; ++t) fct_coef[t] = 1.0;
+    return;
+  }
+  double *IntW = trc_face_weights(m, w1d);
+#pragma omp parallel
+  {
+    double *MomFlxM = (double *)malloc(sizeof(double) * NfpTot), *numflux = (double *)malloc(sizeof(double) * NfpTot);
+    double outward[6];
+#pragma omp for schedule(static)
+    for (int ke = 0; ke < Ne; ++ke) {
+      const long o = (long)Np * ke;
+      trc_face_flux(m, IntW, ke, QTRC_, MOMX_, MOMY_, MOMZ_, alphDENS_M, alphDENS_P, J, MomFlxM, numflux, outward);
+      double net = 0.0;
+      for (int f = 0; f < 6; ++f) net = net + (outward[f] > 0.0 ? outward[f] : 0.0);
+      double Q = 0.0;
+      for (int p = 0; p < Np; ++p) {
+        const double dens_ssm1 = DENS_hyd[o + p] + (1.0 - rk_c_ssm1) * DDENS0_[o + p] + rk_c_ssm1 * DDENS_[o + p];
+        Q = Q + m->Gsqrt[o + p] * J[o + p] * trc_vol_weight(w1d, m->n1, p) *
+                    (dens_ssm1 * QTRC_[o + p] / dt + RHOQ_tp_[o + p]);
+      }
+      double r = Q / (net + 1.0e-10);
+      r = r < 1.0 ? r : 1.0;
+      r = r > 0.0 ? r : 0.0;
+      for (int p = 0; p < Np; ++p) fct_coef[o + p] = r;
+    }
+    free(MomFlxM);
+    free(numflux);
+  }
+  free(IntW);
+}
+
+/* FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:308-337 (TMAR) */
+void oracle_trcadvect_tmar(const oracle_mesh *m, const double *w1d, double *QTRC_, const double *DENS_hyd,
+                           const double *DDENS_, const double *J) {
+  const int Np = m->Np;
+#pragma omp parallel for schedule(static)
+  for (int ke = 0; ke < m->Ne; ++ke) {
+    const long o = (long)Np * ke;
+    double Q0 = 0.0, Q1 = 0.0;
+    for (int p = 0; p < Np; ++p) {
+      const double dens = DENS_hyd[o + p] + DDENS_[o + p];
+      const double q = QTRC_[o + p] > 0.0 ? QTRC_[o + p] : 0.0;
+      const double g = m->Gsqrt[o + p] * J[o + p] * trc_vol_weight(w1d, m->n1, p) * dens;
+      Q0 = Q0 + g * QTRC_[o + p];
+      Q1 = Q1 + g * q;
+    }
+    const double r = Q0 / (Q1 + 1.0e-32);
+    for (int p = 0; p < Np; ++p) QTRC_[o + p] = r * (QTRC_[o + p] > 0.0 ? QTRC_[o + p] : 0.0);
+  }
+}
+
+/* FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:340-404 (save_massflux) with
+ * cal_alphdens_dyn (:447-536); gamma = CPdry / CVdry as there */
+void oracle_trcadvect_save_massflux(const oracle_mesh *m, double *MFLX_x, double *MFLX_y, double *MFLX_z,
+                                    double *alph_M, double *alph_P, const double *DDENS, const double *MOMX,
+                                    const double *MOMY, const double *MOMZ, const double *DPRES,
+                                    const double *DENS_hyd, const double *PRES_hyd, int rkstage, double twh,
+                                    double twv, int is_hevi) {
+  const int Np = m->Np, NfpTot = m->NfpTot, Nfp_v = m->Nfp_v, Ne = m->Ne;
+  const double *nx = m->normal_fn, *ny = m->normal_fn + (long)NfpTot * Ne, *nz = m->normal_fn + 2L * NfpTot * Ne;
+  const double gamm = m->CPdry / m->CVdry;
+  for (long t = 0; t < (long)Np * Ne; ++t) {
+    if (rkstage == 1) {
+      MFLX_x[t] = twh * MOMX[t];
+      MFLX_y[t] = twh * MOMY[t];
+      MFLX_z[t] = twv * MOMZ[t];
+    } else {
+      MFLX_x[t] = MFLX_x[t] + twh * MOMX[t];
+      MFLX_y[t] = MFLX_y[t] + twh * MOMY[t];
+      MFLX_z[t] = MFLX_z[t] + twv * MOMZ[t];
+    }
+  }
+  if (rkstage == 1)
+    for (long t = 0; t < (long)NfpTot * Ne; ++t) alph_M[t] = alph_P[t] = 0.0;
+  for (int ke = 0; ke < Ne; ++ke) {
+    const int ke2d = m->EMap3Dto2D[ke] - 1;
+    for (int fp = 0; fp < NfpTot; ++fp) {
+      const long g = (long)NfpTot * ke + fp;
+      const long iM = m->vmapM[g] - 1, iP = m->vmapP[g] - 1;
+      const long h = (m->IndexH2Dto3D_bnd[fp] - 1) + (long)Nfp_v * ke2d;
+      const double densM = DDENS[iM] + DENS_hyd[iM], densP = DDENS[iP] + DENS_hyd[iP];
+      double Gnn = m->G11[h] * fabs(nx[g]) + m->G22[h] * fabs(ny[g]);
+      if (!is_hevi) Gnn = Gnn + fabs(nz[g]);
+      const double VelM = (MOMX[iM] * nx[g] + MOMY[iM] * ny[g] + MOMZ[iM] * nz[g]) / densM;
+      const double VelP = (MOMX[iP] * nx[g] + MOMY[iP] * ny[g] + MOMZ[iP] * nz[g]) / densP;
+      const double cM = sqrt(Gnn * gamm * (PRES_hyd[iM] + DPRES[iM]) / densM) + fabs(VelM);
+      const double cP = sqrt(Gnn * gamm * (PRES_hyd[iP] + DPRES[iP]) / densP) + fabs(VelP);
+      const double alpha = cM > cP ? cM : cP;
+      const double tw = twh * (fabs(nx[g]) + fabs(ny[g])) + twv * fabs(nz[g]);
+      alph_M[g] = alph_M[g] + tw * alpha * densM * m->Gsqrt[iM];
+      alph_P[g] = alph_P[g] + tw * alpha * densP * m->Gsqrt[iP];
+    }
+  }
 }
 
 /* ---------------------------------------------------------------------- */

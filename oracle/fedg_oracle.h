@@ -121,6 +121,22 @@ void oracle_trcadvect_cal_tend(const oracle_mesh *m, const double *w1d, double *
                                const double *alphDENS_M, const double *alphDENS_P, const double *fct_coef,
                                const double *RHOQ_tp, const double *J);
 
+/* FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:232-305 (calc_fct_coef) */
+void oracle_trcadvect_calc_fct_coef(const oracle_mesh *m, const double *w1d, double *fct_coef, const double *QTRC_,
+                                    const double *MOMX_, const double *MOMY_, const double *MOMZ_,
+                                    const double *RHOQ_tp_, const double *alphDENS_M, const double *alphDENS_P,
+                                    const double *DENS_hyd, const double *DDENS_, const double *DDENS0_,
+                                    double rk_c_ssm1, double dt, const double *J, int disable_limiter);
+/* FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:308-337 (TMAR), in place */
+void oracle_trcadvect_tmar(const oracle_mesh *m, const double *w1d, double *QTRC_, const double *DENS_hyd,
+                           const double *DDENS_, const double *J);
+/* FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:340-404 (save_massflux with cal_alphdens_dyn) */
+void oracle_trcadvect_save_massflux(const oracle_mesh *m, double *MFLX_x, double *MFLX_y, double *MFLX_z,
+                                    double *alph_M, double *alph_P, const double *DDENS, const double *MOMX,
+                                    const double *MOMY, const double *MOMZ, const double *DPRES,
+                                    const double *DENS_hyd, const double *PRES_hyd, int rkstage, double twh,
+                                    double twv, int is_hevi);
+
 /* FDS/scale_atm_dyn_dgm_nonhydro3d_common.F90:624-777, hydrostatic pressure gradient */
 void oracle_calc_phyd_hgrad(const oracle_mesh *m, double *DPhydDx, double *DPhydDy,
                             const double *PRES_hyd, const double *PRES_hyd_ref);

@@ -54,6 +54,9 @@ def _load():
     lib.oracle_heve_add_bnd_contrib.argtypes = [M] + [dp] * 17
     lib.oracle_calc_phyd_hgrad.argtypes = [M] + [dp] * 4
     lib.oracle_trcadvect_cal_tend.argtypes = [M] + [dp] * 11
+    lib.oracle_trcadvect_calc_fct_coef.argtypes = [M] + [dp] * 12 + [C.c_double, C.c_double, dp, C.c_int]
+    lib.oracle_trcadvect_tmar.argtypes = [M] + [dp] * 5
+    lib.oracle_trcadvect_save_massflux.argtypes = [M] + [dp] * 12 + [C.c_int, C.c_double, C.c_double, C.c_int]
     lib.oracle_rk_tab_get.argtypes = [C.c_char_p, C.c_double, C.POINTER(RkTabC)]
     lib.oracle_rk_advance.argtypes = [C.POINTER(RkTabC), C.c_int, C.c_long] + [dp] * 5 + [C.c_long]
     lib.oracle_rk_store_impl.argtypes = [C.POINTER(RkTabC), C.c_int, C.c_long] + [dp] * 4 + [C.c_long]
@@ -200,6 +203,30 @@ class OracleMesh:
         LIB.oracle_trcadvect_cal_tend(self.ref, _d(m.elem.w1d), _d(out), _d(QTRC), _d(MOMX), _d(MOMY), _d(MOMZ),
                                       _d(alphDENS_M), _d(alphDENS_P), _d(fct_coef), _d(RHOQ_tp), _d(J))
         return out
+
+    def trcadvect_calc_fct_coef(self, QTRC, MOMX, MOMY, MOMZ, RHOQ_tp, alphDENS_M, alphDENS_P, DENS_hyd, DDENS,
+                                DDENS0, rk_c_ssm1, dt, J=None, disable_limiter=False):
+        """FCT coefficient (Np,NeA), interior elements written (the halo left zero)."""
+        m = self.mesh
+        out = m.field()
+        J = m.J if J is None else J
+        LIB.oracle_trcadvect_calc_fct_coef(self.ref, _d(m.elem.w1d), _d(out), _d(QTRC), _d(MOMX), _d(MOMY), _d(MOMZ),
+                                           _d(RHOQ_tp), _d(alphDENS_M), _d(alphDENS_P), _d(DENS_hyd), _d(DDENS),
+                                           _d(DDENS0), float(rk_c_ssm1), float(dt), _d(J), int(bool(disable_limiter)))
+        return out
+
+    def trcadvect_tmar(self, QTRC, DENS_hyd, DDENS, J=None):
+        """TMAR in place on QTRC."""
+        m = self.mesh
+        J = m.J if J is None else J
+        LIB.oracle_trcadvect_tmar(self.ref, _d(m.elem.w1d), _d(QTRC), _d(DENS_hyd), _d(DDENS), _d(J))
+
+    def trcadvect_save_massflux(self, MFLX, alph, DDENS, MOMX, MOMY, MOMZ, DPRES, DENS_hyd, PRES_hyd, rkstage,
+                                twh, twv, is_hevi=False):
+        """save_massflux in place on MFLX = (x, y, z) and alph = (M, P)."""
+        LIB.oracle_trcadvect_save_massflux(self.ref, *[_d(a) for a in MFLX], *[_d(a) for a in alph], _d(DDENS),
+                                           _d(MOMX), _d(MOMY), _d(MOMZ), _d(DPRES), _d(DENS_hyd), _d(PRES_hyd),
+                                           int(rkstage), float(twh), float(twv), int(bool(is_hevi)))
 
     # --- HEVI ---------------------------------------------------------------
     def hevi_numflux(self, q, DPRES, bs):

@@ -68,3 +68,72 @@ def test_trcadvect_cal_tend_bitwise(case):
     assert np.all(np.isfinite(g))
     assert np.array_equal(g, ref[:n]), f"max abs diff {np.max(np.abs(g - ref[:n]))}"
     dm.close()
+
+
+def _tracer_state(mesh, rng):
+    n = mesh.Np * mesh.Ne
+    f = {k: mesh.field() for k in ("Q", "MX", "MY", "MZ", "tp", "Dh", "DD", "DD0", "DP", "Ph")}
+    f["Q"][:n] = 1.0e-3 * (1.0 + 0.6 * rng.standard_normal(n))  # some negative values for TMAR
+    f["MX"][:n], f["MY"][:n] = 20.0 * rng.standard_normal(n), 15.0 * rng.standard_normal(n)
+    f["MZ"][:n] = 3.0 * rng.standard_normal(n)
+    f["tp"][:n] = 1.0e-6 * rng.standard_normal(n)
+    f["Dh"][:n] = 1.0 + 0.1 * rng.uniform(-1.0, 1.0, n)
+    f["DD"][:n], f["DD0"][:n] = 0.02 * rng.standard_normal(n), 0.02 * rng.standard_normal(n)
+    f["Ph"][:n] = 9.0e4 + 1.0e3 * rng.standard_normal(n)
+    f["DP"][:n] = 50.0 * rng.standard_normal(n)
+    for a in f.values():
+        mesh.exchange(a)
+    return f
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"p{c[0]}" for c in CASES])
+def test_trcadvect_fct_tmar_massflux_bitwise(case):
+    porder, NeX, NeY, NeZ, periodic = case
+    print(f"trcadvect fct/tmar/massflux p={porder} {NeX}x{NeY}x{NeZ}", flush=True)
+    mesh = BoxMesh(porder, NeX, NeY, NeZ, (0.0, 2.0e3 * NeX), (0.0, 1.5e3 * NeY), (0.0, 1.0e3 * NeZ),
+                   periodic=periodic)
+    rng = np.random.default_rng(200 + porder)
+    random_metrics(mesh, rng)
+    om = O.OracleMesh(mesh)
+    dm = fedg.DeviceMesh(mesh)
+    n, nA, nf = mesh.Np * mesh.Ne, mesh.Np * mesh.NeA, mesh.NfpTot * mesh.Ne
+    f = _tracer_state(mesh, rng)
+    d = lambda a: fedg.DevField(a.size, a)
+    g = {k: d(v) for k, v in f.items()}
+    J = d(mesh.J)
+    # save_massflux: stage 1 resets, stage 2 accumulates; HEVE and HEVI forms
+    for is_hevi in (False, True):
+        MF = [mesh.field(5.0) for _ in range(3)]
+        al = [np.full(nf, 2.0), np.full(nf, 2.0)]
+        gMF, gal = [d(a) for a in MF], [d(a) for a in al]
+        for stage, tw in ((1, (0.25, 0.5)), (2, (0.375, 0.125))):
+            om.trcadvect_save_massflux(MF, al, f["DD"], f["MX"], f["MY"], f["MZ"], f["DP"], f["Dh"], f["Ph"],
+                                       stage, tw[0], tw[1], is_hevi)
+            fedg.trcadvect_save_massflux(dm, *gMF, *gal, g["DD"], g["MX"], g["MY"], g["MZ"], g["DP"], g["Dh"],
+                                         g["Ph"], None, None, None, stage, tw[0], tw[1], is_hevi)
+            for a, b in zip(gMF, MF):
+                assert np.array_equal(a.download()[:n], b[:n]), f"MFLX stage {stage} hevi={is_hevi}"
+            for a, b in zip(gal, al):
+                assert np.array_equal(a.download(), b), f"alph_dens stage {stage} hevi={is_hevi}"
+    aM, aP = al
+    gaM, gaP = d(aM), d(aP)
+    # FCT coefficient, limiter on and off
+    for c, dt in ((1.0, 30.0), (0.4, 5.0)):
+        ref = om.trcadvect_calc_fct_coef(f["Q"], f["MX"], f["MY"], f["MZ"], f["tp"], aM, aP, f["Dh"], f["DD"],
+                                         f["DD0"], c, dt)
+        out = d(np.full(nA, np.nan))
+        fedg.trcadvect_calc_fct_coef(dm, out, g["Q"], g["MX"], g["MY"], g["MZ"], g["tp"], gaM, gaP, g["Dh"],
+                                     g["DD"], g["DD0"], c, dt, J)
+        r = out.download()[:n]
+        assert np.array_equal(r, ref[:n]), f"fct c={c}: max abs diff {np.max(np.abs(r - ref[:n]))}"
+        assert 0.0 <= r.min() and r.max() <= 1.0
+    out = d(np.full(nA, np.nan))
+    fedg.trcadvect_calc_fct_coef(dm, out, None, None, None, None, None, None, None, None, None, None, 1.0, 1.0, None,
+                                 True)
+    assert np.all(out.download()[:n] == 1.0)
+    # TMAR in place
+    Q = f["Q"].copy()
+    om.trcadvect_tmar(Q, f["Dh"], f["DD"])
+    fedg.trcadvect_tmar(dm, g["Q"], g["Dh"], g["DD"], J)
+    assert np.array_equal(g["Q"].download()[:n], Q[:n])
+    dm.close()
