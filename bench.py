@@ -23,8 +23,12 @@ RK), and the HEVI step of the 240x16x8 p=7 inertia-gravity-wave channel
 (IMEX_ARK324, dt = 0.05 s) with the reference's single Newton iteration and
 with Newton to a relative residual of 1e-10.
 
+forms also times one tracer's advection tendency and FCT coefficient
+(atm_dyn_dgm_trcadvect3d_heve) on the same 64x64x16 p=7 mesh.
+
 --workload selects what is timed alone (for profiling): heve (default),
-heve-fp32 or hevi (the HEVI step with the reference's single iteration).
+heve-fp32, hevi (the HEVI step with the reference's single iteration) or
+trcadv (the tracer tendency and FCT coefficient, --steps calls each).
 """
 import argparse
 import json
@@ -109,13 +113,59 @@ def time_hevi(fedg, cases, steps, warmup, max_it, rtol, case=None, dm=None):
             "newton_iterations_per_solve": (it - it0) / max(calls - calls0, 1), "max_it": max_it, "rtol": rtol}, dm
 
 
+# tracer advection tendency, per node at p=7: 13 node fields read (QTRC_,
+# MOMX/Y/Z, fct_coef, RHOQ_tp, J, Gsqrt, G13, G23, E11, E22, E33) and one
+# written, plus per face node (0.75 per node) alphDENS_M/_P, the normal, Fscale
+# and the two vmap indices
+TRC_BYTES_PER_NODE = 14 * 8 + 0.75 * (6 * 8 + 2 * 4)
+
+
+def time_trcadv(fedg, mesh, bs, q, dm, reps):
+    """One tracer's advection tendency and FCT coefficient on the stage mesh,
+    host-timed per call (each call ends with a stream synchronisation)."""
+    import numpy as np
+    n, nA, nf = mesh.Np * mesh.Ne, mesh.Np * mesh.NeA, mesh.NfpTot * mesh.Ne
+    dens = bs.DENS_hyd + q[0]
+    x = np.zeros(nA)
+    x[:n] = mesh.x
+    Q = 1.0e-3 * (1.0 + 0.5 * np.sin(2.0 * np.pi * x / 160.0e3))
+    f = lambda a: fedg.DevField(a.size, a)
+    d = dict(Q=f(Q), MX=f(dens * 10.0), MY=f(dens * 5.0), MZ=f(dens * 0.5), R=f(np.ones(nA)), tp=f(np.zeros(nA)),
+             J=f(mesh.J), Dh=f(bs.DENS_hyd), DD=f(q[0]), aM=f(np.full(nf, 3.0)), aP=f(np.full(nf, 3.0)),
+             out=fedg.DevField(nA))
+    del dens, x, Q
+
+    def tend():
+        fedg.trcadvect_cal_tend(dm, d["out"], d["Q"], d["MX"], d["MY"], d["MZ"], d["aM"], d["aP"], d["R"], d["tp"],
+                                d["J"])
+
+    def fct():
+        fedg.trcadvect_calc_fct_coef(dm, d["out"], d["Q"], d["MX"], d["MY"], d["MZ"], d["tp"], d["aM"], d["aP"],
+                                     d["Dh"], d["DD"], d["DD"], 1.0, 0.04, d["J"])
+
+    res = {}
+    for name, fn in (("cal_tend", tend), ("calc_fct_coef", fct)):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        res["ms_" + name] = (time.perf_counter() - t0) * 1e3 / reps
+    for v in d.values():
+        v.free()
+    achieved = TRC_BYTES_PER_NODE * n / (res["ms_cal_tend"] * 1e-3) / 1e9
+    res.update({"nodes": n, "bytes_per_node": TRC_BYTES_PER_NODE, "cal_tend_GBs": achieved,
+                "cal_tend_frac_of_hbm_peak": achieved / HBM_PEAK_GBS, "reps": reps,
+                "timing": "host wall per call, stream synchronised"})
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("heve", "heve-fp32", "hevi"), default="heve")
+    ap.add_argument("--workload", choices=("heve", "heve-fp32", "hevi", "trcadv"), default="heve")
     ap.add_argument("--no-forms", action="store_true", help="skip the FP32 and HEVI lines of the default run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed steps, run one device copy of Np*Ne doubles (copy_kernel), the "
@@ -135,6 +185,14 @@ def main():
         r, _ = time_hevi(fedg, cases, args.steps, args.warmup, 1, 0.0)
         print(json.dumps({"metric": "DOF·stage/s", "workload": "HEVI inertia-gravity wave 240x16x8 p=7, "
                           "IMEX_ARK324 dt=0.05 s, one Newton iteration", **r}))
+        return
+    if args.workload == "trcadv":
+        case = cases.config2()
+        mesh, bs, q = cases.build(case)
+        dm = fedg.DeviceMesh(mesh)
+        r = time_trcadv(fedg, mesh, bs, q, dm, args.steps)
+        dm.close()
+        print(json.dumps({"metric": "ms per call", "workload": "tracer advection of one tracer, 64x64x16 p=7", **r}))
         return
     bits = 32 if args.workload == "heve-fp32" else 64
     h = time_heve(fedg, cases, bits, args.steps, args.warmup)
@@ -191,6 +249,7 @@ def main():
             "heve_fp32": {"ms_per_step": f32["wall"] * 1e3 / args.steps, "value": dof * 4 * args.steps / f32["wall"],
                           "ms_stage_kernel": f32["ms_kernel"]},
             "hevi_240x16x8_p7": {"one_iteration": hevi1, "newton_1e-10": hevin},
+            "tracer_advection_64x64x16_p7": time_trcadv(fedg, mesh, bs, q, dm, 10),
         }
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", str(min(os.cpu_count() or 1, 16))))

@@ -733,14 +733,18 @@ int fedg_heve_add_bnd_contrib(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, d
 // ELM/scale_element_hexahedral.F90:286-298) are products of the 1D LGL
 // weights of the element's order
 static int trc_weights(fedg_ctx* ctx, TrcW& w) {
-  RefElem e;
-  try {
-    build_refelem(ctx->m.n1 - 1, e);
-  } catch (const std::exception& ex) {
-    set_error(ex.what());
-    return FEDG_ERR_ARG;
+  if (!ctx->have_w1d) {
+    RefElem e;
+    try {
+      build_refelem(ctx->m.n1 - 1, e);
+    } catch (const std::exception& ex) {
+      set_error(ex.what());
+      return FEDG_ERR_ARG;
+    }
+    for (int l = 0; l < 8; ++l) ctx->w1d[l] = l < ctx->m.n1 ? e.w1d[l] : 0.0;
+    ctx->have_w1d = 1;
   }
-  for (int l = 0; l < 8; ++l) w.w[l] = l < ctx->m.n1 ? e.w1d[l] : 0.0;
+  for (int l = 0; l < 8; ++l) w.w[l] = ctx->w1d[l];
   return FEDG_OK;
 }
 

@@ -77,6 +77,8 @@ struct fedg_ctx {
   int device;
   int ncu;               // compute units of the device
   fedg::HeviWork* hevi;  // implicit-solve workspace, allocated on first use
+  int have_w1d;          // w1d holds the 1D LGL weights of the order (first tracer call)
+  double w1d[8];
 };
 
 namespace fedg {
