@@ -1230,6 +1230,7 @@ __global__ void __launch_bounds__(EPB * N1 * N1 * N1) trcadv_kernel(const DevMes
   // FaceIntMat x (J Fscale numflux): row f sums its face's nodes in order from 0
   if (valid && p < 6) {
     double s = 0.0;
+#pragma unroll 8
     for (int an = 0; an < Nfp; ++an) s = s + (a.w[an % N1] * a.w[an / N1]) * sJF[e][p * Nfp + an];
     sOut[e][p] = s;
   }
@@ -1239,6 +1240,7 @@ __global__ void __launch_bounds__(EPB * N1 * N1 * N1) trcadv_kernel(const DevMes
       double net = 0.0;
       for (int f = 0; f < 6; ++f) net = net + (sOut[e][f] > 0.0 ? sOut[e][f] : 0.0);
       double q = 0.0;
+#pragma unroll 16
       for (int n = 0; n < Np; ++n) q = q + sM[e][n];
       double r = q / (net + 1.0e-10);
       r = r < 1.0 ? r : 1.0;
@@ -1323,8 +1325,11 @@ __global__ void __launch_bounds__(EPB * N1 * N1 * N1)
   __syncthreads();
   if (valid && p == 0) {
     double q0 = 0.0, q1 = 0.0;
-    for (int n = 0; n < Np; ++n) q0 = q0 + s0[e][n];
-    for (int n = 0; n < Np; ++n) q1 = q1 + s1[e][n];
+#pragma unroll 16
+    for (int n = 0; n < Np; ++n) {
+      q0 = q0 + s0[e][n];
+      q1 = q1 + s1[e][n];
+    }
     sr[e] = q0 / (q1 + 1.0e-32);
   }
   __syncthreads();

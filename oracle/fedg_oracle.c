@@ -26,6 +26,15 @@
  *  - RK tables / advance: the reference's test FElib/test/common/timeint_rk
  *    (order of convergence on the harmonic oscillator) and the Butcher
  *    coefficients in closed form;
+ *  - tracer advection (cal_tend, calc_fct_coef, TMAR, save_massflux,
+ *    tests/test_oracle_trcadv.py): advection of a global polynomial exact
+ *    away from the boundaries; the FCT coefficient of the upwind element
+ *    against LGL-weighted face sums whose sign differs from the plain sum;
+ *    conservation with random metrics; the limiter drains exactly the
+ *    available mass (and moves each element's mass one element downstream
+ *    when every limiter is active); positivity; TMAR keeps element masses and
+ *    cuts negatives; damping coefficients = sound speed + |normal velocity|
+ *    and equal on both sides of every face;
  *  - halo exchange / BC: the reference's mesh/field test semantics
  *    (FElib/test/FE/field_cubedom3d_hexahedral) on integer-valued fields, and
  *    the SLIP condition's closed form (exterior normal momentum = -interior).
