@@ -113,11 +113,13 @@ def time_hevi(fedg, cases, steps, warmup, max_it, rtol, case=None, dm=None):
             "newton_iterations_per_solve": (it - it0) / max(calls - calls0, 1), "max_it": max_it, "rtol": rtol}, dm
 
 
-# tracer advection tendency, per node at p=7: 13 node fields read (QTRC_,
-# MOMX/Y/Z, fct_coef, RHOQ_tp, J, Gsqrt, G13, G23, E11, E22, E33) and one
-# written, plus per face node (0.75 per node) alphDENS_M/_P, the normal, Fscale
-# and the two vmap indices
-TRC_BYTES_PER_NODE = 14 * 8 + 0.75 * (6 * 8 + 2 * 4)
+# tracer advection tendency, per node at p=7.  General metrics: 13 node fields
+# read (QTRC_, MOMX/Y/Z, fct_coef, RHOQ_tp, J, Gsqrt, G13, G23, E11, E22, E33)
+# and one written, plus per face node (0.75 per node) alphDENS_M/_P, the
+# normal, Fscale and the two vmap indices.  Flat box (metrics constant): six
+# node fields read (J only at face nodes) and one written, plus per face node
+# alphDENS_M/_P, J and the two vmap indices.
+TRC_BYTES_PER_NODE = {False: 14 * 8 + 0.75 * (6 * 8 + 2 * 4), True: 7 * 8 + 0.75 * (3 * 8 + 2 * 4)}
 
 
 def time_trcadv(fedg, mesh, bs, q, dm, reps):
@@ -152,8 +154,9 @@ def time_trcadv(fedg, mesh, bs, q, dm, reps):
         res["ms_" + name] = (time.perf_counter() - t0) * 1e3 / reps
     for v in d.values():
         v.free()
-    achieved = TRC_BYTES_PER_NODE * n / (res["ms_cal_tend"] * 1e-3) / 1e9
-    res.update({"nodes": n, "bytes_per_node": TRC_BYTES_PER_NODE, "cal_tend_GBs": achieved,
+    bpn = TRC_BYTES_PER_NODE[dm.flat]
+    achieved = bpn * n / (res["ms_cal_tend"] * 1e-3) / 1e9
+    res.update({"nodes": n, "flat_metrics": dm.flat, "bytes_per_node": bpn, "cal_tend_GBs": achieved,
                 "cal_tend_frac_of_hbm_peak": achieved / HBM_PEAK_GBS, "reps": reps,
                 "timing": "host wall per call, stream synchronised"})
     return res

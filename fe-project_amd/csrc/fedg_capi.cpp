@@ -589,6 +589,8 @@ int fedg_ctx_destroy(fedg_ctx* ctx) {
   return FEDG_OK;
 }
 
+int fedg_ctx_is_flat(const fedg_ctx* ctx) { return ctx ? ctx->m.flat : 0; }
+
 int fedg_ctx_sizes(const fedg_ctx* ctx, fedg_box_sizes* s) {
   if (!ctx || !s) return FEDG_ERR_ARG;
   const DevMesh& m = ctx->m;

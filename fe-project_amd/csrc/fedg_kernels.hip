@@ -1161,7 +1161,11 @@ enum { TRC_TEND = 0, TRC_FCT = 1 };
 // the same face flux), against the tracer mass available over dt,
 // sum(Gsqrt J IntWeight_lgl (dens_ssm1 QTRC / dt + RHOQ_tp)), summed node by
 // node in order by one thread.
-template <int N1, int EPB, int KIND>
+// FLAT: the flat-terrain uniform box the context detected (Gsqrt = GsqrtH = 1,
+// GI3 = 0, axis-aligned unit normals, Escale and Fscale constant per
+// direction and face): the same expressions with those exact constants in
+// place of the loads.
+template <int N1, int EPB, int KIND, bool FLAT>
 __global__ void __launch_bounds__(EPB * N1 * N1 * N1) trcadv_kernel(const DevMesh m, const TrcArgs a) {
   constexpr int Np = N1 * N1 * N1, Nfp = N1 * N1, NfpTot = 6 * Nfp;
   constexpr bool TEND = KIND == TRC_TEND;
@@ -1197,16 +1201,18 @@ __global__ void __launch_bounds__(EPB * N1 * N1 * N1) trcadv_kernel(const DevMes
       const uint32_t gf = (uint32_t)fp + (uint32_t)NfpTot * ke;
       const uint32_t iM = (uint32_t)m.vmapM[gf], iP = (uint32_t)m.vmapP[gf];
       const int f = fp / Nfp, an = fp - f * Nfp;
-      const double GsH = ldg(m.GsqrtH, (uint32_t)(face_node<N1>(f, an) % Nfp) + (uint32_t)Nfp * ke2d);
-      const double Gsqrt_M = ldg(m.Gsqrt, iM), Gsqrt_P = ldg(m.Gsqrt, iP);
+      const double GsH = FLAT ? 1.0 : ldg(m.GsqrtH, (uint32_t)(face_node<N1>(f, an) % Nfp) + (uint32_t)Nfp * ke2d);
+      const double Gsqrt_M = FLAT ? 1.0 : ldg(m.Gsqrt, iM), Gsqrt_P = FLAT ? 1.0 : ldg(m.Gsqrt, iP);
       const double GsqrtV_M = Gsqrt_M / GsH, GsqrtV_P = Gsqrt_P / GsH;
-      const double G13_M = ldg(m.G13, iM), G13_P = ldg(m.G13, iP);
-      const double G23_M = ldg(m.G23, iM), G23_P = ldg(m.G23, iP);
+      const double G13_M = FLAT ? 0.0 : ldg(m.G13, iM), G13_P = FLAT ? 0.0 : ldg(m.G13, iP);
+      const double G23_M = FLAT ? 0.0 : ldg(m.G23, iM), G23_P = FLAT ? 0.0 : ldg(m.G23, iP);
       const double Q_M = ldg(a.Q, iM), Q_P = ldg(a.Q, iP);
       const double GMX_M = Gsqrt_M * ldg(a.MX, iM), GMX_P = Gsqrt_P * ldg(a.MX, iP);
       const double GMY_M = Gsqrt_M * ldg(a.MY, iM), GMY_P = Gsqrt_P * ldg(a.MY, iP);
       const double GMZ_M = Gsqrt_M * ldg(a.MZ, iM), GMZ_P = Gsqrt_P * ldg(a.MZ, iP);
-      const double nx = ldg(m.nx, gf), ny = ldg(m.ny, gf), nz = ldg(m.nz, gf);
+      const double nx = FLAT ? (f == 1 ? 1.0 : f == 3 ? -1.0 : 0.0) : ldg(m.nx, gf);
+      const double ny = FLAT ? (f == 2 ? 1.0 : f == 0 ? -1.0 : 0.0) : ldg(m.ny, gf);
+      const double nz = FLAT ? (f == 5 ? 1.0 : f == 4 ? -1.0 : 0.0) : ldg(m.nz, gf);
       const double MF_M = GMX_M * nx + GMY_M * ny + ((GMZ_M / GsqrtV_M + G13_M * GMX_M + G23_M * GMY_M) * nz);
       const double MF_P = GMX_P * nx + GMY_P * ny + ((GMZ_P / GsqrtV_P + G13_P * GMX_P + G23_P * GMY_P) * nz);
       const double nf = 0.5 * ((Q_P * MF_P + Q_M * MF_M) - ldg(a.aP, gf) * Q_P + ldg(a.aM, gf) * Q_M);
@@ -1216,14 +1222,14 @@ __global__ void __launch_bounds__(EPB * N1 * N1 * N1) trcadv_kernel(const DevMes
         sRP[e][fp] = ldg(a.R, iP);
         sRM[e][fp] = ldg(a.R, iM);
       }
-      sJF[e][fp] = ldg(a.J, iM) * ldg(m.Fscale, gf) * nf;
+      sJF[e][fp] = ldg(a.J, iM) * (FLAT ? m.fF[f] : ldg(m.Fscale, gf)) * nf;
     }
   }
   if constexpr (!TEND) {
     if (valid) {
       const double IW = (a.w[i] * a.w[j]) * a.w[k];
       const double dens = ldg(a.Dh, o) + (1.0 - a.c) * ldg(a.DD0, o) + a.c * ldg(a.DD, o);
-      sM[e][p] = ldg(m.Gsqrt, o) * ldg(a.J, o) * IW * (dens * ldg(a.Q, o) / a.dtt + ldg(a.tp, o));
+      sM[e][p] = (FLAT ? 1.0 : ldg(m.Gsqrt, o)) * ldg(a.J, o) * IW * (dens * ldg(a.Q, o) / a.dtt + ldg(a.tp, o));
     }
   }
   __syncthreads();
@@ -1256,19 +1262,21 @@ __global__ void __launch_bounds__(EPB * N1 * N1 * N1) trcadv_kernel(const DevMes
         const uint32_t gf = (uint32_t)fp + (uint32_t)NfpTot * ke;
         const double RP = sRP[e][fp], RM = sRM[e][fp];
         const double sg = copysign(1.0, sOut[e][fp / Nfp]);
-        sDF[e][fp] = ldg(m.Fscale, gf) * (sNF[e][fp] * 0.5 * (RP + RM - (RP - RM) * sg) - sQF[e][fp]);
+        sDF[e][fp] = (FLAT ? m.fF[fp / Nfp] : ldg(m.Fscale, gf)) *
+                     (sNF[e][fp] * 0.5 * (RP + RM - (RP - RM) * sg) - sQF[e][fp]);
       }
     }
     // volume flux
     double Gs = 1.0;
     if (valid) {
-      Gs = ldg(m.Gsqrt, o);
+      Gs = FLAT ? 1.0 : ldg(m.Gsqrt, o);
       const double RGsqrt = 1.0 / Gs;
-      const double RGsqrtV = ldg(m.GsqrtH, (uint32_t)(i + N1 * j) + (uint32_t)Nfp * ke2d) * RGsqrt;
+      const double RGsqrtV = (FLAT ? 1.0 : ldg(m.GsqrtH, (uint32_t)(i + N1 * j) + (uint32_t)Nfp * ke2d)) * RGsqrt;
       const double Qo = ldg(a.Q, o), mx = ldg(a.MX, o), my = ldg(a.MY, o), mz = ldg(a.MZ, o);
+      const double G13 = FLAT ? 0.0 : ldg(m.G13, o), G23 = FLAT ? 0.0 : ldg(m.G23, o);
       sF[e][0][p] = Gs * mx * Qo;
       sF[e][1][p] = Gs * my * Qo;
-      sF[e][2][p] = Gs * (mz * RGsqrtV + ldg(m.G13, o) * mx + ldg(m.G23, o) * my) * Qo;
+      sF[e][2][p] = Gs * (mz * RGsqrtV + G13 * mx + G23 * my) * Qo;
     }
     __syncthreads();
     if (!valid) return;
@@ -1291,7 +1299,9 @@ __global__ void __launch_bounds__(EPB * N1 * N1 * N1) trcadv_kernel(const DevMes
     const double lift = L[0] * dfv[i + N1 * k] + L[1] * dfv[Nfp + j + N1 * k] + L[2] * dfv[2 * Nfp + i + N1 * k] +
                         L[3] * dfv[3 * Nfp + j + N1 * k] + L[4] * dfv[4 * Nfp + i + N1 * j] +
                         L[5] * dfv[5 * Nfp + i + N1 * j];
-    const double X = ldg(m.E11, o) * dx + ldg(m.E22, o) * dy + ldg(m.E33, o) * dz + lift;
+    const double E11 = FLAT ? m.fE[0] : ldg(m.E11, o), E22 = FLAT ? m.fE[1] : ldg(m.E22, o);
+    const double E33 = FLAT ? m.fE[2] : ldg(m.E33, o);
+    const double X = E11 * dx + E22 * dy + E33 * dz + lift;
     stg(a.dt, o, -X / Gs + ldg(a.tp, o));
   }
 }
@@ -1682,12 +1692,15 @@ int launch_trcadv(fedg_ctx* ctx, const TrcLaunch& t) {
     constexpr int N1 = decltype(n1c)::value;
     constexpr int EPB = epb_for<N1>();
     const int nblk = (m.Ne + EPB - 1) / EPB;
-    if (t.fct)
-      hipLaunchKernelGGL((trcadv_kernel<N1, EPB, TRC_FCT>), dim3(nblk), dim3(EPB * N1 * N1 * N1), 0, ctx->stream, m,
-                         a);
+    const dim3 grid(nblk), block(EPB * N1 * N1 * N1);
+    if (t.fct && m.flat)
+      hipLaunchKernelGGL((trcadv_kernel<N1, EPB, TRC_FCT, true>), grid, block, 0, ctx->stream, m, a);
+    else if (t.fct)
+      hipLaunchKernelGGL((trcadv_kernel<N1, EPB, TRC_FCT, false>), grid, block, 0, ctx->stream, m, a);
+    else if (m.flat)
+      hipLaunchKernelGGL((trcadv_kernel<N1, EPB, TRC_TEND, true>), grid, block, 0, ctx->stream, m, a);
     else
-      hipLaunchKernelGGL((trcadv_kernel<N1, EPB, TRC_TEND>), dim3(nblk), dim3(EPB * N1 * N1 * N1), 0, ctx->stream,
-                         m, a);
+      hipLaunchKernelGGL((trcadv_kernel<N1, EPB, TRC_TEND, false>), grid, block, 0, ctx->stream, m, a);
     FEDG_HIP_CHECK(hipGetLastError());
     return FEDG_OK;
   });

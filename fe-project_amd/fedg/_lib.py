@@ -79,6 +79,7 @@ def _load():
         "fedg_ctx_create": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(vp)]),
         "fedg_ctx_destroy": (C.c_int, [vp]),
         "fedg_ctx_sizes": (C.c_int, [vp, C.POINTER(BoxSizes)]),
+        "fedg_ctx_is_flat": (C.c_int, [vp]),
         "fedg_drhot2pres": (C.c_int, [vp] + [vp] * 8),
         "fedg_halo_exchange": (C.c_int, [vp, C.c_int, C.POINTER(vp)]),
         "fedg_apply_bc_prgvars": (C.c_int, [vp] + [vp] * 7),
@@ -155,7 +156,7 @@ LIB = _load()
 EXPORTED = (
     "fedg_last_error fedg_version fedg_dev_alloc fedg_dev_free fedg_memcpy_h2d fedg_memcpy_d2h "
     "fedg_memset_dev fedg_device_sync fedg_device_name fedg_stage_probe fedg_refelem_tables fedg_box_get_sizes "
-    "fedg_box_build fedg_ctx_create fedg_ctx_destroy fedg_ctx_sizes fedg_drhot2pres fedg_halo_exchange "
+    "fedg_box_build fedg_ctx_create fedg_ctx_destroy fedg_ctx_sizes fedg_ctx_is_flat fedg_drhot2pres fedg_halo_exchange "
     "fedg_apply_bc_prgvars fedg_heve_cal_tend fedg_heve_cal_tend_cco fedg_heve_add_bnd_contrib fedg_trcadvect_cal_tend fedg_trcadvect_calc_fct_coef fedg_trcadvect_tmar fedg_trcadvect_save_massflux fedg_hevi_cal_tend fedg_hevi_cal_vi fedg_hevi_cal_vi_newton "
     "fedg_modalfilter_matrix fedg_ctx_setup_modalfilter fedg_modalfilter_apply fedg_heve_set_modalfilter "
     "fedg_hevi_set_modalfilter "

@@ -153,6 +153,11 @@ class DeviceMesh:
         v = np.asfortranarray(MF_v, dtype=np.float64).ravel(order="F")
         check(LIB.fedg_ctx_setup_modalfilter(self.handle, _dp(h), _dp(v)), "fedg_ctx_setup_modalfilter")
 
+    @property
+    def flat(self) -> bool:
+        """Whether the context recognised the flat-terrain uniform box (constant-metric kernels)."""
+        return bool(LIB.fedg_ctx_is_flat(self.handle))
+
     def close(self):
         if self.handle:
             LIB.fedg_ctx_destroy(self.handle)

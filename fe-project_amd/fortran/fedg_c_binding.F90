@@ -27,6 +27,9 @@ module fedg_c_binding
   public :: fedg_ctx_create, fedg_ctx_destroy, fedg_last_error
   public :: fedg_heve_cal_tend, fedg_hevi_cal_tend, fedg_hevi_cal_vi
   public :: fedg_drhot2pres, fedg_calc_phyd_hgrad
+  public :: fedg_heve_cal_tend_cco, fedg_heve_add_bnd_contrib
+  public :: fedg_trcadvect_cal_tend, fedg_trcadvect_calc_fct_coef, fedg_trcadvect_tmar
+  public :: fedg_trcadvect_save_massflux
 
   interface
     integer(c_int) function fedg_ctx_create(desc, ctx) bind(C, name="fedg_ctx_create")
@@ -100,5 +103,76 @@ module fedg_c_binding
       real(c_double) :: DPhydDx(*), DPhydDy(*)
       real(c_double), intent(in) :: PRES_hyd(*), PRES_hyd_ref(*)
     end function fedg_calc_phyd_hgrad
+
+    !> atm_dyn_dgm_nonhydro3d_rhot_heve_cal_tend_cco (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve.F90:493-678)
+    integer(c_int) function fedg_heve_cal_tend_cco(ctx, DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt, &
+        DDENS_, MOMX_, MOMY_, MOMZ_, DRHOT_, DPRES_, DENS_hyd, PRES_hyd, PRES_hyd_ref, THERM_hyd, &
+        CORIOLIS, Rtot, CVtot, CPtot, DPhydDx, DPhydDy) bind(C, name="fedg_heve_cal_tend_cco")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: DENS_dt(*), MOMX_dt(*), MOMY_dt(*), MOMZ_dt(*), RHOT_dt(*)
+      real(c_double), intent(in) :: DDENS_(*), MOMX_(*), MOMY_(*), MOMZ_(*), DRHOT_(*), DPRES_(*)
+      real(c_double), intent(in) :: DENS_hyd(*), PRES_hyd(*), PRES_hyd_ref(*), THERM_hyd(*), CORIOLIS(*)
+      real(c_double), intent(in) :: Rtot(*), CVtot(*), CPtot(*), DPhydDx(*), DPhydDy(*)
+    end function fedg_heve_cal_tend_cco
+
+    !> atm_dyn_dgm_nonhydro3d_rhot_heve_add_bnd_contrib_generalvc
+    !! (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_heve_numflux.F90:242-496); Rtot, CVtot, CPtot
+    !! are all given or all c_null_ptr (dry-air constants)
+    integer(c_int) function fedg_heve_add_bnd_contrib(ctx, DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt, &
+        DDENS_, MOMX_, MOMY_, MOMZ_, DRHOT_, DPRES, DENS_hyd, PRES_hyd, THERM_hyd, Rtot, CVtot, CPtot) &
+        bind(C, name="fedg_heve_add_bnd_contrib")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: DENS_dt(*), MOMX_dt(*), MOMY_dt(*), MOMZ_dt(*), RHOT_dt(*), DPRES(*)
+      real(c_double), intent(in) :: DDENS_(*), MOMX_(*), MOMY_(*), MOMZ_(*), DRHOT_(*)
+      real(c_double), intent(in) :: DENS_hyd(*), PRES_hyd(*), THERM_hyd(*)
+      type(c_ptr), value :: Rtot, CVtot, CPtot
+    end function fedg_heve_add_bnd_contrib
+
+    !> atm_dyn_dgm_trcadvect3d_heve_cal_tend (FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:149-231);
+    !! FaceIntMat is built by the context, J is lcmesh%J
+    integer(c_int) function fedg_trcadvect_cal_tend(ctx, QTRC_dt, QTRC_, MOMX_, MOMY_, MOMZ_, &
+        alphDENS_M, alphDENS_P, fct_coef, RHOQ_tp, J) bind(C, name="fedg_trcadvect_cal_tend")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: QTRC_dt(*)
+      real(c_double), intent(in) :: QTRC_(*), MOMX_(*), MOMY_(*), MOMZ_(*), alphDENS_M(*), alphDENS_P(*)
+      real(c_double), intent(in) :: fct_coef(*), RHOQ_tp(*), J(*)
+    end function fedg_trcadvect_cal_tend
+
+    !> atm_dyn_dgm_trcadvect3d_heve_calc_fct_coef (FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:232-305)
+    integer(c_int) function fedg_trcadvect_calc_fct_coef(ctx, fct_coef, QTRC_, MOMX_, MOMY_, MOMZ_, RHOQ_tp_, &
+        alphDENS_M, alphDENS_P, DENS_hyd, DDENS_, DDENS0_, rk_c_ssm1, dt, J, disable_limiter) &
+        bind(C, name="fedg_trcadvect_calc_fct_coef")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: fct_coef(*)
+      real(c_double), intent(in) :: QTRC_(*), MOMX_(*), MOMY_(*), MOMZ_(*), RHOQ_tp_(*)
+      real(c_double), intent(in) :: alphDENS_M(*), alphDENS_P(*), DENS_hyd(*), DDENS_(*), DDENS0_(*), J(*)
+      real(c_double), value :: rk_c_ssm1, dt
+      integer(c_int), value :: disable_limiter
+    end function fedg_trcadvect_calc_fct_coef
+
+    !> atm_dyn_dgm_trcadvect3d_TMAR (FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:308-337)
+    integer(c_int) function fedg_trcadvect_tmar(ctx, QTRC_, DENS_hyd, DDENS_, J) bind(C, name="fedg_trcadvect_tmar")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: QTRC_(*)
+      real(c_double), intent(in) :: DENS_hyd(*), DDENS_(*), J(*)
+    end function fedg_trcadvect_tmar
+
+    !> atm_dyn_dgm_trcadvect3d_save_massflux (FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:340-404)
+    integer(c_int) function fedg_trcadvect_save_massflux(ctx, MFLX_x_tavg, MFLX_y_tavg, MFLX_z_tavg, &
+        alph_dens_M, alph_dens_P, DDENS, MOMX, MOMY, MOMZ, DPRES, DENS_hyd, PRES_hyd, Rtot, CVtot, CPtot, &
+        rkstage, tavg_weight_h, tavg_weight_v, is_hevi) bind(C, name="fedg_trcadvect_save_massflux")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: MFLX_x_tavg(*), MFLX_y_tavg(*), MFLX_z_tavg(*), alph_dens_M(*), alph_dens_P(*)
+      real(c_double), intent(in) :: DDENS(*), MOMX(*), MOMY(*), MOMZ(*), DPRES(*), DENS_hyd(*), PRES_hyd(*)
+      real(c_double), intent(in) :: Rtot(*), CVtot(*), CPtot(*)
+      integer(c_int), value :: rkstage, is_hevi
+      real(c_double), value :: tavg_weight_h, tavg_weight_v
+    end function fedg_trcadvect_save_massflux
   end interface
 end module fedg_c_binding

@@ -178,6 +178,12 @@ int fedg_modalfilter_apply(fedg_ctx *ctx, double *DDENS, double *MOMX, double *M
                            int do_weight_Gsqrt);
 /* sizes: Np, NfpTot, Ne, NeA, nB ... of the context */
 int fedg_ctx_sizes(const fedg_ctx *ctx, fedg_box_sizes *sizes);
+/* 1 when the context recognised the flat-terrain uniform box (unit Gsqrt and
+ * GsqrtH, zero GI3, axis-aligned unit normals, Escale and Fscale constant per
+ * direction and face, the box's own face map); its kernels then take those
+ * values as constants.  No counterpart in the reference (a query of this
+ * library's specialisation). */
+int fedg_ctx_is_flat(const fedg_ctx *ctx);
 
 /* ------------------------------------------------------------------ */
 /* Drop-in entry points.  Device pointers; fields are (Np,NeA) unless noted.

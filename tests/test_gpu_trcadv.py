@@ -39,16 +39,26 @@ def random_metrics(mesh, rng):
     mesh.GsqrtH[:] = 1.0 + 0.05 * rng.uniform(-1.0, 1.0, mesh.GsqrtH.size)
 
 
-@pytest.mark.parametrize("case", CASES, ids=[f"p{c[0]}" for c in CASES])
-def test_trcadvect_cal_tend_bitwise(case):
+def _mesh_and_ctx(case, metrics, rng):
     porder, NeX, NeY, NeZ, periodic = case
-    print(f"trcadvect p={porder} {NeX}x{NeY}x{NeZ}", flush=True)
     mesh = BoxMesh(porder, NeX, NeY, NeZ, (0.0, 2.0e3 * NeX), (0.0, 1.5e3 * NeY), (0.0, 1.0e3 * NeZ),
                    periodic=periodic)
-    rng = np.random.default_rng(100 + porder)
-    random_metrics(mesh, rng)
-    om = O.OracleMesh(mesh)
+    if metrics == "random":
+        random_metrics(mesh, rng)
     dm = fedg.DeviceMesh(mesh)
+    assert dm.flat == (metrics == "flat")
+    return mesh, O.OracleMesh(mesh), dm
+
+
+# "flat": the box's own metrics, which the context recognises and the
+# kernels then take as constants; "random": every metric array read
+@pytest.mark.parametrize("metrics", ["random", "flat"])
+@pytest.mark.parametrize("case", CASES, ids=[f"p{c[0]}" for c in CASES])
+def test_trcadvect_cal_tend_bitwise(case, metrics):
+    porder, NeX, NeY, NeZ, periodic = case
+    print(f"trcadvect p={porder} {NeX}x{NeY}x{NeZ} {metrics}", flush=True)
+    rng = np.random.default_rng(100 + porder)
+    mesh, om, dm = _mesh_and_ctx(case, metrics, rng)
     n = mesh.Np * mesh.Ne
     Q, MX, MY, MZ, R, tp = (mesh.field() for _ in range(6))
     Q[:n] = 1.0e-3 * (1.0 + 0.3 * rng.standard_normal(n))
@@ -86,16 +96,13 @@ def _tracer_state(mesh, rng):
     return f
 
 
+@pytest.mark.parametrize("metrics", ["random", "flat"])
 @pytest.mark.parametrize("case", CASES, ids=[f"p{c[0]}" for c in CASES])
-def test_trcadvect_fct_tmar_massflux_bitwise(case):
+def test_trcadvect_fct_tmar_massflux_bitwise(case, metrics):
     porder, NeX, NeY, NeZ, periodic = case
-    print(f"trcadvect fct/tmar/massflux p={porder} {NeX}x{NeY}x{NeZ}", flush=True)
-    mesh = BoxMesh(porder, NeX, NeY, NeZ, (0.0, 2.0e3 * NeX), (0.0, 1.5e3 * NeY), (0.0, 1.0e3 * NeZ),
-                   periodic=periodic)
+    print(f"trcadvect fct/tmar/massflux p={porder} {NeX}x{NeY}x{NeZ} {metrics}", flush=True)
     rng = np.random.default_rng(200 + porder)
-    random_metrics(mesh, rng)
-    om = O.OracleMesh(mesh)
-    dm = fedg.DeviceMesh(mesh)
+    mesh, om, dm = _mesh_and_ctx(case, metrics, rng)
     n, nA, nf = mesh.Np * mesh.Ne, mesh.Np * mesh.NeA, mesh.NfpTot * mesh.Ne
     f = _tracer_state(mesh, rng)
     d = lambda a: fedg.DevField(a.size, a)
