@@ -860,6 +860,62 @@ int fedg_trcadvect_save_massflux(fedg_ctx* ctx, double* MFLX_x_tavg, double* MFL
   return FEDG_OK;
 }
 
+int fedg_trcadvect_cal_alphdens_advtest(fedg_ctx* ctx, double* alph_dens_M, double* alph_dens_P,
+                                        const double* DDENS_, const double* MOMX_, const double* MOMY_,
+                                        const double* MOMZ_, const double* DENS_hyd) {
+  if (!ctx || !alph_dens_M || !alph_dens_P || !DDENS_ || !MOMX_ || !MOMY_ || !MOMZ_ || !DENS_hyd) return FEDG_ERR_ARG;
+  MflxArgs a{};
+  a.alM = alph_dens_M;
+  a.alP = alph_dens_P;
+  a.DD = DDENS_;
+  a.MX = MOMX_;
+  a.MY = MOMY_;
+  a.MZ = MOMZ_;
+  a.Dh = DENS_hyd;
+  TRY(launch_alphdens_advtest(ctx, a));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return FEDG_OK;
+}
+
+int fedg_trcvar_rk_advance(fedg_ctx* ctx, int nowstage, int nstage, const double* coef_sig_ex,
+                           const double* coef_gam_ex, const double* coef_c_ex, double dt, double* q, double* var0,
+                           double* varTmp, const double* tend, const double* DDENS, const double* DDENS0,
+                           const double* DENS_hyd) {
+  if (!ctx || !coef_sig_ex || !coef_gam_ex || !coef_c_ex || !q || !var0 || !varTmp || !tend || !DDENS || !DDENS0 ||
+      !DENS_hyd)
+    return FEDG_ERR_ARG;
+  if (nstage < 1 || nowstage < 1 || nowstage > nstage) {
+    set_error("fedg_trcvar_rk_advance: need 1 <= nowstage <= nstage");
+    return FEDG_ERR_ARG;
+  }
+  // coef_sig_ex, coef_gam_ex: (nstage+1, nstage) column-major, as timeint_rk holds them
+  const int ld = nstage + 1;
+  auto SIG = [&](int i, int j) { return coef_sig_ex[(i - 1) + ld * (j - 1)]; };
+  auto GAM = [&](int i, int j) { return coef_gam_ex[(i - 1) + ld * (j - 1)]; };
+  const double EPS = 1.0e-16;  // CONST_EPS of SCALE's scale_const
+  TrcRk r{};
+  r.q = q;
+  r.var0 = var0;
+  r.varTmp = varTmp;
+  r.tend = tend;
+  r.DD = DDENS;
+  r.DD0 = DDENS0;
+  r.Dh = DENS_hyd;
+  r.sig_ss = SIG(nowstage + 1, nowstage);
+  r.sig_Ns = SIG(nstage + 1, nowstage);
+  r.one_minus_sig_ss = 1.0 - r.sig_ss;
+  r.gam_ss = dt * GAM(nowstage + 1, nowstage);
+  r.gam_Ns = dt * GAM(nstage + 1, nowstage);
+  r.c_ssm1 = coef_c_ex[nowstage - 1];
+  r.last = nowstage == nstage;
+  r.c_ss = r.last ? 0.0 : coef_c_ex[nowstage];
+  r.first = nowstage == 1;
+  r.acc = !r.last && (fabs(r.sig_Ns) > EPS || fabs(r.gam_Ns) > EPS);
+  TRY(launch_trc_rk(ctx, r));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return FEDG_OK;
+}
+
 int fedg_hevi_cal_tend(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, double* MOMY_dt, double* MOMZ_dt,
                        double* RHOT_dt, const double* DDENS_, const double* MOMX_, const double* MOMY_,
                        const double* MOMZ_, const double* DRHOT_, const double* DPRES_, const double* DENS_hyd,

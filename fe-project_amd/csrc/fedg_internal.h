@@ -139,6 +139,14 @@ struct MflxArgs {
 int launch_trcadv(fedg_ctx* ctx, const TrcLaunch& t);
 int launch_tmar(fedg_ctx* ctx, double* Q, const double* Dh, const double* DD, const double* J, const TrcW& w);
 int launch_save_massflux(fedg_ctx* ctx, const MflxArgs& a);
+int launch_alphdens_advtest(fedg_ctx* ctx, const MflxArgs& a);
+struct TrcRk {
+  double *q, *var0, *varTmp;
+  const double *tend, *DD, *DD0, *Dh;
+  double sig_ss, one_minus_sig_ss, gam_ss, sig_Ns, gam_Ns, c_ssm1, c_ss;
+  int first, last, acc;
+};
+int launch_trc_rk(fedg_ctx* ctx, const TrcRk& r);
 int launch_heve_fused(fedg_ctx* ctx, const TendIn& in, const RkOut& rk, int use_flat);
 int run_stage_probe(int mode, int NeX, int NeY, int NeZ, int reps, double* ms);
 // FP32 perturbation form of the fused stage (flat box, dry air; at p = 7 the

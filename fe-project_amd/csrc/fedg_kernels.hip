@@ -1363,6 +1363,56 @@ __global__ void __launch_bounds__(256) massflux_kernel(const MflxArgs a, int n) 
     stg(a.FZ, o, ldg(a.FZ, o) + a.twv * ldg(a.MZ, o));
   }
 }
+// Low-storage RK update of a tracer weighted by the density of each stage
+// (CMN/scale_timeint_rk.F90:1390-1498, rk_advance_trcvar_low_storage):
+// stage 1 stores var0 = q (DENS_hyd + DDENS0) and clears varTmp; varTmp
+// collects sig_Ns q dens_ssm1 + gam_Ns tend when either is nonzero; q becomes
+// ((1 - sig_ss) var0 + sig_ss q dens_ssm1 + gam_ss tend) / dens_ss, at the
+// last stage (varTmp + sig_ss q dens_ssm1 + gam_ss tend) / (DENS_hyd + DDENS).
+__global__ void __launch_bounds__(256) trc_rk_kernel(const TrcRk r, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint32_t o = (uint32_t)t;
+  const double Dh = ldg(r.Dh, o), DD = ldg(r.DD, o), DD0 = ldg(r.DD0, o);
+  const double q = ldg(r.q, o), tend = ldg(r.tend, o);
+  const double dens_ssm1 = Dh + DD0 + r.c_ssm1 * (DD - DD0);
+  if (r.last) {
+    stg(r.q, o, (ldg(r.varTmp, o) + r.sig_ss * q * dens_ssm1 + r.gam_ss * tend) / (Dh + DD));
+    return;
+  }
+  double v0;
+  double vt = 0.0;
+  if (r.first) {
+    v0 = q * (Dh + DD0);
+    stg(r.var0, o, v0);
+  } else {
+    v0 = ldg(r.var0, o);
+    vt = ldg(r.varTmp, o);
+  }
+  if (r.acc) vt = vt + r.sig_Ns * q * dens_ssm1 + r.gam_Ns * tend;
+  if (r.first || r.acc) stg(r.varTmp, o, vt);
+  const double dens_ss = Dh + DD0 + r.c_ss * (DD - DD0);
+  stg(r.q, o, (r.one_minus_sig_ss * v0 + r.sig_ss * q * dens_ssm1 + r.gam_ss * tend) / dens_ss);
+}
+
+// cal_alphdens_advtest (FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:407-445):
+// alpha = max(|u_n M|, |u_n P|), alph_dens = alpha dens Gsqrt on each side.
+__global__ void __launch_bounds__(256) alphdens_advtest_kernel(const DevMesh m, const MflxArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m.NfpTot * m.Ne) return;
+  const uint32_t gf = (uint32_t)t;
+  const uint32_t iM = (uint32_t)m.vmapM[gf], iP = (uint32_t)m.vmapP[gf];
+  const double densM = ldg(a.DD, iM) + ldg(a.Dh, iM);
+  const double densP = ldg(a.DD, iP) + ldg(a.Dh, iP);
+  const double nx = ldg(m.nx, gf), ny = ldg(m.ny, gf), nz = ldg(m.nz, gf);
+  const double VelM = (ldg(a.MX, iM) * nx + ldg(a.MY, iM) * ny + ldg(a.MZ, iM) * nz) / densM;
+  const double VelP = (ldg(a.MX, iP) * nx + ldg(a.MY, iP) * ny + ldg(a.MZ, iP) * nz) / densP;
+  const double aM = fabs(VelM), aP = fabs(VelP);
+  const double alpha = aM > aP ? aM : aP;
+  stg(a.alM, gf, alpha * densM * ldg(m.Gsqrt, iM));
+  stg(a.alP, gf, alpha * densP * ldg(m.Gsqrt, iP));
+}
+
 template <int N1>
 __global__ void __launch_bounds__(256) alphdens_kernel(const DevMesh m, const MflxArgs a, double gamm) {
   constexpr int Nfp = N1 * N1, NfpTot = 6 * Nfp;
@@ -1717,6 +1767,20 @@ int launch_tmar(fedg_ctx* ctx, double* Q, const double* Dh, const double* DD, co
     FEDG_HIP_CHECK(hipGetLastError());
     return FEDG_OK;
   });
+}
+
+int launch_trc_rk(fedg_ctx* ctx, const TrcRk& r) {
+  const int n = ctx->m.Np * ctx->m.Ne;
+  hipLaunchKernelGGL(trc_rk_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, r, n);
+  FEDG_HIP_CHECK(hipGetLastError());
+  return FEDG_OK;
+}
+
+int launch_alphdens_advtest(fedg_ctx* ctx, const MflxArgs& a) {
+  const int nf = ctx->m.NfpTot * ctx->m.Ne;
+  hipLaunchKernelGGL(alphdens_advtest_kernel, dim3((nf + 255) / 256), dim3(256), 0, ctx->stream, ctx->m, a);
+  FEDG_HIP_CHECK(hipGetLastError());
+  return FEDG_OK;
 }
 
 int launch_save_massflux(fedg_ctx* ctx, const MflxArgs& a) {

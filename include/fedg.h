@@ -305,6 +305,26 @@ int fedg_trcadvect_save_massflux(fedg_ctx *ctx, double *MFLX_x_tavg, double *MFL
                                  const double *CPtot, int rkstage, double tavg_weight_h,
                                  double tavg_weight_v, int is_hevi);
 
+/* atm_dyn_dgm_trcadvect3d_heve_cal_alphdens_advtest
+ * (FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:407-445): the damping
+ * coefficients of the tracer-only advection test, alpha = max(|u.n| of both
+ * sides), alph_dens_M/_P (NfpTot,Ne) = alpha dens Gsqrt of each side. */
+int fedg_trcadvect_cal_alphdens_advtest(fedg_ctx *ctx, double *alph_dens_M, double *alph_dens_P,
+                                        const double *DDENS_, const double *MOMX_, const double *MOMY_,
+                                        const double *MOMZ_, const double *DENS_hyd);
+
+/* timeint_rk%Advance_trcvar for the low-storage schemes
+ * (CMN/scale_timeint_rk.F90:1390-1498, rk_advance_trcvar_low_storage): the
+ * density-weighted RK update of one tracer at stage nowstage (1-based) of
+ * nstage.  coef_sig_ex, coef_gam_ex: (nstage+1, nstage) column-major host
+ * arrays and coef_c_ex (nstage), the timeint_rk object's tables; dt its step.
+ * q, var0, varTmp, tend (the stage's tendency), DDENS, DDENS0, DENS_hyd:
+ * device (Np,NeA), interior elements updated. */
+int fedg_trcvar_rk_advance(fedg_ctx *ctx, int nowstage, int nstage, const double *coef_sig_ex,
+                           const double *coef_gam_ex, const double *coef_c_ex, double dt, double *q,
+                           double *var0, double *varTmp, const double *tend, const double *DDENS,
+                           const double *DDENS0, const double *DENS_hyd);
+
 /* atm_dyn_dgm_nonhydro3d_rhot_hevi_cal_tend
  * (FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi.F90:289-482 with its numerical
  * flux FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi_numflux.F90:232-416): the

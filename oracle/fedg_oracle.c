@@ -634,6 +634,63 @@ void oracle_trcadvect_tmar(const oracle_mesh *m, const double *w1d, double *QTRC
   }
 }
 
+/* FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:407-445 (cal_alphdens_advtest) */
+void oracle_trcadvect_cal_alphdens_advtest(const oracle_mesh *m, double *alph_M, double *alph_P, const double *DDENS_,
+                                           const double *MOMX_, const double *MOMY_, const double *MOMZ_,
+                                           const double *DENS_hyd) {
+  const long nf = (long)m->NfpTot * m->Ne;
+  const double *nx = m->normal_fn, *ny = m->normal_fn + nf, *nz = m->normal_fn + 2 * nf;
+  for (long i = 0; i < nf; ++i) {
+    const long iM = m->vmapM[i] - 1, iP = m->vmapP[i] - 1;
+    const double densM = DDENS_[iM] + DENS_hyd[iM], densP = DDENS_[iP] + DENS_hyd[iP];
+    const double VelM = (MOMX_[iM] * nx[i] + MOMY_[iM] * ny[i] + MOMZ_[iM] * nz[i]) / densM;
+    const double VelP = (MOMX_[iP] * nx[i] + MOMY_[iP] * ny[i] + MOMZ_[iP] * nz[i]) / densP;
+    const double alpha = fabs(VelM) > fabs(VelP) ? fabs(VelM) : fabs(VelP);
+    alph_M[i] = alpha * densM * m->Gsqrt[iM];
+    alph_P[i] = alpha * densP * m->Gsqrt[iP];
+  }
+}
+
+/* CMN/scale_timeint_rk.F90:1390-1498 (rk_advance_trcvar_low_storage2D) over
+ * nodes [0, n); sig, gam: (nstage+1, nstage) column-major; c: (nstage) */
+void oracle_trcvar_rk_advance(long n, int nowstage, int nstage, const double *sig, const double *gam,
+                              const double *c, double dt, double *q, double *var0, double *varTmp,
+                              const double *tend, const double *DDENS, const double *DDENS0, const double *DENS_hyd) {
+  const int ld = nstage + 1;
+#define SIG(i, j) sig[((i)-1) + ld * ((j)-1)]
+#define GAM(i, j) gam[((i)-1) + ld * ((j)-1)]
+  const double EPS = 1.0e-16; /* CONST_EPS */
+  const double sig_ss = SIG(nowstage + 1, nowstage), sig_Ns = SIG(nstage + 1, nowstage);
+  const double one_Minus_sig_ss = 1.0 - sig_ss;
+  const double gam_ss = dt * GAM(nowstage + 1, nowstage), gam_Ns = dt * GAM(nstage + 1, nowstage);
+  const double c_ssm1 = c[nowstage - 1];
+#undef SIG
+#undef GAM
+  if (nowstage == nstage) {
+    for (long i = 0; i < n; ++i) {
+      const double dens_ssm1 = DENS_hyd[i] + DDENS0[i] + c_ssm1 * (DDENS[i] - DDENS0[i]);
+      q[i] = (varTmp[i] + sig_ss * q[i] * dens_ssm1 + gam_ss * tend[i]) / (DENS_hyd[i] + DDENS[i]);
+    }
+    return;
+  }
+  const double c_ss = c[nowstage];
+  if (nowstage == 1)
+    for (long i = 0; i < n; ++i) {
+      var0[i] = q[i] * (DENS_hyd[i] + DDENS0[i]);
+      varTmp[i] = 0.0;
+    }
+  if (fabs(sig_Ns) > EPS || fabs(gam_Ns) > EPS)
+    for (long i = 0; i < n; ++i) {
+      const double dens_ssm1 = DENS_hyd[i] + DDENS0[i] + c_ssm1 * (DDENS[i] - DDENS0[i]);
+      varTmp[i] = varTmp[i] + sig_Ns * q[i] * dens_ssm1 + gam_Ns * tend[i];
+    }
+  for (long i = 0; i < n; ++i) {
+    const double dens_ssm1 = DENS_hyd[i] + DDENS0[i] + c_ssm1 * (DDENS[i] - DDENS0[i]);
+    const double dens_ss = DENS_hyd[i] + DDENS0[i] + c_ss * (DDENS[i] - DDENS0[i]);
+    q[i] = (one_Minus_sig_ss * var0[i] + sig_ss * q[i] * dens_ssm1 + gam_ss * tend[i]) / dens_ss;
+  }
+}
+
 /* FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:340-404 (save_massflux) with
  * cal_alphdens_dyn (:447-536); gamma = CPdry / CVdry as there */
 void oracle_trcadvect_save_massflux(const oracle_mesh *m, double *MFLX_x, double *MFLX_y, double *MFLX_z,

@@ -130,6 +130,14 @@ void oracle_trcadvect_calc_fct_coef(const oracle_mesh *m, const double *w1d, dou
 /* FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:308-337 (TMAR), in place */
 void oracle_trcadvect_tmar(const oracle_mesh *m, const double *w1d, double *QTRC_, const double *DENS_hyd,
                            const double *DDENS_, const double *J);
+/* FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:407-445 (cal_alphdens_advtest) */
+void oracle_trcadvect_cal_alphdens_advtest(const oracle_mesh *m, double *alph_M, double *alph_P, const double *DDENS_,
+                                           const double *MOMX_, const double *MOMY_, const double *MOMZ_,
+                                           const double *DENS_hyd);
+/* CMN/scale_timeint_rk.F90:1390-1498 (low-storage tracer RK update) over nodes [0, n) */
+void oracle_trcvar_rk_advance(long n, int nowstage, int nstage, const double *sig, const double *gam,
+                              const double *c, double dt, double *q, double *var0, double *varTmp,
+                              const double *tend, const double *DDENS, const double *DDENS0, const double *DENS_hyd);
 /* FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:340-404 (save_massflux with cal_alphdens_dyn) */
 void oracle_trcadvect_save_massflux(const oracle_mesh *m, double *MFLX_x, double *MFLX_y, double *MFLX_z,
                                     double *alph_M, double *alph_P, const double *DDENS, const double *MOMX,

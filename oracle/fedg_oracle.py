@@ -56,6 +56,8 @@ def _load():
     lib.oracle_trcadvect_cal_tend.argtypes = [M] + [dp] * 11
     lib.oracle_trcadvect_calc_fct_coef.argtypes = [M] + [dp] * 12 + [C.c_double, C.c_double, dp, C.c_int]
     lib.oracle_trcadvect_tmar.argtypes = [M] + [dp] * 5
+    lib.oracle_trcadvect_cal_alphdens_advtest.argtypes = [M] + [dp] * 7
+    lib.oracle_trcvar_rk_advance.argtypes = [C.c_long, C.c_int, C.c_int, dp, dp, dp, C.c_double] + [dp] * 7
     lib.oracle_trcadvect_save_massflux.argtypes = [M] + [dp] * 12 + [C.c_int, C.c_double, C.c_double, C.c_int]
     lib.oracle_rk_tab_get.argtypes = [C.c_char_p, C.c_double, C.POINTER(RkTabC)]
     lib.oracle_rk_advance.argtypes = [C.POINTER(RkTabC), C.c_int, C.c_long] + [dp] * 5 + [C.c_long]
@@ -227,6 +229,48 @@ class OracleMesh:
         LIB.oracle_trcadvect_save_massflux(self.ref, *[_d(a) for a in MFLX], *[_d(a) for a in alph], _d(DDENS),
                                            _d(MOMX), _d(MOMY), _d(MOMZ), _d(DPRES), _d(DENS_hyd), _d(PRES_hyd),
                                            int(rkstage), float(twh), float(twv), int(bool(is_hevi)))
+
+    def trcadvect_cal_alphdens_advtest(self, DDENS, MOMX, MOMY, MOMZ, DENS_hyd):
+        m = self.mesh
+        aM, aP = np.zeros(m.NfpTot * m.Ne), np.zeros(m.NfpTot * m.Ne)
+        LIB.oracle_trcadvect_cal_alphdens_advtest(self.ref, _d(aM), _d(aP), _d(DDENS), _d(MOMX), _d(MOMY), _d(MOMZ),
+                                                  _d(DENS_hyd))
+        return aM, aP
+
+    def trcvar_rk_advance(self, nowstage, tab, dt, q, var0, varTmp, tend, DDENS, DDENS0, DENS_hyd):
+        """Low-storage tracer RK update over the interior nodes; tab: fedg.tracer.RkTables."""
+        n = self.mesh.Np * self.mesh.Ne
+        LIB.oracle_trcvar_rk_advance(n, int(nowstage), tab.nstage, _d(tab.sig_f), _d(tab.gam_f), _d(tab.c), float(dt),
+                                     _d(q), _d(var0), _d(varTmp), _d(tend), _d(DDENS), _d(DDENS0), _d(DENS_hyd))
+
+    def tracer_advection(self, Q, MOMX, MOMY, MOMZ, DDENS, DENS_hyd, dt, nsteps, scheme="ERK_SSP_3s3o",
+                         disable_limiter=False):
+        """AtmDynDGMDriver_trcadv3d_update with ONLY_TRACERADV_FLAG
+        (FDS/scale_atm_dyn_dgm_driver_trcadv3d.F90:312-560), nsteps steps of
+        the tracer Q (in place) by fixed mass fluxes; the flow fields' halos
+        are exchanged here (closed boundaries: the caller's BC)."""
+        from fedg.tracer import rk_tables
+        m = self.mesh
+        t = rk_tables(scheme)
+        MX, MY, MZ, DD = (a.copy() for a in (MOMX, MOMY, MOMZ, DDENS))
+        Dh = DENS_hyd.copy()
+        for a in (MX, MY, MZ, DD, Dh):
+            self.exchange(a)
+        aM, aP = self.trcadvect_cal_alphdens_advtest(DD, MX, MY, MZ, Dh)
+        tp = m.field()
+        var0, varTmp = m.field(), m.field()
+        for _ in range(nsteps):
+            for s in range(1, t.nstage + 1):
+                self.exchange(Q)
+                dttmp = dt * t.gam[s, s - 1] / t.sig[s, s - 1]
+                R = self.trcadvect_calc_fct_coef(Q, MX, MY, MZ, tp, aM, aP, Dh, DD, DD, t.c[s - 1], dttmp,
+                                                 disable_limiter=disable_limiter)
+                self.exchange(R)
+                tend = self.trcadvect_cal_tend(Q, MX, MY, MZ, aM, aP, R, tp)
+                self.trcvar_rk_advance(s, t, dt, Q, var0, varTmp, tend, DD, DD, Dh)
+                if s == t.nstage and not disable_limiter:
+                    self.trcadvect_tmar(Q, Dh, DD)
+        return Q
 
     # --- HEVI ---------------------------------------------------------------
     def hevi_numflux(self, q, DPRES, bs):
