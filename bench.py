@@ -124,7 +124,8 @@ TRC_BYTES_PER_NODE = {False: 14 * 8 + 0.75 * (6 * 8 + 2 * 4), True: 7 * 8 + 0.75
 
 def time_trcadv(fedg, mesh, bs, q, dm, reps):
     """One tracer's advection tendency and FCT coefficient on the stage mesh,
-    host-timed per call (each call ends with a stream synchronisation)."""
+    host-timed per call (each call ends with a stream synchronisation), and
+    whole steps of the tracer-only advection driver (SSP RK3)."""
     import numpy as np
     n, nA, nf = mesh.Np * mesh.Ne, mesh.Np * mesh.NeA, mesh.NfpTot * mesh.Ne
     dens = bs.DENS_hyd + q[0]
@@ -152,8 +153,25 @@ def time_trcadv(fedg, mesh, bs, q, dm, reps):
         for _ in range(reps):
             fn()
         res["ms_" + name] = (time.perf_counter() - t0) * 1e3 / reps
+    Qh = d["Q"].download()
     for v in d.values():
         v.free()
+    # whole tracer steps of the tracer-only advection driver (SSP RK3, limiter
+    # and TMAR on): per stage two halo exchanges, the FCT coefficient, the
+    # tendency and the density-weighted update
+    from fedg.tracer import TracerAdvection
+    dens = bs.DENS_hyd + q[0]
+    st = TracerAdvection(dm, dens * 10.0, dens * 5.0, dens * 0.5, q[0], bs.DENS_hyd, mesh.J, 0.04,
+                         scheme="ERK_SSP_3s3o")
+    del dens
+    gQ = fedg.DevField(nA, Qh)
+    st.step(gQ, 1)
+    nstep = max(1, reps // 3)
+    t0 = time.perf_counter()
+    st.step(gQ, nstep)
+    res["ms_tracer_step_ssp3"] = (time.perf_counter() - t0) * 1e3 / nstep
+    gQ.free()
+    st.close()
     bpn = TRC_BYTES_PER_NODE[dm.flat]
     achieved = bpn * n / (res["ms_cal_tend"] * 1e-3) / 1e9
     res.update({"nodes": n, "flat_metrics": dm.flat, "bytes_per_node": bpn, "cal_tend_GBs": achieved,
