@@ -916,6 +916,14 @@ int fedg_trcvar_rk_advance(fedg_ctx* ctx, int nowstage, int nstage, const double
   return FEDG_OK;
 }
 
+int fedg_tracer_modalfilter_apply(fedg_ctx* ctx, double* QTRC_, const double* DENS_hyd, const double* DDENS_,
+                                  const double* MFilter_tracer_h1D, const double* MFilter_tracer_v1D) {
+  if (!ctx || !QTRC_ || !DENS_hyd || !DDENS_ || !MFilter_tracer_h1D || !MFilter_tracer_v1D) return FEDG_ERR_ARG;
+  TRY(launch_tracer_modalfilter(ctx, QTRC_, DENS_hyd, DDENS_, MFilter_tracer_h1D, MFilter_tracer_v1D));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return FEDG_OK;
+}
+
 int fedg_hevi_cal_tend(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, double* MOMY_dt, double* MOMZ_dt,
                        double* RHOT_dt, const double* DDENS_, const double* MOMX_, const double* MOMY_,
                        const double* MOMZ_, const double* DRHOT_, const double* DPRES_, const double* DENS_hyd,

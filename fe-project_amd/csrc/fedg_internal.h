@@ -147,6 +147,8 @@ struct TrcRk {
   int first, last, acc;
 };
 int launch_trc_rk(fedg_ctx* ctx, const TrcRk& r);
+int launch_tracer_modalfilter(fedg_ctx* ctx, double* Q, const double* Dh, const double* DD, const double* MFh,
+                              const double* MFv);
 int launch_heve_fused(fedg_ctx* ctx, const TendIn& in, const RkOut& rk, int use_flat);
 int run_stage_probe(int mode, int NeX, int NeY, int NeZ, int reps, double* ms);
 // FP32 perturbation form of the fused stage (flat box, dry air; at p = 7 the

@@ -2022,6 +2022,70 @@ __global__ void __launch_bounds__(EPB* N1* N1* N1) modalfilter_kernel(Fields5 F,
   }
 }
 
+// Tracer modal filter (FDS/scale_atm_dyn_dgm_modalfilter.F90:232-270): the
+// same three passes with the tracer's matrices on weight QTRC, weight =
+// Gsqrt (DENS_hyd + DDENS), then QTRC = out / weight.
+struct TrcMF {
+  double h[64], v[64];  // MFilter_tracer_h1D, MFilter_tracer_v1D (n1,n1)
+};
+template <int N1, int EPB>
+__global__ void __launch_bounds__(EPB* N1* N1* N1)
+    tracer_modalfilter_kernel(double* __restrict__ Q, const double* __restrict__ Gsqrt, const double* __restrict__ Dh,
+                              const double* __restrict__ DD, const TrcMF mf, int Ne) {
+  constexpr int Np = N1 * N1 * N1;
+  __shared__ double a[EPB * Np], b[EPB * Np];
+  const int le = threadIdx.x / Np, t = threadIdx.x % Np;
+  const int ke = blockIdx.x * EPB + le;
+  const bool act = ke < Ne;
+  const int i = t % N1, j = (t / N1) % N1, k = t / (N1 * N1);
+  const size_t o = (size_t)(act ? ke : 0) * Np + t;
+  const double* Fh = mf.h;
+  const double* Fv = mf.v;
+  double* A = a + le * Np;
+  double* B = b + le * Np;
+  double w = 1.0;
+  if (act) {
+    w = Gsqrt[o] * (Dh[o] + DD[o]);
+    A[t] = w * Q[o];
+  } else {
+    A[t] = 0.0;
+  }
+  __syncthreads();
+  double s = Fh[i] * A[N1 * (j + N1 * k)];
+#pragma unroll
+  for (int l = 1; l < N1; ++l) s = s + Fh[i + N1 * l] * A[l + N1 * (j + N1 * k)];
+  B[t] = s;
+  __syncthreads();
+  s = B[i + N1 * N1 * k] * Fh[j];
+#pragma unroll
+  for (int l = 1; l < N1; ++l) s = s + B[i + N1 * (l + N1 * k)] * Fh[j + N1 * l];
+  __syncthreads();
+  A[t] = s;
+  __syncthreads();
+  s = A[i + N1 * j] * Fv[k];
+#pragma unroll
+  for (int l = 1; l < N1; ++l) s = s + A[i + N1 * (j + N1 * l)] * Fv[k + N1 * l];
+  if (act) Q[o] = s / w;
+}
+
+int launch_tracer_modalfilter(fedg_ctx* ctx, double* Q, const double* Dh, const double* DD, const double* MFh,
+                              const double* MFv) {
+  const DevMesh& m = ctx->m;
+  TrcMF mf{};
+  for (int u = 0; u < m.n1 * m.n1; ++u) {
+    mf.h[u] = MFh[u];
+    mf.v[u] = MFv[u];
+  }
+  return dispatch_n1(m.n1, [&](auto n1c) -> int {
+    constexpr int N1 = decltype(n1c)::value;
+    constexpr int EPB = epb_for<N1>();
+    hipLaunchKernelGGL((tracer_modalfilter_kernel<N1, EPB>), dim3((m.Ne + EPB - 1) / EPB), dim3(EPB * N1 * N1 * N1),
+                       0, ctx->stream, Q, m.Gsqrt, Dh, DD, mf, m.Ne);
+    FEDG_HIP_CHECK(hipGetLastError());
+    return FEDG_OK;
+  });
+}
+
 int launch_modalfilter(fedg_ctx* ctx, double* DDENS, double* MOMX, double* MOMY, double* MOMZ, double* DRHOT,
                        int do_weight_Gsqrt) {
   const DevMesh& m = ctx->m;

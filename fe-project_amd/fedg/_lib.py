@@ -90,6 +90,7 @@ def _load():
         "fedg_trcadvect_calc_fct_coef": (C.c_int, [vp] + [vp] * 11 + [C.c_double, C.c_double, vp, C.c_int]),
         "fedg_trcadvect_tmar": (C.c_int, [vp] * 5),
         "fedg_trcadvect_cal_alphdens_advtest": (C.c_int, [vp] * 8),
+        "fedg_tracer_modalfilter_apply": (C.c_int, [vp] * 6),
         "fedg_trcvar_rk_advance": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, C.c_double] + [vp] * 7),
         "fedg_trcadvect_save_massflux": (C.c_int, [vp] + [vp] * 15 + [C.c_int, C.c_double, C.c_double, C.c_int]),
         "fedg_calc_phyd_hgrad": (C.c_int, [vp] + [vp] * 4),
@@ -159,7 +160,7 @@ EXPORTED = (
     "fedg_last_error fedg_version fedg_dev_alloc fedg_dev_free fedg_memcpy_h2d fedg_memcpy_d2h "
     "fedg_memset_dev fedg_device_sync fedg_device_name fedg_stage_probe fedg_refelem_tables fedg_box_get_sizes "
     "fedg_box_build fedg_ctx_create fedg_ctx_destroy fedg_ctx_sizes fedg_ctx_is_flat fedg_drhot2pres fedg_halo_exchange "
-    "fedg_apply_bc_prgvars fedg_heve_cal_tend fedg_heve_cal_tend_cco fedg_heve_add_bnd_contrib fedg_trcadvect_cal_tend fedg_trcadvect_calc_fct_coef fedg_trcadvect_tmar fedg_trcadvect_save_massflux fedg_trcadvect_cal_alphdens_advtest fedg_trcvar_rk_advance fedg_hevi_cal_tend fedg_hevi_cal_vi fedg_hevi_cal_vi_newton "
+    "fedg_apply_bc_prgvars fedg_heve_cal_tend fedg_heve_cal_tend_cco fedg_heve_add_bnd_contrib fedg_trcadvect_cal_tend fedg_trcadvect_calc_fct_coef fedg_trcadvect_tmar fedg_trcadvect_save_massflux fedg_trcadvect_cal_alphdens_advtest fedg_trcvar_rk_advance fedg_tracer_modalfilter_apply fedg_hevi_cal_tend fedg_hevi_cal_vi fedg_hevi_cal_vi_newton "
     "fedg_modalfilter_matrix fedg_ctx_setup_modalfilter fedg_modalfilter_apply fedg_heve_set_modalfilter "
     "fedg_hevi_set_modalfilter "
     "fedg_hevi_cal_vi_guess fedg_mlp_create fedg_mlp_destroy fedg_mlp_param_count fedg_mlp_get_params fedg_mlp_set_params "

@@ -7,6 +7,8 @@ tracer update loop for the tracer-only advection test.
                            with ShuOsher2Butcher (:331-363)
 * ``trcvar_rk_advance``  - timeint_rk%Advance_trcvar, low-storage form
                            (CMN/scale_timeint_rk.F90:1390-1498)
+* ``tracer_modalfilter_apply`` - atm_dyn_dgm_tracer_modalfilter_apply
+                           (FDS/scale_atm_dyn_dgm_modalfilter.F90:232-270)
 * ``cal_alphdens_advtest`` - atm_dyn_dgm_trcadvect3d_heve_cal_alphdens_advtest
                            (FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:407-445)
 * ``TracerAdvection``    - AtmDynDGMDriver_trcadv3d_update with
@@ -111,6 +113,15 @@ def cal_alphdens_advtest(dm: DeviceMesh, alph_dens_M, alph_dens_P, DDENS, MOMX, 
           "fedg_trcadvect_cal_alphdens_advtest")
 
 
+def tracer_modalfilter_apply(dm: DeviceMesh, QTRC, DENS_hyd, DDENS, MF_h, MF_v):
+    """Density-weighted modal filter of a tracer in place; MF_h, MF_v: (n1,n1) host
+    matrices in the reference's column-major order (fedg.modalfilter_matrix)."""
+    h = np.asarray(MF_h, dtype=np.float64).ravel(order="F").copy()
+    v = np.asarray(MF_v, dtype=np.float64).ravel(order="F").copy()
+    check(LIB.fedg_tracer_modalfilter_apply(dm.handle, _p(QTRC), _p(DENS_hyd), _p(DDENS), _dp(h), _dp(v)),
+          "fedg_tracer_modalfilter_apply")
+
+
 class TracerAdvection:
     """Tracer-only advection by fixed mass fluxes (ONLY_TRACERADV_FLAG).
 
@@ -121,9 +132,10 @@ class TracerAdvection:
     device tracer field Q by one step of dt."""
 
     def __init__(self, dm: DeviceMesh, MOMX, MOMY, MOMZ, DDENS, DENS_hyd, J, dt, scheme="ERK_SSP_3s3o",
-                 disable_limiter=False):
+                 disable_limiter=False, modal_filter=None):
         mesh = dm.mesh
         self.dm, self.dt, self.disable = dm, float(dt), bool(disable_limiter)
+        self.mf = modal_filter  # (MF_h, MF_v) of the tracer filter (MODALFILTER_FLAG), or None
         self.tab = rk_tables(scheme)
         nA, nf = mesh.Np * mesh.NeA, mesh.NfpTot * mesh.Ne
         f = lambda a: DevField(a.size, a)
@@ -147,6 +159,8 @@ class TracerAdvection:
                 trcadvect_cal_tend(dm, self.tend, Q, self.MX, self.MY, self.MZ, self.aM, self.aP, self.R, self.tp,
                                    self.J)
                 trcvar_rk_advance(dm, s, t, dt, Q, self.var0, self.varTmp, self.tend, self.DD, self.DD, self.Dh)
+                if s == t.nstage and self.mf is not None:
+                    tracer_modalfilter_apply(dm, Q, self.Dh, self.DD, *self.mf)
                 if s == t.nstage and not self.disable:
                     trcadvect_tmar(dm, Q, self.Dh, self.DD, self.J)
             # QTRC = (DENS_hyd + DDENS_TRC) / (DENS_hyd + DDENS) * QTRC_tmp: DDENS_TRC = DDENS

@@ -29,7 +29,8 @@ module fedg_c_binding
   public :: fedg_drhot2pres, fedg_calc_phyd_hgrad
   public :: fedg_heve_cal_tend_cco, fedg_heve_add_bnd_contrib
   public :: fedg_trcadvect_cal_tend, fedg_trcadvect_calc_fct_coef, fedg_trcadvect_tmar
-  public :: fedg_trcadvect_save_massflux
+  public :: fedg_trcadvect_save_massflux, fedg_trcadvect_cal_alphdens_advtest, fedg_trcvar_rk_advance
+  public :: fedg_tracer_modalfilter_apply
 
   interface
     integer(c_int) function fedg_ctx_create(desc, ctx) bind(C, name="fedg_ctx_create")
@@ -174,5 +175,37 @@ module fedg_c_binding
       integer(c_int), value :: rkstage, is_hevi
       real(c_double), value :: tavg_weight_h, tavg_weight_v
     end function fedg_trcadvect_save_massflux
+
+    !> atm_dyn_dgm_trcadvect3d_heve_cal_alphdens_advtest (FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:407-445)
+    integer(c_int) function fedg_trcadvect_cal_alphdens_advtest(ctx, alph_dens_M, alph_dens_P, DDENS_, MOMX_, &
+        MOMY_, MOMZ_, DENS_hyd) bind(C, name="fedg_trcadvect_cal_alphdens_advtest")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: alph_dens_M(*), alph_dens_P(*)
+      real(c_double), intent(in) :: DDENS_(*), MOMX_(*), MOMY_(*), MOMZ_(*), DENS_hyd(*)
+    end function fedg_trcadvect_cal_alphdens_advtest
+
+    !> timeint_rk%Advance_trcvar, low-storage schemes (CMN/scale_timeint_rk.F90:1390-1498); the
+    !! tables are the timeint_rk object's coef_sig_ex, coef_gam_ex (host) and coef_c_ex
+    integer(c_int) function fedg_trcvar_rk_advance(ctx, nowstage, nstage, coef_sig_ex, coef_gam_ex, coef_c_ex, &
+        dt, q, var0, varTmp, tend, DDENS, DDENS0, DENS_hyd) bind(C, name="fedg_trcvar_rk_advance")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nowstage, nstage
+      real(c_double), intent(in) :: coef_sig_ex(*), coef_gam_ex(*), coef_c_ex(*)
+      real(c_double), value :: dt
+      real(c_double) :: q(*), var0(*), varTmp(*)
+      real(c_double), intent(in) :: tend(*), DDENS(*), DDENS0(*), DENS_hyd(*)
+    end function fedg_trcvar_rk_advance
+
+    !> atm_dyn_dgm_tracer_modalfilter_apply (FDS/scale_atm_dyn_dgm_modalfilter.F90:232-270);
+    !! the filter matrices are host arrays
+    integer(c_int) function fedg_tracer_modalfilter_apply(ctx, QTRC_, DENS_hyd, DDENS_, MFilter_tracer_h1D, &
+        MFilter_tracer_v1D) bind(C, name="fedg_tracer_modalfilter_apply")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: QTRC_(*)
+      real(c_double), intent(in) :: DENS_hyd(*), DDENS_(*), MFilter_tracer_h1D(*), MFilter_tracer_v1D(*)
+    end function fedg_tracer_modalfilter_apply
   end interface
 end module fedg_c_binding

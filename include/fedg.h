@@ -313,6 +313,16 @@ int fedg_trcadvect_cal_alphdens_advtest(fedg_ctx *ctx, double *alph_dens_M, doub
                                         const double *DDENS_, const double *MOMX_, const double *MOMY_,
                                         const double *MOMZ_, const double *DENS_hyd);
 
+/* atm_dyn_dgm_tracer_modalfilter_apply (FDS/scale_atm_dyn_dgm_modalfilter.F90:232-270)
+ * with ModalFilter_tracer (ELM/scale_element_operation_tensorprod3D.F90, the
+ * x, y, z passes of element_operation_kernel_matvec_ModalFilter_P*): per
+ * element, weight = Gsqrt (DENS_hyd + DDENS_), QTRC_ = filter(weight QTRC_) /
+ * weight, in place.  MFilter_tracer_h1D, MFilter_tracer_v1D: host (n1,n1)
+ * column-major (fedg_modalfilter_matrix with the tracer filter's parameters,
+ * PARAM_ATMOS_DYN_TRACER_MODALFILTER). */
+int fedg_tracer_modalfilter_apply(fedg_ctx *ctx, double *QTRC_, const double *DENS_hyd, const double *DDENS_,
+                                  const double *MFilter_tracer_h1D, const double *MFilter_tracer_v1D);
+
 /* timeint_rk%Advance_trcvar for the low-storage schemes
  * (CMN/scale_timeint_rk.F90:1390-1498, rk_advance_trcvar_low_storage): the
  * density-weighted RK update of one tracer at stage nowstage (1-based) of

@@ -1013,6 +1013,30 @@ static void modalfilter_elem(int N, const double *Fh, const double *Fv, const do
       }
 }
 
+/* FDS/scale_atm_dyn_dgm_modalfilter.F90:232-270 (atm_dyn_dgm_tracer_modalfilter_apply):
+ * weight = Gsqrt (DENS_hyd + DDENS), QTRC = filter(weight QTRC) / weight with
+ * the tracer's matrices MF_h, MF_v ((n1,n1) column-major) */
+void oracle_tracer_modalfilter_apply(const oracle_mesh *m, double *QTRC, const double *DENS_hyd, const double *DDENS,
+                                     const double *MF_h, const double *MF_v) {
+  const int N = m->n1, Np = m->Np;
+#pragma omp parallel
+  {
+    double *tmp = (double *)malloc(sizeof(double) * Np * 4);
+    double *work = tmp + Np, *out = tmp + 2 * Np, *weight = tmp + 3 * Np;
+#pragma omp for schedule(static)
+    for (int ke = 0; ke < m->Ne; ++ke) {
+      const long o = (long)Np * ke;
+      for (int kk = 0; kk < Np; ++kk) {
+        weight[kk] = m->Gsqrt[o + kk] * (DENS_hyd[o + kk] + DDENS[o + kk]);
+        tmp[kk] = weight[kk] * QTRC[o + kk];
+      }
+      modalfilter_elem(N, MF_h, MF_v, tmp, work, out);
+      for (int kk = 0; kk < Np; ++kk) QTRC[o + kk] = out[kk] / weight[kk];
+    }
+    free(tmp);
+  }
+}
+
 void oracle_modalfilter_apply(const oracle_mesh *m, double *DDENS, double *MOMX, double *MOMY, double *MOMZ,
                               double *DRHOT, int do_weight_Gsqrt) {
   const int N = m->n1, Np = m->Np;

@@ -61,6 +61,10 @@ typedef struct {
 void oracle_modalfilter_apply(const oracle_mesh *m, double *DDENS, double *MOMX, double *MOMY, double *MOMZ,
                               double *DRHOT, int do_weight_Gsqrt);
 
+/* FDS/scale_atm_dyn_dgm_modalfilter.F90:232-270 (tracer filter, density-weighted) */
+void oracle_tracer_modalfilter_apply(const oracle_mesh *m, double *QTRC, const double *DENS_hyd, const double *DDENS,
+                                     const double *MF_h, const double *MF_v);
+
 /* FDS/scale_atm_dyn_dgm_nonhydro3d_common.F90:428-479 (CPU branch) */
 void oracle_drhot2pres(const oracle_mesh *m, double *PRES, double *DPRES,
                        const double *DRHOT, const double *PRES_hyd, const double *THERM_hyd,

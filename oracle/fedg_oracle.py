@@ -57,6 +57,7 @@ def _load():
     lib.oracle_trcadvect_calc_fct_coef.argtypes = [M] + [dp] * 12 + [C.c_double, C.c_double, dp, C.c_int]
     lib.oracle_trcadvect_tmar.argtypes = [M] + [dp] * 5
     lib.oracle_trcadvect_cal_alphdens_advtest.argtypes = [M] + [dp] * 7
+    lib.oracle_tracer_modalfilter_apply.argtypes = [M] + [dp] * 5
     lib.oracle_trcvar_rk_advance.argtypes = [C.c_long, C.c_int, C.c_int, dp, dp, dp, C.c_double] + [dp] * 7
     lib.oracle_trcadvect_save_massflux.argtypes = [M] + [dp] * 12 + [C.c_int, C.c_double, C.c_double, C.c_int]
     lib.oracle_rk_tab_get.argtypes = [C.c_char_p, C.c_double, C.POINTER(RkTabC)]
@@ -243,8 +244,14 @@ class OracleMesh:
         LIB.oracle_trcvar_rk_advance(n, int(nowstage), tab.nstage, _d(tab.sig_f), _d(tab.gam_f), _d(tab.c), float(dt),
                                      _d(q), _d(var0), _d(varTmp), _d(tend), _d(DDENS), _d(DDENS0), _d(DENS_hyd))
 
+    def tracer_modalfilter_apply(self, QTRC, DENS_hyd, DDENS, MF_h, MF_v):
+        """Density-weighted tracer filter in place; MF_h, MF_v (n1,n1) numpy matrices."""
+        h = np.asarray(MF_h, dtype=np.float64).ravel(order="F").copy()
+        v = np.asarray(MF_v, dtype=np.float64).ravel(order="F").copy()
+        LIB.oracle_tracer_modalfilter_apply(self.ref, _d(QTRC), _d(DENS_hyd), _d(DDENS), _d(h), _d(v))
+
     def tracer_advection(self, Q, MOMX, MOMY, MOMZ, DDENS, DENS_hyd, dt, nsteps, scheme="ERK_SSP_3s3o",
-                         disable_limiter=False):
+                         disable_limiter=False, modal_filter=None):
         """AtmDynDGMDriver_trcadv3d_update with ONLY_TRACERADV_FLAG
         (FDS/scale_atm_dyn_dgm_driver_trcadv3d.F90:312-560), nsteps steps of
         the tracer Q (in place) by fixed mass fluxes; the flow fields' halos
@@ -268,6 +275,8 @@ class OracleMesh:
                 self.exchange(R)
                 tend = self.trcadvect_cal_tend(Q, MX, MY, MZ, aM, aP, R, tp)
                 self.trcvar_rk_advance(s, t, dt, Q, var0, varTmp, tend, DD, DD, Dh)
+                if s == t.nstage and modal_filter is not None:
+                    self.tracer_modalfilter_apply(Q, Dh, DD, *modal_filter)
                 if s == t.nstage and not disable_limiter:
                     self.trcadvect_tmar(Q, Dh, DD)
         return Q

@@ -146,16 +146,37 @@ def test_trcadvect_fct_tmar_massflux_bitwise(case, metrics):
     dm.close()
 
 
+@pytest.mark.parametrize("case", CASES, ids=[f"p{c[0]}" for c in CASES])
+def test_tracer_modalfilter_bitwise(case):
+    print(f"tracer modal filter p={case[0]}", flush=True)
+    rng = np.random.default_rng(400 + case[0])
+    mesh, om, dm = _mesh_and_ctx(case, "random", rng)
+    n = mesh.Np * mesh.Ne
+    Dh, DD, Q = mesh.field(1.1), mesh.field(), mesh.field()
+    DD[:n] = 0.05 * rng.standard_normal(n)
+    Q[:n] = 1e-3 * rng.uniform(0.0, 1.0, n)
+    MFh = fedg.modalfilter_matrix(case[0])
+    MFv = fedg.modalfilter_matrix(case[0], etac=0.4, alpha=10.0, order=8)
+    ref = Q.copy()
+    om.tracer_modalfilter_apply(ref, Dh, DD, MFh, MFv)
+    gQ = fedg.DevField(Q.size, Q)
+    fedg.tracer.tracer_modalfilter_apply(dm, gQ, fedg.DevField(Dh.size, Dh), fedg.DevField(DD.size, DD), MFh, MFv)
+    g = gQ.download()[:n]
+    dm.close()
+    assert np.array_equal(g, ref[:n]), f"max abs diff {np.max(np.abs(g - ref[:n]))}"
+
+
+@pytest.mark.parametrize("filt", [False, True], ids=["", "modalfilter"])
 @pytest.mark.parametrize("limiter", [True, False], ids=["limiter", "no-limiter"])
 @pytest.mark.parametrize("scheme", ["ERK_SSP_3s3o", "ERK_SSP_10s4o_2N"])
 @pytest.mark.parametrize("metrics", ["random", "flat"])
-def test_tracer_advection_steps_bitwise(metrics, scheme, limiter):
+def test_tracer_advection_steps_bitwise(metrics, scheme, limiter, filt):
     """Three steps of the tracer-only advection driver (FCT coefficient,
     tendency, density-weighted RK update and TMAR per stage, on the device)
     against the oracle's loop, bit for bit; with a step profile the limiter and
     TMAR are active."""
     case = (3, 4, 3, 2, (True, True, True))
-    print(f"tracer advection {metrics} {scheme} limiter={limiter}", flush=True)
+    print(f"tracer advection {metrics} {scheme} limiter={limiter} filter={filt}", flush=True)
     rng = np.random.default_rng(300)
     mesh, om, dm = _mesh_and_ctx(case, metrics, rng)
     from fedg.tracer import TracerAdvection
@@ -168,8 +189,11 @@ def test_tracer_advection_steps_bitwise(metrics, scheme, limiter):
     Q = mesh.field()
     Q[:n] = np.where(mesh.x < 4.0e3, 1.0e-3, 0.0) * rng.uniform(0.8, 1.0, n)
     dt = 6.0
-    ref = om.tracer_advection(Q.copy(), M[0], M[1], M[2], DD, Dh, dt, 3, scheme=scheme, disable_limiter=not limiter)
-    st = TracerAdvection(dm, M[0], M[1], M[2], DD, Dh, mesh.J, dt, scheme=scheme, disable_limiter=not limiter)
+    mf = (fedg.modalfilter_matrix(3), fedg.modalfilter_matrix(3)) if filt else None
+    ref = om.tracer_advection(Q.copy(), M[0], M[1], M[2], DD, Dh, dt, 3, scheme=scheme, disable_limiter=not limiter,
+                              modal_filter=mf)
+    st = TracerAdvection(dm, M[0], M[1], M[2], DD, Dh, mesh.J, dt, scheme=scheme, disable_limiter=not limiter,
+                         modal_filter=mf)
     gQ = fedg.DevField(Q.size, Q)
     st.step(gQ, 3)
     g = gQ.download()[:n]

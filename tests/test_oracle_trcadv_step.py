@@ -143,3 +143,43 @@ def test_density_weighted_update_integrates_a_constant_mass_tendency(name):
         else:
             want = (m0 + K[:n] * dt) / (Dh + DD)[:n]
         np.testing.assert_allclose(q[:n], want, rtol=1e-13, atol=0, err_msg=f"{name} stage {s}")
+
+
+@pytest.mark.parametrize("porder", [3, 4])
+def test_tracer_modalfilter_keeps_low_modes_and_element_mass(porder):
+    """atm_dyn_dgm_tracer_modalfilter_apply filters Gsqrt dens QTRC: when that
+    product is a polynomial of degree <= etac p in each direction the filter
+    leaves it alone, and it never changes an element's tracer mass (the mean
+    mode is undamped)."""
+    from fedg import modalfilter_matrix
+    L = 3.0e3
+    mesh = BoxMesh(porder, 3, 2, 2, (0.0, L), (0.0, L), (0.0, L), periodic=(True, True, True))
+    rng = np.random.default_rng(12)
+    n = mesh.Np * mesh.Ne
+    Gsqrt = mesh.field()
+    Gsqrt[:n] = 1.0 + 0.1 * rng.uniform(-1.0, 1.0, n)
+    om = O.OracleMesh(mesh, Gsqrt=Gsqrt)
+    Dh, DD = mesh.field(1.1), mesh.field()
+    DD[:n] = 0.05 * rng.standard_normal(n)
+    weight = Gsqrt[:n] * (Dh + DD)[:n]
+    MF = modalfilter_matrix(porder)  # etac 2/3: modes up to 2p/3 pass unchanged
+    MFv = modalfilter_matrix(porder, etac=0.4, alpha=10.0, order=8)  # only the linear mode passes in z
+    x, y, z = mesh.x / L, mesh.y / L, mesh.z / L
+    deg = (2 * porder) // 3
+    P = (1.0 + x ** deg) * (1.0 + 0.5 * y ** deg) * (1.0 + 0.25 * z)
+    Q = mesh.field()
+    Q[:n] = 1e-3 * P / weight
+    Q0 = Q.copy()
+    om.tracer_modalfilter_apply(Q, Dh, DD, MF, MFv)
+    np.testing.assert_allclose(Q[:n], Q0[:n], rtol=1e-12, atol=0)
+    # random tracer: changed, element masses kept
+    Q[:n] = 1e-3 * rng.uniform(0.5, 1.5, n)
+    Q0 = Q.copy()
+    om.tracer_modalfilter_apply(Q, Dh, DD, MF, MFv)
+    assert np.abs(Q[:n] - Q0[:n]).max() > 1e-6
+    w = mesh.elem.w1d
+    w3 = (w[None, None, :] * w[None, :, None] * w[:, None, None]).ravel()
+    W = np.tile(w3, mesh.Ne) * mesh.J * weight
+    m0 = (W * Q0[:n]).reshape(mesh.Ne, -1).sum(axis=1)
+    m1 = (W * Q[:n]).reshape(mesh.Ne, -1).sum(axis=1)
+    np.testing.assert_allclose(m1, m0, rtol=1e-13, atol=0)
