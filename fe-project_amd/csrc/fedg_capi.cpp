@@ -89,6 +89,9 @@ struct fedg_heve {
   int fp32;                 // stage tendency in the FP32 perturbation form
   int mfilter;              // modal filter after every step
   double* lvl;              // per-level base-state tables (horizontally homogeneous base state) or null
+  // time-averaged mass fluxes for the tracers (QA > 0): MFLX_x/y/z_tavg and
+  // ALPH_DENS_M/P_tavg saved at every stage, or null
+  double *mflx[3], *alph[2];
 };
 
 namespace {
@@ -1330,10 +1333,35 @@ int fedg_heve_destroy(fedg_heve* st) {
 
 int fedg_heve_is_flat(const fedg_heve* st) { return st ? st->flat : 0; }
 
+int fedg_heve_set_massflux(fedg_heve* st, double* MFLX_x_tavg, double* MFLX_y_tavg, double* MFLX_z_tavg,
+                           double* alph_dens_M, double* alph_dens_P) {
+  if (!st) return FEDG_ERR_ARG;
+  const bool any = MFLX_x_tavg || MFLX_y_tavg || MFLX_z_tavg || alph_dens_M || alph_dens_P;
+  const bool all = MFLX_x_tavg && MFLX_y_tavg && MFLX_z_tavg && alph_dens_M && alph_dens_P;
+  if (any && !all) {
+    set_error("fedg_heve_set_massflux: give all five arrays or none");
+    return FEDG_ERR_ARG;
+  }
+  if (all && st->fp32) {
+    set_error("fedg_heve_set_massflux: the mass-flux save needs the FP64 stage (its DPRES)");
+    return FEDG_ERR_UNSUPPORTED;
+  }
+  st->mflx[0] = MFLX_x_tavg;
+  st->mflx[1] = MFLX_y_tavg;
+  st->mflx[2] = MFLX_z_tavg;
+  st->alph[0] = alph_dens_M;
+  st->alph[1] = alph_dens_P;
+  return FEDG_OK;
+}
+
 int fedg_heve_set_precision(fedg_heve* st, int bits) {
   if (!st || (bits != 32 && bits != 64)) return FEDG_ERR_ARG;
   if (bits == 32 && !st->flat) {
     set_error("the FP32 perturbation form is built for the flat-terrain box with dry air");
+    return FEDG_ERR_UNSUPPORTED;
+  }
+  if (bits == 32 && st->mflx[0]) {
+    set_error("the mass-flux save needs the FP64 stage (fedg_heve_set_massflux with NULL first)");
     return FEDG_ERR_UNSUPPORTED;
   }
   st->fp32 = bits == 32;
@@ -1421,6 +1449,32 @@ static int heve_step(fedg_heve* st, std::vector<hipEvent_t>* evs, size_t* ev_i) 
     rk.DPout = st->DP[outb];
     rk.cb = t.b_ex[s - 1] * t.dt;
     rk.ca = (s < t.nstage) ? t.dt * t.a_ex[s][s - 1] : 0.0;
+    if (st->mflx[0]) {
+      // FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:899-915: the stage state
+      // as the stage's cal_tend sees it (exterior values of the exchange and
+      // the velocity BC, resolved through the neighbour map: the stepper's
+      // buffers hold no halo tail), weighted by b_ex(s)
+      double* const* qs = st->buf[cur];
+      MflxArgs a{};
+      a.nbr = 1;
+      a.FX = st->mflx[0];
+      a.FY = st->mflx[1];
+      a.FZ = st->mflx[2];
+      a.alM = st->alph[0];
+      a.alP = st->alph[1];
+      a.DD = qs[0];
+      a.MX = qs[3];
+      a.MY = qs[4];
+      a.MZ = qs[2];
+      a.DP = st->DP[cur];
+      a.Dh = st->DENS_hyd;
+      a.Ph = st->PRES_hyd;
+      a.rkstage = s;
+      a.hevi = 0;
+      a.twh = t.b_ex[s - 1];
+      a.twv = t.b_ex[s - 1];
+      TRY(launch_save_massflux(ctx, a));
+    }
     if (evs) FEDG_HIP_CHECK(hipEventRecord((*evs)[(*ev_i)++], ctx->stream));
     if (st->fp32) TRY(launch_heve_fused32(ctx, in, rk));
     else TRY(launch_heve_fused(ctx, in, rk, st->flat));

@@ -135,6 +135,7 @@ struct MflxArgs {
   const double *DD, *MX, *MY, *MZ, *DP, *Dh, *Ph;
   int rkstage, hevi;
   double twh, twv;
+  int nbr;  // fields without a halo tail: exterior values through DevMesh::nbr / nbr_bc
 };
 int launch_trcadv(fedg_ctx* ctx, const TrcLaunch& t);
 int launch_tmar(fedg_ctx* ctx, double* Q, const double* Dh, const double* DD, const double* J, const TrcW& w);

@@ -1423,15 +1423,24 @@ __global__ void __launch_bounds__(256) alphdens_kernel(const DevMesh m, const Mf
   const int f = fp / Nfp, an = fp - f * Nfp;
   const uint32_t h = (uint32_t)(face_node<N1>(f, an) % Nfp) + (uint32_t)Nfp * m.EMap3Dto2D[ke];
   const uint32_t iM = (uint32_t)m.vmapM[gf], iP = (uint32_t)m.vmapP[gf];
-  const double densM = ldg(a.DD, iM) + ldg(a.Dh, iM);
-  const double densP = ldg(a.DD, iP) + ldg(a.Dh, iP);
+  // a.nbr: the fields have no halo tail (the fused stepper's buffers): the
+  // exterior value is read at the interior node the exchange would copy and
+  // the velocity BC that applyBC would apply is applied here
+  const uint32_t jP = a.nbr ? (uint32_t)m.nbr[gf] : iP;
   const double nx = ldg(m.nx, gf), ny = ldg(m.ny, gf), nz = ldg(m.nz, gf);
+  const double MXM = ldg(a.MX, iM), MYM = ldg(a.MY, iM), MZM = ldg(a.MZ, iM);
+  double MXP = ldg(a.MX, jP), MYP = ldg(a.MY, jP), MZP = ldg(a.MZ, jP);
+  if (a.nbr && m.nbr_bc[gf] != FEDG_BND_NONE)
+    reflect_momentum(m.nbr_bc[gf], ldg(m.Gsqrt, iM), ldg(m.GsqrtH, h), ldg(m.G11, h), ldg(m.G12, h), ldg(m.G22, h),
+                     ldg(m.G13, iM), ldg(m.G23, iM), nx, ny, nz, MXM, MYM, MZM, MXP, MYP, MZP);
+  const double densM = ldg(a.DD, iM) + ldg(a.Dh, iM);
+  const double densP = ldg(a.DD, jP) + ldg(a.Dh, jP);
   double Gnn = ldg(m.G11, h) * fabs(nx) + ldg(m.G22, h) * fabs(ny);
   if (!a.hevi) Gnn = Gnn + fabs(nz);
-  const double VelM = (ldg(a.MX, iM) * nx + ldg(a.MY, iM) * ny + ldg(a.MZ, iM) * nz) / densM;
-  const double VelP = (ldg(a.MX, iP) * nx + ldg(a.MY, iP) * ny + ldg(a.MZ, iP) * nz) / densP;
+  const double VelM = (MXM * nx + MYM * ny + MZM * nz) / densM;
+  const double VelP = (MXP * nx + MYP * ny + MZP * nz) / densP;
   const double cM = sqrt(Gnn * gamm * (ldg(a.Ph, iM) + ldg(a.DP, iM)) / densM) + fabs(VelM);
-  const double cP = sqrt(Gnn * gamm * (ldg(a.Ph, iP) + ldg(a.DP, iP)) / densP) + fabs(VelP);
+  const double cP = sqrt(Gnn * gamm * (ldg(a.Ph, jP) + ldg(a.DP, jP)) / densP) + fabs(VelP);
   const double alpha = cM > cP ? cM : cP;
   const double tw = a.twh * (fabs(nx) + fabs(ny)) + a.twv * fabs(nz);
   const double aM0 = a.rkstage == 1 ? 0.0 : ldg(a.alM, gf);

@@ -291,6 +291,13 @@ class HeveStepper:
     def flat(self) -> bool:
         return bool(LIB.fedg_heve_is_flat(self.handle))
 
+    def set_massflux(self, MFLX_x=None, MFLX_y=None, MFLX_z=None, alph_dens_M=None, alph_dens_P=None):
+        """Save the time-averaged mass fluxes and face damping coefficients of every stage into these
+        device fields (atm_dyn_dgm_trcadvect3d_save_massflux, for the tracer step); no arguments: stop."""
+        check(LIB.fedg_heve_set_massflux(self.handle, *[_p(a) if a is not None else None for a in
+                                                         (MFLX_x, MFLX_y, MFLX_z, alph_dens_M, alph_dens_P)]),
+              "fedg_heve_set_massflux")
+
     def set_precision(self, bits: int):
         """64: the reference's FP64 stage; 32: the FP32 perturbation-form tendency (FP64 state and RK)."""
         check(LIB.fedg_heve_set_precision(self.handle, int(bits)), "fedg_heve_set_precision")

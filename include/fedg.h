@@ -425,6 +425,15 @@ int fedg_heve_create(fedg_ctx *ctx, const char *scheme, double dt, const double 
 int fedg_heve_destroy(fedg_heve *st);
 /* 1 if the flat-terrain uniform-box kernel is used, 0 for the general-metric kernel */
 int fedg_heve_is_flat(const fedg_heve *st);
+/* Tracer coupling (QA > 0): the driver's per-stage call of
+ * atm_dyn_dgm_trcadvect3d_save_massflux (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:899-915)
+ * on every stage state (halo exchanged, velocity BC applied) with weights
+ * b_ex(s), accumulating MFLX_x/y/z_tavg (Np,NeA) and alph_dens_M/_P
+ * (NfpTot,Ne) on the device for the tracer step that follows (the first stage
+ * of each step resets them).  All five device arrays, or all NULL to stop.
+ * FP64 stages only. */
+int fedg_heve_set_massflux(fedg_heve *st, double *MFLX_x_tavg, double *MFLX_y_tavg, double *MFLX_z_tavg,
+                           double *alph_dens_M, double *alph_dens_P);
 /* No counterpart in the reference (one global precision switch, Mkinclude:34-35):
  * bits = 32 evaluates each stage's tendency -- fluxes, face flux, derivative
  * contractions, lift, buoyancy -- in FP32 on the perturbation variables about

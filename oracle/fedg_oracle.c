@@ -952,6 +952,17 @@ void oracle_rk_store_impl(const oracle_rk_tab *t, int stage, long n, double *q, 
 
 /* ---------------------------------------------------------------------- */
 /* FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:614-963, HEVE branch */
+/* Tracer coupling of the driver (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:899-915):
+ * when set, oracle_heve_update calls save_massflux on every stage state. */
+static double *g_mflx[3], *g_alph[2];
+void oracle_heve_set_massflux(double *MFLX_x, double *MFLX_y, double *MFLX_z, double *alph_M, double *alph_P) {
+  g_mflx[0] = MFLX_x;
+  g_mflx[1] = MFLX_y;
+  g_mflx[2] = MFLX_z;
+  g_alph[0] = alph_M;
+  g_alph[1] = alph_P;
+}
+
 void oracle_heve_update(const oracle_mesh *m, const oracle_rk_tab *t, int nsteps,
                         double *const q[5], double *PRES, double *DPRES,
                         const double *DENS_hyd, const double *PRES_hyd, const double *PRES_hyd_ref,
@@ -976,6 +987,10 @@ void oracle_heve_update(const oracle_mesh *m, const oracle_rk_tab *t, int nsteps
                            q[MOMZ_VID], q[RHOT_VID], DPRES, DENS_hyd, PRES_hyd, PRES_hyd_ref, THERM_hyd,
                            CORIOLIS, Rtot, CVtot, CPtot, DPhydDx, DPhydDy);
       /* add_phy_tend adds zero physics tendencies (dry run without physics). */
+      if (g_mflx[0])
+        oracle_trcadvect_save_massflux(m, g_mflx[0], g_mflx[1], g_mflx[2], g_alph[0], g_alph[1], q[DENS_VID],
+                                       q[MOMX_VID], q[MOMY_VID], q[MOMZ_VID], DPRES, DENS_hyd, PRES_hyd, s,
+                                       t->b_ex[s - 1], t->b_ex[s - 1], 0);
       for (int v = 0; v < 5; ++v)
         oracle_rk_advance(t, s, n, q[v], var0 + v * NpNeA, varTmp + v * NpNeA, tend + v * NpNeA, NULL,
                           5 * NpNeA);

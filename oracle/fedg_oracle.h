@@ -180,6 +180,9 @@ void oracle_rk_store_impl(const oracle_rk_tab *t, int stage, long n, double *q, 
  * q[v] are the 5 prognostic fields (Np,NeA) in the order DDENS, DRHOT, MOMZ,
  * MOMX, MOMY (PRGVAR ids 1..5).  work must hold 5*Np*NeA*(2 + tend_buf_size) +
  * 2*Np*NeA doubles.  PRES/DPRES are updated as the reference does. */
+/* when set (all five non-NULL), oracle_heve_update saves the tracer mass fluxes
+ * of every stage (save_massflux, FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:899-915) */
+void oracle_heve_set_massflux(double *MFLX_x, double *MFLX_y, double *MFLX_z, double *alph_M, double *alph_P);
 void oracle_heve_update(const oracle_mesh *m, const oracle_rk_tab *t, int nsteps,
                         double *const q[5], double *PRES, double *DPRES,
                         const double *DENS_hyd, const double *PRES_hyd, const double *PRES_hyd_ref,

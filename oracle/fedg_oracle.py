@@ -58,6 +58,7 @@ def _load():
     lib.oracle_trcadvect_tmar.argtypes = [M] + [dp] * 5
     lib.oracle_trcadvect_cal_alphdens_advtest.argtypes = [M] + [dp] * 7
     lib.oracle_tracer_modalfilter_apply.argtypes = [M] + [dp] * 5
+    lib.oracle_heve_set_massflux.argtypes = [dp] * 5
     lib.oracle_trcvar_rk_advance.argtypes = [C.c_long, C.c_int, C.c_int, dp, dp, dp, C.c_double] + [dp] * 7
     lib.oracle_trcadvect_save_massflux.argtypes = [M] + [dp] * 12 + [C.c_int, C.c_double, C.c_double, C.c_int]
     lib.oracle_rk_tab_get.argtypes = [C.c_char_p, C.c_double, C.POINTER(RkTabC)]
@@ -243,6 +244,14 @@ class OracleMesh:
         n = self.mesh.Np * self.mesh.Ne
         LIB.oracle_trcvar_rk_advance(n, int(nowstage), tab.nstage, _d(tab.sig_f), _d(tab.gam_f), _d(tab.c), float(dt),
                                      _d(q), _d(var0), _d(varTmp), _d(tend), _d(DDENS), _d(DDENS0), _d(DENS_hyd))
+
+    def heve_set_massflux(self, MFLX=None, alph=None):
+        """heve_update saves the tracer mass fluxes of every stage into MFLX = (x, y, z) and
+        alph = (M, P) (the arrays must stay alive); None: stop."""
+        if MFLX is None:
+            LIB.oracle_heve_set_massflux(None, None, None, None, None)
+        else:
+            LIB.oracle_heve_set_massflux(*[_d(a) for a in MFLX], *[_d(a) for a in alph])
 
     def tracer_modalfilter_apply(self, QTRC, DENS_hyd, DDENS, MF_h, MF_v):
         """Density-weighted tracer filter in place; MF_h, MF_v (n1,n1) numpy matrices."""

@@ -294,3 +294,40 @@ def test_config2_200_steps_against_golden():
         assert rel <= 2e-11, name
         assert abs(np.linalg.norm(out[v]) - g["portable_norms"][v]) <= 1e-12 * g["portable_norms"][v]
         assert abs(np.linalg.norm(out[v]) - g["libm_norms"][v]) <= 1e-11 * g["libm_norms"][v]
+
+
+def test_fused_steps_save_massflux_bitwise(setup):
+    """The driver's per-stage save_massflux for the tracers (QA > 0) inside the
+    fused stepper: the time-averaged mass fluxes and face damping
+    coefficients of the last step, and the state, against the oracle's
+    update loop bit for bit (the stage states' halos exchanged and the SLIP
+    condition applied, as the stage's cal_tend sees them)."""
+    s = setup
+    mesh, bs, om, case = s["mesh"], s["bs"], s["om"], s["case"]
+    print(f"save_massflux in the fused stepper p={mesh.porder}", flush=True)
+    dm = fedg.DeviceMesh(mesh)
+    st = fedg.HeveStepper(dm, bs, s["Dx"], s["Dy"], case.dt)
+    nA, nf, n = mesh.Np * mesh.NeA, mesh.NfpTot * mesh.Ne, mesh.Np * mesh.Ne
+    MF = [np.zeros(nA) for _ in range(3)]
+    al = [np.zeros(nf), np.zeros(nf)]
+    gMF = [fedg.DevField(nA) for _ in range(3)]
+    gal = [fedg.DevField(nf), fedg.DevField(nf)]
+    st.set_massflux(*gMF, *gal)
+    q = perturbed_state(mesh, bs, case, om, s["Dx"], s["Dy"], nsteps=1, seed=21)[0]
+    st.set_state(q)
+    om.heve_set_massflux(MF, al)
+    try:
+        om.heve_update(q, bs, s["Dx"], s["Dy"], case.dt, 3)
+    finally:
+        om.heve_set_massflux(None)
+    st.run(3)
+    g = st.get_state()
+    for name, a, b in zip(fedg.VARS, g, q):
+        assert np.array_equal(a, b[:n]), name
+    for k, (a, b) in enumerate(zip(gMF, MF)):
+        assert np.array_equal(a.download()[:n], b[:n]), f"MFLX {'xyz'[k]}"
+    for k, (a, b) in enumerate(zip(gal, al)):
+        assert np.array_equal(a.download(), b), f"alph_dens {'MP'[k]}"
+    st.set_massflux()
+    st.close()
+    dm.close()

@@ -183,3 +183,36 @@ def test_tracer_modalfilter_keeps_low_modes_and_element_mass(porder):
     m0 = (W * Q0[:n]).reshape(mesh.Ne, -1).sum(axis=1)
     m1 = (W * Q[:n]).reshape(mesh.Ne, -1).sum(axis=1)
     np.testing.assert_allclose(m1, m0, rtol=1e-13, atol=0)
+
+
+def test_dynamics_step_saves_the_time_averaged_mass_flux():
+    """With the tracer coupling on, the HEVE update saves sum_s b_s MOM^(s)
+    and the b-weighted damping coefficients.  A uniform horizontal wind over
+    the horizontally homogeneous hydrostatic base state is a steady solution,
+    so after a step MFLX = MOM (sum b_s = 1) and on vertical faces alph_dens =
+    rho c + 0 with c the sound speed, to rounding."""
+    from fedg import cases, constants
+    case = cases.scaled_case(cases.config1(), 3, 2, 3, porder=2)
+    mesh, bs, _ = cases.build(case)
+    om = O.OracleMesh(mesh)
+    Dx, Dy = om.calc_phyd_hgrad(bs)
+    U, V = 12.0, -5.0
+    dens = bs.DENS_hyd
+    q = [mesh.field(), mesh.field(), mesh.field(), dens * U, dens * V]
+    nA, nf, n = mesh.Np * mesh.NeA, mesh.NfpTot * mesh.Ne, mesh.Np * mesh.Ne
+    MF = [np.zeros(nA) for _ in range(3)]
+    al = [np.zeros(nf), np.zeros(nf)]
+    om.heve_set_massflux(MF, al)
+    try:
+        om.heve_update(q, bs, Dx, Dy, case.dt, 1)
+    finally:
+        om.heve_set_massflux(None)
+    np.testing.assert_allclose(MF[0][:n], dens[:n] * U, rtol=1e-10)
+    np.testing.assert_allclose(MF[1][:n], dens[:n] * V, rtol=1e-10)
+    assert np.abs(MF[2][:n]).max() <= 1e-8 * abs(U) * dens[:n].max()
+    nz = mesh.normal_fn[2 * nf:3 * nf]
+    iM = mesh.vmapM - 1
+    gam = constants.SCALE.CPdry / constants.SCALE.CVdry
+    vert = np.abs(nz) == 1.0
+    c = np.sqrt(gam * bs.PRES_hyd[iM] / dens[iM])
+    np.testing.assert_allclose(al[0][vert], (c * dens[iM])[vert], rtol=1e-10)
