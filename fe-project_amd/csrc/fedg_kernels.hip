@@ -1170,12 +1170,12 @@ __global__ void __launch_bounds__(EPB * N1 * N1 * N1) trcadv_kernel(const DevMes
   constexpr int Np = N1 * N1 * N1, Nfp = N1 * N1, NfpTot = 6 * Nfp;
   constexpr bool TEND = KIND == TRC_TEND;
   constexpr int EF = TEND ? EPB : 1;  // arrays only the tendency uses
+  constexpr int NFT = (NfpTot + Np - 1) / Np;  // face nodes per thread
   __shared__ double sDt[N1 * N1];
   __shared__ double sF[EF][3][TEND ? Np : 1];
-  __shared__ double sNF[EF][TEND ? NfpTot : 1];  // numflux
-  __shared__ double sQF[EF][TEND ? NfpTot : 1];  // QTRC_M * MomFlxM
-  __shared__ double sRP[EF][TEND ? NfpTot : 1], sRM[EF][TEND ? NfpTot : 1];
   __shared__ double sDF[EF][TEND ? NfpTot : 1];  // del_flux
+  // numflux, QTRC_M MomFlxM and the two fct_coef of this thread's face nodes
+  double rNF[NFT], rQF[NFT], rRP[NFT], rRM[NFT];
   __shared__ double sJF[EPB][NfpTot];            // J(iM) * Fscale * numflux
   __shared__ double sOut[EPB][6];                // net outward flux of each face
   __shared__ double sM[TEND ? 1 : EPB][TEND ? 1 : Np];  // FCT: mass terms of the nodes
@@ -1183,7 +1183,7 @@ __global__ void __launch_bounds__(EPB * N1 * N1 * N1) trcadv_kernel(const DevMes
   const int tid = threadIdx.x;
   const int e = tid / Np;
   const int p = tid - e * Np;
-  const int ke = blockIdx.x * EPB + e;
+  const int ke = xcd_remap(blockIdx.x, gridDim.x) * EPB + e;  // x-neighbours on one XCD's L2
   const bool valid = ke < m.Ne;
   const int i = p % N1, j = (p / N1) % N1, k = p / Nfp;
   const uint32_t o = (uint32_t)Np * (valid ? ke : 0) + p;
@@ -1197,7 +1197,10 @@ __global__ void __launch_bounds__(EPB * N1 * N1 * N1) trcadv_kernel(const DevMes
     for (int t = tid; t < N1 * N1; t += blockDim.x) sDt[(t % N1) * N1 + t / N1] = m.D1D[t];
   const int ke2d = valid ? m.EMap3Dto2D[ke] : 0;
   if (valid) {
-    for (int fp = p; fp < NfpTot; fp += Np) {
+#pragma unroll
+    for (int u = 0; u < NFT; ++u) {
+      const int fp = p + u * Np;
+      if (fp >= NfpTot) break;
       const uint32_t gf = (uint32_t)fp + (uint32_t)NfpTot * ke;
       const uint32_t iM = (uint32_t)m.vmapM[gf], iP = (uint32_t)m.vmapP[gf];
       const int f = fp / Nfp, an = fp - f * Nfp;
@@ -1217,12 +1220,28 @@ __global__ void __launch_bounds__(EPB * N1 * N1 * N1) trcadv_kernel(const DevMes
       const double MF_P = GMX_P * nx + GMY_P * ny + ((GMZ_P / GsqrtV_P + G13_P * GMX_P + G23_P * GMY_P) * nz);
       const double nf = 0.5 * ((Q_P * MF_P + Q_M * MF_M) - ldg(a.aP, gf) * Q_P + ldg(a.aM, gf) * Q_M);
       if constexpr (TEND) {
-        sNF[e][fp] = nf;
-        sQF[e][fp] = Q_M * MF_M;
-        sRP[e][fp] = ldg(a.R, iP);
-        sRM[e][fp] = ldg(a.R, iM);
+        rNF[u] = nf;
+        rQF[u] = Q_M * MF_M;
+        rRP[u] = ldg(a.R, iP);
+        rRM[u] = ldg(a.R, iM);
       }
       sJF[e][fp] = ldg(a.J, iM) * (FLAT ? m.fF[f] : ldg(m.Fscale, gf)) * nf;
+    }
+  }
+  // volume flux, formed before the face phase's barrier so that its loads
+  // overlap the face gathers (sF is not read before the third barrier)
+  double Gs = 1.0, tpo = 0.0;
+  if constexpr (TEND) {
+    if (valid) {
+      Gs = FLAT ? 1.0 : ldg(m.Gsqrt, o);
+      tpo = ldg(a.tp, o);
+      const double RGsqrt = 1.0 / Gs;
+      const double RGsqrtV = (FLAT ? 1.0 : ldg(m.GsqrtH, (uint32_t)(i + N1 * j) + (uint32_t)Nfp * ke2d)) * RGsqrt;
+      const double Qo = ldg(a.Q, o), mx = ldg(a.MX, o), my = ldg(a.MY, o), mz = ldg(a.MZ, o);
+      const double G13 = FLAT ? 0.0 : ldg(m.G13, o), G23 = FLAT ? 0.0 : ldg(m.G23, o);
+      sF[e][0][p] = Gs * mx * Qo;
+      sF[e][1][p] = Gs * my * Qo;
+      sF[e][2][p] = Gs * (mz * RGsqrtV + G13 * mx + G23 * my) * Qo;
     }
   }
   if constexpr (!TEND) {
@@ -1258,25 +1277,15 @@ __global__ void __launch_bounds__(EPB * N1 * N1 * N1) trcadv_kernel(const DevMes
   } else {
     __syncthreads();
     if (valid) {
-      for (int fp = p; fp < NfpTot; fp += Np) {
+#pragma unroll
+      for (int u = 0; u < NFT; ++u) {
+        const int fp = p + u * Np;
+        if (fp >= NfpTot) break;
         const uint32_t gf = (uint32_t)fp + (uint32_t)NfpTot * ke;
-        const double RP = sRP[e][fp], RM = sRM[e][fp];
+        const double RP = rRP[u], RM = rRM[u];
         const double sg = copysign(1.0, sOut[e][fp / Nfp]);
-        sDF[e][fp] = (FLAT ? m.fF[fp / Nfp] : ldg(m.Fscale, gf)) *
-                     (sNF[e][fp] * 0.5 * (RP + RM - (RP - RM) * sg) - sQF[e][fp]);
+        sDF[e][fp] = (FLAT ? m.fF[fp / Nfp] : ldg(m.Fscale, gf)) * (rNF[u] * 0.5 * (RP + RM - (RP - RM) * sg) - rQF[u]);
       }
-    }
-    // volume flux
-    double Gs = 1.0;
-    if (valid) {
-      Gs = FLAT ? 1.0 : ldg(m.Gsqrt, o);
-      const double RGsqrt = 1.0 / Gs;
-      const double RGsqrtV = (FLAT ? 1.0 : ldg(m.GsqrtH, (uint32_t)(i + N1 * j) + (uint32_t)Nfp * ke2d)) * RGsqrt;
-      const double Qo = ldg(a.Q, o), mx = ldg(a.MX, o), my = ldg(a.MY, o), mz = ldg(a.MZ, o);
-      const double G13 = FLAT ? 0.0 : ldg(m.G13, o), G23 = FLAT ? 0.0 : ldg(m.G23, o);
-      sF[e][0][p] = Gs * mx * Qo;
-      sF[e][1][p] = Gs * my * Qo;
-      sF[e][2][p] = Gs * (mz * RGsqrtV + G13 * mx + G23 * my) * Qo;
     }
     __syncthreads();
     if (!valid) return;
@@ -1302,7 +1311,7 @@ __global__ void __launch_bounds__(EPB * N1 * N1 * N1) trcadv_kernel(const DevMes
     const double E11 = FLAT ? m.fE[0] : ldg(m.E11, o), E22 = FLAT ? m.fE[1] : ldg(m.E22, o);
     const double E33 = FLAT ? m.fE[2] : ldg(m.E33, o);
     const double X = E11 * dx + E22 * dy + E33 * dz + lift;
-    stg(a.dt, o, -X / Gs + ldg(a.tp, o));
+    stg(a.dt, o, -X / Gs + tpo);
   }
 }
 
