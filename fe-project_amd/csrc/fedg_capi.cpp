@@ -508,7 +508,22 @@ int fedg_ctx_create(const fedg_mesh_desc* d, fedg_ctx** out) {
       for (int dd = 0; dd < 3; ++dd)
         if (d->normal_fn[t + nf * dd] != nref[f][dd]) flat = false;
     }
-    // the box's own face map, and one velocity BC per closed boundary
+    // the box's own face map, node by node (the flat kernels form the own and
+    // the neighbour's face node arithmetically), and one velocity BC per
+    // closed boundary
+    const int n1 = m.n1, n2 = n1 * n1, Nn = n1 - 1;
+    auto face_node_h = [&](int f, int a) {
+      const int a0 = a % n1, a1 = a / n1;
+      switch (f) {
+        case 0: return a0 + n2 * a1;
+        case 1: return Nn + n1 * a0 + n2 * a1;
+        case 2: return a0 + n1 * Nn + n2 * a1;
+        case 3: return n1 * a0 + n2 * a1;
+        case 4: return a;
+        default: return a + n2 * Nn;
+      }
+    };
+    const int opp[6] = {2, 3, 0, 1, 5, 4};
     int bnd_bc[6] = {-1, -1, -1, -1, -1, -1};
     int periodic[3] = {1, 1, 1};
     for (int ke = 0; ke < m.Ne && flat; ++ke) {
@@ -516,6 +531,8 @@ int fedg_ctx_create(const fedg_mesh_desc* d, fedg_ctx** out) {
       for (int fp = 0; fp < NfpTot && flat; ++fp) {
         const size_t t = fp + (size_t)NfpTot * ke;
         const int f = fp / Nfp;
+        const int a = fp - f * Nfp;
+        if (vm[t] != Np * ke + face_node_h(f, a)) flat = false;
         int ni = ei, nj = ej, nk = ek, axis;
         switch (f) {
           case 0: nj -= 1; axis = 1; break;
@@ -527,7 +544,7 @@ int fedg_ctx_create(const fedg_mesh_desc* d, fedg_ctx** out) {
         }
         const bool outside = ni < 0 || ni >= m.NeX || nj < 0 || nj >= m.NeY || nk < 0 || nk >= m.NeZ;
         if (!outside) {
-          if ((size_t)vp[t] >= NpNe || vp[t] / Np != ni + m.NeX * (nj + m.NeY * nk)) flat = false;
+          if (vp[t] != Np * (ni + m.NeX * (nj + m.NeY * nk)) + face_node_h(opp[f], a)) flat = false;
           continue;
         }
         if ((size_t)vp[t] < NpNe) {
@@ -544,7 +561,7 @@ int fedg_ctx_create(const fedg_mesh_desc* d, fedg_ctx** out) {
         } else {
           if (nbc[t] != FEDG_BND_NONE) flat = false;
           const int wi = (ni + m.NeX) % m.NeX, wj = (nj + m.NeY) % m.NeY, wk = (nk + m.NeZ) % m.NeZ;
-          if (src_elem != wi + m.NeX * (wj + m.NeY * wk)) flat = false;
+          if (nbr[t] != Np * (wi + m.NeX * (wj + m.NeY * wk)) + face_node_h(opp[f], a)) flat = false;
         }
       }
     }
