@@ -215,7 +215,10 @@ int fedg_device_name(char* buf, int buflen) {
   FEDG_HIP_CHECK(hipGetDevice(&dev));
   hipDeviceProp_t prop;
   FEDG_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
-  snprintf(buf, (size_t)buflen, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+  if (prop.name[0])
+    snprintf(buf, (size_t)buflen, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+  else  // some runtimes leave the marketing name empty
+    snprintf(buf, (size_t)buflen, "%s, %d CUs", prop.gcnArchName, prop.multiProcessorCount);
   return FEDG_OK;
 }
 
