@@ -1612,6 +1612,8 @@ struct fedg_hevi {
   float* hy;
   int hcap, hcount, hper;
   unsigned hseed;
+  // time-averaged mass fluxes for the tracers (QA > 0), or null
+  double *mflx[3], *alph[2];
 };
 
 int fedg_hevi_destroy(fedg_hevi* st) {
@@ -1791,10 +1793,50 @@ static int hevi_step(fedg_hevi* st, hipEvent_t* ev) {
     if (ev) FEDG_HIP_CHECK(hipEventRecord(ev[2], ctx->stream));
     TRY(launch_hevi_explicit(ctx, in, Te));
     if (ev) FEDG_HIP_CHECK(hipEventRecord(ev[3], ctx->stream));
+    if (st->mflx[0]) {
+      // FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:899-915: weights b_ex(s)
+      // horizontally and b_im(s) vertically (IMEX), is_hevi; the state and
+      // DPRES carry their exchanged halo and the velocity BC here
+      MflxArgs a{};
+      a.FX = st->mflx[0];
+      a.FY = st->mflx[1];
+      a.FZ = st->mflx[2];
+      a.alM = st->alph[0];
+      a.alP = st->alph[1];
+      a.DD = st->q[0];
+      a.MX = st->q[3];
+      a.MY = st->q[4];
+      a.MZ = st->q[2];
+      a.DP = st->DPRES;
+      a.Dh = st->DENS_hyd;
+      a.Ph = st->PRES_hyd;
+      a.rkstage = s;
+      a.hevi = 1;
+      a.twh = t->b_ex[s - 1];
+      a.twv = t->imex ? t->b_im[s - 1] : t->b_ex[s - 1];
+      TRY(launch_save_massflux(ctx, a));
+    }
     for (int v = 0; v < 5; ++v) TRY(tint_advance_one(t, s, st->q[v], v + 1));
     if (ev) ev += 4;
   }
   if (st->mfilter) TRY(launch_modalfilter(ctx, st->q[0], st->q[3], st->q[4], st->q[2], st->q[1], 1));
+  return FEDG_OK;
+}
+
+int fedg_hevi_set_massflux(fedg_hevi* st, double* MFLX_x_tavg, double* MFLX_y_tavg, double* MFLX_z_tavg,
+                           double* alph_dens_M, double* alph_dens_P) {
+  if (!st) return FEDG_ERR_ARG;
+  const bool any = MFLX_x_tavg || MFLX_y_tavg || MFLX_z_tavg || alph_dens_M || alph_dens_P;
+  const bool all = MFLX_x_tavg && MFLX_y_tavg && MFLX_z_tavg && alph_dens_M && alph_dens_P;
+  if (any && !all) {
+    set_error("fedg_hevi_set_massflux: give all five arrays or none");
+    return FEDG_ERR_ARG;
+  }
+  st->mflx[0] = MFLX_x_tavg;
+  st->mflx[1] = MFLX_y_tavg;
+  st->mflx[2] = MFLX_z_tavg;
+  st->alph[0] = alph_dens_M;
+  st->alph[1] = alph_dens_P;
   return FEDG_OK;
 }
 

@@ -141,6 +141,7 @@ def _load():
         "fedg_heve_destroy": (C.c_int, [vp]),
         "fedg_heve_is_flat": (C.c_int, [vp]),
         "fedg_heve_set_massflux": (C.c_int, [vp] * 6),
+        "fedg_hevi_set_massflux": (C.c_int, [vp] * 6),
         "fedg_heve_set_state": (C.c_int, [vp, C.POINTER(vp)]),
         "fedg_heve_get_state": (C.c_int, [vp, C.POINTER(vp)]),
         "fedg_heve_state_dev": (C.c_int, [vp, C.POINTER(vp)]),
@@ -170,7 +171,7 @@ EXPORTED = (
     "fedg_heve_set_precision fedg_hevi_create fedg_hevi_destroy fedg_hevi_set_state fedg_hevi_get_state fedg_hevi_run "
     "fedg_hevi_run_timed fedg_hevi_newton_stats fedg_calc_phyd_hgrad fedg_tint_create fedg_tint_destroy "
     "fedg_tint_info fedg_tint_tend_ex fedg_tint_tend_im fedg_tint_store_var0 fedg_tint_advance "
-    "fedg_tint_store_implicit fedg_heve_create fedg_heve_destroy fedg_heve_is_flat fedg_heve_set_massflux fedg_heve_set_state "
+    "fedg_tint_store_implicit fedg_heve_create fedg_heve_destroy fedg_heve_is_flat fedg_heve_set_massflux fedg_hevi_set_massflux fedg_heve_set_state "
     "fedg_heve_get_state fedg_heve_state_dev fedg_heve_run fedg_heve_run_timed fedg_heve_get_pressure"
 ).split()
 

@@ -422,6 +422,12 @@ class HeviStepper:
         check(LIB.fedg_hevi_newton_stats(self.handle, C.byref(it), C.byref(calls)), "fedg_hevi_newton_stats")
         return it.value, calls.value
 
+    def set_massflux(self, MFLX_x=None, MFLX_y=None, MFLX_z=None, alph_dens_M=None, alph_dens_P=None):
+        """Save the tracer mass fluxes of every stage (save_massflux, HEVI weights); no arguments: stop."""
+        check(LIB.fedg_hevi_set_massflux(self.handle, *[_p(a) if a is not None else None for a in
+                                                         (MFLX_x, MFLX_y, MFLX_z, alph_dens_M, alph_dens_P)]),
+              "fedg_hevi_set_massflux")
+
     def close(self):
         if self.handle:
             LIB.fedg_hevi_destroy(self.handle)

@@ -477,6 +477,11 @@ int fedg_hevi_destroy(fedg_hevi *st);
 /* q: 5 host arrays (Np,Ne) in variable-id order */
 int fedg_hevi_set_state(fedg_hevi *st, const double *const *q);
 int fedg_hevi_get_state(fedg_hevi *st, double *const *q);
+/* The tracer coupling of fedg_heve_set_massflux for the HEVI stepper: the
+ * same per-stage save_massflux with weights b_ex(s) horizontally and b_im(s)
+ * vertically and the HEVI form of the damping coefficients (no |nz| in Gnn). */
+int fedg_hevi_set_massflux(fedg_hevi *st, double *MFLX_x_tavg, double *MFLX_y_tavg, double *MFLX_z_tavg,
+                           double *alph_dens_M, double *alph_dens_P);
 int fedg_hevi_run(fedg_hevi *st, int nsteps);
 /* nsteps steps bracketed by HIP events: total milliseconds and the mean
  * duration per stage of the implicit solve and of the explicit tendency */

@@ -955,6 +955,13 @@ void oracle_rk_store_impl(const oracle_rk_tab *t, int stage, long n, double *q, 
 /* Tracer coupling of the driver (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:899-915):
  * when set, oracle_heve_update calls save_massflux on every stage state. */
 static double *g_mflx[3], *g_alph[2];
+void oracle_save_massflux_hook(const oracle_mesh *m, double *const q[5], const double *DPRES, const double *DENS_hyd,
+                               const double *PRES_hyd, int stage, double twh, double twv, int is_hevi) {
+  if (g_mflx[0])
+    oracle_trcadvect_save_massflux(m, g_mflx[0], g_mflx[1], g_mflx[2], g_alph[0], g_alph[1], q[DENS_VID],
+                                   q[MOMX_VID], q[MOMY_VID], q[MOMZ_VID], DPRES, DENS_hyd, PRES_hyd, stage, twh, twv,
+                                   is_hevi);
+}
 void oracle_heve_set_massflux(double *MFLX_x, double *MFLX_y, double *MFLX_z, double *alph_M, double *alph_P) {
   g_mflx[0] = MFLX_x;
   g_mflx[1] = MFLX_y;
@@ -987,10 +994,7 @@ void oracle_heve_update(const oracle_mesh *m, const oracle_rk_tab *t, int nsteps
                            q[MOMZ_VID], q[RHOT_VID], DPRES, DENS_hyd, PRES_hyd, PRES_hyd_ref, THERM_hyd,
                            CORIOLIS, Rtot, CVtot, CPtot, DPhydDx, DPhydDy);
       /* add_phy_tend adds zero physics tendencies (dry run without physics). */
-      if (g_mflx[0])
-        oracle_trcadvect_save_massflux(m, g_mflx[0], g_mflx[1], g_mflx[2], g_alph[0], g_alph[1], q[DENS_VID],
-                                       q[MOMX_VID], q[MOMY_VID], q[MOMZ_VID], DPRES, DENS_hyd, PRES_hyd, s,
-                                       t->b_ex[s - 1], t->b_ex[s - 1], 0);
+      oracle_save_massflux_hook(m, q, DPRES, DENS_hyd, PRES_hyd, s, t->b_ex[s - 1], t->b_ex[s - 1], 0);
       for (int v = 0; v < 5; ++v)
         oracle_rk_advance(t, s, n, q[v], var0 + v * NpNeA, varTmp + v * NpNeA, tend + v * NpNeA, NULL,
                           5 * NpNeA);

@@ -183,6 +183,9 @@ void oracle_rk_store_impl(const oracle_rk_tab *t, int stage, long n, double *q, 
 /* when set (all five non-NULL), oracle_heve_update saves the tracer mass fluxes
  * of every stage (save_massflux, FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:899-915) */
 void oracle_heve_set_massflux(double *MFLX_x, double *MFLX_y, double *MFLX_z, double *alph_M, double *alph_P);
+/* the save_massflux call of one stage when the coupling is set (both update loops) */
+void oracle_save_massflux_hook(const oracle_mesh *m, double *const q[5], const double *DPRES, const double *DENS_hyd,
+                               const double *PRES_hyd, int stage, double twh, double twv, int is_hevi);
 void oracle_heve_update(const oracle_mesh *m, const oracle_rk_tab *t, int nsteps,
                         double *const q[5], double *PRES, double *DPRES,
                         const double *DENS_hyd, const double *PRES_hyd, const double *PRES_hyd_ref,

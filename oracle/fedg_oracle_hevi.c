@@ -991,6 +991,9 @@ void oracle_hevi_update(const oracle_mesh *m, const oracle_rk_tab *t, int nsteps
                            Te + MOMZ_VID * NpNeA, Te + RHOT_VID * NpNeA, q[DENS_VID], q[MOMX_VID], q[MOMY_VID],
                            q[MOMZ_VID], q[RHOT_VID], DPRES, DENS_hyd, PRES_hyd, PRES_hyd_ref, THERM_hyd, CORIOLIS,
                            Rtot, CVtot, CPtot, DPhydDx, DPhydDy);
+      /* FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:899-915, tracer coupling */
+      oracle_save_massflux_hook(m, q, DPRES, DENS_hyd, PRES_hyd, s, t->b_ex[s - 1],
+                                t->imex ? t->b_im[s - 1] : t->b_ex[s - 1], 1);
       for (int v = 0; v < 5; ++v)
         oracle_rk_advance(t, s, n, q[v], var0 + v * NpNeA, varTmp + v * NpNeA, tend_ex + v * NpNeA,
                           tend_im + v * NpNeA, 5 * NpNeA);

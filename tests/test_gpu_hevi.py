@@ -141,3 +141,41 @@ def test_hevi_steps_match_oracle(p, max_it, rtol):
     assert calls == 3 * 3  # three implicit stages per ARK324 step
     assert (it == calls) if max_it == 1 else (calls <= it <= max_it * calls)
     st.close()
+
+
+def test_hevi_steps_save_massflux_bitwise():
+    """The tracer coupling in the HEVI stepper: per-stage save_massflux with
+    b_ex / b_im weights and the HEVI damping coefficients, against the
+    oracle's HEVI loop bit for bit (state, mass fluxes, coefficients)."""
+    print("save_massflux in the HEVI stepper p=3", flush=True)
+    case = cases.scaled_case(cases.config4(), 4, 2, 4, porder=3)
+    mesh, bs, q = cases.build(case)
+    om = O.OracleMesh(mesh)
+    Dx, Dy = om.calc_phyd_hgrad(bs)
+    dm = fedg.DeviceMesh(mesh)
+    st = fedg.HeviStepper(dm, bs, Dx, Dy, case.dt, scheme=case.scheme, max_it=1, rtol=0.0)
+    nA, nf, n = mesh.Np * mesh.NeA, mesh.NfpTot * mesh.Ne, mesh.Np * mesh.Ne
+    MF = [np.zeros(nA) for _ in range(3)]
+    al = [np.zeros(nf), np.zeros(nf)]
+    gMF = [fedg.DevField(nA) for _ in range(3)]
+    gal = [fedg.DevField(nf), fedg.DevField(nf)]
+    st.set_massflux(*gMF, *gal)
+    st.set_state(q)
+    st.run(2)
+    qq = [a.copy() for a in q]
+    om.heve_set_massflux(MF, al)  # the oracle's coupling hook serves both update loops
+    try:
+        om.hevi_update(qq, bs, Dx, Dy, case.dt, 2, scheme=case.scheme, max_it=1, rtol=0.0)
+    finally:
+        om.heve_set_massflux(None)
+    g = st.get_state()
+    for name, a, b in zip(fedg.VARS, g, qq):
+        assert np.array_equal(a, b[:n]), name
+    for k, (a, b) in enumerate(zip(gMF, MF)):
+        assert np.array_equal(a.download()[:n], b[:n]), f"MFLX {'xyz'[k]}"
+    for k, (a, b) in enumerate(zip(gal, al)):
+        assert np.array_equal(a.download(), b), f"alph_dens {'MP'[k]}"
+    assert np.abs(al[0]).max() > 0.0
+    st.set_massflux()
+    st.close()
+    dm.close()
