@@ -34,7 +34,13 @@
  *    available mass (and moves each element's mass one element downstream
  *    when every limiter is active); positivity; TMAR keeps element masses and
  *    cuts negatives; damping coefficients = sound speed + |normal velocity|
- *    and equal on both sides of every face;
+ *    and equal on both sides of every face; the low-storage tracer RK update
+ *    integrates a constant mass tendency exactly under a moving density, and
+ *    the tracer-only driver loop keeps a uniform tracer, conserves mass,
+ *    stays non-negative with the limiter and converges with the order
+ *    (tests/test_oracle_trcadv_step.py); the tracer filter keeps low modes
+ *    and element masses; the dynamics loop's mass-flux save gives MFLX = MOM
+ *    for a steady wind;
  *  - halo exchange / BC: the reference's mesh/field test semantics
  *    (FElib/test/FE/field_cubedom3d_hexahedral) on integer-valued fields, and
  *    the SLIP condition's closed form (exterior normal momentum = -interior).
