@@ -71,6 +71,26 @@ def _p(f):
     return f.ptr if isinstance(f, (DevField, DevField32)) else f
 
 
+def _massflux_fields(m, fields):
+    """Check the five tracer-coupling fields of set_massflux (all or none) and return them.
+
+    MFLX_x/_y/_z need Np*NeA values, alph_dens_M/_P NfpTot*Ne; the stepper writes them at every
+    stage, so the caller's DevFields must live until the next set_massflux or close: the steppers
+    keep a reference to them."""
+    given = [f is not None for f in fields]
+    if not any(given):
+        return None
+    if not all(given):
+        raise ValueError("set_massflux takes all five fields or none")
+    need = (m.Np * m.NeA,) * 3 + (m.NfpTot * m.Ne,) * 2
+    for f, k, name in zip(fields, need, ("MFLX_x", "MFLX_y", "MFLX_z", "alph_dens_M", "alph_dens_P")):
+        if not isinstance(f, DevField):
+            raise TypeError(f"{name} must be a DevField")
+        if f.n < k:
+            raise ValueError(f"{name} holds {f.n} values, needs {k}")
+    return tuple(fields)
+
+
 def heve_cal_tend(dm: DeviceMesh, out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DPRES, DENS_hyd, PRES_hyd, PRES_hyd_ref,
                   THERM_hyd, CORIOLIS, Rtot, CVtot, CPtot, DPhydDx, DPhydDy):
     """out = (DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt): device fields."""
@@ -294,9 +314,10 @@ class HeveStepper:
     def set_massflux(self, MFLX_x=None, MFLX_y=None, MFLX_z=None, alph_dens_M=None, alph_dens_P=None):
         """Save the time-averaged mass fluxes and face damping coefficients of every stage into these
         device fields (atm_dyn_dgm_trcadvect3d_save_massflux, for the tracer step); no arguments: stop."""
-        check(LIB.fedg_heve_set_massflux(self.handle, *[_p(a) if a is not None else None for a in
-                                                         (MFLX_x, MFLX_y, MFLX_z, alph_dens_M, alph_dens_P)]),
+        fl = _massflux_fields(self.dm.mesh, (MFLX_x, MFLX_y, MFLX_z, alph_dens_M, alph_dens_P))
+        check(LIB.fedg_heve_set_massflux(self.handle, *([_p(a) for a in fl] if fl else [None] * 5)),
               "fedg_heve_set_massflux")
+        self._mflx = fl
 
     def set_precision(self, bits: int):
         """64: the reference's FP64 stage; 32: the FP32 perturbation-form tendency (FP64 state and RK)."""
@@ -336,6 +357,7 @@ class HeveStepper:
         if self.handle:
             LIB.fedg_heve_destroy(self.handle)
             self.handle = None
+        self._mflx = None  # the stepper no longer writes the mass-flux fields
 
     def __del__(self):
         try:
@@ -424,14 +446,16 @@ class HeviStepper:
 
     def set_massflux(self, MFLX_x=None, MFLX_y=None, MFLX_z=None, alph_dens_M=None, alph_dens_P=None):
         """Save the tracer mass fluxes of every stage (save_massflux, HEVI weights); no arguments: stop."""
-        check(LIB.fedg_hevi_set_massflux(self.handle, *[_p(a) if a is not None else None for a in
-                                                         (MFLX_x, MFLX_y, MFLX_z, alph_dens_M, alph_dens_P)]),
+        fl = _massflux_fields(self.dm.mesh, (MFLX_x, MFLX_y, MFLX_z, alph_dens_M, alph_dens_P))
+        check(LIB.fedg_hevi_set_massflux(self.handle, *([_p(a) for a in fl] if fl else [None] * 5)),
               "fedg_hevi_set_massflux")
+        self._mflx = fl
 
     def close(self):
         if self.handle:
             LIB.fedg_hevi_destroy(self.handle)
             self.handle = None
+        self._mflx = None  # the stepper no longer writes the mass-flux fields
 
     def __del__(self):
         try:

@@ -431,7 +431,9 @@ int fedg_heve_is_flat(const fedg_heve *st);
  * b_ex(s), accumulating MFLX_x/y/z_tavg (Np,NeA) and alph_dens_M/_P
  * (NfpTot,Ne) on the device for the tracer step that follows (the first stage
  * of each step resets them).  All five device arrays, or all NULL to stop.
- * FP64 stages only. */
+ * FP64 stages only.  The stepper keeps the pointers and writes through them at
+ * every stage: the arrays must stay allocated until the next call or
+ * fedg_heve_destroy; MFLX_* hold Np*NeA values, alph_dens_* NfpTot*Ne. */
 int fedg_heve_set_massflux(fedg_heve *st, double *MFLX_x_tavg, double *MFLX_y_tavg, double *MFLX_z_tavg,
                            double *alph_dens_M, double *alph_dens_P);
 /* No counterpart in the reference (one global precision switch, Mkinclude:34-35):

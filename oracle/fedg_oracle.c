@@ -8,10 +8,22 @@
  * without -ffp-contract=fast.
  *
  * How each function is pinned (tests/test_oracle_*.py):
- *  - tensor-product operators (dirX/dirY/dirZ, Lift, Div): the reference's own
+ *  - tensor-product operators (dirX/dirY/dirZ, Lift): the reference's own
  *    unit test semantics (FElib/test/FE/element_operation_hexahedral:
- *    derivative of 4x^p+3y^p+2z^p, Div with Escale/Gsqrt closed form, Lift vs
- *    a dense Lift matrix), plus a numpy evaluation of the same contractions;
+ *    derivative of 4x^p+3y^p+2z^p, Lift vs a dense M^-1 E), plus a numpy
+ *    evaluation of the same contractions (tests/test_refelem.py);
+ *  - Div with metrics (tests/test_oracle_general_metric.py): the reference's
+ *    Div test (Escale = (1,2,0.2), division by a non-constant Gsqrt,
+ *    test_element_operation_hexahedral.f90:115-144) carried through all five
+ *    tendencies with Gsqrt = GsqrtH(x,y) GsqrtV(z), G13/G23, Coriolis f(x,y)
+ *    and DPhydDx/Dy against sympy closed forms; a uniform wind preserved over
+ *    Gal-Chen terrain; conservation of all five tendencies with random
+ *    discontinuous metrics; the face flux against the contravariant Rusanov
+ *    flux with random metrics;
+ *  - calc_phyd_hgrad: closed forms of horizontally varying PRES_hyd and
+ *    PRES_hyd_ref with flat and general metrics (scaled Escale), zero for a
+ *    height-only pressure over terrain, and a vanishing periodic integral of
+ *    GsqrtV x gradient with random metrics (the lift of the face jump);
  *  - VFilterPM1: identity on polynomials of degree <= p-1 in z, removes the
  *    degree-p Legendre mode (closed form);
  *  - numflux: consistency (equal states give only the pressure and advective
