@@ -391,6 +391,15 @@ int fedg_ctx_create(const fedg_mesh_desc* d, fedg_ctx** out) {
   if (hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess ||
       ctx->ncu < 1)
     ctx->ncu = 256;
+  {
+    // the no-allocation workspace's address on this (current) device
+    const int rcs = noalloc_scratch_setup(ctx->device);
+    if (rcs != FEDG_OK) {
+      (void)hipStreamDestroy(ctx->stream);
+      delete ctx;
+      return rcs;
+    }
+  }
   const size_t NpNe = (size_t)Np * m.Ne, NpNeA = (size_t)Np * m.NeA, nf = (size_t)NfpTot * m.Ne;
   const size_t n2d = (size_t)Nfp * m.Ne2D;
   int rc = FEDG_OK;

@@ -60,4 +60,8 @@ int hevi_harvest(fedg_ctx* ctx, HeviWork* w, const double* const q[5], const dou
                  const double* const Tex[5], const double scale[15], double impl_fac, double dt, long line0,
                  long stride, int nrows, float* X, float* Y);
 
+// Device address of the no-allocation workspace on `device` (the current
+// device), looked up once per device; called by fedg_ctx_create.
+int noalloc_scratch_setup(int device);
+
 }  // namespace fedg

@@ -25,6 +25,8 @@
 // the reference's serial loops, so results are bit for bit those of the CPU
 // restatement.
 #include <cfloat>
+#include <cmath>
+#include <mutex>
 
 #include "fedg_device.h"
 #include "fedg_hevi.h"
@@ -1063,4 +1065,108 @@ int hevi_harvest(fedg_ctx* ctx, HeviWork* w, const double* const q[5], const dou
   return FEDG_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Workspace of fedg_hevi_cal_vi_noalloc: a file-scope device array in place
+// of hipMalloc, for callers that may not allocate (HIP graph capture, serving
+// loops).  Every device that runs the library holds its own copy until the
+// process ends; every call on a device shares it and finds the previous
+// call's contents (the solve writes each part before reading it: the iterate
+// is copied from var0, the rhs, damping and linearisation coefficients and
+// the partial sums are written by their kernels, so nothing needs clearing).
+constexpr size_t kScratchDoubles = (size_t)8 << 20;  // 64 MiB
+
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex scratch_mu;
+double* scratch_dev[kScratchDevices];  // address of `scratch` on each device, set once
+
+// a part of the workspace rounded up to 32 doubles (256 B alignment)
+size_t part(size_t n) { return (n + 31) & ~(size_t)31; }
+
+// doubles of the workspace a solve on mesh m needs (the HeviWork layout)
+size_t noalloc_need(const DevMesh& m) {
+  const size_t n = (size_t)m.Np * m.Ne;
+  return 5 * part(n) + part((size_t)2 * m.Nfp * m.Ne) + part(3 * n) + part(2 * n) + part(3 * n) + part(256);
+}
+}  // namespace
+
+// Looks up the device address of `scratch` on `device`, which must be the
+// current device; once per device (later calls return at once).
+int noalloc_scratch_setup(int device) {
+  if (device < 0 || device >= kScratchDevices) {
+    set_error("device index beyond the workspace table of fedg_hevi_cal_vi_noalloc (64 devices)");
+    return FEDG_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (scratch_dev[device]) return FEDG_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || ptr == nullptr) {
+    set_error(std::string("hipGetSymbolAddress of the no-allocation workspace: ") + hipGetErrorString(e));
+    return FEDG_ERR_HIP;
+  }
+  scratch_dev[device] = static_cast<double*>(ptr);
+  return FEDG_OK;
+}
+
 }  // namespace fedg
+
+extern "C" int fedg_hevi_cal_vi_noalloc(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, double* MOMY_dt,
+                                        double* MOMZ_dt, double* RHOT_dt, const double* DDENS_,
+                                        const double* MOMX_, const double* MOMY_, const double* MOMZ_,
+                                        const double* DRHOT_, const double* DENS_hyd, const double* PRES_hyd,
+                                        const double* DDENS0_, const double* MOMX0_, const double* MOMY0_,
+                                        const double* MOMZ0_, const double* DRHOT0_, const double* Rtot,
+                                        const double* CVtot, const double* CPtot, double impl_fac, double dt,
+                                        int max_it, double rtol, double* resid, int* iterations) {
+  using namespace fedg;
+  (void)dt;
+  if (!ctx || !DENS_dt || !MOMX_dt || !MOMY_dt || !MOMZ_dt || !RHOT_dt || !DDENS_ || !MOMX_ || !MOMY_ || !MOMZ_ ||
+      !DRHOT_ || !DENS_hyd || !PRES_hyd || !DDENS0_ || !MOMX0_ || !MOMY0_ || !MOMZ0_ || !DRHOT0_ || !Rtot ||
+      !CVtot || !CPtot || max_it < 1 || !std::isfinite(impl_fac))
+    return FEDG_ERR_ARG;
+  const DevMesh& m = ctx->m;
+  if (m.Ne2D <= 0 || m.Ne % m.Ne2D != 0) {
+    set_error("HEVI: the mesh is not a stack of Ne2D-element horizontal layers");
+    return FEDG_ERR_UNSUPPORTED;
+  }
+  const size_t need = noalloc_need(m);
+  if (need > kScratchDoubles) {
+    set_error("fedg_hevi_cal_vi_noalloc: the mesh needs " + std::to_string(need * sizeof(double)) +
+              " bytes of workspace, the fixed workspace holds " + std::to_string(kScratchDoubles * sizeof(double)) +
+              " (64 MiB); use fedg_hevi_cal_vi_newton");
+    return FEDG_ERR_UNSUPPORTED;
+  }
+  double* p = (ctx->device >= 0 && ctx->device < kScratchDevices) ? scratch_dev[ctx->device] : nullptr;
+  if (p == nullptr) {
+    set_error("fedg_hevi_cal_vi_noalloc: the workspace of this context's device was not set up");
+    return FEDG_ERR_HIP;
+  }
+  const size_t n = (size_t)m.Np * m.Ne;
+  HeviWork w{};
+  for (int v = 0; v < 5; ++v) {
+    w.pv[v] = p;
+    p += part(n);
+  }
+  w.alph = p;
+  p += part((size_t)2 * m.Nfp * m.Ne);
+  w.b3 = p;
+  p += part(3 * n);
+  w.buv = p;
+  p += part(2 * n);
+  w.coef = p;
+  p += part(3 * n);
+  w.partial = p;
+  double* tend[5] = {DENS_dt, RHOT_dt, MOMZ_dt, MOMX_dt, MOMY_dt};
+  const double* q[5] = {DDENS_, DRHOT_, MOMZ_, MOMX_, MOMY_};
+  const double* q0[5] = {DDENS0_, DRHOT0_, MOMZ0_, MOMX0_, MOMY0_};
+  int it = 0;
+  const int rc = hevi_cal_vi(ctx, &w, tend, q, q0, DENS_hyd, PRES_hyd, Rtot, CVtot, CPtot, impl_fac, max_it, rtol,
+                             resid, &it, nullptr);
+  if (rc != FEDG_OK) return rc;
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  if (iterations) *iterations = it;
+  return FEDG_OK;
+}

@@ -129,6 +129,8 @@ def _load():
         "fedg_hevi_cal_vi": (C.c_int, [vp] + [vp] * 20 + [C.c_double, C.c_double]),
         "fedg_hevi_cal_vi_newton": (C.c_int, [vp] + [vp] * 20 + [C.c_double, C.c_double, C.c_int, C.c_double,
                                                                  C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+        "fedg_hevi_cal_vi_noalloc": (C.c_int, [vp] + [vp] * 20 + [C.c_double, C.c_double, C.c_int, C.c_double,
+                                                                  C.POINTER(C.c_double), C.POINTER(C.c_int)]),
         "fedg_tint_create": (C.c_int, [vp, C.c_char_p, C.c_double, C.POINTER(vp)]),
         "fedg_tint_destroy": (C.c_int, [vp]),
         "fedg_tint_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -162,7 +164,7 @@ EXPORTED = (
     "fedg_last_error fedg_version fedg_dev_alloc fedg_dev_free fedg_memcpy_h2d fedg_memcpy_d2h "
     "fedg_memset_dev fedg_device_sync fedg_device_name fedg_stage_probe fedg_refelem_tables fedg_box_get_sizes "
     "fedg_box_build fedg_ctx_create fedg_ctx_destroy fedg_ctx_sizes fedg_ctx_is_flat fedg_drhot2pres fedg_halo_exchange "
-    "fedg_apply_bc_prgvars fedg_heve_cal_tend fedg_heve_cal_tend_cco fedg_heve_add_bnd_contrib fedg_trcadvect_cal_tend fedg_trcadvect_calc_fct_coef fedg_trcadvect_tmar fedg_trcadvect_save_massflux fedg_trcadvect_cal_alphdens_advtest fedg_trcvar_rk_advance fedg_tracer_modalfilter_apply fedg_hevi_cal_tend fedg_hevi_cal_vi fedg_hevi_cal_vi_newton "
+    "fedg_apply_bc_prgvars fedg_heve_cal_tend fedg_heve_cal_tend_cco fedg_heve_add_bnd_contrib fedg_trcadvect_cal_tend fedg_trcadvect_calc_fct_coef fedg_trcadvect_tmar fedg_trcadvect_save_massflux fedg_trcadvect_cal_alphdens_advtest fedg_trcvar_rk_advance fedg_tracer_modalfilter_apply fedg_hevi_cal_tend fedg_hevi_cal_vi fedg_hevi_cal_vi_newton fedg_hevi_cal_vi_noalloc "
     "fedg_modalfilter_matrix fedg_ctx_setup_modalfilter fedg_modalfilter_apply fedg_heve_set_modalfilter "
     "fedg_hevi_set_modalfilter "
     "fedg_hevi_cal_vi_guess fedg_mlp_create fedg_mlp_destroy fedg_mlp_param_count fedg_mlp_get_params fedg_mlp_set_params "

@@ -161,11 +161,13 @@ def hevi_cal_tend(dm: DeviceMesh, out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DPRES, DE
 
 
 def hevi_cal_vi(dm: DeviceMesh, out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DENS_hyd, PRES_hyd, DDENS0, MOMX0, MOMY0,
-                MOMZ0, DRHOT0, Rtot, CVtot, CPtot, impl_fac, dt, max_it=None, rtol=0.0):
+                MOMZ0, DRHOT0, Rtot, CVtot, CPtot, impl_fac, dt, max_it=None, rtol=0.0, noalloc=False):
     """Implicit HEVI tendencies into out = (DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt) (device fields).
 
     max_it None: the reference's single Newton iteration (fedg_hevi_cal_vi).
-    Otherwise Newton to rtol (fedg_hevi_cal_vi_newton); returns (iterations, residual norms)."""
+    Otherwise Newton to rtol (fedg_hevi_cal_vi_newton, or with noalloc the
+    variant on the library's fixed workspace, fedg_hevi_cal_vi_noalloc);
+    returns (iterations, residual norms)."""
     args = [_p(a) for a in (*out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DENS_hyd, PRES_hyd, DDENS0, MOMX0, MOMY0, MOMZ0,
                             DRHOT0, Rtot, CVtot, CPtot)]
     if max_it is None:
@@ -173,8 +175,9 @@ def hevi_cal_vi(dm: DeviceMesh, out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DENS_hyd, P
         return 1, None
     resid = (C.c_double * max(int(max_it), 1))()
     it = C.c_int(0)
-    check(LIB.fedg_hevi_cal_vi_newton(dm.handle, *args, float(impl_fac), float(dt), int(max_it), float(rtol), resid,
-                                      C.byref(it)), "fedg_hevi_cal_vi_newton")
+    name = "fedg_hevi_cal_vi_noalloc" if noalloc else "fedg_hevi_cal_vi_newton"
+    check(getattr(LIB, name)(dm.handle, *args, float(impl_fac), float(dt), int(max_it), float(rtol), resid,
+                             C.byref(it)), name)
     n = min(it.value + 1, int(max_it))
     return it.value, np.array(resid[:n])
 

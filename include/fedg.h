@@ -382,6 +382,26 @@ int fedg_hevi_cal_vi_newton(fedg_ctx *ctx, double *DENS_dt, double *MOMX_dt, dou
                             const double *CPtot, double impl_fac, double dt, int max_it, double rtol,
                             double *resid, int *iterations);
 
+/* No counterpart in the reference: fedg_hevi_cal_vi_newton (same arguments,
+ * same results bit for bit) for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, hosts that forbid
+ * hipMalloc).  Its implicit-solve workspace is a fixed 64 MiB device array of
+ * the library instead of a hipMalloc'd one; the call allocates and frees
+ * nothing.  Meshes whose workspace (13 Np Ne + 2 Nfp Ne doubles, each part
+ * rounded to 256 B, plus 2 KiB) exceeds 64 MiB are rejected with
+ * FEDG_ERR_UNSUPPORTED (at p = 7 about 1200 elements).  Every call on one
+ * device shares the workspace: calls that overlap in time on the same device
+ * (several host threads, several streams) race and must be serialised by the
+ * caller. */
+int fedg_hevi_cal_vi_noalloc(fedg_ctx *ctx, double *DENS_dt, double *MOMX_dt, double *MOMY_dt,
+                             double *MOMZ_dt, double *RHOT_dt, const double *DDENS_, const double *MOMX_,
+                             const double *MOMY_, const double *MOMZ_, const double *DRHOT_,
+                             const double *DENS_hyd, const double *PRES_hyd, const double *DDENS0_,
+                             const double *MOMX0_, const double *MOMY0_, const double *MOMZ0_,
+                             const double *DRHOT0_, const double *Rtot, const double *CVtot,
+                             const double *CPtot, double impl_fac, double dt, int max_it, double rtol,
+                             double *resid, int *iterations);
+
 /* atm_dyn_dgm_nonhydro3d_common_calc_phyd_hgrad_lc
  * (FDS/scale_atm_dyn_dgm_nonhydro3d_common.F90:624-777), halos of PRES_hyd and
  * PRES_hyd_ref filled. */
