@@ -15,8 +15,9 @@ the timed steps.  traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE /
 WRITE_SIZE (profiles/heve_stage_flat_p7_64x64x16.md), scaled by a copy kernel.
 
 cpu_baseline: the CPU oracle (C restatement of the reference Fortran, OpenMP,
-FP64, the C library's pow) timed on this host over one RK4 step of the same
-mesh.
+FP64, the C library's pow) timed on this host over whole RK4 steps of the
+same mesh (at least 10 s), with the CPU model, nproc, the affinity mask and
+the thread pinning recorded.
 
 forms: the same step with the FP32 perturbation-form tendency (FP64 state and
 RK), and the HEVI step of the 240x16x8 p=7 inertia-gravity-wave channel
@@ -56,21 +57,52 @@ def read_traffic():
     return None
 
 
-def cpu_baseline(mesh, bs, q, dt, threads):
+def host_info():
+    """CPU model, logical CPUs of the host and of this process's affinity mask."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    return model, os.cpu_count(), affinity
+
+
+def cpu_baseline(mesh, bs, q, dt, threads, min_seconds=10.0):
+    """The CPU oracle (OpenMP, FP64, the C library's pow) over RK4 steps of the
+    same mesh until at least min_seconds of wall time; threads pinned one per
+    core (OMP_PROC_BIND=close, OMP_PLACES=cores unless the environment sets them)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     import fedg_oracle as O
     O.set_pow_mode(O.POW_LIBM)
     om = O.OracleMesh(mesh)
     Dx, Dy = om.calc_phyd_hgrad(bs)
     qq = [a.copy() for a in q]
+    steps = 0
     t0 = time.perf_counter()
-    om.heve_update(qq, bs, Dx, Dy, dt, 1)
-    sec = time.perf_counter() - t0
-    dof_stage = 5 * mesh.Np * mesh.Ne * 4
+    while True:
+        om.heve_update(qq, bs, Dx, Dy, dt, 1)
+        steps += 1
+        sec = time.perf_counter() - t0
+        if sec >= min_seconds:
+            break
+    dof_stage = 5 * mesh.Np * mesh.Ne * 4 * steps
+    model, ncpu, affinity = host_info()
     return {"value": dof_stage / sec, "unit": "DOF·stage/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"1 RK4 step (4 stages) of the same 64x64x16 p=7 mesh, {dof_stage:.3e} DOF·stage, "
-                      f"{sec:.2f} s"}
+            "sample": f"{steps} RK4 steps ({4 * steps} stages) of the same 64x64x16 p=7 mesh, "
+                      f"{dof_stage:.3e} DOF·stage, {sec:.2f} s",
+            "cpu_model": model, "nproc": ncpu, "affinity_cpus": affinity,
+            "pinning": f"OMP_PROC_BIND={os.environ['OMP_PROC_BIND']} OMP_PLACES={os.environ['OMP_PLACES']}"}
 
 
 def time_heve(fedg, cases, bits, steps, warmup):
@@ -273,7 +305,11 @@ def main():
             "tracer_advection_64x64x16_p7": time_trcadv(fedg, mesh, bs, q, dm, 10),
         }
     if not args.no_cpu_baseline:
-        threads = int(os.environ.get("OMP_NUM_THREADS", str(min(os.cpu_count() or 1, 16))))
+        # the host cores this GPU may use: OMP_NUM_THREADS when the environment
+        # sets it (the GPU hosts here give each GPU a 16-core share, whatever
+        # nproc shows), else the process's affinity mask
+        ncore = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        threads = int(os.environ.get("OMP_NUM_THREADS", str(ncore)))
         out["cpu_baseline"] = cpu_baseline(mesh, bs, q, case.dt, threads)
     else:
         out["cpu_baseline"] = None
