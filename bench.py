@@ -22,7 +22,11 @@ the thread pinning recorded.
 forms: the same step with the FP32 perturbation-form tendency (FP64 state and
 RK), and the HEVI step of the 240x16x8 p=7 inertia-gravity-wave channel
 (IMEX_ARK324, dt = 0.05 s) with the reference's single Newton iteration and
-with Newton to a relative residual of 1e-10.
+with Newton to a relative residual of 1e-10; and the metric's second form,
+the FP32-tendency HEVI step with the learned Newton guess (trained on the GPU
+first) against plain FP64 at the same Newton tolerance, on the channel and on
+its wide-element variant (5 km elements, dt = 0.2 s), with the errors against
+the FP64 run.
 
 forms also times one tracer's advection tendency and FCT coefficient
 (atm_dyn_dgm_trcadvect3d_heve) on the same 64x64x16 p=7 mesh.
@@ -143,6 +147,92 @@ def time_hevi(fedg, cases, steps, warmup, max_it, rtol, case=None, dm=None):
     return {"ms_per_step": wall * 1e3 / steps, "value": dof * 4 * steps / wall,
             "ms_implicit_per_stage": im, "ms_explicit_per_stage": ex,
             "newton_iterations_per_solve": (it - it0) / max(calls - calls0, 1), "max_it": max_it, "rtol": rtol}, dm
+
+
+def time_hevi_learned(fedg, cases, steps, dx=1.25e3, dt=None):
+    """The metric's second form on the 240x16x8 p=7 channel: the FP32-tendency
+    HEVI step with the learned Newton guess, against plain FP64, both with
+    Newton to 1e-10 from the same state over the same steps.  The guess network
+    is trained first on the GPU (harvest from 8x2x8-element pieces of the
+    channel with the same element size and step, seeded wave amplitudes,
+    centres and winds; 1500 Adam steps), as tests/test_gpu_learned_guess.py
+    does.  dx: horizontal element size (1.25 km: the channel; 5 km: wide
+    elements, dt 0.2 s)."""
+    import numpy as np
+    from fedg import learned
+    rtol, max_it = 1e-10, 8
+    base = cases.config4()
+    piece = cases.scaled_case(base, 8, 2, 8)
+    piece.Lx, piece.Ly = 8 * dx, 2 * dx
+    if dt is not None:
+        piece.dt = dt
+    mesh_p, bs_p, _ = cases.build(piece)
+    dm_p = fedg.DeviceMesh(mesh_p)
+    Dx_p, Dy_p = fedg.calc_phyd_hgrad(dm_p, bs_p.PRES_hyd, bs_p.PRES_hyd_ref)
+    states = []
+    for sd in (11, 12, 13, 14):
+        rng = np.random.default_rng(sd)
+        states.append(cases.igw_state(mesh_p, bs_p, U=float(rng.uniform(10.0, 30.0)),
+                                      dtheta=float(rng.uniform(0.005, 0.05)), xc=float(rng.uniform(0.2, 0.8) * piece.Lx)))
+    t0 = time.perf_counter()
+    X, Y = learned.harvest(dm_p, bs_p, Dx_p, Dy_p, piece.dt, piece.scheme, states, nsteps=6, per_stage=256,
+                           rtol=rtol, warm=2)
+    L = learned.line_length(mesh_p)
+    scale = learned.scales_of(X, Y, L)
+    Xn, Yn = learned.normalise(X, Y, scale, L)
+    net = fedg.Mlp(10 * L + 1, 256, 3, 5 * L, seed=5)
+    losses = learned.train(net, Xn, Yn, steps=1500, batch=512, lr=1e-3)
+    t_train = time.perf_counter() - t0
+    dm_p.close()
+
+    case = cases.config4(dt=dt)
+    case.Lx, case.Ly = 240 * dx, 16 * dx
+    mesh, bs, _ = cases.build(case)
+    q = cases.igw_state(mesh, bs, U=20.0, xc=case.Lx / 3.0)
+    dm = fedg.DeviceMesh(mesh)
+    Dx, Dy = fedg.calc_phyd_hgrad(dm, bs.PRES_hyd, bs.PRES_hyd_ref)
+    runs = {}
+    for label, bits, guess in (("fp64_plain", 64, False), ("fp64_learned", 64, True), ("fp32_learned", 32, True)):
+        st = fedg.HeviStepper(dm, bs, Dx, Dy, case.dt, scheme=case.scheme, max_it=max_it, rtol=rtol)
+        st.set_precision(bits)
+        st.set_state(q)
+        if guess:
+            st.set_guess(net, scale)
+        st.run(1)
+        it0, c0 = st.newton_stats()
+        fedg.check(fedg.LIB.fedg_device_sync(), "sync")
+        ta = time.perf_counter()
+        st.run(steps)
+        wall = time.perf_counter() - ta
+        it, c = st.newton_stats()
+        lines, fb = st.guess_stats()
+        rp, rs = st.guess_residuals()
+        runs[label] = dict(state=st.get_state(), ms=wall * 1e3 / steps, it=(it - it0) / max(c - c0, 1),
+                           fallback=fb / max(lines, 1), resid=(rs / rp if rp > 0 else None))
+        st.close()
+    dm.close()
+    net.close()
+    ref = runs["fp64_plain"]["state"]
+    # the full state's scale of every prognostic (MOMY of the y-uniform channel
+    # is roundoff in both runs, so its error relative to itself is noise)
+    n = mesh.Np * mesh.Ne
+    full = [float(np.linalg.norm(ref[0] + bs.DENS_hyd[:n])), float(np.linalg.norm(ref[1] + bs.THERM_hyd[:n]))]
+    full += [float(np.linalg.norm(ref[3]))] * 3
+    out = {"workload": f"240x16x8 p=7 channel, {dx / 1e3:g} km elements, dt={case.dt} s, U=20 m/s, "
+                       f"IMEX_ARK324, Newton to {rtol}", "steps": steps,
+           "guess_training": {"rows": int(X.shape[0]), "adam_steps": 1500, "seconds": t_train,
+                              "loss_first_last": [float(np.mean(losses[:20])), float(np.mean(losses[-20:]))]}}
+    for label, r in runs.items():
+        e = {}
+        for name, a, b, fs in zip(fedg.VARS, r["state"], ref, full):
+            nb = float(np.linalg.norm(b))
+            e[name] = {"rel_l2": float(np.linalg.norm(a - b)) / nb if nb > 0 else None,
+                       "rel_linf": float(np.max(np.abs(a - b)) / np.max(np.abs(b))) if nb > 0 else None,
+                       "rel_l2_full_state": float(np.linalg.norm(a - b)) / fs}
+        out[label] = {"ms_per_step": r["ms"], "newton_iterations_per_solve": r["it"], "fallback_rate": r["fallback"],
+                      "start_residual_over_plain": r["resid"], "error_vs_fp64_plain": e}
+    out["speedup_fp32_learned_over_fp64_plain"] = runs["fp64_plain"]["ms"] / runs["fp32_learned"]["ms"]
+    return out
 
 
 # tracer advection tendency, per node at p=7.  General metrics: 13 node fields
@@ -302,6 +392,8 @@ def main():
             "heve_fp32": {"ms_per_step": f32["wall"] * 1e3 / args.steps, "value": dof * 4 * args.steps / f32["wall"],
                           "ms_stage_kernel": f32["ms_kernel"]},
             "hevi_240x16x8_p7": {"one_iteration": hevi1, "newton_1e-10": hevin},
+            "hevi_fp32_learned": time_hevi_learned(fedg, cases, 3),
+            "hevi_fp32_learned_wide_elements": time_hevi_learned(fedg, cases, 3, dx=5.0e3, dt=0.2),
             "tracer_advection_64x64x16_p7": time_trcadv(fedg, mesh, bs, q, dm, 10),
         }
     if not args.no_cpu_baseline:

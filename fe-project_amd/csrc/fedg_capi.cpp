@@ -96,6 +96,29 @@ struct fedg_heve {
 
 namespace {
 
+// Horizontally homogeneous base state (every element of a level holds the
+// same values, bit for bit): the p = 7 flat-box kernels read DENS_hyd,
+// PRES_hyd, THERM_hyd, DPhydDx, DPhydDy (host arrays f) from per-level
+// (5, NeZ, Np) tables, which stay in L2, instead of five full fields.
+// *out stays null when the base state varies horizontally.
+int level_tables(const DevMesh& m, const double* const f[5], double** out) {
+  *out = nullptr;
+  if (!(m.NeX > 0 && m.NeY > 0 && m.NeZ > 0 && m.Ne == m.NeX * m.NeY * m.NeZ)) return FEDG_OK;
+  const size_t Np = (size_t)m.Np, per = (size_t)m.NeX * m.NeY;
+  bool hom = true;
+  for (int a = 0; a < 5 && hom; ++a)
+    for (size_t ke = 0; ke < (size_t)m.Ne && hom; ++ke) {
+      const size_t rep = (ke / per) * per;
+      hom = memcmp(f[a] + ke * Np, f[a] + rep * Np, Np * sizeof(double)) == 0;
+    }
+  if (!hom) return FEDG_OK;
+  std::vector<double> tab((size_t)5 * m.NeZ * Np);
+  for (int a = 0; a < 5; ++a)
+    for (int kz = 0; kz < m.NeZ; ++kz)
+      memcpy(&tab[((size_t)a * m.NeZ + kz) * Np], f[a] + (size_t)kz * per * Np, Np * sizeof(double));
+  return dev_upload<double>(out, tab.data(), tab.size());
+}
+
 int butcher(const char* name, double dt, fedg_tint* t) {
   memset(t->a_ex, 0, sizeof(t->a_ex));
   memset(t->a_im, 0, sizeof(t->a_im));
@@ -1323,28 +1346,13 @@ int fedg_heve_create(fedg_ctx* ctx, const char* scheme, double dt, const double*
       return FEDG_ERR_HIP;
     }
   }
-  // horizontally homogeneous base state (every element of a level holds the
-  // same values, bit for bit): the flat p = 7 stage reads it from per-level
-  // tables, which stay in L2, instead of five full fields
-  if (st->flat && m.n1 == 8 && m.NeX > 0 && m.NeY > 0 && m.NeZ > 0 && m.Ne == m.NeX * m.NeY * m.NeZ) {
+  // horizontally homogeneous base state: per-level tables for the flat p = 7 stage
+  if (st->flat && m.n1 == 8) {
     const double* f[5] = {DENS_hyd, PRES_hyd, THERM_hyd, DPhydDx, DPhydDy};
-    const size_t Np = (size_t)m.Np, per = (size_t)m.NeX * m.NeY;
-    bool hom = true;
-    for (int a = 0; a < 5 && hom; ++a)
-      for (size_t ke = 0; ke < (size_t)m.Ne && hom; ++ke) {
-        const size_t rep = (ke / per) * per;
-        hom = memcmp(f[a] + ke * Np, f[a] + rep * Np, Np * sizeof(double)) == 0;
-      }
-    if (hom) {
-      std::vector<double> tab((size_t)5 * m.NeZ * Np);
-      for (int a = 0; a < 5; ++a)
-        for (int kz = 0; kz < m.NeZ; ++kz)
-          memcpy(&tab[((size_t)a * m.NeZ + kz) * Np], f[a] + (size_t)kz * per * Np, Np * sizeof(double));
-      rc = dev_upload<double>(&st->lvl, tab.data(), tab.size());
-      if (rc != FEDG_OK) {
-        fedg_heve_destroy(st);
-        return rc;
-      }
+    rc = level_tables(m, f, &st->lvl);
+    if (rc != FEDG_OK) {
+      fedg_heve_destroy(st);
+      return rc;
     }
   }
   st->q0 = 0;
@@ -1623,12 +1631,19 @@ struct fedg_hevi {
   unsigned hseed;
   // time-averaged mass fluxes for the tracers (QA > 0), or null
   double *mflx[3], *alph[2];
+  // flat-terrain box at p = 7 with dry air: the explicit tendency on the p = 7
+  // layout (base state from per-level tables when homogeneous), in FP64 or,
+  // with fp32, in the FP32 perturbation form
+  int flat7;
+  double* lvl;
+  int fp32;
 };
 
 int fedg_hevi_destroy(fedg_hevi* st) {
   if (!st) return FEDG_OK;
   if (st->tint) fedg_tint_destroy(st->tint);
   if (st->block) (void)hipFree(st->block);
+  if (st->lvl) (void)hipFree(st->lvl);
   delete st;
   return FEDG_OK;
 }
@@ -1683,7 +1698,29 @@ int fedg_hevi_create(fedg_ctx* ctx, const char* scheme, double dt, const double*
     fedg_hevi_destroy(st);
     return FEDG_ERR_HIP;
   }
+  const size_t NpNe = (size_t)m.Np * m.Ne;
+  const bool dry = all_equal(Rtot, NpNe, ctx->c.Rdry) && all_equal(CVtot, NpNe, ctx->c.CVdry) &&
+                   all_equal(CPtot, NpNe, ctx->c.CPdry);
+  st->flat7 = m.flat && m.n1 == 8 && m.lift_tensor && dry;
+  if (st->flat7) {
+    const double* f[5] = {DENS_hyd, PRES_hyd, THERM_hyd, DPhydDx, DPhydDy};
+    rc = level_tables(m, f, &st->lvl);
+    if (rc != FEDG_OK) {
+      fedg_hevi_destroy(st);
+      return rc;
+    }
+  }
   *out = st;
+  return FEDG_OK;
+}
+
+int fedg_hevi_set_precision(fedg_hevi* st, int bits) {
+  if (!st || (bits != 32 && bits != 64)) return FEDG_ERR_ARG;
+  if (bits == 32 && !st->flat7) {
+    set_error("the FP32 explicit HEVI tendency is built for the flat-terrain box at p = 7 with dry air");
+    return FEDG_ERR_UNSUPPORTED;
+  }
+  st->fp32 = bits == 32;
   return FEDG_OK;
 }
 
@@ -1800,7 +1837,13 @@ static int hevi_step(fedg_hevi* st, hipEvent_t* ev) {
     in.CVtot = st->CVtot;
     in.CPtot = st->CPtot;
     if (ev) FEDG_HIP_CHECK(hipEventRecord(ev[2], ctx->stream));
-    TRY(launch_hevi_explicit(ctx, in, Te));
+    if (st->flat7) {
+      in.Rtot = in.CVtot = in.CPtot = nullptr;  // dry constants
+      in.lvl = st->lvl;
+      TRY(launch_hevi_explicit7(ctx, in, Te, st->fp32));
+    } else {
+      TRY(launch_hevi_explicit(ctx, in, Te));
+    }
     if (ev) FEDG_HIP_CHECK(hipEventRecord(ev[3], ctx->stream));
     if (st->mflx[0]) {
       // FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:899-915: weights b_ex(s)

@@ -108,11 +108,16 @@ struct RkOut {
   double* varTmp[5];         // fused: RK accumulator
   double* DPout;             // fused: pressure perturbation of the next stage state
   double ca, cb;             // dt*a(s+1,s), dt*b(s)
-  int mode;                  // 0: first stage, 1: middle, 2: last, 3: single-stage scheme
+  int mode;                  // 0: first stage, 1: middle, 2: last, 3: single-stage scheme,
+                             // 4: the tendency itself into out (p = 7 flat-box kernels)
 };
 
 int launch_heve_dropin(fedg_ctx* ctx, const TendIn& in, double* const out[5]);
 int launch_hevi_explicit(fedg_ctx* ctx, const TendIn& in, double* const out[5]);
+// the same tendency on the p = 7 flat-box layout (dry air, in.Rtot null;
+// in.lvl: base state from per-level tables), FP64 (bit for bit the same) or
+// with the tendency in FP32 perturbation form (fp32)
+int launch_hevi_explicit7(fedg_ctx* ctx, const TendIn& in, double* const out[5], int fp32);
 // drop-in tendency split for exchange overlap: interior part, then the face
 // contribution added in place (DPRES: its halo tail is computed first)
 int launch_heve_cco(fedg_ctx* ctx, const TendIn& in, double* const out[5]);

@@ -437,6 +437,31 @@ __device__ __forceinline__ void rusanov32(const Side32& M, const Side32& P, floa
   df[VV] = h * (P.MY * VelP - M.MY * VelM + dp * ny - alpha * (P.MY - M.MY));
 }
 
+// HEVI explicit face flux in FP32 for the flat box (rusanov_hevi with
+// Gsqrt = 1, G13 = G23 = 0): dissipation across lateral faces only (swV =
+// 1 - nz^2), mass and rhotheta carried by the horizontal velocity, no
+// vertical pressure jump.
+__device__ __forceinline__ void rusanov32_hevi(const Side32& M, const Side32& P, float nx, float ny, float nz,
+                                               float Fs, float gamm, float df[5]) {
+  const float DensM = M.DD + M.Dh, DensP = P.DD + P.Dh;
+  const float RhotM = M.Th + M.DT, RhotP = P.Th + P.DT;
+  const float VelhM = (M.MX * nx + M.MY * ny) / DensM;
+  const float VelhP = (P.MX * nx + P.MY * ny) / DensP;
+  const float VelM = VelhM + (M.MZ * nz) / DensM;
+  const float VelP = VelhP + (P.MZ * nz) / DensP;
+  const float swV = 1.f - nz * nz;
+  const float aM = sqrtf(gamm * (M.Ph + M.DP) / DensM) + fabsf(VelM);
+  const float aP = sqrtf(gamm * (P.Ph + P.DP) / DensP) + fabsf(VelP);
+  const float alpha = swV * (aM > aP ? aM : aP);
+  const float h = 0.5f * Fs;
+  df[VD] = h * (DensP * VelhP - DensM * VelhM - alpha * (P.DD - M.DD));
+  df[VT] = h * (RhotP * VelhP - RhotM * VelhM - alpha * (P.DT - M.DT));
+  df[VW] = h * (P.MZ * VelP - M.MZ * VelM - alpha * (P.MZ - M.MZ));
+  const float dp = P.DP - M.DP;
+  df[VU] = h * (P.MX * VelP - M.MX * VelM + dp * nx - alpha * (P.MX - M.MX));
+  df[VV] = h * (P.MY * VelP - M.MY * VelM + dp * ny - alpha * (P.MY - M.MY));
+}
+
 // ---------------------------------------------------------------------------
 // p = 7 fused stage kernel on the flat-terrain uniform box: one element per
 // 256-thread workgroup, thread (i,j,kk) owning the two nodes (i,j,kk) and
@@ -452,8 +477,13 @@ __device__ __forceinline__ void rusanov32(const Side32& M, const Side32& P, floa
 //    no barrier (their interior values are loaded, not exchanged);
 //  - 44 KB of LDS and at most 256 VGPRs without spills: two elements are
 //    resident per CU (a tighter register bound spills and runs 2x slower).
+// HEVI: the explicit tendency of the HEVI scheme (rusanov_hevi; no vertical
+// pressure flux of MOMZ, no vertical flux of DDENS and DRHOT, no buoyancy),
+// FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi.F90:289-482, as heve_kernel's
+// MODE_DROPIN_HEVI computes it.  rk.mode 4: the tendency itself is written to
+// rk.out (no RK update, no DPRES).
 // Arithmetic is identical, operation for operation, to heve_kernel.
-template <typename R, bool LVL>
+template <typename R, bool LVL, bool HEVI = false>
 __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, const TendIn& in, const RkOut& rk) {
   constexpr int N1 = 8, Np = 512, Nfp = 64, NfpTot = 384;
   constexpr bool F64 = std::is_same<R, double>::value;
@@ -611,7 +641,8 @@ __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, co
       P.Dhyd = Pv[5]; P.Phyd = Pv[6]; P.Thyd = Pv[7]; P.DP = Pv[8];
       if (bc != FEDG_BND_NONE)
         reflect_momentum(bc, 1.0, 1.0, 1.0, 0.0, 1.0, 0.0, 0.0, nx, ny, nz, M.MX, M.MY, M.MZ, P.MX, P.MY, P.MZ);
-      rusanov(M, P, nx, ny, nz, m.fF[f], gamm, df);
+      if constexpr (HEVI) rusanov_hevi(M, P, nx, ny, nz, m.fF[f], gamm, df);
+      else rusanov(M, P, nx, ny, nz, m.fF[f], gamm, df);
     } else {
       const float nx = (f == 1) ? 1.f : (f == 3) ? -1.f : 0.f;
       const float ny = (f == 2) ? 1.f : (f == 0) ? -1.f : 0.f;
@@ -629,7 +660,8 @@ __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, co
       } else if (bc == FEDG_BND_NOSLIP) {
         P.MX = -M.MX; P.MY = -M.MY; P.MZ = -M.MZ;
       }
-      rusanov32(M, P, nx, ny, nz, (float)m.fF[f], gamm, df);
+      if constexpr (HEVI) rusanov32_hevi(M, P, nx, ny, nz, (float)m.fF[f], gamm, df);
+      else rusanov32(M, P, nx, ny, nz, (float)m.fF[f], gamm, df);
     }
 #pragma unroll
     for (int v = 0; v < 5; ++v) sDF[v][fp] = df[v];
@@ -684,7 +716,9 @@ __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, co
       f0 = m0 * pt_; f1 = m1 * pt_; f2 = m2 * pt_;
     } else if (v == VW) {
       const R w_ = q[VW] * RD;
-      f0 = m0 * w_; f1 = m1 * w_; f2 = m2 * w_ + Gs * DP * RGsqrtV;
+      f0 = m0 * w_; f1 = m1 * w_;
+      if constexpr (HEVI) f2 = m2 * w_;
+      else f2 = m2 * w_ + Gs * DP * RGsqrtV;
     } else if (v == VU) {
       const R G = Gs * DP;
       const R u_ = q[VU] * RD;
@@ -805,15 +839,22 @@ __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, co
     const R liftb = Ly0 * dfv[i + N1 * (kk + 4)] + Lx1 * dfv[Nfp + j + N1 * (kk + 4)] +
                          Ly2 * dfv[2 * Nfp + i + N1 * (kk + 4)] + Lx3 * dfv[3 * Nfp + j + N1 * (kk + 4)] +
                          Lz4b * fz4 + Lz5b * fz5;
-    const R Xa = E11 * dxa + E22 * dya + E33 * dza + lifta;
-    const R Xb = E11 * dxb + E22 * dyb + E33 * dzb + liftb;
+    // HEVI: DDENS and DRHOT have no vertical flux term, MOMZ no buoyancy
+    const bool horiz = HEVI && (v == VD || v == VT);
+    const R Xa = horiz ? E11 * dxa + E22 * dya + lifta : E11 * dxa + E22 * dya + E33 * dza + lifta;
+    const R Xb = horiz ? E11 * dxb + E22 * dyb + liftb : E11 * dxb + E22 * dyb + E33 * dzb + liftb;
     R Ta, Tb;
     if (v == VD || v == VT) {
       Ta = -Xa * RGsqrt;
       Tb = -Xb * RGsqrt;
     } else if (v == VW) {
-      Ta = -Xa * RGsqrt - GRAV * drhoa;
-      Tb = -Xb * RGsqrt - GRAV * drhob;
+      if constexpr (HEVI) {
+        Ta = -Xa * RGsqrt;
+        Tb = -Xb * RGsqrt;
+      } else {
+        Ta = -Xa * RGsqrt - GRAV * drhoa;
+        Tb = -Xb * RGsqrt - GRAV * drhob;
+      }
     } else if (v == VU) {
       Ta = -sa + cor * ra[VV];
       Ta = Ta - Xa * RGsqrt;
@@ -826,6 +867,12 @@ __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, co
       Tb = Tb - Xb * RGsqrt;
     }
     const double TA = (double)Ta, TB = (double)Tb;
+    if (rk.mode == 4) {  // the tendency itself
+      stg(rk.out[v], o0, TA);
+      stg(rk.out[v], o1, TB);
+      __builtin_amdgcn_sched_barrier(0);
+      continue;
+    }
     // RK stage update (CMN/scale_timeint_rk.F90:2232-2350, tend_buf_size = 1)
     double na, nb;
     switch (rk.mode) {
@@ -896,6 +943,15 @@ __global__ void __launch_bounds__(256, 2) heve7_f32_kernel(const DevMesh m, cons
 __global__ void __launch_bounds__(256, 2) heve7_f32_lvl_kernel(const DevMesh m, const Consts c, const TendIn in,
                                                                const RkOut rk) {
   heve7_body<float, true>(m, c, in, rk);
+}
+
+// The explicit HEVI tendency on the same layout (rk.mode 4): FP64, or FP32 in
+// perturbation form about the FP64 base state; LVL: base state from the
+// per-level tables.
+template <typename R, bool LVL>
+__global__ void __launch_bounds__(256, 2) heve7_hevi_kernel(const DevMesh m, const Consts c, const TendIn in,
+                                                            const RkOut rk) {
+  heve7_body<R, LVL, true>(m, c, in, rk);
 }
 
 template <int N1, int EPB>
@@ -1922,6 +1978,31 @@ int launch_heve_fused(fedg_ctx* ctx, const TendIn& in, const RkOut& rk, int use_
   }
   if (use_flat) return launch_heve_mode<MODE_FUSED_FLAT>(ctx, in, rk);
   return launch_heve_mode<MODE_FUSED_GENERAL>(ctx, in, rk);
+}
+
+int launch_hevi_explicit7(fedg_ctx* ctx, const TendIn& in, double* const out[5], int fp32) {
+  const DevMesh& m = ctx->m;
+  if (!(m.flat && m.n1 == 8 && m.lift_tensor) || in.Rtot != nullptr) {
+    set_error("the p = 7 explicit HEVI tendency is built for the flat-terrain box with dry air");
+    return FEDG_ERR_UNSUPPORTED;
+  }
+  RkOut rk{};
+  for (int v = 0; v < 5; ++v) rk.out[v] = out[v];
+  rk.mode = 4;
+  if (fp32) {
+    if (in.lvl)
+      hipLaunchKernelGGL((heve7_hevi_kernel<float, true>), dim3(m.Ne), dim3(256), 0, ctx->stream, m, ctx->c, in, rk);
+    else
+      hipLaunchKernelGGL((heve7_hevi_kernel<float, false>), dim3(m.Ne), dim3(256), 0, ctx->stream, m, ctx->c, in, rk);
+  } else {
+    if (in.lvl)
+      hipLaunchKernelGGL((heve7_hevi_kernel<double, true>), dim3(m.Ne), dim3(256), 0, ctx->stream, m, ctx->c, in, rk);
+    else
+      hipLaunchKernelGGL((heve7_hevi_kernel<double, false>), dim3(m.Ne), dim3(256), 0, ctx->stream, m, ctx->c, in,
+                         rk);
+  }
+  FEDG_HIP_CHECK(hipGetLastError());
+  return FEDG_OK;
 }
 
 int launch_drhot2pres(fedg_ctx* ctx, double* PRES, double* DPRES, const double* DRHOT, const double* PRES_hyd,

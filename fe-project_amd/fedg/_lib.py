@@ -96,6 +96,7 @@ def _load():
         "fedg_calc_phyd_hgrad": (C.c_int, [vp] + [vp] * 4),
         "fedg_hevi_cal_tend": (C.c_int, [vp] + [vp] * 21),
         "fedg_heve_set_precision": (C.c_int, [vp, C.c_int]),
+        "fedg_hevi_set_precision": (C.c_int, [vp, C.c_int]),
         "fedg_modalfilter_matrix": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_int, vp]),
         "fedg_ctx_setup_modalfilter": (C.c_int, [vp, vp, vp]),
         "fedg_modalfilter_apply": (C.c_int, [vp, vp, vp, vp, vp, vp, C.c_int]),
@@ -170,7 +171,7 @@ EXPORTED = (
     "fedg_hevi_cal_vi_guess fedg_mlp_create fedg_mlp_destroy fedg_mlp_param_count fedg_mlp_get_params fedg_mlp_set_params "
     "fedg_mlp_forward fedg_mlp_loss_grad fedg_mlp_adam_step fedg_hevi_set_guess fedg_hevi_guess_stats "
     "fedg_hevi_harvest fedg_hevi_harvest_count fedg_hevi_guess_residuals "
-    "fedg_heve_set_precision fedg_hevi_create fedg_hevi_destroy fedg_hevi_set_state fedg_hevi_get_state fedg_hevi_run "
+    "fedg_heve_set_precision fedg_hevi_set_precision fedg_hevi_create fedg_hevi_destroy fedg_hevi_set_state fedg_hevi_get_state fedg_hevi_run "
     "fedg_hevi_run_timed fedg_hevi_newton_stats fedg_calc_phyd_hgrad fedg_tint_create fedg_tint_destroy "
     "fedg_tint_info fedg_tint_tend_ex fedg_tint_tend_im fedg_tint_store_var0 fedg_tint_advance "
     "fedg_tint_store_implicit fedg_heve_create fedg_heve_destroy fedg_heve_is_flat fedg_heve_set_massflux fedg_hevi_set_massflux fedg_heve_set_state "

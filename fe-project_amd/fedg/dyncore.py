@@ -414,6 +414,11 @@ class HeviStepper:
         """MODALFILTER_FLAG: filter after every step (DeviceMesh.setup_modalfilter first)."""
         check(LIB.fedg_hevi_set_modalfilter(self.handle, int(bool(on))), "fedg_hevi_set_modalfilter")
 
+    def set_precision(self, bits: int):
+        """64: the reference's FP64 explicit tendency; 32: the explicit tendency in FP32
+        perturbation form (flat box, p = 7, dry air; state, IMEX and Newton solve FP64)."""
+        check(LIB.fedg_hevi_set_precision(self.handle, int(bits)), "fedg_hevi_set_precision")
+
     def set_guess(self, mlp, scale):
         """Initial Newton iterate from mlp (None: q^n); scale: 5 state, 5 tendency-feature, 5 increment scales."""
         if len(scale) != 15:

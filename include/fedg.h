@@ -513,6 +513,13 @@ int fedg_hevi_run_timed(fedg_hevi *st, int nsteps, double *ms_total, double *ms_
 int fedg_hevi_newton_stats(const fedg_hevi *st, long *iterations, long *calls);
 /* as fedg_heve_set_modalfilter */
 int fedg_hevi_set_modalfilter(fedg_hevi *st, int on);
+/* No counterpart in the reference: the explicit tendency of the HEVI step on
+ * the flat-terrain box at p = 7 with dry air, evaluated in FP32 perturbation
+ * form about the FP64 base state (bits = 32; the state, the IMEX combination
+ * and the implicit Newton solve stay FP64) or in FP64 (bits = 64, the
+ * default, bit for bit the reference's).  bits = 32 elsewhere:
+ * FEDG_ERR_UNSUPPORTED. */
+int fedg_hevi_set_precision(fedg_hevi *st, int bits);
 
 
 /* ------------------------------------------------------------------ */
