@@ -4,7 +4,7 @@ Python host mirror of the reference's dynamical-core interfaces over the C ABI
 of libfedg.so (include/fedg.h).  Import fails when the native library is not
 built: there is no CPU fallback.
 """
-from . import cases, constants, tracer  # noqa: F401
+from . import cases, constants, restart, tracer  # noqa: F401
 from ._lib import EXPORTED, LIB, LIB_PATH, FedgError, check  # noqa: F401
 from .dyncore import (VARS, DevField, DevField32, HeveStepper, HeviStepper, Mlp, TimeIntRK, apply_bc,  # noqa: F401
                       calc_phyd_hgrad, calc_pressure, device_name, halo_exchange, heve_add_bnd_contrib, heve_cal_tend,
