@@ -32,6 +32,7 @@
 #include "fedg_hevi.h"
 #include "fedg_mlp.h"
 
+
 namespace fedg {
 namespace {
 
@@ -256,7 +257,8 @@ __global__ void __launch_bounds__(N1* N1* N1) hevi_rhs3_kernel(const HeviArgs a)
 // ---------------------------------------------------------------------------
 // Wavefront-cooperative pivoted LU + solve of solve_NnodeN_var3 / _uv
 // (FDS/scale_atm_dyn_dgm_hevi_common_linalgebra.F90:2142-2445) for an n x n
-// block A (column-major in LDS) and nr right-hand sides X (n x nr in LDS).
+// block (rows in the lanes' registers) and NR right-hand sides (solution to
+// LDS).
 // ---------------------------------------------------------------------------
 // Line solves.  A wavefront holds 64/SEG lines, one per segment of SEG lanes;
 // lane r of a segment owns row r of the level's n x n block in registers and
@@ -270,80 +272,103 @@ __global__ void __launch_bounds__(N1* N1* N1) hevi_rhs3_kernel(const HeviArgs a)
 // exact zero, which can only turn a -0 entry into +0).
 
 // Cross-lane exchanges inside a segment: DPP permutations within 16-lane
-// rows, ds_swizzle between the two rows of a 32-lane segment.
+// rows, ds_swizzle between the two rows of a 32-lane segment (every DPP
+// control used here reads a valid lane, so no lane keeps an old value).
 template <int CTRL>
 __device__ __forceinline__ int dpp_i(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
 }
 
-// pivot candidate: larger value wins, equal values go to the smaller key
-// (position * 64 + lane)
-__device__ __forceinline__ void piv_take(double& v, int& k, double ov, int ok) {
-  const bool t = (ov > v) || (ov == v && ok < k);
-  v = t ? ov : v;
-  k = t ? ok : k;
-}
-
+// maximum over the segment of a double that is never NaN
 template <int CTRL>
-__device__ __forceinline__ void piv_dpp(double& v, int& k) {
-  const int lo = dpp_i<CTRL>(__double2loint(v));
-  const int hi = dpp_i<CTRL>(__double2hiint(v));
-  piv_take(v, k, __hiloint2double(hi, lo), dpp_i<CTRL>(k));
+__device__ __forceinline__ double dpp_d(double v) {
+  return __hiloint2double(dpp_i<CTRL>(__double2hiint(v)), dpp_i<CTRL>(__double2loint(v)));
 }
-
-// all-reduce of (value, key) over the segment
+__device__ __forceinline__ double dmax(double a, double b) { return b > a ? b : a; }
 template <int SEG>
-__device__ __forceinline__ void piv_reduce(double& v, int& k) {
-  piv_dpp<0xB1>(v, k);   // quad_perm [1,0,3,2]: partner lane ^ 1
-  piv_dpp<0x4E>(v, k);   // quad_perm [2,3,0,1]: partner lane ^ 2
-  piv_dpp<0x141>(v, k);  // row_half_mirror: the other quad of the 8
-  if constexpr (SEG >= 16) piv_dpp<0x140>(v, k);  // row_mirror: the other 8 of the row
+__device__ __forceinline__ double seg_max(double v) {
+  v = dmax(v, dpp_d<0xB1>(v));
+  v = dmax(v, dpp_d<0x4E>(v));
+  v = dmax(v, dpp_d<0x141>(v));
+  if constexpr (SEG >= 16) v = dmax(v, dpp_d<0x140>(v));
   if constexpr (SEG >= 32) {
-    constexpr int X16 = (0x10 << 10) | 0x1F;  // swizzle bitmask mode: lane ^ 16 within 32
-    const int lo = __builtin_amdgcn_ds_swizzle(__double2loint(v), X16);
-    const int hi = __builtin_amdgcn_ds_swizzle(__double2hiint(v), X16);
-    piv_take(v, k, __hiloint2double(hi, lo), __builtin_amdgcn_ds_swizzle(k, X16));
+    constexpr int X16 = (0x10 << 10) | 0x1F;
+    v = dmax(v, __hiloint2double(__builtin_amdgcn_ds_swizzle(__double2hiint(v), X16),
+                                 __builtin_amdgcn_ds_swizzle(__double2loint(v), X16)));
   }
-  if constexpr (SEG >= 64) piv_take(v, k, __shfl_xor(v, 32, 64), __shfl_xor(k, 32, 64));
+  if constexpr (SEG >= 64) v = dmax(v, __shfl_xor(v, 32, 64));
+  return v;
+}
+template <int SEG>
+__device__ __forceinline__ int seg_min_i(int v) {
+  v = min(v, dpp_i<0xB1>(v));
+  v = min(v, dpp_i<0x4E>(v));
+  v = min(v, dpp_i<0x141>(v));
+  if constexpr (SEG >= 16) v = min(v, dpp_i<0x140>(v));
+  if constexpr (SEG >= 32) v = min(v, __builtin_amdgcn_ds_swizzle(v, (0x10 << 10) | 0x1F));
+  if constexpr (SEG >= 64) v = min(v, __shfl_xor(v, 32, 64));
+  return v;
+}
+// lowest lane of the segment whose predicate is set (the predicate holds for
+// at least one lane of every segment); *several: more than one lane holds it
+template <int SEG>
+__device__ __forceinline__ int seg_first(bool pred, bool* several) {
+  const unsigned long long bal = __ballot(pred);
+  const int sb = threadIdx.x & ~(SEG - 1);
+  if constexpr (SEG == 64) {
+    *several = __popcll(bal) > 1;
+    return __builtin_ctzll(bal);
+  } else {
+    const unsigned s = (unsigned)(bal >> sb) & (unsigned)((1ull << SEG) - 1);
+    *several = __popc(s) > 1;
+    return sb + __builtin_ctz(s);
+  }
 }
 
 // LU with partial pivoting of the segment's n x n block (row r in A of lane
-// r), applied to nr <= NR right-hand sides as they go (x[q] of lane r: entry
-// r of rhs q; the row operations on them are the forward substitution), then
-// back substitution.  The solution goes to X (LDS, column-major n x NR);
-// P2L (LDS, n ints) receives the lane of every row position.  Every lane of
-// the single-wavefront workgroup calls it.
+// r), applied to NR right-hand sides as it goes (x[q] of lane r: entry r of
+// rhs q; the row operations on them are the forward substitution), then back
+// substitution.  Rows are never moved: a row's position (pos) changes when
+// it is swapped with the pivot row, and the pivot row and the entries solved
+// travel by cross-lane permutes or LDS.  Every lane of the single-wavefront
+// workgroup calls it.
+//  - the pivot search reduces |A(i,k)| alone and finds the row by a ballot;
+//    only when several rows hold the maximum does a second reduction pick
+//    the first position among them;
+//  - every lane forms 1 / A(i,k) before the search, and the pivot lane's
+//    reciprocal is the one broadcast;
+//  - all NR right-hand sides are carried (unused ones hold zeros);
+//  - after the elimination each row lane stores its row of U (the reciprocal
+//    of the pivot on the diagonal) in the segment's triangular LDS block U
+//    (row i at i n - i (i - 1) / 2), and the back substitution runs on the NR
+//    right-hand-side lanes alone, reading U from LDS.
+// X (LDS, column-major n x NR) receives the solution.  U holds
+// n (n + 1) / 2 + SEG doubles (the last SEG are write sinks).
 template <int n, int SEG, int NR>
-__device__ __forceinline__ void lu_solve_rows(double (&A)[n], double (&x)[NR], int nr, bool act, int* P2L,
-                                              double* X) {
+__device__ __forceinline__ void lu_solve_tri(double (&A)[n], double (&x)[NR], bool act, double* U, double* X) {
   const int lane = threadIdx.x;
   int pos = act ? (lane % SEG) : (1 << 20);
 #pragma unroll
   for (int k = 0; k < n; ++k) {
-    // pivot: the first position >= k of max |A(i,k)|; a NaN on the diagonal
-    // keeps k, a NaN below it is never chosen (the reference's strict '>')
+    const double inv = 1.0 / A[k];
+    const bool elig = act && pos >= k;
     double av = -2.0;
-    int key = 1 << 30;
-    if (act && pos >= k) {
+    if (elig) {
       const double t = fabs(A[k]);
       av = (t == t) ? t : (pos == k ? __builtin_huge_val() : -1.0);
-      key = pos * 64 + lane;
     }
-    piv_reduce<SEG>(av, key);
-    const int ppos = key >> 6, lp = key & 63;
-    if (act) {
-      if (pos == k) pos = ppos;
-      else if (pos == ppos) pos = k;
+    const double mx = seg_max<SEG>(av);
+    bool several;
+    int lp = seg_first<SEG>(av == mx, &several);
+    if (__any(several)) {
+      const int first = seg_min_i<SEG>(av == mx ? pos : (1 << 30));
+      lp = seg_first<SEG>(av == mx && pos == first, &several);
     }
-    const double dinv = 1.0 / __shfl(A[k], lp, 64);
-    const bool below = act && pos > k;
+    const bool isp = lane == lp;
+    const bool below = elig && !isp;
+    const double dinv = __shfl(inv, lp, 64);
     const double l = A[k] * dinv;
-    // rows that are not below the pivot subtract l0 * p = 0, which leaves
-    // every finite entry unchanged (up to the sign of a zero entry) and saves
-    // the per-entry select
     const double l0 = below ? l : 0.0;
-    // the pivot row's entries one at a time (the pivot lane's row is not
-    // changed in this step)
 #pragma unroll
     for (int j = k + 1; j < n; ++j) {
       const double pj = __shfl(A[j], lp, 64);
@@ -352,32 +377,40 @@ __device__ __forceinline__ void lu_solve_rows(double (&A)[n], double (&x)[NR], i
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
       const double pq = __shfl(x[q], lp, 64);
-      if (q < nr) x[q] = x[q] - pq * l0;
+      x[q] = x[q] - pq * l0;
     }
-    if (lane == lp) A[k] = dinv;
+    const int ppos = __shfl(pos, lp, 64);
+    if (act) {
+      if (pos == k) pos = ppos;
+      else if (isp) pos = k;
+    }
+    if (isp) A[k] = dinv;
     else if (below) A[k] = l;
   }
-  // the forward-reduced right-hand sides to X, the lane of every position
+  // rows of U and the forward-reduced right-hand sides to LDS
+  constexpr int nt = n * (n + 1) / 2;
+  const int sink = nt + lane % SEG;
+  const int base = act ? pos * n - pos * (pos - 1) / 2 - pos : 0;
+#pragma unroll
+  for (int j = 0; j < n; ++j) U[(act && j >= pos) ? base + j : sink] = A[j];
   if (act) {
 #pragma unroll
-    for (int q = 0; q < NR; ++q)
-      if (q < nr) X[pos + n * q] = x[q];
-    P2L[pos] = lane;
+    for (int q = 0; q < NR; ++q) X[pos + n * q] = x[q];
   }
   __syncthreads();
-  // back substitution, one lane per right-hand side, each row's sum in
-  // ascending column order; the factors' rows come from their lanes
+  // back substitution on the right-hand-side lanes, each row's sum in
+  // ascending column order
   const int rq = lane % SEG;
-  const bool chain = rq < nr;
-  double xs[n];
-#pragma clang loop unroll(full)
-  for (int i = n - 1; i >= 0; --i) {
-    const int li = P2L[i];
-    double t = chain ? X[i + n * rq] : 0.0;
-#pragma clang loop unroll(full)
-    for (int j = i + 1; j < n; ++j) t = t - __shfl(A[j], li, 64) * xs[j];
-    xs[i] = t * __shfl(A[i], li, 64);
-    if (chain) X[i + n * rq] = xs[i];
+  if (rq < NR) {
+    double* Xq = X + n * rq;
+#pragma unroll
+    for (int i = n - 1; i >= 0; --i) {
+      const double* Ui = U + (i * n - i * (i - 1) / 2 - i);
+      double t = Xq[i];
+#pragma unroll
+      for (int j = i + 1; j < n; ++j) t = t - Ui[j] * Xq[j];
+      Xq[i] = t * Ui[i];
+    }
   }
   __syncthreads();
 }
@@ -402,8 +435,9 @@ __global__ void __launch_bounds__(64) hevi_solve_uv_kernel(const HeviArgs a) {
   const int nline = m.Ne2D * Nfp;
   const int NeZ = a.NeZ;
   const uint32_t nn = (uint32_t)Np * m.Ne;
-  double* X = lds + (size_t)sg * (N + (size_t)NeZ * 3 * N);  // per level: b(:,1), b(:,2), G
-  int* P2L = reinterpret_cast<int*>(X + (size_t)NeZ * 3 * N);
+  constexpr int nU = N * (N + 1) / 2 + SEG;
+  double* U = lds + (size_t)sg * (nU + (size_t)NeZ * 3 * N);  // the block's U rows (triangular)
+  double* X = U + nU;  // per level: b(:,1), b(:,2), G
   const int pv = act ? r : 0;
   for (int g = blockIdx.x; g * LPW < nline; g += gridDim.x) {
     const int line = g * LPW + sg;
@@ -449,7 +483,8 @@ __global__ void __launch_bounds__(64) hevi_solve_uv_kernel(const HeviArgs a) {
         x[0] = x[0] - lvv * Xp[N - 1];
         x[1] = x[1] - lvv * Xp[(N - 1) + N];
       }
-      lu_solve_rows<N, SEG, 3>(A, x, kz == NeZ - 1 ? 2 : 3, act, P2L, X + (size_t)kz * 3 * N);
+      // at the top level the G column is zero and its solution unused
+      lu_solve_tri<N, SEG, 3>(A, x, act, U, X + (size_t)kz * 3 * N);
     }
     for (int kz = NeZ - 2; kz >= 0; --kz) {
       double* Xk = X + (size_t)kz * 3 * N;
@@ -488,12 +523,13 @@ __global__ void __launch_bounds__(64) hevi_solve3_kernel(const HeviArgs a) {
   const int NeZ = a.NeZ;
   const uint32_t nn = (uint32_t)Np * m.Ne;
   const double impl_fac = a.impl_fac;
+  constexpr int nU = n3 * (n3 + 1) / 2 + SEG;
   double* sD = lds;      // D1D
   double* sI = sD + N * N;  // IntrpMat_VPOrdM1_tr
-  double* cf = sI + N * N + (size_t)sg * (13 * N + n3 + (size_t)NeZ * 4 * n3);  // [3 levels][POT, WT, DPDRHOT][N]
+  double* cf = sI + N * N + (size_t)sg * (13 * N + nU + (size_t)NeZ * 4 * n3);  // [3 levels][POT, WT, DPDRHOT][N]
   double* t12 = cf + 9 * N;  // [face][tmp1, tmp2][N]
-  int* P2L = reinterpret_cast<int*>(t12 + 4 * N);  // lane of each row position
-  double* X = t12 + 4 * N + n3;  // NeZ * 4 * n3: b, G(:,1..3) per level
+  double* U = t12 + 4 * N;   // the block's U rows (triangular)
+  double* X = U + nU;        // NeZ * 4 * n3: b, G(:,1..3) per level
   for (int e = lane; e < N * N; e += 64) {
     sD[e] = m.D1D[e];
     sI[e] = m.Itr[e];
@@ -611,7 +647,8 @@ __global__ void __launch_bounds__(64) hevi_solve3_kernel(const HeviArgs a) {
           A[c] = A[c] - lm[0] * Xp[p1 + n3 * (1 + c)] - lm[1] * Xp[p2 + n3 * (1 + c)] - lm[2] * Xp[p3 + n3 * (1 + c)];
         x[0] = x[0] - lm[0] * Xp[p1] - lm[1] * Xp[p2] - lm[2] * Xp[p3];
       }
-      lu_solve_rows<n3, SEG, 4>(A, x, kz == NeZ - 1 ? 1 : 4, act, P2L, X + (size_t)kz * 4 * n3);
+      // at the top level the G columns are zero and their solution unused
+      lu_solve_tri<n3, SEG, 4>(A, x, act, U, X + (size_t)kz * 4 * n3);
     }
     for (int kz = NeZ - 2; kz >= 0; --kz) {
       double* Xk = X + (size_t)kz * 4 * n3;
@@ -800,13 +837,15 @@ LineArgs line_args(const DevMesh& m, const HeviArgs& a, const double* const Tex[
 
 template <int N1>
 size_t lds_uv(int NeZ) {
-  return sizeof(double) * (size_t)(64 / LineSeg<N1>::SEGU) * (N1 + (size_t)NeZ * 3 * N1);
+  constexpr int nU = N1 * (N1 + 1) / 2 + LineSeg<N1>::SEGU;
+  return sizeof(double) * (size_t)(64 / LineSeg<N1>::SEGU) * (nU + (size_t)NeZ * 3 * N1);
 }
 template <int N1>
 size_t lds_3(int NeZ) {
   constexpr int n3 = 3 * N1;
+  constexpr int nU = n3 * (n3 + 1) / 2 + LineSeg<N1>::SEG3;
   return sizeof(double) *
-         (2 * N1 * N1 + (size_t)(64 / LineSeg<N1>::SEG3) * (13 * N1 + n3 + (size_t)NeZ * 4 * n3));
+         (2 * N1 * N1 + (size_t)(64 / LineSeg<N1>::SEG3) * (13 * N1 + nU + (size_t)NeZ * 4 * n3));
 }
 
 template <int N1>
