@@ -1251,6 +1251,40 @@ static int tint_advance_one(fedg_tint* t, int s, double* q, int varID) {
   return FEDG_OK;
 }
 
+// tint_advance_one of an IMEX scheme for the five prognostics (VID order) in
+// one pass (the same operations per node; launch_imex_advance)
+static int imex_advance_all(fedg_tint* t, int s, double* const q[5]) {
+  if (!t->imex || t->nstage < 2 || s > kImexMaxStages) {
+    for (int v = 0; v < 5; ++v) TRY(tint_advance_one(t, s, q[v], v + 1));
+    return FEDG_OK;
+  }
+  const size_t NpNeA = (size_t)t->ctx->m.Np * t->ctx->m.NeA;
+  const int ind = t->indmap[s - 1];
+  ImexAdvance a{};
+  for (int v = 0; v < 5; ++v) {
+    a.q[v] = q[v];
+    a.vt[v] = t->varTmp + (size_t)v * NpNeA;
+    a.v0[v] = t->var0 + (size_t)v * NpNeA;
+    a.Te[v] = tend_ptr(t, t->tend_ex, ind, v + 1);
+    a.Ti[v] = tend_ptr(t, t->tend_im, ind, v + 1);
+  }
+  a.cbe = t->b_ex[s - 1] * t->dt;
+  a.cbi = t->b_im[s - 1] * t->dt;
+  a.s = s;
+  a.last = s == t->nstage;
+  if (!a.last) {
+    for (int s2 = 1; s2 <= s; ++s2) {
+      a.ae[s2 - 1] = t->dt * t->a_ex[s][s2 - 1];
+      a.ai[s2 - 1] = t->dt * t->a_im[s][s2 - 1];
+      for (int v = 0; v < 5; ++v) {
+        a.Tes[s2 - 1][v] = tend_ptr(t, t->tend_ex, s2 - 1, v + 1);
+        a.Tis[s2 - 1][v] = tend_ptr(t, t->tend_im, s2 - 1, v + 1);
+      }
+    }
+  }
+  return launch_imex_advance(t->ctx, a);
+}
+
 int fedg_tint_advance(fedg_tint* t, int stage, int nvar, double* const* q, const int* varIDs) {
   if (!t || stage < 1 || stage > t->nstage || nvar < 0 || (nvar > 0 && (!q || !varIDs))) return FEDG_ERR_ARG;
   for (int v = 0; v < nvar; ++v) {
@@ -1789,9 +1823,18 @@ static int hevi_step(fedg_hevi* st, hipEvent_t* ev) {
     for (int v = 0; v < 5; ++v) g.Tex[v] = Tprev[v];
     for (int v = 0; v < 15; ++v) g.scale[v] = st->scale[v];
     g.dt = t->dt;
+    // StoreImplicit fused into the pass that forms the implicit tendency
+    // (not while harvesting: the harvest reads the stage state after the solve)
+    const bool fuse_store = fabs(impl_fac) > 0.0 && !st->hx;
+    HeviStore sto{};
+    for (int v = 0; v < 5; ++v) {
+      sto.q[v] = st->q[v];
+      sto.var0[v] = s == 1 ? t->var0 + v * nA : nullptr;
+      sto.varTmp[v] = s == 1 ? t->varTmp + v * nA : nullptr;
+    }
     if (ev) FEDG_HIP_CHECK(hipEventRecord(ev[0], ctx->stream));
     TRY(hevi_cal_vi(ctx, w, Ti, qv, v0, st->DENS_hyd, st->PRES_hyd, st->Rtot, st->CVtot, st->CPtot, impl_fac,
-                    st->max_it, st->rtol, nullptr, &it, st->mlp ? &g : nullptr));
+                    st->max_it, st->rtol, nullptr, &it, st->mlp ? &g : nullptr, fuse_store ? &sto : nullptr));
     if (ev) FEDG_HIP_CHECK(hipEventRecord(ev[1], ctx->stream));
     st->guess_lines += g.lines;
     st->guess_fallbacks += g.fallbacks;
@@ -1812,13 +1855,9 @@ static int hevi_step(fedg_hevi* st, hipEvent_t* ev) {
       st->newton_iters += it;
       st->newton_calls += 1;
     }
-    for (int v = 0; v < 5; ++v) {  // StoreImplicit
-      if (s == 1) {
-        TRY(launch_copy(ctx, t->var0 + v * nA, st->q[v], n));
-        TRY(launch_copy(ctx, t->varTmp + v * nA, st->q[v], n));
-      }
-      TRY(launch_axpy_acc(ctx, st->q[v], impl_fac, Ti[v], n));
-    }
+    if (!fuse_store)  // StoreImplicit
+      TRY(launch_store_implicit(ctx, st->q, Ti, s == 1 ? sto.var0 : nullptr, s == 1 ? sto.varTmp : nullptr,
+                                impl_fac));
     for (int v = 0; v < 5; ++v) TRY(launch_halo_exchange(ctx, st->q[v]));
     TRY(launch_drhot2pres(ctx, st->PRES, st->DPRES, st->q[1], st->PRES_hyd, st->THERM_hyd, st->Rtot, st->CVtot,
                           st->CPtot, n));
@@ -1868,7 +1907,7 @@ static int hevi_step(fedg_hevi* st, hipEvent_t* ev) {
       a.twv = t->imex ? t->b_im[s - 1] : t->b_ex[s - 1];
       TRY(launch_save_massflux(ctx, a));
     }
-    for (int v = 0; v < 5; ++v) TRY(tint_advance_one(t, s, st->q[v], v + 1));
+    TRY(imex_advance_all(t, s, st->q));
     if (ev) ev += 4;
   }
   if (st->mfilter) TRY(launch_modalfilter(ctx, st->q[0], st->q[3], st->q[4], st->q[2], st->q[1], 1));

@@ -40,6 +40,17 @@ struct HeviGuess {
   double r2_start;       // out: ||G||^2 at the iterates the lines started from
 };
 
+// The IMEX StoreImplicit of the HEVI stepper (CMN/scale_timeint_rk.F90,
+// StoreImplicit) done by the pass that forms the implicit tendency: q = q +
+// impl_fac T per node after T is written, and, when var0 / varTmp are given
+// (first stage), their copies of q before it.  q is the solve's stage state
+// (the q argument of hevi_cal_vi).
+struct HeviStore {
+  double* q[5];
+  double* var0[5];    // or null
+  double* varTmp[5];  // or null
+};
+
 int hevi_work_get(fedg_ctx* ctx, HeviWork** w);
 void hevi_work_free(fedg_ctx* ctx);
 
@@ -48,10 +59,12 @@ void hevi_work_free(fedg_ctx* ctx);
 // MOMY).  impl_fac == 0: the implicit operator itself.  max_it Newton
 // iterations (the reference: 1), stopping early once the residual norm has
 // dropped by rtol when rtol > 0; resid (host, max_it) receives the norms.
+// store (impl_fac != 0 only): the stepper's StoreImplicit, fused into the
+// pass that forms the tendency.
 int hevi_cal_vi(fedg_ctx* ctx, HeviWork* w, double* const tend[5], const double* const q[5],
                 const double* const var0[5], const double* DENS_hyd, const double* PRES_hyd, const double* Rtot,
                 const double* CVtot, const double* CPtot, double impl_fac, int max_it, double rtol, double* resid,
-                int* it_done, HeviGuess* guess = nullptr);
+                int* it_done, HeviGuess* guess = nullptr, const HeviStore* store = nullptr);
 
 // Features and targets of nrows lines (line = (line0 + r*stride) mod nlines)
 // of a converged solve (iterate in w->pv) into X (nrows x n_in) and Y

@@ -680,6 +680,33 @@ __global__ void hevi_tend_kernel(double* T0, double* T1, double* T2, double* T3,
   T4[t] = (p4[t] - q4[t]) / impl_fac;
 }
 
+// the same tendency, then the stepper's StoreImplicit on the node: var0 and
+// varTmp (first stage) take q, and q = q + impl_fac T
+struct TendStore {
+  double* T[5];
+  const double* p[5];
+  double* q[5];
+  double* var0[5];
+  double* varTmp[5];
+  double impl_fac;
+  int copy;
+};
+__global__ void hevi_tend_store_kernel(const TendStore a, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const double q = a.q[v][t];
+    const double T = (a.p[v][t] - q) / a.impl_fac;
+    a.T[v][t] = T;
+    if (a.copy) {
+      a.var0[v][t] = q;
+      a.varTmp[v][t] = q;
+    }
+    a.q[v][t] = q + a.impl_fac * T;
+  }
+}
+
 // sum of squares of a vector, per-block partial sums (residual norms)
 __global__ void sumsq_kernel(const double* x, long n, double* partial) {
   __shared__ double s[256];
@@ -1020,8 +1047,12 @@ int run_operator(fedg_ctx* ctx, HeviArgs& a) {
 int hevi_cal_vi(fedg_ctx* ctx, HeviWork* w, double* const tend[5], const double* const q[5],
                 const double* const var0[5], const double* DENS_hyd, const double* PRES_hyd, const double* Rtot,
                 const double* CVtot, const double* CPtot, double impl_fac, int max_it, double rtol, double* resid,
-                int* it_done, HeviGuess* guess) {
+                int* it_done, HeviGuess* guess, const HeviStore* store) {
   const DevMesh& m = ctx->m;
+  if (store && !(fabs(impl_fac) > 0.0)) {
+    set_error("hevi_cal_vi: the fused StoreImplicit needs impl_fac != 0");
+    return FEDG_ERR_ARG;
+  }
   HeviArgs a{};
   a.m = m;
   a.c = ctx->c;
@@ -1058,6 +1089,25 @@ int hevi_cal_vi(fedg_ctx* ctx, HeviWork* w, double* const tend[5], const double*
       const int rc = run_newton<N1>(ctx, a, max_it, rtol, resid, w->partial, w->partial_h, it_done, r0);
       if (rc != FEDG_OK) return rc;
       const int bs = 256;
+      if (store) {
+        TendStore ts{};
+        for (int v = 0; v < 5; ++v) {
+          ts.T[v] = tend[v];
+          ts.p[v] = w->pv[v];
+          ts.q[v] = store->q[v];
+          ts.var0[v] = store->var0[v];
+          ts.varTmp[v] = store->varTmp[v];
+          if (store->q[v] != q[v]) {
+            set_error("hevi_cal_vi: the StoreImplicit state is not the solve's stage state");
+            return FEDG_ERR_ARG;
+          }
+        }
+        ts.copy = store->var0[0] != nullptr;
+        ts.impl_fac = impl_fac;
+        hipLaunchKernelGGL(hevi_tend_store_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, ctx->stream, ts, n);
+        FEDG_HIP_CHECK(hipGetLastError());
+        return FEDG_OK;
+      }
       hipLaunchKernelGGL(hevi_tend_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, ctx->stream, tend[VD], tend[VT],
                          tend[VW], tend[VU], tend[VV], w->pv[VD], w->pv[VT], w->pv[VW], w->pv[VU], w->pv[VV], q[VD],
                          q[VT], q[VW], q[VU], q[VV], impl_fac, n);

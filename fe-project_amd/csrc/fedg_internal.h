@@ -178,6 +178,29 @@ int launch_axpy2_acc(fedg_ctx* ctx, double* y, double ca, const double* x, doubl
 int launch_axpy2_set(fedg_ctx* ctx, double* y, const double* a, double ca, const double* x, double cb,
                      const double* z, long n);  // y = a + ca*x + cb*z
 int launch_copy(fedg_ctx* ctx, double* y, const double* x, long n);
+// IMEX stage advance of the five prognostics in one pass, per node the
+// operations of the per-variable passes in their order:
+//   s < nstage:  vt = vt + cbe Te(s) + cbi Ti(s);  q = v0, then for s2 = 1..s
+//                q = q + ae(s2) Te(s2) + ai(s2) Ti(s2)
+//   s = nstage:  q = vt + cbe Te(s) + cbi Ti(s)
+constexpr int kImexMaxStages = 7;
+struct ImexAdvance {
+  double* q[5];
+  double* vt[5];
+  const double* v0[5];
+  const double* Te[5];                      // stage s
+  const double* Ti[5];
+  const double* Tes[kImexMaxStages][5];     // stages 1..s (s < nstage)
+  const double* Tis[kImexMaxStages][5];
+  double ae[kImexMaxStages], ai[kImexMaxStages];
+  double cbe, cbi;
+  int s, last;
+};
+int launch_imex_advance(fedg_ctx* ctx, const ImexAdvance& a);
+// IMEX StoreImplicit of the five prognostics in one pass: when var0 / varTmp
+// are given (first stage) they take q, then q = q + impl_fac Ti
+int launch_store_implicit(fedg_ctx* ctx, double* const q[5], const double* const Ti[5], double* const var0[5],
+                          double* const varTmp[5], double impl_fac);
 // atm_dyn_dgm_modalfilter_apply on elements 1..Ne (needs m.MFh, m.MFv)
 int launch_modalfilter(fedg_ctx* ctx, double* DDENS, double* MOMX, double* MOMY, double* MOMZ, double* DRHOT,
                        int do_weight_Gsqrt);

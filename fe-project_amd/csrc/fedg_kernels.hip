@@ -1688,6 +1688,52 @@ __global__ void axpy2_set_kernel(double* y, const double* a, double ca, const do
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) y[t] = a[t] + ca * x[t] + cb * z[t];
 }
+// one pass of the IMEX stage advance over the five prognostics
+// (blockIdx.y: variable); the expressions of axpy2_acc / axpy2_set.  S: the
+// stage (its loads unrolled so that they are all in flight at once), 0: last
+template <int S>
+__global__ void __launch_bounds__(256) imex_advance_kernel(const ImexAdvance a, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = blockIdx.y;
+  if (t >= n) return;
+  const double Te = a.Te[v][t], Ti = a.Ti[v][t];
+  if constexpr (S == 0) {
+    a.q[v][t] = a.vt[v][t] + a.cbe * Te + a.cbi * Ti;
+  } else {
+    double e[S], im[S];
+#pragma unroll
+    for (int s2 = 0; s2 < S; ++s2) {
+      e[s2] = a.Tes[s2][v][t];
+      im[s2] = a.Tis[s2][v][t];
+    }
+    a.vt[v][t] = a.vt[v][t] + a.cbe * Te + a.cbi * Ti;
+    double q = a.v0[v][t];
+#pragma unroll
+    for (int s2 = 0; s2 < S; ++s2) q = q + a.ae[s2] * e[s2] + a.ai[s2] * im[s2];
+    a.q[v][t] = q;
+  }
+}
+
+struct StoreImpl {
+  double* q[5];
+  const double* Ti[5];
+  double* var0[5];
+  double* varTmp[5];
+  double impl_fac;
+  int copy;
+};
+__global__ void __launch_bounds__(256) store_implicit_kernel(const StoreImpl a, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = blockIdx.y;
+  if (t >= n) return;
+  const double q = a.q[v][t];
+  if (a.copy) {
+    a.var0[v][t] = q;
+    a.varTmp[v][t] = q;
+  }
+  a.q[v][t] = q + a.impl_fac * a.Ti[v][t];
+}
+
 // Access-pattern probe (measurement only): the loads and stores of one p = 7
 // fused stage on a periodic NeX x NeY x NeZ box without its arithmetic.  Every
 // workgroup reads NR fields of its element's 512 nodes, the exterior values
@@ -1927,6 +1973,44 @@ int run_stage_probe(int mode, int NeX, int NeY, int NeZ, int reps, double* ms) {
   for (auto b : buf)
     if (b) (void)hipFree(b);
   return rc;
+}
+
+int launch_imex_advance(fedg_ctx* ctx, const ImexAdvance& a) {
+  const long n = (long)ctx->m.Np * ctx->m.Ne;
+  if (a.s < 1 || a.s > kImexMaxStages) {
+    set_error("IMEX advance: stage out of range");
+    return FEDG_ERR_ARG;
+  }
+  const dim3 g(nblocks(n, 256), 5);
+  switch (a.last ? 0 : a.s) {
+    case 0: hipLaunchKernelGGL(imex_advance_kernel<0>, g, dim3(256), 0, ctx->stream, a, n); break;
+    case 1: hipLaunchKernelGGL(imex_advance_kernel<1>, g, dim3(256), 0, ctx->stream, a, n); break;
+    case 2: hipLaunchKernelGGL(imex_advance_kernel<2>, g, dim3(256), 0, ctx->stream, a, n); break;
+    case 3: hipLaunchKernelGGL(imex_advance_kernel<3>, g, dim3(256), 0, ctx->stream, a, n); break;
+    case 4: hipLaunchKernelGGL(imex_advance_kernel<4>, g, dim3(256), 0, ctx->stream, a, n); break;
+    case 5: hipLaunchKernelGGL(imex_advance_kernel<5>, g, dim3(256), 0, ctx->stream, a, n); break;
+    case 6: hipLaunchKernelGGL(imex_advance_kernel<6>, g, dim3(256), 0, ctx->stream, a, n); break;
+    default: hipLaunchKernelGGL(imex_advance_kernel<7>, g, dim3(256), 0, ctx->stream, a, n); break;
+  }
+  FEDG_HIP_CHECK(hipGetLastError());
+  return FEDG_OK;
+}
+
+int launch_store_implicit(fedg_ctx* ctx, double* const q[5], const double* const Ti[5], double* const var0[5],
+                          double* const varTmp[5], double impl_fac) {
+  const long n = (long)ctx->m.Np * ctx->m.Ne;
+  StoreImpl a{};
+  a.copy = var0 != nullptr && varTmp != nullptr;
+  for (int v = 0; v < 5; ++v) {
+    a.q[v] = q[v];
+    a.Ti[v] = Ti[v];
+    a.var0[v] = a.copy ? var0[v] : nullptr;
+    a.varTmp[v] = a.copy ? varTmp[v] : nullptr;
+  }
+  a.impl_fac = impl_fac;
+  hipLaunchKernelGGL(store_implicit_kernel, dim3(nblocks(n, 256), 5), dim3(256), 0, ctx->stream, a, n);
+  FEDG_HIP_CHECK(hipGetLastError());
+  return FEDG_OK;
 }
 
 int launch_heve_fused32(fedg_ctx* ctx, const TendIn& in, const RkOut& rk) {
