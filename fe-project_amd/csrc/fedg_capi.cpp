@@ -92,6 +92,8 @@ struct fedg_heve {
   // time-averaged mass fluxes for the tracers (QA > 0): MFLX_x/y/z_tavg and
   // ALPH_DENS_M/P_tavg saved at every stage, or null
   double *mflx[3], *alph[2];
+  // physics tendencies (DENS_tp, RHOT_tp, MOMZ_tp, MOMX_tp, MOMY_tp, RHOH_p) or null
+  const double* tp[6];
 };
 
 namespace {
@@ -1425,6 +1427,47 @@ int fedg_heve_set_massflux(fedg_heve* st, double* MFLX_x_tavg, double* MFLX_y_ta
   return FEDG_OK;
 }
 
+// the six physics-tendency pointers in the stepper's order, checked: all or none
+static int phys_pointers(const double* DENS_tp, const double* MOMX_tp, const double* MOMY_tp, const double* MOMZ_tp,
+                         const double* RHOT_tp, const double* RHOH_p, const double* tp[6]) {
+  const double* in[6] = {DENS_tp, RHOT_tp, MOMZ_tp, MOMX_tp, MOMY_tp, RHOH_p};
+  int given = 0;
+  for (int v = 0; v < 6; ++v) given += in[v] != nullptr;
+  if (given != 0 && given != 6) {
+    set_error("physics tendencies: give all six arrays or none");
+    return FEDG_ERR_ARG;
+  }
+  for (int v = 0; v < 6; ++v) tp[v] = in[v];
+  return FEDG_OK;
+}
+
+int fedg_heve_set_phys_tend(fedg_heve* st, const double* DENS_tp, const double* MOMX_tp, const double* MOMY_tp,
+                            const double* MOMZ_tp, const double* RHOT_tp, const double* RHOH_p) {
+  if (!st) return FEDG_ERR_ARG;
+  const double* tp[6];
+  TRY(phys_pointers(DENS_tp, MOMX_tp, MOMY_tp, MOMZ_tp, RHOT_tp, RHOH_p, tp));
+  if (tp[0] && st->fp32) {
+    set_error("fedg_heve_set_phys_tend: physics tendencies join the FP64 stage only");
+    return FEDG_ERR_UNSUPPORTED;
+  }
+  for (int v = 0; v < 6; ++v) st->tp[v] = tp[v];
+  return FEDG_OK;
+}
+
+int fedg_add_phy_tend(fedg_ctx* ctx, double* DENS_dt, double* MOMX_dt, double* MOMY_dt, double* MOMZ_dt,
+                      double* RHOT_dt, const double* DENS_tp, const double* MOMX_tp, const double* MOMY_tp,
+                      const double* MOMZ_tp, const double* RHOT_tp, const double* RHOH_p, const double* PRES,
+                      const double* Rtot, const double* CPtot) {
+  if (!ctx || !DENS_dt || !MOMX_dt || !MOMY_dt || !MOMZ_dt || !RHOT_dt || !DENS_tp || !MOMX_tp || !MOMY_tp ||
+      !MOMZ_tp || !RHOT_tp || !RHOH_p || !PRES || !Rtot || !CPtot)
+    return FEDG_ERR_ARG;
+  double* T[5] = {DENS_dt, RHOT_dt, MOMZ_dt, MOMX_dt, MOMY_dt};
+  const double* tp[6] = {DENS_tp, RHOT_tp, MOMZ_tp, MOMX_tp, MOMY_tp, RHOH_p};
+  TRY(launch_add_phy_tend(ctx, T, tp, PRES, Rtot, CPtot));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return FEDG_OK;
+}
+
 int fedg_heve_set_precision(fedg_heve* st, int bits) {
   if (!st || (bits != 32 && bits != 64)) return FEDG_ERR_ARG;
   if (bits == 32 && !st->flat) {
@@ -1433,6 +1476,10 @@ int fedg_heve_set_precision(fedg_heve* st, int bits) {
   }
   if (bits == 32 && st->mflx[0]) {
     set_error("the mass-flux save needs the FP64 stage (fedg_heve_set_massflux with NULL first)");
+    return FEDG_ERR_UNSUPPORTED;
+  }
+  if (bits == 32 && st->tp[0]) {
+    set_error("physics tendencies join the FP64 stage only (fedg_heve_set_phys_tend with NULL first)");
     return FEDG_ERR_UNSUPPORTED;
   }
   st->fp32 = bits == 32;
@@ -1497,6 +1544,7 @@ static int heve_step(fedg_heve* st, std::vector<hipEvent_t>* evs, size_t* ev_i) 
   in.CVtot = st->CVtot;
   in.CPtot = st->CPtot;
   in.lvl = st->lvl;
+  for (int v = 0; v < 6; ++v) in.tp[v] = st->tp[v];
   const int q0 = st->q0;
   int cur = q0;
   int last_out = q0;
@@ -1671,6 +1719,8 @@ struct fedg_hevi {
   int flat7;
   double* lvl;
   int fp32;
+  // physics tendencies (DENS_tp, RHOT_tp, MOMZ_tp, MOMX_tp, MOMY_tp, RHOH_p) or null
+  const double* tp[6];
 };
 
 int fedg_hevi_destroy(fedg_hevi* st) {
@@ -1756,6 +1806,12 @@ int fedg_hevi_set_precision(fedg_hevi* st, int bits) {
   }
   st->fp32 = bits == 32;
   return FEDG_OK;
+}
+
+int fedg_hevi_set_phys_tend(fedg_hevi* st, const double* DENS_tp, const double* MOMX_tp, const double* MOMY_tp,
+                            const double* MOMZ_tp, const double* RHOT_tp, const double* RHOH_p) {
+  if (!st) return FEDG_ERR_ARG;
+  return phys_pointers(DENS_tp, MOMX_tp, MOMY_tp, MOMZ_tp, RHOT_tp, RHOH_p, st->tp);
 }
 
 int fedg_hevi_set_state(fedg_hevi* st, const double* const* q) {
@@ -1884,6 +1940,8 @@ static int hevi_step(fedg_hevi* st, hipEvent_t* ev) {
       TRY(launch_hevi_explicit(ctx, in, Te));
     }
     if (ev) FEDG_HIP_CHECK(hipEventRecord(ev[3], ctx->stream));
+    // add_phy_tend on the explicit tendency (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:846-856)
+    if (st->tp[0]) TRY(launch_add_phy_tend(ctx, Te, st->tp, st->PRES, st->Rtot, st->CPtot));
     if (st->mflx[0]) {
       // FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:899-915: weights b_ex(s)
       // horizontally and b_im(s) vertically (IMEX), is_hevi; the state and

@@ -145,6 +145,22 @@ __device__ __forceinline__ double dpres_of(double DRHOT, double THERM_hyd, doubl
   const double PRES = c.PRES00 * fedg_pow(Rtot * rP0 * RHOT, CPtot / CVtot);
   return PRES - PRES_hyd;
 }
+// PRES of a node as DRHOT2PRES forms it (the PRES of dpres_of)
+__device__ __forceinline__ double pres_of(double DRHOT, double THERM_hyd, double Rtot, double CVtot, double CPtot,
+                                          const Consts& c) {
+  const double rP0 = 1.0 / c.PRES00;
+  const double RHOT = THERM_hyd + DRHOT;
+  return c.PRES00 * fedg_pow(Rtot * rP0 * RHOT, CPtot / CVtot);
+}
+// The THERM line of add_phy_tend (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:
+// 1143-1175, ENTOT_CONSERVE_SCHEME_FLAG off): T + RHOT_tp + RHOH_p / (CPtot
+// EXNER), EXNER = (PRES / PRES00)^(Rtot / CPtot)
+__device__ __forceinline__ double phys_add_rhot(double T, double RHOT_tp, double RHOH_p, double PRES, double Rtot,
+                                                double CPtot, const Consts& c) {
+  const double rP0 = 1.0 / c.PRES00;
+  const double EXNER = fedg_pow(PRES * rP0, Rtot / CPtot);
+  return T + RHOT_tp + RHOH_p / (CPtot * EXNER);
+}
 // The same with the pow tables at `tab` (a copy in LDS): the same bits.
 __device__ __forceinline__ double dpres_of_tab(double DRHOT, double THERM_hyd, double PRES_hyd, double Rtot,
                                                double CVtot, double CPtot, const Consts& c, const double* tab) {

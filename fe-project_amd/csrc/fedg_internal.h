@@ -100,6 +100,10 @@ struct TendIn {
   // (5, NeZ, Np) tables by element level when every element of a level holds
   // the same values (horizontally homogeneous base state), else null
   const double* lvl;
+  // fused: physics tendencies added to the dynamics tendency before the RK
+  // update (add_phy_tend): DENS_tp, RHOT_tp, MOMZ_tp, MOMX_tp, MOMY_tp (VID
+  // order), then RHOH_p; null when there are none
+  const double* tp[6];
 };
 
 struct RkOut {
@@ -197,6 +201,10 @@ struct ImexAdvance {
   int s, last;
 };
 int launch_imex_advance(fedg_ctx* ctx, const ImexAdvance& a);
+// add_phy_tend on the five tendencies T (VID order) of elements 1..Ne:
+// tp = DENS_tp, RHOT_tp, MOMZ_tp, MOMX_tp, MOMY_tp, RHOH_p; PRES, Rtot, CPtot
+int launch_add_phy_tend(fedg_ctx* ctx, double* const T[5], const double* const tp[6], const double* PRES,
+                        const double* Rtot, const double* CPtot);
 // IMEX StoreImplicit of the five prognostics in one pass: when var0 / varTmp
 // are given (first stage) they take q, then q = q + impl_fac Ti
 int launch_store_implicit(fedg_ctx* ctx, double* const q[5], const double* const Ti[5], double* const var0[5],

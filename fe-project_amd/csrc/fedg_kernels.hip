@@ -358,6 +358,23 @@ heve_kernel(const DevMesh m, const Consts c, const TendIn in, const RkOut rk) {
       T = -dpy - cor * q[VU];
       T = T - X * RGsqrt;
     }
+    if constexpr (FUSED) {
+      // add_phy_tend (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:1098-1178):
+      // the physics tendencies join the stage's tendency before the update
+      if (valid && in.tp[0] != nullptr) {
+        if (v == VT) {
+          double Rt = c.Rdry, Cv = c.CVdry, Cp = c.CPdry;
+          if (in.Rtot != nullptr) {
+            Rt = ldg(in.Rtot, o);
+            Cv = ldg(in.CVtot, o);
+            Cp = ldg(in.CPtot, o);
+          }
+          T = phys_add_rhot(T, ldg(in.tp[VT], o), ldg(in.tp[5], o), pres_of(q[VT], Th, Rt, Cv, Cp, c), Rt, Cp, c);
+        } else {
+          T = T + ldg(in.tp[v], o);
+        }
+      }
+    }
     if (valid) {
       if constexpr (DROPIN) {
         stg(rk.out[v], o, T);
@@ -1714,6 +1731,25 @@ __global__ void __launch_bounds__(256) imex_advance_kernel(const ImexAdvance a, 
   }
 }
 
+struct PhysTend {
+  double* T[5];
+  const double* tp[6];
+  const double *PRES, *Rtot, *CPtot;
+};
+// add_phy_tend (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:1098-1178,
+// ENTOT_CONSERVE_SCHEME_FLAG off) on one node per thread
+__global__ void __launch_bounds__(256) add_phy_tend_kernel(const PhysTend a, const Consts c, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint32_t o = (uint32_t)t;
+  a.T[VD][t] = a.T[VD][t] + ldg(a.tp[VD], o);
+  a.T[VW][t] = a.T[VW][t] + ldg(a.tp[VW], o);
+  a.T[VU][t] = a.T[VU][t] + ldg(a.tp[VU], o);
+  a.T[VV][t] = a.T[VV][t] + ldg(a.tp[VV], o);
+  a.T[VT][t] = phys_add_rhot(a.T[VT][t], ldg(a.tp[VT], o), ldg(a.tp[5], o), ldg(a.PRES, o), ldg(a.Rtot, o),
+                             ldg(a.CPtot, o), c);
+}
+
 struct StoreImpl {
   double* q[5];
   const double* Ti[5];
@@ -1996,6 +2032,20 @@ int launch_imex_advance(fedg_ctx* ctx, const ImexAdvance& a) {
   return FEDG_OK;
 }
 
+int launch_add_phy_tend(fedg_ctx* ctx, double* const T[5], const double* const tp[6], const double* PRES,
+                        const double* Rtot, const double* CPtot) {
+  const long n = (long)ctx->m.Np * ctx->m.Ne;
+  PhysTend a{};
+  for (int v = 0; v < 5; ++v) a.T[v] = T[v];
+  for (int v = 0; v < 6; ++v) a.tp[v] = tp[v];
+  a.PRES = PRES;
+  a.Rtot = Rtot;
+  a.CPtot = CPtot;
+  hipLaunchKernelGGL(add_phy_tend_kernel, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream, a, ctx->c, n);
+  FEDG_HIP_CHECK(hipGetLastError());
+  return FEDG_OK;
+}
+
 int launch_store_implicit(fedg_ctx* ctx, double* const q[5], const double* const Ti[5], double* const var0[5],
                           double* const varTmp[5], double impl_fac) {
   const long n = (long)ctx->m.Np * ctx->m.Ne;
@@ -2052,7 +2102,8 @@ int launch_heve_fused32(fedg_ctx* ctx, const TendIn& in, const RkOut& rk) {
 
 int launch_heve_fused(fedg_ctx* ctx, const TendIn& in, const RkOut& rk, int use_flat) {
   const DevMesh& m = ctx->m;
-  if (use_flat && m.n1 == 8 && m.lift_tensor) {
+  // the p = 7 layout carries no physics tendencies: with them, the general layout
+  if (use_flat && m.n1 == 8 && m.lift_tensor && in.tp[0] == nullptr) {
     if (in.lvl)
       hipLaunchKernelGGL(heve7_lvl_kernel, dim3(m.Ne), dim3(256), 0, ctx->stream, m, ctx->c, in, rk);
     else

@@ -6,7 +6,8 @@ built: there is no CPU fallback.
 """
 from . import cases, constants, restart, tracer  # noqa: F401
 from ._lib import EXPORTED, LIB, LIB_PATH, FedgError, check  # noqa: F401
-from .dyncore import (VARS, DevField, DevField32, HeveStepper, HeviStepper, Mlp, TimeIntRK, apply_bc,  # noqa: F401
+from .dyncore import (VARS, DevField, DevField32, HeveStepper, HeviStepper, Mlp, TimeIntRK, add_phy_tend,  # noqa: F401
+                      apply_bc,
                       calc_phyd_hgrad, calc_pressure, device_name, halo_exchange, heve_add_bnd_contrib, heve_cal_tend,
                       heve_cal_tend_cco, hevi_cal_tend, trcadvect_cal_tend,
                       trcadvect_calc_fct_coef, trcadvect_save_massflux, trcadvect_tmar,

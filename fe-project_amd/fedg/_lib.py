@@ -145,6 +145,9 @@ def _load():
         "fedg_heve_is_flat": (C.c_int, [vp]),
         "fedg_heve_set_massflux": (C.c_int, [vp] * 6),
         "fedg_hevi_set_massflux": (C.c_int, [vp] * 6),
+        "fedg_add_phy_tend": (C.c_int, [vp] * 15),
+        "fedg_heve_set_phys_tend": (C.c_int, [vp] * 7),
+        "fedg_hevi_set_phys_tend": (C.c_int, [vp] * 7),
         "fedg_heve_set_state": (C.c_int, [vp, C.POINTER(vp)]),
         "fedg_heve_get_state": (C.c_int, [vp, C.POINTER(vp)]),
         "fedg_heve_state_dev": (C.c_int, [vp, C.POINTER(vp)]),
@@ -175,7 +178,8 @@ EXPORTED = (
     "fedg_hevi_run_timed fedg_hevi_newton_stats fedg_calc_phyd_hgrad fedg_tint_create fedg_tint_destroy "
     "fedg_tint_info fedg_tint_tend_ex fedg_tint_tend_im fedg_tint_store_var0 fedg_tint_advance "
     "fedg_tint_store_implicit fedg_heve_create fedg_heve_destroy fedg_heve_is_flat fedg_heve_set_massflux fedg_hevi_set_massflux fedg_heve_set_state "
-    "fedg_heve_get_state fedg_heve_state_dev fedg_heve_run fedg_heve_run_timed fedg_heve_get_pressure"
+    "fedg_heve_get_state fedg_heve_state_dev fedg_heve_run fedg_heve_run_timed fedg_heve_get_pressure "
+    "fedg_add_phy_tend fedg_heve_set_phys_tend fedg_hevi_set_phys_tend"
 ).split()
 
 

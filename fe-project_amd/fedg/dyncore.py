@@ -202,6 +202,28 @@ def hevi_cal_vi_guess(dm: DeviceMesh, out, DDENS, MOMX, MOMY, MOMZ, DRHOT, DENS_
     return it.value, np.array(resid[:n]), fb.value
 
 
+def add_phy_tend(dm: DeviceMesh, out, DENS_tp, MOMX_tp, MOMY_tp, MOMZ_tp, RHOT_tp, RHOH_p, PRES, Rtot, CPtot):
+    """add_phy_tend in place on out = (DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt): device fields."""
+    args = [_p(a) for a in (*out, DENS_tp, MOMX_tp, MOMY_tp, MOMZ_tp, RHOT_tp, RHOH_p, PRES, Rtot, CPtot)]
+    check(LIB.fedg_add_phy_tend(dm.handle, *args), "fedg_add_phy_tend")
+
+
+def _phys_fields(m, fields):
+    """Check the six physics-tendency fields of set_phys_tend (all or none, DevFields of Np*NeA
+    values) and return them; the steppers read them at every stage, so they keep a reference."""
+    given = [f is not None for f in fields]
+    if not any(given):
+        return None
+    if not all(given):
+        raise ValueError("set_phys_tend takes all six fields or none")
+    for f, name in zip(fields, ("DENS_tp", "MOMX_tp", "MOMY_tp", "MOMZ_tp", "RHOT_tp", "RHOH_p")):
+        if not isinstance(f, DevField):
+            raise TypeError(f"{name} must be a DevField")
+        if f.n < m.Np * m.NeA:
+            raise ValueError(f"{name} holds {f.n} values, needs {m.Np * m.NeA}")
+    return tuple(fields)
+
+
 def calc_pressure(dm: DeviceMesh, PRES, DPRES, DRHOT, PRES_hyd, THERM_hyd, Rtot, CVtot, CPtot):
     check(LIB.fedg_drhot2pres(dm.handle, *[_p(a) for a in (PRES, DPRES, DRHOT, PRES_hyd, THERM_hyd, Rtot, CVtot,
                                                            CPtot)]), "fedg_drhot2pres")
@@ -322,6 +344,13 @@ class HeveStepper:
               "fedg_heve_set_massflux")
         self._mflx = fl
 
+    def set_phys_tend(self, DENS_tp=None, MOMX_tp=None, MOMY_tp=None, MOMZ_tp=None, RHOT_tp=None, RHOH_p=None):
+        """Physics tendencies added to every stage's tendency (add_phy_tend); no arguments: stop."""
+        fl = _phys_fields(self.dm.mesh, (DENS_tp, MOMX_tp, MOMY_tp, MOMZ_tp, RHOT_tp, RHOH_p))
+        check(LIB.fedg_heve_set_phys_tend(self.handle, *([_p(a) for a in fl] if fl else [None] * 6)),
+              "fedg_heve_set_phys_tend")
+        self._tp = fl
+
     def set_precision(self, bits: int):
         """64: the reference's FP64 stage; 32: the FP32 perturbation-form tendency (FP64 state and RK)."""
         check(LIB.fedg_heve_set_precision(self.handle, int(bits)), "fedg_heve_set_precision")
@@ -361,6 +390,7 @@ class HeveStepper:
             LIB.fedg_heve_destroy(self.handle)
             self.handle = None
         self._mflx = None  # the stepper no longer writes the mass-flux fields
+        self._tp = None  # nor reads the physics tendencies
 
     def __del__(self):
         try:
@@ -452,6 +482,13 @@ class HeviStepper:
         check(LIB.fedg_hevi_newton_stats(self.handle, C.byref(it), C.byref(calls)), "fedg_hevi_newton_stats")
         return it.value, calls.value
 
+    def set_phys_tend(self, DENS_tp=None, MOMX_tp=None, MOMY_tp=None, MOMZ_tp=None, RHOT_tp=None, RHOH_p=None):
+        """Physics tendencies added to every stage's explicit tendency (add_phy_tend); no arguments: stop."""
+        fl = _phys_fields(self.dm.mesh, (DENS_tp, MOMX_tp, MOMY_tp, MOMZ_tp, RHOT_tp, RHOH_p))
+        check(LIB.fedg_hevi_set_phys_tend(self.handle, *([_p(a) for a in fl] if fl else [None] * 6)),
+              "fedg_hevi_set_phys_tend")
+        self._tp = fl
+
     def set_massflux(self, MFLX_x=None, MFLX_y=None, MFLX_z=None, alph_dens_M=None, alph_dens_P=None):
         """Save the tracer mass fluxes of every stage (save_massflux, HEVI weights); no arguments: stop."""
         fl = _massflux_fields(self.dm.mesh, (MFLX_x, MFLX_y, MFLX_z, alph_dens_M, alph_dens_P))
@@ -464,6 +501,7 @@ class HeviStepper:
             LIB.fedg_hevi_destroy(self.handle)
             self.handle = None
         self._mflx = None  # the stepper no longer writes the mass-flux fields
+        self._tp = None  # nor reads the physics tendencies
 
     def __del__(self):
         try:

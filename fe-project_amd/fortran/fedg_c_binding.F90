@@ -64,7 +64,7 @@ module fedg_c_binding
   public :: fedg_trcadvect_save_massflux, fedg_trcadvect_cal_alphdens_advtest
   public :: fedg_tracer_modalfilter_apply, fedg_trcvar_rk_advance
   public :: fedg_hevi_cal_tend, fedg_hevi_cal_vi, fedg_hevi_cal_vi_newton, fedg_hevi_cal_vi_noalloc
-  public :: fedg_calc_phyd_hgrad
+  public :: fedg_calc_phyd_hgrad, fedg_add_phy_tend, fedg_heve_set_phys_tend, fedg_hevi_set_phys_tend
   public :: fedg_tint_create, fedg_tint_destroy, fedg_tint_info, fedg_tint_tend_ex, fedg_tint_tend_im
   public :: fedg_tint_store_var0, fedg_tint_advance, fedg_tint_store_implicit
   public :: fedg_heve_create, fedg_heve_destroy, fedg_heve_is_flat, fedg_heve_set_massflux
@@ -430,6 +430,16 @@ module fedg_c_binding
       real(c_double), intent(in) :: PRES_hyd(*), PRES_hyd_ref(*)
     end function fedg_calc_phyd_hgrad
 
+    !> add_phy_tend (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:1098-1178, ENTOT_CONSERVE_SCHEME_FLAG off)
+    integer(c_int) function fedg_add_phy_tend(ctx, DENS_dt, MOMX_dt, MOMY_dt, MOMZ_dt, RHOT_dt, &
+        DENS_tp, MOMX_tp, MOMY_tp, MOMZ_tp, RHOT_tp, RHOH_p, PRES, Rtot, CPtot) bind(C, name="fedg_add_phy_tend")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: DENS_dt(*), MOMX_dt(*), MOMY_dt(*), MOMZ_dt(*), RHOT_dt(*)
+      real(c_double), intent(in) :: DENS_tp(*), MOMX_tp(*), MOMY_tp(*), MOMZ_tp(*), RHOT_tp(*), RHOH_p(*)
+      real(c_double), intent(in) :: PRES(*), Rtot(*), CPtot(*)
+    end function fedg_add_phy_tend
+
     ! ---- time integration (timeint_rk, CMN/scale_timeint_rk.F90) --------------
     !> timeint_rk%Init (CMN/scale_timeint_rk.F90:135-229); scheme: a NUL-terminated name
     integer(c_int) function fedg_tint_create(ctx, scheme, dt, tint) bind(C, name="fedg_tint_create")
@@ -522,6 +532,13 @@ module fedg_c_binding
       type(c_ptr), value :: st, MFLX_x_tavg, MFLX_y_tavg, MFLX_z_tavg, alph_dens_M, alph_dens_P
     end function fedg_heve_set_massflux
 
+    !> physics tendencies added at every stage (add_phy_tend): six device addresses or six c_null_ptr
+    integer(c_int) function fedg_heve_set_phys_tend(st, DENS_tp, MOMX_tp, MOMY_tp, MOMZ_tp, RHOT_tp, RHOH_p) &
+        bind(C, name="fedg_heve_set_phys_tend")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: st, DENS_tp, MOMX_tp, MOMY_tp, MOMZ_tp, RHOT_tp, RHOH_p
+    end function fedg_heve_set_phys_tend
+
     integer(c_int) function fedg_heve_set_precision(st, bits) bind(C, name="fedg_heve_set_precision")
       import :: c_int, c_ptr
       type(c_ptr), value :: st
@@ -609,6 +626,12 @@ module fedg_c_binding
       import :: c_int, c_ptr
       type(c_ptr), value :: st, MFLX_x_tavg, MFLX_y_tavg, MFLX_z_tavg, alph_dens_M, alph_dens_P
     end function fedg_hevi_set_massflux
+
+    integer(c_int) function fedg_hevi_set_phys_tend(st, DENS_tp, MOMX_tp, MOMY_tp, MOMZ_tp, RHOT_tp, RHOH_p) &
+        bind(C, name="fedg_hevi_set_phys_tend")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: st, DENS_tp, MOMX_tp, MOMY_tp, MOMZ_tp, RHOT_tp, RHOH_p
+    end function fedg_hevi_set_phys_tend
 
     integer(c_int) function fedg_hevi_run(st, nsteps) bind(C, name="fedg_hevi_run")
       import :: c_int, c_ptr

@@ -408,6 +408,17 @@ int fedg_hevi_cal_vi_noalloc(fedg_ctx *ctx, double *DENS_dt, double *MOMX_dt, do
 int fedg_calc_phyd_hgrad(fedg_ctx *ctx, double *DPhydDx, double *DPhydDy, const double *PRES_hyd,
                          const double *PRES_hyd_ref);
 
+/* add_phy_tend (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:1098-1178, the
+ * ENTOT_CONSERVE_SCHEME_FLAG = .false. branch, which the rhotheta form of the
+ * dynamics takes): the physics tendencies added to the dynamics tendencies
+ * of elements 1..Ne in place, DENS_dt += DENS_tp, MOMX/Y/Z_dt += MOMX/Y/Z_tp,
+ * RHOT_dt = RHOT_dt + RHOT_tp + RHOH_p / (CPtot EXNER) with EXNER =
+ * (PRES / PRES00)^(Rtot / CPtot).  Device (Np,NeA) fields. */
+int fedg_add_phy_tend(fedg_ctx *ctx, double *DENS_dt, double *MOMX_dt, double *MOMY_dt, double *MOMZ_dt,
+                      double *RHOT_dt, const double *DENS_tp, const double *MOMX_tp, const double *MOMY_tp,
+                      const double *MOMZ_tp, const double *RHOT_tp, const double *RHOH_p, const double *PRES,
+                      const double *Rtot, const double *CPtot);
+
 /* ------------------------------------------------------------------ */
 /* Time integration (timeint_rk, CMN/scale_timeint_rk.F90:35-104, 135-229).
  * The integrator owns var0/varTmp/tend_buf2D_ex(/_im) on the device, sized
@@ -456,6 +467,16 @@ int fedg_heve_is_flat(const fedg_heve *st);
  * fedg_heve_destroy; MFLX_* hold Np*NeA values, alph_dens_* NfpTot*Ne. */
 int fedg_heve_set_massflux(fedg_heve *st, double *MFLX_x_tavg, double *MFLX_y_tavg, double *MFLX_z_tavg,
                            double *alph_dens_M, double *alph_dens_P);
+/* Physics tendencies of the driver's add_phy_tend
+ * (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:846-856, 1098-1178): added
+ * to every stage's dynamics tendency before the RK update, with the stage's
+ * PRES in EXNER.  Six device (Np,NeA) arrays (DENS_tp, MOMX_tp, MOMY_tp,
+ * MOMZ_tp, RHOT_tp, RHOH_p), constant over the step as in the reference, or
+ * all NULL to stop.  The stepper keeps the pointers: the arrays stay
+ * allocated until the next call or fedg_heve_destroy.  FP64 stages only; at
+ * p = 7 the stages then run on the general-layout kernel. */
+int fedg_heve_set_phys_tend(fedg_heve *st, const double *DENS_tp, const double *MOMX_tp, const double *MOMY_tp,
+                            const double *MOMZ_tp, const double *RHOT_tp, const double *RHOH_p);
 /* No counterpart in the reference (one global precision switch, Mkinclude:34-35):
  * bits = 32 evaluates each stage's tendency -- fluxes, face flux, derivative
  * contractions, lift, buoyancy -- in FP32 on the perturbation variables about
@@ -504,6 +525,11 @@ int fedg_hevi_get_state(fedg_hevi *st, double *const *q);
  * vertically and the HEVI form of the damping coefficients (no |nz| in Gnn). */
 int fedg_hevi_set_massflux(fedg_hevi *st, double *MFLX_x_tavg, double *MFLX_y_tavg, double *MFLX_z_tavg,
                            double *alph_dens_M, double *alph_dens_P);
+/* The physics tendencies of fedg_heve_set_phys_tend for the HEVI stepper:
+ * added to every stage's explicit tendency (the reference's add_phy_tend
+ * after cal_tend_ex). */
+int fedg_hevi_set_phys_tend(fedg_hevi *st, const double *DENS_tp, const double *MOMX_tp, const double *MOMY_tp,
+                            const double *MOMZ_tp, const double *RHOT_tp, const double *RHOH_p);
 int fedg_hevi_run(fedg_hevi *st, int nsteps);
 /* nsteps steps bracketed by HIP events: total milliseconds and the mean
  * duration per stage of the implicit solve and of the explicit tendency */

@@ -988,6 +988,41 @@ void oracle_heve_set_massflux(double *MFLX_x, double *MFLX_y, double *MFLX_z, do
   g_alph[1] = alph_P;
 }
 
+/* FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:1098-1178 (add_phy_tend), the
+ * ENTOT_CONSERVE_SCHEME_FLAG = .false. branch; the OpenACC (:1157-1163) and
+ * CPU (:1166-1172) forms compute the same expression per node */
+void oracle_add_phy_tend(const oracle_mesh *m, double *const T[5], const double *const tp[6], const double *PRES,
+                         const double *Rtot, const double *CPtot) {
+  const double rP0 = 1.0 / m->PRES00;
+  const long n = (long)m->Np * m->Ne;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) {
+    T[DENS_VID][i] = T[DENS_VID][i] + tp[DENS_VID][i];
+    T[MOMZ_VID][i] = T[MOMZ_VID][i] + tp[MOMZ_VID][i];
+    T[MOMX_VID][i] = T[MOMX_VID][i] + tp[MOMX_VID][i];
+    T[MOMY_VID][i] = T[MOMY_VID][i] + tp[MOMY_VID][i];
+  }
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) {
+    const double EXNER = opow(PRES[i] * rP0, Rtot[i] / CPtot[i]);
+    T[RHOT_VID][i] = T[RHOT_VID][i] + tp[RHOT_VID][i] + tp[5][i] / (CPtot[i] * EXNER);
+  }
+}
+static const double *g_tp[6];
+void oracle_set_phys_tend(const double *tp0, const double *tp1, const double *tp2, const double *tp3,
+                          const double *tp4, const double *tp5) {
+  g_tp[0] = tp0;
+  g_tp[1] = tp1;
+  g_tp[2] = tp2;
+  g_tp[3] = tp3;
+  g_tp[4] = tp4;
+  g_tp[5] = tp5;
+}
+void oracle_phys_tend_hook(const oracle_mesh *m, double *const T[5], const double *PRES, const double *Rtot,
+                           const double *CPtot) {
+  if (g_tp[0]) oracle_add_phy_tend(m, T, g_tp, PRES, Rtot, CPtot);
+}
+
 void oracle_heve_update(const oracle_mesh *m, const oracle_rk_tab *t, int nsteps,
                         double *const q[5], double *PRES, double *DPRES,
                         const double *DENS_hyd, const double *PRES_hyd, const double *PRES_hyd_ref,
@@ -1011,7 +1046,11 @@ void oracle_heve_update(const oracle_mesh *m, const oracle_rk_tab *t, int nsteps
                            T + MOMZ_VID * NpNeA, T + RHOT_VID * NpNeA, q[DENS_VID], q[MOMX_VID], q[MOMY_VID],
                            q[MOMZ_VID], q[RHOT_VID], DPRES, DENS_hyd, PRES_hyd, PRES_hyd_ref, THERM_hyd,
                            CORIOLIS, Rtot, CVtot, CPtot, DPhydDx, DPhydDy);
-      /* add_phy_tend adds zero physics tendencies (dry run without physics). */
+      {
+        double *Tv[5];
+        for (int v = 0; v < 5; ++v) Tv[v] = T + v * NpNeA;
+        oracle_phys_tend_hook(m, Tv, PRES, Rtot, CPtot);
+      }
       oracle_save_massflux_hook(m, q, DPRES, DENS_hyd, PRES_hyd, s, t->b_ex[s - 1], t->b_ex[s - 1], 0);
       for (int v = 0; v < 5; ++v)
         oracle_rk_advance(t, s, n, q[v], var0 + v * NpNeA, varTmp + v * NpNeA, tend + v * NpNeA, NULL,

@@ -186,6 +186,17 @@ void oracle_heve_set_massflux(double *MFLX_x, double *MFLX_y, double *MFLX_z, do
 /* the save_massflux call of one stage when the coupling is set (both update loops) */
 void oracle_save_massflux_hook(const oracle_mesh *m, double *const q[5], const double *DPRES, const double *DENS_hyd,
                                const double *PRES_hyd, int stage, double twh, double twv, int is_hevi);
+/* FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:1098-1178, add_phy_tend with
+ * ENTOT_CONSERVE_SCHEME_FLAG off: T (VID order) += tp (VID order), the THERM
+ * line + RHOH_p / (CPtot EXNER); tp[5] = RHOH_p */
+void oracle_add_phy_tend(const oracle_mesh *m, double *const T[5], const double *const tp[6], const double *PRES,
+                         const double *Rtot, const double *CPtot);
+/* when set (six non-NULL), both update loops call add_phy_tend after the
+ * explicit tendency of every stage (FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:846-856) */
+void oracle_set_phys_tend(const double *tp0, const double *tp1, const double *tp2, const double *tp3,
+                          const double *tp4, const double *tp5);
+void oracle_phys_tend_hook(const oracle_mesh *m, double *const T[5], const double *PRES, const double *Rtot,
+                           const double *CPtot);
 void oracle_heve_update(const oracle_mesh *m, const oracle_rk_tab *t, int nsteps,
                         double *const q[5], double *PRES, double *DPRES,
                         const double *DENS_hyd, const double *PRES_hyd, const double *PRES_hyd_ref,

@@ -59,6 +59,8 @@ def _load():
     lib.oracle_trcadvect_cal_alphdens_advtest.argtypes = [M] + [dp] * 7
     lib.oracle_tracer_modalfilter_apply.argtypes = [M] + [dp] * 5
     lib.oracle_heve_set_massflux.argtypes = [dp] * 5
+    lib.oracle_add_phy_tend.argtypes = [M, dp * 5, dp * 6, dp, dp, dp]
+    lib.oracle_set_phys_tend.argtypes = [dp] * 6
     lib.oracle_trcvar_rk_advance.argtypes = [C.c_long, C.c_int, C.c_int, dp, dp, dp, C.c_double] + [dp] * 7
     lib.oracle_trcadvect_save_massflux.argtypes = [M] + [dp] * 12 + [C.c_int, C.c_double, C.c_double, C.c_int]
     lib.oracle_rk_tab_get.argtypes = [C.c_char_p, C.c_double, C.POINTER(RkTabC)]
@@ -252,6 +254,21 @@ class OracleMesh:
             LIB.oracle_heve_set_massflux(None, None, None, None, None)
         else:
             LIB.oracle_heve_set_massflux(*[_d(a) for a in MFLX], *[_d(a) for a in alph])
+
+    def add_phy_tend(self, T, tp, PRES, Rtot, CPtot):
+        """add_phy_tend in place on T (5 fields, VID order); tp = (DENS_tp, RHOT_tp, MOMZ_tp, MOMX_tp,
+        MOMY_tp, RHOH_p)."""
+        Ta = (dp * 5)(*[_d(a) for a in T])
+        ta = (dp * 6)(*[_d(a) for a in tp])
+        LIB.oracle_add_phy_tend(self.ref, Ta, ta, _d(PRES), _d(Rtot), _d(CPtot))
+
+    def set_phys_tend(self, tp=None):
+        """heve_update / hevi_update add these physics tendencies (DENS_tp, RHOT_tp, MOMZ_tp, MOMX_tp,
+        MOMY_tp, RHOH_p) at every stage (the arrays must stay alive); None: stop."""
+        if tp is None:
+            LIB.oracle_set_phys_tend(None, None, None, None, None, None)
+        else:
+            LIB.oracle_set_phys_tend(*[_d(a) for a in tp])
 
     def tracer_modalfilter_apply(self, QTRC, DENS_hyd, DDENS, MF_h, MF_v):
         """Density-weighted tracer filter in place; MF_h, MF_v (n1,n1) numpy matrices."""

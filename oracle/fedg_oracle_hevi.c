@@ -991,6 +991,11 @@ void oracle_hevi_update(const oracle_mesh *m, const oracle_rk_tab *t, int nsteps
                            Te + MOMZ_VID * NpNeA, Te + RHOT_VID * NpNeA, q[DENS_VID], q[MOMX_VID], q[MOMY_VID],
                            q[MOMZ_VID], q[RHOT_VID], DPRES, DENS_hyd, PRES_hyd, PRES_hyd_ref, THERM_hyd, CORIOLIS,
                            Rtot, CVtot, CPtot, DPhydDx, DPhydDy);
+      {
+        double *Tv[5];
+        for (int v = 0; v < 5; ++v) Tv[v] = Te + v * NpNeA;
+        oracle_phys_tend_hook(m, Tv, PRES, Rtot, CPtot);
+      }
       /* FDS/scale_atm_dyn_dgm_driver_nonhydro3d.F90:899-915, tracer coupling */
       oracle_save_massflux_hook(m, q, DPRES, DENS_hyd, PRES_hyd, s, t->b_ex[s - 1],
                                 t->imex ? t->b_im[s - 1] : t->b_ex[s - 1], 1);
