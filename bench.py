@@ -83,6 +83,8 @@ def cpu_baseline(mesh, bs, q, dt, threads, min_seconds=10.0):
     """The CPU oracle (OpenMP, FP64, the C library's pow) over RK4 steps of the
     same mesh until at least min_seconds of wall time; threads pinned one per
     core (OMP_PROC_BIND=close, OMP_PLACES=cores unless the environment sets them)."""
+    # the process's affinity mask before the OpenMP runtime binds this thread to one core
+    model, ncpu, affinity = host_info()
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
     os.environ.setdefault("OMP_PROC_BIND", "close")
@@ -101,7 +103,6 @@ def cpu_baseline(mesh, bs, q, dt, threads, min_seconds=10.0):
         if sec >= min_seconds:
             break
     dof_stage = 5 * mesh.Np * mesh.Ne * 4 * steps
-    model, ncpu, affinity = host_info()
     return {"value": dof_stage / sec, "unit": "DOF·stage/s", "cores": O.num_threads(), "kind": "port",
             "sample": f"{steps} RK4 steps ({4 * steps} stages) of the same 64x64x16 p=7 mesh, "
                       f"{dof_stage:.3e} DOF·stage, {sec:.2f} s",

@@ -143,17 +143,20 @@ def test_hevi_steps_match_oracle(p, max_it, rtol):
     st.close()
 
 
-def test_hevi_steps_save_massflux_bitwise():
+@pytest.mark.parametrize("p,NeZ,max_it,rtol", [(3, 4, 1, 0.0), (7, 8, 6, 1e-10)],
+                         ids=["p3-nez4-one-iteration", "p7-nez8-newton"])
+def test_hevi_steps_save_massflux_bitwise(p, NeZ, max_it, rtol):
     """The tracer coupling in the HEVI stepper: per-stage save_massflux with
     b_ex / b_im weights and the HEVI damping coefficients, against the
-    oracle's HEVI loop bit for bit (state, mass fluxes, coefficients)."""
-    print("save_massflux in the HEVI stepper p=3", flush=True)
-    case = cases.scaled_case(cases.config4(), 4, 2, 4, porder=3)
+    oracle's HEVI loop bit for bit (state, mass fluxes, coefficients); also at
+    the channel's vertical (NeZ = 8, p = 7) with Newton to a tolerance."""
+    print(f"save_massflux in the HEVI stepper p={p} NeZ={NeZ} max_it={max_it}", flush=True)
+    case = cases.scaled_case(cases.config4(), 4, 2, NeZ, porder=p)
     mesh, bs, q = cases.build(case)
     om = O.OracleMesh(mesh)
     Dx, Dy = om.calc_phyd_hgrad(bs)
     dm = fedg.DeviceMesh(mesh)
-    st = fedg.HeviStepper(dm, bs, Dx, Dy, case.dt, scheme=case.scheme, max_it=1, rtol=0.0)
+    st = fedg.HeviStepper(dm, bs, Dx, Dy, case.dt, scheme=case.scheme, max_it=max_it, rtol=rtol)
     nA, nf, n = mesh.Np * mesh.NeA, mesh.NfpTot * mesh.Ne, mesh.Np * mesh.Ne
     MF = [np.zeros(nA) for _ in range(3)]
     al = [np.zeros(nf), np.zeros(nf)]
@@ -165,7 +168,7 @@ def test_hevi_steps_save_massflux_bitwise():
     qq = [a.copy() for a in q]
     om.heve_set_massflux(MF, al)  # the oracle's coupling hook serves both update loops
     try:
-        om.hevi_update(qq, bs, Dx, Dy, case.dt, 2, scheme=case.scheme, max_it=1, rtol=0.0)
+        om.hevi_update(qq, bs, Dx, Dy, case.dt, 2, scheme=case.scheme, max_it=max_it, rtol=rtol)
     finally:
         om.heve_set_massflux(None)
     g = st.get_state()

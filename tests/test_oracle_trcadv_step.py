@@ -216,3 +216,39 @@ def test_dynamics_step_saves_the_time_averaged_mass_flux():
     vert = np.abs(nz) == 1.0
     c = np.sqrt(gam * bs.PRES_hyd[iM] / dens[iM])
     np.testing.assert_allclose(al[0][vert], (c * dens[iM])[vert], rtol=1e-10)
+
+
+def test_hevi_step_saves_the_time_averaged_mass_flux():
+    """The same steady uniform wind through the HEVI update (IMEX_ARK324): the
+    b_ex weights (lateral) and the b_im weights (vertical) each sum to one, so
+    MFLX = MOM; the damping coefficient takes the HEVI form, Gnn without |nz|:
+    on lateral faces alph_dens = (c + |u.n|) rho, on vertical faces the sound
+    speed drops out and alph_dens = |w| rho = 0 to rounding (the HEVE form
+    would give rho c there)."""
+    from fedg import cases, constants
+    case = cases.scaled_case(cases.config4(), 3, 2, 4, porder=3)
+    mesh, bs, _ = cases.build(case)
+    om = O.OracleMesh(mesh)
+    Dx, Dy = om.calc_phyd_hgrad(bs)
+    U, V = 12.0, -5.0
+    dens = bs.DENS_hyd
+    q = [mesh.field(), mesh.field(), mesh.field(), dens * U, dens * V]
+    nA, nf, n = mesh.Np * mesh.NeA, mesh.NfpTot * mesh.Ne, mesh.Np * mesh.Ne
+    MF = [np.zeros(nA) for _ in range(3)]
+    al = [np.zeros(nf), np.zeros(nf)]
+    om.heve_set_massflux(MF, al)
+    try:
+        om.hevi_update(q, bs, Dx, Dy, case.dt, 1, scheme="IMEX_ARK324", max_it=1, rtol=0.0)
+    finally:
+        om.heve_set_massflux(None)
+    np.testing.assert_allclose(MF[0][:n], dens[:n] * U, rtol=1e-10)
+    np.testing.assert_allclose(MF[1][:n], dens[:n] * V, rtol=1e-10)
+    assert np.abs(MF[2][:n]).max() <= 1e-8 * abs(U) * dens[:n].max()
+    nx, ny, nz = (mesh.normal_fn[k * nf:(k + 1) * nf] for k in range(3))
+    iM = mesh.vmapM - 1
+    gam = constants.SCALE.CPdry / constants.SCALE.CVdry
+    c = np.sqrt(gam * bs.PRES_hyd[iM] / dens[iM])
+    xf, yf, vert = np.abs(nx) == 1.0, np.abs(ny) == 1.0, np.abs(nz) == 1.0
+    np.testing.assert_allclose(al[0][xf], ((c + abs(U)) * dens[iM])[xf], rtol=1e-10)
+    np.testing.assert_allclose(al[0][yf], ((c + abs(V)) * dens[iM])[yf], rtol=1e-10)
+    assert np.abs(al[0][vert]).max() <= 1e-8 * (c * dens[iM])[vert].max()
