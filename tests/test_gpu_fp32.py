@@ -37,7 +37,10 @@ def errors(a, b):
     return rl2, rli
 
 
-@pytest.mark.parametrize("which,tol", [("config1", 1e-5), ("config3", 1e-4)], ids=["100-steps", "one-hour"])
+# measured on MI355X (the kernels are deterministic): largest relative L2 over
+# the prognostics 3.5e-7 after 100 steps (DRHOT) and 1.05e-5 after one hour
+# (MOMX, MOMY); the bounds are about twice that
+@pytest.mark.parametrize("which,tol", [("config1", 7e-7), ("config3", 2.5e-5)], ids=["100-steps", "one-hour"])
 def test_fp32_form_error_against_fp64(which, tol):
     """config1: 100 steps; config3: one simulated hour with the modal filter (both runs)."""
     case = getattr(cases, which)()
@@ -69,4 +72,4 @@ def test_fp32_form_p7_error_against_fp64():
         assert np.all(np.isfinite(a[v])), name
         rl2, rli = errors(a[v], b[v])
         print(f"p=7 100 steps {name}: FP32 form vs FP64 relative L2 {rl2:.2e}, Linf {rli:.2e}")
-        assert rl2 <= 1e-5, name
+        assert rl2 <= 2e-7, name  # measured at most 9.4e-8 (DRHOT)
