@@ -411,4 +411,94 @@ int build_box(const BoxDesc& d, const RefElem& e, BoxMesh& m) {
   return 0;
 }
 
+// Terrain-following coordinate z = zeta + (1 - zeta / zTop) topo
+// (MeshUtil_VCoord_GetMetric, MSH/scale_meshutil_vcoord.F90:81-176):
+//   GsqrtV = dz/dzeta = 1 - topo / zTop,
+//   G13, G23 = -(1 - zeta / zTop) / GsqrtV * d topo / dx, dy,
+// the horizontal gradient of the topography taken as the DG gradient on the
+// 2-D quadrilateral mesh: Escale D topo plus the lift of the central jump
+// 0.5 (topo+ - topo-) n across the element faces (cal_del_flux, :181-209),
+// with the exact-mass lift of the element (its 1-D columns invM1D(:, face
+// node), ELM/scale_element_quadrilateral.F90:218-241); then the metric tensor
+// GIJ (the identity in this Cartesian box) applied as in
+// calc_vcoordinate_metrics_lc (MSH/scale_mesh_topography.F90:213-258), and the
+// halo slots filled by the single-tile exchange before
+// Set_geometric_with_vcoord stores Gsqrt = GsqrtV and GI3
+// (MSH/scale_mesh_cubedom3d.F90:292-334).
+int set_topography(const BoxDesc& d, const RefElem& e, const double* topo, double zTop, BoxMesh& m,
+                   std::vector<double>& zlev) {
+  if (!topo || !(zTop > 0.0) || d.porder != e.p) return 1;
+  const int NeX = d.NeX, NeY = d.NeY, NeZ = d.NeZ, n1 = e.n1, N = n1 - 1, Np = e.Np, Nfp = e.Nfp_v;
+  const int Ne2D = NeX * NeY, Ne = m.Ne;
+  const size_t NpNe = (size_t)Np * Ne;
+  const double dx = (d.xmax - d.xmin) / NeX, dy = (d.ymax - d.ymin) / NeY;
+  const double Ex = 2.0 / dx, Ey = 2.0 / dy;  // Escale(1,1), Escale(2,2) of the 2-D element
+  const double Fx = 2.0 / dx, Fy = 2.0 / dy;  // Fscale of its x and y faces (sJ / J)
+  const double* D = e.D1D.data();
+  const double* iM = e.invM1D.data();
+  // exterior topography value across a face: the neighbour's node (periodic
+  // wrap) or, at a closed boundary, the node itself (the exchange's copy)
+  auto topo_at = [&](int ii, int jj, int a) { return topo[a + (size_t)Nfp * (ii + NeX * jj)]; };
+  std::vector<double> gx((size_t)Nfp * Ne2D), gy((size_t)Nfp * Ne2D);
+  for (int jj = 0; jj < NeY; ++jj)
+    for (int ii = 0; ii < NeX; ++ii) {
+      const int ke2 = ii + NeX * jj;
+      const int iw = ii > 0 ? ii - 1 : (d.periodic_x ? NeX - 1 : -1);
+      const int ie = ii < NeX - 1 ? ii + 1 : (d.periodic_x ? 0 : -1);
+      const int js = jj > 0 ? jj - 1 : (d.periodic_y ? NeY - 1 : -1);
+      const int jn = jj < NeY - 1 ? jj + 1 : (d.periodic_y ? 0 : -1);
+      for (int j = 0; j < n1; ++j)
+        for (int i = 0; i < n1; ++i) {
+          double fx = 0.0, fy = 0.0;
+          for (int l = 0; l < n1; ++l) {
+            fx += D[i + n1 * l] * topo_at(ii, jj, l + n1 * j);
+            fy += D[j + n1 * l] * topo_at(ii, jj, i + n1 * l);
+          }
+          // faces x+ (i = N) and x- (i = 0) along the node's row j, y+ and y- along column i
+          const double tE = ie >= 0 ? topo_at(ie, jj, 0 + n1 * j) : topo_at(ii, jj, N + n1 * j);
+          const double tW = iw >= 0 ? topo_at(iw, jj, N + n1 * j) : topo_at(ii, jj, 0 + n1 * j);
+          const double tN = jn >= 0 ? topo_at(ii, jn, i + n1 * 0) : topo_at(ii, jj, i + n1 * N);
+          const double tS = js >= 0 ? topo_at(ii, js, i + n1 * N) : topo_at(ii, jj, i + n1 * 0);
+          const double dE = 0.5 * (tE - topo_at(ii, jj, N + n1 * j)) * (+1.0);
+          const double dW = 0.5 * (tW - topo_at(ii, jj, 0 + n1 * j)) * (-1.0);
+          const double dN = 0.5 * (tN - topo_at(ii, jj, i + n1 * N)) * (+1.0);
+          const double dS = 0.5 * (tS - topo_at(ii, jj, i + n1 * 0)) * (-1.0);
+          const double liftx = iM[i + n1 * N] * (Fx * dE) + iM[i + n1 * 0] * (Fx * dW);
+          const double lifty = iM[j + n1 * N] * (Fy * dN) + iM[j + n1 * 0] * (Fy * dS);
+          gx[(i + n1 * j) + (size_t)Nfp * ke2] = Ex * fx + liftx;
+          gy[(i + n1 * j) + (size_t)Nfp * ke2] = Ey * fy + lifty;
+        }
+    }
+  zlev.assign((size_t)Np * m.NeA, 0.0);
+  const size_t NpNeA = (size_t)Np * m.NeA;
+  for (int ke = 0; ke < Ne; ++ke) {
+    const int ke2 = m.EMap3Dto2D[ke] - 1;
+    for (int p = 0; p < Np; ++p) {
+      const int h = p % Nfp;  // IndexH2Dto3D
+      const size_t o = p + (size_t)Np * ke;
+      const double zeta = m.pos_en[o + NpNe * 2];
+      const double tp = topo[h + (size_t)Nfp * ke2];
+      double coef = 1.0 - zeta / zTop;
+      zlev[o] = zeta + coef * tp;
+      const double GsqrtV = 1.0 - tp / zTop;
+      coef = -coef / GsqrtV;
+      const double G13 = coef * gx[h + (size_t)Nfp * ke2];
+      const double G23 = coef * gy[h + (size_t)Nfp * ke2];
+      m.Gsqrt[o] = GsqrtV;
+      // GIJ = identity: tmp_G13 = G11 G13 + G12 G23, tmp_G23 = G21 G13 + G22 G23
+      m.GI3[o] = 1.0 * G13 + 0.0 * G23;
+      m.GI3[o + NpNeA] = 0.0 * G13 + 1.0 * G23;
+    }
+  }
+  for (int b = 0; b < m.nB; ++b) {
+    const size_t dst = NpNe + b, src = (size_t)(m.halo_src[b] - 1);
+    zlev[dst] = zlev[src];
+    m.Gsqrt[dst] = m.Gsqrt[src];
+    m.GI3[dst] = m.GI3[src];
+    m.GI3[dst + NpNeA] = m.GI3[src + NpNeA];
+  }
+  (void)NeZ;
+  return 0;
+}
+
 }  // namespace fedg

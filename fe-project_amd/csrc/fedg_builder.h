@@ -59,4 +59,11 @@ struct BoxMesh {
 
 int build_box(const BoxDesc& d, const RefElem& e, BoxMesh& m);
 
+// Terrain-following vertical coordinate over a topography topo (Nfp_v, Ne2D)
+// of the box's horizontal mesh and a model top zTop: GsqrtV, G13, G23 into
+// m.Gsqrt and m.GI3 (halo slots filled as the single-tile exchange fills
+// them) and zlev (Np, NeA), the height of every node.  m comes from build_box.
+int set_topography(const BoxDesc& d, const RefElem& e, const double* topo, double zTop, BoxMesh& m,
+                   std::vector<double>& zlev);
+
 }  // namespace fedg

@@ -373,6 +373,41 @@ int fedg_box_build(const fedg_box_desc* d, fedg_box_arrays* out) {
   return FEDG_OK;
 }
 
+int fedg_box_topography_metrics(const fedg_box_desc* d, const double* topo, double zTop, double* Gsqrt, double* GI3,
+                                double* zlev) {
+  if (!d || !topo || !(zTop > 0.0)) return FEDG_ERR_ARG;
+  fedg_box_sizes s;
+  TRY(fedg_box_get_sizes(d, &s));
+  if (!(d->zmin == 0.0) || !(zTop >= d->zmax)) {
+    set_error("terrain-following coordinate: the box must start at zeta = 0 and zTop must be at or above its top");
+    return FEDG_ERR_ARG;
+  }
+  RefElem e;
+  BoxMesh m;
+  std::vector<double> zl;
+  try {
+    build_refelem(d->porder, e);
+    BoxDesc bd{d->porder, d->NeX, d->NeY, d->NeZ, d->xmin, d->xmax, d->ymin, d->ymax, d->zmin, d->zmax,
+               d->periodic_x, d->periodic_y, d->periodic_z};
+    if (build_box(bd, e, m) != 0 || set_topography(bd, e, topo, zTop, m, zl) != 0) {
+      set_error("terrain-following metrics: construction failed");
+      return FEDG_ERR_ARG;
+    }
+  } catch (const std::exception& ex) {
+    set_error(ex.what());
+    return FEDG_ERR_ARG;
+  }
+  for (size_t t = 0; t < m.Gsqrt.size(); ++t)
+    if (!(m.Gsqrt[t] > 0.0)) {
+      set_error("terrain-following coordinate: the topography reaches zTop (GsqrtV <= 0)");
+      return FEDG_ERR_ARG;
+    }
+  if (Gsqrt) memcpy(Gsqrt, m.Gsqrt.data(), m.Gsqrt.size() * sizeof(double));
+  if (GI3) memcpy(GI3, m.GI3.data(), m.GI3.size() * sizeof(double));
+  if (zlev) memcpy(zlev, zl.data(), zl.size() * sizeof(double));
+  return FEDG_OK;
+}
+
 // ---------------------------------------------------------------------------
 int fedg_ctx_create(const fedg_mesh_desc* d, fedg_ctx** out) {
   if (!d || !out) return FEDG_ERR_ARG;

@@ -40,7 +40,7 @@ def calc_rhot_hyd(PRES_hyd, c=constants.SCALE):
 def base_state(mesh: BoxMesh, kind="constT", T0=300.0, PT0=300.0, BVF=0.01, Psfc=1.0e5,
                c=constants.SCALE) -> BaseState:
     n = mesh.Np * mesh.Ne
-    z = mesh.z
+    z = mesh.height
     D = mesh.field()
     P = mesh.field()
     if kind == "constT":
@@ -78,7 +78,7 @@ def cosine_bell(mesh: BoxMesh, amp, r, center):
     """qmax * (0.5 (1 + cos(pi r)))  for r <= 1 (mkinitutil_calc_cosinebell, exponent 1)."""
     rx = ry = rz = r
     xc, yc, zc = center
-    rr = np.sqrt(((mesh.x - xc) / rx) ** 2 + ((mesh.y - yc) / ry) ** 2 + ((mesh.z - zc) / rz) ** 2)
+    rr = np.sqrt(((mesh.x - xc) / rx) ** 2 + ((mesh.y - yc) / ry) ** 2 + ((mesh.height - zc) / rz) ** 2)
     return np.where(rr <= 1.0, amp * (0.5 * (1.0 + np.cos(np.pi * rr))), 0.0)
 
 

@@ -87,7 +87,34 @@ class BoxMesh:
         check(LIB.fedg_box_build(C.byref(self.desc), C.byref(arr)), "fedg_box_build")
         self.elem = RefElem(porder)
 
+    def set_topography(self, topo, zTop=None):
+        """Terrain-following vertical coordinate over a topography (MeshTopography_set_vcoordinate,
+        MSH/scale_mesh_topography.F90:159-258): topo is a callable f(x, y) evaluated at the horizontal
+        nodes or an (Nfp_v*Ne2D,) array; zTop defaults to the box top.  Replaces Gsqrt and GI3 by the
+        coordinate's metrics and sets zlev, the height of every node (``height``)."""
+        zTop = float(self.desc.zmax if zTop is None else zTop)
+        n2 = self.Nfp_v * self.Ne2D
+        if callable(topo):
+            xs = self.x[: self.Np * self.Ne2D].reshape(self.Ne2D, self.Np)[:, : self.Nfp_v].ravel()
+            ys = self.y[: self.Np * self.Ne2D].reshape(self.Ne2D, self.Np)[:, : self.Nfp_v].ravel()
+            t = np.ascontiguousarray(topo(xs, ys), dtype=np.float64)
+        else:
+            t = np.ascontiguousarray(topo, dtype=np.float64)
+        if t.size != n2:
+            raise ValueError(f"topography needs {n2} values (Nfp_v*Ne2D)")
+        zlev = np.zeros(self.Np * self.NeA)
+        check(LIB.fedg_box_topography_metrics(C.byref(self.desc), _dp(t), zTop, _dp(self.Gsqrt), _dp(self.GI3),
+                                              _dp(zlev)), "fedg_box_topography_metrics")
+        self.topo, self.zTop, self.zlev = t, zTop, zlev
+        return self
+
     # coordinates of the nodes, (Np*Ne,) each
+    @property
+    def height(self):
+        """Height of the nodes: zlev over a topography, else the box's z."""
+        z = getattr(self, "zlev", None)
+        return self.z if z is None else z[: self.Np * self.Ne]
+
     @property
     def x(self):
         return self.pos_en[: self.Np * self.Ne]

@@ -56,6 +56,7 @@ module fedg_c_binding
   public :: fedg_dev_alloc, fedg_dev_free, fedg_memcpy_h2d, fedg_memcpy_d2h, fedg_memset_dev
   public :: fedg_device_sync, fedg_device_name, fedg_stage_probe
   public :: fedg_refelem_tables, fedg_modalfilter_matrix, fedg_box_get_sizes, fedg_box_build
+  public :: fedg_box_topography_metrics
   public :: fedg_ctx_create, fedg_ctx_destroy, fedg_ctx_setup_modalfilter, fedg_modalfilter_apply
   public :: fedg_ctx_sizes, fedg_ctx_is_flat
   public :: fedg_drhot2pres, fedg_halo_exchange, fedg_apply_bc_prgvars
@@ -166,6 +167,17 @@ module fedg_c_binding
       type(fedg_box_desc), intent(in) :: desc
       type(fedg_box_arrays), intent(in) :: out
     end function fedg_box_build
+
+    !> terrain-following metrics of the box (MSH/scale_mesh_topography.F90:159-258,
+    !! MSH/scale_meshutil_vcoord.F90:81-209); outputs may be c_null_ptr
+    integer(c_int) function fedg_box_topography_metrics(desc, topo, zTop, Gsqrt, GI3, zlev) &
+        bind(C, name="fedg_box_topography_metrics")
+      import :: c_int, c_ptr, c_double, fedg_box_desc
+      type(fedg_box_desc), intent(in) :: desc
+      real(c_double), intent(in) :: topo(*)
+      real(c_double), value :: zTop
+      type(c_ptr), value :: Gsqrt, GI3, zlev
+    end function fedg_box_topography_metrics
 
     ! ---- context ---------------------------------------------------------------
     integer(c_int) function fedg_ctx_create(desc, ctx) bind(C, name="fedg_ctx_create")

@@ -125,6 +125,20 @@ typedef struct {
 
 int fedg_box_get_sizes(const fedg_box_desc *desc, fedg_box_sizes *sizes);
 int fedg_box_build(const fedg_box_desc *desc, fedg_box_arrays *out);
+/* Terrain-following vertical coordinate of the box (host): MeshTopography_set_vcoordinate
+ * (MSH/scale_mesh_topography.F90:159-258) with MeshUtil_VCoord_GetMetric
+ * (MSH/scale_meshutil_vcoord.F90:81-209, TERRAIN_FOLLOWING) and
+ * Set_geometric_with_vcoord (MSH/scale_mesh_cubedom3d.F90:292-334).  The box
+ * of desc is the computational one (zeta from zmin = 0 to zmax <= zTop);
+ * topo (Nfp_v,Ne2D) holds the topography at the 2-D nodes of the horizontal
+ * mesh (IndexH2Dto3D order).  Outputs (any may be NULL): Gsqrt (Np,NeA) =
+ * GsqrtV = 1 - topo/zTop; GI3 (Np,NeA,2) = G13, G23 = -(1 - zeta/zTop) /
+ * GsqrtV d topo/dx, dy with the DG gradient of topo (central face jumps
+ * lifted); zlev (Np,NeA) = zeta + (1 - zeta/zTop) topo.  Halo slots hold the
+ * single-tile exchange's values.  The other metrics of fedg_box_build
+ * (Escale, normals, Fscale, GsqrtH) are unchanged by the vertical coordinate. */
+int fedg_box_topography_metrics(const fedg_box_desc *desc, const double *topo, double zTop, double *Gsqrt,
+                                double *GI3, double *zlev);
 
 /* ------------------------------------------------------------------ */
 /* Context: flattened LocalMesh3D + ElementBase3D + ElementOperationTensorProd3D,
