@@ -914,6 +914,74 @@ int fedg_trcadvect_tmar(fedg_ctx* ctx, double* QTRC_, const double* DENS_hyd, co
   return FEDG_OK;
 }
 
+static int trc_thermo(int nq, const double* Mq, const double* Rq, const double* CVq, const double* CPq,
+                      TrcThermo& th) {
+  if (nq < 1 || nq > kMaxTrc || !Mq || !Rq || !CVq || !CPq) {
+    set_error("water species: 1 to " + std::to_string(kMaxTrc) + " tracers with their TRACER_MASS, _R, _CV, _CP");
+    return FEDG_ERR_ARG;
+  }
+  th = TrcThermo{};
+  th.nq = nq;
+  for (int iq = 0; iq < nq; ++iq) {
+    th.Mq[iq] = Mq[iq];
+    th.Rq[iq] = Rq[iq];
+    th.CVq[iq] = CVq[iq];
+    th.CPq[iq] = CPq[iq];
+  }
+  return FEDG_OK;
+}
+
+int fedg_tracer_specific_heat(fedg_ctx* ctx, int nq, double* const* QTRC, const double* TRACER_MASS,
+                              const double* TRACER_R, const double* TRACER_CV, const double* TRACER_CP, double* Rtot,
+                              double* CVtot, double* CPtot) {
+  if (!ctx || !QTRC || !Rtot || !CVtot || !CPtot) return FEDG_ERR_ARG;
+  TrcThermo th;
+  TRY(trc_thermo(nq, TRACER_MASS, TRACER_R, TRACER_CV, TRACER_CP, th));
+  const double* Q[kMaxTrc] = {};
+  for (int iq = 0; iq < nq; ++iq) {
+    if (!QTRC[iq]) return FEDG_ERR_ARG;
+    Q[iq] = QTRC[iq];
+  }
+  TRY(launch_specific_heat(ctx, Q, th, Rtot, CVtot, CPtot));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return FEDG_OK;
+}
+
+int fedg_trc_negative_fixer(fedg_ctx* ctx, int nq, double* const* QTRC, double* DDENS, double* PRES, double* CVtot,
+                            double* CPtot, double* Rtot, const double* DENS_hyd, const double* PRES_hyd,
+                            const double* J, const double* zlev, double dt, int QLA, int QIA,
+                            const double* TRACER_MASS, const double* TRACER_R, const double* TRACER_CV,
+                            const double* TRACER_CP, double* DRHOT) {
+  (void)dt;  // the reference takes dt and does not use it
+  if (!ctx || !QTRC || !DDENS || !PRES || !CVtot || !CPtot || !Rtot || !DENS_hyd || !PRES_hyd || !J || !zlev)
+    return FEDG_ERR_ARG;
+  if (QLA < 0 || QIA < 0 || nq != 1 + QLA + QIA) {
+    set_error("negative fixer: the tracers are the water species, nq = 1 + QLA + QIA (the reference leaves "
+              "any further tracer undefined)");
+    return FEDG_ERR_ARG;
+  }
+  NegFix a{};
+  TRY(trc_thermo(nq, TRACER_MASS, TRACER_R, TRACER_CV, TRACER_CP, a.th));
+  for (int iq = 0; iq < nq; ++iq) {
+    if (!QTRC[iq]) return FEDG_ERR_ARG;
+    a.Q[iq] = QTRC[iq];
+  }
+  a.DD = DDENS;
+  a.PRES = PRES;
+  a.CV = CVtot;
+  a.CP = CPtot;
+  a.R = Rtot;
+  a.DRHOT = DRHOT;
+  a.Dh = DENS_hyd;
+  a.Ph = PRES_hyd;
+  a.J = J;
+  a.zlev = zlev;
+  TRY(trc_weights(ctx, a.w));
+  TRY(launch_negative_fixer(ctx, a));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return FEDG_OK;
+}
+
 int fedg_trcadvect_save_massflux(fedg_ctx* ctx, double* MFLX_x_tavg, double* MFLX_y_tavg, double* MFLX_z_tavg,
                                  double* alph_dens_M, double* alph_dens_P, const double* DDENS, const double* MOMX,
                                  const double* MOMY, const double* MOMZ, const double* DPRES, const double* DENS_hyd,

@@ -157,6 +157,24 @@ struct TrcRk {
   int first, last, acc;
 };
 int launch_trc_rk(fedg_ctx* ctx, const TrcRk& r);
+// water species of the coupled dynamics + tracer step (at most kMaxTrc)
+constexpr int kMaxTrc = 8;
+struct TrcThermo {
+  int nq;                                                  // tracers
+  double Mq[kMaxTrc], Rq[kMaxTrc], CVq[kMaxTrc], CPq[kMaxTrc];  // TRACER_MASS, _R, _CV, _CP
+};
+// ATMOS_THERMODYN_specific_heat on every node of elements 1..Ne
+int launch_specific_heat(fedg_ctx* ctx, const double* const Q[kMaxTrc], const TrcThermo& th, double* Rtot,
+                         double* CVtot, double* CPtot);
+struct NegFix {
+  double* Q[kMaxTrc];
+  double *DD, *PRES, *CV, *CP, *R, *DRHOT;  // DRHOT may be null
+  const double *Dh, *Ph, *J, *zlev;
+  TrcThermo th;
+  TrcW w;
+};
+// atm_phy_mp_dgm_common_negative_fixer on elements 1..Ne
+int launch_negative_fixer(fedg_ctx* ctx, const NegFix& a);
 int launch_tracer_modalfilter(fedg_ctx* ctx, double* Q, const double* Dh, const double* DD, const double* MFh,
                               const double* MFv);
 int launch_heve_fused(fedg_ctx* ctx, const TendIn& in, const RkOut& rk, int use_flat);

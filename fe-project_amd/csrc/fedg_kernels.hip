@@ -1428,6 +1428,136 @@ __global__ void __launch_bounds__(EPB * N1 * N1 * N1)
   if (valid) stg(Q, o, sr[e] * qp);
 }
 
+// ATMOS_THERMODYN_specific_heat of the SCALE library (scale_atmos_thermodyn,
+// not part of the reference tree; the published mixing rule): qdry = 1 -
+// sum q_i M_i, Rtot = sum q_i R_i + qdry Rdry, likewise CVtot and CPtot, the
+// sums over the tracers in order.
+__device__ __forceinline__ void specific_heat(const double* q, const TrcThermo& th, const Consts& c, double& R,
+                                              double& CV, double& CP) {
+  double qdry = 1.0, r = 0.0, cv = 0.0, cp = 0.0;
+  for (int iq = 0; iq < th.nq; ++iq) {
+    qdry = qdry - q[iq] * th.Mq[iq];
+    r = r + q[iq] * th.Rq[iq];
+    cv = cv + q[iq] * th.CVq[iq];
+    cp = cp + q[iq] * th.CPq[iq];
+  }
+  R = r + qdry * c.Rdry;
+  CV = cv + qdry * c.CVdry;
+  CP = cp + qdry * c.CPdry;
+}
+
+struct SpecHeat {
+  const double* Q[kMaxTrc];
+  double *R, *CV, *CP;
+  TrcThermo th;
+};
+__global__ void __launch_bounds__(256) specific_heat_kernel(const SpecHeat a, const Consts c, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double q[kMaxTrc];
+  for (int iq = 0; iq < a.th.nq; ++iq) q[iq] = a.Q[iq][t];
+  double R, CV, CP;
+  specific_heat(q, a.th, c, R, CV, CP);
+  a.R[t] = R;
+  a.CV[t] = CV;
+  a.CP[t] = CP;
+}
+
+// atm_phy_mp_dgm_common_negative_fixer
+// (FElib/src/microphysics/scale_atm_phy_mp_dgm_common.F90:429-556), one
+// element per workgroup, one node per thread; every element sum is taken
+// over the nodes in order by one thread, as Fortran's sum over p = 1..Np.
+// The water species are all nq tracers (the reference's loop over
+// 1 + QLA + QIA; its later loop over QA reads only those).
+template <int N1, int EPB>
+__global__ void __launch_bounds__(EPB * N1 * N1 * N1) negfix_kernel(const DevMesh m, const Consts c, const NegFix a) {
+  constexpr int Np = N1 * N1 * N1;
+  constexpr double TRC_EPS = 1.0e-128;
+  __shared__ double s0[EPB][Np], s1[EPB][Np];
+  __shared__ double sr[EPB][2];
+  const int tid = threadIdx.x;
+  const int e = tid / Np;
+  const int p = tid - e * Np;
+  const int ke = blockIdx.x * EPB + e;
+  const bool valid = ke < m.Ne;
+  const int i = p % N1, j = (p / N1) % N1, k = p / (N1 * N1);
+  const uint32_t o = (uint32_t)Np * (valid ? ke : 0) + p;
+  const int nq = a.th.nq;
+  double DENS = 0.0, DD0 = 0.0, DD = 0.0, IE0 = 0.0, TEMP = 0.0, IE = 0.0, intw = 0.0;
+  double Dh = 0.0;
+  double tm1[kMaxTrc];
+  if (valid) {
+    Dh = ldg(a.Dh, o);
+    DD = ldg(a.DD, o);
+    DENS = Dh + DD;
+    DD0 = DD;
+    const double CV = ldg(a.CV, o), P = ldg(a.PRES, o), R = ldg(a.R, o);
+    IE0 = CV * P / R;
+    TEMP = IE0 / (DENS * CV);
+    IE = IE0;
+    intw = ldg(m.Gsqrt, o) * ldg(a.J, o) * ((a.w.w[i] * a.w.w[j]) * a.w.w[k]);
+  }
+  auto esum = [&](double v0, double v1, double& r0, double& r1) {
+    s0[e][p] = v0;
+    s1[e][p] = v1;
+    __syncthreads();
+    if (valid && p == 0) {
+      double q0 = 0.0, q1 = 0.0;
+#pragma unroll 16
+      for (int n = 0; n < Np; ++n) {
+        q0 = q0 + s0[e][n];
+        q1 = q1 + s1[e][n];
+      }
+      sr[e][0] = q0;
+      sr[e][1] = q1;
+    }
+    __syncthreads();
+    r0 = sr[e][0];
+    r1 = sr[e][1];
+    __syncthreads();
+  };
+  for (int iq = 0; iq < nq; ++iq) {
+    double tm0 = 0.0, t1 = 0.0;
+    if (valid) {
+      tm0 = DENS * ldg(a.Q[iq], o);
+      t1 = tm0 > TRC_EPS ? tm0 : TRC_EPS;
+    }
+    double M0, M1;
+    esum(intw * tm0, intw * t1, M0, M1);
+    if (valid) {
+      t1 = (M0 > 0.0 ? M0 : 0.0) / M1 * t1;
+      DD = DD + (t1 - tm0);
+      IE = IE + (t1 - tm0) * a.th.CVq[iq] * TEMP;
+    }
+    tm1[iq] = t1;
+  }
+  double qn[kMaxTrc];
+  if (valid) {
+    DENS = Dh + DD;
+    for (int iq = 0; iq < nq; ++iq) {
+      qn[iq] = tm1[iq] / DENS;
+      stg(a.Q[iq], o, qn[iq]);
+    }
+  }
+  double E0, E1;
+  esum(intw * IE0, intw * IE, E0, E1);
+  if (!valid) return;
+  IE = E0 / E1 * IE;
+  double R, CV, CP;
+  specific_heat(qn, a.th, c, R, CV, CP);
+  stg(a.R, o, R);
+  stg(a.CV, o, CV);
+  stg(a.CP, o, CP);
+  IE = IE - (DD - DD0) * c.GRAV * ldg(a.zlev, o);
+  const double P = IE * R / CV;
+  stg(a.PRES, o, P);
+  stg(a.DD, o, DD);
+  if (a.DRHOT) {
+    stg(a.DRHOT, o, c.PRES00 / R * fedg_pow(P / c.PRES00, CV / CP) -
+                        c.PRES00 / c.Rdry * fedg_pow(ldg(a.Ph, o) / c.PRES00, c.CVdry / c.CPdry));
+  }
+}
+
 // save_massflux (:340-404): time-weighted mass fluxes over the RK stages,
 // and the damping coefficients of cal_alphdens_dyn (:447-536) accumulated
 // per face node.
@@ -1920,6 +2050,33 @@ int launch_tmar(fedg_ctx* ctx, double* Q, const double* Dh, const double* DD, co
     const int nblk = (m.Ne + EPB - 1) / EPB;
     hipLaunchKernelGGL((tmar_kernel<N1, EPB>), dim3(nblk), dim3(EPB * N1 * N1 * N1), 0, ctx->stream, m, Q, Dh, DD, J,
                        w);
+    FEDG_HIP_CHECK(hipGetLastError());
+    return FEDG_OK;
+  });
+}
+
+int launch_specific_heat(fedg_ctx* ctx, const double* const Q[kMaxTrc], const TrcThermo& th, double* Rtot,
+                         double* CVtot, double* CPtot) {
+  const long n = (long)ctx->m.Np * ctx->m.Ne;
+  SpecHeat a{};
+  for (int iq = 0; iq < th.nq; ++iq) a.Q[iq] = Q[iq];
+  a.R = Rtot;
+  a.CV = CVtot;
+  a.CP = CPtot;
+  a.th = th;
+  hipLaunchKernelGGL(specific_heat_kernel, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream, a, ctx->c, n);
+  FEDG_HIP_CHECK(hipGetLastError());
+  return FEDG_OK;
+}
+
+int launch_negative_fixer(fedg_ctx* ctx, const NegFix& a) {
+  const DevMesh& m = ctx->m;
+  return dispatch_n1(m.n1, [&](auto n1c) -> int {
+    constexpr int N1 = decltype(n1c)::value;
+    constexpr int EPB = epb_for<N1>();
+    const int nblk = (m.Ne + EPB - 1) / EPB;
+    hipLaunchKernelGGL((negfix_kernel<N1, EPB>), dim3(nblk), dim3(EPB * N1 * N1 * N1), 0, ctx->stream, m, ctx->c,
+                       a);
     FEDG_HIP_CHECK(hipGetLastError());
     return FEDG_OK;
   });

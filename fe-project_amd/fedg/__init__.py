@@ -10,6 +10,7 @@ from .dyncore import (VARS, DevField, DevField32, HeveStepper, HeviStepper, Mlp,
                       apply_bc,
                       calc_phyd_hgrad, calc_pressure, device_name, halo_exchange, heve_add_bnd_contrib, heve_cal_tend,
                       heve_cal_tend_cco, hevi_cal_tend, trcadvect_cal_tend,
-                      trcadvect_calc_fct_coef, trcadvect_save_massflux, trcadvect_tmar,
+                      trcadvect_calc_fct_coef, trcadvect_save_massflux, trcadvect_tmar, tracer_specific_heat,
+                      trc_negative_fixer,
                       hevi_cal_vi, hevi_cal_vi_guess, modalfilter_apply, modalfilter_matrix, stage_probe)
 from .mesh import BND_NONE, BND_NOSLIP, BND_SLIP, BoxMesh, DeviceMesh, RefElem  # noqa: F401

@@ -140,6 +140,33 @@ def trcadvect_tmar(dm: DeviceMesh, QTRC, DENS_hyd, DDENS, J):
     check(LIB.fedg_trcadvect_tmar(dm.handle, _p(QTRC), _p(DENS_hyd), _p(DDENS), _p(J)), "fedg_trcadvect_tmar")
 
 
+def _thermo(thermo, nq):
+    th = [np.ascontiguousarray(t, dtype=np.float64) for t in thermo]
+    if len(th) != 4 or any(t.size != nq for t in th):
+        raise ValueError("thermo = (TRACER_MASS, TRACER_R, TRACER_CV, TRACER_CP), one value per tracer each")
+    return th
+
+
+def tracer_specific_heat(dm: DeviceMesh, Q, thermo, Rtot, CVtot, CPtot):
+    """Rtot, CVtot, CPtot of the mixture of the tracers Q (device fields), as update_pressure_specific_heat
+    (FDS/scale_atm_dyn_dgm_driver_trcadv3d.F90:653-716); thermo = (TRACER_MASS, _R, _CV, _CP)."""
+    th = _thermo(thermo, len(Q))
+    arr = (C.c_void_p * len(Q))(*[_p(a) for a in Q])
+    check(LIB.fedg_tracer_specific_heat(dm.handle, len(Q), arr, *[t.ctypes.data_as(C.c_void_p) for t in th],
+                                        _p(Rtot), _p(CVtot), _p(CPtot)), "fedg_tracer_specific_heat")
+
+
+def trc_negative_fixer(dm: DeviceMesh, Q, DDENS, PRES, CVtot, CPtot, Rtot, DENS_hyd, PRES_hyd, J, zlev, thermo,
+                       QLA=0, QIA=0, DRHOT=None, dt=0.0):
+    """fix_negative_val of the coupled tracer step (atm_phy_mp_dgm_common_negative_fixer), in place."""
+    th = _thermo(thermo, len(Q))
+    arr = (C.c_void_p * len(Q))(*[_p(a) for a in Q])
+    check(LIB.fedg_trc_negative_fixer(dm.handle, len(Q), arr, _p(DDENS), _p(PRES), _p(CVtot), _p(CPtot), _p(Rtot),
+                                      _p(DENS_hyd), _p(PRES_hyd), _p(J), _p(zlev), float(dt), int(QLA), int(QIA),
+                                      *[t.ctypes.data_as(C.c_void_p) for t in th],
+                                      None if DRHOT is None else _p(DRHOT)), "fedg_trc_negative_fixer")
+
+
 def trcadvect_save_massflux(dm: DeviceMesh, MFLX_x, MFLX_y, MFLX_z, alph_dens_M, alph_dens_P, DDENS, MOMX, MOMY, MOMZ,
                             DPRES, DENS_hyd, PRES_hyd, Rtot, CVtot, CPtot, rkstage, tavg_weight_h, tavg_weight_v,
                             is_hevi=False):

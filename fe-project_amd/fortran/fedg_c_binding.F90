@@ -64,6 +64,7 @@ module fedg_c_binding
   public :: fedg_trcadvect_cal_tend, fedg_trcadvect_calc_fct_coef, fedg_trcadvect_tmar
   public :: fedg_trcadvect_save_massflux, fedg_trcadvect_cal_alphdens_advtest
   public :: fedg_tracer_modalfilter_apply, fedg_trcvar_rk_advance
+  public :: fedg_tracer_specific_heat, fedg_trc_negative_fixer
   public :: fedg_hevi_cal_tend, fedg_hevi_cal_vi, fedg_hevi_cal_vi_newton, fedg_hevi_cal_vi_noalloc
   public :: fedg_calc_phyd_hgrad, fedg_add_phy_tend, fedg_heve_set_phys_tend, fedg_hevi_set_phys_tend
   public :: fedg_tint_create, fedg_tint_destroy, fedg_tint_info, fedg_tint_tend_ex, fedg_tint_tend_im
@@ -334,6 +335,36 @@ module fedg_c_binding
       real(c_double), value :: tavg_weight_h, tavg_weight_v
       integer(c_int), value :: is_hevi
     end function fedg_trcadvect_save_massflux
+
+    !> specific heats of the mixture, update_pressure_specific_heat
+    !! (FDS/scale_atm_dyn_dgm_driver_trcadv3d.F90:653-716); QTRC: device addresses of nq fields
+    integer(c_int) function fedg_tracer_specific_heat(ctx, nq, QTRC, TRACER_MASS, TRACER_R, TRACER_CV, TRACER_CP, &
+        Rtot, CVtot, CPtot) bind(C, name="fedg_tracer_specific_heat")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nq
+      type(c_ptr), intent(in) :: QTRC(*)
+      real(c_double), intent(in) :: TRACER_MASS(*), TRACER_R(*), TRACER_CV(*), TRACER_CP(*)
+      real(c_double) :: Rtot(*), CVtot(*), CPtot(*)
+    end function fedg_tracer_specific_heat
+
+    !> fix_negative_val (FDS/scale_atm_dyn_dgm_driver_trcadv3d.F90:596-651) with
+    !! atm_phy_mp_dgm_common_negative_fixer (FElib/src/microphysics/scale_atm_phy_mp_dgm_common.F90:429-556);
+    !! DRHOT may be c_null_ptr
+    integer(c_int) function fedg_trc_negative_fixer(ctx, nq, QTRC, DDENS, PRES, CVtot, CPtot, Rtot, DENS_hyd, &
+        PRES_hyd, J, zlev, dt, QLA, QIA, TRACER_MASS, TRACER_R, TRACER_CV, TRACER_CP, DRHOT) &
+        bind(C, name="fedg_trc_negative_fixer")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nq
+      type(c_ptr), intent(in) :: QTRC(*)
+      real(c_double) :: DDENS(*), PRES(*), CVtot(*), CPtot(*), Rtot(*)
+      real(c_double), intent(in) :: DENS_hyd(*), PRES_hyd(*), J(*), zlev(*)
+      real(c_double), value :: dt
+      integer(c_int), value :: QLA, QIA
+      real(c_double), intent(in) :: TRACER_MASS(*), TRACER_R(*), TRACER_CV(*), TRACER_CP(*)
+      type(c_ptr), value :: DRHOT
+    end function fedg_trc_negative_fixer
 
     !> atm_dyn_dgm_trcadvect3d_heve_cal_alphdens_advtest (FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:407-445)
     integer(c_int) function fedg_trcadvect_cal_alphdens_advtest(ctx, alph_dens_M, alph_dens_P, DDENS_, MOMX_, &

@@ -319,6 +319,35 @@ int fedg_trcadvect_save_massflux(fedg_ctx *ctx, double *MFLX_x_tavg, double *MFL
                                  const double *CPtot, int rkstage, double tavg_weight_h,
                                  double tavg_weight_v, int is_hevi);
 
+/* Coupled dynamics + tracer step, after the tracer update
+ * (FDS/scale_atm_dyn_dgm_driver_trcadv3d.F90:540-553):
+ *  - fedg_tracer_specific_heat = the specific-heat part of
+ *    update_pressure_specific_heat (:653-716): ATMOS_THERMODYN_specific_heat of
+ *    the SCALE library (not in the reference tree; its mixing rule qdry = 1 -
+ *    sum q_i TRACER_MASS_i, Rtot = sum q_i TRACER_R_i + qdry Rdry, likewise CVtot
+ *    and CPtot) on every node of elements 1..Ne; the pressure follows with
+ *    fedg_drhot2pres, as the driver's calc_pressure;
+ *  - fedg_trc_negative_fixer = fix_negative_val (:596-651) with
+ *    atm_phy_mp_dgm_common_negative_fixer
+ *    (FElib/src/microphysics/scale_atm_phy_mp_dgm_common.F90:429-556): per
+ *    element, each water species' negative mass removed with the element's
+ *    mass kept (max(M0, 0) / M1 scaling), DDENS, the internal energy (kept as
+ *    an element integral, then the potential-energy change Grav zlev
+ *    removed), PRES, Rtot, CVtot, CPtot and, when DRHOT is not NULL, DRHOT
+ *    updated in place.
+ * QTRC: nq device (Np,NeA) fields (1 <= nq <= 8); TRACER_MASS/_R/_CV/_CP:
+ * nq host values each; J (Np,Ne) lcmesh%J; zlev (Np,Ne) the node heights; the
+ * fixer's tracers are the water species (nq = 1 + QLA + QIA); dt is accepted
+ * and unused, as in the reference. */
+int fedg_tracer_specific_heat(fedg_ctx *ctx, int nq, double *const *QTRC, const double *TRACER_MASS,
+                              const double *TRACER_R, const double *TRACER_CV, const double *TRACER_CP,
+                              double *Rtot, double *CVtot, double *CPtot);
+int fedg_trc_negative_fixer(fedg_ctx *ctx, int nq, double *const *QTRC, double *DDENS, double *PRES,
+                            double *CVtot, double *CPtot, double *Rtot, const double *DENS_hyd,
+                            const double *PRES_hyd, const double *J, const double *zlev, double dt, int QLA,
+                            int QIA, const double *TRACER_MASS, const double *TRACER_R, const double *TRACER_CV,
+                            const double *TRACER_CP, double *DRHOT);
+
 /* atm_dyn_dgm_trcadvect3d_heve_cal_alphdens_advtest
  * (FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:407-445): the damping
  * coefficients of the tracer-only advection test, alpha = max(|u.n| of both

@@ -652,6 +652,98 @@ void oracle_trcadvect_tmar(const oracle_mesh *m, const double *w1d, double *QTRC
   }
 }
 
+/* SCALE's ATMOS_THERMODYN_specific_heat (scale_atmos_thermodyn, not in the
+ * reference tree): the published mixing rule, the tracer sums in order */
+static void spec_heat(int nq, const double *q, const double *Mq, const double *Rq, const double *CVq,
+                      const double *CPq, const oracle_mesh *m, double *R, double *CV, double *CP) {
+  double qdry = 1.0, r = 0.0, cv = 0.0, cp = 0.0;
+  for (int iq = 0; iq < nq; ++iq) {
+    qdry = qdry - q[iq] * Mq[iq];
+    r = r + q[iq] * Rq[iq];
+    cv = cv + q[iq] * CVq[iq];
+    cp = cp + q[iq] * CPq[iq];
+  }
+  *R = r + qdry * m->Rdry;
+  *CV = cv + qdry * m->CVdry;
+  *CP = cp + qdry * m->CPdry;
+}
+
+/* FDS/scale_atm_dyn_dgm_driver_trcadv3d.F90:653-716 (update_pressure_specific_heat,
+ * its specific-heat part) over the nodes of elements 1..Ne */
+void oracle_tracer_specific_heat(const oracle_mesh *m, int nq, double *const *Q, const double *Mq, const double *Rq,
+                                 const double *CVq, const double *CPq, double *Rtot, double *CVtot, double *CPtot) {
+  const long n = (long)m->Np * m->Ne;
+#pragma omp parallel for schedule(static)
+  for (long t = 0; t < n; ++t) {
+    double q[8];
+    for (int iq = 0; iq < nq; ++iq) q[iq] = Q[iq][t];
+    spec_heat(nq, q, Mq, Rq, CVq, CPq, m, &Rtot[t], &CVtot[t], &CPtot[t]);
+  }
+}
+
+/* FElib/src/microphysics/scale_atm_phy_mp_dgm_common.F90:429-556
+ * (atm_phy_mp_dgm_common_negative_fixer; its water species are the nq =
+ * 1 + QLA + QIA tracers), called by fix_negative_val
+ * (FDS/scale_atm_dyn_dgm_driver_trcadv3d.F90:596-651) */
+void oracle_trc_negative_fixer(const oracle_mesh *m, const double *w1d, int nq, double *const *Q, double *DDENS,
+                               double *PRES, double *CVtot, double *CPtot, double *Rtot, const double *DENS_hyd,
+                               const double *PRES_hyd, const double *J, const double *zlev, const double *Mq,
+                               const double *Rq, const double *CVq, const double *CPq, double *DRHOT) {
+  const int Np = m->Np;
+  const double TRC_EPS = 1e-128;
+#pragma omp parallel for schedule(static)
+  for (int ke = 0; ke < m->Ne; ++ke) {
+    const long o = (long)Np * ke;
+    double DENS[512], DDENS0[512], IE0[512], IE[512], TEMP[512], intw[512], TM0[512];
+    double TM1[8][512], Qt[8][512];
+    for (int p = 0; p < Np; ++p) {
+      DENS[p] = DENS_hyd[o + p] + DDENS[o + p];
+      DDENS0[p] = DDENS[o + p];
+      IE0[p] = CVtot[o + p] * PRES[o + p] / Rtot[o + p];
+      TEMP[p] = IE0[p] / (DENS[p] * CVtot[o + p]);
+      IE[p] = IE0[p];
+      intw[p] = m->Gsqrt[o + p] * J[o + p] * trc_vol_weight(w1d, m->n1, p);
+    }
+    for (int iq = 0; iq < nq; ++iq) {
+      for (int p = 0; p < Np; ++p) {
+        TM0[p] = DENS[p] * Q[iq][o + p];
+        TM1[iq][p] = TM0[p] > TRC_EPS ? TM0[p] : TRC_EPS;
+      }
+      double M0 = 0.0, M1 = 0.0;
+      for (int p = 0; p < Np; ++p) M0 = M0 + intw[p] * TM0[p];
+      for (int p = 0; p < Np; ++p) M1 = M1 + intw[p] * TM1[iq][p];
+      for (int p = 0; p < Np; ++p) {
+        TM1[iq][p] = (M0 > 0.0 ? M0 : 0.0) / M1 * TM1[iq][p];
+        DDENS[o + p] = DDENS[o + p] + (TM1[iq][p] - TM0[p]);
+        IE[p] = IE[p] + (TM1[iq][p] - TM0[p]) * CVq[iq] * TEMP[p];
+      }
+    }
+    for (int p = 0; p < Np; ++p) DENS[p] = DENS_hyd[o + p] + DDENS[o + p];
+    for (int iq = 0; iq < nq; ++iq)
+      for (int p = 0; p < Np; ++p) {
+        Qt[iq][p] = TM1[iq][p] / DENS[p];
+        Q[iq][o + p] = Qt[iq][p];
+      }
+    double E0 = 0.0, E1 = 0.0;
+    for (int p = 0; p < Np; ++p) E0 = E0 + intw[p] * IE0[p];
+    for (int p = 0; p < Np; ++p) E1 = E1 + intw[p] * IE[p];
+    for (int p = 0; p < Np; ++p) {
+      IE[p] = E0 / E1 * IE[p];
+      double q[8], R, CV, CP;
+      for (int iq = 0; iq < nq; ++iq) q[iq] = Qt[iq][p];
+      spec_heat(nq, q, Mq, Rq, CVq, CPq, m, &R, &CV, &CP);
+      Rtot[o + p] = R;
+      CVtot[o + p] = CV;
+      CPtot[o + p] = CP;
+      IE[p] = IE[p] - (DDENS[o + p] - DDENS0[p]) * m->GRAV * zlev[o + p];
+      PRES[o + p] = IE[p] * R / CV;
+      if (DRHOT)
+        DRHOT[o + p] = m->PRES00 / R * opow(PRES[o + p] / m->PRES00, CV / CP) -
+                       m->PRES00 / m->Rdry * opow(PRES_hyd[o + p] / m->PRES00, m->CVdry / m->CPdry);
+    }
+  }
+}
+
 /* FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:407-445 (cal_alphdens_advtest) */
 void oracle_trcadvect_cal_alphdens_advtest(const oracle_mesh *m, double *alph_M, double *alph_P, const double *DDENS_,
                                            const double *MOMX_, const double *MOMY_, const double *MOMZ_,

@@ -197,6 +197,15 @@ void oracle_set_phys_tend(const double *tp0, const double *tp1, const double *tp
                           const double *tp4, const double *tp5);
 void oracle_phys_tend_hook(const oracle_mesh *m, double *const T[5], const double *PRES, const double *Rtot,
                            const double *CPtot);
+/* FDS/scale_atm_dyn_dgm_driver_trcadv3d.F90:653-716, the specific heats of the
+ * mixture (SCALE's ATMOS_THERMODYN_specific_heat mixing rule) */
+void oracle_tracer_specific_heat(const oracle_mesh *m, int nq, double *const *Q, const double *Mq, const double *Rq,
+                                 const double *CVq, const double *CPq, double *Rtot, double *CVtot, double *CPtot);
+/* FElib/src/microphysics/scale_atm_phy_mp_dgm_common.F90:429-556; nq <= 8, Np <= 512 */
+void oracle_trc_negative_fixer(const oracle_mesh *m, const double *w1d, int nq, double *const *Q, double *DDENS,
+                               double *PRES, double *CVtot, double *CPtot, double *Rtot, const double *DENS_hyd,
+                               const double *PRES_hyd, const double *J, const double *zlev, const double *Mq,
+                               const double *Rq, const double *CVq, const double *CPq, double *DRHOT);
 void oracle_heve_update(const oracle_mesh *m, const oracle_rk_tab *t, int nsteps,
                         double *const q[5], double *PRES, double *DPRES,
                         const double *DENS_hyd, const double *PRES_hyd, const double *PRES_hyd_ref,

@@ -56,6 +56,8 @@ def _load():
     lib.oracle_trcadvect_cal_tend.argtypes = [M] + [dp] * 11
     lib.oracle_trcadvect_calc_fct_coef.argtypes = [M] + [dp] * 12 + [C.c_double, C.c_double, dp, C.c_int]
     lib.oracle_trcadvect_tmar.argtypes = [M] + [dp] * 5
+    lib.oracle_tracer_specific_heat.argtypes = [M, C.c_int, C.POINTER(dp)] + [dp] * 7
+    lib.oracle_trc_negative_fixer.argtypes = [M, dp, C.c_int, C.POINTER(dp)] + [dp] * 14
     lib.oracle_trcadvect_cal_alphdens_advtest.argtypes = [M] + [dp] * 7
     lib.oracle_tracer_modalfilter_apply.argtypes = [M] + [dp] * 5
     lib.oracle_heve_set_massflux.argtypes = [dp] * 5
@@ -226,6 +228,22 @@ class OracleMesh:
         m = self.mesh
         J = m.J if J is None else J
         LIB.oracle_trcadvect_tmar(self.ref, _d(m.elem.w1d), _d(QTRC), _d(DENS_hyd), _d(DDENS), _d(J))
+
+    def tracer_specific_heat(self, Q, thermo, Rtot, CVtot, CPtot):
+        """Rtot, CVtot, CPtot of the mixture of the tracers Q; thermo = (TRACER_MASS, _R, _CV, _CP)."""
+        Qa = (dp * len(Q))(*[_d(a) for a in Q])
+        th = [np.ascontiguousarray(t, dtype=np.float64) for t in thermo]
+        LIB.oracle_tracer_specific_heat(self.ref, len(Q), Qa, *[_d(t) for t in th], _d(Rtot), _d(CVtot), _d(CPtot))
+
+    def trc_negative_fixer(self, Q, DDENS, PRES, CVtot, CPtot, Rtot, DENS_hyd, PRES_hyd, J, zlev, thermo,
+                           DRHOT=None):
+        """The water species' negative fixer in place (Q, DDENS, PRES, CVtot, CPtot, Rtot, DRHOT)."""
+        m = self.mesh
+        Qa = (dp * len(Q))(*[_d(a) for a in Q])
+        th = [np.ascontiguousarray(t, dtype=np.float64) for t in thermo]
+        LIB.oracle_trc_negative_fixer(self.ref, _d(m.elem.w1d), len(Q), Qa, _d(DDENS), _d(PRES), _d(CVtot),
+                                      _d(CPtot), _d(Rtot), _d(DENS_hyd), _d(PRES_hyd), _d(J), _d(zlev),
+                                      *[_d(t) for t in th], _d(DRHOT) if DRHOT is not None else None)
 
     def trcadvect_save_massflux(self, MFLX, alph, DDENS, MOMX, MOMY, MOMZ, DPRES, DENS_hyd, PRES_hyd, rkstage,
                                 twh, twv, is_hevi=False):
