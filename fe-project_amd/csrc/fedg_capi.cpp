@@ -947,6 +947,94 @@ int fedg_tracer_specific_heat(fedg_ctx* ctx, int nq, double* const* QTRC, const 
   return FEDG_OK;
 }
 
+int fedg_atmos_calc_diagvar(fedg_ctx* ctx, const char* field_name, double* var_out, const double* DDENS_,
+                            const double* MOMX_, const double* MOMY_, const double* MOMZ_, const double* PRES_, int nq,
+                            double* const* QTRC, const double* TRACER_MASS, const double* TRACER_CV,
+                            const double* TRACER_ENGI0, const double* DENS_hyd, const double* PRES_hyd,
+                            const double* Rtot, const double* CVtot, const double* CPtot, const double* zlev,
+                            const double* pos_z) {
+  (void)CVtot;  // an argument of the reference's routine that none of its formulas reads
+  if (!ctx || !field_name || !var_out) return FEDG_ERR_ARG;
+  static const struct {
+    const char* name;
+    int kind;
+  } names[] = {{"DENS", kDiagDENS},   {"U", kDiagU},         {"V", kDiagV},
+               {"W", kDiagW},         {"Umet", kDiagUmet},   {"Vmet", kDiagVmet},
+               {"PRES_diff", kDiagPRES_diff}, {"T", kDiagT}, {"T_diff", kDiagT_diff},
+               {"PT", kDiagPT},       {"PT_diff", kDiagPT_diff}, {"QDRY", kDiagQDRY},
+               {"ENGK", kDiagENGK},   {"ENGP", kDiagENGP},   {"ENGI", kDiagENGI},
+               {"ENGT", kDiagENGT}};
+  const std::string name(field_name);
+  // the reference's case('PRES') is empty: var_out is left as it is
+  if (name == "PRES") return FEDG_OK;
+  int kind = -1;
+  for (const auto& e : names)
+    if (name == e.name) kind = e.kind;
+  if (kind < 0) {
+    if (name == "RH" || name == "RHL")
+      set_error("diagnostic variable " + name + " needs the saturation vapour pressure over liquid water of the "
+                "SCALE library (ATMOS_SATURATION_psat_liq), which this library does not restate");
+    else
+      set_error("the name of the diagnostic variable is not supported: " + name);
+    return FEDG_ERR_ARG;
+  }
+  DiagVar a{};
+  a.kind = kind;
+  a.out = var_out;
+  a.DD = DDENS_;
+  a.MX = MOMX_;
+  a.MY = MOMY_;
+  a.MZ = MOMZ_;
+  a.P = PRES_;
+  a.Dh = DENS_hyd;
+  a.Ph = PRES_hyd;
+  a.R = Rtot;
+  a.CP = CPtot;
+  a.zlev = zlev;
+  a.zpos = pos_z;
+  const bool engi = kind == kDiagENGI || kind == kDiagENGT;
+  const bool engk = kind == kDiagENGK || kind == kDiagENGT;
+  // the fields each formula reads
+  bool ok = true;
+  if (kind != kDiagPRES_diff && kind != kDiagQDRY) ok = ok && DDENS_ && DENS_hyd;
+  if (kind == kDiagU || kind == kDiagUmet || engk) ok = ok && MOMX_;
+  if (kind == kDiagV || kind == kDiagVmet || engk) ok = ok && MOMY_;
+  if (kind == kDiagW || engk) ok = ok && MOMZ_;
+  if (kind == kDiagPRES_diff || kind == kDiagT || kind == kDiagT_diff || kind == kDiagPT || kind == kDiagPT_diff ||
+      engi)
+    ok = ok && PRES_;
+  if (kind == kDiagPRES_diff || kind == kDiagT_diff) ok = ok && PRES_hyd;
+  if (kind == kDiagT || kind == kDiagT_diff || kind == kDiagPT || kind == kDiagPT_diff || engi) ok = ok && Rtot;
+  if (kind == kDiagPT || kind == kDiagPT_diff) ok = ok && CPtot;
+  if (kind == kDiagPT_diff) ok = ok && PRES_hyd;
+  if (kind == kDiagENGP) ok = ok && zlev;
+  if (kind == kDiagENGT) ok = ok && pos_z;
+  if (!ok) {
+    set_error("diagnostic variable " + name + ": a field its formula reads is NULL");
+    return FEDG_ERR_ARG;
+  }
+  if (kind == kDiagQDRY || engi) {
+    if (nq < 0 || nq > kMaxTrc || (nq > 0 && (!QTRC || !TRACER_MASS || (engi && (!TRACER_CV || !TRACER_ENGI0))))) {
+      set_error("diagnostic variable " + name + ": 0 to " + std::to_string(kMaxTrc) +
+                " tracers with TRACER_MASS (and TRACER_CV, TRACER_ENGI0 for the energies)");
+      return FEDG_ERR_ARG;
+    }
+    a.nq = nq;
+    for (int iq = 0; iq < nq; ++iq) {
+      if (!QTRC[iq]) return FEDG_ERR_ARG;
+      a.Q[iq] = QTRC[iq];
+      a.Mq[iq] = TRACER_MASS[iq];
+      if (engi) {
+        a.CVq[iq] = TRACER_CV[iq];
+        a.E0q[iq] = TRACER_ENGI0[iq];
+      }
+    }
+  }
+  TRY(launch_diagvar(ctx, a));
+  FEDG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return FEDG_OK;
+}
+
 int fedg_trc_negative_fixer(fedg_ctx* ctx, int nq, double* const* QTRC, double* DDENS, double* PRES, double* CVtot,
                             double* CPtot, double* Rtot, const double* DENS_hyd, const double* PRES_hyd,
                             const double* J, const double* zlev, double dt, int QLA, int QIA,

@@ -166,6 +166,19 @@ struct TrcThermo {
 // ATMOS_THERMODYN_specific_heat on every node of elements 1..Ne
 int launch_specific_heat(fedg_ctx* ctx, const double* const Q[kMaxTrc], const TrcThermo& th, double* Rtot,
                          double* CVtot, double* CPtot);
+// diagnostic variables of vars_calc_diagnoseVar_lc on elements 1..Ne
+enum DiagKind {
+  kDiagDENS, kDiagU, kDiagV, kDiagW, kDiagUmet, kDiagVmet, kDiagPRES_diff, kDiagT, kDiagT_diff, kDiagPT,
+  kDiagPT_diff, kDiagQDRY, kDiagENGK, kDiagENGP, kDiagENGI, kDiagENGT
+};
+struct DiagVar {
+  int kind, nq;
+  double* out;
+  const double *DD, *MX, *MY, *MZ, *P, *Dh, *Ph, *R, *CP, *zlev, *zpos;
+  const double* Q[kMaxTrc];
+  double Mq[kMaxTrc], CVq[kMaxTrc], E0q[kMaxTrc];  // TRACER_MASS, TRACER_CV, TRACER_ENGI0
+};
+int launch_diagvar(fedg_ctx* ctx, const DiagVar& a);
 struct NegFix {
   double* Q[kMaxTrc];
   double *DD, *PRES, *CV, *CP, *R, *DRHOT;  // DRHOT may be null

@@ -1463,6 +1463,68 @@ __global__ void __launch_bounds__(256) specific_heat_kernel(const SpecHeat a, co
   a.CP[t] = CP;
 }
 
+// vars_calc_diagnoseVar_lc (model/atm_nonhydro3d/src/atmos/mod_atmos_vars_container.F90:1100-1366)
+// and the Umet/Vmet branch of AtmosVarsContainer_CalcDiagvar (:431-444), one
+// node per thread; each formula keeps the reference's order of operations.
+// The horizontal metric G_ij of the box mesh is the identity, written as the
+// reference's products (G_11 MOMX + G_12 MOMY) so that the bits agree.
+__global__ void __launch_bounds__(256) diagvar_kernel(const DiagVar a, const Consts c, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  constexpr double G11 = 1.0, G12 = 0.0, G22 = 1.0;
+  double r = 0.0;
+  switch (a.kind) {
+    case kDiagDENS: r = a.DD[t] + a.Dh[t]; break;
+    case kDiagU: r = a.MX[t] / (a.DD[t] + a.Dh[t]); break;
+    case kDiagV: r = a.MY[t] / (a.DD[t] + a.Dh[t]); break;
+    case kDiagW: r = a.MZ[t] / (a.DD[t] + a.Dh[t]); break;
+    case kDiagUmet: r = a.MX[t] / (a.Dh[t] + a.DD[t]); break;
+    case kDiagVmet: r = a.MY[t] / (a.Dh[t] + a.DD[t]); break;
+    case kDiagPRES_diff: r = a.P[t] - a.Ph[t]; break;
+    case kDiagT: r = a.P[t] / (a.R[t] * (a.DD[t] + a.Dh[t])); break;
+    case kDiagT_diff:
+      r = a.P[t] / (a.R[t] * (a.DD[t] + a.Dh[t])) - a.Ph[t] / (c.Rdry * a.Dh[t]);
+      break;
+    case kDiagPT:
+    case kDiagPT_diff: {
+      const double P = a.P[t], R = a.R[t];
+      const double dens = a.DD[t] + a.Dh[t];
+      r = P / (R * dens) * fedg_pow(c.PRES00 / P, R / a.CP[t]);
+      if (a.kind == kDiagPT_diff)
+        r = r - c.PRES00 / c.Rdry * fedg_pow(a.Ph[t] / c.PRES00, c.CVdry / c.CPdry) / a.Dh[t];
+      break;
+    }
+    case kDiagQDRY: {
+      double qdry = 1.0;
+      for (int iq = 0; iq < a.nq; ++iq) qdry = qdry - a.Q[iq][t] * a.Mq[iq];
+      r = qdry;
+      break;
+    }
+    default: {  // ENGK, ENGP, ENGI, ENGT
+      const double dens = a.DD[t] + a.Dh[t];
+      double engk = 0.0, engi = 0.0;
+      if (a.kind == kDiagENGK || a.kind == kDiagENGT) {
+        const double mx = a.MX[t], my = a.MY[t], mz = a.MZ[t];
+        const double u1 = G11 * mx + G12 * my;
+        const double u2 = G12 * mx + G22 * my;
+        engk = 0.5 * (mx * u1 + my * u2 + mz * mz) / dens;
+      }
+      if (a.kind == kDiagENGI || a.kind == kDiagENGT) {
+        const double P = a.P[t], R = a.R[t];
+        double qdry = 1.0;
+        for (int iq = 0; iq < a.nq; ++iq) qdry = qdry - a.Q[iq][t] * a.Mq[iq];
+        engi = qdry * P / R * c.CVdry;
+        for (int iq = 0; iq < a.nq; ++iq) engi = engi + a.Q[iq][t] * (P / R * a.CVq[iq] + dens * a.E0q[iq]);
+      }
+      if (a.kind == kDiagENGK) r = engk;
+      else if (a.kind == kDiagENGI) r = engi;
+      else if (a.kind == kDiagENGP) r = dens * c.GRAV * a.zlev[t];
+      else r = engk + engi + dens * c.GRAV * a.zpos[t];
+    }
+  }
+  a.out[t] = r;
+}
+
 // atm_phy_mp_dgm_common_negative_fixer
 // (FElib/src/microphysics/scale_atm_phy_mp_dgm_common.F90:429-556), one
 // element per workgroup, one node per thread; every element sum is taken
@@ -2065,6 +2127,13 @@ int launch_specific_heat(fedg_ctx* ctx, const double* const Q[kMaxTrc], const Tr
   a.CP = CPtot;
   a.th = th;
   hipLaunchKernelGGL(specific_heat_kernel, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream, a, ctx->c, n);
+  FEDG_HIP_CHECK(hipGetLastError());
+  return FEDG_OK;
+}
+
+int launch_diagvar(fedg_ctx* ctx, const DiagVar& a) {
+  const long n = (long)ctx->m.Np * ctx->m.Ne;
+  hipLaunchKernelGGL(diagvar_kernel, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream, a, ctx->c, n);
   FEDG_HIP_CHECK(hipGetLastError());
   return FEDG_OK;
 }

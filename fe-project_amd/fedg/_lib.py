@@ -150,6 +150,7 @@ def _load():
         "fedg_tracer_specific_heat": (C.c_int, [vp, C.c_int, C.POINTER(vp)] + [vp] * 7),
         "fedg_trc_negative_fixer": (C.c_int, [vp, C.c_int, C.POINTER(vp)] + [vp] * 9 +
                                     [C.c_double, C.c_int, C.c_int] + [vp] * 5),
+        "fedg_atmos_calc_diagvar": (C.c_int, [vp, C.c_char_p] + [vp] * 6 + [C.c_int, C.POINTER(vp)] + [vp] * 10),
         "fedg_heve_set_phys_tend": (C.c_int, [vp] * 7),
         "fedg_hevi_set_phys_tend": (C.c_int, [vp] * 7),
         "fedg_heve_set_state": (C.c_int, [vp, C.POINTER(vp)]),
@@ -184,7 +185,7 @@ EXPORTED = (
     "fedg_tint_store_implicit fedg_heve_create fedg_heve_destroy fedg_heve_is_flat fedg_heve_set_massflux fedg_hevi_set_massflux fedg_heve_set_state "
     "fedg_heve_get_state fedg_heve_state_dev fedg_heve_run fedg_heve_run_timed fedg_heve_get_pressure "
     "fedg_add_phy_tend fedg_heve_set_phys_tend fedg_hevi_set_phys_tend fedg_box_topography_metrics "
-    "fedg_tracer_specific_heat fedg_trc_negative_fixer"
+    "fedg_tracer_specific_heat fedg_trc_negative_fixer fedg_atmos_calc_diagvar"
 ).split()
 
 

@@ -167,6 +167,32 @@ def trc_negative_fixer(dm: DeviceMesh, Q, DDENS, PRES, CVtot, CPtot, Rtot, DENS_
                                       None if DRHOT is None else _p(DRHOT)), "fedg_trc_negative_fixer")
 
 
+def atmos_calc_diagvar(dm: DeviceMesh, name: str, out, DDENS=None, MOMX=None, MOMY=None, MOMZ=None, PRES=None,
+                       DENS_hyd=None, PRES_hyd=None, Rtot=None, CVtot=None, CPtot=None, Q=(), mass=None, cv=None,
+                       engi0=None, zlev=None, pos_z=None):
+    """A diagnostic variable by name into the device field out (Np*Ne values), as
+    AtmosVars_CalcDiagvar (model/atm_nonhydro3d/src/atmos/mod_atmos_vars.F90:926-934 ->
+    mod_atmos_vars_container.F90:378-452, 1100-1366): DENS, U, V, W, Umet, Vmet, PRES_diff, T,
+    T_diff, PT, PT_diff, QDRY, ENGK, ENGP (zlev), ENGI, ENGT (pos_z, the computational z).
+    Q: tracer device fields; mass / cv / engi0: TRACER_MASS, TRACER_CV, TRACER_ENGI0 per tracer."""
+    nq = len(Q)
+    arr = (C.c_void_p * max(nq, 1))(*[_p(a) for a in Q])
+    tr = []
+    for t in (mass, cv, engi0):
+        if t is None:
+            tr.append(None)
+        else:
+            t = np.ascontiguousarray(t, dtype=np.float64)
+            if t.size != nq:
+                raise ValueError("TRACER_MASS / _CV / _ENGI0 need one value per tracer")
+            tr.append(t)
+    ptrs = [None if t is None else t.ctypes.data_as(C.c_void_p) for t in tr]
+    check(LIB.fedg_atmos_calc_diagvar(dm.handle, name.encode(), _p(out), *[_p(a) for a in (DDENS, MOMX, MOMY, MOMZ,
+                                                                                          PRES)],
+                                      nq, arr, *ptrs, *[_p(a) for a in (DENS_hyd, PRES_hyd, Rtot, CVtot, CPtot, zlev,
+                                                                       pos_z)]), "fedg_atmos_calc_diagvar")
+
+
 def trcadvect_save_massflux(dm: DeviceMesh, MFLX_x, MFLX_y, MFLX_z, alph_dens_M, alph_dens_P, DDENS, MOMX, MOMY, MOMZ,
                             DPRES, DENS_hyd, PRES_hyd, Rtot, CVtot, CPtot, rkstage, tavg_weight_h, tavg_weight_v,
                             is_hevi=False):

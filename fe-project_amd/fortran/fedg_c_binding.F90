@@ -64,7 +64,7 @@ module fedg_c_binding
   public :: fedg_trcadvect_cal_tend, fedg_trcadvect_calc_fct_coef, fedg_trcadvect_tmar
   public :: fedg_trcadvect_save_massflux, fedg_trcadvect_cal_alphdens_advtest
   public :: fedg_tracer_modalfilter_apply, fedg_trcvar_rk_advance
-  public :: fedg_tracer_specific_heat, fedg_trc_negative_fixer
+  public :: fedg_tracer_specific_heat, fedg_trc_negative_fixer, fedg_atmos_calc_diagvar
   public :: fedg_hevi_cal_tend, fedg_hevi_cal_vi, fedg_hevi_cal_vi_newton, fedg_hevi_cal_vi_noalloc
   public :: fedg_calc_phyd_hgrad, fedg_add_phy_tend, fedg_heve_set_phys_tend, fedg_hevi_set_phys_tend
   public :: fedg_tint_create, fedg_tint_destroy, fedg_tint_info, fedg_tint_tend_ex, fedg_tint_tend_im
@@ -365,6 +365,21 @@ module fedg_c_binding
       real(c_double), intent(in) :: TRACER_MASS(*), TRACER_R(*), TRACER_CV(*), TRACER_CP(*)
       type(c_ptr), value :: DRHOT
     end function fedg_trc_negative_fixer
+
+    !> vars_calc_diagnoseVar_lc (model/atm_nonhydro3d/src/atmos/mod_atmos_vars_container.F90:1100-1366)
+    !! by name: field_name is trim(name)//c_null_char; a field the named formula does not read
+    !! may be c_null_ptr, so every field is passed by c_loc
+    integer(c_int) function fedg_atmos_calc_diagvar(ctx, field_name, var_out, DDENS_, MOMX_, MOMY_, MOMZ_, &
+        PRES_, nq, QTRC, TRACER_MASS, TRACER_CV, TRACER_ENGI0, DENS_hyd, PRES_hyd, Rtot, CVtot, CPtot, &
+        zlev, pos_z) bind(C, name="fedg_atmos_calc_diagvar")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: field_name(*)
+      type(c_ptr), value :: var_out, DDENS_, MOMX_, MOMY_, MOMZ_, PRES_
+      integer(c_int), value :: nq
+      type(c_ptr), value :: QTRC, TRACER_MASS, TRACER_CV, TRACER_ENGI0
+      type(c_ptr), value :: DENS_hyd, PRES_hyd, Rtot, CVtot, CPtot, zlev, pos_z
+    end function fedg_atmos_calc_diagvar
 
     !> atm_dyn_dgm_trcadvect3d_heve_cal_alphdens_advtest (FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:407-445)
     integer(c_int) function fedg_trcadvect_cal_alphdens_advtest(ctx, alph_dens_M, alph_dens_P, DDENS_, MOMX_, &

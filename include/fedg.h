@@ -348,6 +348,28 @@ int fedg_trc_negative_fixer(fedg_ctx *ctx, int nq, double *const *QTRC, double *
                             int QIA, const double *TRACER_MASS, const double *TRACER_R, const double *TRACER_CV,
                             const double *TRACER_CP, double *DRHOT);
 
+/* A diagnostic variable of the atmosphere's history output: AtmosVars_CalcDiagvar
+ * -> AtmosVarsContainer_CalcDiagvar -> vars_calc_diagnoseVar_lc
+ * (model/atm_nonhydro3d/src/atmos/mod_atmos_vars_container.F90:378-452, 1100-1366),
+ * by its name (field_name) on every node of elements 1..Ne of var_out (Np,Ne):
+ * DENS, U, V, W, PRES_diff, T, T_diff, PT, PT_diff, ENGK, ENGP (with zlev, the
+ * node heights), ENGI, ENGT (with pos_z, lcmesh%pos_en(:,:,3): the
+ * computational coordinate, as the reference's ENGT), Umet and Vmet (the
+ * regional mesh's Calc_UVmet, mod_atmos_mesh_rm.F90:299-323, is the identity),
+ * and QDRY (the AUXVAR of Calc_SpecificHeat: 1 - sum QTRC_i TRACER_MASS_i, in
+ * tracer order).  QDRY_ of ENGI / ENGT is that same value.  "PRES" leaves
+ * var_out as it is (an empty case in the reference); RH / RHL are rejected
+ * (they need the SCALE library's saturation vapour pressure); any other name is
+ * an error, as the reference's PRC_abort.  The tracers (nq of them, 0 for dry
+ * air) are read only by QDRY, ENGI and ENGT; CVtot is accepted and unused, as
+ * in the reference; fields a formula does not read may be NULL. */
+int fedg_atmos_calc_diagvar(fedg_ctx *ctx, const char *field_name, double *var_out, const double *DDENS_,
+                            const double *MOMX_, const double *MOMY_, const double *MOMZ_,
+                            const double *PRES_, int nq, double *const *QTRC, const double *TRACER_MASS,
+                            const double *TRACER_CV, const double *TRACER_ENGI0, const double *DENS_hyd,
+                            const double *PRES_hyd, const double *Rtot, const double *CVtot,
+                            const double *CPtot, const double *zlev, const double *pos_z);
+
 /* atm_dyn_dgm_trcadvect3d_heve_cal_alphdens_advtest
  * (FDS/scale_atm_dyn_dgm_trcadvect3d_heve.F90:407-445): the damping
  * coefficients of the tracer-only advection test, alpha = max(|u.n| of both

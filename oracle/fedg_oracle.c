@@ -53,6 +53,11 @@
  *    (tests/test_oracle_trcadv_step.py); the tracer filter keeps low modes
  *    and element masses; the dynamics loop's mass-flux save gives MFLX = MOM
  *    for a steady wind;
+ *  - diagnostic variables (oracle_atmos_calc_diagvar,
+ *    tests/test_oracle_diagvar.py): the physics each formula encodes on
+ *    states of known temperature (ideal gas T, PT = T at PRES00, vanishing
+ *    deviations of the base state, kinetic / potential / internal energy in
+ *    closed form, ENGT with the computational z over terrain);
  *  - halo exchange / BC: the reference's mesh/field test semantics
  *    (FElib/test/FE/field_cubedom3d_hexahedral) on integer-valued fields, and
  *    the SLIP condition's closed form (exterior normal momentum = -interior).
@@ -679,6 +684,88 @@ void oracle_tracer_specific_heat(const oracle_mesh *m, int nq, double *const *Q,
     for (int iq = 0; iq < nq; ++iq) q[iq] = Q[iq][t];
     spec_heat(nq, q, Mq, Rq, CVq, CPq, m, &Rtot[t], &CVtot[t], &CPtot[t]);
   }
+}
+
+/* model/atm_nonhydro3d/src/atmos/mod_atmos_vars_container.F90:1100-1366
+ * (vars_calc_diagnoseVar_lc), :431-444 (Umet, Vmet of the regional mesh:
+ * MOMX, MOMY over DENS_hyd + DDENS) and QDRY of Calc_SpecificHeat; the box
+ * mesh's G_ij is the identity.  Returns 0, or -1 for a name the reference
+ * does not diagnose here. */
+int oracle_atmos_calc_diagvar(const oracle_mesh *m, const char *name, double *out, const double *DD,
+                              const double *MX, const double *MY, const double *MZ, const double *P, int nq,
+                              double *const *Q, const double *Mq, const double *CVq, const double *E0q,
+                              const double *Dh, const double *Ph, const double *R, const double *CP,
+                              const double *zlev, const double *zpos) {
+  const long n = (long)m->Np * m->Ne;
+  const double G_11 = 1.0, G_12 = 0.0, G_22 = 1.0;
+  int ok = 1;
+#define NODES _Pragma("omp parallel for schedule(static)") for (long i = 0; i < n; ++i)
+  if (!strcmp(name, "DENS")) {
+    NODES out[i] = DD[i] + Dh[i];
+  } else if (!strcmp(name, "U")) {
+    NODES { double DENS = DD[i] + Dh[i]; out[i] = MX[i] / DENS; }
+  } else if (!strcmp(name, "V")) {
+    NODES { double DENS = DD[i] + Dh[i]; out[i] = MY[i] / DENS; }
+  } else if (!strcmp(name, "W")) {
+    NODES { double DENS = DD[i] + Dh[i]; out[i] = MZ[i] / DENS; }
+  } else if (!strcmp(name, "Umet")) {
+    NODES out[i] = MX[i] / (Dh[i] + DD[i]);
+  } else if (!strcmp(name, "Vmet")) {
+    NODES out[i] = MY[i] / (Dh[i] + DD[i]);
+  } else if (!strcmp(name, "PRES")) {
+    /* empty case */
+  } else if (!strcmp(name, "PRES_diff")) {
+    NODES out[i] = P[i] - Ph[i];
+  } else if (!strcmp(name, "T")) {
+    NODES out[i] = P[i] / (R[i] * (DD[i] + Dh[i]));
+  } else if (!strcmp(name, "T_diff")) {
+    NODES out[i] = P[i] / (R[i] * (DD[i] + Dh[i])) - Ph[i] / (m->Rdry * Dh[i]);
+  } else if (!strcmp(name, "PT")) {
+    NODES {
+      double DENS = DD[i] + Dh[i];
+      out[i] = P[i] / (R[i] * DENS) * oracle_opow(m->PRES00 / P[i], R[i] / CP[i]);
+    }
+  } else if (!strcmp(name, "PT_diff")) {
+    NODES {
+      double DENS = DD[i] + Dh[i];
+      out[i] = P[i] / (R[i] * DENS) * oracle_opow(m->PRES00 / P[i], R[i] / CP[i]) -
+               m->PRES00 / m->Rdry * oracle_opow(Ph[i] / m->PRES00, m->CVdry / m->CPdry) / Dh[i];
+    }
+  } else if (!strcmp(name, "QDRY")) {
+    NODES {
+      double qdry = 1.0;
+      for (int iq = 0; iq < nq; ++iq) qdry = qdry - Q[iq][i] * Mq[iq];
+      out[i] = qdry;
+    }
+  } else if (!strcmp(name, "ENGK")) {
+    NODES {
+      double DENS = DD[i] + Dh[i];
+      double mom_u1 = G_11 * MX[i] + G_12 * MY[i];
+      double mom_u2 = G_12 * MX[i] + G_22 * MY[i];
+      out[i] = 0.5 * (MX[i] * mom_u1 + MY[i] * mom_u2 + MZ[i] * MZ[i]) / DENS;
+    }
+  } else if (!strcmp(name, "ENGP")) {
+    NODES { double DENS = DD[i] + Dh[i]; out[i] = DENS * m->GRAV * zlev[i]; }
+  } else if (!strcmp(name, "ENGI") || !strcmp(name, "ENGT")) {
+    const int engt = !strcmp(name, "ENGT");
+    NODES {
+      double DENS = DD[i] + Dh[i];
+      double QDRY = 1.0;
+      for (int iq = 0; iq < nq; ++iq) QDRY = QDRY - Q[iq][i] * Mq[iq];
+      double v = QDRY * P[i] / R[i] * m->CVdry;
+      for (int iq = 0; iq < nq; ++iq) v = v + Q[iq][i] * (P[i] / R[i] * CVq[iq] + DENS * E0q[iq]);
+      if (engt) {
+        double mom_u1 = G_11 * MX[i] + G_12 * MY[i];
+        double mom_u2 = G_12 * MX[i] + G_22 * MY[i];
+        v = 0.5 * (MX[i] * mom_u1 + MY[i] * mom_u2 + MZ[i] * MZ[i]) / DENS + v + DENS * m->GRAV * zpos[i];
+      }
+      out[i] = v;
+    }
+  } else {
+    ok = 0;
+  }
+#undef NODES
+  return ok ? 0 : -1;
 }
 
 /* FElib/src/microphysics/scale_atm_phy_mp_dgm_common.F90:429-556

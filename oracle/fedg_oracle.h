@@ -201,6 +201,13 @@ void oracle_phys_tend_hook(const oracle_mesh *m, double *const T[5], const doubl
  * mixture (SCALE's ATMOS_THERMODYN_specific_heat mixing rule) */
 void oracle_tracer_specific_heat(const oracle_mesh *m, int nq, double *const *Q, const double *Mq, const double *Rq,
                                  const double *CVq, const double *CPq, double *Rtot, double *CVtot, double *CPtot);
+/* model/atm_nonhydro3d/src/atmos/mod_atmos_vars_container.F90:1100-1366 (and
+ * Umet / Vmet / QDRY); -1 for an unknown name */
+int oracle_atmos_calc_diagvar(const oracle_mesh *m, const char *name, double *out, const double *DD,
+                              const double *MX, const double *MY, const double *MZ, const double *P, int nq,
+                              double *const *Q, const double *Mq, const double *CVq, const double *E0q,
+                              const double *Dh, const double *Ph, const double *R, const double *CP,
+                              const double *zlev, const double *zpos);
 /* FElib/src/microphysics/scale_atm_phy_mp_dgm_common.F90:429-556; nq <= 8, Np <= 512 */
 void oracle_trc_negative_fixer(const oracle_mesh *m, const double *w1d, int nq, double *const *Q, double *DDENS,
                                double *PRES, double *CVtot, double *CPtot, double *Rtot, const double *DENS_hyd,

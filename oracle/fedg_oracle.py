@@ -58,6 +58,8 @@ def _load():
     lib.oracle_trcadvect_tmar.argtypes = [M] + [dp] * 5
     lib.oracle_tracer_specific_heat.argtypes = [M, C.c_int, C.POINTER(dp)] + [dp] * 7
     lib.oracle_trc_negative_fixer.argtypes = [M, dp, C.c_int, C.POINTER(dp)] + [dp] * 14
+    lib.oracle_atmos_calc_diagvar.argtypes = [M, C.c_char_p] + [dp] * 6 + [C.c_int, C.POINTER(dp)] + [dp] * 9
+    lib.oracle_atmos_calc_diagvar.restype = C.c_int
     lib.oracle_trcadvect_cal_alphdens_advtest.argtypes = [M] + [dp] * 7
     lib.oracle_tracer_modalfilter_apply.argtypes = [M] + [dp] * 5
     lib.oracle_heve_set_massflux.argtypes = [dp] * 5
@@ -234,6 +236,21 @@ class OracleMesh:
         Qa = (dp * len(Q))(*[_d(a) for a in Q])
         th = [np.ascontiguousarray(t, dtype=np.float64) for t in thermo]
         LIB.oracle_tracer_specific_heat(self.ref, len(Q), Qa, *[_d(t) for t in th], _d(Rtot), _d(CVtot), _d(CPtot))
+
+    def atmos_calc_diagvar(self, name, DDENS, MOMX, MOMY, MOMZ, PRES, DENS_hyd, PRES_hyd, Rtot, CPtot, Q=(),
+                           mass=None, cv=None, engi0=None, zlev=None, pos_z=None):
+        """A diagnostic variable by name on the interior nodes (vars_calc_diagnoseVar_lc)."""
+        m = self.mesh
+        out = np.zeros(m.Np * m.Ne)
+        Qa = (dp * max(len(Q), 1))(*[_d(a) for a in Q])
+        tr = [None if t is None else np.ascontiguousarray(t, dtype=np.float64) for t in (mass, cv, engi0)]
+        rc = LIB.oracle_atmos_calc_diagvar(self.ref, name.encode(), _d(out), _d(DDENS), _d(MOMX), _d(MOMY), _d(MOMZ),
+                                           _d(PRES), len(Q), Qa, *[None if t is None else _d(t) for t in tr],
+                                           _d(DENS_hyd), _d(PRES_hyd), _d(Rtot), _d(CPtot),
+                                           None if zlev is None else _d(zlev), None if pos_z is None else _d(pos_z))
+        if rc != 0:
+            raise ValueError(f"diagnostic variable {name} is not supported")
+        return out
 
     def trc_negative_fixer(self, Q, DDENS, PRES, CVtot, CPtot, Rtot, DENS_hyd, PRES_hyd, J, zlev, thermo,
                            DRHOT=None):
