@@ -480,28 +480,21 @@ __device__ __forceinline__ void rusanov32_hevi(const Side32& M, const Side32& P,
 }
 
 // ---------------------------------------------------------------------------
-// p = 7 fused stage kernel on the flat-terrain uniform box: one element per
-// 256-thread workgroup, thread (i,j,kk) owning the two nodes (i,j,kk) and
-// (i,j,kk+4), so that wavefront kk holds two horizontal node planes.
-//  - each thread's D row i, D row j and the Lift entries along x and y stay
-//    lane-varying; the z rows D(kk,:), D(kk+4,:), the VFilterPM1 columns and
-//    the vertical Lift entries are wavefront-uniform and come from scalar loads;
-//  - face work: wavefront w evaluates lateral face w (y-, x+, y+, x-), one face
-//    node per lane, so its neighbour logic is uniform; wavefronts 0 and 3 also
-//    evaluate the bottom / top face, whose interior node is their own;
-//  - every global load of the stage (own state, both sides of the face nodes,
-//    RK operands) is issued before the first barrier, and the face fluxes need
-//    no barrier (their interior values are loaded, not exchanged);
-//  - 44 KB of LDS and at most 256 VGPRs without spills: two elements are
-//    resident per CU (a tighter register bound spills and runs 2x slower).
-// HEVI: the explicit tendency of the HEVI scheme (rusanov_hevi; no vertical
-// pressure flux of MOMZ, no vertical flux of DDENS and DRHOT, no buoyancy),
-// FDS/scale_atm_dyn_dgm_nonhydro3d_rhot_hevi.F90:289-482, as heve_kernel's
-// MODE_DROPIN_HEVI computes it.  rk.mode 4: the tendency itself is written to
-// rk.out (no RK update, no DPRES).
-// Arithmetic is identical, operation for operation, to heve_kernel.
+// p = 7 fused stage kernel, one node per thread: one element per 512-thread
+// workgroup, wavefront k holding the horizontal node plane k.  The arithmetic
+// of every node is that of heve7_body, operation for operation; the layout
+// keeps fewer values per thread (at most 128 VGPRs), so that four wavefronts
+// share each SIMD (two elements per CU) and hide each other's LDS and memory
+// latency.
+//  - face work: wavefront 0 evaluates the bottom face and wavefront 7 the top
+//    face, whose interior nodes are their own; wavefronts 1, 2, 5, 6 the
+//    lateral faces y-, x+, y+, x- (interior and exterior sides loaded);
+//  - the z rows D(k,:), the VFilterPM1 column and the vertical Lift entries
+//    are wavefront-uniform; D rows i and j and the lateral Lift entries are
+//    lane-varying.
 template <typename R, bool LVL, bool HEVI = false>
-__device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, const TendIn& in, const RkOut& rk) {
+__device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, const TendIn& in, const RkOut& rk,
+                                            const int ke) {
   constexpr int N1 = 8, Np = 512, Nfp = 64, NfpTot = 384;
   constexpr bool F64 = std::is_same<R, double>::value;
   __shared__ R sDt[N1 * N1];
@@ -509,74 +502,58 @@ __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, co
   __shared__ R sF[2][3][Np];
   __shared__ R sDF[5][NfpTot];
   __shared__ R sRho[Np];
-  // tables of the pressure's pow (FP64 form), read by the stage update after
-  // the first barrier
   __shared__ double sPow[F64 ? 640 : 1];
 
   const int t = threadIdx.x;
   const int i = t & 7, j = (t >> 3) & 7;
-  const int kk = __builtin_amdgcn_readfirstlane(t >> 6);  // wavefront index
-  const int ke = xcd_remap(blockIdx.x, gridDim.x);
-  const int p0 = i + N1 * j + Nfp * kk, p1 = p0 + 4 * Nfp;
+  const int k = __builtin_amdgcn_readfirstlane(t >> 6);  // wavefront index = node plane
+  const int p = t;                                        // node i + N1 j + Nfp k
   const uint32_t base = (uint32_t)Np * ke;
-  const uint32_t o0 = base + p0, o1 = base + p1;
+  const uint32_t o = base + p;
   const R gamm = (R)(c.CPdry / c.CVdry);
   if (t < N1 * N1) {
     sDt[(t % N1) * N1 + t / N1] = (R)m.D1D[t];
     sI[t] = (R)m.Itr[t];
   }
-  if constexpr (F64) pow_tables_to(sPow, t, 256);
+  if constexpr (F64) pow_tables_to(sPow, t, 512);
 
-  // ---- batch 1: own state of both nodes ----
-  double qa[5], qb[5];  // FP64 state for the RK update
+  // ---- own state ----
+  double q[5];
 #pragma unroll
-  for (int v = 0; v < 5; ++v) {
-    qa[v] = ldg(in.q[v], o0);
-    qb[v] = ldg(in.q[v], o1);
-  }
-  // the state in the tendency's arithmetic (the FP64 state itself at R = double)
-  [[maybe_unused]] R raf[5], rbf[5];
+  for (int v = 0; v < 5; ++v) q[v] = ldg(in.q[v], o);
+  [[maybe_unused]] R rf[5];
   if constexpr (!F64) {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      raf[v] = (R)qa[v];
-      rbf[v] = (R)qb[v];
-    }
+    for (int v = 0; v < 5; ++v) rf[v] = (R)q[v];
   }
-  R* const ra = [&]() -> R* {
-    if constexpr (F64) return qa;
-    else return raf;
+  R* const r = [&]() -> R* {
+    if constexpr (F64) return q;
+    else return rf;
   }();
-  R* const rb = [&]() -> R* {
-    if constexpr (F64) return qb;
-    else return rbf;
-  }();
-  // a base-state field (0 DENS_hyd, 1 PRES_hyd, 2 THERM_hyd, 3 DPhydDx, 4
-  // DPhydDy) at global node g = node of an element of level lev: from the
-  // per-level tables when the base state is horizontally homogeneous (LVL),
-  // else from the field
   const int ek0 = ke / (m.NeX * m.NeY);
   auto hyd = [&](const double* fld, int fi, uint32_t g, int lev, int node) -> double {
     if constexpr (LVL) return ldg(in.lvl, (uint32_t)((fi * m.NeZ + lev) * Np + node));
     else return ldg(fld, g);
   };
-  const R Dha = (R)hyd(in.DENS_hyd, 0, o0, ek0, p0), Dhb = (R)hyd(in.DENS_hyd, 0, o1, ek0, p1);
-  const R Tha = (R)hyd(in.THERM_hyd, 2, o0, ek0, p0), Thb = (R)hyd(in.THERM_hyd, 2, o1, ek0, p1);
-  const R DPa = (R)ldg(in.DPRES, o0), DPb = (R)ldg(in.DPRES, o1);
+  const R Dh = (R)hyd(in.DENS_hyd, 0, o, ek0, p);
+  const R Th = (R)hyd(in.THERM_hyd, 2, o, ek0, p);
+  const R DP = (R)ldg(in.DPRES, o);
 
-  // ---- face nodes of this thread: lateral face kk (node lane = t & 63) and,
-  // for wavefronts 0 and 3, the bottom / top face node (i,j) ----
+  // ---- this wavefront's face: the interior (M) and exterior (P) sides ----
   const int ei = ke % m.NeX, ej = (ke / m.NeX) % m.NeY, ek = ke / (m.NeX * m.NeY);
   const int lane = t & 63;
-  const int fL = kk;  // 0: y-, 1: x+, 2: y+, 3: x-
-  const int pmL = face_node<N1>(fL, lane);
-  uint32_t iPL;
-  int nodePL;  // node of the exterior face value within its element
-  int bcL = FEDG_BND_NONE;
-  {
+  // wavefront -> face: 0 bottom (4), 7 top (5), 1/2/5/6 lateral y-/x+/y+/x- (0..3), 3/4 none
+  const int f = (k == 0) ? 4 : (k == 7) ? 5 : (k == 1 || k == 2) ? k - 1 : (k == 5 || k == 6) ? k - 3 : -1;
+  R Mv[9], Pv[9];
+  int bc = FEDG_BND_NONE;
+  int fp = 0;  // face node slot of sDF
+  if (f >= 0 && f < 4) {
+    const int pm = face_node<N1>(f, lane);
+    uint32_t iP;
+    int nodeP;
     int ni = ei, nj = ej;
     bool closed = false;
-    switch (fL) {
+    switch (f) {
       case 0: nj -= 1; if (nj < 0) { if (m.periodic[1]) nj = m.NeY - 1; else closed = true; } break;
       case 1: ni += 1; if (ni >= m.NeX) { if (m.periodic[0]) ni = 0; else closed = true; } break;
       case 2: nj += 1; if (nj >= m.NeY) { if (m.periodic[1]) nj = 0; else closed = true; } break;
@@ -584,66 +561,63 @@ __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, co
     }
     constexpr int opp[4] = {2, 3, 0, 1};
     if (closed) {
-      iPL = base + pmL;
-      nodePL = pmL;
-      bcL = m.bnd_bc[fL];
+      iP = base + pm;
+      nodeP = pm;
+      bc = m.bnd_bc[f];
     } else {
-      nodePL = face_node<N1>(opp[fL], lane);
-      iPL = (uint32_t)Np * (ni + m.NeX * (nj + m.NeY * ek)) + nodePL;
+      nodeP = face_node<N1>(opp[f], lane);
+      iP = (uint32_t)Np * (ni + m.NeX * (nj + m.NeY * ek)) + nodeP;
     }
-  }
-  const uint32_t iML = base + pmL;
-  R ML[9], PL[9];  // DD, DT, MZ, MX, MY, Dhyd, Phyd, Thyd, DP
+    const uint32_t iM = base + pm;
 #pragma unroll
-  for (int v = 0; v < 5; ++v) {
-    ML[v] = (R)ldg(in.q[v], iML);
-    PL[v] = (R)ldg(in.q[v], iPL);
-  }
-  ML[5] = (R)hyd(in.DENS_hyd, 0, iML, ek, pmL); ML[6] = (R)hyd(in.PRES_hyd, 1, iML, ek, pmL);
-  ML[7] = (R)hyd(in.THERM_hyd, 2, iML, ek, pmL);
-  ML[8] = (R)ldg(in.DPRES, iML);
-  PL[5] = (R)hyd(in.DENS_hyd, 0, iPL, ek, nodePL); PL[6] = (R)hyd(in.PRES_hyd, 1, iPL, ek, nodePL);
-  PL[7] = (R)hyd(in.THERM_hyd, 2, iPL, ek, nodePL);
-  PL[8] = (R)ldg(in.DPRES, iPL);
-  // vertical face (wavefronts 0 and 3): interior node is this thread's node
-  const bool hasV = (kk == 0) || (kk == 3);
-  const int fV = (kk == 0) ? 4 : 5;
-  R PV[9];
-  R PhV = R(0);  // PRES_hyd of the interior node of the vertical face
-  int bcV = FEDG_BND_NONE;
-  if (hasV) {
-    int nk = ek + (fV == 4 ? -1 : 1);
+    for (int v = 0; v < 5; ++v) {
+      Mv[v] = (R)ldg(in.q[v], iM);
+      Pv[v] = (R)ldg(in.q[v], iP);
+    }
+    Mv[5] = (R)hyd(in.DENS_hyd, 0, iM, ek, pm); Mv[6] = (R)hyd(in.PRES_hyd, 1, iM, ek, pm);
+    Mv[7] = (R)hyd(in.THERM_hyd, 2, iM, ek, pm);
+    Mv[8] = (R)ldg(in.DPRES, iM);
+    Pv[5] = (R)hyd(in.DENS_hyd, 0, iP, ek, nodeP); Pv[6] = (R)hyd(in.PRES_hyd, 1, iP, ek, nodeP);
+    Pv[7] = (R)hyd(in.THERM_hyd, 2, iP, ek, nodeP);
+    Pv[8] = (R)ldg(in.DPRES, iP);
+    fp = f * Nfp + lane;
+  } else if (f >= 4) {
+    // the interior node is this thread's node (i, j, 0) or (i, j, 7)
+    int nk = ek + (f == 4 ? -1 : 1);
     bool closed = false;
     if (nk < 0 || nk >= m.NeZ) {
       if (m.periodic[2]) nk = (nk + m.NeZ) % m.NeZ;
       else closed = true;
     }
-    const uint32_t own = (fV == 4) ? o0 : o1;
-    const int nodeOwn = (fV == 4) ? p0 : p1;
-    uint32_t iPV;
-    int nodePV = nodeOwn, levPV = ek;
+    uint32_t iP;
+    int nodeP = p, levP = ek;
     if (closed) {
-      iPV = own;
-      bcV = m.bnd_bc[fV];
+      iP = o;
+      bc = m.bnd_bc[f];
     } else {
-      nodePV = i + N1 * j + Nfp * (fV == 4 ? 7 : 0);
-      levPV = nk;
-      iPV = (uint32_t)Np * (ei + m.NeX * (ej + m.NeY * nk)) + (uint32_t)nodePV;
+      nodeP = i + N1 * j + Nfp * (f == 4 ? 7 : 0);
+      levP = nk;
+      iP = (uint32_t)Np * (ei + m.NeX * (ej + m.NeY * nk)) + (uint32_t)nodeP;
     }
 #pragma unroll
-    for (int v = 0; v < 5; ++v) PV[v] = (R)ldg(in.q[v], iPV);
-    PV[5] = (R)hyd(in.DENS_hyd, 0, iPV, levPV, nodePV); PV[6] = (R)hyd(in.PRES_hyd, 1, iPV, levPV, nodePV);
-    PV[7] = (R)hyd(in.THERM_hyd, 2, iPV, levPV, nodePV);
-    PV[8] = (R)ldg(in.DPRES, iPV);
-    PhV = (R)hyd(in.PRES_hyd, 1, own, ek, nodeOwn);
+    for (int v = 0; v < 5; ++v) Pv[v] = (R)ldg(in.q[v], iP);
+    Pv[5] = (R)hyd(in.DENS_hyd, 0, iP, levP, nodeP); Pv[6] = (R)hyd(in.PRES_hyd, 1, iP, levP, nodeP);
+    Pv[7] = (R)hyd(in.THERM_hyd, 2, iP, levP, nodeP);
+    Pv[8] = (R)ldg(in.DPRES, iP);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Mv[v] = r[v];
+    Mv[5] = Dh;
+    Mv[6] = (R)hyd(in.PRES_hyd, 1, o, ek, p);
+    Mv[7] = Th;
+    Mv[8] = DP;
+    fp = f * Nfp + i + N1 * j;
   }
 
   // VFilterPM1 input
-  sRho[p0] = ra[VD];
-  sRho[p1] = rb[VD];
+  sRho[p] = r[VD];
 
-  // ---- face fluxes ----
-  auto face = [&](int f, const R* Mv, R* Pv, int bc, int fp) {
+  // ---- face flux of this wavefront's face ----
+  if (f >= 0) {
     R df[5];
     if constexpr (F64) {
       Side M, P;
@@ -669,7 +643,7 @@ __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, co
       M.Dh = Mv[5]; M.Ph = Mv[6]; M.Th = Mv[7]; M.DP = Mv[8];
       P.DD = Pv[0]; P.DT = Pv[1]; P.MZ = Pv[2]; P.MX = Pv[3]; P.MY = Pv[4];
       P.Dh = Pv[5]; P.Ph = Pv[6]; P.Th = Pv[7]; P.DP = Pv[8];
-      if (bc == FEDG_BND_SLIP) {  // closed boundary: the exterior side is the interior one, reflected
+      if (bc == FEDG_BND_SLIP) {
         const float mn = M.MX * nx + M.MY * ny + M.MZ * nz;
         P.MX = M.MX - 2.f * mn * nx;
         P.MY = M.MY - 2.f * mn * ny;
@@ -682,293 +656,211 @@ __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, co
     }
 #pragma unroll
     for (int v = 0; v < 5; ++v) sDF[v][fp] = df[v];
-  };
-  face(fL, ML, PL, bcL, fL * Nfp + lane);
-  __builtin_amdgcn_sched_barrier(0);
-  if (hasV) {
-    R MV[9];
-#pragma unroll
-    for (int v = 0; v < 5; ++v) MV[v] = (fV == 4) ? ra[v] : rb[v];
-    MV[5] = (fV == 4) ? Dha : Dhb;
-    MV[6] = PhV;
-    MV[7] = (fV == 4) ? Tha : Thb;
-    MV[8] = (fV == 4) ? DPa : DPb;
-    face(fV, MV, PV, bcV, fV * Nfp + i + N1 * j);
   }
   __builtin_amdgcn_sched_barrier(0);
 
   const int ke2d = ke % (m.NeX * m.NeY);
   const uint32_t o2d = (uint32_t)(i + N1 * j) + (uint32_t)Nfp * ke2d;
-
-  // D(kk,:) and D(kk+4,:), wavefront-uniform (scalar loads)
-  double Dka[N1], Dkb[N1];
+  double Dk[N1];  // D(k,:), wavefront-uniform
 #pragma unroll
-  for (int l = 0; l < N1; ++l) {
-    Dka[l] = m.D1D[kk + N1 * l];
-    Dkb[l] = m.D1D[kk + 4 + N1 * l];
-  }
-  // Lift entries: lane-varying along x and y, wavefront-uniform along z
-  const double* LT = m.liftT;  // (N1,6): entry of face f on the line through the node
+  for (int l = 0; l < N1; ++l) Dk[l] = m.D1D[k + N1 * l];
+  const double* LT = m.liftT;
   const double Ly0 = LT[j + N1 * 0], Lx1 = LT[i + N1 * 1], Ly2 = LT[j + N1 * 2], Lx3 = LT[i + N1 * 3];
-  const double Lz4a = LT[kk + N1 * 4], Lz5a = LT[kk + N1 * 5];
-  const double Lz4b = LT[kk + 4 + N1 * 4], Lz5b = LT[kk + 4 + N1 * 5];
+  const double Lz4 = LT[k + N1 * 4], Lz5 = LT[k + N1 * 5];
 
   const R E11 = (R)m.fE[0], E22 = (R)m.fE[1], E33 = (R)m.fE[2];
-  const R RDa = R(1) / (ra[VD] + Dha), RDb = R(1) / (rb[VD] + Dhb);
-  // Gsqrt = GsqrtH = 1: GsqrtV = 1/1, RGsqrtV = 1/GsqrtV, RGsqrt = 1/1
+  const R RD = R(1) / (r[VD] + Dh);
   const R GsqrtV = R(1) / R(1);
   const R RGsqrtV = R(1) / GsqrtV;
   const R RGsqrt = R(1) / R(1);
   const R Gs = R(1), g13 = R(0), g23 = R(0);
   const R GRAV = (R)c.GRAV;
 
-  auto flux = [&](int v, const R* q, R RD, R Th, R DP, R& f0, R& f1, R& f2) {
-    const R m0 = Gs * q[VU];
-    const R m1 = Gs * q[VV];
-    const R m2 = Gs * (q[VW] * RGsqrtV + g13 * q[VU] + g23 * q[VV]);
+  auto flux = [&](int v, R& f0, R& f1, R& f2) {
+    const R m0 = Gs * r[VU];
+    const R m1 = Gs * r[VV];
+    const R m2 = Gs * (r[VW] * RGsqrtV + g13 * r[VU] + g23 * r[VV]);
     if (v == VD) {
       f0 = m0; f1 = m1; f2 = m2;
     } else if (v == VT) {
-      const R pt_ = (Th + q[VT]) * RD;
+      const R pt_ = (Th + r[VT]) * RD;
       f0 = m0 * pt_; f1 = m1 * pt_; f2 = m2 * pt_;
     } else if (v == VW) {
-      const R w_ = q[VW] * RD;
+      const R w_ = r[VW] * RD;
       f0 = m0 * w_; f1 = m1 * w_;
       if constexpr (HEVI) f2 = m2 * w_;
       else f2 = m2 * w_ + Gs * DP * RGsqrtV;
     } else if (v == VU) {
       const R G = Gs * DP;
-      const R u_ = q[VU] * RD;
+      const R u_ = r[VU] * RD;
       f0 = m0 * u_ + G; f1 = m1 * u_; f2 = m2 * u_ + G * g13;
     } else {
       const R G = Gs * DP;
-      const R v_ = q[VV] * RD;
+      const R v_ = r[VV] * RD;
       f0 = m0 * v_; f1 = m1 * v_ + G; f2 = m2 * v_ + G * g23;
     }
   };
 
-  // a variable's pointwise operands (RK operands, DPhydDx/Dy and Coriolis,
-  // PRES_hyd), loaded one variable ahead so that their latency overlaps the
-  // previous variable's contractions
   struct Ops {
-    double rA0, rA1, rB0, rB1, sa, sb, cor, Pha, Phb;
+    double rA, rB, s, cor, Ph;
   };
   auto load_ops = [&](int v) {
-    Ops o{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (rk.mode == 1) {
-      o.rA0 = ldg(rk.var0[v], o0);
-      o.rA1 = ldg(rk.var0[v], o1);
-    }
-    if (rk.mode == 1 || rk.mode == 2) {
-      o.rB0 = ldg(rk.varTmp[v], o0);
-      o.rB1 = ldg(rk.varTmp[v], o1);
-    }
+    Ops op{0.0, 0.0, 0.0, 0.0, 0.0};
+    if (rk.mode == 1) op.rA = ldg(rk.var0[v], o);
+    if (rk.mode == 1 || rk.mode == 2) op.rB = ldg(rk.varTmp[v], o);
     if (v == VU) {
-      o.sa = hyd(in.DPhydDx, 3, o0, ek0, p0);
-      o.sb = hyd(in.DPhydDx, 3, o1, ek0, p1);
-      o.cor = ldg(in.CORIOLIS, o2d);
+      op.s = hyd(in.DPhydDx, 3, o, ek0, p);
+      op.cor = ldg(in.CORIOLIS, o2d);
     } else if (v == VV) {
-      o.sa = hyd(in.DPhydDy, 4, o0, ek0, p0);
-      o.sb = hyd(in.DPhydDy, 4, o1, ek0, p1);
-      o.cor = ldg(in.CORIOLIS, o2d);
+      op.s = hyd(in.DPhydDy, 4, o, ek0, p);
+      op.cor = ldg(in.CORIOLIS, o2d);
     } else if (v == VT) {
-      o.Pha = hyd(in.PRES_hyd, 1, o0, ek0, p0);
-      o.Phb = hyd(in.PRES_hyd, 1, o1, ek0, p1);
+      op.Ph = hyd(in.PRES_hyd, 1, o, ek0, p);
     }
-    return o;
+    return op;
   };
   Ops nxt = load_ops(0);
-  R drhoa = R(0), drhob = R(0);
+  R drho = R(0);
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
     const Ops cur = nxt;
     if (v < 4) nxt = load_ops(v + 1);
-    const double rA0 = cur.rA0, rA1 = cur.rA1, rB0 = cur.rB0, rB1 = cur.rB1;
-    const R sa = (R)cur.sa, sb = (R)cur.sb, cor = (R)cur.cor;
-    const double Pha = cur.Pha, Phb = cur.Phb;
+    const R s = (R)cur.s, cor = (R)cur.cor;
     const int b = v & 1;
-    R f0, f1, f2;
-    flux(v, ra, RDa, Tha, DPa, f0, f1, f2);
-    sF[b][0][p0] = f0; sF[b][1][p0] = f1; sF[b][2][p0] = f2;
-    flux(v, rb, RDb, Thb, DPb, f0, f1, f2);
-    sF[b][0][p1] = f0; sF[b][1][p1] = f1; sF[b][2][p1] = f2;
+    {
+      R f0, f1, f2;
+      flux(v, f0, f1, f2);
+      sF[b][0][p] = f0; sF[b][1][p] = f1; sF[b][2][p] = f2;
+    }
     __syncthreads();
     if (v == 0) {
-      // VFilterPM1 of DDENS (ELM/..._tensorprod3D.F90:2828-2839)
       R zl[N1];
 #pragma unroll
       for (int l = 0; l < N1; ++l) zl[l] = sRho[i + N1 * (j + N1 * l)];
-      drhoa = zl[0] * sI[N1 * kk];
-      drhob = zl[0] * sI[N1 * (kk + 4)];
+      drho = zl[0] * sI[N1 * k];
 #pragma unroll
-      for (int l = 1; l < N1; ++l) {
-        drhoa = drhoa + zl[l] * sI[l + N1 * kk];
-        drhob = drhob + zl[l] * sI[l + N1 * (kk + 4)];
-      }
+      for (int l = 1; l < N1; ++l) drho = drho + zl[l] * sI[l + N1 * k];
     }
     const R* Fx = sF[b][0];
     const R* Fy = sF[b][1];
     const R* Fz = sF[b][2];
-    R dxa, dxb, dya, dyb, dza, dzb;
+    R dx, dy, dz;
     {
       R Di[N1];
 #pragma unroll
       for (int l = 0; l < N1; ++l) Di[l] = sDt[l + N1 * i];
-      dxa = Di[0] * Fx[N1 * (j + N1 * kk)];
-      dxb = Di[0] * Fx[N1 * (j + N1 * (kk + 4))];
+      dx = Di[0] * Fx[N1 * (j + N1 * k)];
 #pragma unroll
-      for (int l = 1; l < N1; ++l) {
-        dxa = dxa + Di[l] * Fx[l + N1 * (j + N1 * kk)];
-        dxb = dxb + Di[l] * Fx[l + N1 * (j + N1 * (kk + 4))];
-      }
+      for (int l = 1; l < N1; ++l) dx = dx + Di[l] * Fx[l + N1 * (j + N1 * k)];
     }
-    __builtin_amdgcn_sched_barrier(0);
     {
       R Dj[N1];
 #pragma unroll
       for (int l = 0; l < N1; ++l) Dj[l] = sDt[l + N1 * j];
-      dya = Fy[i + N1 * (N1 * kk)] * Dj[0];
-      dyb = Fy[i + N1 * (N1 * (kk + 4))] * Dj[0];
+      dy = Fy[i + N1 * (N1 * k)] * Dj[0];
 #pragma unroll
-      for (int l = 1; l < N1; ++l) {
-        dya = dya + Fy[i + N1 * (l + N1 * kk)] * Dj[l];
-        dyb = dyb + Fy[i + N1 * (l + N1 * (kk + 4))] * Dj[l];
-      }
+      for (int l = 1; l < N1; ++l) dy = dy + Fy[i + N1 * (l + N1 * k)] * Dj[l];
     }
-    __builtin_amdgcn_sched_barrier(0);
     {
       R zl[N1];
 #pragma unroll
       for (int l = 0; l < N1; ++l) zl[l] = Fz[i + N1 * (j + N1 * l)];
-      dza = zl[0] * Dka[0];
-      dzb = zl[0] * Dkb[0];
+      dz = zl[0] * Dk[0];
 #pragma unroll
-      for (int l = 1; l < N1; ++l) {
-        dza = dza + zl[l] * Dka[l];
-        dzb = dzb + zl[l] * Dkb[l];
-      }
+      for (int l = 1; l < N1; ++l) dz = dz + zl[l] * Dk[l];
     }
-    __builtin_amdgcn_sched_barrier(0);
     const R* dfv = sDF[v];
     const R fz4 = dfv[4 * Nfp + i + N1 * j], fz5 = dfv[5 * Nfp + i + N1 * j];
-    const R lifta = Ly0 * dfv[i + N1 * kk] + Lx1 * dfv[Nfp + j + N1 * kk] + Ly2 * dfv[2 * Nfp + i + N1 * kk] +
-                         Lx3 * dfv[3 * Nfp + j + N1 * kk] + Lz4a * fz4 + Lz5a * fz5;
-    const R liftb = Ly0 * dfv[i + N1 * (kk + 4)] + Lx1 * dfv[Nfp + j + N1 * (kk + 4)] +
-                         Ly2 * dfv[2 * Nfp + i + N1 * (kk + 4)] + Lx3 * dfv[3 * Nfp + j + N1 * (kk + 4)] +
-                         Lz4b * fz4 + Lz5b * fz5;
-    // HEVI: DDENS and DRHOT have no vertical flux term, MOMZ no buoyancy
+    const R lift = Ly0 * dfv[i + N1 * k] + Lx1 * dfv[Nfp + j + N1 * k] + Ly2 * dfv[2 * Nfp + i + N1 * k] +
+                   Lx3 * dfv[3 * Nfp + j + N1 * k] + Lz4 * fz4 + Lz5 * fz5;
     const bool horiz = HEVI && (v == VD || v == VT);
-    const R Xa = horiz ? E11 * dxa + E22 * dya + lifta : E11 * dxa + E22 * dya + E33 * dza + lifta;
-    const R Xb = horiz ? E11 * dxb + E22 * dyb + liftb : E11 * dxb + E22 * dyb + E33 * dzb + liftb;
-    R Ta, Tb;
+    const R X = horiz ? E11 * dx + E22 * dy + lift : E11 * dx + E22 * dy + E33 * dz + lift;
+    R T;
     if (v == VD || v == VT) {
-      Ta = -Xa * RGsqrt;
-      Tb = -Xb * RGsqrt;
+      T = -X * RGsqrt;
     } else if (v == VW) {
-      if constexpr (HEVI) {
-        Ta = -Xa * RGsqrt;
-        Tb = -Xb * RGsqrt;
-      } else {
-        Ta = -Xa * RGsqrt - GRAV * drhoa;
-        Tb = -Xb * RGsqrt - GRAV * drhob;
-      }
+      if constexpr (HEVI) T = -X * RGsqrt;
+      else T = -X * RGsqrt - GRAV * drho;
     } else if (v == VU) {
-      Ta = -sa + cor * ra[VV];
-      Ta = Ta - Xa * RGsqrt;
-      Tb = -sb + cor * rb[VV];
-      Tb = Tb - Xb * RGsqrt;
+      T = -s + cor * r[VV];
+      T = T - X * RGsqrt;
     } else {
-      Ta = -sa - cor * ra[VU];
-      Ta = Ta - Xa * RGsqrt;
-      Tb = -sb - cor * rb[VU];
-      Tb = Tb - Xb * RGsqrt;
+      T = -s - cor * r[VU];
+      T = T - X * RGsqrt;
     }
-    const double TA = (double)Ta, TB = (double)Tb;
-    if (rk.mode == 4) {  // the tendency itself
-      stg(rk.out[v], o0, TA);
-      stg(rk.out[v], o1, TB);
-      __builtin_amdgcn_sched_barrier(0);
+    const double TD = (double)T;
+    if (rk.mode == 4) {
+      stg(rk.out[v], o, TD);
       continue;
     }
-    // RK stage update (CMN/scale_timeint_rk.F90:2232-2350, tend_buf_size = 1)
-    double na, nb;
+    double nq;
     switch (rk.mode) {
       case 0:
-        stg(rk.varTmp[v], o0, qa[v] + rk.cb * TA);
-        stg(rk.varTmp[v], o1, qb[v] + rk.cb * TB);
-        na = qa[v] + rk.ca * TA;
-        nb = qb[v] + rk.ca * TB;
+        stg(rk.varTmp[v], o, q[v] + rk.cb * TD);
+        nq = q[v] + rk.ca * TD;
         break;
       case 1:
-        stg(rk.varTmp[v], o0, rB0 + rk.cb * TA);
-        stg(rk.varTmp[v], o1, rB1 + rk.cb * TB);
-        na = rA0 + rk.ca * TA;
-        nb = rA1 + rk.ca * TB;
+        stg(rk.varTmp[v], o, cur.rB + rk.cb * TD);
+        nq = cur.rA + rk.ca * TD;
         break;
       case 2:
-        na = rB0 + rk.cb * TA;
-        nb = rB1 + rk.cb * TB;
+        nq = cur.rB + rk.cb * TD;
         break;
       default:
-        na = qa[v] + rk.cb * TA;
-        nb = qb[v] + rk.cb * TB;
+        nq = q[v] + rk.cb * TD;
         break;
     }
-    stg(rk.out[v], o0, na);
-    stg(rk.out[v], o1, nb);
+    stg(rk.out[v], o, nq);
     if (v == VT) {
       if constexpr (F64) {
-        if (in.Rtot != nullptr) {
-          stg(rk.DPout, o0, dpres_of_tab(na, Tha, Pha, ldg(in.Rtot, o0), ldg(in.CVtot, o0), ldg(in.CPtot, o0), c, sPow));
-          stg(rk.DPout, o1, dpres_of_tab(nb, Thb, Phb, ldg(in.Rtot, o1), ldg(in.CVtot, o1), ldg(in.CPtot, o1), c, sPow));
-        } else {
-          stg(rk.DPout, o0, dpres_of_tab(na, Tha, Pha, c.Rdry, c.CVdry, c.CPdry, c, sPow));
-          stg(rk.DPout, o1, dpres_of_tab(nb, Thb, Phb, c.Rdry, c.CVdry, c.CPdry, c, sPow));
-        }
+        if (in.Rtot != nullptr)
+          stg(rk.DPout, o, dpres_of_tab(nq, Th, cur.Ph, ldg(in.Rtot, o), ldg(in.CVtot, o), ldg(in.CPtot, o), c, sPow));
+        else
+          stg(rk.DPout, o, dpres_of_tab(nq, Th, cur.Ph, c.Rdry, c.CVdry, c.CPdry, c, sPow));
       } else {
-        // perturbation form of the pressure (dry air, the base state's gamma)
-        stg(rk.DPout, o0, (double)dpres32((float)na, Tha, (float)Pha, gamm));
-        stg(rk.DPout, o1, (double)dpres32((float)nb, Thb, (float)Phb, gamm));
+        stg(rk.DPout, o, (double)dpres32((float)nq, Th, (float)cur.Ph, gamm));
       }
     }
-    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
-__global__ void __launch_bounds__(256, 2) heve7_flat_kernel(const DevMesh m, const Consts c, const TendIn in,
-                                                            const RkOut rk) {
-  heve7_body<double, false>(m, c, in, rk);
+// 512 threads, at most 128 VGPRs: two workgroups (16 wavefronts) per CU
+#define FEDG_HEVE7_BOUNDS __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
+// at most 80 VGPRs: three workgroups per CU (the FP32 HEVI tendency)
+#define FEDG_HEVE7_BOUNDS3 __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6)))
+__global__ void FEDG_HEVE7_BOUNDS heve7_flat_kernel(const DevMesh m, const Consts c, const TendIn in, const RkOut rk) {
+  heve7_body<double, false>(m, c, in, rk, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // base state and its horizontal gradient read from per-level tables
 // (horizontally homogeneous base state): the same values, a fraction of the
 // bytes (the tables stay in L2)
-__global__ void __launch_bounds__(256, 2) heve7_lvl_kernel(const DevMesh m, const Consts c, const TendIn in,
-                                                           const RkOut rk) {
-  heve7_body<double, true>(m, c, in, rk);
+__global__ void FEDG_HEVE7_BOUNDS heve7_lvl_kernel(const DevMesh m, const Consts c, const TendIn in, const RkOut rk) {
+  heve7_body<double, true>(m, c, in, rk, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // The same p = 7 stage with the tendency in FP32 about the FP64 base state
 // (perturbation form; state, base state and RK update FP64; the pressure
-// perturbation of the new state in FP32 as dpres32).  Half the LDS and fewer
-// registers; two workgroups per CU all the same (a bound for three spills and
-// runs 20 % slower).
-__global__ void __launch_bounds__(256, 2) heve7_f32_kernel(const DevMesh m, const Consts c, const TendIn in,
-                                                           const RkOut rk) {
-  heve7_body<float, false>(m, c, in, rk);
+// perturbation of the new state in FP32 as dpres32).
+__global__ void FEDG_HEVE7_BOUNDS heve7_f32_kernel(const DevMesh m, const Consts c, const TendIn in, const RkOut rk) {
+  heve7_body<float, false>(m, c, in, rk, xcd_remap(blockIdx.x, gridDim.x));
 }
-__global__ void __launch_bounds__(256, 2) heve7_f32_lvl_kernel(const DevMesh m, const Consts c, const TendIn in,
-                                                               const RkOut rk) {
-  heve7_body<float, true>(m, c, in, rk);
+__global__ void FEDG_HEVE7_BOUNDS heve7_f32_lvl_kernel(const DevMesh m, const Consts c, const TendIn in,
+                                                       const RkOut rk) {
+  heve7_body<float, true>(m, c, in, rk, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // The explicit HEVI tendency on the same layout (rk.mode 4): FP64, or FP32 in
 // perturbation form about the FP64 base state; LVL: base state from the
 // per-level tables.
-template <typename R, bool LVL>
-__global__ void __launch_bounds__(256, 2) heve7_hevi_kernel(const DevMesh m, const Consts c, const TendIn in,
-                                                            const RkOut rk) {
-  heve7_body<R, LVL, true>(m, c, in, rk);
+template <bool LVL>
+__global__ void FEDG_HEVE7_BOUNDS heve7_hevi_kernel(const DevMesh m, const Consts c, const TendIn in, const RkOut rk) {
+  heve7_body<double, LVL, true>(m, c, in, rk, xcd_remap(blockIdx.x, gridDim.x));
+}
+template <bool LVL>
+__global__ void FEDG_HEVE7_BOUNDS3 heve7_hevi32_kernel(const DevMesh m, const Consts c, const TendIn in,
+                                                       const RkOut rk) {
+  heve7_body<float, LVL, true>(m, c, in, rk, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 template <int N1, int EPB>
@@ -2297,9 +2189,9 @@ int launch_heve_fused32(fedg_ctx* ctx, const TendIn& in, const RkOut& rk) {
   }
   if (m.n1 == 8 && m.lift_tensor) {
     if (in.lvl)
-      hipLaunchKernelGGL(heve7_f32_lvl_kernel, dim3(m.Ne), dim3(256), 0, ctx->stream, m, ctx->c, in, rk);
+      hipLaunchKernelGGL(heve7_f32_lvl_kernel, dim3(m.Ne), dim3(512), 0, ctx->stream, m, ctx->c, in, rk);
     else
-      hipLaunchKernelGGL(heve7_f32_kernel, dim3(m.Ne), dim3(256), 0, ctx->stream, m, ctx->c, in, rk);
+      hipLaunchKernelGGL(heve7_f32_kernel, dim3(m.Ne), dim3(512), 0, ctx->stream, m, ctx->c, in, rk);
     FEDG_HIP_CHECK(hipGetLastError());
     return FEDG_OK;
   }
@@ -2331,9 +2223,9 @@ int launch_heve_fused(fedg_ctx* ctx, const TendIn& in, const RkOut& rk, int use_
   // the p = 7 layout carries no physics tendencies: with them, the general layout
   if (use_flat && m.n1 == 8 && m.lift_tensor && in.tp[0] == nullptr) {
     if (in.lvl)
-      hipLaunchKernelGGL(heve7_lvl_kernel, dim3(m.Ne), dim3(256), 0, ctx->stream, m, ctx->c, in, rk);
+      hipLaunchKernelGGL(heve7_lvl_kernel, dim3(m.Ne), dim3(512), 0, ctx->stream, m, ctx->c, in, rk);
     else
-      hipLaunchKernelGGL(heve7_flat_kernel, dim3(m.Ne), dim3(256), 0, ctx->stream, m, ctx->c, in, rk);
+      hipLaunchKernelGGL(heve7_flat_kernel, dim3(m.Ne), dim3(512), 0, ctx->stream, m, ctx->c, in, rk);
     FEDG_HIP_CHECK(hipGetLastError());
     return FEDG_OK;
   }
@@ -2352,14 +2244,14 @@ int launch_hevi_explicit7(fedg_ctx* ctx, const TendIn& in, double* const out[5],
   rk.mode = 4;
   if (fp32) {
     if (in.lvl)
-      hipLaunchKernelGGL((heve7_hevi_kernel<float, true>), dim3(m.Ne), dim3(256), 0, ctx->stream, m, ctx->c, in, rk);
+      hipLaunchKernelGGL((heve7_hevi32_kernel<true>), dim3(m.Ne), dim3(512), 0, ctx->stream, m, ctx->c, in, rk);
     else
-      hipLaunchKernelGGL((heve7_hevi_kernel<float, false>), dim3(m.Ne), dim3(256), 0, ctx->stream, m, ctx->c, in, rk);
+      hipLaunchKernelGGL((heve7_hevi32_kernel<false>), dim3(m.Ne), dim3(512), 0, ctx->stream, m, ctx->c, in, rk);
   } else {
     if (in.lvl)
-      hipLaunchKernelGGL((heve7_hevi_kernel<double, true>), dim3(m.Ne), dim3(256), 0, ctx->stream, m, ctx->c, in, rk);
+      hipLaunchKernelGGL((heve7_hevi_kernel<true>), dim3(m.Ne), dim3(512), 0, ctx->stream, m, ctx->c, in, rk);
     else
-      hipLaunchKernelGGL((heve7_hevi_kernel<double, false>), dim3(m.Ne), dim3(256), 0, ctx->stream, m, ctx->c, in,
+      hipLaunchKernelGGL((heve7_hevi_kernel<false>), dim3(m.Ne), dim3(512), 0, ctx->stream, m, ctx->c, in,
                          rk);
   }
   FEDG_HIP_CHECK(hipGetLastError());
