@@ -78,10 +78,17 @@ __device__ __forceinline__ uint32_t vz_nbr(const HeviArgs& a, int ke, int f, int
 
 // ---------------------------------------------------------------------------
 // 1. vi_cal_del_flux_dyn_uv (:1015-1158) + eval_Ax_uv (:482-586)
-template <int N1>
+// FLAT (the flat-terrain uniform box the context detected): the metric
+// arrays hold Gsqrt = GsqrtH = 1, G13 = G23 = 0, G11 = G22 = 1, G12 = 0,
+// nz = -1 / +1 on the bottom / top faces, E33 and Fscale constant and a
+// tensor-product Lift, so those values are taken from the context's
+// constants instead of loaded; every operation is kept, so the results are
+// the same bits.
+template <int N1, bool FLAT>
 __global__ void __launch_bounds__(N1* N1* N1) hevi_rhs_uv_kernel(const HeviArgs a) {
   constexpr int Np = N1 * N1 * N1, Nfp = N1 * N1;
   __shared__ double sdf[2][2][Nfp];  // face (bottom, top), var (MOMX, MOMY), node
+  __shared__ double sPow[640];        // the pow tables (the same values as the global ones)
   const int p = threadIdx.x;
   const int ke = blockIdx.x;
   const int ij = p % Nfp, k = p / Nfp;
@@ -90,17 +97,19 @@ __global__ void __launch_bounds__(N1* N1* N1) hevi_rhs_uv_kernel(const HeviArgs 
   const uint32_t o = (uint32_t)Np * ke + p;
   const double gamm = c.CPdry / c.CVdry, rgamm = c.CVdry / c.CPdry;
   const double PRES0ovRdry = c.PRES00 / c.Rdry, rP0 = 1.0 / c.PRES00;
+  pow_tables_to(sPow, p, Np);
+  __syncthreads();
   if (k == 0 || k == N1 - 1) {
     for (int f = 0; f < 2; ++f) {
       if ((f == 0) != (k == 0)) continue;
       const int fp = (4 + f) * Nfp + ij;  // reference face-node index
       const uint32_t gfp = (uint32_t)fp + (uint32_t)(6 * Nfp) * ke;
       const uint32_t iM = o, iP = vz_nbr(a, ke, f, ij, N1);
-      const double nz = ldg(m.nz, gfp);
-      const int ke2d = m.EMap3Dto2D[ke];
-      const double G11_ = ldg(m.G11, (uint32_t)ij + (uint32_t)Nfp * ke2d);
-      const double G12_ = ldg(m.G12, (uint32_t)ij + (uint32_t)Nfp * ke2d);
-      const double G22_ = ldg(m.G22, (uint32_t)ij + (uint32_t)Nfp * ke2d);
+      const double nz = FLAT ? (f == 0 ? -1.0 : 1.0) : ldg(m.nz, gfp);
+      const int ke2d = FLAT ? 0 : m.EMap3Dto2D[ke];
+      const double G11_ = FLAT ? 1.0 : ldg(m.G11, (uint32_t)ij + (uint32_t)Nfp * ke2d);
+      const double G12_ = FLAT ? 0.0 : ldg(m.G12, (uint32_t)ij + (uint32_t)Nfp * ke2d);
+      const double G22_ = FLAT ? 1.0 : ldg(m.G22, (uint32_t)ij + (uint32_t)Nfp * ke2d);
       double wt0[2], pres0[2], rdens0[2], Gnn[2];
       const uint32_t ix[2] = {iM, iP};
       for (int s = 0; s < 2; ++s) {
@@ -108,14 +117,14 @@ __global__ void __launch_bounds__(N1* N1* N1) hevi_rhs_uv_kernel(const HeviArgs 
         const int ken = n / Np;
         const double gam = 1.0;
         const double rgam2 = 1.0 / (gam * gam);
-        const double RGsqrtV = 1.0 / gsqrtv(a, ken, n - (uint32_t)Np * ken, N1);
+        const double RGsqrtV = 1.0 / (FLAT ? 1.0 : gsqrtv(a, ken, n - (uint32_t)Np * ken, N1));
         const double MOMX0 = ldg(a.q0[VU], n), MOMY0 = ldg(a.q0[VV], n), DRHOT0 = ldg(a.q0[VT], n);
         const double Rt = ldg(a.Rtot, n), CPovCV = ldg(a.CPtot, n) / ldg(a.CVtot, n);
-        const double g13 = ldg(m.G13, n), g23 = ldg(m.G23, n);
+        const double g13 = FLAT ? 0.0 : ldg(m.G13, n), g23 = FLAT ? 0.0 : ldg(m.G23, n);
         rdens0[s] = 1.0 / (ldg(a.DENS_hyd, n) + ldg(a.q0[VD], n));
         wt0[s] = (ldg(a.q0[VW], n) * RGsqrtV + g13 * MOMX0 + g23 * MOMY0) * rdens0[s];
-        const double rhot_hyd = PRES0ovRdry * fedg_pow(ldg(a.PRES_hyd, n) / c.PRES00, rgamm);
-        pres0[s] = c.PRES00 * fedg_pow(Rt * rP0 * (rhot_hyd + DRHOT0), CPovCV);
+        const double rhot_hyd = PRES0ovRdry * fedg_pow_tab(ldg(a.PRES_hyd, n) / c.PRES00, rgamm, sPow);
+        pres0[s] = c.PRES00 * fedg_pow_tab(Rt * rP0 * (rhot_hyd + DRHOT0), CPovCV, sPow);
         const double Gxz = rgam2 * (G11_ * g13 + G12_ * g23);
         const double Gyz = rgam2 * (G12_ * g13 + G22_ * g23);
         Gnn[s] = (1.0 * (RGsqrtV * RGsqrtV) + g13 * Gxz + g23 * Gyz) * fabs(nz);
@@ -124,17 +133,18 @@ __global__ void __launch_bounds__(N1* N1* N1) hevi_rhs_uv_kernel(const HeviArgs 
       const double sP = fabs(wt0[1]) + sqrt(Gnn[1] * gamm * pres0[1] * rdens0[1]);
       const double al = (nz * nz) * (sM > sP ? sM : sP);
       stg(a.alph, (uint32_t)(f * Nfp + ij) + (uint32_t)(2 * Nfp) * ke, al);
-      const double tmp1 = -0.5 * ldg(m.Fscale, gfp) * al;
+      const double tmp1 = -0.5 * (FLAT ? m.fF[4 + f] : ldg(m.Fscale, gfp)) * al;
       sdf[f][0][ij] = tmp1 * (ldg(a.pv[VU], iP) - ldg(a.pv[VU], iM));
       sdf[f][1][ij] = tmp1 * (ldg(a.pv[VV], iP) - ldg(a.pv[VV], iM));
     }
   }
   __syncthreads();
   // Lift of the vertical-face jumps (the lateral faces carry none)
-  const double L4 = ldg(m.Lift, (uint32_t)(p + 4 * Np)), L5 = ldg(m.Lift, (uint32_t)(p + 5 * Np));
+  const double L4 = FLAT ? m.liftT[k + N1 * 4] : ldg(m.Lift, (uint32_t)(p + 4 * Np));
+  const double L5 = FLAT ? m.liftT[k + N1 * 5] : ldg(m.Lift, (uint32_t)(p + 5 * Np));
   const double LDx = L4 * sdf[0][0][ij] + L5 * sdf[1][0][ij];
   const double LDy = L4 * sdf[0][1][ij] + L5 * sdf[1][1][ij];
-  const double RGsqrtV = 1.0 / gsqrtv(a, ke, p, N1);
+  const double RGsqrtV = 1.0 / (FLAT ? 1.0 : gsqrtv(a, ke, p, N1));
   const double Tx = -LDx * RGsqrtV, Ty = -LDy * RGsqrtV;
   if (a.want_b) {
     const uint32_t n = (uint32_t)Np * m.Ne;
@@ -148,7 +158,7 @@ __global__ void __launch_bounds__(N1* N1* N1) hevi_rhs_uv_kernel(const HeviArgs 
 
 // ---------------------------------------------------------------------------
 // 3. vi_cal_del_flux_dyn (:1161-1328) + eval_Ax (:111-326)
-template <int N1>
+template <int N1, bool FLAT>
 __global__ void __launch_bounds__(N1* N1* N1) hevi_rhs3_kernel(const HeviArgs a) {
   constexpr int Np = N1 * N1 * N1, Nfp = N1 * N1;
   __shared__ double sF[3][Np];      // DENS, MOMZ, RHOT vertical fluxes
@@ -156,6 +166,7 @@ __global__ void __launch_bounds__(N1* N1* N1) hevi_rhs3_kernel(const HeviArgs a)
   __shared__ double sdf[2][3][Nfp];  // face (bottom, top), var (DENS, MOMZ, RHOT), node
   __shared__ double sD[N1 * N1];
   __shared__ double sI[N1 * N1];
+  __shared__ double sPow[640];  // the pow tables (the same values as the global ones)
   const int p = threadIdx.x;
   const int ke = blockIdx.x;
   const int ij = p % Nfp, k = p / Nfp;
@@ -169,6 +180,8 @@ __global__ void __launch_bounds__(N1* N1* N1) hevi_rhs3_kernel(const HeviArgs a)
     sD[p] = m.D1D[p];
     sI[p] = m.Itr[p];
   }
+  pow_tables_to(sPow, p, Np);
+  __syncthreads();
   // vertical face jumps of the iterate
   if (k == 0 || k == N1 - 1) {
     for (int f = 0; f < 2; ++f) {
@@ -181,39 +194,42 @@ __global__ void __launch_bounds__(N1* N1* N1) hevi_rhs3_kernel(const HeviArgs a)
       for (int s = 0; s < 2; ++s) {
         const uint32_t n = ix[s];
         const int ken = n / Np;
-        GV[s] = gsqrtv(a, ken, n - (uint32_t)Np * ken, N1);
+        GV[s] = FLAT ? 1.0 : gsqrtv(a, ken, n - (uint32_t)Np * ken, N1);
         DD[s] = ldg(a.pv[VD], n);
         MZ[s] = ldg(a.pv[VW], n);
         DT[s] = ldg(a.pv[VT], n);
         const double dens = ldg(a.DENS_hyd, n) + DD[s];
         const double PH = ldg(a.PRES_hyd, n);
-        const double rhot_hyd = PRES0ovRdry * fedg_pow(PH / c.PRES00, rgamm);
+        const double rhot_hyd = PRES0ovRdry * fedg_pow_tab(PH / c.PRES00, rgamm, sPow);
         pott[s] = (rhot_hyd + DT[s]) / dens;
-        dpres[s] = c.PRES00 * fedg_pow(ldg(a.Rtot, n) * rP0 * dens * pott[s], ldg(a.CPtot, n) / ldg(a.CVtot, n));
+        dpres[s] = c.PRES00 * fedg_pow_tab(ldg(a.Rtot, n) * rP0 * dens * pott[s], ldg(a.CPtot, n) / ldg(a.CVtot, n), sPow);
         dpres[s] = dpres[s] - PH;
-        MW[s] = MZ[s] + GV[s] * ldg(m.G13, n) * ldg(a.pv[VU], n) + GV[s] * ldg(m.G23, n) * ldg(a.pv[VV], n);
+        const double g13 = FLAT ? 0.0 : ldg(m.G13, n), g23 = FLAT ? 0.0 : ldg(m.G23, n);
+        MW[s] = MZ[s] + GV[s] * g13 * ldg(a.pv[VU], n) + GV[s] * g23 * ldg(a.pv[VV], n);
       }
       if ((kz == 0 || kz == a.NeZ - 1) && iM == iP) {
-        MZ[1] = -MZ[0] - 2.0 * GV[0] * (ldg(m.G13, iM) * ldg(a.pv[VU], iM) + ldg(m.G23, iM) * ldg(a.pv[VV], iM));
+        const double g13 = FLAT ? 0.0 : ldg(m.G13, iM), g23 = FLAT ? 0.0 : ldg(m.G23, iM);
+        MZ[1] = -MZ[0] - 2.0 * GV[0] * (g13 * ldg(a.pv[VU], iM) + g23 * ldg(a.pv[VV], iM));
         MW[1] = -MW[0];
       }
-      const double nz = ldg(m.nz, gfp);
+      const double nz = FLAT ? (f == 0 ? -1.0 : 1.0) : ldg(m.nz, gfp);
       const double al = ldg(a.alph, (uint32_t)(f * Nfp + ij) + (uint32_t)(2 * Nfp) * ke);
-      const double tmp1 = 0.5 * ldg(m.Fscale, gfp);
+      const double tmp1 = 0.5 * (FLAT ? m.fF[4 + f] : ldg(m.Fscale, gfp));
       sdf[f][0][ij] = tmp1 * (+(MW[1] - MW[0]) * nz - al * (DD[1] - DD[0]));
       sdf[f][1][ij] = tmp1 * (+(dpres[1] - dpres[0]) * nz - al * (MZ[1] - MZ[0]));
       sdf[f][2][ij] = tmp1 * (+(pott[1] * MW[1] - pott[0] * MW[0]) * nz - al * (DT[1] - DT[0]));
     }
   }
   // vertical fluxes of the node
-  const double GV = gsqrtv(a, ke, p, N1);
+  const double GV = FLAT ? 1.0 : gsqrtv(a, ke, p, N1);
   const double PVD = ldg(a.pv[VD], o), PVW = ldg(a.pv[VW], o), PVT = ldg(a.pv[VT], o);
-  const double FD = PVW + GV * ldg(m.G13, o) * ldg(a.pv[VU], o) + GV * ldg(m.G23, o) * ldg(a.pv[VV], o);
+  const double G13o = FLAT ? 0.0 : ldg(m.G13, o), G23o = FLAT ? 0.0 : ldg(m.G23, o);
+  const double FD = PVW + GV * G13o * ldg(a.pv[VU], o) + GV * G23o * ldg(a.pv[VV], o);
   const double PH = ldg(a.PRES_hyd, o), DH = ldg(a.DENS_hyd, o);
-  const double RHOT = P0ovR * fedg_pow(PH / c.PRES00, rgamm) + PVT;
+  const double RHOT = P0ovR * fedg_pow_tab(PH / c.PRES00, rgamm, sPow) + PVT;
   const double pt_ = RHOT / (PVD + DH);
   const double Rt = ldg(a.Rtot, o), CPt = ldg(a.CPtot, o), CVt = ldg(a.CVtot, o);
-  const double FW = c.PRES00 * fedg_pow(Rt * rP0 * RHOT, CPt / CVt) - PH;
+  const double FW = c.PRES00 * fedg_pow_tab(Rt * rP0 * RHOT, CPt / CVt, sPow) - PH;
   sF[0][p] = FD;
   sF[1][p] = FW;
   sF[2][p] = pt_ * FD;
@@ -231,8 +247,9 @@ __global__ void __launch_bounds__(N1* N1* N1) hevi_rhs3_kernel(const HeviArgs a)
   double drho = sRho[ij] * sI[N1 * k];
 #pragma unroll
   for (int l = 1; l < N1; ++l) drho = drho + sRho[ij + Nfp * l] * sI[l + N1 * k];
-  const double L4 = ldg(m.Lift, (uint32_t)(p + 4 * Np)), L5 = ldg(m.Lift, (uint32_t)(p + 5 * Np));
-  const double E33 = ldg(m.E33, o);
+  const double L4 = FLAT ? m.liftT[k + N1 * 4] : ldg(m.Lift, (uint32_t)(p + 4 * Np));
+  const double L5 = FLAT ? m.liftT[k + N1 * 5] : ldg(m.Lift, (uint32_t)(p + 5 * Np));
+  const double E33 = FLAT ? m.fE[2] : ldg(m.E33, o);
   const double RGsqrtV = 1.0 / GV;
   const double TD = -(E33 * dz[0] + (L4 * sdf[0][0][ij] + L5 * sdf[1][0][ij])) * RGsqrtV;
   const double TT = -(E33 * dz[2] + (L4 * sdf[0][2][ij] + L5 * sdf[1][2][ij])) * RGsqrtV;
@@ -243,7 +260,7 @@ __global__ void __launch_bounds__(N1* N1* N1) hevi_rhs3_kernel(const HeviArgs a)
     stg(a.coef, o, RHOT / dens);
     stg(a.coef, o + n, FD / dens);
     const double CPovCV = CPt / CVt;
-    stg(a.coef, o + 2 * n, CPovCV * c.PRES00 * fedg_pow(Rt / c.PRES00 * RHOT, CPovCV) / RHOT);
+    stg(a.coef, o + 2 * n, CPovCV * c.PRES00 * fedg_pow_tab(Rt / c.PRES00 * RHOT, CPovCV, sPow) / RHOT);
     stg(a.b3, o, a.impl_fac * TD - PVD + ldg(a.qs[VD], o));
     stg(a.b3, o + n, a.impl_fac * TW - PVW + ldg(a.qs[VW], o));
     stg(a.b3, o + 2 * n, a.impl_fac * TT - PVT + ldg(a.qs[VT], o));
@@ -862,6 +879,24 @@ LineArgs line_args(const DevMesh& m, const HeviArgs& a, const double* const Tex[
   return l;
 }
 
+// the residual kernels, on the flat box without the metric loads
+template <int N1>
+void launch_rhs_uv(fedg_ctx* ctx, const HeviArgs& a) {
+  const DevMesh& m = ctx->m;
+  if (m.flat && m.lift_tensor)
+    hipLaunchKernelGGL((hevi_rhs_uv_kernel<N1, true>), dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
+  else
+    hipLaunchKernelGGL((hevi_rhs_uv_kernel<N1, false>), dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
+}
+template <int N1>
+void launch_rhs3(fedg_ctx* ctx, const HeviArgs& a) {
+  const DevMesh& m = ctx->m;
+  if (m.flat && m.lift_tensor)
+    hipLaunchKernelGGL((hevi_rhs3_kernel<N1, true>), dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
+  else
+    hipLaunchKernelGGL((hevi_rhs3_kernel<N1, false>), dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
+}
+
 template <int N1>
 size_t lds_uv(int NeZ) {
   constexpr int nU = N1 * (N1 + 1) / 2 + LineSeg<N1>::SEGU;
@@ -911,7 +946,7 @@ int run_newton(fedg_ctx* ctx, HeviArgs& a, int max_it, double rtol, double* resi
   double r_prev = 0.0;
   int it = 0;
   for (; it < max_it; ++it) {
-    hipLaunchKernelGGL(hevi_rhs_uv_kernel<N1>, dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
+    launch_rhs_uv<N1>(ctx, a);
     FEDG_HIP_CHECK(hipGetLastError());
     double r2 = 0.0;
     const bool track = rtol > 0.0 || resid != nullptr;
@@ -924,7 +959,7 @@ int run_newton(fedg_ctx* ctx, HeviArgs& a, int max_it, double rtol, double* resi
     }
     hipLaunchKernelGGL(hevi_solve_uv_kernel<N1>, dim3(nbuv), dim3(64), luv, ctx->stream, a);
     FEDG_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(hevi_rhs3_kernel<N1>, dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
+    launch_rhs3<N1>(ctx, a);
     FEDG_HIP_CHECK(hipGetLastError());
     if (track) {
       hipLaunchKernelGGL(sumsq_kernel, dim3(256), dim3(256), 0, ctx->stream, a.b3, 3 * n, partial);
@@ -986,9 +1021,9 @@ int predict_iterate(fedg_ctx* ctx, HeviWork* w, HeviArgs& a, HeviGuess* g, doubl
   LineArgs la = line_args(m, a, g->Tex, g->scale, g->dt);
   const long nn = (long)m.Np * m.Ne;
   // residual of every line at q^n (pv holds q^n)
-  hipLaunchKernelGGL(hevi_rhs_uv_kernel<N1>, dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
+  launch_rhs_uv<N1>(ctx, a);
   FEDG_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(hevi_rhs3_kernel<N1>, dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
+  launch_rhs3<N1>(ctx, a);
   FEDG_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(line_resid_kernel, dim3(nlines), dim3(L), 0, ctx->stream, la, a.buv, a.b3, nn, w->r_plain);
   FEDG_HIP_CHECK(hipGetLastError());
@@ -1012,9 +1047,9 @@ int predict_iterate(fedg_ctx* ctx, HeviWork* w, HeviArgs& a, HeviGuess* g, doubl
   }
   // residual at the prediction; lines where it is not smaller restart from q^n
   la.line0 = 0;
-  hipLaunchKernelGGL(hevi_rhs_uv_kernel<N1>, dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
+  launch_rhs_uv<N1>(ctx, a);
   FEDG_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(hevi_rhs3_kernel<N1>, dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
+  launch_rhs3<N1>(ctx, a);
   FEDG_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(line_resid_kernel, dim3(nlines), dim3(L), 0, ctx->stream, la, a.buv, a.b3, nn, w->r_guess);
   FEDG_HIP_CHECK(hipGetLastError());
@@ -1034,10 +1069,9 @@ int predict_iterate(fedg_ctx* ctx, HeviWork* w, HeviArgs& a, HeviGuess* g, doubl
 
 template <int N1>
 int run_operator(fedg_ctx* ctx, HeviArgs& a) {
-  const DevMesh& m = ctx->m;
-  hipLaunchKernelGGL(hevi_rhs_uv_kernel<N1>, dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
+  launch_rhs_uv<N1>(ctx, a);
   FEDG_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(hevi_rhs3_kernel<N1>, dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
+  launch_rhs3<N1>(ctx, a);
   FEDG_HIP_CHECK(hipGetLastError());
   return FEDG_OK;
 }
