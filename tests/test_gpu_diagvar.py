@@ -144,3 +144,59 @@ def test_history_of_a_hevi_run_over_terrain(tmp_path):
     path, states = _history_run(tmp_path, mesh, bs, q, fedg.HeviStepper, 3, 3, 0.5)
     assert sorted(states) == [0, 3]
     _check_history(path, mesh, bs, states, 0.5)
+
+
+def test_history_with_tracers(tmp_path):
+    """Moist history: the tracers' own items, QDRY and the energies from the
+    mixture's specific heats (fedg_tracer_specific_heat), against the oracle."""
+    print("history HEVE p=3 4x2x3 moist", flush=True)
+    from test_oracle_negfix import THERMO
+    mesh = fedg.BoxMesh(3, 4, 2, 3, (0.0, 1e4), (0.0, 5e3), (0.0, 3e3), periodic=(True, True, False))
+    bs = cases.base_state(mesh, "constBVFreq")
+    q = cases.warm_bubble_state(mesh, bs, [(2.0, 1.5e3, (5e3, 2.5e3, 1.2e3))])
+    n = mesh.Np * mesh.Ne
+    rng = np.random.default_rng(5)
+    Qh = [0.01 * rng.uniform(0.5, 1.0, n), 1e-4 * rng.uniform(0.0, 1.0, n), 1e-5 * rng.uniform(0.0, 1.0, n)]
+    names = ("QV", "QC", "QI")
+    dm = fedg.DeviceMesh(mesh)
+    Dx, Dy = fedg.calc_phyd_hgrad(dm, bs.PRES_hyd, bs.PRES_hyd_ref)
+    st = fedg.HeveStepper(dm, bs, Dx, Dy, 0.05)
+    st.set_state(q)
+    gQ = [dev(a) for a in Qh]
+    tracers = {nm: (g, MASS[i], THERMO[1][i], CVQ[i], THERMO[3][i], ENGI0[i], nm, "kg/kg")
+               for i, (nm, g) in enumerate(zip(names, gQ))}
+    items = ("QV", "QI", "QDRY", "ENGI", "ENGT", "T", "RTOT", "CPTOT")
+    hf = history.HistoryFile(mesh, 0.05, basename=str(tmp_path / "hmoist"), tinterval=0.1, datatype="REAL8")
+    ah = history.AtmosHistory(dm, bs, hf, items, tracers=tracers)
+    states = {}
+    for step in range(3):
+        if step:
+            st.run(1)
+        s = st.get_state()
+        if ah.history(step, s):
+            states[step] = s
+    ah.close()
+    hf.close()
+    st.close()
+    assert sorted(states) == [0, 2]
+    om = O.OracleMesh(mesh)
+    R, CV, CP = mesh.field(), mesh.field(), mesh.field()
+    om.tracer_specific_heat([np.concatenate([a, np.zeros(mesh.field().size - n)]) for a in Qh], THERMO, R, CV, CP)
+    bsm = cases.BaseState(DENS_hyd=bs.DENS_hyd, PRES_hyd=bs.PRES_hyd, THERM_hyd=bs.THERM_hyd,
+                          PRES_hyd_ref=bs.PRES_hyd_ref, Rtot=R, CVtot=CV, CPtot=CP, CORIOLIS=bs.CORIOLIS,
+                          PT_hyd=bs.PT_hyd)
+    time, _, data = history.read_history(hf.path, mesh)
+    Dh, Ph = (np.ascontiguousarray(a[:n]) for a in (bs.DENS_hyd, bs.PRES_hyd))
+    zlev, pos_z = np.ascontiguousarray(mesh.height[:n]), np.ascontiguousarray(mesh.z[:n])
+    for k, step in enumerate(sorted(states)):
+        DD, DR, MZ, MX, MY = states[step]
+        DRf = mesh.field()
+        DRf[:n] = DR[:n]
+        P = np.ascontiguousarray(om.drhot2pres(DRf, bsm)[0][:n])
+        want = {"QV": Qh[0], "QI": Qh[2], "RTOT": R[:n], "CPTOT": CP[:n]}
+        for name in ("QDRY", "ENGI", "ENGT", "T"):
+            want[name] = om.atmos_calc_diagvar(name, DD[:n], MX[:n], MY[:n], MZ[:n], P, Dh, Ph,
+                                               np.ascontiguousarray(R[:n]), np.ascontiguousarray(CP[:n]), Q=Qh,
+                                               mass=MASS, cv=CVQ, engi0=ENGI0, zlev=zlev, pos_z=pos_z)
+        for name in items:
+            assert np.array_equal(data[name][k], want[name]), f"step {step} {name}"
