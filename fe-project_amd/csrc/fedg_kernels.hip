@@ -491,7 +491,10 @@ __device__ __forceinline__ void rusanov32_hevi(const Side32& M, const Side32& P,
 //    lateral faces y-, x+, y+, x- (interior and exterior sides loaded);
 //  - the z rows D(k,:), the VFilterPM1 column and the vertical Lift entries
 //    are wavefront-uniform; D rows i and j and the lateral Lift entries are
-//    lane-varying.
+//    lane-varying;
+//  - the pointwise fluxes of two variables go to LDS between barriers (three
+//    barriers per stage), the contractions, lift, tendency and RK update of
+//    each variable follow, the next variable's RK operands loaded ahead.
 template <typename R, bool LVL, bool HEVI = false>
 __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, const TendIn& in, const RkOut& rk,
                                             const int ke) {
@@ -499,7 +502,7 @@ __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, co
   constexpr bool F64 = std::is_same<R, double>::value;
   __shared__ R sDt[N1 * N1];
   __shared__ R sI[N1 * N1];
-  __shared__ R sF[2][3][Np];
+  __shared__ R sF[4][3][Np];
   __shared__ R sDF[5][NfpTot];
   __shared__ R sRho[Np];
   __shared__ double sPow[F64 ? 640 : 1];
@@ -721,18 +724,17 @@ __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, co
   };
   Ops nxt = load_ops(0);
   R drho = R(0);
-#pragma unroll
-  for (int v = 0; v < 5; ++v) {
+  auto wr = [&](int v) {
+    R f0, f1, f2;
+    flux(v, f0, f1, f2);
+    const int b = v & 3;
+    sF[b][0][p] = f0; sF[b][1][p] = f1; sF[b][2][p] = f2;
+  };
+  auto cu = [&](int v) {
     const Ops cur = nxt;
     if (v < 4) nxt = load_ops(v + 1);
     const R s = (R)cur.s, cor = (R)cur.cor;
-    const int b = v & 1;
-    {
-      R f0, f1, f2;
-      flux(v, f0, f1, f2);
-      sF[b][0][p] = f0; sF[b][1][p] = f1; sF[b][2][p] = f2;
-    }
-    __syncthreads();
+    const int b = v & 3;
     if (v == 0) {
       R zl[N1];
 #pragma unroll
@@ -791,7 +793,7 @@ __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, co
     const double TD = (double)T;
     if (rk.mode == 4) {
       stg(rk.out[v], o, TD);
-      continue;
+      return;
     }
     double nq;
     switch (rk.mode) {
@@ -821,7 +823,22 @@ __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, co
         stg(rk.DPout, o, (double)dpres32((float)nq, Th, (float)cur.Ph, gamm));
       }
     }
-  }
+  };
+  // the fluxes of two variables per barrier in four LDS buffers: variable 4
+  // reuses buffer 0, whose last reads (variable 0) precede the second barrier
+  wr(0); wr(1);
+  __syncthreads();
+  cu(0);
+  __builtin_amdgcn_sched_barrier(0);
+  cu(1);
+  wr(2); wr(3);
+  __syncthreads();
+  cu(2);
+  __builtin_amdgcn_sched_barrier(0);
+  cu(3);
+  wr(4);
+  __syncthreads();
+  cu(4);
 }
 
 // 512 threads, at most 128 VGPRs: two workgroups (16 wavefronts) per CU
