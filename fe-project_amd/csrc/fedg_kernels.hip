@@ -487,8 +487,10 @@ __device__ __forceinline__ void rusanov32_hevi(const Side32& M, const Side32& P,
 // share each SIMD (two elements per CU) and hide each other's LDS and memory
 // latency.
 //  - face work: wavefront 0 evaluates the bottom face and wavefront 7 the top
-//    face, whose interior nodes are their own; wavefronts 1, 2, 5, 6 the
-//    lateral faces y-, x+, y+, x- (interior and exterior sides loaded);
+//    face, whose interior nodes are their own; wavefronts 1, 2, 3, 4 the
+//    lateral faces y-, x+, y+, x- (interior and exterior sides loaded), so
+//    that wavefronts w and w + 4, which usually share a SIMD, hold one
+//    lateral face between them;
 //  - the z rows D(k,:), the VFilterPM1 column and the vertical Lift entries
 //    are wavefront-uniform; D rows i and j and the lateral Lift entries are
 //    lane-varying;
@@ -545,8 +547,8 @@ __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, co
   // ---- this wavefront's face: the interior (M) and exterior (P) sides ----
   const int ei = ke % m.NeX, ej = (ke / m.NeX) % m.NeY, ek = ke / (m.NeX * m.NeY);
   const int lane = t & 63;
-  // wavefront -> face: 0 bottom (4), 7 top (5), 1/2/5/6 lateral y-/x+/y+/x- (0..3), 3/4 none
-  const int f = (k == 0) ? 4 : (k == 7) ? 5 : (k == 1 || k == 2) ? k - 1 : (k == 5 || k == 6) ? k - 3 : -1;
+  // wavefront -> face: 0 bottom (4), 7 top (5), 1/2/3/4 lateral y-/x+/y+/x- (0..3), 5/6 none
+  const int f = (k == 0) ? 4 : (k == 7) ? 5 : (k >= 1 && k <= 4) ? k - 1 : -1;
   R Mv[9], Pv[9];
   int bc = FEDG_BND_NONE;
   int fp = 0;  // face node slot of sDF
