@@ -220,6 +220,15 @@ __device__ __forceinline__ double ldg(const double* base, uint32_t idx) {
 __device__ __forceinline__ void stg(double* base, uint32_t idx, double v) {
   *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + (idx << 3)) = v;
 }
+// The same with the non-temporal hint, for data a kernel streams once (read
+// once, or written for the next launch): it does not displace the lines that
+// other workgroups are about to read again (the face values of neighbours).
+__device__ __forceinline__ double ldg_nt(const double* base, uint32_t idx) {
+  return __builtin_nontemporal_load(reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + (idx << 3)));
+}
+__device__ __forceinline__ void stg_nt(double* base, uint32_t idx, double v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<double*>(reinterpret_cast<char*>(base) + (idx << 3)));
+}
 
 }  // namespace
 }  // namespace fedg

@@ -496,7 +496,10 @@ __device__ __forceinline__ void rusanov32_hevi(const Side32& M, const Side32& P,
 //    lane-varying;
 //  - the pointwise fluxes of two variables go to LDS between barriers (three
 //    barriers per stage), the contractions, lift, tendency and RK update of
-//    each variable follow, the next variable's RK operands loaded ahead.
+//    each variable follow, the next variable's RK operands loaded ahead;
+//  - the RK operands are loaded and the stage's outputs stored with the
+//    non-temporal hint (streamed once), so that the state and DPRES lines the
+//    neighbouring elements gather stay in the caches.
 template <typename R, bool LVL, bool HEVI = false>
 __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, const TendIn& in, const RkOut& rk,
                                             const int ke) {
@@ -711,8 +714,8 @@ __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, co
   };
   auto load_ops = [&](int v) {
     Ops op{0.0, 0.0, 0.0, 0.0, 0.0};
-    if (rk.mode == 1) op.rA = ldg(rk.var0[v], o);
-    if (rk.mode == 1 || rk.mode == 2) op.rB = ldg(rk.varTmp[v], o);
+    if (rk.mode == 1) op.rA = ldg_nt(rk.var0[v], o);
+    if (rk.mode == 1 || rk.mode == 2) op.rB = ldg_nt(rk.varTmp[v], o);
     if (v == VU) {
       op.s = hyd(in.DPhydDx, 3, o, ek0, p);
       op.cor = ldg(in.CORIOLIS, o2d);
@@ -794,17 +797,17 @@ __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, co
     }
     const double TD = (double)T;
     if (rk.mode == 4) {
-      stg(rk.out[v], o, TD);
+      stg_nt(rk.out[v], o, TD);
       return;
     }
     double nq;
     switch (rk.mode) {
       case 0:
-        stg(rk.varTmp[v], o, q[v] + rk.cb * TD);
+        stg_nt(rk.varTmp[v], o, q[v] + rk.cb * TD);
         nq = q[v] + rk.ca * TD;
         break;
       case 1:
-        stg(rk.varTmp[v], o, cur.rB + rk.cb * TD);
+        stg_nt(rk.varTmp[v], o, cur.rB + rk.cb * TD);
         nq = cur.rA + rk.ca * TD;
         break;
       case 2:
@@ -814,15 +817,15 @@ __device__ __forceinline__ void heve7_body(const DevMesh& m, const Consts& c, co
         nq = q[v] + rk.cb * TD;
         break;
     }
-    stg(rk.out[v], o, nq);
+    stg_nt(rk.out[v], o, nq);
     if (v == VT) {
       if constexpr (F64) {
         if (in.Rtot != nullptr)
-          stg(rk.DPout, o, dpres_of_tab(nq, Th, cur.Ph, ldg(in.Rtot, o), ldg(in.CVtot, o), ldg(in.CPtot, o), c, sPow));
+          stg_nt(rk.DPout, o, dpres_of_tab(nq, Th, cur.Ph, ldg(in.Rtot, o), ldg(in.CVtot, o), ldg(in.CPtot, o), c, sPow));
         else
-          stg(rk.DPout, o, dpres_of_tab(nq, Th, cur.Ph, c.Rdry, c.CVdry, c.CPdry, c, sPow));
+          stg_nt(rk.DPout, o, dpres_of_tab(nq, Th, cur.Ph, c.Rdry, c.CVdry, c.CPdry, c, sPow));
       } else {
-        stg(rk.DPout, o, (double)dpres32((float)nq, Th, (float)cur.Ph, gamm));
+        stg_nt(rk.DPout, o, (double)dpres32((float)nq, Th, (float)cur.Ph, gamm));
       }
     }
   };
