@@ -1886,6 +1886,8 @@ struct ProbeArgs {
   double* w[11];
   int NeX, NeY, NeZ;
 };
+// The fields beyond the first NG (the RK operands) are read, and all fields
+// written, with the non-temporal hint, as the stage kernel streams them.
 template <int NR, int NW, int NG>
 __global__ void __launch_bounds__(256) stage_probe_kernel(const ProbeArgs p) {
   const int e = blockIdx.x, t = threadIdx.x;
@@ -1893,8 +1895,13 @@ __global__ void __launch_bounds__(256) stage_probe_kernel(const ProbeArgs p) {
   double a0 = 0.0, a1 = 0.0;
 #pragma unroll
   for (int f = 0; f < NR; ++f) {
-    a0 += p.r[f][base + t];
-    a1 += p.r[f][base + t + 256];
+    if (f < NG) {
+      a0 += p.r[f][base + t];
+      a1 += p.r[f][base + t + 256];
+    } else {
+      a0 += __builtin_nontemporal_load(p.r[f] + base + t);
+      a1 += __builtin_nontemporal_load(p.r[f] + base + t + 256);
+    }
   }
   const int ei = e % p.NeX, ej = (e / p.NeX) % p.NeY, ek = e / (p.NeX * p.NeY);
   for (int idx = t; idx < 384; idx += 256) {
@@ -1913,8 +1920,8 @@ __global__ void __launch_bounds__(256) stage_probe_kernel(const ProbeArgs p) {
   }
 #pragma unroll
   for (int f = 0; f < NW; ++f) {
-    p.w[f][base + t] = a0 + f;
-    p.w[f][base + t + 256] = a1 + f;
+    __builtin_nontemporal_store(a0 + f, p.w[f] + base + t);
+    __builtin_nontemporal_store(a1 + f, p.w[f] + base + t + 256);
   }
 }
 
