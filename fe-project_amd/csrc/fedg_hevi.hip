@@ -85,7 +85,7 @@ __device__ __forceinline__ uint32_t vz_nbr(const HeviArgs& a, int ke, int f, int
 // constants instead of loaded; every operation is kept, so the results are
 // the same bits.
 template <int N1, bool FLAT>
-__global__ void __launch_bounds__(N1* N1* N1) hevi_rhs_uv_kernel(const HeviArgs a) {
+__device__ __forceinline__ void hevi_rhs_uv_body(const HeviArgs& a) {
   constexpr int Np = N1 * N1 * N1, Nfp = N1 * N1;
   __shared__ double sdf[2][2][Nfp];  // face (bottom, top), var (MOMX, MOMY), node
   __shared__ double sPow[640];        // the pow tables (the same values as the global ones)
@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(N1* N1* N1) hevi_rhs_uv_kernel(const HeviArgs 
 // ---------------------------------------------------------------------------
 // 3. vi_cal_del_flux_dyn (:1161-1328) + eval_Ax (:111-326)
 template <int N1, bool FLAT>
-__global__ void __launch_bounds__(N1* N1* N1) hevi_rhs3_kernel(const HeviArgs a) {
+__device__ __forceinline__ void hevi_rhs3_body(const HeviArgs& a) {
   constexpr int Np = N1 * N1 * N1, Nfp = N1 * N1;
   __shared__ double sF[3][Np];      // DENS, MOMZ, RHOT vertical fluxes
   __shared__ double sRho[Np];
@@ -879,11 +879,32 @@ LineArgs line_args(const DevMesh& m, const HeviArgs& a, const double* const Tex[
   return l;
 }
 
+template <int N1, bool FLAT>
+__global__ void __launch_bounds__(N1* N1* N1) hevi_rhs_uv_kernel(const HeviArgs a) {
+  hevi_rhs_uv_body<N1, FLAT>(a);
+}
+template <int N1, bool FLAT>
+__global__ void __launch_bounds__(N1* N1* N1) hevi_rhs3_kernel(const HeviArgs a) {
+  hevi_rhs3_body<N1, FLAT>(a);
+}
+// p = 7 on the flat box at no more than 128 VGPRs (a few registers spill):
+// two 512-thread workgroups per CU instead of one at 135-143 VGPRs
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
+hevi_rhs_uv8_flat_kernel(const HeviArgs a) {
+  hevi_rhs_uv_body<8, true>(a);
+}
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
+hevi_rhs38_flat_kernel(const HeviArgs a) {
+  hevi_rhs3_body<8, true>(a);
+}
+
 // the residual kernels, on the flat box without the metric loads
 template <int N1>
 void launch_rhs_uv(fedg_ctx* ctx, const HeviArgs& a) {
   const DevMesh& m = ctx->m;
-  if (m.flat && m.lift_tensor)
+  if (N1 == 8 && m.flat && m.lift_tensor)
+    hipLaunchKernelGGL(hevi_rhs_uv8_flat_kernel, dim3(m.Ne), dim3(512), 0, ctx->stream, a);
+  else if (m.flat && m.lift_tensor)
     hipLaunchKernelGGL((hevi_rhs_uv_kernel<N1, true>), dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
   else
     hipLaunchKernelGGL((hevi_rhs_uv_kernel<N1, false>), dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
@@ -891,7 +912,9 @@ void launch_rhs_uv(fedg_ctx* ctx, const HeviArgs& a) {
 template <int N1>
 void launch_rhs3(fedg_ctx* ctx, const HeviArgs& a) {
   const DevMesh& m = ctx->m;
-  if (m.flat && m.lift_tensor)
+  if (N1 == 8 && m.flat && m.lift_tensor)
+    hipLaunchKernelGGL(hevi_rhs38_flat_kernel, dim3(m.Ne), dim3(512), 0, ctx->stream, a);
+  else if (m.flat && m.lift_tensor)
     hipLaunchKernelGGL((hevi_rhs3_kernel<N1, true>), dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
   else
     hipLaunchKernelGGL((hevi_rhs3_kernel<N1, false>), dim3(m.Ne), dim3(N1 * N1 * N1), 0, ctx->stream, a);
