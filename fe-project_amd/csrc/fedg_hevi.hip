@@ -41,7 +41,8 @@ struct HeviArgs {
   Consts c;
   const double* qs[5];     // stage state q* (VID order), (Np,NeA)
   const double* q0[5];     // var0 = q^n
-  double* pv[5];           // Newton iterate, (Np,Ne)
+  double* pv[5];           // Newton iterate, (Np,Ne), written by the solves
+  const double* pin[5];    // the iterate as read: pv, or q^n for a variable no solve has updated yet
   const double* DENS_hyd;
   const double* PRES_hyd;
   const double* Rtot;
@@ -134,8 +135,8 @@ __device__ __forceinline__ void hevi_rhs_uv_body(const HeviArgs& a) {
       const double al = (nz * nz) * (sM > sP ? sM : sP);
       stg(a.alph, (uint32_t)(f * Nfp + ij) + (uint32_t)(2 * Nfp) * ke, al);
       const double tmp1 = -0.5 * (FLAT ? m.fF[4 + f] : ldg(m.Fscale, gfp)) * al;
-      sdf[f][0][ij] = tmp1 * (ldg(a.pv[VU], iP) - ldg(a.pv[VU], iM));
-      sdf[f][1][ij] = tmp1 * (ldg(a.pv[VV], iP) - ldg(a.pv[VV], iM));
+      sdf[f][0][ij] = tmp1 * (ldg(a.pin[VU], iP) - ldg(a.pin[VU], iM));
+      sdf[f][1][ij] = tmp1 * (ldg(a.pin[VV], iP) - ldg(a.pin[VV], iM));
     }
   }
   __syncthreads();
@@ -148,8 +149,8 @@ __device__ __forceinline__ void hevi_rhs_uv_body(const HeviArgs& a) {
   const double Tx = -LDx * RGsqrtV, Ty = -LDy * RGsqrtV;
   if (a.want_b) {
     const uint32_t n = (uint32_t)Np * m.Ne;
-    stg(a.buv, o, a.impl_fac * Tx - ldg(a.pv[VU], o) + ldg(a.qs[VU], o));
-    stg(a.buv, o + n, a.impl_fac * Ty - ldg(a.pv[VV], o) + ldg(a.qs[VV], o));
+    stg(a.buv, o, a.impl_fac * Tx - ldg(a.pin[VU], o) + ldg(a.qs[VU], o));
+    stg(a.buv, o + n, a.impl_fac * Ty - ldg(a.pin[VV], o) + ldg(a.qs[VV], o));
   } else {
     stg(a.T[VU], o, Tx);
     stg(a.T[VV], o, Ty);
@@ -195,9 +196,9 @@ __device__ __forceinline__ void hevi_rhs3_body(const HeviArgs& a) {
         const uint32_t n = ix[s];
         const int ken = n / Np;
         GV[s] = FLAT ? 1.0 : gsqrtv(a, ken, n - (uint32_t)Np * ken, N1);
-        DD[s] = ldg(a.pv[VD], n);
-        MZ[s] = ldg(a.pv[VW], n);
-        DT[s] = ldg(a.pv[VT], n);
+        DD[s] = ldg(a.pin[VD], n);
+        MZ[s] = ldg(a.pin[VW], n);
+        DT[s] = ldg(a.pin[VT], n);
         const double dens = ldg(a.DENS_hyd, n) + DD[s];
         const double PH = ldg(a.PRES_hyd, n);
         const double rhot_hyd = PRES0ovRdry * fedg_pow_tab(PH / c.PRES00, rgamm, sPow);
@@ -205,11 +206,11 @@ __device__ __forceinline__ void hevi_rhs3_body(const HeviArgs& a) {
         dpres[s] = c.PRES00 * fedg_pow_tab(ldg(a.Rtot, n) * rP0 * dens * pott[s], ldg(a.CPtot, n) / ldg(a.CVtot, n), sPow);
         dpres[s] = dpres[s] - PH;
         const double g13 = FLAT ? 0.0 : ldg(m.G13, n), g23 = FLAT ? 0.0 : ldg(m.G23, n);
-        MW[s] = MZ[s] + GV[s] * g13 * ldg(a.pv[VU], n) + GV[s] * g23 * ldg(a.pv[VV], n);
+        MW[s] = MZ[s] + GV[s] * g13 * ldg(a.pin[VU], n) + GV[s] * g23 * ldg(a.pin[VV], n);
       }
       if ((kz == 0 || kz == a.NeZ - 1) && iM == iP) {
         const double g13 = FLAT ? 0.0 : ldg(m.G13, iM), g23 = FLAT ? 0.0 : ldg(m.G23, iM);
-        MZ[1] = -MZ[0] - 2.0 * GV[0] * (g13 * ldg(a.pv[VU], iM) + g23 * ldg(a.pv[VV], iM));
+        MZ[1] = -MZ[0] - 2.0 * GV[0] * (g13 * ldg(a.pin[VU], iM) + g23 * ldg(a.pin[VV], iM));
         MW[1] = -MW[0];
       }
       const double nz = FLAT ? (f == 0 ? -1.0 : 1.0) : ldg(m.nz, gfp);
@@ -222,9 +223,9 @@ __device__ __forceinline__ void hevi_rhs3_body(const HeviArgs& a) {
   }
   // vertical fluxes of the node
   const double GV = FLAT ? 1.0 : gsqrtv(a, ke, p, N1);
-  const double PVD = ldg(a.pv[VD], o), PVW = ldg(a.pv[VW], o), PVT = ldg(a.pv[VT], o);
+  const double PVD = ldg(a.pin[VD], o), PVW = ldg(a.pin[VW], o), PVT = ldg(a.pin[VT], o);
   const double G13o = FLAT ? 0.0 : ldg(m.G13, o), G23o = FLAT ? 0.0 : ldg(m.G23, o);
-  const double FD = PVW + GV * G13o * ldg(a.pv[VU], o) + GV * G23o * ldg(a.pv[VV], o);
+  const double FD = PVW + GV * G13o * ldg(a.pin[VU], o) + GV * G23o * ldg(a.pin[VV], o);
   const double PH = ldg(a.PRES_hyd, o), DH = ldg(a.DENS_hyd, o);
   const double RHOT = P0ovR * fedg_pow_tab(PH / c.PRES00, rgamm, sPow) + PVT;
   const double pt_ = RHOT / (PVD + DH);
@@ -517,8 +518,8 @@ __global__ void __launch_bounds__(64) hevi_solve_uv_kernel(const HeviArgs a) {
       for (int kz = 0; kz < NeZ; ++kz) {
         const uint32_t q = (uint32_t)Np * (ke2D + kz * m.Ne2D) + ij + Nfp * pv;
         const double* Xk = X + (size_t)kz * 3 * N;
-        stg(a.pv[VU], q, ldg(a.pv[VU], q) + Xk[pv]);
-        stg(a.pv[VV], q, ldg(a.pv[VV], q) + Xk[pv + N]);
+        stg(a.pv[VU], q, ldg(a.pin[VU], q) + Xk[pv]);
+        stg(a.pv[VV], q, ldg(a.pin[VV], q) + Xk[pv + N]);
       }
     }
     __syncthreads();
@@ -675,9 +676,10 @@ __global__ void __launch_bounds__(64) hevi_solve3_kernel(const HeviArgs a) {
     }
     if (lv && act) {
       double* dst = v == 0 ? a.pv[VD] : (v == 1 ? a.pv[VW] : a.pv[VT]);
+      const double* src = v == 0 ? a.pin[VD] : (v == 1 ? a.pin[VW] : a.pin[VT]);
       for (int kz = 0; kz < NeZ; ++kz) {
         const uint32_t q = (uint32_t)Np * (ke2D + kz * m.Ne2D) + ij + Nfp * pv;
-        stg(dst, q, ldg(dst, q) + X[(size_t)kz * 4 * n3 + r]);
+        stg(dst, q, ldg(src, q) + X[(size_t)kz * 4 * n3 + r]);
       }
     }
   }
@@ -982,6 +984,8 @@ int run_newton(fedg_ctx* ctx, HeviArgs& a, int max_it, double rtol, double* resi
     }
     hipLaunchKernelGGL(hevi_solve_uv_kernel<N1>, dim3(nbuv), dim3(64), luv, ctx->stream, a);
     FEDG_HIP_CHECK(hipGetLastError());
+    a.pin[VU] = a.pv[VU];
+    a.pin[VV] = a.pv[VV];
     launch_rhs3<N1>(ctx, a);
     FEDG_HIP_CHECK(hipGetLastError());
     if (track) {
@@ -999,6 +1003,9 @@ int run_newton(fedg_ctx* ctx, HeviArgs& a, int max_it, double rtol, double* resi
     }
     hipLaunchKernelGGL(hevi_solve3_kernel<N1>, dim3(nb3), dim3(64), l3, ctx->stream, a);
     FEDG_HIP_CHECK(hipGetLastError());
+    a.pin[VD] = a.pv[VD];
+    a.pin[VW] = a.pv[VW];
+    a.pin[VT] = a.pv[VT];
   }
   *it_done = it;
   return FEDG_OK;
@@ -1117,6 +1124,7 @@ int hevi_cal_vi(fedg_ctx* ctx, HeviWork* w, double* const tend[5], const double*
     a.qs[v] = q[v];
     a.q0[v] = var0[v];
     a.pv[v] = w->pv[v];
+    a.pin[v] = var0[v];
     a.T[v] = tend[v];
   }
   a.DENS_hyd = DENS_hyd;
@@ -1132,11 +1140,19 @@ int hevi_cal_vi(fedg_ctx* ctx, HeviWork* w, double* const tend[5], const double*
   a.NeZ = m.Ne / m.Ne2D;
   const long n = (long)m.Np * m.Ne;
   *it_done = 0;
-  for (int v = 0; v < 5; ++v)
-    FEDG_HIP_CHECK(hipMemcpyAsync(w->pv[v], var0[v], sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+  // The first Newton iterate is q^n: the kernels read it from var0 until a
+  // solve has written the variable's update to pv (run_newton moves a.pin).
+  // The predicted iterate and a solve of no iteration start from a copy.
+  const bool implicit = fabs(impl_fac) > 0.0;
+  if (implicit && ((guess && guess->mlp) || max_it < 1)) {
+    for (int v = 0; v < 5; ++v) {
+      FEDG_HIP_CHECK(hipMemcpyAsync(w->pv[v], var0[v], sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+      a.pin[v] = w->pv[v];
+    }
+  }
   auto dispatch = [&](auto n1c) -> int {
     constexpr int N1 = decltype(n1c)::value;
-    if (fabs(impl_fac) > 0.0) {
+    if (implicit) {
       a.want_b = 1;
       double r0 = 0.0;
       if (guess && guess->mlp) {
@@ -1217,8 +1233,9 @@ int hevi_harvest(fedg_ctx* ctx, HeviWork* w, const double* const q[5], const dou
 // loops).  Every device that runs the library holds its own copy until the
 // process ends; every call on a device shares it and finds the previous
 // call's contents (the solve writes each part before reading it: the iterate
-// is copied from var0, the rhs, damping and linearisation coefficients and
-// the partial sums are written by their kernels, so nothing needs clearing).
+// is read from var0 until a solve has written it, the rhs, damping and
+// linearisation coefficients and the partial sums are written by their
+// kernels, so nothing needs clearing).
 constexpr size_t kScratchDoubles = (size_t)8 << 20;  // 64 MiB
 
 static __device__ __align__(256) double scratch[kScratchDoubles];
