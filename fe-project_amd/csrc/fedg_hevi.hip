@@ -84,7 +84,9 @@ __device__ __forceinline__ uint32_t vz_nbr(const HeviArgs& a, int ke, int f, int
 // nz = -1 / +1 on the bottom / top faces, E33 and Fscale constant and a
 // tensor-product Lift, so those values are taken from the context's
 // constants instead of loaded; every operation is kept, so the results are
-// the same bits.
+// the same bits.  The outputs (and, in hevi_rhs3_body, the coefficients) and
+// the stage state q* are streamed with the non-temporal hint, so that the
+// iterate lines the vertical neighbours gather stay cached.
 template <int N1, bool FLAT>
 __device__ __forceinline__ void hevi_rhs_uv_body(const HeviArgs& a) {
   constexpr int Np = N1 * N1 * N1, Nfp = N1 * N1;
@@ -149,11 +151,11 @@ __device__ __forceinline__ void hevi_rhs_uv_body(const HeviArgs& a) {
   const double Tx = -LDx * RGsqrtV, Ty = -LDy * RGsqrtV;
   if (a.want_b) {
     const uint32_t n = (uint32_t)Np * m.Ne;
-    stg(a.buv, o, a.impl_fac * Tx - ldg(a.pin[VU], o) + ldg(a.qs[VU], o));
-    stg(a.buv, o + n, a.impl_fac * Ty - ldg(a.pin[VV], o) + ldg(a.qs[VV], o));
+    stg_nt(a.buv, o, a.impl_fac * Tx - ldg(a.pin[VU], o) + ldg_nt(a.qs[VU], o));
+    stg_nt(a.buv, o + n, a.impl_fac * Ty - ldg(a.pin[VV], o) + ldg_nt(a.qs[VV], o));
   } else {
-    stg(a.T[VU], o, Tx);
-    stg(a.T[VV], o, Ty);
+    stg_nt(a.T[VU], o, Tx);
+    stg_nt(a.T[VV], o, Ty);
   }
 }
 
@@ -258,17 +260,17 @@ __device__ __forceinline__ void hevi_rhs3_body(const HeviArgs& a) {
   if (a.want_b) {
     const uint32_t n = (uint32_t)Np * m.Ne;
     const double dens = DH + PVD;
-    stg(a.coef, o, RHOT / dens);
-    stg(a.coef, o + n, FD / dens);
+    stg_nt(a.coef, o, RHOT / dens);
+    stg_nt(a.coef, o + n, FD / dens);
     const double CPovCV = CPt / CVt;
-    stg(a.coef, o + 2 * n, CPovCV * c.PRES00 * fedg_pow_tab(Rt / c.PRES00 * RHOT, CPovCV, sPow) / RHOT);
-    stg(a.b3, o, a.impl_fac * TD - PVD + ldg(a.qs[VD], o));
-    stg(a.b3, o + n, a.impl_fac * TW - PVW + ldg(a.qs[VW], o));
-    stg(a.b3, o + 2 * n, a.impl_fac * TT - PVT + ldg(a.qs[VT], o));
+    stg_nt(a.coef, o + 2 * n, CPovCV * c.PRES00 * fedg_pow_tab(Rt / c.PRES00 * RHOT, CPovCV, sPow) / RHOT);
+    stg_nt(a.b3, o, a.impl_fac * TD - PVD + ldg_nt(a.qs[VD], o));
+    stg_nt(a.b3, o + n, a.impl_fac * TW - PVW + ldg_nt(a.qs[VW], o));
+    stg_nt(a.b3, o + 2 * n, a.impl_fac * TT - PVT + ldg_nt(a.qs[VT], o));
   } else {
-    stg(a.T[VD], o, TD);
-    stg(a.T[VW], o, TW);
-    stg(a.T[VT], o, TT);
+    stg_nt(a.T[VD], o, TD);
+    stg_nt(a.T[VW], o, TW);
+    stg_nt(a.T[VT], o, TT);
   }
 }
 
@@ -700,7 +702,8 @@ __global__ void hevi_tend_kernel(double* T0, double* T1, double* T2, double* T3,
 }
 
 // the same tendency, then the stepper's StoreImplicit on the node: var0 and
-// varTmp (first stage) take q, and q = q + impl_fac T
+// varTmp (first stage) take q, and q = q + impl_fac T (streamed: non-temporal
+// loads and stores)
 struct TendStore {
   double* T[5];
   const double* p[5];
@@ -715,14 +718,14 @@ __global__ void hevi_tend_store_kernel(const TendStore a, long n) {
   if (t >= n) return;
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
-    const double q = a.q[v][t];
-    const double T = (a.p[v][t] - q) / a.impl_fac;
-    a.T[v][t] = T;
+    const double q = __builtin_nontemporal_load(a.q[v] + t);
+    const double T = (__builtin_nontemporal_load(a.p[v] + t) - q) / a.impl_fac;
+    __builtin_nontemporal_store(T, a.T[v] + t);
     if (a.copy) {
-      a.var0[v][t] = q;
-      a.varTmp[v][t] = q;
+      __builtin_nontemporal_store(q, a.var0[v] + t);
+      __builtin_nontemporal_store(q, a.varTmp[v] + t);
     }
-    a.q[v][t] = q + a.impl_fac * T;
+    __builtin_nontemporal_store(q + a.impl_fac * T, a.q[v] + t);
   }
 }
 

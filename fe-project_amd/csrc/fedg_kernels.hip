@@ -1813,27 +1813,29 @@ __global__ void axpy2_set_kernel(double* y, const double* a, double ca, const do
 }
 // one pass of the IMEX stage advance over the five prognostics
 // (blockIdx.y: variable); the expressions of axpy2_acc / axpy2_set.  S: the
-// stage (its loads unrolled so that they are all in flight at once), 0: last
+// stage (its loads unrolled so that they are all in flight at once), 0: last.
+// Every operand is streamed once: loads and stores carry the non-temporal hint
+// (as do StoreImplicit's).
 template <int S>
 __global__ void __launch_bounds__(256) imex_advance_kernel(const ImexAdvance a, long n) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int v = blockIdx.y;
   if (t >= n) return;
-  const double Te = a.Te[v][t], Ti = a.Ti[v][t];
+  const double Te = __builtin_nontemporal_load(a.Te[v] + t), Ti = __builtin_nontemporal_load(a.Ti[v] + t);
   if constexpr (S == 0) {
-    a.q[v][t] = a.vt[v][t] + a.cbe * Te + a.cbi * Ti;
+    __builtin_nontemporal_store(__builtin_nontemporal_load(a.vt[v] + t) + a.cbe * Te + a.cbi * Ti, a.q[v] + t);
   } else {
     double e[S], im[S];
 #pragma unroll
     for (int s2 = 0; s2 < S; ++s2) {
-      e[s2] = a.Tes[s2][v][t];
-      im[s2] = a.Tis[s2][v][t];
+      e[s2] = __builtin_nontemporal_load(a.Tes[s2][v] + t);
+      im[s2] = __builtin_nontemporal_load(a.Tis[s2][v] + t);
     }
-    a.vt[v][t] = a.vt[v][t] + a.cbe * Te + a.cbi * Ti;
-    double q = a.v0[v][t];
+    __builtin_nontemporal_store(__builtin_nontemporal_load(a.vt[v] + t) + a.cbe * Te + a.cbi * Ti, a.vt[v] + t);
+    double q = __builtin_nontemporal_load(a.v0[v] + t);
 #pragma unroll
     for (int s2 = 0; s2 < S; ++s2) q = q + a.ae[s2] * e[s2] + a.ai[s2] * im[s2];
-    a.q[v][t] = q;
+    __builtin_nontemporal_store(q, a.q[v] + t);
   }
 }
 
@@ -1868,12 +1870,12 @@ __global__ void __launch_bounds__(256) store_implicit_kernel(const StoreImpl a, 
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int v = blockIdx.y;
   if (t >= n) return;
-  const double q = a.q[v][t];
+  const double q = __builtin_nontemporal_load(a.q[v] + t);
   if (a.copy) {
-    a.var0[v][t] = q;
-    a.varTmp[v][t] = q;
+    __builtin_nontemporal_store(q, a.var0[v] + t);
+    __builtin_nontemporal_store(q, a.varTmp[v] + t);
   }
-  a.q[v][t] = q + a.impl_fac * a.Ti[v][t];
+  __builtin_nontemporal_store(q + a.impl_fac * __builtin_nontemporal_load(a.Ti[v] + t), a.q[v] + t);
 }
 
 // Access-pattern probe (measurement only): the loads and stores of one p = 7
