@@ -1889,6 +1889,7 @@ struct fedg_hevi {
   double* q[5];  // (Np,NeA) state, VID order
   double *PRES, *DPRES;
   double *DENS_hyd, *PRES_hyd, *THERM_hyd, *DPhydDx, *DPhydDy, *Rtot, *CVtot, *CPtot, *CORIOLIS;
+  double* RHOT_hyd;  // the hydrostatic RHOT of the vertical residual kernels, formed once from PRES_hyd
   int max_it;
   double rtol;
   int mfilter;  // modal filter after every step
@@ -1946,7 +1947,7 @@ int fedg_hevi_create(fedg_ctx* ctx, const char* scheme, double dt, const double*
   if (rc == FEDG_OK) rc = hevi_work_get(ctx, &w);
   const DevMesh& m = ctx->m;
   const size_t nA = (size_t)m.Np * m.NeA, cor = (size_t)m.Nfp * m.Ne2DA;
-  if (rc == FEDG_OK) rc = dev_upload<double>(&st->block, nullptr, nA * 15 + cor);
+  if (rc == FEDG_OK) rc = dev_upload<double>(&st->block, nullptr, nA * 16 + cor);
   if (rc != FEDG_OK) {
     fedg_hevi_destroy(st);
     return rc;
@@ -1966,6 +1967,13 @@ int fedg_hevi_create(fedg_ctx* ctx, const char* scheme, double dt, const double*
       fedg_hevi_destroy(st);
       return FEDG_ERR_HIP;
     }
+  }
+  st->RHOT_hyd = p;
+  p += nA;
+  rc = launch_rhot_hyd(ctx, st->RHOT_hyd, st->PRES_hyd, (long)nA);
+  if (rc != FEDG_OK) {
+    fedg_hevi_destroy(st);
+    return rc;
   }
   st->CORIOLIS = p;
   if (hipMemcpy(p, CORIOLIS, sizeof(double) * cor, hipMemcpyHostToDevice) != hipSuccess) {
@@ -2081,7 +2089,8 @@ static int hevi_step(fedg_hevi* st, hipEvent_t* ev) {
     }
     if (ev) FEDG_HIP_CHECK(hipEventRecord(ev[0], ctx->stream));
     TRY(hevi_cal_vi(ctx, w, Ti, qv, v0, st->DENS_hyd, st->PRES_hyd, st->Rtot, st->CVtot, st->CPtot, impl_fac,
-                    st->max_it, st->rtol, nullptr, &it, st->mlp ? &g : nullptr, fuse_store ? &sto : nullptr));
+                    st->max_it, st->rtol, nullptr, &it, st->mlp ? &g : nullptr, fuse_store ? &sto : nullptr,
+                    st->RHOT_hyd));
     if (ev) FEDG_HIP_CHECK(hipEventRecord(ev[1], ctx->stream));
     st->guess_lines += g.lines;
     st->guess_fallbacks += g.fallbacks;

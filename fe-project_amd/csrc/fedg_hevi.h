@@ -64,7 +64,12 @@ void hevi_work_free(fedg_ctx* ctx);
 int hevi_cal_vi(fedg_ctx* ctx, HeviWork* w, double* const tend[5], const double* const q[5],
                 const double* const var0[5], const double* DENS_hyd, const double* PRES_hyd, const double* Rtot,
                 const double* CVtot, const double* CPtot, double impl_fac, int max_it, double rtol, double* resid,
-                int* it_done, HeviGuess* guess = nullptr, const HeviStore* store = nullptr);
+                int* it_done, HeviGuess* guess = nullptr, const HeviStore* store = nullptr,
+                const double* RHOT_hyd = nullptr);
+// RHOT_hyd = PRES00/Rdry (PRES_hyd/PRES00)^(CVdry/CPdry) over n nodes: the
+// value the residual kernels form from PRES_hyd, bit for bit (their RHOT_hyd
+// argument of hevi_cal_vi)
+int launch_rhot_hyd(fedg_ctx* ctx, double* RHOT_hyd, const double* PRES_hyd, long n);
 
 // Features and targets of nrows lines (line = (line0 + r*stride) mod nlines)
 // of a converged solve (iterate in w->pv) into X (nrows x n_in) and Y

@@ -45,6 +45,7 @@ struct HeviArgs {
   const double* pin[5];    // the iterate as read: pv, or q^n for a variable no solve has updated yet
   const double* DENS_hyd;
   const double* PRES_hyd;
+  const double* RHOT_hyd;  // PRES00/Rdry (PRES_hyd/PRES00)^(CVdry/CPdry) per node, or null: formed in the kernels
   const double* Rtot;
   const double* CVtot;
   const double* CPtot;
@@ -126,7 +127,8 @@ __device__ __forceinline__ void hevi_rhs_uv_body(const HeviArgs& a) {
         const double g13 = FLAT ? 0.0 : ldg(m.G13, n), g23 = FLAT ? 0.0 : ldg(m.G23, n);
         rdens0[s] = 1.0 / (ldg(a.DENS_hyd, n) + ldg(a.q0[VD], n));
         wt0[s] = (ldg(a.q0[VW], n) * RGsqrtV + g13 * MOMX0 + g23 * MOMY0) * rdens0[s];
-        const double rhot_hyd = PRES0ovRdry * fedg_pow_tab(ldg(a.PRES_hyd, n) / c.PRES00, rgamm, sPow);
+        const double rhot_hyd = a.RHOT_hyd ? ldg(a.RHOT_hyd, n)
+                                           : PRES0ovRdry * fedg_pow_tab(ldg(a.PRES_hyd, n) / c.PRES00, rgamm, sPow);
         pres0[s] = c.PRES00 * fedg_pow_tab(Rt * rP0 * (rhot_hyd + DRHOT0), CPovCV, sPow);
         const double Gxz = rgam2 * (G11_ * g13 + G12_ * g23);
         const double Gyz = rgam2 * (G12_ * g13 + G22_ * g23);
@@ -203,7 +205,7 @@ __device__ __forceinline__ void hevi_rhs3_body(const HeviArgs& a) {
         DT[s] = ldg(a.pin[VT], n);
         const double dens = ldg(a.DENS_hyd, n) + DD[s];
         const double PH = ldg(a.PRES_hyd, n);
-        const double rhot_hyd = PRES0ovRdry * fedg_pow_tab(PH / c.PRES00, rgamm, sPow);
+        const double rhot_hyd = a.RHOT_hyd ? ldg(a.RHOT_hyd, n) : PRES0ovRdry * fedg_pow_tab(PH / c.PRES00, rgamm, sPow);
         pott[s] = (rhot_hyd + DT[s]) / dens;
         dpres[s] = c.PRES00 * fedg_pow_tab(ldg(a.Rtot, n) * rP0 * dens * pott[s], ldg(a.CPtot, n) / ldg(a.CVtot, n), sPow);
         dpres[s] = dpres[s] - PH;
@@ -229,7 +231,7 @@ __device__ __forceinline__ void hevi_rhs3_body(const HeviArgs& a) {
   const double G13o = FLAT ? 0.0 : ldg(m.G13, o), G23o = FLAT ? 0.0 : ldg(m.G23, o);
   const double FD = PVW + GV * G13o * ldg(a.pin[VU], o) + GV * G23o * ldg(a.pin[VV], o);
   const double PH = ldg(a.PRES_hyd, o), DH = ldg(a.DENS_hyd, o);
-  const double RHOT = P0ovR * fedg_pow_tab(PH / c.PRES00, rgamm, sPow) + PVT;
+  const double RHOT = (a.RHOT_hyd ? ldg(a.RHOT_hyd, o) : P0ovR * fedg_pow_tab(PH / c.PRES00, rgamm, sPow)) + PVT;
   const double pt_ = RHOT / (PVD + DH);
   const double Rt = ldg(a.Rtot, o), CPt = ldg(a.CPtot, o), CVt = ldg(a.CVtot, o);
   const double FW = c.PRES00 * fedg_pow_tab(Rt * rP0 * RHOT, CPt / CVt, sPow) - PH;
@@ -892,8 +894,9 @@ template <int N1, bool FLAT>
 __global__ void __launch_bounds__(N1* N1* N1) hevi_rhs3_kernel(const HeviArgs a) {
   hevi_rhs3_body<N1, FLAT>(a);
 }
-// p = 7 on the flat box at no more than 128 VGPRs (a few registers spill):
-// two 512-thread workgroups per CU instead of one at 135-143 VGPRs
+// p = 7 on the flat box at no more than 128 VGPRs, so that two 512-thread
+// workgroups share a CU (the kernels held 135-143 VGPRs when every node formed
+// the hydrostatic RHOT with two pows; with it loaded, 70-72)
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
 hevi_rhs_uv8_flat_kernel(const HeviArgs a) {
   hevi_rhs_uv_body<8, true>(a);
@@ -1111,10 +1114,29 @@ int run_operator(fedg_ctx* ctx, HeviArgs& a) {
 
 }  // namespace
 
+namespace {
+// the hydrostatic RHOT the residual kernels form from PRES_hyd (the same
+// expression and pow, so the same bits), once per base state
+__global__ void rhot_hyd_kernel(double* out, const double* PRES_hyd, const Consts c, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double rgamm = c.CVdry / c.CPdry;
+  const double PRES0ovRdry = c.PRES00 / c.Rdry;
+  out[t] = PRES0ovRdry * fedg_pow(PRES_hyd[t] / c.PRES00, rgamm);
+}
+}  // namespace
+
+int launch_rhot_hyd(fedg_ctx* ctx, double* RHOT_hyd, const double* PRES_hyd, long n) {
+  if (n <= 0) return FEDG_OK;
+  hipLaunchKernelGGL(rhot_hyd_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, RHOT_hyd, PRES_hyd, ctx->c, n);
+  FEDG_HIP_CHECK(hipGetLastError());
+  return FEDG_OK;
+}
+
 int hevi_cal_vi(fedg_ctx* ctx, HeviWork* w, double* const tend[5], const double* const q[5],
                 const double* const var0[5], const double* DENS_hyd, const double* PRES_hyd, const double* Rtot,
                 const double* CVtot, const double* CPtot, double impl_fac, int max_it, double rtol, double* resid,
-                int* it_done, HeviGuess* guess, const HeviStore* store) {
+                int* it_done, HeviGuess* guess, const HeviStore* store, const double* RHOT_hyd) {
   const DevMesh& m = ctx->m;
   if (store && !(fabs(impl_fac) > 0.0)) {
     set_error("hevi_cal_vi: the fused StoreImplicit needs impl_fac != 0");
@@ -1132,6 +1154,7 @@ int hevi_cal_vi(fedg_ctx* ctx, HeviWork* w, double* const tend[5], const double*
   }
   a.DENS_hyd = DENS_hyd;
   a.PRES_hyd = PRES_hyd;
+  a.RHOT_hyd = RHOT_hyd;
   a.Rtot = Rtot;
   a.CVtot = CVtot;
   a.CPtot = CPtot;
