@@ -2090,7 +2090,7 @@ static int hevi_step(fedg_hevi* st, hipEvent_t* ev) {
     if (ev) FEDG_HIP_CHECK(hipEventRecord(ev[0], ctx->stream));
     TRY(hevi_cal_vi(ctx, w, Ti, qv, v0, st->DENS_hyd, st->PRES_hyd, st->Rtot, st->CVtot, st->CPtot, impl_fac,
                     st->max_it, st->rtol, nullptr, &it, st->mlp ? &g : nullptr, fuse_store ? &sto : nullptr,
-                    st->RHOT_hyd));
+                    st->RHOT_hyd, s > 1));  // stage 1 formed alph from var0
     if (ev) FEDG_HIP_CHECK(hipEventRecord(ev[1], ctx->stream));
     st->guess_lines += g.lines;
     st->guess_fallbacks += g.fallbacks;

@@ -65,7 +65,10 @@ int hevi_cal_vi(fedg_ctx* ctx, HeviWork* w, double* const tend[5], const double*
                 const double* const var0[5], const double* DENS_hyd, const double* PRES_hyd, const double* Rtot,
                 const double* CVtot, const double* CPtot, double impl_fac, int max_it, double rtol, double* resid,
                 int* it_done, HeviGuess* guess = nullptr, const HeviStore* store = nullptr,
-                const double* RHOT_hyd = nullptr);
+                const double* RHOT_hyd = nullptr, int alph_ready = 0);
+// alph_ready: the workspace's vertical-face damping coefficients (alph) were
+// formed from this same var0 by an earlier call on the context (a later
+// stage of the same step), so the residual kernels load them
 // RHOT_hyd = PRES00/Rdry (PRES_hyd/PRES00)^(CVdry/CPdry) over n nodes: the
 // value the residual kernels form from PRES_hyd, bit for bit (their RHOT_hyd
 // argument of hevi_cal_vi)

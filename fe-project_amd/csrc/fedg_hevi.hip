@@ -46,6 +46,7 @@ struct HeviArgs {
   const double* DENS_hyd;
   const double* PRES_hyd;
   const double* RHOT_hyd;  // PRES00/Rdry (PRES_hyd/PRES00)^(CVdry/CPdry) per node, or null: formed in the kernels
+  int alph_ready;          // alph holds the damping coefficients of q0 (a residual kernel formed them)
   const double* Rtot;
   const double* CVtot;
   const double* CPtot;
@@ -101,43 +102,51 @@ __device__ __forceinline__ void hevi_rhs_uv_body(const HeviArgs& a) {
   const uint32_t o = (uint32_t)Np * ke + p;
   const double gamm = c.CPdry / c.CVdry, rgamm = c.CVdry / c.CPdry;
   const double PRES0ovRdry = c.PRES00 / c.Rdry, rP0 = 1.0 / c.PRES00;
-  pow_tables_to(sPow, p, Np);
-  __syncthreads();
+  if (!a.alph_ready) {
+    pow_tables_to(sPow, p, Np);
+    __syncthreads();
+  }
   if (k == 0 || k == N1 - 1) {
     for (int f = 0; f < 2; ++f) {
       if ((f == 0) != (k == 0)) continue;
       const int fp = (4 + f) * Nfp + ij;  // reference face-node index
       const uint32_t gfp = (uint32_t)fp + (uint32_t)(6 * Nfp) * ke;
       const uint32_t iM = o, iP = vz_nbr(a, ke, f, ij, N1);
-      const double nz = FLAT ? (f == 0 ? -1.0 : 1.0) : ldg(m.nz, gfp);
-      const int ke2d = FLAT ? 0 : m.EMap3Dto2D[ke];
-      const double G11_ = FLAT ? 1.0 : ldg(m.G11, (uint32_t)ij + (uint32_t)Nfp * ke2d);
-      const double G12_ = FLAT ? 0.0 : ldg(m.G12, (uint32_t)ij + (uint32_t)Nfp * ke2d);
-      const double G22_ = FLAT ? 1.0 : ldg(m.G22, (uint32_t)ij + (uint32_t)Nfp * ke2d);
-      double wt0[2], pres0[2], rdens0[2], Gnn[2];
-      const uint32_t ix[2] = {iM, iP};
-      for (int s = 0; s < 2; ++s) {
-        const uint32_t n = ix[s];
-        const int ken = n / Np;
-        const double gam = 1.0;
-        const double rgam2 = 1.0 / (gam * gam);
-        const double RGsqrtV = 1.0 / (FLAT ? 1.0 : gsqrtv(a, ken, n - (uint32_t)Np * ken, N1));
-        const double MOMX0 = ldg(a.q0[VU], n), MOMY0 = ldg(a.q0[VV], n), DRHOT0 = ldg(a.q0[VT], n);
-        const double Rt = ldg(a.Rtot, n), CPovCV = ldg(a.CPtot, n) / ldg(a.CVtot, n);
-        const double g13 = FLAT ? 0.0 : ldg(m.G13, n), g23 = FLAT ? 0.0 : ldg(m.G23, n);
-        rdens0[s] = 1.0 / (ldg(a.DENS_hyd, n) + ldg(a.q0[VD], n));
-        wt0[s] = (ldg(a.q0[VW], n) * RGsqrtV + g13 * MOMX0 + g23 * MOMY0) * rdens0[s];
-        const double rhot_hyd = a.RHOT_hyd ? ldg(a.RHOT_hyd, n)
-                                           : PRES0ovRdry * fedg_pow_tab(ldg(a.PRES_hyd, n) / c.PRES00, rgamm, sPow);
-        pres0[s] = c.PRES00 * fedg_pow_tab(Rt * rP0 * (rhot_hyd + DRHOT0), CPovCV, sPow);
-        const double Gxz = rgam2 * (G11_ * g13 + G12_ * g23);
-        const double Gyz = rgam2 * (G12_ * g13 + G22_ * g23);
-        Gnn[s] = (1.0 * (RGsqrtV * RGsqrtV) + g13 * Gxz + g23 * Gyz) * fabs(nz);
+      const uint32_t ia = (uint32_t)(f * Nfp + ij) + (uint32_t)(2 * Nfp) * ke;
+      double al;
+      if (a.alph_ready) {
+        al = ldg(a.alph, ia);
+      } else {
+        const double nz = FLAT ? (f == 0 ? -1.0 : 1.0) : ldg(m.nz, gfp);
+        const int ke2d = FLAT ? 0 : m.EMap3Dto2D[ke];
+        const double G11_ = FLAT ? 1.0 : ldg(m.G11, (uint32_t)ij + (uint32_t)Nfp * ke2d);
+        const double G12_ = FLAT ? 0.0 : ldg(m.G12, (uint32_t)ij + (uint32_t)Nfp * ke2d);
+        const double G22_ = FLAT ? 1.0 : ldg(m.G22, (uint32_t)ij + (uint32_t)Nfp * ke2d);
+        double wt0[2], pres0[2], rdens0[2], Gnn[2];
+        const uint32_t ix[2] = {iM, iP};
+        for (int s = 0; s < 2; ++s) {
+          const uint32_t n = ix[s];
+          const int ken = n / Np;
+          const double gam = 1.0;
+          const double rgam2 = 1.0 / (gam * gam);
+          const double RGsqrtV = 1.0 / (FLAT ? 1.0 : gsqrtv(a, ken, n - (uint32_t)Np * ken, N1));
+          const double MOMX0 = ldg(a.q0[VU], n), MOMY0 = ldg(a.q0[VV], n), DRHOT0 = ldg(a.q0[VT], n);
+          const double Rt = ldg(a.Rtot, n), CPovCV = ldg(a.CPtot, n) / ldg(a.CVtot, n);
+          const double g13 = FLAT ? 0.0 : ldg(m.G13, n), g23 = FLAT ? 0.0 : ldg(m.G23, n);
+          rdens0[s] = 1.0 / (ldg(a.DENS_hyd, n) + ldg(a.q0[VD], n));
+          wt0[s] = (ldg(a.q0[VW], n) * RGsqrtV + g13 * MOMX0 + g23 * MOMY0) * rdens0[s];
+          const double rhot_hyd = a.RHOT_hyd ? ldg(a.RHOT_hyd, n)
+                                             : PRES0ovRdry * fedg_pow_tab(ldg(a.PRES_hyd, n) / c.PRES00, rgamm, sPow);
+          pres0[s] = c.PRES00 * fedg_pow_tab(Rt * rP0 * (rhot_hyd + DRHOT0), CPovCV, sPow);
+          const double Gxz = rgam2 * (G11_ * g13 + G12_ * g23);
+          const double Gyz = rgam2 * (G12_ * g13 + G22_ * g23);
+          Gnn[s] = (1.0 * (RGsqrtV * RGsqrtV) + g13 * Gxz + g23 * Gyz) * fabs(nz);
+        }
+        const double sM = fabs(wt0[0]) + sqrt(Gnn[0] * gamm * pres0[0] * rdens0[0]);
+        const double sP = fabs(wt0[1]) + sqrt(Gnn[1] * gamm * pres0[1] * rdens0[1]);
+        al = (nz * nz) * (sM > sP ? sM : sP);
+        stg(a.alph, ia, al);
       }
-      const double sM = fabs(wt0[0]) + sqrt(Gnn[0] * gamm * pres0[0] * rdens0[0]);
-      const double sP = fabs(wt0[1]) + sqrt(Gnn[1] * gamm * pres0[1] * rdens0[1]);
-      const double al = (nz * nz) * (sM > sP ? sM : sP);
-      stg(a.alph, (uint32_t)(f * Nfp + ij) + (uint32_t)(2 * Nfp) * ke, al);
       const double tmp1 = -0.5 * (FLAT ? m.fF[4 + f] : ldg(m.Fscale, gfp)) * al;
       sdf[f][0][ij] = tmp1 * (ldg(a.pin[VU], iP) - ldg(a.pin[VU], iM));
       sdf[f][1][ij] = tmp1 * (ldg(a.pin[VV], iP) - ldg(a.pin[VV], iM));
@@ -979,6 +988,7 @@ int run_newton(fedg_ctx* ctx, HeviArgs& a, int max_it, double rtol, double* resi
   for (; it < max_it; ++it) {
     launch_rhs_uv<N1>(ctx, a);
     FEDG_HIP_CHECK(hipGetLastError());
+    a.alph_ready = 1;  // the damping coefficients depend on q0 alone
     double r2 = 0.0;
     const bool track = rtol > 0.0 || resid != nullptr;
     if (track) {
@@ -1059,6 +1069,7 @@ int predict_iterate(fedg_ctx* ctx, HeviWork* w, HeviArgs& a, HeviGuess* g, doubl
   // residual of every line at q^n (pv holds q^n)
   launch_rhs_uv<N1>(ctx, a);
   FEDG_HIP_CHECK(hipGetLastError());
+  a.alph_ready = 1;
   launch_rhs3<N1>(ctx, a);
   FEDG_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(line_resid_kernel, dim3(nlines), dim3(L), 0, ctx->stream, la, a.buv, a.b3, nn, w->r_plain);
@@ -1136,7 +1147,8 @@ int launch_rhot_hyd(fedg_ctx* ctx, double* RHOT_hyd, const double* PRES_hyd, lon
 int hevi_cal_vi(fedg_ctx* ctx, HeviWork* w, double* const tend[5], const double* const q[5],
                 const double* const var0[5], const double* DENS_hyd, const double* PRES_hyd, const double* Rtot,
                 const double* CVtot, const double* CPtot, double impl_fac, int max_it, double rtol, double* resid,
-                int* it_done, HeviGuess* guess, const HeviStore* store, const double* RHOT_hyd) {
+                int* it_done, HeviGuess* guess, const HeviStore* store, const double* RHOT_hyd,
+                int alph_ready) {
   const DevMesh& m = ctx->m;
   if (store && !(fabs(impl_fac) > 0.0)) {
     set_error("hevi_cal_vi: the fused StoreImplicit needs impl_fac != 0");
@@ -1155,6 +1167,7 @@ int hevi_cal_vi(fedg_ctx* ctx, HeviWork* w, double* const tend[5], const double*
   a.DENS_hyd = DENS_hyd;
   a.PRES_hyd = PRES_hyd;
   a.RHOT_hyd = RHOT_hyd;
+  a.alph_ready = alph_ready;
   a.Rtot = Rtot;
   a.CVtot = CVtot;
   a.CPtot = CPtot;
