@@ -2054,7 +2054,12 @@ static int hevi_step(fedg_hevi* st, hipEvent_t* ev) {
   const size_t nA = (size_t)m.Np * m.NeA;
   HeviWork* w = nullptr;
   TRY(hevi_work_get(ctx, &w));
-  for (int v = 0; v < 5; ++v) TRY(launch_copy(ctx, t->var0 + v * nA, st->q[v], n));  // StoreVar0
+  // StoreVar0.  With an explicit first stage (the IMEX tables here) the
+  // first stage evaluates the implicit operator without changing q, so it
+  // reads var0 from q itself and its StoreImplicit pass writes the copy.
+  const bool var0_at_stage1 = t->a_im[0][0] == 0.0;
+  if (!var0_at_stage1)
+    for (int v = 0; v < 5; ++v) TRY(launch_copy(ctx, t->var0 + v * nA, st->q[v], n));
   for (int s = 1; s <= t->nstage; ++s) {
     const int ind = t->indmap[s - 1];
     const double impl_fac = t->a_im[s - 1][s - 1] * t->dt;
@@ -2066,7 +2071,7 @@ static int hevi_step(fedg_hevi* st, hipEvent_t* ev) {
       Ti[v] = tend_ptr(t, t->tend_im, ind, v + 1);
       Te[v] = tend_ptr(t, t->tend_ex, ind, v + 1);
       qv[v] = st->q[v];
-      v0[v] = t->var0 + v * nA;
+      v0[v] = (s == 1 && var0_at_stage1) ? st->q[v] : t->var0 + v * nA;
     }
     int it = 0;
     // explicit tendency of the previous stage: a feature of the learned guess
