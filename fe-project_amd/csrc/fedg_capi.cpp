@@ -2120,8 +2120,10 @@ static int hevi_step(fedg_hevi* st, hipEvent_t* ev) {
       TRY(launch_store_implicit(ctx, st->q, Ti, s == 1 ? sto.var0 : nullptr, s == 1 ? sto.varTmp : nullptr,
                                 impl_fac));
     for (int v = 0; v < 5; ++v) TRY(launch_halo_exchange(ctx, st->q[v]));
-    TRY(launch_drhot2pres(ctx, st->PRES, st->DPRES, st->q[1], st->PRES_hyd, st->THERM_hyd, st->Rtot, st->CVtot,
-                          st->CPtot, n));
+    // PRES only for the physics tendencies; dry constants on the verified dry box
+    TRY(launch_drhot2pres(ctx, st->tp[0] ? st->PRES : nullptr, st->DPRES, st->q[1], st->PRES_hyd, st->THERM_hyd,
+                          st->flat7 ? nullptr : st->Rtot, st->flat7 ? nullptr : st->CVtot,
+                          st->flat7 ? nullptr : st->CPtot, n));
     TRY(launch_halo_exchange(ctx, st->DPRES));
     TRY(launch_apply_bc(ctx, st->q[3], st->q[4], st->q[2]));
     TendIn in{};

@@ -1676,8 +1676,12 @@ __global__ void drhot2pres_kernel(double* PRES, double* DPRES, const double* DRH
   if (t >= n) return;
   const double rP0 = 1.0 / c.PRES00;
   const double RHOT = THERM_hyd[t] + DRHOT[t];
-  const double P = c.PRES00 * fedg_pow(Rtot[t] * rP0 * RHOT, CPtot[t] / CVtot[t]);
-  PRES[t] = P;
+  // null Rtot / CVtot / CPtot: dry air (the caller verified the fields hold
+  // the dry constants); null PRES: only DPRES is wanted
+  const double Rt = Rtot ? Rtot[t] : c.Rdry;
+  const double CVt = CVtot ? CVtot[t] : c.CVdry, CPt = CPtot ? CPtot[t] : c.CPdry;
+  const double P = c.PRES00 * fedg_pow(Rt * rP0 * RHOT, CPt / CVt);
+  if (PRES) PRES[t] = P;
   DPRES[t] = P - PRES_hyd[t];
 }
 
